@@ -1,0 +1,332 @@
+/*
+ * ba_hip.h — C ABI of the MI355X (gfx950) Levenberg–Marquardt bundle-adjustment path.
+ *
+ * This library is a drop-in for one entry point of openmvg-ba:
+ *
+ *     impl BundleAdjuster for X {
+ *         fn name(&self) -> &'static str;
+ *         fn solve(&mut self, &BundleProblem, SolveConfig) -> Result<Solution, SolveError>;
+ *     }                                   (crates/openmvg-ba/src/solver.rs:1562-1573)
+ *
+ * restricted to the path the native adjuster takes for problems with free intrinsics
+ * (crates/openmvg-ba/src/native/mod.rs:352-367 -> native/self_calibration.rs:223-249):
+ * pixel residuals, perspective cameras with one focal length, free (f[, k1[, k2]]),
+ * a fixed principal point, full-pose or fixed poses.  Anything else returns
+ * BA_ERR_UNSUPPORTED, which callers treat like SolveError::Unsupported
+ * (crates/openmvg-sfm/src/ba/native.rs:79-85: the caller falls back to another adjuster).
+ *
+ * Conventions
+ *   - Plain C types only; no allocation crosses the boundary.  Every output buffer is
+ *     allocated by the caller with the sizes of the problem it passes.
+ *   - Problem arrays are host pointers for ba_hip_solve (the library copies them to HBM)
+ *     and device pointers for the per-kernel hooks (ba_hip_linearize ... ba_hip_cost_grad).
+ *   - Rotations are row-major 3x3 world-to-camera matrices; a pose is (R, C) with
+ *     camera point = R (X - C), as openmvg-core's WorldToCameraPose.
+ *   - Return codes mirror SolveError (crates/openmvg-ba/src/solver.rs:1474-1493).
+ *   - One solve at a time per handle (the reference trait takes &mut self; SfM serialises
+ *     calls through a Mutex, crates/openmvg-sfm/src/ba/native.rs:30-33).  Results do not
+ *     depend on scheduling: every reduction runs in a fixed order, with no float atomics.
+ */
+#ifndef OPENMVG_BA_HIP_H
+#define OPENMVG_BA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes: SolveError (solver.rs:1474-1493) plus ProblemError (problem.rs build) ---- */
+enum ba_status {
+    BA_OK = 0,
+    BA_ERR_INVALID_CONFIG = 1,      /* SolveError::InvalidConfig */
+    BA_ERR_UNSUPPORTED = 2,         /* SolveError::Unsupported: the sanctioned decline */
+    BA_ERR_RESOURCE_EXHAUSTED = 3,  /* SolveError::ResourceExhausted */
+    BA_ERR_NUMERICAL = 4,           /* SolveError::Numerical */
+    BA_ERR_INTERNAL = 5,            /* SolveError::Internal (also: HIP runtime failures) */
+    BA_ERR_INVALID_PROBLEM = 6      /* ProblemError from BundleProblemBuilder::build (problem.rs:492-576) */
+};
+
+/* ---- camera model: openmvg-core Camera, the variants self-calibration reads ---- */
+enum ba_camera_model {
+    BA_CAMERA_PINHOLE = 0,          /* Camera::Pinhole */
+    BA_CAMERA_BROWN_NONE = 1,       /* Camera::BrownConrady with RadialDistortion::None */
+    BA_CAMERA_BROWN_K1 = 2,         /* RadialDistortion::K1 */
+    BA_CAMERA_BROWN_K1K2 = 3,       /* RadialDistortion::K1K2 */
+    BA_CAMERA_BROWN_K1K2K3 = 4,     /* RadialDistortion::K1K2K3 */
+    BA_CAMERA_FISHEYE = 5,          /* unsupported for self-calibration */
+    BA_CAMERA_SPHERICAL = 6         /* unsupported for self-calibration */
+};
+
+/* BundleIntrinsic (problem.rs:136-185): calibration, variable mask, certification. */
+typedef struct ba_intrinsic {
+    int32_t model;                  /* enum ba_camera_model */
+    double fx, fy, cx, cy;
+    double k1, k2, k3, p1, p2;
+    uint32_t width, height;
+    uint8_t fixed_calibration;      /* CameraIntrinsics::is_fixed */
+    uint8_t var_focal, var_principal_point, var_radial, var_tangential; /* IntrinsicVariables */
+    uint8_t certification_waived;   /* IntrinsicCertification::Waived (else Required) */
+    uint8_t _pad[2];
+} ba_intrinsic;
+
+enum ba_residual_space { BA_RESIDUAL_PIXELS = 0, BA_RESIDUAL_ANGULAR = 1 };
+enum ba_robust_loss { BA_LOSS_SQUARED = 0, BA_LOSS_HUBER = 1, BA_LOSS_CAUCHY = 2 };
+enum ba_gauge { BA_GAUGE_FREE = 0, BA_GAUGE_FIXED_BASELINE = 1 };
+
+/* BundleProblem (problem.rs:447-455) as structure-of-arrays views. */
+typedef struct ba_problem {
+    size_t n_poses;
+    const double* pose_rotation;          /* n_poses*9, row-major world-to-camera */
+    const double* pose_center;            /* n_poses*3, world camera centre */
+    const uint8_t* pose_var_rotation;     /* n_poses, PoseVariables::rotation */
+    const uint8_t* pose_var_translation;  /* n_poses, PoseVariables::translation */
+    size_t n_intrinsics;
+    const ba_intrinsic* intrinsics;       /* n_intrinsics */
+    size_t n_points;
+    const double* point_xyz;              /* n_points*3 */
+    const uint8_t* point_fixed;           /* n_points, or NULL when no point is fixed */
+    size_t n_observations;
+    const uint32_t* obs_pose;             /* n_observations */
+    const uint32_t* obs_intrinsic;        /* n_observations */
+    const uint32_t* obs_point;            /* n_observations */
+    const double* obs_pixel;              /* n_observations*2, pixels */
+    int32_t residual_space;               /* enum ba_residual_space */
+    int32_t robust_loss;                  /* enum ba_robust_loss */
+    double loss_scale;                    /* Huber/Cauchy ResidualScale value, pixels */
+    int32_t gauge;                        /* enum ba_gauge */
+    uint32_t gauge_first, gauge_second;   /* Gauge::FixedBaseline */
+    double gauge_distance;
+} ba_problem;
+
+enum ba_linear_solver { BA_SOLVER_AUTO = 0, BA_SOLVER_DIRECT = 1, BA_SOLVER_ITERATIVE = 2 };
+
+/* Arithmetic of the inner loop.  PLAIN runs kernels (a)-(d) in FP64; MIXED runs (a)-(c)
+ * in FP32 with compensated cost and gradient sums, factors (d) in FP32 and recovers the
+ * Newton step by FP64 iterative refinement.  The LM gain ratio, damping update and
+ * acceptance test are FP64 in both forms.  (No counterpart in the reference, which is FP64.) */
+enum ba_precision { BA_PRECISION_FP64 = 0, BA_PRECISION_MIXED = 1 };
+
+/* SolveConfig (solver.rs:74-155) plus the two knobs of this implementation. */
+typedef struct ba_solve_config {
+    uint32_t max_iterations;              /* default 50 */
+    double function_tolerance;            /* default 1e-6 */
+    double gradient_tolerance;            /* default 1e-10 */
+    double parameter_tolerance;           /* default 1e-8 */
+    int32_t linear_solver;                /* enum ba_linear_solver */
+    uint64_t workspace_budget_bytes;      /* 0 = automatic (1 GiB planning budget) */
+    int32_t precision;                    /* enum ba_precision */
+    uint32_t pcg_max_iterations;          /* 0 = the native cap min(600, D) */
+    double pcg_relative_tolerance;        /* 0 = the native forcing tolerance 1e-1 */
+} ba_solve_config;
+
+/* SolveStatus (solver.rs:172-185) with ConvergenceReason folded in. */
+enum ba_solve_status {
+    BA_STATUS_CONVERGED_GRADIENT = 0,
+    BA_STATUS_CONVERGED_STEP = 1,
+    BA_STATUS_CONVERGED_COST = 2,
+    BA_STATUS_MAX_ITERATIONS = 3,
+    BA_STATUS_STALLED = 4,
+    BA_STATUS_DEGENERATE = 5
+};
+
+/* SolveReport (solver.rs:461-473) and the kernel report fields it is built from
+ * (kernel/multi_view/sparse/self_calibration/mod.rs:569-638, native/reporting.rs:24-76). */
+typedef struct ba_solve_report {
+    int32_t status;                       /* enum ba_solve_status */
+    int32_t linear_solver_used;           /* BA_SOLVER_DIRECT or BA_SOLVER_ITERATIVE */
+    uint32_t iterations;
+    uint32_t accepted_steps;
+    uint32_t rejected_steps;
+    uint32_t rejected_linearization, rejected_step_solve, rejected_candidate,
+             rejected_insufficient_reduction;
+    double initial_cost, final_cost;
+    double initial_rmse, final_rmse;      /* per-axis pixel RMSE x sqrt(2), as reporting.rs:72-73 */
+    double gradient_inf_norm;             /* scaled, at the final zero-damping assembly */
+    double step_norm;
+    double initial_damping, final_damping;
+    double last_gain_ratio;               /* NaN when none */
+    uint32_t reduced_dimension;           /* D of ReducedLayout::try_radial */
+    uint32_t weak_point_count;
+    double minimum_point_rcond;
+    uint32_t pcg_iterations;              /* retained_inner_iterations */
+    uint32_t quarantined_points;
+    double maximum_pcg_relative_residual; /* NaN when no PCG step ran */
+    double normalization_scale;
+    uint64_t observation_count, point_count, pose_count;
+} ba_solve_report;
+
+/* Optional per-iteration record: the LM trajectory a parity test compares. */
+typedef struct ba_iteration_trace {
+    uint32_t capacity;                    /* entries each array can hold */
+    uint32_t count;                       /* written by the library */
+    double* cost;                         /* cost at the linearisation */
+    double* damping;                      /* lambda used by the iteration */
+    double* candidate_cost;               /* NaN when no candidate was formed */
+    double* predicted_reduction;          /* NaN when none */
+    double* gain_ratio;                   /* NaN when none */
+    uint8_t* accepted;
+    uint32_t* pcg_iterations;
+} ba_iteration_trace;
+
+/* Solution (solver.rs:803-876): caller-allocated buffers with the problem's sizes. */
+typedef struct ba_solution {
+    double* pose_rotation;                /* n_poses*9 */
+    double* pose_center;                  /* n_poses*3 */
+    double* intrinsic_focal;              /* n_intrinsics (fx = fy) */
+    double* intrinsic_k1;                 /* n_intrinsics */
+    double* intrinsic_k2;                 /* n_intrinsics */
+    double* point_xyz;                    /* n_points*3 */
+    uint8_t* point_quarantined;           /* n_points, or NULL */
+    uint8_t* intrinsic_unconstrained;     /* n_intrinsics, or NULL */
+} ba_solution;
+
+typedef struct ba_hip_handle ba_hip_handle;
+
+/* BundleAdjuster::name (solver.rs:1563). */
+const char* ba_hip_name(void);
+
+/* Create a handle bound to HIP device `device` (its streams and buffers).  The handle is
+ * the adjuster object of the reference (NativeBundleAdjuster, native/mod.rs:117-118). */
+int ba_hip_create(int device, ba_hip_handle** out);
+void ba_hip_destroy(ba_hip_handle* handle);
+
+/* BundleAdjuster::solve (solver.rs:1572) for the self-calibrating pixel path
+ * (native/mod.rs:260-367, native/self_calibration.rs:223-249).  Host buffers.
+ * `trace` and `message` may be NULL.  On error, `solution` is left unspecified and the
+ * problem is untouched (the reference's atomic-error contract, sparse/self_calibration/mod.rs:44-48). */
+int ba_hip_solve(ba_hip_handle* handle, const ba_problem* problem, const ba_solve_config* config,
+                 ba_solution* solution, ba_solve_report* report, ba_iteration_trace* trace,
+                 char* message, size_t message_len);
+
+/* ----------------------------------------------------------------------------------------
+ * Lowered ("native") problem: what PreparedSelfCalibratingProblem::try_new builds
+ * (native/self_calibration.rs:47-197) and the workspace index maps
+ * (kernel/multi_view/sparse/workspace/topology.rs:351-397, workspace/pattern.rs:270-468).
+ * ---------------------------------------------------------------------------------------- */
+typedef struct ba_native_sizes {
+    size_t n_cameras, n_groups, n_points, n_observations;
+    size_t dimension;                     /* reduced camera dimension D */
+    size_t n_point_parameters;            /* length of the per-point active-parameter list */
+    size_t n_pattern_entries;             /* lower-triangle nonzeros of the reduced pattern */
+    size_t n_quarantined;                 /* points held out before the solve (quarantine.rs:33-53) */
+} ba_native_sizes;
+
+typedef struct ba_native_problem {
+    /* state, anchor-first camera order, anchor-frame coordinates */
+    double* camera_rotation;              /* n_cameras*9 */
+    double* camera_translation;           /* n_cameras*3 */
+    double* group_params;                 /* n_groups*3: focal, k1, k2 */
+    uint32_t* camera_group;               /* n_cameras */
+    uint32_t* camera_source;              /* n_cameras: input pose index of each camera */
+    double* point_xyz;                    /* n_points*3 (input points minus the quarantined) */
+    uint32_t* point_source;               /* n_points: input point index */
+    uint32_t* obs_camera;                 /* n_observations, sorted by (point, input index) */
+    uint32_t* obs_source;                 /* n_observations: input observation index */
+    double* obs_uv;                       /* n_observations*2, principal point subtracted */
+    uint32_t* point_offsets;              /* n_points+1 */
+    uint8_t* camera_fixed;                /* n_cameras */
+    uint8_t* point_fixed;                 /* n_points */
+    /* layout */
+    uint32_t* camera_offset;              /* n_cameras */
+    uint32_t* camera_dim;                 /* n_cameras: 0, 5 (baseline camera) or 6 */
+    uint32_t* group_offset;               /* n_groups */
+    uint32_t* group_dim;                  /* n_groups: 0..3 */
+    uint32_t* cross_offsets;              /* n_points+1 (workspace/pattern.rs:306-379) */
+    uint32_t* point_parameters;           /* n_point_parameters, first-encounter order */
+    uint32_t* pattern_col_ptr;            /* dimension+1, lower CSC (pattern.rs:441-468) */
+    uint32_t* pattern_row_idx;            /* n_pattern_entries */
+    /* gauge: FixedBaseline camera or -1 for Free */
+    int32_t gauge_camera;
+    int32_t _pad;
+    double scene_scale;                   /* optimizer.rs:620-627 at the initial state */
+} ba_native_problem;
+
+/* Lower `problem` (host buffers).  Call once with native == NULL to get the sizes, then with
+ * buffers of those sizes. */
+int ba_hip_prepare(const ba_problem* problem, ba_native_sizes* sizes, ba_native_problem* native,
+                   char* message, size_t message_len);
+
+/* ---- per-kernel hooks on device buffers (the tests' parity seams) ----------------------- */
+
+/* Device view of a lowered problem at one state. */
+typedef struct ba_device_problem {
+    size_t n_cameras, n_groups, n_points, n_observations, dimension;
+    const double* camera_rotation;        /* n_cameras*9 */
+    const double* camera_translation;     /* n_cameras*3 */
+    const double* group_params;           /* n_groups*3 */
+    const uint32_t* camera_group;
+    const double* point_xyz;              /* n_points*3 */
+    const uint32_t* obs_camera;
+    const uint32_t* obs_point;            /* n_observations: owning point */
+    const double* obs_uv;                 /* n_observations*2 */
+    const uint32_t* point_offsets;        /* n_points+1 */
+    const uint8_t* point_fixed;           /* n_points */
+    const uint32_t* camera_offset;
+    const uint32_t* camera_dim;
+    const uint32_t* group_offset;
+    const uint32_t* group_dim;
+    int32_t loss;                         /* enum ba_robust_loss, kernel scale below */
+    double loss_scale;                    /* Huber delta / Cauchy kernel scale (scale/sqrt 2) */
+    int32_t gauge_camera;                 /* -1 = Free */
+    double scene_scale;
+    double baseline;                      /* |t| of the gauge camera */
+    double basis[6];                      /* S^2 tangent basis rows of the gauge camera */
+} ba_device_problem;
+
+/* Kernel (a): per observation, robust-weighted residual r~ (2), point Jacobian J~_X (2x3) and
+ * camera-block Jacobian J~_c (2x9: pose run then intrinsic run, zero-padded), the robust
+ * cost and the validity flag.  Reference: schur/point.rs:239-355 and
+ * self_calibration/model.rs:26-129,308-324.  Outputs are device arrays of n_observations rows. */
+int ba_hip_linearize(ba_hip_handle* handle, const ba_device_problem* problem,
+                     double* residual /*2*n*/, double* point_jacobian /*6*n*/,
+                     double* camera_jacobian /*18*n*/, double* cost /*n*/, uint8_t* valid /*n*/);
+
+/* Kernel (b): point elimination and the reduced camera system at damping `lambda`:
+ * S (dense D x D, lower triangle filled, row-major), reduced gradient g^ (D), unreduced
+ * gradient (D), clamped damping diagonal (D), per-point inverse damped block (6), inverse
+ * gradient (3), and the scalar outputs (cost, point model reduction, scaled gradient norm,
+ * valid flag).  Reference: schur/assembly.rs:893-1069, point_block.rs:85-168. */
+typedef struct ba_schur_scalars {
+    double cost, squared_residual, point_model_reduction, scaled_gradient_inf_norm;
+    double minimum_point_rcond;
+    uint32_t weak_point_count, valid;
+} ba_schur_scalars;
+int ba_hip_schur(ba_hip_handle* handle, const ba_device_problem* problem, double lambda,
+                 double* schur /*D*D*/, double* gradient /*D*/, double* full_gradient /*D*/,
+                 double* diagonal /*D*/, ba_schur_scalars* scalars /*host*/);
+
+/* Kernel (c): block-Jacobi PCG on the implicit Schur system at damping `lambda`
+ * (self_calibration_retained.rs:512-579, pcg.rs:23-119).  Writes the step (D, device) and
+ * returns iterations and the true relative residual. */
+int ba_hip_pcg(ba_hip_handle* handle, const ba_device_problem* problem, double lambda,
+               double relative_tolerance, uint32_t max_iterations, double* step /*D*/,
+               uint32_t* iterations, double* relative_residual, int32_t* accepted);
+
+/* Kernel (d): dense Cholesky factor of the n x n SPD matrix `a` (device, row-major, lower
+ * triangle read) followed by the solve a x = b; `x` (device, n) receives the solution.
+ * Fails (BA_ERR_NUMERICAL) on a pivot <= 64 eps max|diag| (dense/linear_solver.rs:9-61).
+ * precision: BA_PRECISION_FP64, or BA_PRECISION_MIXED for the FP32 factor with FP64
+ * iterative refinement. */
+int ba_hip_cholesky_solve(ba_hip_handle* handle, size_t n, const double* a, const double* b,
+                          double* x, int32_t precision);
+
+/* Differentiation form: robust reprojection cost sum_obs rho(|r|) of a BAL-style scene
+ * (per camera: angle-axis w(3), t(3), f, k1, k2; per point X(3)) and, when grad_camera is
+ * non-NULL, its gradient with respect to every camera and point parameter.  The backward
+ * kernel recomputes the Jacobians.  `is_fp32` selects float buffers (all float) instead of
+ * double.  Deterministic: per-camera and per-point sums run in a fixed order.
+ * No counterpart entry point in the reference; it evaluates the objective of
+ * numerics/robust.rs:8-44 with the camera model of self_calibration/model.rs:26-129. */
+int ba_hip_cost_grad(ba_hip_handle* handle, int32_t is_fp32, size_t n_cameras, size_t n_points,
+                     size_t n_observations, const void* cameras /*9*n_cameras*/,
+                     const void* points /*3*n_points*/, const uint32_t* obs_camera,
+                     const uint32_t* obs_point, const void* obs_uv /*2*n_obs*/,
+                     int32_t loss, double loss_scale, void* cost_out /*1*/,
+                     void* grad_camera /*9*n_cameras or NULL*/, void* grad_point /*3*n_points*/);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPENMVG_BA_HIP_H */
