@@ -155,6 +155,10 @@ typedef struct ba_solve_report {
     double maximum_pcg_relative_residual; /* NaN when no PCG step ran */
     double normalization_scale;
     uint64_t observation_count, point_count, pose_count;
+    /* wall-clock split of the call (no counterpart in the reference report) */
+    double seconds_setup;                 /* lowering, index maps, uploads */
+    double seconds_iterations;            /* the LM loop */
+    double seconds_final;                 /* final zero-damping assembly and download */
 } ba_solve_report;
 
 /* Optional per-iteration record: the LM trajectory a parity test compares. */
@@ -276,7 +280,8 @@ typedef struct ba_device_problem {
 } ba_device_problem;
 
 /* Kernel (a): per observation, robust-weighted residual r~ (2), point Jacobian J~_X (2x3) and
- * camera-block Jacobian J~_c (2x9: pose run then intrinsic run, zero-padded), the robust
+ * camera-block Jacobian J~_c (2x9 row-major: pose columns 0..5, intrinsic columns 6..8, zero
+ * beyond the free dimension of each run), the robust
  * cost and the validity flag.  Reference: schur/point.rs:239-355 and
  * self_calibration/model.rs:26-129,308-324.  Outputs are device arrays of n_observations rows. */
 int ba_hip_linearize(ba_hip_handle* handle, const ba_device_problem* problem,

@@ -1,0 +1,656 @@
+// ba_host.cpp — problem checks, lowering and index maps of the MI355X bundle adjuster.
+//
+// Follows crates/openmvg-ba/src:
+//   problem.rs:492-576            BundleProblemBuilder::build checks
+//   solver.rs:122-141             SolveConfig::validate
+//   quarantine.rs:33-96           degenerate input points
+//   native/mod.rs:260-340,457-497 quarantine pass, free-intrinsic test, one intrinsic per pose
+//   native/gauge.rs:9-330         pose-variable mask, gauge anchor and scale
+//   native/world_rebase.rs        anchor-frame conversion
+//   native/self_calibration.rs    PreparedSelfCalibratingProblem::try_new, radial_camera,
+//                                 free_radial_dimension
+//   kernel/multi_view/sparse/workspace/topology.rs:351-397   ReducedLayout::try_radial
+//   kernel/multi_view/sparse/workspace/pattern.rs:306-468    point parameters, block pattern
+#include "ba_host.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace ba {
+namespace {
+
+struct M3 {
+    double a[9];
+};
+inline M3 rot_of(const ba_problem* p, size_t i) {
+    M3 m;
+    std::memcpy(m.a, p->pose_rotation + 9 * i, sizeof(m.a));
+    return m;
+}
+inline void center_of(const ba_problem* p, size_t i, double c[3]) {
+    std::memcpy(c, p->pose_center + 3 * i, 3 * sizeof(double));
+}
+inline bool is_fixed_pose(const ba_problem* p, size_t i) {
+    return !p->pose_var_rotation[i] && !p->pose_var_translation[i];
+}
+inline bool is_fixed_point(const ba_problem* p, size_t i) { return p->point_fixed && p->point_fixed[i]; }
+inline bool is_fixed_intrinsic(const ba_intrinsic& in) {
+    return !in.var_focal && !in.var_principal_point && !in.var_radial && !in.var_tangential;
+}
+inline double sq_dist(const double a[3], const double b[3]) {
+    double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+    return d0 * d0 + d1 * d1 + d2 * d2;
+}
+// row-major a * b^T, with the product sums in index order
+inline void mul_abt(const double* a, const double* b, double* o) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += a[3 * i + k] * b[3 * j + k];
+            o[3 * i + j] = s;
+        }
+}
+inline void mul_ab(const double* a, const double* b, double* o) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += a[3 * i + k] * b[3 * k + j];
+            o[3 * i + j] = s;
+        }
+}
+inline void mul_av(const double* a, const double* v, double* o) {
+    for (int i = 0; i < 3; ++i) o[i] = (a[3 * i] * v[0] + a[3 * i + 1] * v[1]) + a[3 * i + 2] * v[2];
+}
+inline void mul_atv(const double* a, const double* v, double* o) {
+    for (int i = 0; i < 3; ++i) o[i] = (a[i] * v[0] + a[3 + i] * v[1]) + a[6 + i] * v[2];
+}
+
+// TrackGeometry of kernel/point_geometry.rs:29-101, the input-quarantine subset.
+struct Geometry {
+    double near2, far2;
+    double first[3];
+    size_t count = 0;
+    bool has_first = false, parallax = false, invalid = false;
+    explicit Geometry(double scene) : near2(INFINITY), far2(scene * scene) {}
+    void push(const double o[3]) {
+        double sq = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+        if (!(std::isfinite(sq) && sq > 0.0)) {
+            invalid = true;
+            return;
+        }
+        count += 1;
+        near2 = std::fmin(near2, sq);
+        far2 = std::fmax(far2, sq);
+        double inv = 1.0 / std::sqrt(sq);
+        double ray[3] = {o[0] * inv, o[1] * inv, o[2] * inv};
+        if (!has_first) {
+            has_first = true;
+            std::memcpy(first, ray, sizeof(ray));
+            return;
+        }
+        if (parallax) return;
+        double dot = ray[0] * first[0] + ray[1] * first[1] + ray[2] * first[2];
+        double c0 = ray[1] * first[2] - ray[2] * first[1], c1 = ray[2] * first[0] - ray[0] * first[2],
+               c2 = ray[0] * first[1] - ray[1] * first[0];
+        parallax = dot < 0.0 || c0 * c0 + c1 * c1 + c2 * c2 >= 1.7e-5 * 1.7e-5;
+    }
+    bool degenerate() const { return invalid || (count >= 2 && (!parallax || near2 < 1e-4 * 1e-4 * far2)); }
+};
+
+}  // namespace
+
+Status validate_config(const ba_solve_config* c) {
+    if (c->max_iterations == 0) return error(BA_ERR_INVALID_CONFIG, "max_iterations must be positive");
+    if (!std::isfinite(c->function_tolerance) || c->function_tolerance <= 0.0)
+        return error(BA_ERR_INVALID_CONFIG, "function_tolerance");
+    if (!std::isfinite(c->gradient_tolerance) || c->gradient_tolerance <= 0.0)
+        return error(BA_ERR_INVALID_CONFIG, "gradient_tolerance");
+    if (!std::isfinite(c->parameter_tolerance) || c->parameter_tolerance <= 0.0)
+        return error(BA_ERR_INVALID_CONFIG, "parameter_tolerance");
+    if (c->linear_solver < BA_SOLVER_AUTO || c->linear_solver > BA_SOLVER_ITERATIVE)
+        return error(BA_ERR_INVALID_CONFIG, "linear_solver");
+    if (c->precision != BA_PRECISION_FP64 && c->precision != BA_PRECISION_MIXED)
+        return error(BA_ERR_INVALID_CONFIG, "precision");
+    if (!(c->pcg_relative_tolerance >= 0.0 && c->pcg_relative_tolerance < 1.0))
+        return error(BA_ERR_INVALID_CONFIG, "pcg_relative_tolerance");
+    return Status{};
+}
+
+Status validate_problem(const ba_problem* p) {
+    if (!p) return error(BA_ERR_INVALID_PROBLEM, "null problem");
+    if (p->n_poses == 0) return error(BA_ERR_INVALID_PROBLEM, "empty poses");
+    if (p->n_intrinsics == 0) return error(BA_ERR_INVALID_PROBLEM, "empty intrinsics");
+    if (p->n_points == 0) return error(BA_ERR_INVALID_PROBLEM, "empty points");
+    if (p->n_observations == 0) return error(BA_ERR_INVALID_PROBLEM, "empty observations");
+    for (size_t i = 0; i < p->n_poses; ++i) {
+        const double* R = p->pose_rotation + 9 * i;
+        const double* C = p->pose_center + 3 * i;
+        bool ok = std::isfinite(C[0]) && std::isfinite(C[1]) && std::isfinite(C[2]);
+        double rrt[9];
+        mul_abt(R, R, rrt);
+        for (int k = 0; k < 9 && ok; ++k)
+            ok = std::fabs(rrt[k] - ((k % 4 == 0) ? 1.0 : 0.0)) <= 9.5367431640625e-7;
+        if (!ok) return error(BA_ERR_INVALID_PROBLEM, "invalid pose " + std::to_string(i));
+    }
+    for (size_t i = 0; i < 3 * p->n_points; ++i)
+        if (!std::isfinite(p->point_xyz[i])) return error(BA_ERR_INVALID_PROBLEM, "non-finite point");
+    for (size_t o = 0; o < p->n_observations; ++o) {
+        if (p->obs_pose[o] >= p->n_poses) return error(BA_ERR_INVALID_PROBLEM, "observation pose out of bounds");
+        if (p->obs_point[o] >= p->n_points) return error(BA_ERR_INVALID_PROBLEM, "observation point out of bounds");
+        if (p->obs_intrinsic[o] >= p->n_intrinsics)
+            return error(BA_ERR_INVALID_PROBLEM, "observation intrinsic out of bounds");
+        if (!std::isfinite(p->obs_pixel[2 * o]) || !std::isfinite(p->obs_pixel[2 * o + 1]))
+            return error(BA_ERR_INVALID_PROBLEM, "non-finite observation");
+    }
+    if (p->robust_loss != BA_LOSS_SQUARED && p->residual_space != BA_RESIDUAL_PIXELS)
+        return error(BA_ERR_INVALID_PROBLEM, "loss unit mismatch");
+    if (p->robust_loss != BA_LOSS_SQUARED && !(std::isfinite(p->loss_scale) && p->loss_scale > 0.0))
+        return error(BA_ERR_INVALID_PROBLEM, "invalid loss scale");
+    if (p->gauge == BA_GAUGE_FIXED_BASELINE) {
+        if (p->gauge_first >= p->n_poses || p->gauge_second >= p->n_poses)
+            return error(BA_ERR_INVALID_PROBLEM, "gauge pose out of bounds");
+        if (p->gauge_first == p->gauge_second) return error(BA_ERR_INVALID_PROBLEM, "gauge uses one pose");
+        if (!std::isfinite(p->gauge_distance) || p->gauge_distance <= 0.0)
+            return error(BA_ERR_INVALID_PROBLEM, "invalid baseline distance");
+    }
+    return Status{};
+}
+
+Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern) {
+    const size_t P = p->n_poses, N = p->n_points, O = p->n_observations, G = p->n_intrinsics;
+    // ---- input quarantine (quarantine.rs:33-96) ----
+    L.n_input_points = N;
+    L.quarantined.assign(N, 0);
+    {
+        std::vector<uint32_t> count(N, 0);
+        for (size_t o = 0; o < O; ++o) count[p->obs_point[o]] += 1;
+        std::vector<double> off(3 * O), dist(O);
+        for (size_t o = 0; o < O; ++o) {
+            const double* X = p->point_xyz + 3 * p->obs_point[o];
+            const double* C = p->pose_center + 3 * p->obs_pose[o];
+            for (int k = 0; k < 3; ++k) off[3 * o + k] = X[k] - C[k];
+            dist[o] = std::sqrt(off[3 * o] * off[3 * o] + off[3 * o + 1] * off[3 * o + 1] + off[3 * o + 2] * off[3 * o + 2]);
+        }
+        size_t mid = O / 2;
+        std::nth_element(dist.begin(), dist.begin() + mid, dist.end());
+        double scene = dist[mid];
+        std::vector<Geometry> geo(N, Geometry(scene));
+        for (size_t o = 0; o < O; ++o) geo[p->obs_point[o]].push(&off[3 * o]);
+        for (size_t i = 0; i < N; ++i)
+            L.quarantined[i] = (count[i] < 2 || geo[i].degenerate()) && !is_fixed_point(p, i);
+    }
+    L.n_quarantined = 0;
+    for (auto q : L.quarantined) L.n_quarantined += q;
+    if (L.n_quarantined == N)
+        return error(BA_ERR_UNSUPPORTED,
+                     "no point can be adjusted: every point is seen fewer than twice or without parallax");
+    if (p->residual_space != BA_RESIDUAL_PIXELS)
+        return error(BA_ERR_UNSUPPORTED, "self-calibration requires pixel residuals");
+    std::vector<uint32_t> kept_obs;
+    kept_obs.reserve(O);
+    for (size_t o = 0; o < O; ++o)
+        if (!L.quarantined[p->obs_point[o]]) kept_obs.push_back((uint32_t)o);
+    // ---- native/mod.rs:457-497 ----
+    bool free_intrinsic = false;
+    for (uint32_t o : kept_obs) free_intrinsic |= !is_fixed_intrinsic(p->intrinsics[p->obs_intrinsic[o]]);
+    if (!free_intrinsic)
+        return error(BA_ERR_UNSUPPORTED, "every observed intrinsic is fixed: the calibrated path is not this adjuster's");
+    std::vector<int64_t> pose_intrinsic(P, -1);
+    std::vector<uint8_t> observed(P, 0);
+    for (uint32_t o : kept_obs) {
+        int64_t& slot = pose_intrinsic[p->obs_pose[o]];
+        if (slot >= 0 && slot != (int64_t)p->obs_intrinsic[o])
+            return error(BA_ERR_UNSUPPORTED, "native self-calibration requires one intrinsic group per observed pose");
+        slot = p->obs_intrinsic[o];
+        observed[p->obs_pose[o]] = 1;
+    }
+    for (auto& s : pose_intrinsic)
+        if (s < 0) s = 0;
+    // ---- common_pose_variables (gauge.rs:9-40) ----
+    for (size_t i = 0; i < P; ++i) {
+        if (!observed[i] || is_fixed_pose(p, i)) continue;
+        if (!p->pose_var_translation[i])
+            return error(BA_ERR_UNSUPPORTED, "direct native lowering cannot optimize rotation without translation");
+        if (!p->pose_var_rotation[i])
+            return error(BA_ERR_UNSUPPORTED, "self-calibration requires full-pose or fixed camera blocks");
+    }
+    // ---- intrinsic checks (native/self_calibration.rs:63-85) ----
+    for (size_t g = 0; g < G; ++g) {
+        const ba_intrinsic& in = p->intrinsics[g];
+        if (in.var_radial && in.model == BA_CAMERA_BROWN_K1K2K3)
+            return error(BA_ERR_UNSUPPORTED, "native self-calibration refines at most two declared radial coefficients");
+        if (!is_fixed_intrinsic(in) && (!in.var_focal || in.var_principal_point || in.var_tangential))
+            return error(BA_ERR_UNSUPPORTED,
+                         "self-calibration refines the focal length and radial coefficients with a fixed principal point");
+    }
+    // ---- gauge_plan (gauge.rs:84-107, 231-330) ----
+    bool observes_fixed_point = false;
+    for (uint32_t o : kept_obs) observes_fixed_point |= is_fixed_point(p, p->obs_point[o]);
+    size_t anchor = 0;
+    double scale = 1.0;
+    int64_t gauge_pose = -1;
+    if (p->gauge == BA_GAUGE_FREE) {
+        if (observes_fixed_point) {
+            bool fixed_observed = false;
+            for (size_t i = 0; i < P; ++i) fixed_observed |= observed[i] && is_fixed_pose(p, i);
+            if (!fixed_observed)
+                return error(BA_ERR_UNSUPPORTED,
+                             "native BA with fixed world points requires an explicitly fixed observed pose or fixed-baseline anchor");
+        }
+        int64_t first_observed = -1, first_fixed = -1;
+        for (size_t i = 0; i < P && first_fixed < 0; ++i) {
+            if (observed[i] && is_fixed_pose(p, i)) first_fixed = (int64_t)i;
+            if (observed[i] && first_observed < 0) first_observed = (int64_t)i;
+        }
+        anchor = (size_t)(first_fixed >= 0 ? first_fixed : (first_observed >= 0 ? first_observed : 0));
+        double ca[3];
+        center_of(p, anchor, ca);
+        bool scale_set = false;
+        for (size_t i = 0; i < P && !scale_set; ++i) {
+            double c[3];
+            center_of(p, i, c);
+            scale_set = i != anchor && observed[i] && is_fixed_pose(p, i) && sq_dist(c, ca) > 0.0;
+        }
+        for (size_t k = 0; k < kept_obs.size() && !scale_set; ++k) {
+            size_t q = p->obs_point[kept_obs[k]];
+            scale_set = is_fixed_point(p, q) && sq_dist(p->point_xyz + 3 * q, ca) > 0.0;
+        }
+        if (!scale_set) {  // farthest_camera: largest squared distance, ties to the lower index
+            double best = -1.0;
+            for (size_t i = 0; i < P; ++i) {
+                if (i == anchor || !observed[i] || is_fixed_pose(p, i)) continue;
+                double c[3];
+                center_of(p, i, c);
+                double d = sq_dist(c, ca);
+                if (std::isfinite(d) && d > best) {
+                    best = d;
+                    gauge_pose = (int64_t)i;
+                }
+            }
+            if (gauge_pose < 0 || !(best > 0.0))
+                return error(BA_ERR_UNSUPPORTED, "free-scale BA has no nonzero active baseline");
+        }
+    } else {
+        size_t first = p->gauge_first, second = p->gauge_second;
+        double c1[3], c2[3];
+        center_of(p, first, c1);
+        center_of(p, second, c2);
+        double current = std::sqrt(sq_dist(c2, c1));
+        if (!std::isfinite(current) || current <= 0.0)
+            return error(BA_ERR_UNSUPPORTED, "fixed-baseline cameras have no usable separation");
+        double raw = p->gauge_distance / current;
+        double tol = 256.0 * DBL_EPSILON * std::fmax(std::fmax(std::fabs(current), std::fabs(p->gauge_distance)), 1.0);
+        scale = std::fabs(current - p->gauge_distance) <= tol ? 1.0 : raw;
+        if (scale != 1.0) {
+            for (size_t i = 0; i < P; ++i)
+                if (i != first && !p->pose_var_translation[i])
+                    return error(BA_ERR_UNSUPPORTED, "fixed-baseline scaling would move fixed state");
+            for (size_t i = 0; i < N; ++i)
+                if (is_fixed_point(p, i)) return error(BA_ERR_UNSUPPORTED, "fixed-baseline scaling would move fixed state");
+            if (L.n_quarantined > 0)
+                return error(BA_ERR_UNSUPPORTED, "native BA cannot change its physical baseline while holding out points");
+        }
+        gauge_pose = p->pose_var_translation[second] ? (int64_t)second : -1;
+        anchor = first;
+    }
+    // ---- world rebase (world_rebase.rs:19-59) ----
+    std::memcpy(L.anchor_rotation, p->pose_rotation + 9 * anchor, sizeof(L.anchor_rotation));
+    center_of(p, anchor, L.anchor_center);
+    L.rebase_scale = scale;
+    L.camera_source.assign(1, (uint32_t)anchor);
+    for (size_t i = 0; i < P; ++i)
+        if (i != anchor) L.camera_source.push_back((uint32_t)i);
+    std::vector<uint32_t> dest(P);
+    for (size_t d = 0; d < P; ++d) dest[L.camera_source[d]] = (uint32_t)d;
+    L.R.assign(9 * P, 0.0);
+    L.t.assign(3 * P, 0.0);
+    for (size_t d = 0; d < P; ++d) {
+        size_t s = L.camera_source[d];
+        const double* R = p->pose_rotation + 9 * s;
+        const double* C = p->pose_center + 3 * s;
+        if (std::memcmp(R, L.anchor_rotation, sizeof(L.anchor_rotation)) == 0 && C[0] == L.anchor_center[0] &&
+            C[1] == L.anchor_center[1] && C[2] == L.anchor_center[2]) {
+            const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+            std::memcpy(&L.R[9 * d], I, sizeof(I));
+            continue;
+        }
+        mul_abt(R, L.anchor_rotation, &L.R[9 * d]);
+        double diff[3] = {L.anchor_center[0] - C[0], L.anchor_center[1] - C[1], L.anchor_center[2] - C[2]};
+        double rd[3];
+        mul_av(R, diff, rd);
+        for (int k = 0; k < 3; ++k) L.t[3 * d + k] = rd[k] * scale;
+    }
+    // ---- points without the quarantined (problem.rs:650-701) ----
+    std::vector<uint32_t> new_index(N, UINT32_MAX);
+    L.point_source.clear();
+    for (size_t i = 0; i < N; ++i)
+        if (!L.quarantined[i]) {
+            new_index[i] = (uint32_t)L.point_source.size();
+            L.point_source.push_back((uint32_t)i);
+        }
+    const size_t NP = L.point_source.size();
+    L.X.assign(3 * NP, 0.0);
+    L.point_fixed.assign(NP, 0);
+    for (size_t q = 0; q < NP; ++q) {
+        const double* Xi = p->point_xyz + 3 * L.point_source[q];
+        double diff[3] = {Xi[0] - L.anchor_center[0], Xi[1] - L.anchor_center[1], Xi[2] - L.anchor_center[2]};
+        double rd[3];
+        mul_av(L.anchor_rotation, diff, rd);
+        for (int k = 0; k < 3; ++k) L.X[3 * q + k] = rd[k] * scale;
+        L.point_fixed[q] = is_fixed_point(p, L.point_source[q]);
+    }
+    // ---- radial groups (radial_camera :483-512, free_radial_dimension :442-451) ----
+    L.gp.assign(3 * G, 0.0);
+    L.grp_fixed.assign(G, 0);
+    L.grp_free.assign(G, 1);
+    for (size_t g = 0; g < G; ++g) {
+        const ba_intrinsic& in = p->intrinsics[g];
+        if (in.model == BA_CAMERA_FISHEYE || in.model == BA_CAMERA_SPHERICAL)
+            return error(BA_ERR_UNSUPPORTED, "self-calibration supports perspective cameras only");
+        if (in.model < BA_CAMERA_PINHOLE || in.model > BA_CAMERA_SPHERICAL)
+            return error(BA_ERR_INVALID_PROBLEM, "unknown camera model");
+        bool brown = in.model != BA_CAMERA_PINHOLE;
+        double k1 = brown && in.model >= BA_CAMERA_BROWN_K1 ? in.k1 : 0.0;
+        double k2 = brown && in.model >= BA_CAMERA_BROWN_K1K2 ? in.k2 : 0.0;
+        double k3 = brown && in.model >= BA_CAMERA_BROWN_K1K2K3 ? in.k3 : 0.0;
+        double p1 = brown ? in.p1 : 0.0, p2 = brown ? in.p2 : 0.0;
+        if (in.fx != in.fy || k3 != 0.0 || p1 != 0.0 || p2 != 0.0)
+            return error(BA_ERR_UNSUPPORTED,
+                         "self-calibration requires isotropic focal length, k1+k2 radial distortion, and no tangential distortion");
+        L.gp[3 * g] = in.fx;
+        L.gp[3 * g + 1] = k1;
+        L.gp[3 * g + 2] = k2;
+        L.grp_fixed[g] = is_fixed_intrinsic(in);
+        if (brown && in.var_radial)
+            L.grp_free[g] = in.model == BA_CAMERA_BROWN_NONE ? 1 : (in.model == BA_CAMERA_BROWN_K1 ? 2 : 3);
+    }
+    L.cam_group.assign(P, 0);
+    L.cam_fixed.assign(P, 0);
+    for (size_t d = 0; d < P; ++d) {
+        size_t s = L.camera_source[d];
+        L.cam_group[d] = (uint32_t)pose_intrinsic[s];
+        L.cam_fixed[d] = s == anchor || is_fixed_pose(p, s);
+    }
+    L.gauge_camera = gauge_pose < 0 ? -1 : (int)dest[gauge_pose];
+    // ---- observations sorted by (point, input index), principal point subtracted ----
+    std::vector<uint32_t> order = kept_obs;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return new_index[p->obs_point[a]] < new_index[p->obs_point[b]];
+    });
+    const size_t NO = order.size();
+    L.obs_cam.resize(NO);
+    L.obs_point.resize(NO);
+    L.obs_source = order;
+    L.uv.resize(2 * NO);
+    for (size_t k = 0; k < NO; ++k) {
+        uint32_t s = order[k];
+        const ba_intrinsic& in = p->intrinsics[p->obs_intrinsic[s]];
+        L.obs_cam[k] = dest[p->obs_pose[s]];
+        L.obs_point[k] = new_index[p->obs_point[s]];
+        L.uv[2 * k] = p->obs_pixel[2 * s] - in.cx;
+        L.uv[2 * k + 1] = p->obs_pixel[2 * s + 1] - in.cy;
+    }
+    L.point_off.assign(NP + 1, 0);
+    for (size_t k = 0; k < NO; ++k) L.point_off[L.obs_point[k] + 1] += 1;
+    for (size_t q = 0; q < NP; ++q) L.point_off[q + 1] += L.point_off[q];
+    // ---- loss (problem.rs:372-382) ----
+    L.loss = p->robust_loss;
+    L.loss_scale = p->robust_loss == BA_LOSS_CAUCHY ? p->loss_scale / std::sqrt(2.0) : p->loss_scale;
+    // ---- layout (topology.rs:351-397) ----
+    L.cam_off.assign(P, 0);
+    L.cam_dim.assign(P, 0);
+    size_t D = 0;
+    for (size_t c = 0; c < P; ++c) {
+        uint32_t local = (c == 0 || L.cam_fixed[c]) ? 0u : (L.gauge_camera == (int)c ? 5u : 6u);
+        L.cam_off[c] = (uint32_t)D;
+        L.cam_dim[c] = local;
+        D += local;
+    }
+    L.grp_off.assign(G, 0);
+    L.grp_dim.assign(G, 0);
+    for (size_t g = 0; g < G; ++g) {
+        uint32_t local = L.grp_fixed[g] ? 0u : L.grp_free[g];
+        L.grp_off[g] = (uint32_t)D;
+        L.grp_dim[g] = local;
+        D += local;
+    }
+    L.D = D;
+    // ---- scene scale at the initial state (optimizer.rs:620-627) ----
+    if (L.gauge_camera >= 0) {
+        const double* tg = &L.t[3 * L.gauge_camera];
+        L.scene_scale = std::sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
+    } else {
+        double s = 0.0;
+        for (size_t q = 0; q < NP; ++q)
+            s += L.X[3 * q] * L.X[3 * q] + L.X[3 * q + 1] * L.X[3 * q + 1] + L.X[3 * q + 2] * L.X[3 * q + 2];
+        L.scene_scale = std::fmax(std::sqrt(s / (double)NP), DBL_MIN);
+    }
+    // ---- per-point active parameters, first-encounter order (pattern.rs:306-379) ----
+    L.cross_off.assign(NP + 1, 0);
+    L.params.clear();
+    {
+        std::vector<uint8_t> mark(D, 0);
+        size_t start = 0;
+        for (size_t q = 0; q < NP; ++q) {
+            if (!L.point_fixed[q])
+                for (uint32_t o = L.point_off[q]; o < L.point_off[q + 1]; ++o) {
+                    uint32_t c = L.obs_cam[o], g = L.cam_group[c];
+                    for (uint32_t k = 0; k < L.cam_dim[c]; ++k)
+                        if (!mark[L.cam_off[c] + k]) {
+                            mark[L.cam_off[c] + k] = 1;
+                            L.params.push_back(L.cam_off[c] + k);
+                        }
+                    for (uint32_t k = 0; k < L.grp_dim[g]; ++k)
+                        if (!mark[L.grp_off[g] + k]) {
+                            mark[L.grp_off[g] + k] = 1;
+                            L.params.push_back(L.grp_off[g] + k);
+                        }
+                }
+            for (size_t k = start; k < L.params.size(); ++k) mark[L.params[k]] = 0;
+            start = L.params.size();
+            L.cross_off[q + 1] = (uint32_t)start;
+        }
+    }
+    if (!build_pattern) return Status{};
+    // ---- reduced block pattern (pattern.rs:381-468) ----
+    const size_t NB = P + G;
+    auto bdim = [&](size_t b) { return b < P ? L.cam_dim[b] : L.grp_dim[b - P]; };
+    auto boff = [&](size_t b) { return b < P ? L.cam_off[b] : L.grp_off[b - P]; };
+    std::vector<std::vector<uint32_t>> rows(NB);
+    for (size_t b = 0; b < NB; ++b)
+        if (bdim(b)) rows[b].push_back((uint32_t)b);
+    std::vector<uint32_t> active;
+    for (size_t q = 0; q < NP; ++q) {
+        active.clear();
+        for (uint32_t o = L.point_off[q]; o < L.point_off[q + 1]; ++o) {
+            uint32_t c = L.obs_cam[o], g = L.cam_group[c];
+            if (L.point_fixed[q]) {
+                if (L.cam_dim[c] && L.grp_dim[g]) rows[c].push_back((uint32_t)(P + g));
+                continue;
+            }
+            if (L.cam_dim[c]) active.push_back(c);
+            if (L.grp_dim[g]) active.push_back((uint32_t)(P + g));
+        }
+        std::sort(active.begin(), active.end());
+        active.erase(std::unique(active.begin(), active.end()), active.end());
+        for (size_t r = 0; r < active.size(); ++r)
+            for (size_t c = 0; c <= r; ++c) rows[active[c]].push_back(active[r]);
+    }
+    L.col_ptr.assign(1, 0);
+    L.row_idx.clear();
+    for (size_t cb = 0; cb < NB; ++cb) {
+        auto& rb = rows[cb];
+        std::sort(rb.begin(), rb.end());
+        rb.erase(std::unique(rb.begin(), rb.end()), rb.end());
+        for (uint32_t cl = 0; cl < bdim(cb); ++cl) {
+            for (uint32_t r : rb) {
+                for (uint32_t k = (r == cb ? cl : 0u); k < bdim(r); ++k) L.row_idx.push_back(boff(r) + k);
+            }
+            L.col_ptr.push_back((uint32_t)L.row_idx.size());
+        }
+    }
+    return Status{};
+}
+
+BlockLists build_block_lists(const Lowered& L) {
+    BlockLists B;
+    const size_t P = L.n_cameras(), G = L.n_groups();
+    B.n_blocks = P + G;
+    B.block_off.resize(B.n_blocks);
+    B.block_dim.resize(B.n_blocks);
+    for (size_t c = 0; c < P; ++c) {
+        B.block_off[c] = L.cam_off[c];
+        B.block_dim[c] = L.cam_dim[c];
+    }
+    for (size_t g = 0; g < G; ++g) {
+        B.block_off[P + g] = L.grp_off[g];
+        B.block_dim[P + g] = L.grp_dim[g];
+    }
+    // counting sort by block keeps the observation index ascending inside each block
+    std::vector<uint32_t> count(B.n_blocks + 1, 0);
+    const size_t NO = L.n_obs();
+    for (size_t o = 0; o < NO; ++o) {
+        uint32_t c = L.obs_cam[o];
+        if (L.cam_dim[c]) count[c + 1]++;
+        uint32_t g = L.cam_group[c];
+        if (L.grp_dim[g]) count[P + g + 1]++;
+    }
+    for (size_t b = 0; b < B.n_blocks; ++b) count[b + 1] += count[b];
+    B.obs_off = count;
+    B.obs.resize(count[B.n_blocks]);
+    std::vector<uint32_t> cursor(count.begin(), count.end() - 1);
+    for (size_t o = 0; o < NO; ++o) {
+        uint32_t c = L.obs_cam[o];
+        if (L.cam_dim[c]) B.obs[cursor[c]++] = (uint32_t)o;
+        uint32_t g = L.cam_group[c];
+        if (L.grp_dim[g]) B.obs[cursor[P + g]++] = (uint32_t)o;
+    }
+    return B;
+}
+
+PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all) {
+    PairLists out;
+    const size_t P = L.n_cameras(), NB = B.n_blocks;
+    // Enumerate terms point by point; key each by its block pair, then bucket by key with a
+    // stable counting sort so every pair's terms stay in (point, i, j) order.
+    struct Term {
+        uint64_t key;
+        uint32_t i, j;
+    };
+    std::vector<Term> terms;
+    const size_t NP = L.n_points();
+    auto blocks_of = [&](uint32_t o, uint32_t out_b[2]) {
+        uint32_t c = L.obs_cam[o], g = L.cam_group[c];
+        int n = 0;
+        if (L.cam_dim[c]) out_b[n++] = c;
+        if (L.grp_dim[g]) out_b[n++] = (uint32_t)(P + g);
+        return n;
+    };
+    for (size_t q = 0; q < NP; ++q) {
+        uint32_t b0 = L.point_off[q], b1 = L.point_off[q + 1];
+        bool fixed = L.point_fixed[q];
+        for (uint32_t i = b0; i < b1; ++i) {
+            uint32_t bi[2];
+            int ni = blocks_of(i, bi);
+            for (uint32_t j = b0; j < b1; ++j) {
+                if (fixed && i != j) continue;
+                uint32_t bj[2];
+                int nj = blocks_of(j, bj);
+                for (int x = 0; x < ni; ++x)
+                    for (int y = 0; y < nj; ++y) {
+                        uint32_t A = bi[x], Bb = bj[y];
+                        if (A < Bb) continue;
+                        if (!all) {
+                            bool diag = A == Bb;
+                            bool edge = i == j && A >= P && Bb < P && L.cam_group[Bb] == A - P;
+                            if (!diag && !edge) continue;
+                            if (edge && !diag) {  // edges carry only the U Gram term
+                                terms.push_back(Term{(uint64_t)A * NB + Bb, i, j | TERM_U});
+                                continue;
+                            }
+                        }
+                        uint32_t flags = (fixed ? 0u : TERM_SCHUR) | (i == j ? TERM_U : 0u);
+                        if (flags) terms.push_back(Term{(uint64_t)A * NB + Bb, i, j | flags});
+                    }
+            }
+        }
+    }
+    // distinct keys in ascending order
+    std::vector<uint64_t> keys;
+    keys.reserve(terms.size());
+    for (auto& t : terms) keys.push_back(t.key);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const size_t NPAIR = keys.size();
+    out.block_a.resize(NPAIR);
+    out.block_b.resize(NPAIR);
+    out.term_off.assign(NPAIR + 1, 0);
+    out.value_off.assign(NPAIR + 1, 0);
+    std::vector<uint32_t> pair_of(terms.size());
+    for (size_t k = 0; k < terms.size(); ++k) {
+        size_t idx = (size_t)(std::lower_bound(keys.begin(), keys.end(), terms[k].key) - keys.begin());
+        pair_of[k] = (uint32_t)idx;
+        out.term_off[idx + 1]++;
+    }
+    for (size_t k = 0; k < NPAIR; ++k) {
+        out.block_a[k] = (uint32_t)(keys[k] / NB);
+        out.block_b[k] = (uint32_t)(keys[k] % NB);
+        out.term_off[k + 1] += out.term_off[k];
+        out.value_off[k + 1] = out.value_off[k] + (uint64_t)B.block_dim[out.block_a[k]] * B.block_dim[out.block_b[k]];
+    }
+    out.term_i.resize(terms.size());
+    out.term_j.resize(terms.size());
+    std::vector<uint32_t> cursor(out.term_off.begin(), out.term_off.end() - 1);
+    for (size_t k = 0; k < terms.size(); ++k) {
+        uint32_t at = cursor[pair_of[k]]++;
+        out.term_i[at] = terms[k].i;
+        out.term_j[at] = terms[k].j;
+    }
+    return out;
+}
+
+void raise_solution(const ba_problem* p, const Lowered& L, const double* R, const double* t, const double* gp,
+                    const double* X, ba_solution* out) {
+    const size_t P = p->n_poses;
+    std::memcpy(out->pose_rotation, p->pose_rotation, 9 * P * sizeof(double));
+    std::memcpy(out->pose_center, p->pose_center, 3 * P * sizeof(double));
+    for (size_t d = 0; d < L.n_cameras(); ++d) {
+        size_t s = L.camera_source[d];
+        if (is_fixed_pose(p, s)) continue;
+        const double* Rn = R + 9 * d;
+        mul_ab(Rn, L.anchor_rotation, out->pose_rotation + 9 * s);
+        double rc[3], rel[3], w[3];
+        mul_atv(Rn, t + 3 * d, rc);
+        for (int k = 0; k < 3; ++k) rel[k] = -rc[k];
+        mul_atv(L.anchor_rotation, rel, w);
+        for (int k = 0; k < 3; ++k) out->pose_center[3 * s + k] = L.anchor_center[k] + w[k];
+    }
+    std::memcpy(out->point_xyz, p->point_xyz, 3 * p->n_points * sizeof(double));
+    for (size_t q = 0; q < L.n_points(); ++q) {
+        size_t s = L.point_source[q];
+        if (is_fixed_point(p, s)) continue;
+        double w[3];
+        mul_atv(L.anchor_rotation, X + 3 * q, w);
+        for (int k = 0; k < 3; ++k) out->point_xyz[3 * s + k] = L.anchor_center[k] + w[k];
+    }
+    if (out->point_quarantined)
+        for (size_t i = 0; i < p->n_points; ++i) out->point_quarantined[i] = L.quarantined[i];
+    for (size_t g = 0; g < p->n_intrinsics; ++g) {
+        const ba_intrinsic& in = p->intrinsics[g];
+        if (L.grp_fixed[g]) {
+            out->intrinsic_focal[g] = in.fx;
+            out->intrinsic_k1[g] = in.model >= BA_CAMERA_BROWN_K1 && in.model <= BA_CAMERA_BROWN_K1K2K3 ? in.k1 : 0.0;
+            out->intrinsic_k2[g] = in.model >= BA_CAMERA_BROWN_K1K2 && in.model <= BA_CAMERA_BROWN_K1K2K3 ? in.k2 : 0.0;
+        } else {
+            out->intrinsic_focal[g] = gp[3 * g];
+            out->intrinsic_k1[g] = gp[3 * g + 1];
+            out->intrinsic_k2[g] = gp[3 * g + 2];
+        }
+        if (out->intrinsic_unconstrained) out->intrinsic_unconstrained[g] = 0;
+    }
+}
+
+}  // namespace ba

@@ -1,0 +1,89 @@
+// ba_host.hpp — host side of the MI355X bundle adjuster: lowering of a BundleProblem to the
+// native self-calibrating problem, its index maps, and the lists that drive the
+// deterministic reductions on the GPU.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ba_hip.h"
+
+namespace ba {
+
+struct Status {
+    int code = BA_OK;
+    std::string message;
+    bool ok() const { return code == BA_OK; }
+};
+inline Status error(int code, std::string message) { return Status{code, std::move(message)}; }
+
+// The lowered problem: PreparedSelfCalibratingProblem (native/self_calibration.rs:30-44) plus the
+// workspace layout and index maps (workspace/topology.rs:351-397, workspace/pattern.rs:306-468).
+struct Lowered {
+    // input bookkeeping
+    size_t n_input_points = 0, n_quarantined = 0;
+    std::vector<uint8_t> quarantined;      // per input point
+    std::vector<uint32_t> point_source;    // native point -> input point
+    std::vector<uint32_t> camera_source;   // native camera -> input pose
+    std::vector<uint32_t> obs_source;      // native observation -> input observation
+    double anchor_rotation[9];             // WorldRebase (native/world_rebase.rs:12-87)
+    double anchor_center[3];
+    double rebase_scale = 1.0;
+    // native state, anchor-first cameras
+    std::vector<double> R, t, gp, X, uv;   // 9/cam, 3/cam, 3/group, 3/point, 2/obs
+    std::vector<uint32_t> cam_group, obs_cam, obs_point, point_off;
+    std::vector<uint8_t> cam_fixed, point_fixed, grp_fixed;
+    std::vector<uint32_t> grp_free;
+    int gauge_camera = -1;
+    int loss = 0;
+    double loss_scale = 0.0;               // kernel scale (Cauchy: scale / sqrt 2)
+    double scene_scale = 1.0;
+    // layout
+    size_t D = 0;
+    std::vector<uint32_t> cam_off, cam_dim, grp_off, grp_dim;
+    // reference index maps
+    std::vector<uint32_t> cross_off, params, col_ptr, row_idx;
+    size_t n_cameras() const { return cam_group.size(); }
+    size_t n_groups() const { return gp.size() / 3; }
+    size_t n_points() const { return point_fixed.size(); }
+    size_t n_obs() const { return obs_cam.size(); }
+};
+
+Status validate_problem(const ba_problem* p);
+Status validate_config(const ba_solve_config* c);
+Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern);
+
+// Block b < n_cameras is the pose block of camera b; block n_cameras + g the intrinsic block
+// of group g.  Rows of a per-observation 9-row camera Jacobian: pose rows 0..5, intrinsic 6..8.
+struct BlockLists {
+    size_t n_blocks = 0;
+    std::vector<uint32_t> block_off, block_dim;  // parameter offset and dimension
+    std::vector<uint32_t> obs_off;               // n_blocks+1: observations touching a block
+    std::vector<uint32_t> obs;                   // ascending observation index per block
+};
+BlockLists build_block_lists(const Lowered& L);
+
+// Lower-triangle block pairs (A >= B) of the reduced camera system and their terms.  A term
+// (i, j) is an ordered observation pair of one free point whose camera blocks hold A (from i)
+// and B (from j); FLAG_U marks i == j, where the pair also carries the U Gram term.
+constexpr uint32_t TERM_U = 1u << 30;       // add J~_i[A]^T J~_i[B]
+constexpr uint32_t TERM_SCHUR = 1u << 31;   // subtract W_i[A] V^-1 W_j[B]^T
+constexpr uint32_t TERM_INDEX = (1u << 30) - 1;
+struct PairLists {
+    std::vector<uint32_t> block_a, block_b;     // per pair
+    std::vector<uint32_t> term_off;             // n_pairs+1
+    std::vector<uint32_t> term_i, term_j;       // term_j carries the flags in its top bits
+    std::vector<uint64_t> value_off;            // n_pairs+1: offset of the pair's dA*dB values
+};
+// all = every block pair of the pattern (direct solve); otherwise the diagonal blocks and the
+// pose-intrinsic edges of each camera (the retained operator and its preconditioner).
+PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all);
+
+// Raise the native state back into the caller's frame (native/self_calibration.rs:375-436,
+// native/world_rebase.rs:61-86).
+void raise_solution(const ba_problem* p, const Lowered& L, const double* R, const double* t,
+                    const double* gp, const double* X, ba_solution* out);
+
+}  // namespace ba

@@ -1,0 +1,1204 @@
+// ba_kernels.hip — HIP kernels (gfx950) of the Levenberg–Marquardt inner loop.
+//
+//   (a)  k_linearize      one lane per observation: robust-weighted residual, point and camera
+//                         Jacobians (schur/point.rs:239-355, self_calibration/model.rs:26-129)
+//   (b)  k_point_elim     one lane per point: V, g_p, damped 3x3 inverse, V^-1 g_p and the
+//                         retained rows R = V^-1 W (schur/assembly.rs:893-1039, point_block.rs)
+//        k_block_gradient one workgroup per camera block: reduced/full gradient and diagonal
+//        k_pairs          one workgroup per lower block pair of the reduced system: U and
+//                         W V^-1 W^T by a segmented reduction over pre-sorted observation pairs
+//   (a') k_candidate      one lane per point: cached back-substitution and the candidate rule
+//                         (sparse/self_calibration/candidate.rs:25-233, point_geometry.rs:153-168)
+//   (c)  k_pcg_*          implicit-Schur block-Jacobi PCG (self_calibration_retained.rs:512-887,
+//                         pcg.rs:23-132) with its control state kept on the device
+// Every reduction runs over a fixed partition in a fixed order; no floating-point atomics.
+#include <hip/hip_runtime.h>
+
+#include "ba_kernels.hpp"
+#include "ba_math.hpp"
+
+namespace ba {
+
+namespace {
+
+constexpr int RED_BLOCKS = 512;  // fixed first-stage partition of the scalar reductions
+constexpr int NT = 256;
+
+inline unsigned grid_for(size_t n, int nt) { return (unsigned)((n + nt - 1) / nt); }
+
+// Fixed-shape tree sum over the NT threads of a block.
+template <int N>
+__device__ double tree_sum(double v, double* sh) {
+    int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = N / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] += sh[t + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+template <int N>
+__device__ double tree_max(double v, double* sh) {
+    int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = N / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+template <int N>
+__device__ double tree_min(double v, double* sh) {
+    int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = N / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = fmin(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Parameter index of row k (0..8) of observation o's 9-row camera Jacobian, or -1.
+__device__ __forceinline__ int row_param(const Topo& tp, uint32_t cam, int k) {
+    if (k < 6) return k < (int)tp.cam_dim[cam] ? (int)(tp.cam_off[cam] + k) : -1;
+    uint32_t g = tp.cam_group[cam];
+    int l = k - 6;
+    return l < (int)tp.grp_dim[g] ? (int)(tp.grp_off[g] + l) : -1;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------------------------
+// Gauge state (schur/layout.rs:101-113): baseline, direction and S^2 tangent basis of the
+// FixedBaseline camera at the current state.
+// ----------------------------------------------------------------------------------------------
+template <class T>
+__global__ void k_gauge(const T* cam_t, int camera, double scene_scale, GaugeDev* out) {
+    GaugeDev g;
+    g.camera = camera;
+    g.scene_scale = scene_scale;
+    g.baseline = 0.0;
+    for (int k = 0; k < 3; ++k) g.direction[k] = 0.0;
+    for (int k = 0; k < 6; ++k) g.basis[k] = 0.0;
+    if (camera >= 0) {
+        double t[3] = {(double)cam_t[3 * camera], (double)cam_t[3 * camera + 1], (double)cam_t[3 * camera + 2]};
+        double b = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        g.baseline = b;
+        double dir[3] = {t[0] / b, t[1] / b, t[2] / b};
+        for (int k = 0; k < 3; ++k) g.direction[k] = dir[k];
+        // s2_tangent_basis (numerics/manifold.rs:129-141)
+        double n = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        double tv[3] = {dir[0] / n, dir[1] / n, dir[2] / n};
+        double ref[3] = {0, 0, 0};
+        double a0 = fabs(tv[0]), a1 = fabs(tv[1]), a2 = fabs(tv[2]);
+        if (a0 <= a1 && a0 <= a2)
+            ref[0] = 1.0;
+        else if (a1 <= a2)
+            ref[1] = 1.0;
+        else
+            ref[2] = 1.0;
+        double c0[3] = {tv[1] * ref[2] - tv[2] * ref[1], tv[2] * ref[0] - tv[0] * ref[2],
+                        tv[0] * ref[1] - tv[1] * ref[0]};
+        double n0 = sqrt(c0[0] * c0[0] + c0[1] * c0[1] + c0[2] * c0[2]);
+        double b0[3] = {c0[0] / n0, c0[1] / n0, c0[2] / n0};
+        double c1[3] = {tv[1] * b0[2] - tv[2] * b0[1], tv[2] * b0[0] - tv[0] * b0[2], tv[0] * b0[1] - tv[1] * b0[0]};
+        double n1 = sqrt(c1[0] * c1[0] + c1[1] * c1[1] + c1[2] * c1[2]);
+        for (int k = 0; k < 3; ++k) {
+            g.basis[k] = b0[k];
+            g.basis[3 + k] = c1[k] / n1;
+        }
+    }
+    *out = g;
+}
+
+// ----------------------------------------------------------------------------------------------
+// (a) k_linearize: one lane per observation.
+// State is FP64; the camera-frame point, residual and robust cost are evaluated in FP64 and the
+// Jacobians in T.  Outputs are robust-weighted: r~ = sqrt(w) r, J~ = sqrt(w) J.
+// ----------------------------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(NT) k_linearize(Topo tp, const double* __restrict__ Rg, const double* __restrict__ tg,
+                                                  const double* __restrict__ gp, const double* __restrict__ X,
+                                                  const double* __restrict__ uv, const GaugeDev* __restrict__ gauge,
+                                                  Lin<T> out) {
+    uint32_t o = blockIdx.x * NT + threadIdx.x;
+    if (o >= tp.no) return;
+    const uint32_t c = tp.obs_cam[o], p = tp.obs_pt[o], g = tp.cam_group[c];
+    const double* R = Rg + 9 * (size_t)c;
+    double Xp[3] = {X[3 * (size_t)p], X[3 * (size_t)p + 1], X[3 * (size_t)p + 2]};
+    double rot[3];
+    rotate(R, Xp, rot);
+    double cp[3] = {rot[0] + tg[3 * c], rot[1] + tg[3 * c + 1], rot[2] + tg[3 * c + 2]};
+    const double f = gp[3 * g], k1 = gp[3 * g + 1], k2 = gp[3 * g + 2];
+    double turn = 0.0;
+    bool has_turn = first_turn(k1, k2, &turn);
+    double res[2], J[2][3], Ji[2][3];
+    bool ok = radial_linearize<double>(f, k1, k2, turn, has_turn, cp, uv[2 * (size_t)o], uv[2 * (size_t)o + 1], res, J, Ji);
+    T* rj = out.rj + 8 * (size_t)o;
+    T* jc = out.jc + 18 * (size_t)o;
+    if (!ok) {
+        out.valid[o] = 0;
+        out.cost[o] = T(0);
+        out.sq[o] = T(0);
+        for (int k = 0; k < 8; ++k) rj[k] = T(0);
+        for (int k = 0; k < 18; ++k) jc[k] = T(0);
+        return;
+    }
+    double rn = sqrt(res[0] * res[0] + res[1] * res[1]);
+    double sw = loss_sqrt_weight<double>(tp.loss, tp.loss_scale, rn);
+    out.cost[o] = (T)loss_cost<double>(tp.loss, tp.loss_scale, rn);
+    out.sq[o] = (T)(rn * rn);
+    out.valid[o] = 1;
+    const double s = gauge->scene_scale;
+    rj[0] = (T)(sw * res[0]);
+    rj[1] = (T)(sw * res[1]);
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double v = J[r][0] * R[k] + J[r][1] * R[3 + k] + J[r][2] * R[6 + k];
+            rj[2 + 3 * r + k] = (T)(sw * (v * s));
+        }
+    // pose motion (schur/point.rs:730-762): rotation columns -[RX]_x, translation columns
+    const uint32_t pd = tp.cam_dim[c];
+    double mot[3][6] = {{0.0, rot[2], -rot[1], 0, 0, 0}, {-rot[2], 0.0, rot[0], 0, 0, 0}, {rot[1], -rot[0], 0.0, 0, 0, 0}};
+    if ((int)c == gauge->camera) {
+        for (int q = 0; q < 3; ++q) {
+            mot[q][3] = gauge->baseline * gauge->basis[q];
+            mot[q][4] = gauge->baseline * gauge->basis[3 + q];
+        }
+    } else {
+        for (int q = 0; q < 3; ++q) mot[q][3 + q] = s;
+    }
+    const uint32_t idim = tp.grp_dim[g];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            double v = (k < (int)pd) ? (J[r][0] * mot[0][k] + J[r][1] * mot[1][k] + J[r][2] * mot[2][k]) : 0.0;
+            jc[9 * r + k] = (T)(sw * v);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) jc[9 * r + 6 + k] = (T)(k < (int)idim ? sw * Ji[r][k] : 0.0);
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// (b) k_point_elim: one lane per point.
+// ----------------------------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(NT) k_point_elim(Topo tp, Lin<T> lin, T lambda, int want_rcond, PointElim<T> pe) {
+    uint32_t p = blockIdx.x * NT + threadIdx.x;
+    if (p >= tp.np) return;
+    const uint32_t b = tp.pt_off[p], e = tp.pt_off[p + 1];
+    T V[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    T gv[3] = {T(0), T(0), T(0)};
+    double cost = 0.0, sq = 0.0;
+    bool lin_ok = true;
+    for (uint32_t o = b; o < e; ++o) {
+        if (!lin.valid[o]) {
+            lin_ok = false;
+            break;
+        }
+        const T* rj = lin.rj + 8 * (size_t)o;
+        cost += (double)lin.cost[o];
+        sq += (double)lin.sq[o];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            T wr = rj[r];
+            T a0 = rj[2 + 3 * r], a1 = rj[3 + 3 * r], a2 = rj[4 + 3 * r];
+            gv[0] += a0 * wr;
+            gv[1] += a1 * wr;
+            gv[2] += a2 * wr;
+            V[0] += a0 * a0;
+            V[1] += a1 * a0;
+            V[2] += a1 * a1;
+            V[3] += a2 * a0;
+            V[4] += a2 * a1;
+            V[5] += a2 * a2;
+        }
+    }
+    double* ps = pe.pscal + 6 * (size_t)p;
+    T* yp = pe.y + 3 * (size_t)p;
+    if (!lin_ok) {
+        ps[0] = 0.0;
+        ps[1] = 0.0;
+        ps[2] = 0.0;
+        ps[3] = 0.0;
+        ps[4] = 1.0;
+        ps[5] = 0.0;  // neither linearised nor eliminated
+        yp[0] = yp[1] = yp[2] = T(0);
+        return;
+    }
+    ps[0] = cost;
+    ps[1] = sq;
+    if (tp.pt_fixed[p]) {
+        ps[2] = 0.0;
+        ps[3] = 0.0;
+        ps[4] = 1.0;
+        ps[5] = 3.0;  // linearised; nothing to eliminate
+        yp[0] = yp[1] = yp[2] = T(0);
+        return;
+    }
+    // scaled gradient norm of the point block (schur/assembly.rs:1081-1096)
+    double d3[3] = {(double)V[0], (double)V[2], (double)V[5]};
+    double g3[3] = {(double)gv[0], (double)gv[1], (double)gv[2]};
+    double dscale = fmax(fmax(fmax(0.0, d3[0]), d3[1]), d3[2]);
+    double sgin;
+    if (!(isfinite(dscale) && dscale > 0.0)) {
+        sgin = (g3[0] == 0.0 && g3[1] == 0.0 && g3[2] == 0.0) ? 0.0 : INFINITY;
+    } else {
+        double fl = 64.0 * DBL_EPSILON * dscale;
+        sgin = 0.0;
+        for (int k = 0; k < 3; ++k) sgin = fmax(sgin, fabs(g3[k]) / sqrt(fmax(d3[k], fl)));
+    }
+    ps[3] = sgin;
+    // damping diagonal and damped block (point_block.rs:90-110)
+    T tscale = fmax(fmax(fmax(T(0), V[0]), V[2]), V[5]);
+    bool elim = isfinite(tscale) && tscale > T(0);
+    T fl = T(64) * Eps<T>::value * tscale;
+    T dd[3] = {fmax(V[0], fl), fmax(V[2], fl), fmax(V[5], fl)};
+    T Vd[6] = {V[0] + lambda * dd[0], V[1], V[2] + lambda * dd[1], V[3], V[4], V[5] + lambda * dd[2]};
+    T inv[6];
+    elim = elim && point_block_inverse(Vd, inv);
+    if (!elim) {
+        ps[2] = 0.0;
+        ps[4] = 0.0;
+        ps[5] = (lambda == T(0)) ? 5.0 : 1.0;  // linearised, not eliminated, weak at zero damping
+        yp[0] = yp[1] = yp[2] = T(0);
+        return;
+    }
+    double rcond = 1.0;
+    bool weak = false;
+    if (want_rcond) {  // PointBlockFactor::reciprocal_condition_number (point_block.rs:170-183)
+        double m[9] = {(double)V[0], (double)V[1], (double)V[3], (double)V[1], (double)V[2],
+                       (double)V[4], (double)V[3], (double)V[4], (double)V[5]};
+        double iv[9] = {(double)inv[0], (double)inv[1], (double)inv[3], (double)inv[1], (double)inv[2],
+                        (double)inv[4], (double)inv[3], (double)inv[4], (double)inv[5]};
+        double nm = 0.0, ni = 0.0;
+        for (int r = 0; r < 3; ++r) {
+            double s1 = 0.0, s2 = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                s1 += fabs(m[3 * r + c]);
+                s2 += fabs(iv[3 * r + c]);
+            }
+            nm = fmax(nm, s1);
+            ni = fmax(ni, s2);
+        }
+        double den = nm * ni;
+        rcond = (isfinite(den) && den > 0.0) ? 1.0 / den : 0.0;
+        weak = rcond <= 64.0 * DBL_EPSILON * 3.0;
+    }
+    T y[3];
+    sym3_mul(inv, gv, y);
+    ps[2] = 0.5 * (g3[0] * (double)y[0] + g3[1] * (double)y[1] + g3[2] * (double)y[2]);
+    ps[4] = rcond;
+    ps[5] = weak ? 7.0 : 3.0;
+    for (int k = 0; k < 3; ++k) {
+        yp[k] = y[k];
+        pe.ddiag[3 * (size_t)p + k] = dd[k];
+    }
+    for (int k = 0; k < 6; ++k) {
+        pe.vinv[6 * (size_t)p + k] = inv[k];
+        pe.vdamp[6 * (size_t)p + k] = Vd[k];
+    }
+    // retained rows: W_o = J~_c^T J~_X per observation (9 x 3), R_o = V^-1 W_o
+    for (uint32_t o = b; o < e; ++o) {
+        const T* rj = lin.rj + 8 * (size_t)o;
+        const T* jc = lin.jc + 18 * (size_t)o;
+        T* rr = pe.rrows + 27 * (size_t)o;
+        T x0[3] = {rj[2], rj[3], rj[4]}, x1[3] = {rj[5], rj[6], rj[7]};
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            T c0 = jc[k], c1 = jc[9 + k];
+            T w[3] = {x0[0] * c0 + x1[0] * c1, x0[1] * c0 + x1[1] * c1, x0[2] * c0 + x1[2] * c1};
+            T v[3];
+            sym3_mul(inv, w, v);
+            rr[3 * k] = v[0];
+            rr[3 * k + 1] = v[1];
+            rr[3 * k + 2] = v[2];
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Deterministic scalar reductions over points.
+// ----------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT) k_reduce_points_1(const double* __restrict__ ps, uint32_t np, double* partials) {
+    __shared__ double sh[NT];
+    uint32_t b0 = (uint32_t)((uint64_t)np * blockIdx.x / RED_BLOCKS);
+    uint32_t b1 = (uint32_t)((uint64_t)np * (blockIdx.x + 1) / RED_BLOCKS);
+    double cost = 0, sq = 0, pmr = 0, sgin = 0, rc = 1.0, weak = 0, bad = 0, unproj = 0;
+    for (uint32_t p = b0 + threadIdx.x; p < b1; p += NT) {
+        const double* s = ps + 6 * (size_t)p;
+        int flags = (int)s[5];
+        if (flags & 1) {
+            cost += s[0];
+            sq += s[1];
+        }
+        if ((flags & 3) == 3) {
+            pmr += s[2];
+            sgin = fmax(sgin, s[3]);
+        }
+        if ((flags & 1) && !(flags & 2)) sgin = fmax(sgin, s[3]);
+        rc = fmin(rc, s[4]);
+        if (flags & 4) weak += 1.0;
+        if ((flags & 3) != 3) bad += 1.0;
+        if (!(flags & 1)) unproj += 1.0;
+    }
+    double v[8];
+    v[0] = tree_sum<NT>(cost, sh);
+    v[1] = tree_sum<NT>(sq, sh);
+    v[2] = tree_sum<NT>(pmr, sh);
+    v[3] = tree_max<NT>(sgin, sh);
+    v[4] = tree_min<NT>(rc, sh);
+    v[5] = tree_sum<NT>(weak, sh);
+    v[6] = tree_sum<NT>(bad, sh);
+    v[7] = tree_sum<NT>(unproj, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 8; ++k) partials[8 * blockIdx.x + k] = v[k];
+}
+__global__ void __launch_bounds__(NT) k_reduce_points_2(const double* __restrict__ partials, AsmScalars* out) {
+    __shared__ double sh[NT];
+    double a[8] = {0, 0, 0, 0, 1.0, 0, 0, 0};
+    for (int b = threadIdx.x; b < RED_BLOCKS; b += NT) {
+        const double* q = partials + 8 * b;
+        a[0] += q[0];
+        a[1] += q[1];
+        a[2] += q[2];
+        a[3] = fmax(a[3], q[3]);
+        a[4] = fmin(a[4], q[4]);
+        a[5] += q[5];
+        a[6] += q[6];
+        a[7] += q[7];
+    }
+    double v0 = tree_sum<NT>(a[0], sh), v1 = tree_sum<NT>(a[1], sh), v2 = tree_sum<NT>(a[2], sh);
+    double v3 = tree_max<NT>(a[3], sh), v4 = tree_min<NT>(a[4], sh), v5 = tree_sum<NT>(a[5], sh);
+    double v6 = tree_sum<NT>(a[6], sh), v7 = tree_sum<NT>(a[7], sh);
+    if (threadIdx.x == 0) {
+        out->cost = v0;
+        out->sq = v1;
+        out->pmr = v2;
+        out->sgin = v3;
+        out->min_rcond = v4;
+        out->weak = v5;
+        out->invalid = v6;
+        out->unprojectable = v7;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// k_block_gradient: one workgroup per block; chunks of the block's observation list are summed
+// per thread in list order and the chunk partials in chunk order.
+// ----------------------------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(NT) k_block_gradient(Topo tp, Lin<T> lin, PointElim<T> pe, double* grad,
+                                                       double* full_grad, double* diag) {
+    __shared__ double sh[3][NT];
+    const uint32_t A = blockIdx.x;
+    const int dA = (int)tp.blk_dim[A];
+    if (dA == 0) return;
+    const int rowbase = A < tp.nc ? 0 : 6;
+    const uint32_t lb = tp.blk_obs_off[A], le = tp.blk_obs_off[A + 1];
+    const uint32_t len = le - lb;
+    const int chunks = NT / dA;
+    const int t = threadIdx.x;
+    const int r = t % dA, ch = t / dA;
+    double full = 0.0, red = 0.0, dg = 0.0;
+    if (ch < chunks) {
+        uint32_t c0 = lb + (uint32_t)((uint64_t)len * ch / chunks);
+        uint32_t c1 = lb + (uint32_t)((uint64_t)len * (ch + 1) / chunks);
+        for (uint32_t k = c0; k < c1; ++k) {
+            uint32_t i = tp.blk_obs[k];
+            const T* rj = lin.rj + 8 * (size_t)i;
+            const T* jc = lin.jc + 18 * (size_t)i;
+            double j0 = (double)jc[rowbase + r], j1 = (double)jc[9 + rowbase + r];
+            full += j0 * (double)rj[0] + j1 * (double)rj[1];
+            dg += j0 * j0 + j1 * j1;
+            uint32_t p = tp.obs_pt[i];
+            if (!tp.pt_fixed[p]) {
+                const T* y = pe.y + 3 * (size_t)p;
+                double wy0 = (double)rj[2] * (double)y[0] + (double)rj[3] * (double)y[1] + (double)rj[4] * (double)y[2];
+                double wy1 = (double)rj[5] * (double)y[0] + (double)rj[6] * (double)y[1] + (double)rj[7] * (double)y[2];
+                red += j0 * wy0 + j1 * wy1;
+            }
+        }
+    }
+    sh[0][t] = full;
+    sh[1][t] = red;
+    sh[2][t] = dg;
+    __syncthreads();
+    if (t < dA) {
+        double a = 0, bsum = 0, c = 0;
+        for (int q = 0; q < chunks; ++q) {
+            a += sh[0][q * dA + t];
+            bsum += sh[1][q * dA + t];
+            c += sh[2][q * dA + t];
+        }
+        uint32_t off = tp.blk_off[A] + t;
+        full_grad[off] = a;
+        grad[off] = a - bsum;
+        diag[off] = c;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// k_pairs: one workgroup per lower block pair (A >= B).  Output (r, c) of the dA x dB block is
+// owned by threads t = chunk * nout + (r * dB + c); the pair's term list, pre-sorted by
+// (point, i, j), is split into contiguous chunks summed in order, then the chunk partials are
+// summed in chunk order.
+// ----------------------------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(NT) k_pairs(Topo tp, Lin<T> lin, PointElim<T> pe, const uint32_t* __restrict__ pair_a,
+                                              const uint32_t* __restrict__ pair_b, const uint32_t* __restrict__ term_off,
+                                              const uint32_t* __restrict__ term_i, const uint32_t* __restrict__ term_j,
+                                              const uint64_t* __restrict__ value_off, double* u_vals, double* s_vals) {
+    __shared__ double shu[NT];
+    __shared__ double shs[NT];
+    const uint32_t k = blockIdx.x;
+    const uint32_t A = pair_a[k], B = pair_b[k];
+    const int dA = (int)tp.blk_dim[A], dB = (int)tp.blk_dim[B];
+    const int nout = dA * dB;
+    const int ra = A < tp.nc ? 0 : 6, rb = B < tp.nc ? 0 : 6;
+    const int chunks = NT / nout;
+    const int t = threadIdx.x;
+    const int o = t % nout, ch = t / nout;
+    const int r = o / dB, c = o % dB;
+    double accu = 0.0, accs = 0.0;
+    const uint32_t tb = term_off[k], te = term_off[k + 1];
+    const uint32_t len = te - tb;
+    if (ch < chunks) {
+        uint32_t c0 = tb + (uint32_t)((uint64_t)len * ch / chunks);
+        uint32_t c1 = tb + (uint32_t)((uint64_t)len * (ch + 1) / chunks);
+        for (uint32_t q = c0; q < c1; ++q) {
+            uint32_t i = term_i[q], jf = term_j[q];
+            uint32_t j = jf & ((1u << 30) - 1);
+            if (jf & (1u << 30)) {
+                const T* jc = lin.jc + 18 * (size_t)i;
+                accu += (double)jc[ra + r] * (double)jc[rb + c] + (double)jc[9 + ra + r] * (double)jc[9 + rb + c];
+            }
+            if (jf & (1u << 31)) {
+                uint32_t p = tp.obs_pt[i];
+                const T* vd = pe.vdamp + 6 * (size_t)p;
+                const T* Ri = pe.rrows + 27 * (size_t)i + 3 * (ra + r);
+                const T* Rj = pe.rrows + 27 * (size_t)j + 3 * (rb + c);
+                T w[3];
+                T rjv[3] = {Rj[0], Rj[1], Rj[2]};
+                sym3_mul(vd, rjv, w);
+                accs += (double)Ri[0] * (double)w[0] + (double)Ri[1] * (double)w[1] + (double)Ri[2] * (double)w[2];
+            }
+        }
+    }
+    shu[t] = accu;
+    shs[t] = accs;
+    __syncthreads();
+    if (t < nout) {
+        double su = 0.0, ss = 0.0;
+        for (int q = 0; q < chunks; ++q) {
+            su += shu[q * nout + t];
+            ss += shs[q * nout + t];
+        }
+        uint64_t base = value_off[k];
+        u_vals[base + t] = su;
+        s_vals[base + t] = ss;
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_scatter_dense(Topo tp, uint32_t npairs, const uint32_t* __restrict__ pair_a,
+                                                      const uint32_t* __restrict__ pair_b,
+                                                      const uint64_t* __restrict__ value_off,
+                                                      const double* __restrict__ u_vals,
+                                                      const double* __restrict__ s_vals, double* S, uint32_t ld) {
+    const uint32_t k = blockIdx.x;
+    const uint32_t A = pair_a[k], B = pair_b[k];
+    const int dA = (int)tp.blk_dim[A], dB = (int)tp.blk_dim[B];
+    const int t = threadIdx.x;
+    if (t >= dA * dB) return;
+    const int r = t / dB, c = t % dB;
+    const uint64_t at = value_off[k] + t;
+    const double v = u_vals[at] - s_vals[at];
+    const size_t row = tp.blk_off[A] + r, col = tp.blk_off[B] + c;
+    S[row * ld + col] = v;
+    S[col * ld + row] = v;
+}
+
+// ----------------------------------------------------------------------------------------------
+// k_finalize (single workgroup): damping-diagonal clamp per block (schur/assembly.rs:729-766),
+// lambda * diag on the reduced diagonal, scaled gradient norm (:1098-1112), finiteness of g.
+// ----------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_finalize(Topo tp, double lambda, double* diag, const double* full_grad,
+                                                   const double* grad, double* S, uint32_t ld,
+                                                   const int32_t* diag_pair, const uint64_t* value_off,
+                                                   double* u_vals, AsmScalars* sc) {
+    __shared__ double sh[1024];
+    const int t = threadIdx.x;
+    // min/max of the unclamped diagonal
+    double mn = INFINITY, mx = 0.0;
+    for (uint32_t i = t; i < tp.D; i += 1024) {
+        mn = fmin(mn, diag[i]);
+        mx = fmax(mx, diag[i]);
+    }
+    mn = tree_min<1024>(mn, sh);
+    mx = tree_max<1024>(mx, sh);
+    __syncthreads();
+    double clamped = 0.0;
+    for (uint32_t A = t; A < tp.nblocks; A += 1024) {
+        const uint32_t n = tp.blk_dim[A], off = tp.blk_off[A];
+        if (n == 0) continue;
+        double scale = 0.0;
+        for (uint32_t k = 0; k < n; ++k) scale = fmax(scale, diag[off + k]);
+        double floor = scale > 0.0 ? 64.0 * DBL_EPSILON * scale : 1.0;
+        for (uint32_t k = 0; k < n; ++k)
+            if (diag[off + k] < floor) {
+                diag[off + k] = floor;
+                clamped += 1.0;
+            }
+        if (S) {
+            for (uint32_t k = 0; k < n; ++k) S[(size_t)(off + k) * ld + off + k] += lambda * diag[off + k];
+        } else if (diag_pair) {
+            int32_t pr = diag_pair[A];
+            if (pr >= 0)
+                for (uint32_t k = 0; k < n; ++k) u_vals[value_off[pr] + k * n + k] += lambda * diag[off + k];
+        }
+    }
+    clamped = tree_sum<1024>(clamped, sh);
+    double sg = 0.0, nonfinite = 0.0;
+    for (uint32_t i = t; i < tp.D; i += 1024) {
+        double cv = diag[i], gv = full_grad[i], v;
+        if (isfinite(cv) && cv > 0.0)
+            v = fabs(gv) / sqrt(cv);
+        else
+            v = gv == 0.0 ? 0.0 : INFINITY;
+        sg = fmax(sg, v);
+        if (!isfinite(grad[i])) nonfinite += 1.0;
+    }
+    sg = tree_max<1024>(sg, sh);
+    nonfinite = tree_sum<1024>(nonfinite, sh);
+    if (t == 0) {
+        sc->sgin_prepared = sg;
+        sc->min_diag = tp.D == 0 ? 0.0 : mn;
+        sc->max_diag = mx;
+        sc->clamped = clamped;
+        sc->nonfinite = nonfinite;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Retraction (kernel/multi_view/optimizer.rs:386-437, self_calibration/optimizer.rs:565-586)
+// ----------------------------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(NT) k_retract(Topo tp, const double* R, const double* t, const double* gp,
+                                                const double* step, const GaugeDev* gauge, double* Rc, double* tc,
+                                                double* gpc) {
+    uint32_t i = blockIdx.x * NT + threadIdx.x;
+    if (i < tp.nc) {
+        const uint32_t pd = tp.cam_dim[i], off = tp.cam_off[i];
+        if (pd == 0) {
+            for (int k = 0; k < 9; ++k) Rc[9 * i + k] = R[9 * i + k];
+            for (int k = 0; k < 3; ++k) tc[3 * i + k] = t[3 * i + k];
+        } else {
+            double up[9];
+            double w[3] = {step[off], step[off + 1], step[off + 2]};
+            exp_so3(w, up);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    double s = 0.0;
+                    for (int c = 0; c < 3; ++c) s += up[3 * a + c] * R[9 * i + 3 * c + b];
+                    Rc[9 * i + 3 * a + b] = s;
+                }
+            if ((int)i == gauge->camera) {
+                double dir[3];
+                retract_s2(gauge->direction, gauge->basis, step[off + 3], step[off + 4], dir);
+                for (int k = 0; k < 3; ++k) tc[3 * i + k] = gauge->baseline * dir[k];
+            } else {
+                for (int k = 0; k < 3; ++k) tc[3 * i + k] = t[3 * i + k] + gauge->scene_scale * step[off + 3 + k];
+            }
+        }
+    }
+    if (i < tp.ng) {
+        const uint32_t d = tp.grp_dim[i], off = tp.grp_off[i];
+        double delta[3] = {0.0, 0.0, 0.0};
+        for (uint32_t k = 0; k < d; ++k) delta[k] = step[off + k];
+        if (d == 0) {
+            for (int k = 0; k < 3; ++k) gpc[3 * i + k] = gp[3 * i + k];
+        } else {
+            gpc[3 * i] = gp[3 * i] * exp(delta[0]);
+            gpc[3 * i + 1] = gp[3 * i + 1] + delta[1];
+            gpc[3 * i + 2] = gp[3 * i + 2] + delta[2];
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// (a') k_candidate: one lane per point.  cscal per point: cost, step^2 (moved), damping norm
+// (moved), rejected flag.
+// ----------------------------------------------------------------------------------------------
+namespace {
+enum { ADMIT_PROJECTION = 0, ADMIT_COLLAPSE = 1, ADMIT_GEOMETRY = 2 };
+__device__ bool admitted_cost(const Topo& tp, uint32_t b, uint32_t e, const double* R, const double* t,
+                              const double* gpc, const double X[3], const double* uv, double scene,
+                              int admission, double* out) {
+    TrackGeometry<double> geo(scene);
+    double total = 0.0;
+    for (uint32_t o = b; o < e; ++o) {
+        const uint32_t c = tp.obs_cam[o], g = tp.cam_group[c];
+        const double* Rc = R + 9 * (size_t)c;
+        double rot[3];
+        rotate(Rc, X, rot);
+        double cp[3] = {rot[0] + t[3 * c], rot[1] + t[3 * c + 1], rot[2] + t[3 * c + 2]};
+        const double f = gpc[3 * g], k1 = gpc[3 * g + 1], k2 = gpc[3 * g + 2];
+        double turn = 0.0;
+        bool has_turn = first_turn(k1, k2, &turn);
+        double res[2];
+        if (!radial_residual<double>(f, k1, k2, turn, has_turn, cp, uv[2 * (size_t)o], uv[2 * (size_t)o + 1], res))
+            return false;
+        double w[3];
+        rotate_t(Rc, cp, w);
+        geo.push(w);
+        total += loss_cost<double>(tp.loss, tp.loss_scale, sqrt(res[0] * res[0] + res[1] * res[1]));
+    }
+    bool ok = admission == ADMIT_PROJECTION ? true : (admission == ADMIT_COLLAPSE ? !geo.collapsed() : !geo.degenerate());
+    if (ok) *out = total;
+    return ok;
+}
+}  // namespace
+
+template <class T>
+__global__ void __launch_bounds__(NT) k_candidate(Topo tp, const double* R, const double* t, const double* Rc,
+                                                  const double* tc, const double* gpc, const double* X,
+                                                  const double* uv, PointElim<T> pe, const double* step,
+                                                  const GaugeDev* gauge, double* Xc, double* cscal) {
+    uint32_t p = blockIdx.x * NT + threadIdx.x;
+    if (p >= tp.np) return;
+    const uint32_t b = tp.pt_off[p], e = tp.pt_off[p + 1];
+    const double scene = gauge->scene_scale;
+    double prev[3] = {X[3 * (size_t)p], X[3 * (size_t)p + 1], X[3 * (size_t)p + 2]};
+    double* cs = cscal + 4 * (size_t)p;
+    if (tp.pt_fixed[p]) {
+        for (int k = 0; k < 3; ++k) Xc[3 * (size_t)p + k] = prev[k];
+        double c;
+        bool ok = admitted_cost(tp, b, e, Rc, tc, gpc, prev, uv, 0.0, ADMIT_PROJECTION, &c);
+        cs[0] = ok ? c : 0.0;
+        cs[1] = 0.0;
+        cs[2] = 0.0;
+        cs[3] = ok ? 0.0 : 1.0;
+        return;
+    }
+    // cached back-substitution (candidate.rs:358-383): dp = -(V^-1 g + sum R dc)
+    double s3[3] = {(double)pe.y[3 * (size_t)p], (double)pe.y[3 * (size_t)p + 1], (double)pe.y[3 * (size_t)p + 2]};
+    for (uint32_t o = b; o < e; ++o) {
+        const uint32_t c = tp.obs_cam[o];
+        const T* rr = pe.rrows + 27 * (size_t)o;
+        for (int k = 0; k < 9; ++k) {
+            int q = row_param(tp, c, k);
+            if (q < 0) continue;
+            double v = step[q];
+            s3[0] += (double)rr[3 * k] * v;
+            s3[1] += (double)rr[3 * k + 1] * v;
+            s3[2] += (double)rr[3 * k + 2] * v;
+        }
+    }
+    double dp[3] = {-s3[0], -s3[1], -s3[2]};
+    const T* dd = pe.ddiag + 3 * (size_t)p;
+    double dn = (double)dd[0] * dp[0] * dp[0] + (double)dd[1] * dp[1] * dp[1] + (double)dd[2] * dp[2] * dp[2];
+    double stepped[3] = {prev[0] + scene * dp[0], prev[1] + scene * dp[1], prev[2] + scene * dp[2]};
+    // candidate_rule (point_geometry.rs:153-168)
+    double c;
+    if (admitted_cost(tp, b, e, Rc, tc, gpc, stepped, uv, scene, ADMIT_GEOMETRY, &c)) {
+        for (int k = 0; k < 3; ++k) Xc[3 * (size_t)p + k] = stepped[k];
+        cs[0] = c;
+        cs[1] = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+        cs[2] = dn;
+        cs[3] = 0.0;
+        return;
+    }
+    bool held = admitted_cost(tp, b, e, Rc, tc, gpc, prev, uv, scene, ADMIT_COLLAPSE, &c);
+    if (!held) {
+        double dummy;
+        bool cur_ok = admitted_cost(tp, b, e, R, t, gpc, prev, uv, scene, ADMIT_COLLAPSE, &dummy);
+        held = !cur_ok && admitted_cost(tp, b, e, Rc, tc, gpc, prev, uv, scene, ADMIT_PROJECTION, &c);
+    }
+    for (int k = 0; k < 3; ++k) Xc[3 * (size_t)p + k] = prev[k];
+    cs[0] = held ? c : 0.0;
+    cs[1] = 0.0;
+    cs[2] = 0.0;
+    cs[3] = held ? 0.0 : 1.0;
+}
+
+__global__ void __launch_bounds__(NT) k_reduce_cand_1(const double* __restrict__ cs, uint32_t np, double* partials) {
+    __shared__ double sh[NT];
+    uint32_t b0 = (uint32_t)((uint64_t)np * blockIdx.x / RED_BLOCKS);
+    uint32_t b1 = (uint32_t)((uint64_t)np * (blockIdx.x + 1) / RED_BLOCKS);
+    double a[4] = {0, 0, 0, 0};
+    for (uint32_t p = b0 + threadIdx.x; p < b1; p += NT)
+        for (int k = 0; k < 4; ++k) a[k] += cs[4 * (size_t)p + k];
+    for (int k = 0; k < 4; ++k) {
+        double v = tree_sum<NT>(a[k], sh);
+        if (threadIdx.x == 0) partials[4 * blockIdx.x + k] = v;
+    }
+}
+__global__ void __launch_bounds__(NT) k_reduce_cand_2(const double* __restrict__ partials, CandScalars* out) {
+    __shared__ double sh[NT];
+    double a[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < RED_BLOCKS; b += NT)
+        for (int k = 0; k < 4; ++k) a[k] += partials[4 * b + k];
+    double v[4];
+    for (int k = 0; k < 4; ++k) v[k] = tree_sum<NT>(a[k], sh);
+    if (threadIdx.x == 0) {
+        out->cost = v[0];
+        out->step2 = v[1];
+        out->damping_norm = v[2];
+        out->rejected = v[3];
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_step_dots(uint32_t D, const double* g, const double* diag, const double* x,
+                                                    const double* product, double* dots) {
+    __shared__ double sh[1024];
+    double a = 0, b = 0, c = 0, d = 0;
+    for (uint32_t i = threadIdx.x; i < D; i += 1024) {
+        a += g[i] * x[i];
+        b += diag[i] * x[i] * x[i];
+        c += x[i] * x[i];
+        if (product) d += x[i] * product[i];
+    }
+    a = tree_sum<1024>(a, sh);
+    b = tree_sum<1024>(b, sh);
+    c = tree_sum<1024>(c, sh);
+    d = tree_sum<1024>(d, sh);
+    if (threadIdx.x == 0) {
+        dots[0] = a;
+        dots[1] = b;
+        dots[2] = c;
+        dots[3] = d;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// (c) PCG.  Preconditioner blocks: Jacobi-scaled Cholesky of U_AA(damped) - Schur_AA
+// (retained/block.rs:171-250); stored per block as 45 packed lower values + 9 scales.
+// ----------------------------------------------------------------------------------------------
+constexpr int FACT = 54;
+__device__ __forceinline__ int pidx(int r, int c) { return r >= c ? r * (r + 1) / 2 + c : c * (c + 1) / 2 + r; }
+
+__global__ void __launch_bounds__(NT) k_precond_factor(Topo tp, const int32_t* diag_pair, const uint64_t* value_off,
+                                                       const double* u_vals, const double* s_vals, double* factors,
+                                                       int* fail) {
+    uint32_t A = blockIdx.x * NT + threadIdx.x;
+    if (A >= tp.nblocks) return;
+    const int n = (int)tp.blk_dim[A];
+    double* F = factors + FACT * (size_t)A;
+    if (n == 0) return;
+    int32_t pr = diag_pair[A];
+    if (pr < 0) {
+        *fail = 1;
+        return;
+    }
+    const double* U = u_vals + value_off[pr];
+    const double* Sv = s_vals + value_off[pr];
+    double m[45];
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c <= r; ++c) m[pidx(r, c)] = U[r * n + c] - Sv[r * n + c];
+    double ds = 0.0;
+    for (int i = 0; i < n; ++i) ds = fmax(ds, m[pidx(i, i)]);
+    if (!(isfinite(ds) && ds > 0.0)) {
+        *fail = 1;
+        return;
+    }
+    double floor = 64.0 * DBL_EPSILON * ds;
+    double* L = F;
+    double* sc = F + 45;
+    for (int i = 0; i < 45; ++i) L[i] = 0.0;
+    for (int i = 0; i < n; ++i) sc[i] = sqrt(fmax(m[pidx(i, i)], floor));
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c <= r; ++c) {
+            double v = m[pidx(r, c)] / (sc[r] * sc[c]);
+            for (int k = 0; k < c; ++k) v -= L[pidx(r, k)] * L[pidx(c, k)];
+            if (r == c) {
+                if (!(isfinite(v) && v > 64.0 * DBL_EPSILON)) {
+                    *fail = 1;
+                    return;
+                }
+                L[pidx(r, c)] = sqrt(v);
+            } else {
+                L[pidx(r, c)] = v / L[pidx(c, c)];
+            }
+        }
+}
+
+namespace {
+__device__ void precond_apply_block(const double* F, int n, const double* right, double* sol) {
+    const double* L = F;
+    const double* sc = F + 45;
+    for (int r = 0; r < n; ++r) {
+        double v = right[r] / sc[r];
+        for (int c = 0; c < r; ++c) v -= L[pidx(r, c)] * sol[c];
+        sol[r] = v / L[pidx(r, r)];
+    }
+    for (int r = n - 1; r >= 0; --r) {
+        double v = sol[r];
+        for (int c = r + 1; c < n; ++c) v -= L[pidx(c, r)] * sol[c];
+        sol[r] = v / L[pidx(r, r)];
+    }
+    for (int i = 0; i < n; ++i) sol[i] /= sc[i];
+}
+enum { PCG_NORMAL = 0, PCG_CONFIRM = 1, PCG_FINAL = 2 };
+
+// z = M^-1 r over all blocks, then rho = r.z (single workgroup)
+__device__ double apply_precond_dot(const Topo& tp, const double* factors, const double* r, double* z, double* sh) {
+    double acc = 0.0;
+    for (uint32_t A = threadIdx.x; A < tp.nblocks; A += 1024) {
+        const int n = (int)tp.blk_dim[A];
+        if (n == 0) continue;
+        const uint32_t off = tp.blk_off[A];
+        double sol[9];
+        precond_apply_block(factors + FACT * (size_t)A, n, r + off, sol);
+        for (int k = 0; k < n; ++k) z[off + k] = sol[k];
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tp.D; i += 1024) acc += r[i] * z[i];
+    return tree_sum<1024>(acc, sh);
+}
+__device__ double dot_all(uint32_t D, const double* a, const double* b, double* sh) {
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < D; i += 1024) acc += a[i] * b[i];
+    return tree_sum<1024>(acc, sh);
+}
+}  // namespace
+
+__global__ void __launch_bounds__(1024) k_pcg_init(Topo tp, const double* grad, const double* factors, double* x,
+                                                   double* r, double* z, double* d, double* q, PcgState* st,
+                                                   double tol, int max_iter) {
+    __shared__ double sh[1024];
+    const uint32_t D = tp.D;
+    for (uint32_t i = threadIdx.x; i < D; i += 1024) {
+        x[i] = 0.0;
+        q[i] = 0.0;
+        r[i] = -grad[i];
+    }
+    __syncthreads();
+    double n0 = sqrt(dot_all(D, r, r, sh));
+    PcgState s;
+    s.tol = tol;
+    s.max_iter = max_iter;
+    s.iter = 0;
+    s.n0 = n0;
+    s.rel = NAN;
+    s.rho = 0.0;
+    s.done = 0;
+    s.accepted = 0;
+    s.mode = PCG_NORMAL;
+    s.pad = 0;
+    if (!isfinite(n0)) {
+        s.done = 1;
+    } else if (n0 == 0.0) {
+        s.done = 1;
+        s.accepted = 1;
+        s.rel = 0.0;
+    } else {
+        double rho = apply_precond_dot(tp, factors, r, z, sh);
+        for (uint32_t i = threadIdx.x; i < D; i += 1024) d[i] = z[i];
+        if (!(isfinite(rho) && rho > 0.0)) {
+            s.done = 1;
+        } else {
+            s.rho = rho;
+            s.mode = max_iter == 0 ? PCG_FINAL : PCG_NORMAL;
+        }
+    }
+    if (threadIdx.x == 0) *st = s;
+}
+
+// Matrix-free S v: pass 1 per point (w_p = V^ sum_o R_o v), pass 2 per block.
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_pass1(Topo tp, PointElim<T> pe, const double* x, const double* d, double* w,
+                                                  const PcgState* st) {
+    if (st->done) return;
+    const double* v = st->mode == PCG_NORMAL ? d : x;
+    uint32_t p = blockIdx.x * NT + threadIdx.x;
+    if (p >= tp.np) return;
+    double s3[3] = {0.0, 0.0, 0.0};
+    if (!tp.pt_fixed[p]) {
+        for (uint32_t o = tp.pt_off[p]; o < tp.pt_off[p + 1]; ++o) {
+            const uint32_t c = tp.obs_cam[o];
+            const T* rr = pe.rrows + 27 * (size_t)o;
+            for (int k = 0; k < 9; ++k) {
+                int q = row_param(tp, c, k);
+                if (q < 0) continue;
+                double val = v[q];
+                s3[0] += (double)rr[3 * k] * val;
+                s3[1] += (double)rr[3 * k + 1] * val;
+                s3[2] += (double)rr[3 * k + 2] * val;
+            }
+        }
+    }
+    const T* vd = pe.vdamp + 6 * (size_t)p;
+    double m[6] = {(double)vd[0], (double)vd[1], (double)vd[2], (double)vd[3], (double)vd[4], (double)vd[5]};
+    double out[3];
+    sym3_mul(m, s3, out);
+    w[3 * (size_t)p] = out[0];
+    w[3 * (size_t)p + 1] = out[1];
+    w[3 * (size_t)p + 2] = out[2];
+}
+
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_pass2(Topo tp, PointElim<T> pe, const int32_t* diag_pair,
+                                                  const int32_t* edge_pair, const uint64_t* value_off,
+                                                  const double* u_vals, const double* x, const double* d,
+                                                  const double* w, double* q, const PcgState* st) {
+    __shared__ double sh[NT];
+    if (st->done) return;
+    const double* v = st->mode == PCG_NORMAL ? d : x;
+    const uint32_t A = blockIdx.x;
+    const int dA = (int)tp.blk_dim[A];
+    if (dA == 0) return;
+    const int rowbase = A < tp.nc ? 0 : 6;
+    const uint32_t lb = tp.blk_obs_off[A], le = tp.blk_obs_off[A + 1];
+    const uint32_t len = le - lb;
+    const int chunks = NT / dA;
+    const int t = threadIdx.x;
+    const int r = t % dA, ch = t / dA;
+    double acc = 0.0;
+    if (ch < chunks) {
+        uint32_t c0 = lb + (uint32_t)((uint64_t)len * ch / chunks);
+        uint32_t c1 = lb + (uint32_t)((uint64_t)len * (ch + 1) / chunks);
+        for (uint32_t k = c0; k < c1; ++k) {
+            uint32_t i = tp.blk_obs[k];
+            uint32_t p = tp.obs_pt[i];
+            if (tp.pt_fixed[p]) continue;
+            const T* rr = pe.rrows + 27 * (size_t)i + 3 * (rowbase + r);
+            const double* wp = w + 3 * (size_t)p;
+            acc += (double)rr[0] * wp[0] + (double)rr[1] * wp[1] + (double)rr[2] * wp[2];
+        }
+    }
+    sh[t] = acc;
+    __syncthreads();
+    if (t < dA) {
+        double coupling = 0.0;
+        for (int c = 0; c < chunks; ++c) coupling += sh[c * dA + t];
+        // base: U_AA v_A plus the pose-intrinsic edge terms (operator.rs:563-594)
+        const uint32_t off = tp.blk_off[A];
+        double base = 0.0;
+        int32_t pr = diag_pair[A];
+        if (pr >= 0) {
+            const double* U = u_vals + value_off[pr];
+            for (int c = 0; c < dA; ++c) base += U[t * dA + c] * v[off + c];
+        }
+        if (A < tp.nc) {
+            int32_t ep = edge_pair[A];
+            if (ep >= 0) {  // pair (group block, pose block): values [intr row][pose col]
+                uint32_t g = tp.cam_group[A];
+                const int di = (int)tp.grp_dim[g];
+                const double* E = u_vals + value_off[ep];
+                for (int l = 0; l < di; ++l) base += E[l * dA + t] * v[tp.grp_off[g] + l];
+            }
+        } else {
+            uint32_t g = A - tp.nc;
+            for (uint32_t k = tp.grp_cam_off[g]; k < tp.grp_cam_off[g + 1]; ++k) {
+                uint32_t cam = tp.grp_cams[k];
+                int32_t ep = edge_pair[cam];
+                if (ep < 0) continue;
+                const int dp = (int)tp.cam_dim[cam];
+                const double* E = u_vals + value_off[ep];
+                for (int l = 0; l < dp; ++l) base += E[t * dp + l] * v[tp.cam_off[cam] + l];
+            }
+        }
+        q[off + t] = base - coupling;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_pcg_update(Topo tp, const double* grad, const double* factors, double* x,
+                                                     double* r, double* z, double* d, const double* q, PcgState* st) {
+    __shared__ double sh[1024];
+    const uint32_t D = tp.D;
+    PcgState s = *st;
+    if (s.done) return;
+    if (s.mode == PCG_NORMAL) {
+        double curv = dot_all(D, d, q, sh);
+        double alpha = s.rho / curv;
+        if (!(isfinite(curv) && curv > 0.0) || !isfinite(alpha)) {
+            s.done = 1;
+        } else {
+            s.iter += 1;
+            for (uint32_t i = threadIdx.x; i < D; i += 1024) {
+                x[i] += alpha * d[i];
+                r[i] -= alpha * q[i];
+            }
+            __syncthreads();
+            double ratio = sqrt(dot_all(D, r, r, sh)) / s.n0;
+            if (!isfinite(ratio)) {
+                s.done = 1;
+            } else if (ratio <= s.tol) {
+                s.mode = PCG_CONFIRM;
+            } else {
+                double next = apply_precond_dot(tp, factors, r, z, sh);
+                double beta = next / s.rho;
+                if (!(isfinite(next) && next > 0.0) || !isfinite(beta)) {
+                    s.done = 1;
+                } else {
+                    for (uint32_t i = threadIdx.x; i < D; i += 1024) d[i] = z[i] + beta * d[i];
+                    s.rho = next;
+                    if (s.iter >= s.max_iter) s.mode = PCG_FINAL;
+                }
+            }
+        }
+    } else {
+        // q = S x: the true residual
+        for (uint32_t i = threadIdx.x; i < D; i += 1024) r[i] = -grad[i] - q[i];
+        __syncthreads();
+        double tr = sqrt(dot_all(D, r, r, sh)) / s.n0;
+        if (s.mode == PCG_FINAL) {
+            s.done = 1;
+            s.rel = isfinite(tr) ? tr : NAN;
+        } else if (isfinite(tr) && tr <= s.tol) {
+            s.done = 1;
+            s.accepted = 1;
+            s.rel = tr;
+        } else if (!isfinite(tr)) {
+            s.done = 1;
+        } else {
+            double rho = apply_precond_dot(tp, factors, r, z, sh);
+            for (uint32_t i = threadIdx.x; i < D; i += 1024) d[i] = z[i];
+            if (!(isfinite(rho) && rho > 0.0)) {
+                s.done = 1;
+                s.rel = tr;
+            } else {
+                s.rho = rho;
+                s.mode = s.iter >= s.max_iter ? PCG_FINAL : PCG_NORMAL;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *st = s;
+}
+
+// ----------------------------------------------------------------------------------------------
+// launchers
+// ----------------------------------------------------------------------------------------------
+template <class T>
+void launch_gauge(hipStream_t s, const T* cam_t, int camera, double scene_scale, GaugeDev* out) {
+    hipLaunchKernelGGL(k_gauge<T>, dim3(1), dim3(1), 0, s, cam_t, camera, scene_scale, out);
+}
+template <class T>
+void launch_linearize(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
+                      const double* X, const double* uv, const GaugeDev* gauge, Lin<T> out) {
+    if (tp.no == 0) return;
+    hipLaunchKernelGGL(k_linearize<T>, dim3(grid_for(tp.no, NT)), dim3(NT), 0, s, tp, R, t, gp, X, uv, gauge, out);
+}
+template <class T>
+void launch_point_elim(hipStream_t s, const Topo& tp, const Lin<T> lin, T lambda, bool want_rcond, PointElim<T> out) {
+    if (tp.np == 0) return;
+    hipLaunchKernelGGL(k_point_elim<T>, dim3(grid_for(tp.np, NT)), dim3(NT), 0, s, tp, lin, lambda, (int)want_rcond, out);
+}
+void launch_reduce_points(hipStream_t s, const double* pscal, uint32_t np, double* partials, AsmScalars* out) {
+    hipLaunchKernelGGL(k_reduce_points_1, dim3(RED_BLOCKS), dim3(NT), 0, s, pscal, np, partials);
+    hipLaunchKernelGGL(k_reduce_points_2, dim3(1), dim3(NT), 0, s, partials, out);
+}
+template <class T>
+void launch_block_gradient(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, double* grad,
+                           double* full_grad, double* diag) {
+    if (tp.nblocks == 0) return;
+    hipLaunchKernelGGL(k_block_gradient<T>, dim3(tp.nblocks), dim3(NT), 0, s, tp, lin, pe, grad, full_grad, diag);
+}
+template <class T>
+void launch_pairs(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, uint32_t npairs,
+                  const uint32_t* pair_a, const uint32_t* pair_b, const uint32_t* term_off, const uint32_t* term_i,
+                  const uint32_t* term_j, const uint64_t* value_off, double* u_vals, double* s_vals) {
+    if (npairs == 0) return;
+    hipLaunchKernelGGL(k_pairs<T>, dim3(npairs), dim3(NT), 0, s, tp, lin, pe, pair_a, pair_b, term_off, term_i, term_j,
+                       value_off, u_vals, s_vals);
+}
+void launch_scatter_dense(hipStream_t s, const Topo& tp, uint32_t npairs, const uint32_t* pair_a, const uint32_t* pair_b,
+                          const uint64_t* value_off, const double* u_vals, const double* s_vals, double* S, uint32_t ld) {
+    if (npairs == 0) return;
+    hipLaunchKernelGGL(k_scatter_dense, dim3(npairs), dim3(64), 0, s, tp, npairs, pair_a, pair_b, value_off, u_vals,
+                       s_vals, S, ld);
+}
+void launch_finalize(hipStream_t s, const Topo& tp, double lambda, double* diag, const double* full_grad,
+                     const double* grad, double* S, uint32_t ld, const int32_t* diag_pair, const uint64_t* value_off,
+                     double* u_vals, double* /*s_vals*/, AsmScalars* sc) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, tp, lambda, diag, full_grad, grad, S, ld, diag_pair,
+                       value_off, u_vals, sc);
+}
+void launch_retract(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
+                    const double* step, const GaugeDev* gauge, double* Rc, double* tc, double* gpc) {
+    uint32_t n = tp.nc > tp.ng ? tp.nc : tp.ng;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_retract<double>, dim3(grid_for(n, NT)), dim3(NT), 0, s, tp, R, t, gp, step, gauge, Rc, tc, gpc);
+}
+template <class T>
+void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* Rc,
+                      const double* tc, const double* gpc, const double* X, const double* uv, const PointElim<T> pe,
+                      const double* step, const GaugeDev* gauge, double* Xc, double* cscal) {
+    if (tp.np == 0) return;
+    hipLaunchKernelGGL(k_candidate<T>, dim3(grid_for(tp.np, NT)), dim3(NT), 0, s, tp, R, t, Rc, tc, gpc, X, uv, pe,
+                       step, gauge, Xc, cscal);
+}
+void launch_reduce_candidates(hipStream_t s, const double* cscal, uint32_t np, double* partials, CandScalars* out) {
+    hipLaunchKernelGGL(k_reduce_cand_1, dim3(RED_BLOCKS), dim3(NT), 0, s, cscal, np, partials);
+    hipLaunchKernelGGL(k_reduce_cand_2, dim3(1), dim3(NT), 0, s, partials, out);
+}
+void launch_step_dots(hipStream_t s, uint32_t D, const double* g, const double* diag, const double* x,
+                      const double* product, double* dots) {
+    hipLaunchKernelGGL(k_step_dots, dim3(1), dim3(1024), 0, s, D, g, diag, x, product, dots);
+}
+void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pair, const uint64_t* value_off,
+                           const double* u_vals, const double* s_vals, double* factors, int* fail) {
+    if (tp.nblocks == 0) return;
+    hipLaunchKernelGGL(k_precond_factor, dim3(grid_for(tp.nblocks, NT)), dim3(NT), 0, s, tp, diag_pair, value_off,
+                       u_vals, s_vals, factors, fail);
+}
+void launch_pcg_init(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x, double* r,
+                     double* z, double* d, double* q, PcgState* st, double tol, int max_iter) {
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(1024), 0, s, tp, grad, factors, x, r, z, d, q, st, tol, max_iter);
+}
+template <class T>
+void launch_pcg_matvec(hipStream_t s, const Topo& tp, const PointElim<T> pe, const int32_t* diag_pair,
+                       const int32_t* edge_pair, const uint64_t* value_off, const double* u_vals, const double* x,
+                       const double* d, double* w, double* q, const PcgState* st) {
+    if (tp.np) hipLaunchKernelGGL(k_pcg_pass1<T>, dim3(grid_for(tp.np, NT)), dim3(NT), 0, s, tp, pe, x, d, w, st);
+    if (tp.nblocks)
+        hipLaunchKernelGGL(k_pcg_pass2<T>, dim3(tp.nblocks), dim3(NT), 0, s, tp, pe, diag_pair, edge_pair, value_off,
+                           u_vals, x, d, w, q, st);
+}
+void launch_pcg_update(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x, double* r,
+                       double* z, double* d, const double* q, PcgState* st) {
+    hipLaunchKernelGGL(k_pcg_update, dim3(1), dim3(1024), 0, s, tp, grad, factors, x, r, z, d, q, st);
+}
+
+__global__ void k_negate(uint32_t n, const double* in, double* out) {
+    uint32_t i = blockIdx.x * NT + threadIdx.x;
+    if (i < n) out[i] = -in[i];
+}
+void launch_negate(hipStream_t s, uint32_t n, const double* in, double* out) {
+    if (n) hipLaunchKernelGGL(k_negate, dim3(grid_for(n, NT)), dim3(NT), 0, s, n, in, out);
+}
+
+template void launch_gauge<double>(hipStream_t, const double*, int, double, GaugeDev*);
+template void launch_point_elim<double>(hipStream_t, const Topo&, const Lin<double>, double, bool, PointElim<double>);
+template void launch_block_gradient<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>,
+                                            double*, double*, double*);
+template void launch_pairs<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>, uint32_t,
+                                   const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,
+                                   const uint64_t*, double*, double*);
+template void launch_linearize<double>(hipStream_t, const Topo&, const double*, const double*, const double*,
+                                       const double*, const double*, const GaugeDev*, Lin<double>);
+template void launch_candidate<double>(hipStream_t, const Topo&, const double*, const double*, const double*,
+                                       const double*, const double*, const double*, const double*,
+                                       const PointElim<double>, const double*, const GaugeDev*, double*, double*);
+template void launch_pcg_matvec<double>(hipStream_t, const Topo&, const PointElim<double>, const int32_t*,
+                                        const int32_t*, const uint64_t*, const double*, const double*, const double*,
+                                        double*, double*, const PcgState*);
+
+}  // namespace ba

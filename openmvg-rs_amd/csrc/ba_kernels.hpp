@@ -1,0 +1,153 @@
+// ba_kernels.hpp — launchers of the HIP kernels of the LM inner loop.
+//
+// Every launcher enqueues on `stream` and returns immediately.  Template parameter T is the
+// arithmetic of the per-observation and per-point work: double (plain form) or float (mixed
+// form).  Reductions run in a fixed order with no floating-point atomics, so the results do not
+// depend on scheduling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace ba {
+
+// Gauge state of one linearisation (schur/layout.rs:79-125), computed on the device.
+struct GaugeDev {
+    double baseline;
+    double scene_scale;
+    double direction[3];
+    double basis[6];
+    int camera;  // -1 when free
+    int _pad;
+};
+
+// Topology shared by all kernels (device pointers).
+struct Topo {
+    uint32_t nc, ng, np, no, D, nblocks;
+    const uint32_t* obs_cam;
+    const uint32_t* obs_pt;
+    const uint32_t* pt_off;
+    const uint8_t* pt_fixed;
+    const uint32_t* cam_group;
+    const uint32_t* cam_off;
+    const uint32_t* cam_dim;
+    const uint32_t* grp_off;
+    const uint32_t* grp_dim;
+    const uint32_t* blk_off;      // parameter offset per block
+    const uint32_t* blk_dim;
+    const uint32_t* blk_obs_off;  // observations per block
+    const uint32_t* blk_obs;
+    const uint32_t* grp_cam_off;  // cameras per group (for intrinsic-block edge terms)
+    const uint32_t* grp_cams;
+    int loss;
+    double loss_scale;
+};
+
+// Per-observation linearisation (kernel (a)) outputs; T-typed.
+template <class T>
+struct Lin {
+    T* rj;      // 8 per observation: r~ (2), J~_X (2x3 row-major)
+    T* jc;      // 18 per observation: J~_c (2x9), rows [pose 0..5 | intrinsic 6..8]
+    T* cost;    // robust cost per observation
+    T* sq;      // squared residual norm per observation
+    uint8_t* valid;
+};
+
+// Per-point elimination (kernel (b)) outputs.
+template <class T>
+struct PointElim {
+    T* vinv;     // 6 per point: packed inverse of the damped block V^
+    T* vdamp;    // 6 per point: packed damped block V^
+    T* y;        // 3 per point: V^-1 g_p
+    T* ddiag;    // 3 per point: damping diagonal max(V_ii, floor)
+    T* rrows;    // 27 per observation: R = V^-1 W rows (9 rows x 3)
+    double* pscal;  // 6 per point: cost, squared residual, point model reduction, scaled gradient
+                    // norm, rcond, flags (bit0 linearised, bit1 eliminated, bit2 weak)
+};
+
+// Scalars of one assembly (double) gathered by reduce_assembly.
+struct AsmScalars {
+    double cost, sq, pmr, sgin;
+    double min_rcond;
+    double weak;
+    double invalid;     // number of points that failed linearisation or elimination
+    double unprojectable;  // number of points with an observation outside the projection domain
+    double sgin_prepared, min_diag, max_diag, clamped, nonfinite;
+};
+
+struct CandScalars {
+    double cost, step2, damping_norm, rejected;
+};
+
+template <class T>
+void launch_gauge(hipStream_t s, const T* cam_t, int camera, double scene_scale, GaugeDev* out);
+template <class T>
+void launch_linearize(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
+                      const double* X, const double* uv, const GaugeDev* gauge, Lin<T> out);
+template <class T>
+void launch_point_elim(hipStream_t s, const Topo& tp, const Lin<T> lin, T lambda, bool want_rcond,
+                       PointElim<T> out);
+// Sum of per-point scalars in a fixed order -> AsmScalars fields cost..invalid.
+void launch_reduce_points(hipStream_t s, const double* pscal, uint32_t np, double* partials,
+                          AsmScalars* out);
+// Per block: full gradient J~^T r~, reduced gradient (full minus W V^-1 g), unclamped diagonal.
+template <class T>
+void launch_block_gradient(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe,
+                           double* grad, double* full_grad, double* diag);
+// Block-pair values: U (Gram) and Schur (W V^-1 W^T) parts, in pair value space.
+template <class T>
+void launch_pairs(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, uint32_t npairs,
+                  const uint32_t* pair_a, const uint32_t* pair_b, const uint32_t* term_off,
+                  const uint32_t* term_i, const uint32_t* term_j, const uint64_t* value_off, double* u_vals,
+                  double* s_vals);
+// Dense S (row-major, both triangles) = U - Schur from pair value space.
+void launch_scatter_dense(hipStream_t s, const Topo& tp, uint32_t npairs, const uint32_t* pair_a,
+                          const uint32_t* pair_b, const uint64_t* value_off, const double* u_vals,
+                          const double* s_vals, double* S, uint32_t ld);
+// Clamp the damping diagonal per block, add lambda * diag to the reduced diagonal (dense S when
+// S != null, else the diagonal pairs' U and Schur-block values), and finish the scaled gradient.
+void launch_finalize(hipStream_t s, const Topo& tp, double lambda, double* diag, const double* full_grad,
+                     const double* grad, double* S, uint32_t ld, const int32_t* diag_pair,
+                     const uint64_t* value_off, double* u_vals, double* s_vals, AsmScalars* sc);
+
+// Candidate pass (kernel (a')): back-substitution and candidate rule per point.
+void launch_retract(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
+                    const double* step, const GaugeDev* gauge, double* Rc, double* tc, double* gpc);
+template <class T>
+void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* Rc,
+                      const double* tc, const double* gpc, const double* X, const double* uv,
+                      const PointElim<T> pe, const double* step, const GaugeDev* gauge, double* Xc,
+                      double* cscal);
+void launch_reduce_candidates(hipStream_t s, const double* cscal, uint32_t np, double* partials,
+                              CandScalars* out);
+// dots[0] = g.x, dots[1] = diag.x^2, dots[2] = x.x, dots[3] = x.product
+void launch_step_dots(hipStream_t s, uint32_t D, const double* g, const double* diag, const double* x,
+                      const double* product, double* dots);
+
+// ---- implicit-Schur PCG (kernel (c)) ----
+struct PcgState {
+    double rho, n0, rel, tol;
+    int mode, iter, max_iter, done, accepted, pad;
+};
+// Factor the block-Jacobi preconditioner: U_AA(damped) - Schur_AA per block.
+void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pair, const uint64_t* value_off,
+                           const double* u_vals, const double* s_vals, double* factors, int* fail);
+void launch_pcg_init(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x,
+                     double* r, double* z, double* d, double* q, PcgState* st, double tol, int max_iter);
+template <class T>
+void launch_pcg_matvec(hipStream_t s, const Topo& tp, const PointElim<T> pe, const int32_t* diag_pair,
+                       const int32_t* edge_pair, const uint64_t* value_off, const double* u_vals,
+                       const double* x, const double* d, double* w, double* q, const PcgState* st);
+void launch_pcg_update(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x,
+                       double* r, double* z, double* d, const double* q, PcgState* st);
+
+// ---- dense Cholesky (kernel (d)) ----
+// Factor the lower triangle of the n x n row-major matrix A in place (MFMA FP64 trailing
+// updates) and solve A x = b.  *fail != 0 after a pivot at or below 64 eps max|diag(A)|.
+void launch_cholesky_solve(hipStream_t s, double* A, uint32_t n, uint32_t ld, const double* b, double* x,
+                           double* work, int* fail);
+size_t cholesky_work_doubles(uint32_t n);
+void launch_negate(hipStream_t s, uint32_t n, const double* in, double* out);
+}  // namespace ba

@@ -7,6 +7,7 @@ host-side mirror of the reference's problem/solver types and a ctypes binding.
 
 from . import _abi
 from .problem import (
+    HipBundleAdjuster,
     BundleProblem,
     Internal,
     Intrinsics,
@@ -22,7 +23,7 @@ from .problem import (
 )
 
 __all__ = [
-    "BundleProblem", "Intrinsics", "SolveConfig", "SolveReport", "Solution", "SolveError",
+    "HipBundleAdjuster", "BundleProblem", "Intrinsics", "SolveConfig", "SolveReport", "Solution", "SolveError",
     "InvalidConfig", "Unsupported", "ResourceExhausted", "Numerical", "Internal", "ProblemError",
     "_abi",
 ]

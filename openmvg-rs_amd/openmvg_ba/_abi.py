@@ -121,6 +121,7 @@ class SolveReportC(C.Structure):
         ("maximum_pcg_relative_residual", C.c_double),
         ("normalization_scale", C.c_double),
         ("observation_count", C.c_uint64), ("point_count", C.c_uint64), ("pose_count", C.c_uint64),
+        ("seconds_setup", C.c_double), ("seconds_iterations", C.c_double), ("seconds_final", C.c_double),
     ]
 
 

@@ -400,3 +400,31 @@ def obs_point_index(point_offsets: np.ndarray) -> np.ndarray:
     """Owning point of each observation of a point-major CSR."""
     counts = np.diff(point_offsets.astype(np.int64))
     return np.repeat(np.arange(len(counts), dtype=np.uint32), counts)
+
+
+class HipBundleAdjuster:
+    """BundleAdjuster (solver.rs:1562-1573) backed by the MI355X library.
+
+    ``solve`` returns a Solution or raises the SolveError subclass the library reports;
+    Unsupported is the sanctioned decline a caller may route to another adjuster.
+    """
+
+    def __init__(self, device: int = 0):
+        from ._lib import Handle
+
+        self._handle = Handle(device)
+
+    def name(self) -> str:
+        return self._handle.lib.ba_hip_name().decode()
+
+    def solve(self, problem: BundleProblem, config: SolveConfig | None = None, trace_capacity=None) -> Solution:
+        config = config or SolveConfig()
+        return solve_with(self._handle.lib, "ba_hip_solve", problem, config, self._handle.ptr,
+                          trace_capacity=trace_capacity)
+
+    def prepare(self, problem: BundleProblem) -> NativeProblem:
+        return prepare_with(self._handle.lib, "ba_hip_prepare", problem)
+
+    @property
+    def handle(self):
+        return self._handle

@@ -2344,7 +2344,10 @@ int bao_linearize(const ba_device_problem* dp, double* residual, double* pj, dou
             for (int r = 0; r < 2; ++r) {
                 residual[2 * o + r] = ol.sw * ol.res[r];
                 for (int c = 0; c < 3; ++c) pj[6 * o + 3 * r + c] = ol.sw * ol.pj[r][c];
-                for (int c = 0; c < 9; ++c) cj[18 * o + 9 * r + c] = ol.sw * ol.cj[r][c];
+                // fixed slots: pose columns 0..5, intrinsic columns 6..8, zero beyond each run
+                for (int c = 0; c < 9; ++c) cj[18 * o + 9 * r + c] = 0.0;
+                for (int c = 0; c < ol.pose_dim; ++c) cj[18 * o + 9 * r + c] = ol.sw * ol.cj[r][c];
+                for (int c = 0; c < ol.intr_dim; ++c) cj[18 * o + 9 * r + 6 + c] = ol.sw * ol.cj[r][ol.pose_dim + c];
             }
             cost[o] = ol.cost;
         }
