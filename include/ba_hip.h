@@ -119,7 +119,20 @@ typedef struct ba_solve_config {
     int32_t precision;                    /* enum ba_precision */
     uint32_t pcg_max_iterations;          /* 0 = the native cap min(600, D) */
     double pcg_relative_tolerance;        /* 0 = the native forcing tolerance 1e-1 */
+    int32_t profile_kernel;               /* enum ba_profile_kernel: time this kernel's launches */
 } ba_solve_config;
+
+/* Kernels that ba_solve_config.profile_kernel can time with HIP events on the solve's stream. */
+enum ba_profile_kernel {
+    BA_PROFILE_NONE = 0,
+    BA_PROFILE_LINEARIZE = 1,      /* kernel (a) */
+    BA_PROFILE_POINT_ELIM = 2,     /* kernel (b), per-point elimination */
+    BA_PROFILE_PCG_POINTS = 3,     /* kernel (c), point pass of the implicit Schur product */
+    BA_PROFILE_PCG_BLOCKS = 4,     /* kernel (c), camera-block pass of the implicit Schur product */
+    BA_PROFILE_PAIRS = 5,          /* kernel (b), block-pair segmented reduction */
+    BA_PROFILE_CHOLESKY = 6,       /* kernel (d), factor and solve */
+    BA_PROFILE_CANDIDATE = 7       /* candidate pass */
+};
 
 /* SolveStatus (solver.rs:172-185) with ConvergenceReason folded in. */
 enum ba_solve_status {
@@ -159,6 +172,8 @@ typedef struct ba_solve_report {
     double seconds_setup;                 /* lowering, index maps, uploads */
     double seconds_iterations;            /* the LM loop */
     double seconds_final;                 /* final zero-damping assembly and download */
+    uint32_t profile_launches;            /* launches of the profiled kernel */
+    double profile_seconds;               /* their summed HIP-event durations */
 } ba_solve_report;
 
 /* Optional per-iteration record: the LM trajectory a parity test compares. */
@@ -172,6 +187,8 @@ typedef struct ba_iteration_trace {
     double* gain_ratio;                   /* NaN when none */
     uint8_t* accepted;
     uint32_t* pcg_iterations;
+    double* seconds;                      /* wall time from the start of the LM loop to the end
+                                             of the iteration */
 } ba_iteration_trace;
 
 /* Solution (solver.rs:803-876): caller-allocated buffers with the problem's sizes. */

@@ -1164,10 +1164,14 @@ void launch_pcg_init(hipStream_t s, const Topo& tp, const double* grad, const do
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(1024), 0, s, tp, grad, factors, x, r, z, d, q, st, tol, max_iter);
 }
 template <class T>
-void launch_pcg_matvec(hipStream_t s, const Topo& tp, const PointElim<T> pe, const int32_t* diag_pair,
-                       const int32_t* edge_pair, const uint64_t* value_off, const double* u_vals, const double* x,
-                       const double* d, double* w, double* q, const PcgState* st) {
+void launch_pcg_points(hipStream_t s, const Topo& tp, const PointElim<T> pe, const double* x, const double* d,
+                       double* w, const PcgState* st) {
     if (tp.np) hipLaunchKernelGGL(k_pcg_pass1<T>, dim3(grid_for(tp.np, NT)), dim3(NT), 0, s, tp, pe, x, d, w, st);
+}
+template <class T>
+void launch_pcg_blocks(hipStream_t s, const Topo& tp, const PointElim<T> pe, const int32_t* diag_pair,
+                       const int32_t* edge_pair, const uint64_t* value_off, const double* u_vals, const double* x,
+                       const double* d, const double* w, double* q, const PcgState* st) {
     if (tp.nblocks)
         hipLaunchKernelGGL(k_pcg_pass2<T>, dim3(tp.nblocks), dim3(NT), 0, s, tp, pe, diag_pair, edge_pair, value_off,
                            u_vals, x, d, w, q, st);
@@ -1197,8 +1201,10 @@ template void launch_linearize<double>(hipStream_t, const Topo&, const double*, 
 template void launch_candidate<double>(hipStream_t, const Topo&, const double*, const double*, const double*,
                                        const double*, const double*, const double*, const double*,
                                        const PointElim<double>, const double*, const GaugeDev*, double*, double*);
-template void launch_pcg_matvec<double>(hipStream_t, const Topo&, const PointElim<double>, const int32_t*,
-                                        const int32_t*, const uint64_t*, const double*, const double*, const double*,
-                                        double*, double*, const PcgState*);
+template void launch_pcg_points<double>(hipStream_t, const Topo&, const PointElim<double>, const double*,
+                                        const double*, double*, const PcgState*);
+template void launch_pcg_blocks<double>(hipStream_t, const Topo&, const PointElim<double>, const int32_t*,
+                                        const int32_t*, const uint64_t*, const double*, const double*,
+                                        const double*, const double*, double*, const PcgState*);
 
 }  // namespace ba

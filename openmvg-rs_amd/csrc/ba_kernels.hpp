@@ -136,10 +136,14 @@ void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pa
                            const double* u_vals, const double* s_vals, double* factors, int* fail);
 void launch_pcg_init(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x,
                      double* r, double* z, double* d, double* q, PcgState* st, double tol, int max_iter);
+// Implicit Schur product q = S v (v = d or x by the PCG mode): point pass then block pass.
 template <class T>
-void launch_pcg_matvec(hipStream_t s, const Topo& tp, const PointElim<T> pe, const int32_t* diag_pair,
+void launch_pcg_points(hipStream_t s, const Topo& tp, const PointElim<T> pe, const double* x, const double* d,
+                       double* w, const PcgState* st);
+template <class T>
+void launch_pcg_blocks(hipStream_t s, const Topo& tp, const PointElim<T> pe, const int32_t* diag_pair,
                        const int32_t* edge_pair, const uint64_t* value_off, const double* u_vals,
-                       const double* x, const double* d, double* w, double* q, const PcgState* st);
+                       const double* x, const double* d, const double* w, double* q, const PcgState* st);
 void launch_pcg_update(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x,
                        double* r, double* z, double* d, const double* q, PcgState* st);
 

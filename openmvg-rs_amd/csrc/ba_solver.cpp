@@ -106,7 +106,45 @@ struct Nielsen {  // numerics/refine/nonlinear/controller.rs:15-91
 // Device copy of a lowered problem and every buffer of the loop.
 class Engine {
    public:
-    Engine(hipStream_t s, const Lowered& L, bool iterative) : s_(s), L_(L), iterative_(iterative) {}
+    Engine(hipStream_t s, const Lowered& L, bool iterative, int profile = BA_PROFILE_NONE)
+        : s_(s), L_(L), iterative_(iterative), profile_(profile) {}
+    ~Engine() {
+        for (auto& e : events_) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    }
+
+    // Run `launch`, bracketed by HIP events on this stream when `which` is the profiled kernel.
+    template <class F>
+    void timed(int which, F&& launch) {
+        if (which != profile_) {
+            launch();
+            return;
+        }
+        if (used_ == events_.size()) {
+            hipEvent_t a, b;
+            HIP_OK(hipEventCreate(&a));
+            HIP_OK(hipEventCreate(&b));
+            events_.emplace_back(a, b);
+        }
+        auto& e = events_[used_++];
+        HIP_OK(hipEventRecord(e.first, s_));
+        launch();
+        HIP_OK(hipEventRecord(e.second, s_));
+    }
+    // Launch count and summed duration of the profiled kernel so far.
+    void profile_result(uint32_t* launches, double* seconds) {
+        HIP_OK(hipStreamSynchronize(s_));
+        double total = 0.0;
+        for (size_t k = 0; k < used_; ++k) {
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, events_[k].first, events_[k].second));
+            total += ms * 1e-3;
+        }
+        *launches = (uint32_t)used_;
+        *seconds = total;
+    }
 
     void setup() {
         B_ = build_block_lists(L_);
@@ -240,13 +278,18 @@ class Engine {
         tp.loss = loss;
         tp.loss_scale = loss_scale;
         if (!fixed_gauge_) launch_gauge<double>(s_, t_[state].p, L_.gauge_camera, L_.scene_scale, gauge_.p);
-        launch_linearize<double>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p, lin());
-        launch_point_elim<double>(s_, tp, lin(), lambda, lambda == 0.0, pe());
+        timed(BA_PROFILE_LINEARIZE, [&] {
+            launch_linearize<double>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
+                                     lin());
+        });
+        timed(BA_PROFILE_POINT_ELIM, [&] { launch_point_elim<double>(s_, tp, lin(), lambda, lambda == 0.0, pe()); });
         launch_reduce_points(s_, pscal_.p, tp.np, partials_.p, asm_.p);
         launch_block_gradient<double>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
         if (reduced_system) {
-            launch_pairs<double>(s_, tp, lin(), pe(), (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p, term_off_.p,
-                                 term_i_.p, term_j_.p, value_off_.p, u_vals_.p, s_vals_.p);
+            timed(BA_PROFILE_PAIRS, [&] {
+                launch_pairs<double>(s_, tp, lin(), pe(), (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p,
+                                     term_off_.p, term_i_.p, term_j_.p, value_off_.p, u_vals_.p, s_vals_.p);
+            });
             if (!iterative_) {
                 HIP_OK(hipMemsetAsync(S_.p, 0, S_.n * sizeof(double), s_));
                 launch_scatter_dense(s_, tp, (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p, value_off_.p,
@@ -270,7 +313,8 @@ class Engine {
     bool solve_direct() {
         if (tp_.D == 0) return true;
         launch_negate(s_, tp_.D, grad_.p, rhs_.p);
-        launch_cholesky_solve(s_, S_.p, tp_.D, tp_.D, rhs_.p, step_.p, work_.p, fail_.p);
+        timed(BA_PROFILE_CHOLESKY,
+              [&] { launch_cholesky_solve(s_, S_.p, tp_.D, tp_.D, rhs_.p, step_.p, work_.p, fail_.p); });
         HIP_OK(hipMemcpyAsync(h_int_.p, fail_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
         HIP_OK(hipStreamSynchronize(s_));
         return *h_int_.p == 0;
@@ -296,8 +340,12 @@ class Engine {
             HIP_OK(hipStreamSynchronize(s_));
             if (h_pcg_.p->done) break;
             for (int k = 0; k < batch; ++k) {
-                launch_pcg_matvec<double>(s_, tp_, pe(), diag_pair_.p, edge_pair_.p, value_off_.p, u_vals_.p, px_.p,
-                                          pd_.p, pw_.p, pq_.p, pcg_.p);
+                timed(BA_PROFILE_PCG_POINTS,
+                      [&] { launch_pcg_points<double>(s_, tp_, pe(), px_.p, pd_.p, pw_.p, pcg_.p); });
+                timed(BA_PROFILE_PCG_BLOCKS, [&] {
+                    launch_pcg_blocks<double>(s_, tp_, pe(), diag_pair_.p, edge_pair_.p, value_off_.p, u_vals_.p,
+                                              px_.p, pd_.p, pw_.p, pq_.p, pcg_.p);
+                });
                 launch_pcg_update(s_, tp_, grad_.p, factors_.p, px_.p, pr_.p, pz_.p, pd_.p, pq_.p, pcg_.p);
             }
         }
@@ -318,8 +366,10 @@ class Engine {
         tp.loss = loss;
         tp.loss_scale = loss_scale;
         launch_retract(s_, tp, R_[state].p, t_[state].p, gp_[state].p, step_.p, gauge_.p, R_[cs].p, t_[cs].p, gp_[cs].p);
-        launch_candidate<double>(s_, tp, R_[state].p, t_[state].p, R_[cs].p, t_[cs].p, gp_[cs].p, X_[state].p, uv_.p,
-                                 pe(), step_.p, gauge_.p, X_[cs].p, cscal_.p);
+        timed(BA_PROFILE_CANDIDATE, [&] {
+            launch_candidate<double>(s_, tp, R_[state].p, t_[state].p, R_[cs].p, t_[cs].p, gp_[cs].p, X_[state].p,
+                                     uv_.p, pe(), step_.p, gauge_.p, X_[cs].p, cscal_.p);
+        });
         launch_reduce_candidates(s_, cscal_.p, tp.np, partials_.p, cand_.p);
         launch_step_dots(s_, tp.D, grad_.p, diag_.p, step_.p, with_product ? pq_.p : nullptr, dots_.p);
         HIP_OK(hipMemcpyAsync(h_cand_.p, cand_.p, sizeof(CandScalars), hipMemcpyDeviceToHost, s_));
@@ -384,6 +434,9 @@ class Engine {
     Pinned<PcgState> h_pcg_;
     double h_dots_[4];
     bool fixed_gauge_ = false;
+    int profile_ = BA_PROFILE_NONE;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
+    size_t used_ = 0;
 };
 
 // A Lowered copy of a device-resident lowered problem (the per-kernel hooks).
@@ -452,10 +505,12 @@ GaugeDev gauge_from(const ba_device_problem* dp) {
     return g;
 }
 
+double g_loop_start = 0.0;
 void trace_push(ba_iteration_trace* tr, double cost, double damping, double cand, double pred, double gain, bool acc,
                 uint32_t pcg) {
     if (!tr || tr->count >= tr->capacity) return;
     uint32_t i = tr->count++;
+    if (tr->seconds) tr->seconds[i] = now() - g_loop_start;
     if (tr->cost) tr->cost[i] = cost;
     if (tr->damping) tr->damping[i] = damping;
     if (tr->candidate_cost) tr->candidate_cost[i] = cand;
@@ -501,6 +556,7 @@ Status refine(Engine& E, const Lowered& L, const ba_solve_config* cfg, ba_solve_
         AsmScalars a0 = E.assemble(state, 0.0, L.loss, L.loss_scale, false);
         if (a0.unprojectable > 0.0) return error(BA_ERR_NUMERICAL, "point not projectable at the initial state");
     }
+    g_loop_start = now();
     const double initial_damping = iterative ? 1e-1 : 1e-4;  // sparse/api.rs:300-305
     Nielsen damping{initial_damping};
     uint32_t iterations = 0, accepted = 0, target_accepted = 0;
@@ -760,7 +816,7 @@ int ba_hip_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* c
         if (!st.ok()) return finish(st, msg, len);
         const bool iterative = cfg->linear_solver == BA_SOLVER_ITERATIVE;
         if (trace) trace->count = 0;
-        Engine E(h->stream, L, iterative);
+        Engine E(h->stream, L, iterative, cfg->profile_kernel);
         E.setup();
         double t1 = now();
         ba_solve_report r;
@@ -775,6 +831,7 @@ int ba_hip_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* c
         r.seconds_setup = t1 - t0;
         r.seconds_iterations = t2 - t1;
         r.seconds_final = t3 - t2;
+        E.profile_result(&r.profile_launches, &r.profile_seconds);
         if (rep) *rep = r;
         return BA_OK;
     } catch (const HipError& e) {

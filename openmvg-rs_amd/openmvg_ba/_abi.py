@@ -40,6 +40,15 @@ SOLVER_ITERATIVE = 2
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 
+PROFILE_NONE = 0
+PROFILE_LINEARIZE = 1
+PROFILE_POINT_ELIM = 2
+PROFILE_PCG_POINTS = 3
+PROFILE_PCG_BLOCKS = 4
+PROFILE_PAIRS = 5
+PROFILE_CHOLESKY = 6
+PROFILE_CANDIDATE = 7
+
 STATUS_CONVERGED_GRADIENT = 0
 STATUS_CONVERGED_STEP = 1
 STATUS_CONVERGED_COST = 2
@@ -101,6 +110,7 @@ class SolveConfigC(C.Structure):
         ("precision", C.c_int32),
         ("pcg_max_iterations", C.c_uint32),
         ("pcg_relative_tolerance", C.c_double),
+        ("profile_kernel", C.c_int32),
     ]
 
 
@@ -122,6 +132,7 @@ class SolveReportC(C.Structure):
         ("normalization_scale", C.c_double),
         ("observation_count", C.c_uint64), ("point_count", C.c_uint64), ("pose_count", C.c_uint64),
         ("seconds_setup", C.c_double), ("seconds_iterations", C.c_double), ("seconds_final", C.c_double),
+        ("profile_launches", C.c_uint32), ("profile_seconds", C.c_double),
     ]
 
 
@@ -129,7 +140,7 @@ class IterationTrace(C.Structure):
     _fields_ = [
         ("capacity", C.c_uint32), ("count", C.c_uint32),
         ("cost", _d), ("damping", _d), ("candidate_cost", _d), ("predicted_reduction", _d),
-        ("gain_ratio", _d), ("accepted", _u8), ("pcg_iterations", _u32),
+        ("gain_ratio", _d), ("accepted", _u8), ("pcg_iterations", _u32), ("seconds", _d),
     ]
 
 

@@ -209,6 +209,7 @@ class SolveConfig:
     precision: int = A.PRECISION_FP64
     pcg_max_iterations: int = 0
     pcg_relative_tolerance: float = 0.0
+    profile_kernel: int = A.PROFILE_NONE
 
     def with_(self, **kw) -> "SolveConfig":
         return dataclasses.replace(self, **kw)
@@ -224,6 +225,7 @@ class SolveConfig:
         c.precision = self.precision
         c.pcg_max_iterations = self.pcg_max_iterations
         c.pcg_relative_tolerance = self.pcg_relative_tolerance
+        c.profile_kernel = self.profile_kernel
         return c
 
 
@@ -275,11 +277,11 @@ def solve_with(lib, function: str, problem: BundleProblem, config: SolveConfig, 
     tr_arrays = dict(cost=np.full(cap, np.nan), damping=np.full(cap, np.nan),
                      candidate_cost=np.full(cap, np.nan), predicted_reduction=np.full(cap, np.nan),
                      gain_ratio=np.full(cap, np.nan), accepted=np.zeros(cap, dtype=np.uint8),
-                     pcg_iterations=np.zeros(cap, dtype=np.uint32))
+                     pcg_iterations=np.zeros(cap, dtype=np.uint32), seconds=np.full(cap, np.nan))
     tr = A.IterationTrace()
     tr.capacity = cap
     tr.count = 0
-    for name in ("cost", "damping", "candidate_cost", "predicted_reduction", "gain_ratio"):
+    for name in ("cost", "damping", "candidate_cost", "predicted_reduction", "gain_ratio", "seconds"):
         setattr(tr, name, A.ptr(tr_arrays[name], C.c_double))
     tr.accepted = A.ptr(tr_arrays["accepted"], C.c_uint8)
     tr.pcg_iterations = A.ptr(tr_arrays["pcg_iterations"], C.c_uint32)

@@ -142,6 +142,57 @@ def _observe(R, C, f, k1, k2, X, rng, cap_of_point, noise, outlier_fraction=0.0,
     return obs_pose, obs_point, pix, kept
 
 
+def _observe_grid(R, C, f, k1, k2, X, rng, cap, noise, outlier_fraction, nx, ny, chunk=131072):
+    """Tracks for cameras on an nx-by-ny unit grid looking along +z: each point draws up to
+    cap[i] distinct cameras uniformly from the grid window that sees it (half-angles 0.46 and
+    0.34 of the depth, inside the 0.512/0.384 field of view), keeps the ones whose exact
+    projection falls in the image, and is dropped below 2 views."""
+    n_pts = len(X)
+    kmax = int(cap.max())
+    poses, points, pixels = [], [], []
+    kept = np.zeros(n_pts, dtype=bool)
+    for i0 in range(0, n_pts, chunk):
+        i1 = min(n_pts, i0 + chunk)
+        Xc = X[i0:i1]
+        gx = Xc[:, 0] + (nx - 1) / 2.0
+        gy = Xc[:, 1] + (ny - 1) / 2.0
+        ix0 = np.clip(np.ceil(gx - 0.46 * Xc[:, 2]), 0, nx - 1).astype(np.int64)
+        ix1 = np.clip(np.floor(gx + 0.46 * Xc[:, 2]), 0, nx - 1).astype(np.int64)
+        iy0 = np.clip(np.ceil(gy - 0.34 * Xc[:, 2]), 0, ny - 1).astype(np.int64)
+        iy1 = np.clip(np.floor(gy + 0.34 * Xc[:, 2]), 0, ny - 1).astype(np.int64)
+        wx = np.maximum(ix1 - ix0 + 1, 0)
+        wy = np.maximum(iy1 - iy0 + 1, 0)
+        count = wx * wy
+        wmax = int(count.max())
+        keys = rng.random((i1 - i0, wmax), dtype=np.float32)
+        keys[np.arange(wmax)[None, :] >= count[:, None]] = 2.0
+        kk = min(kmax, wmax)
+        part = np.argpartition(keys, kk - 1, axis=1)[:, :kk] if kk < wmax else np.broadcast_to(
+            np.arange(wmax), (i1 - i0, wmax)).copy()
+        order = np.argsort(np.take_along_axis(keys, part, axis=1), axis=1, kind="stable")
+        sel = np.take_along_axis(part, order, axis=1)
+        take = (np.arange(sel.shape[1])[None, :] < np.minimum(cap[i0:i1], count)[:, None])
+        cam = (iy0[:, None] + sel // np.maximum(wx, 1)[:, None]) * nx + ix0[:, None] + sel % np.maximum(wx, 1)[:, None]
+        cam = np.where(take, cam, 0)
+        uv, z = radial_project(R[cam], C[cam], f[cam], k1[cam], k2[cam], Xc[:, None, :])
+        vis = take & (z > 0.1) & (uv[..., 0] >= 0) & (uv[..., 0] < WIDTH) & (uv[..., 1] >= 0) & (uv[..., 1] < HEIGHT)
+        ok = vis.sum(axis=1) >= 2
+        kept[i0:i1] = ok
+        vis &= ok[:, None]
+        pi, ci = np.nonzero(vis)
+        poses.append(cam[pi, ci])
+        points.append(pi + i0)
+        pixels.append(uv[pi, ci])
+    obs_pose = np.concatenate(poses).astype(np.uint32)
+    obs_point = np.concatenate(points).astype(np.int64)
+    pix = np.concatenate(pixels)
+    pix = pix + rng.normal(0.0, noise, pix.shape)
+    if outlier_fraction > 0:
+        out = rng.random(len(pix)) < outlier_fraction
+        pix[out] = np.stack([rng.uniform(0, WIDTH, out.sum()), rng.uniform(0, HEIGHT, out.sum())], -1)
+    return obs_pose, obs_point, pix, kept
+
+
 def _finish(name, R, C, f, k1, k2, X, obs_pose, obs_point, pix, kept, init, intr_model, var_radial,
             loss=A.LOSS_SQUARED, loss_scale=0.0, waived=True):
     """Renumber kept points, perturb the state and build the BundleProblem."""
@@ -202,27 +253,7 @@ def grid_scene(seed: int = 0xBA0002, nx: int = 20, ny: int = 10, n_points: int =
         cap = np.clip(rng.geometric(1.0 / 6.0, n_points), 2, 50)
     else:
         cap = 2 + rng.binomial(10, 0.4, n_points)
-    # candidate cameras: the grid neighbourhood within the widest field of view
-    reach_x = int(np.ceil(15.0 * 0.6)) + 1
-    reach_y = int(np.ceil(15.0 * 0.45)) + 1
-    ox, oy = np.meshgrid(np.arange(-reach_x, reach_x + 1), np.arange(-reach_y, reach_y + 1), indexing="xy")
-    ox, oy = ox.ravel(), oy.ravel()
-
-    def candidates(i0, i1):
-        cxi = np.rint(X[i0:i1, 0] + (nx - 1) / 2.0).astype(np.int64)
-        cyi = np.rint(X[i0:i1, 1] + (ny - 1) / 2.0).astype(np.int64)
-        ix = cxi[:, None] + ox[None, :]
-        iy = cyi[:, None] + oy[None, :]
-        ok = (ix >= 0) & (ix < nx) & (iy >= 0) & (iy < ny)
-        idx = np.where(ok, iy * nx + ix, 0)
-        return np.where(ok, idx, -1)
-
-    def cand_safe(i0, i1):
-        c = candidates(i0, i1)
-        return c
-
-    op, opt, pix, kept = _observe(R, C, f, k1, k2, X, rng, cap, noise, outlier_fraction,
-                                  candidates=cand_safe)
+    op, opt, pix, kept = _observe_grid(R, C, f, k1, k2, X, rng, cap, noise, outlier_fraction, nx, ny)
 
     def init(R, C, Xk, f, k1, k2):
         dR = exp_so3(rng.normal(0.0, 0.002 * perturb, (len(R), 3)))

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -637,13 +638,105 @@ static bool linearize_obs(const View& v, size_t obs, const double X[3], ObsLin& 
 // ---------------------------------------------------------------------------------------
 // Assembly: schur/assembly.rs:832-1069 and schur/point.rs:176-448
 // ---------------------------------------------------------------------------------------
+// Reduced-system storage: dense lower triangle when the Schur complement is materialised,
+// otherwise the retained Hessian's blocks and pose-intrinsic edges
+// (sparse/self_calibration_retained.rs:78-205).
+struct HStore {
+    bool dense = true;
+    size_t D = 0, nc = 0;
+    std::vector<double> full;              // D*D, entries (r, c) with r >= c
+    std::vector<uint32_t> pblock, plocal;  // parameter -> block, local index
+    std::vector<std::vector<double>> blocks;  // dim*dim per block
+    std::vector<std::vector<double>> edges;   // per camera: pose_dim * 3 (pose_local * 3 + intrinsic_local)
+    std::vector<uint32_t> bdim, cam_group;
+    bool valid = true;
+    void init_dense(size_t d) {
+        dense = true;
+        D = d;
+        full.assign(d * d, 0.0);
+    }
+    void init_blocks(const uint32_t* cam_off, const uint32_t* cam_dim, const uint32_t* grp_off,
+                     const uint32_t* grp_dim, const uint32_t* cgroup, size_t ncams, size_t ngroups, size_t d) {
+        dense = false;
+        D = d;
+        nc = ncams;
+        pblock.assign(d, 0);
+        plocal.assign(d, 0);
+        bdim.assign(ncams + ngroups, 0);
+        blocks.assign(ncams + ngroups, {});
+        edges.assign(ncams, {});
+        cam_group.assign(cgroup, cgroup + ncams);
+        for (size_t c = 0; c < ncams; ++c) {
+            bdim[c] = cam_dim[c];
+            for (uint32_t k = 0; k < cam_dim[c]; ++k) {
+                pblock[cam_off[c] + k] = (uint32_t)c;
+                plocal[cam_off[c] + k] = k;
+            }
+            blocks[c].assign(cam_dim[c] * cam_dim[c], 0.0);
+            edges[c].assign(cam_dim[c] * 3, 0.0);
+        }
+        for (size_t g = 0; g < ngroups; ++g) {
+            bdim[ncams + g] = grp_dim[g];
+            for (uint32_t k = 0; k < grp_dim[g]; ++k) {
+                pblock[grp_off[g] + k] = (uint32_t)(ncams + g);
+                plocal[grp_off[g] + k] = k;
+            }
+            blocks[ncams + g].assign(grp_dim[g] * grp_dim[g], 0.0);
+        }
+    }
+    // add `v` at (r, c) of the symmetric matrix, r >= c in storage
+    void add(size_t r, size_t c, double v) {
+        if (r < c) std::swap(r, c);
+        if (dense) {
+            full[r * D + c] += v;
+            return;
+        }
+        uint32_t br = pblock[r], bc = pblock[c];
+        if (br == bc) {
+            uint32_t n = bdim[br];
+            blocks[br][plocal[r] * n + plocal[c]] += v;
+            return;
+        }
+        // r is the intrinsic parameter (groups follow poses), c the pose parameter
+        if (br >= nc && bc < nc && cam_group[bc] == br - nc) {
+            edges[bc][plocal[c] * 3 + plocal[r]] += v;
+            return;
+        }
+        valid = false;
+    }
+    double get(size_t r, size_t c) const {
+        if (r < c) std::swap(r, c);
+        if (dense) return full[r * D + c];
+        uint32_t br = pblock[r], bc = pblock[c];
+        if (br == bc) return blocks[br][plocal[r] * bdim[br] + plocal[c]];
+        if (br >= nc && bc < nc && cam_group[bc] == br - nc) return edges[bc][plocal[c] * 3 + plocal[r]];
+        return 0.0;
+    }
+    bool finite() const {
+        if (!valid) return false;
+        if (dense) {
+            for (size_t r = 0; r < D; ++r)
+                for (size_t c = 0; c <= r; ++c)
+                    if (!std::isfinite(full[r * D + c])) return false;
+            return true;
+        }
+        for (auto& b : blocks)
+            for (double v : b)
+                if (!std::isfinite(v)) return false;
+        for (auto& e : edges)
+            for (double v : e)
+                if (!std::isfinite(v)) return false;
+        return true;
+    }
+};
+
 struct Assembly {
     bool valid = true;
     size_t weak = 0;
     double min_rcond = 1.0, sgin = 0.0, pmr = 0.0, cost = 0.0, sq = 0.0;
     double min_diag = 0.0, max_diag = 0.0;
     size_t clamp = 0;
-    std::vector<double> H;            // D*D lower (row >= col) used
+    HStore H;                         // S when materialised, else the retained U blocks
     std::vector<double> g, fg, diag;  // reduced gradient, full gradient, damping diagonal
     // point caches (the Cached back-substitution and retained PCG)
     std::vector<uint8_t> cache_valid;
@@ -651,12 +744,7 @@ struct Assembly {
     std::vector<double> inv_cross;                                       // 3 per cross entry
 };
 
-static inline void add_sym(std::vector<double>& H, size_t D, size_t r, size_t c, double v) {
-    if (r >= c)
-        H[r * D + c] += v;
-    else
-        H[c * D + r] += v;
-}
+static inline void add_sym(HStore& H, size_t /*D*/, size_t r, size_t c, double v) { H.add(r, c, v); }
 
 static double scaled_gradient_inf_norm(const double g[3], const double d[3]) {  // :1081-1096
     double scale = 0.0;
@@ -687,7 +775,10 @@ static size_t clamp_block(double* d, size_t n) {  // :744-766
 static Assembly assemble(const View& v, double lambda, bool materialize, const uint32_t* cross_off) {
     const size_t D = v.D;
     Assembly a;
-    a.H.assign(D * D, 0.0);
+    if (materialize)
+        a.H.init_dense(D);
+    else
+        a.H.init_blocks(v.cam_off, v.cam_dim, v.grp_off, v.grp_dim, v.cam_group, v.nc, v.ng, D);
     a.g.assign(D, 0.0);
     a.fg.assign(D, 0.0);
     a.diag.assign(D, 0.0);
@@ -859,7 +950,7 @@ static Assembly assemble(const View& v, double lambda, bool materialize, const u
     for (size_t c = 0; c < v.nc; ++c) clamped += clamp_block(&a.diag[v.cam_off[c]], v.cam_dim[c]);
     for (size_t g = 0; g < v.ng; ++g) clamped += clamp_block(&a.diag[v.grp_off[g]], v.grp_dim[g]);
     a.clamp = clamped;
-    for (size_t i = 0; i < D; ++i) a.H[i * D + i] += lambda * a.diag[i];
+    for (size_t i = 0; i < D; ++i) a.H.add(i, i, lambda * a.diag[i]);
     double prepared = 0.0;
     for (size_t i = 0; i < D; ++i) {
         double cv = a.diag[i], gv = a.fg[i], s;
@@ -870,13 +961,7 @@ static Assembly assemble(const View& v, double lambda, bool materialize, const u
         prepared = fmax_(prepared, s);
     }
     a.sgin = fmax_(a.sgin, prepared);
-    bool finite = true;
-    for (size_t r = 0; r < D && finite; ++r)
-        for (size_t c = 0; c <= r; ++c)
-            if (!std::isfinite(a.H[r * D + c])) {
-                finite = false;
-                break;
-            }
+    bool finite = a.H.finite();
     for (size_t i = 0; i < D; ++i)
         if (!std::isfinite(a.g[i])) finite = false;
     a.valid = a.valid && finite;
@@ -1125,13 +1210,12 @@ static bool build_retained(const View& v, const Assembly& a, const uint32_t* cro
     ret.param_block.assign(v.D, 0);
     for (size_t b = 0; b < ret.nblocks; ++b)
         for (size_t k = 0; k < ret.bdim[b]; ++k) ret.param_block[ret.boff[b] + k] = b;
-    const size_t D = v.D;
     ret.blocks.assign(ret.nblocks, Packed{});
     for (size_t b = 0; b < ret.nblocks; ++b) {
         std::memset(ret.blocks[b].v, 0, sizeof(ret.blocks[b].v));
         size_t o = ret.boff[b];
         for (size_t r = 0; r < ret.bdim[b]; ++r)
-            for (size_t c = 0; c <= r; ++c) ret.blocks[b].v[Packed::idx(r, c)] = a.H[(o + r) * D + o + c];
+            for (size_t c = 0; c <= r; ++c) ret.blocks[b].v[Packed::idx(r, c)] = a.H.get(o + r, o + c);
     }
     ret.edges.clear();
     for (size_t c = 0; c < v.nc; ++c) {
@@ -1144,7 +1228,7 @@ static bool build_retained(const View& v, const Assembly& a, const uint32_t* cro
         for (size_t pl = 0; pl < v.cam_dim[c]; ++pl)
             for (size_t il = 0; il < v.grp_dim[g]; ++il) {
                 size_t r = v.grp_off[g] + il, col = v.cam_off[c] + pl;  // intrinsic rows follow poses
-                e.v[pl * 3 + il] = a.H[r * D + col];
+                e.v[pl * 3 + il] = a.H.get(r, col);
             }
         ret.edges.push_back(e);
     }
@@ -1819,10 +1903,15 @@ struct SolveOut {
     State state;
 };
 
+static double g_loop_start = 0.0;
+static double wall() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 static void trace_push(ba_iteration_trace* tr, double cost, double damping, double cand, double pred, double gain,
                        bool acc, uint32_t pcg) {
     if (!tr || tr->count >= tr->capacity) return;
     uint32_t i = tr->count++;
+    if (tr->seconds) tr->seconds[i] = wall() - g_loop_start;
     if (tr->cost) tr->cost[i] = cost;
     if (tr->damping) tr->damping[i] = damping;
     if (tr->candidate_cost) tr->candidate_cost[i] = cand;
@@ -1869,6 +1958,7 @@ static Err refine(const Lowered& L, const ba_solve_config* cfg, State& st, ba_so
             target = i;
             break;
         }
+    g_loop_start = wall();
     double initial_damping = iterative ? 1e-1 : 1e-4;  // api.rs:300-305
     Nielsen damping{initial_damping};
     size_t iterations = 0, accepted = 0, target_accepted = 0;
@@ -1943,7 +2033,7 @@ static Err refine(const Lowered& L, const ba_solve_config* cfg, State& st, ba_so
             double pcg_rel = NaN;
             std::vector<double> product;
             if (!iterative) {
-                std::vector<double> F = a.H;
+                std::vector<double> F = a.H.full;
                 std::vector<double> rhs(D);
                 for (size_t i = 0; i < D; ++i) rhs[i] = -a.g[i];
                 solved = D == 0 ? true : dense_cholesky_solve(F, D, rhs.data(), step.data());
@@ -2386,7 +2476,7 @@ int bao_assemble(const ba_device_problem* dp, double lambda, double* schur, doub
     Assembly a = assemble(v, lambda, true, cross_off.data());
     size_t D = v.D;
     for (size_t r = 0; r < D; ++r)
-        for (size_t c = 0; c < D; ++c) schur[r * D + c] = r >= c ? a.H[r * D + c] : a.H[c * D + r];
+        for (size_t c = 0; c < D; ++c) schur[r * D + c] = a.H.get(r, c);
     std::memcpy(gradient, a.g.data(), D * sizeof(double));
     std::memcpy(full_gradient, a.fg.data(), D * sizeof(double));
     std::memcpy(diagonal, a.diag.data(), D * sizeof(double));
