@@ -170,8 +170,8 @@ typedef struct ba_solve_report {
     uint64_t observation_count, point_count, pose_count;
     /* wall-clock split of the call (no counterpart in the reference report) */
     double seconds_setup;                 /* lowering, index maps, uploads */
-    double seconds_iterations;            /* the LM loop */
-    double seconds_final;                 /* final zero-damping assembly and download */
+    double seconds_iterations;            /* the LM loop: first iteration's start to the last one's end */
+    double seconds_final;                 /* projection check, final zero-damping assembly, download */
     uint32_t profile_launches;            /* launches of the profiled kernel */
     double profile_seconds;               /* their summed HIP-event durations */
 } ba_solve_report;
@@ -334,19 +334,39 @@ int ba_hip_pcg(ba_hip_handle* handle, const ba_device_problem* problem, double l
 int ba_hip_cholesky_solve(ba_hip_handle* handle, size_t n, const double* a, const double* b,
                           double* x, int32_t precision);
 
-/* Differentiation form: robust reprojection cost sum_obs rho(|r|) of a BAL-style scene
- * (per camera: angle-axis w(3), t(3), f, k1, k2; per point X(3)) and, when grad_camera is
- * non-NULL, its gradient with respect to every camera and point parameter.  The backward
- * kernel recomputes the Jacobians.  `is_fp32` selects float buffers (all float) instead of
- * double.  Deterministic: per-camera and per-point sums run in a fixed order.
- * No counterpart entry point in the reference; it evaluates the objective of
- * numerics/robust.rs:8-44 with the camera model of self_calibration/model.rs:26-129. */
-int ba_hip_cost_grad(ba_hip_handle* handle, int32_t is_fp32, size_t n_cameras, size_t n_points,
-                     size_t n_observations, const void* cameras /*9*n_cameras*/,
-                     const void* points /*3*n_points*/, const uint32_t* obs_camera,
-                     const uint32_t* obs_point, const void* obs_uv /*2*n_obs*/,
-                     int32_t loss, double loss_scale, void* cost_out /*1*/,
-                     void* grad_camera /*9*n_cameras or NULL*/, void* grad_point /*3*n_points*/);
+/* ---- Differentiation form ------------------------------------------------------------------
+ * Robust reprojection cost  C = sum_o rho(|r_o|)  over observations o = (camera c, point p, pixel m)
+ * with the radial model of self_calibration/model.rs:26-129 (X_c = exp(w) X + t, n = X_c.xy / X_c.z,
+ * r_o = f (1 + k1 |n|^2 + k2 |n|^4) n - m, m already relative to the principal point) and rho of
+ * numerics/robust.rs:8-44.  Cameras are 9 values [w(3) angle-axis, t(3), f, k1, k2]; points are
+ * [X, Y, Z].  Observations outside the model's projection domain (model.rs:26-75) contribute
+ * nothing and are counted.  The backward pass recomputes every Jacobian; its camera rotation
+ * gradient is J_l(w)^T times the tangent gradient of the reference's pose motion
+ * (schur/point.rs:730-762).  No counterpart entry points in the reference: they expose its
+ * objective to automatic differentiation.  Sums run over fixed partitions in a fixed order, so
+ * results do not depend on scheduling. */
+typedef struct ba_reprojection_topology ba_reprojection_topology;
+
+/* Index the observations once: obs_camera / obs_point are HOST arrays of n_observations entries,
+ * checked against n_cameras / n_points.  The topology keeps device copies and the workspace of
+ * every later ba_hip_cost_grad call on it. */
+int ba_hip_reprojection_topology_create(ba_hip_handle* handle, size_t n_cameras, size_t n_points,
+                                        size_t n_observations, const uint32_t* obs_camera,
+                                        const uint32_t* obs_point, ba_reprojection_topology** out,
+                                        char* message, size_t message_len);
+void ba_hip_reprojection_topology_destroy(ba_reprojection_topology* topology);
+
+/* Forward (grad_camera and grad_point NULL) or forward + backward on `stream` (a hipStream_t,
+ * NULL for the handle's stream; no host synchronisation when a stream is given).  All pointers
+ * are device memory of the element type is_fp32 ? float : double: cameras (9 n_cameras), points
+ * (3 n_points), obs_uv (2 n_observations), cost_out (1).  grad_scale (1 element, NULL = 1)
+ * multiplies both gradients: dC/dcameras (9 n_cameras) and dC/dpoints (3 n_points).
+ * invalid_out (uint32, NULL to skip) receives the number of out-of-domain observations. */
+int ba_hip_cost_grad(ba_hip_handle* handle, const ba_reprojection_topology* topology, int32_t is_fp32,
+                     const void* cameras, const void* points, const void* obs_uv, int32_t loss,
+                     double loss_scale, const void* grad_scale, void* cost_out, uint32_t* invalid_out,
+                     void* grad_camera, void* grad_point, void* stream, char* message,
+                     size_t message_len);
 
 #ifdef __cplusplus
 }

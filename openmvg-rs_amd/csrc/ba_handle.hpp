@@ -1,0 +1,9 @@
+// ba_handle.hpp — the opaque handle behind ba_hip_handle*: one device and one non-blocking stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+struct ba_hip_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};

@@ -18,13 +18,9 @@
 #include <vector>
 
 #include "../../include/ba_hip.h"
+#include "ba_handle.hpp"
 #include "ba_host.hpp"
 #include "ba_kernels.hpp"
-
-struct ba_hip_handle {
-    int device = 0;
-    hipStream_t stream = nullptr;
-};
 
 namespace ba {
 namespace {
@@ -505,12 +501,18 @@ GaugeDev gauge_from(const ba_device_problem* dp) {
     return g;
 }
 
-double g_loop_start = 0.0;
-void trace_push(ba_iteration_trace* tr, double cost, double damping, double cand, double pred, double gain, bool acc,
+// Per-iteration trace rows; `loop_start` is the host clock when the LM loop began.
+struct TraceSink {
+    ba_iteration_trace* tr = nullptr;
+    double loop_start = 0.0;
+};
+
+void trace_push(const TraceSink& sink, double cost, double damping, double cand, double pred, double gain, bool acc,
                 uint32_t pcg) {
+    ba_iteration_trace* tr = sink.tr;
     if (!tr || tr->count >= tr->capacity) return;
     uint32_t i = tr->count++;
-    if (tr->seconds) tr->seconds[i] = now() - g_loop_start;
+    if (tr->seconds) tr->seconds[i] = now() - sink.loop_start;
     if (tr->cost) tr->cost[i] = cost;
     if (tr->damping) tr->damping[i] = damping;
     if (tr->candidate_cost) tr->candidate_cost[i] = cand;
@@ -527,8 +529,10 @@ struct StageDef {
 };
 
 // The LM loop (sparse/self_calibration/mod.rs:212-639) on the GPU engine.
-Status refine(Engine& E, const Lowered& L, const ba_solve_config* cfg, ba_solve_report& rep, ba_iteration_trace* trace,
-              int* final_state) {
+// `loop_seconds` receives the host time from the first iteration's start to the end of the last one;
+// every iteration ends in a synchronous scalar readback, so this brackets completed device work.
+Status refine(Engine& E, const Lowered& L, const ba_solve_config* cfg, ba_solve_report& rep,
+              ba_iteration_trace* trace_rows, int* final_state, double* loop_seconds) {
     const size_t D = L.D;
     const bool iterative = E.iterative();
     const double no = (double)L.n_obs();
@@ -556,7 +560,7 @@ Status refine(Engine& E, const Lowered& L, const ba_solve_config* cfg, ba_solve_
         AsmScalars a0 = E.assemble(state, 0.0, L.loss, L.loss_scale, false);
         if (a0.unprojectable > 0.0) return error(BA_ERR_NUMERICAL, "point not projectable at the initial state");
     }
-    g_loop_start = now();
+    TraceSink trace{trace_rows, now()};
     const double initial_damping = iterative ? 1e-1 : 1e-4;  // sparse/api.rs:300-305
     Nielsen damping{initial_damping};
     uint32_t iterations = 0, accepted = 0, target_accepted = 0;
@@ -678,6 +682,7 @@ Status refine(Engine& E, const Lowered& L, const ba_solve_config* cfg, ba_solve_
             }
         }
     }
+    *loop_seconds = now() - trace.loop_start;
     AsmScalars fa = E.assemble(state, 0.0, L.loss, L.loss_scale, false);
     std::memset(&rep, 0, sizeof(rep));
     if (conv >= 0)
@@ -821,16 +826,16 @@ int ba_hip_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* c
         double t1 = now();
         ba_solve_report r;
         int state = 0;
-        st = refine(E, L, cfg, r, trace, &state);
+        double loop_seconds = 0.0;
+        st = refine(E, L, cfg, r, trace, &state, &loop_seconds);
         if (!st.ok()) return finish(st, msg, len);
-        double t2 = now();
         std::vector<double> R, t, gp, X;
         E.download(state, R, t, gp, X);
         raise_solution(p, L, R.data(), t.data(), gp.data(), X.data(), sol);
         double t3 = now();
         r.seconds_setup = t1 - t0;
-        r.seconds_iterations = t2 - t1;
-        r.seconds_final = t3 - t2;
+        r.seconds_iterations = loop_seconds;
+        r.seconds_final = (t3 - t1) - loop_seconds;
         E.profile_result(&r.profile_launches, &r.profile_seconds);
         if (rep) *rep = r;
         return BA_OK;

@@ -38,9 +38,12 @@ def load() -> C.CDLL:
     lib.ba_hip_schur.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp]
     lib.ba_hip_pcg.argtypes = [vp, vp, C.c_double, C.c_double, C.c_uint32, vp, vp, vp, vp]
     lib.ba_hip_cholesky_solve.argtypes = [vp, C.c_size_t, vp, vp, vp, C.c_int32]
-    if hasattr(lib, "ba_hip_cost_grad"):
-        lib.ba_hip_cost_grad.argtypes = [vp, C.c_int32, C.c_size_t, C.c_size_t, C.c_size_t, vp, vp, vp, vp, vp,
-                                         C.c_int32, C.c_double, vp, vp, vp]
+    sz = C.c_size_t
+    lib.ba_hip_reprojection_topology_create.argtypes = [vp, sz, sz, sz, vp, vp, C.POINTER(vp), C.c_char_p, sz]
+    lib.ba_hip_reprojection_topology_destroy.argtypes = [vp]
+    lib.ba_hip_reprojection_topology_destroy.restype = None
+    lib.ba_hip_cost_grad.argtypes = [vp, vp, C.c_int32, vp, vp, vp, C.c_int32, C.c_double, vp, vp, vp, vp, vp, vp,
+                                     C.c_char_p, sz]
     _lib = lib
     return lib
 

@@ -2580,6 +2580,77 @@ double bao_first_positive_radial_turn(const double cam[3]) {
     double r;
     return first_positive_radial_turn(Radial{cam[0], cam[1], cam[2]}, &r) ? r : NaN;
 }
+// Robust reprojection objective of a BAL-style scene and its gradient (differentiation form):
+// per observation the radial residual and Jacobians of model.rs:26-129, the robust weight of
+// robust.rs:8-44 (d rho/dr = w r with w = sqrt_weight^2), the pose motion of schur/point.rs:730-762
+// (d X_c / d omega = -[R X]x for the left perturbation exp(omega) R) and, for the angle-axis
+// parameter itself, the chain rule through the SO(3) left Jacobian
+// J_l(w) = I + (1 - cos t)/t^2 [w]x + (t - sin t)/t^3 [w]x^2.  Sums in input order.
+int bao_cost_grad(size_t nc, size_t np, size_t no, const double* cams, const double* pts, const uint32_t* oc,
+                  const uint32_t* op, const double* uv, int32_t loss, double scale, double* cost,
+                  uint32_t* invalid, double* gcam, double* gpt) {
+    std::vector<Mat3> R(nc);
+    for (size_t c = 0; c < nc; ++c) R[c] = exp_so3(cams + 9 * c);
+    std::vector<double> gomega(3 * nc, 0.0);
+    if (gcam) std::fill(gcam, gcam + 9 * nc, 0.0);
+    if (gpt) std::fill(gpt, gpt + 3 * np, 0.0);
+    double total = 0.0;
+    uint32_t bad = 0;
+    for (size_t i = 0; i < no; ++i) {
+        const uint32_t c = oc[i], p = op[i];
+        const double* q = cams + 9 * c;
+        const double* X = pts + 3 * p;
+        double RX[3];
+        for (int r = 0; r < 3; ++r) RX[r] = R[c][r][0] * X[0] + R[c][r][1] * X[1] + R[c][r][2] * X[2];
+        const double Xc[3] = {RX[0] + q[3], RX[1] + q[4], RX[2] + q[5]};
+        double res[2], J[2][3], Ji[2][3];
+        if (!radial_linearization(Radial{q[6], q[7], q[8]}, uv + 2 * i, Xc, res, J, Ji)) {
+            ++bad;
+            continue;
+        }
+        const double nrm = std::sqrt(res[0] * res[0] + res[1] * res[1]);
+        total += loss_cost(Loss{loss, scale}, nrm);
+        if (!gcam) continue;
+        const double sw = loss_sqrt_weight(Loss{loss, scale}, nrm);
+        const double psi[2] = {sw * sw * res[0], sw * sw * res[1]};
+        double e[3];
+        for (int k = 0; k < 3; ++k) e[k] = J[0][k] * psi[0] + J[1][k] * psi[1];
+        for (int k = 0; k < 3; ++k) gpt[3 * p + k] += R[c][0][k] * e[0] + R[c][1][k] * e[1] + R[c][2][k] * e[2];
+        gomega[3 * c + 0] += RX[1] * e[2] - RX[2] * e[1];
+        gomega[3 * c + 1] += RX[2] * e[0] - RX[0] * e[2];
+        gomega[3 * c + 2] += RX[0] * e[1] - RX[1] * e[0];
+        for (int k = 0; k < 3; ++k) gcam[9 * c + 3 + k] += e[k];
+        gcam[9 * c + 6] += (Ji[0][0] * psi[0] + Ji[1][0] * psi[1]) / q[6];
+        gcam[9 * c + 7] += Ji[0][1] * psi[0] + Ji[1][1] * psi[1];
+        gcam[9 * c + 8] += Ji[0][2] * psi[0] + Ji[1][2] * psi[1];
+    }
+    if (gcam)
+        for (size_t c = 0; c < nc; ++c) {
+            const double* w = cams + 9 * c;
+            const double t2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2], t = std::sqrt(t2);
+            double A, B;
+            if (t < 1e-4) {
+                A = 0.5 - t2 / 24.0;
+                B = 1.0 / 6.0 - t2 / 120.0;
+            } else {
+                A = (1.0 - std::cos(t)) / t2;
+                B = (t - std::sin(t)) / (t2 * t);
+            }
+            const double K[3][3] = {{0, -w[2], w[1]}, {w[2], 0, -w[0]}, {-w[1], w[0], 0}};
+            double Jl[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int k = 0; k < 3; ++k) {
+                    double k2 = K[r][0] * K[0][k] + K[r][1] * K[1][k] + K[r][2] * K[2][k];
+                    Jl[r][k] = (r == k ? 1.0 : 0.0) + A * K[r][k] + B * k2;
+                }
+            for (int k = 0; k < 3; ++k)
+                gcam[9 * c + k] =
+                    Jl[0][k] * gomega[3 * c] + Jl[1][k] * gomega[3 * c + 1] + Jl[2][k] * gomega[3 * c + 2];
+        }
+    if (cost) *cost = total;
+    if (invalid) *invalid = bad;
+    return BA_OK;
+}
 double bao_loss_cost(int32_t loss, double scale, double r) { return loss_cost(Loss{loss, scale}, r); }
 double bao_loss_sqrt_weight(int32_t loss, double scale, double r) { return loss_sqrt_weight(Loss{loss, scale}, r); }
 void bao_exp_so3(const double v[3], double R[9]) {

@@ -85,6 +85,12 @@ int bao_nielsen_accept(double* value, double* nu, double actual, double predicte
 void bao_nielsen_reject(double* value, double* nu);
 /* stages() of continuation.rs:26-74: writes 3 (loss, iterations) pairs. */
 void bao_stages(int32_t loss, uint32_t max_iterations, int32_t loss_out[3], uint32_t iterations_out[3]);
+/* Robust reprojection objective sum_o rho(|r_o|) of a BAL-style scene (cameras 9 values each:
+ * angle-axis w, t, f, k1, k2; X_c = exp(w) X + t) and its gradient; gcam/gpt may be NULL. */
+int bao_cost_grad(size_t n_cameras, size_t n_points, size_t n_observations, const double* cameras,
+                  const double* points, const uint32_t* obs_camera, const uint32_t* obs_point,
+                  const double* obs_uv, int32_t loss, double loss_scale, double* cost, uint32_t* invalid,
+                  double* grad_camera, double* grad_point);
 /* Dense cost of a self-calibrating problem in the reference's native frame with an optional
  * radial intrinsic prior (schur/priors.rs:302-331), for the golden initial cost. */
 double bao_native_cost(const ba_device_problem* problem, size_t n_priors, const uint32_t* prior_group,
