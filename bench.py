@@ -103,6 +103,43 @@ def cpu_baseline(scene_fn, n_points_sample: int, n_obs_full: int, iterations: in
     }
 
 
+def bench_differentiation(steps: int) -> dict:
+    """Forward+backward observations/s of the differentiable reprojection cost: 1,000 cameras,
+    200,000 points, 1,000,000 observations, Huber 1 px (seed 0xBA0005), FP32 and FP64, through
+    the torch.autograd.Function (forward call + backward call) on torch's current stream."""
+    import torch
+
+    from openmvg_ba import _abi as A, synthetic
+    from openmvg_ba.autodiff import ReprojectionTopology, reprojection_cost
+
+    cams, pts, oc, op, uv = synthetic.bal_scene()
+    topo = ReprojectionTopology(oc, op, len(cams), len(pts))
+    out = {"workload": f"{len(cams)} cameras, {len(pts)} points, {len(oc)} observations, Huber 1px, seed 0xBA0005"}
+    for name, dtype in (("fp32", torch.float32), ("fp64", torch.float64)):
+        tc = torch.tensor(cams, device="cuda", dtype=dtype, requires_grad=True)
+        tp = torch.tensor(pts, device="cuda", dtype=dtype, requires_grad=True)
+        tuv = torch.tensor(uv, device="cuda", dtype=dtype)
+
+        def step():
+            tc.grad = None
+            tp.grad = None
+            reprojection_cost(tc, tp, tuv, topo, A.LOSS_HUBER, 1.0).backward()
+
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / 1e3 / steps
+        out[name] = {"obs_per_s": len(oc) / sec, "ms_per_forward_backward": sec * 1e3}
+    topo.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +153,7 @@ def main() -> int:
     ap.add_argument("--cpu-points", type=int, default=200_000, help="points of the CPU-baseline subsample")
     ap.add_argument("--cpu-iterations", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the other forms' measurements")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -190,6 +228,11 @@ def main() -> int:
         log(f"converge: {conv.report.iterations} iterations, {conv.report.status}, cost {c_star:.9e}, "
             f"time-to-cost {hit}")
 
+    forms = {}
+    if not args.no_forms:
+        forms["differentiation"] = bench_differentiation(20)
+        log(f"differentiation: {forms['differentiation']}")
+
     cpu = None
     if not args.no_cpu:
         t0 = time.perf_counter()
@@ -232,6 +275,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "time_to_cost": ttc,
+        "forms": forms,
     }
     print(json.dumps(line), flush=True)
     return 0

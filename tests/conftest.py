@@ -12,3 +12,11 @@ for path in (ROOT / "openmvg-rs_amd", ROOT / "tests"):
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libba_hip.so")
     config.addinivalue_line("markers", "slow: larger CPU parity cases")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    """The CPU oracle (oracle/_build/libba_oracle.so), the checker of every parity test."""
+    from oracle_lib import load_oracle
+
+    return load_oracle()

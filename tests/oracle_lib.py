@@ -38,6 +38,10 @@ def load_oracle() -> C.CDLL:
     lib.bao_assemble.argtypes = [C.c_void_p, d] + [C.c_void_p] * 7 + [C.c_void_p]
     lib.bao_pcg.argtypes = [C.c_void_p, d, d, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.bao_track_geometry.argtypes = [C.c_size_t, C.c_void_p, d]
+    lib.bao_exp_so3.argtypes = [C.c_void_p, C.c_void_p]
+    lib.bao_radial_projection.argtypes = [C.c_void_p] * 5
+    lib.bao_radial_linearization.argtypes = [C.c_void_p] * 6
+    lib.bao_cost_grad.argtypes = [C.c_size_t] * 3 + [C.c_void_p] * 5 + [C.c_int32, d] + [C.c_void_p] * 4
     lib.bao_point_block_factor.argtypes = [C.c_void_p, d, C.c_void_p]
     lib.bao_point_block_rcond.argtypes = [C.c_void_p]
     return lib
