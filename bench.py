@@ -198,7 +198,7 @@ def main() -> int:
         f"{rep.final_cost:.6e}, status {rep.status}")
 
     # roofline of the dominant kernel: HIP events around each of its launches
-    kernel_ids = {"pcg_points": A.PROFILE_PCG_POINTS, "pcg_blocks": A.PROFILE_PCG_BLOCKS,
+    kernel_ids = {"pcg_points": A.PROFILE_PCG_POINTS, "pcg_cameras": A.PROFILE_PCG_CAMERAS,
                   "linearize": A.PROFILE_LINEARIZE}
     prof = ba.solve(prob, base.with_(max_iterations=max(2, min(args.steps, 4)),
                                      profile_kernel=kernel_ids[args.roofline_kernel]))

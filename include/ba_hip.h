@@ -128,7 +128,7 @@ enum ba_profile_kernel {
     BA_PROFILE_LINEARIZE = 1,      /* kernel (a) */
     BA_PROFILE_POINT_ELIM = 2,     /* kernel (b), per-point elimination */
     BA_PROFILE_PCG_POINTS = 3,     /* kernel (c), point pass of the implicit Schur product */
-    BA_PROFILE_PCG_BLOCKS = 4,     /* kernel (c), camera-block pass of the implicit Schur product */
+    BA_PROFILE_PCG_CAMERAS = 4,    /* kernel (c), camera pass of the implicit Schur product */
     BA_PROFILE_PAIRS = 5,          /* kernel (b), block-pair segmented reduction */
     BA_PROFILE_CHOLESKY = 6,       /* kernel (d), factor and solve */
     BA_PROFILE_CANDIDATE = 7       /* candidate pass */

@@ -128,13 +128,16 @@ __global__ void k_gauge(const T* cam_t, int camera, double scene_scale, GaugeDev
 // State is FP64; the camera-frame point, residual and robust cost are evaluated in FP64 and the
 // Jacobians in T.  Outputs are robust-weighted: r~ = sqrt(w) r, J~ = sqrt(w) J.
 // ----------------------------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(NT) k_linearize(Topo tp, const double* __restrict__ Rg, const double* __restrict__ tg,
-                                                  const double* __restrict__ gp, const double* __restrict__ X,
-                                                  const double* __restrict__ uv, const GaugeDev* __restrict__ gauge,
-                                                  Lin<T> out) {
-    uint32_t o = blockIdx.x * NT + threadIdx.x;
-    if (o >= tp.no) return;
+namespace {
+// Linearisation of observation o (point-major index) at the state (R, t, gp, X): robust-weighted
+// residual r~, point Jacobian J~_X (2x3, world frame, times the scene scale) and camera Jacobian
+// J~_c (2x9: pose columns 0..5 by the pose motion of schur/point.rs:730-762, intrinsic columns
+// 6..8), zero beyond the camera's and group's free dimensions.  False outside the domain.
+__device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t o, const double* __restrict__ Rg,
+                                              const double* __restrict__ tg, const double* __restrict__ gp,
+                                              const double* __restrict__ X, const double* __restrict__ uv,
+                                              const GaugeDev* __restrict__ gauge, double r[2], double jx[6],
+                                              double jc[18], double* cost, double* sq) {
     const uint32_t c = tp.obs_cam[o], p = tp.obs_pt[o], g = tp.cam_group[c];
     const double* R = Rg + 9 * (size_t)c;
     double Xp[3] = {X[3 * (size_t)p], X[3 * (size_t)p + 1], X[3 * (size_t)p + 2]};
@@ -145,33 +148,24 @@ __global__ void __launch_bounds__(NT) k_linearize(Topo tp, const double* __restr
     double turn = 0.0;
     bool has_turn = first_turn(k1, k2, &turn);
     double res[2], J[2][3], Ji[2][3];
-    bool ok = radial_linearize<double>(f, k1, k2, turn, has_turn, cp, uv[2 * (size_t)o], uv[2 * (size_t)o + 1], res, J, Ji);
-    T* rj = out.rj + 8 * (size_t)o;
-    T* jc = out.jc + 18 * (size_t)o;
-    if (!ok) {
-        out.valid[o] = 0;
-        out.cost[o] = T(0);
-        out.sq[o] = T(0);
-        for (int k = 0; k < 8; ++k) rj[k] = T(0);
-        for (int k = 0; k < 18; ++k) jc[k] = T(0);
-        return;
-    }
+    if (!radial_linearize<double>(f, k1, k2, turn, has_turn, cp, uv[2 * (size_t)o], uv[2 * (size_t)o + 1], res, J, Ji))
+        return false;
     double rn = sqrt(res[0] * res[0] + res[1] * res[1]);
     double sw = loss_sqrt_weight<double>(tp.loss, tp.loss_scale, rn);
-    out.cost[o] = (T)loss_cost<double>(tp.loss, tp.loss_scale, rn);
-    out.sq[o] = (T)(rn * rn);
-    out.valid[o] = 1;
+    *cost = loss_cost<double>(tp.loss, tp.loss_scale, rn);
+    *sq = rn * rn;
     const double s = gauge->scene_scale;
-    rj[0] = (T)(sw * res[0]);
-    rj[1] = (T)(sw * res[1]);
+    r[0] = sw * res[0];
+    r[1] = sw * res[1];
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            double v = J[r][0] * R[k] + J[r][1] * R[3 + k] + J[r][2] * R[6 + k];
-            rj[2 + 3 * r + k] = (T)(sw * (v * s));
+            double v = J[a][0] * R[k] + J[a][1] * R[3 + k] + J[a][2] * R[6 + k];
+            jx[3 * a + k] = sw * (v * s);
         }
-    // pose motion (schur/point.rs:730-762): rotation columns -[RX]_x, translation columns
+    // pose motion: rotation columns -[RX]_x, translation columns (scene scale, or the baseline
+    // camera's S^2 tangent basis times the baseline)
     const uint32_t pd = tp.cam_dim[c];
     double mot[3][6] = {{0.0, rot[2], -rot[1], 0, 0, 0}, {-rot[2], 0.0, rot[0], 0, 0, 0}, {rot[1], -rot[0], 0.0, 0, 0, 0}};
     if ((int)c == gauge->camera) {
@@ -184,15 +178,79 @@ __global__ void __launch_bounds__(NT) k_linearize(Topo tp, const double* __restr
     }
     const uint32_t idim = tp.grp_dim[g];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int a = 0; a < 2; ++a) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            double v = (k < (int)pd) ? (J[r][0] * mot[0][k] + J[r][1] * mot[1][k] + J[r][2] * mot[2][k]) : 0.0;
-            jc[9 * r + k] = (T)(sw * v);
+            double v = (k < (int)pd) ? (J[a][0] * mot[0][k] + J[a][1] * mot[1][k] + J[a][2] * mot[2][k]) : 0.0;
+            jc[9 * a + k] = sw * v;
         }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) jc[9 * r + 6 + k] = (T)(k < (int)idim ? sw * Ji[r][k] : 0.0);
+        for (int k = 0; k < 3; ++k) jc[9 * a + 6 + k] = k < (int)idim ? sw * Ji[a][k] : 0.0;
     }
+    return true;
+}
+}  // namespace
+
+// ----------------------------------------------------------------------------------------------
+// (a) k_linearize: one lane per observation (point-major order).
+// State is FP64; the camera-frame point, residual and robust cost are evaluated in FP64 and the
+// Jacobians stored in T.  Outputs are robust-weighted: r~ = sqrt(w) r, J~ = sqrt(w) J.  With
+// out.jpm set, the Jacobians are also written as structure-of-arrays rows (coalesced).
+// ----------------------------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(NT) k_linearize(Topo tp, const double* __restrict__ Rg, const double* __restrict__ tg,
+                                                  const double* __restrict__ gp, const double* __restrict__ X,
+                                                  const double* __restrict__ uv, const GaugeDev* __restrict__ gauge,
+                                                  Lin<T> out) {
+    uint32_t o = blockIdx.x * NT + threadIdx.x;
+    if (o >= tp.no) return;
+    double r[2], jx[6], jc[18], cost = 0.0, sq = 0.0;
+    const bool ok = linearize_obs(tp, o, Rg, tg, gp, X, uv, gauge, r, jx, jc, &cost, &sq);
+    if (!ok) {
+        for (int k = 0; k < 2; ++k) r[k] = 0.0;
+        for (int k = 0; k < 6; ++k) jx[k] = 0.0;
+        for (int k = 0; k < 18; ++k) jc[k] = 0.0;
+        cost = sq = 0.0;
+    }
+    out.valid[o] = ok ? 1 : 0;
+    out.cost[o] = (T)cost;
+    out.sq[o] = (T)sq;
+    T* rj = out.rj + 8 * (size_t)o;
+    T* jo = out.jc + 18 * (size_t)o;
+    rj[0] = (T)r[0];
+    rj[1] = (T)r[1];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rj[2 + k] = (T)jx[k];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) jo[k] = (T)jc[k];
+    if (out.jpm) {
+        const size_t no = tp.no;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) out.jpm[(SOA_JX + k) * no + o] = (T)jx[k];
+#pragma unroll
+        for (int k = 0; k < 18; ++k) out.jpm[(SOA_JC + k) * no + o] = (T)jc[k];
+    }
+}
+
+// Camera-major twin of k_linearize: lane j linearises observation cm_obs[j] and writes the
+// structure-of-arrays camera-major Jacobian copy (coalesced writes; gathered, cached reads).
+template <class T>
+__global__ void __launch_bounds__(NT) k_linearize_cm(Topo tp, const double* __restrict__ Rg,
+                                                     const double* __restrict__ tg, const double* __restrict__ gp,
+                                                     const double* __restrict__ X, const double* __restrict__ uv,
+                                                     const GaugeDev* __restrict__ gauge, Lin<T> out) {
+    uint32_t j = blockIdx.x * NT + threadIdx.x;
+    if (j >= tp.no) return;
+    double r[2], jx[6], jc[18], cost, sq;
+    if (!linearize_obs(tp, tp.cm_obs[j], Rg, tg, gp, X, uv, gauge, r, jx, jc, &cost, &sq)) {
+        for (int k = 0; k < 6; ++k) jx[k] = 0.0;
+        for (int k = 0; k < 18; ++k) jc[k] = 0.0;
+    }
+    const size_t no = tp.no;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out.jcm[(SOA_JX + k) * no + j] = (T)jx[k];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) out.jcm[(SOA_JC + k) * no + j] = (T)jc[k];
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -920,101 +978,145 @@ __global__ void __launch_bounds__(1024) k_pcg_init(Topo tp, const double* grad, 
     if (threadIdx.x == 0) *st = s;
 }
 
-// Matrix-free S v: pass 1 per point (w_p = V^ sum_o R_o v), pass 2 per block.
+// Matrix-free S v, S = U + lambda diag - W V^-1 W^T with U = sum_o J~_c,o^T J~_c,o and
+// W_o = J~_c,o^T J~_X,o (the retained operator of self_calibration_retained/operator.rs:26-76,
+// evaluated from the Jacobians instead of stored block products).
+namespace {
+// Values of v on the 9 camera-Jacobian columns of camera c (0 beyond the free dimensions).
+__device__ __forceinline__ void camera_values(const Topo& tp, uint32_t c, const double* v, double vc[9]) {
+    const uint32_t pd = tp.cam_dim[c], po = tp.cam_off[c], g = tp.cam_group[c];
+    const uint32_t gd = tp.grp_dim[g], go = tp.grp_off[g];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) vc[k] = k < (int)pd ? v[po + k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vc[6 + k] = k < (int)gd ? v[go + k] : 0.0;
+}
+}  // namespace
+
+// Point pass: one workgroup per point chunk, one lane per observation; per-observation
+// e_o = J~_X,o^T (J~_c,o v) is summed per point in observation order, then w_p = V^-1 sum e_o.
 template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_pass1(Topo tp, PointElim<T> pe, const double* x, const double* d, double* w,
-                                                  const PcgState* st) {
+__global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointElim<T> pe, const double* x,
+                                                   const double* d, double* w, const PcgState* st) {
+    __shared__ double sh[3][NT];
     if (st->done) return;
     const double* v = st->mode == PCG_NORMAL ? d : x;
-    uint32_t p = blockIdx.x * NT + threadIdx.x;
-    if (p >= tp.np) return;
-    double s3[3] = {0.0, 0.0, 0.0};
-    if (!tp.pt_fixed[p]) {
-        for (uint32_t o = tp.pt_off[p]; o < tp.pt_off[p + 1]; ++o) {
-            const uint32_t c = tp.obs_cam[o];
-            const T* rr = pe.rrows + 27 * (size_t)o;
-            for (int k = 0; k < 9; ++k) {
-                int q = row_param(tp, c, k);
-                if (q < 0) continue;
-                double val = v[q];
-                s3[0] += (double)rr[3 * k] * val;
-                s3[1] += (double)rr[3 * k + 1] * val;
-                s3[2] += (double)rr[3 * k + 2] * val;
-            }
+    const uint32_t k = blockIdx.x, t = threadIdx.x;
+    const uint32_t ob = tp.chunk_obs[k], oe = tp.chunk_obs[k + 1];
+    const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
+    const size_t no = tp.no;
+    double acc[3] = {0.0, 0.0, 0.0};
+    // lanes take observations ob + t, ob + t + NT, ... (one pass unless one point has > NT)
+    for (uint32_t o = ob + t; o < oe; o += NT) {
+        double vc[9];
+        camera_values(tp, tp.obs_cam[o], v, vc);
+        double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            t0 += (double)lin.jpm[(SOA_JC + q) * no + o] * vc[q];
+            t1 += (double)lin.jpm[(SOA_JC + 9 + q) * no + o] * vc[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            acc[q] += (double)lin.jpm[(SOA_JX + q) * no + o] * t0 + (double)lin.jpm[(SOA_JX + 3 + q) * no + o] * t1;
+    }
+    sh[0][t] = acc[0];
+    sh[1][t] = acc[1];
+    sh[2][t] = acc[2];
+    __syncthreads();
+    if (oe - ob > NT) {  // a single point: fixed-shape tree over the lanes
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if ((int)t < s)
+                for (int q = 0; q < 3; ++q) sh[q][t] += sh[q][t + s];
+            __syncthreads();
         }
     }
-    const T* vd = pe.vdamp + 6 * (size_t)p;
-    double m[6] = {(double)vd[0], (double)vd[1], (double)vd[2], (double)vd[3], (double)vd[4], (double)vd[5]};
-    double out[3];
-    sym3_mul(m, s3, out);
+    if (t >= pend - pb) return;
+    const uint32_t p = pb + t;
+    double s3[3] = {0.0, 0.0, 0.0};
+    if (oe - ob > NT) {
+        for (int q = 0; q < 3; ++q) s3[q] = sh[q][0];
+    } else {
+        for (uint32_t o = tp.pt_off[p]; o < tp.pt_off[p + 1]; ++o)
+            for (int q = 0; q < 3; ++q) s3[q] += sh[q][o - ob];
+    }
+    double out[3] = {0.0, 0.0, 0.0};
+    if (!tp.pt_fixed[p]) {
+        const T* vi = pe.vinv + 6 * (size_t)p;
+        double m[6] = {(double)vi[0], (double)vi[1], (double)vi[2], (double)vi[3], (double)vi[4], (double)vi[5]};
+        sym3_mul(m, s3, out);
+    }
     w[3 * (size_t)p] = out[0];
     w[3 * (size_t)p + 1] = out[1];
     w[3 * (size_t)p + 2] = out[2];
 }
 
+// Camera pass: one workgroup per camera over its camera-major observations;
+// P_c = sum_o J~_c,o^T (J~_c,o v - J~_X,o w_p), a fixed-shape tree over the lanes.
 template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_pass2(Topo tp, PointElim<T> pe, const int32_t* diag_pair,
-                                                  const int32_t* edge_pair, const uint64_t* value_off,
-                                                  const double* u_vals, const double* x, const double* d,
-                                                  const double* w, double* q, const PcgState* st) {
-    __shared__ double sh[NT];
+__global__ void __launch_bounds__(NT) k_pcg_cameras(Topo tp, Lin<T> lin, const double* x, const double* d,
+                                                    const double* w, double* P, const PcgState* st) {
+    __shared__ double sh[9][NT];
     if (st->done) return;
     const double* v = st->mode == PCG_NORMAL ? d : x;
-    const uint32_t A = blockIdx.x;
-    const int dA = (int)tp.blk_dim[A];
-    if (dA == 0) return;
-    const int rowbase = A < tp.nc ? 0 : 6;
-    const uint32_t lb = tp.blk_obs_off[A], le = tp.blk_obs_off[A + 1];
-    const uint32_t len = le - lb;
-    const int chunks = NT / dA;
-    const int t = threadIdx.x;
-    const int r = t % dA, ch = t / dA;
-    double acc = 0.0;
-    if (ch < chunks) {
-        uint32_t c0 = lb + (uint32_t)((uint64_t)len * ch / chunks);
-        uint32_t c1 = lb + (uint32_t)((uint64_t)len * (ch + 1) / chunks);
-        for (uint32_t k = c0; k < c1; ++k) {
-            uint32_t i = tp.blk_obs[k];
-            uint32_t p = tp.obs_pt[i];
-            if (tp.pt_fixed[p]) continue;
-            const T* rr = pe.rrows + 27 * (size_t)i + 3 * (rowbase + r);
-            const double* wp = w + 3 * (size_t)p;
-            acc += (double)rr[0] * wp[0] + (double)rr[1] * wp[1] + (double)rr[2] * wp[2];
+    const uint32_t c = blockIdx.x, t = threadIdx.x;
+    const uint32_t jb = tp.cm_off[c], je = tp.cm_off[c + 1];
+    const size_t no = tp.no;
+    double vc[9];
+    camera_values(tp, c, v, vc);
+    double acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+    for (uint32_t j = jb + t; j < je; j += NT) {
+        const uint32_t p = tp.cm_pt[j];
+        const double w0 = w[3 * (size_t)p], w1 = w[3 * (size_t)p + 1], w2 = w[3 * (size_t)p + 2];
+        double jc0[9], jc1[9];
+        double u0 = 0.0, u1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            jc0[q] = (double)lin.jcm[(SOA_JC + q) * no + j];
+            jc1[q] = (double)lin.jcm[(SOA_JC + 9 + q) * no + j];
+            u0 += jc0[q] * vc[q];
+            u1 += jc1[q] * vc[q];
         }
+        u0 -= (double)lin.jcm[(SOA_JX + 0) * no + j] * w0 + (double)lin.jcm[(SOA_JX + 1) * no + j] * w1 +
+              (double)lin.jcm[(SOA_JX + 2) * no + j] * w2;
+        u1 -= (double)lin.jcm[(SOA_JX + 3) * no + j] * w0 + (double)lin.jcm[(SOA_JX + 4) * no + j] * w1 +
+              (double)lin.jcm[(SOA_JX + 5) * no + j] * w2;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[q] += jc0[q] * u0 + jc1[q] * u1;
     }
-    sh[t] = acc;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) sh[q][t] = acc[q];
     __syncthreads();
-    if (t < dA) {
-        double coupling = 0.0;
-        for (int c = 0; c < chunks; ++c) coupling += sh[c * dA + t];
-        // base: U_AA v_A plus the pose-intrinsic edge terms (operator.rs:563-594)
-        const uint32_t off = tp.blk_off[A];
-        double base = 0.0;
-        int32_t pr = diag_pair[A];
-        if (pr >= 0) {
-            const double* U = u_vals + value_off[pr];
-            for (int c = 0; c < dA; ++c) base += U[t * dA + c] * v[off + c];
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if ((int)t < s)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) sh[q][t] += sh[q][t + s];
+        __syncthreads();
+    }
+    if (t < 9) P[9 * (size_t)c + t] = sh[t][0];
+}
+
+// Block pass: q_A = (rows of P for block A, summed over the group's cameras in ascending order)
+// + lambda diag_A v_A; one lane per block.
+__global__ void __launch_bounds__(NT) k_pcg_blocks(Topo tp, double lambda, const double* diag, const double* x,
+                                                   const double* d, const double* P, double* q,
+                                                   const PcgState* st) {
+    if (st->done) return;
+    const double* v = st->mode == PCG_NORMAL ? d : x;
+    const uint32_t A = blockIdx.x * NT + threadIdx.x;
+    if (A >= tp.nblocks) return;
+    const uint32_t n = tp.blk_dim[A], off = tp.blk_off[A];
+    if (A < tp.nc) {
+        for (uint32_t k = 0; k < n; ++k) q[off + k] = P[9 * (size_t)A + k] + lambda * diag[off + k] * v[off + k];
+    } else {
+        const uint32_t g = A - tp.nc;
+        for (uint32_t k = 0; k < n; ++k) {
+            double s = 0.0;
+            for (uint32_t i = tp.grp_cam_off[g]; i < tp.grp_cam_off[g + 1]; ++i) s += P[9 * (size_t)tp.grp_cams[i] + 6 + k];
+            q[off + k] = s + lambda * diag[off + k] * v[off + k];
         }
-        if (A < tp.nc) {
-            int32_t ep = edge_pair[A];
-            if (ep >= 0) {  // pair (group block, pose block): values [intr row][pose col]
-                uint32_t g = tp.cam_group[A];
-                const int di = (int)tp.grp_dim[g];
-                const double* E = u_vals + value_off[ep];
-                for (int l = 0; l < di; ++l) base += E[l * dA + t] * v[tp.grp_off[g] + l];
-            }
-        } else {
-            uint32_t g = A - tp.nc;
-            for (uint32_t k = tp.grp_cam_off[g]; k < tp.grp_cam_off[g + 1]; ++k) {
-                uint32_t cam = tp.grp_cams[k];
-                int32_t ep = edge_pair[cam];
-                if (ep < 0) continue;
-                const int dp = (int)tp.cam_dim[cam];
-                const double* E = u_vals + value_off[ep];
-                for (int l = 0; l < dp; ++l) base += E[t * dp + l] * v[tp.cam_off[cam] + l];
-            }
-        }
-        q[off + t] = base - coupling;
     }
 }
 
@@ -1164,17 +1266,26 @@ void launch_pcg_init(hipStream_t s, const Topo& tp, const double* grad, const do
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(1024), 0, s, tp, grad, factors, x, r, z, d, q, st, tol, max_iter);
 }
 template <class T>
-void launch_pcg_points(hipStream_t s, const Topo& tp, const PointElim<T> pe, const double* x, const double* d,
-                       double* w, const PcgState* st) {
-    if (tp.np) hipLaunchKernelGGL(k_pcg_pass1<T>, dim3(grid_for(tp.np, NT)), dim3(NT), 0, s, tp, pe, x, d, w, st);
+void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const double* x,
+                       const double* d, double* w, const PcgState* st) {
+    if (tp.nchunks) hipLaunchKernelGGL(k_pcg_points<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, lin, pe, x, d, w, st);
 }
 template <class T>
-void launch_pcg_blocks(hipStream_t s, const Topo& tp, const PointElim<T> pe, const int32_t* diag_pair,
-                       const int32_t* edge_pair, const uint64_t* value_off, const double* u_vals, const double* x,
-                       const double* d, const double* w, double* q, const PcgState* st) {
+void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const double* x, const double* d,
+                        const double* w, double* P, const PcgState* st) {
+    if (tp.nc) hipLaunchKernelGGL(k_pcg_cameras<T>, dim3(tp.nc), dim3(NT), 0, s, tp, lin, x, d, w, P, st);
+}
+void launch_pcg_blocks(hipStream_t s, const Topo& tp, double lambda, const double* diag, const double* x,
+                       const double* d, const double* P, double* q, const PcgState* st) {
     if (tp.nblocks)
-        hipLaunchKernelGGL(k_pcg_pass2<T>, dim3(tp.nblocks), dim3(NT), 0, s, tp, pe, diag_pair, edge_pair, value_off,
-                           u_vals, x, d, w, q, st);
+        hipLaunchKernelGGL(k_pcg_blocks, dim3(grid_for(tp.nblocks, NT)), dim3(NT), 0, s, tp, lambda, diag, x, d, P, q,
+                           st);
+}
+template <class T>
+void launch_linearize_cm(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
+                         const double* X, const double* uv, const GaugeDev* gauge, Lin<T> out) {
+    if (tp.no == 0) return;
+    hipLaunchKernelGGL(k_linearize_cm<T>, dim3(grid_for(tp.no, NT)), dim3(NT), 0, s, tp, R, t, gp, X, uv, gauge, out);
 }
 void launch_pcg_update(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x, double* r,
                        double* z, double* d, const double* q, PcgState* st) {
@@ -1201,10 +1312,11 @@ template void launch_linearize<double>(hipStream_t, const Topo&, const double*, 
 template void launch_candidate<double>(hipStream_t, const Topo&, const double*, const double*, const double*,
                                        const double*, const double*, const double*, const double*,
                                        const PointElim<double>, const double*, const GaugeDev*, double*, double*);
-template void launch_pcg_points<double>(hipStream_t, const Topo&, const PointElim<double>, const double*,
-                                        const double*, double*, const PcgState*);
-template void launch_pcg_blocks<double>(hipStream_t, const Topo&, const PointElim<double>, const int32_t*,
-                                        const int32_t*, const uint64_t*, const double*, const double*,
+template void launch_pcg_points<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>,
                                         const double*, const double*, double*, const PcgState*);
+template void launch_pcg_cameras<double>(hipStream_t, const Topo&, const Lin<double>, const double*, const double*,
+                                         const double*, double*, const PcgState*);
+template void launch_linearize_cm<double>(hipStream_t, const Topo&, const double*, const double*, const double*,
+                                          const double*, const double*, const GaugeDev*, Lin<double>);
 
 }  // namespace ba

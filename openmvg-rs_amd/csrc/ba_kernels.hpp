@@ -41,9 +41,22 @@ struct Topo {
     const uint32_t* blk_obs;
     const uint32_t* grp_cam_off;  // cameras per group (for intrinsic-block edge terms)
     const uint32_t* grp_cams;
+    // Point chunks of the point-major passes: chunk k holds points chunk_pt[k]..chunk_pt[k+1]
+    // and their observations chunk_obs[k]..chunk_obs[k+1] (at most NT, or one point with more).
+    uint32_t nchunks;
+    const uint32_t* chunk_pt;
+    const uint32_t* chunk_obs;
+    // Camera-major order: positions cm_off[c]..cm_off[c+1] hold camera c's observations in
+    // ascending point-major index; cm_obs gives that index and cm_pt its point.
+    const uint32_t* cm_off;
+    const uint32_t* cm_obs;
+    const uint32_t* cm_pt;
     int loss;
     double loss_scale;
 };
+
+// Rows of the structure-of-arrays Jacobian copies (each row holds one value per observation).
+enum : int { SOA_JX = 0, SOA_JC = 6, SOA_ROWS = 24 };
 
 // Per-observation linearisation (kernel (a)) outputs; T-typed.
 template <class T>
@@ -53,6 +66,8 @@ struct Lin {
     T* cost;    // robust cost per observation
     T* sq;      // squared residual norm per observation
     uint8_t* valid;
+    T* jpm;     // SOA_ROWS x no, point-major: J~_X (2x3 row-major) then J~_c (2x9); or null
+    T* jcm;     // SOA_ROWS x no, camera-major (position j holds observation cm_obs[j]); or null
 };
 
 // Per-point elimination (kernel (b)) outputs.
@@ -136,14 +151,22 @@ void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pa
                            const double* u_vals, const double* s_vals, double* factors, int* fail);
 void launch_pcg_init(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x,
                      double* r, double* z, double* d, double* q, PcgState* st, double tol, int max_iter);
-// Implicit Schur product q = S v (v = d or x by the PCG mode): point pass then block pass.
+// Implicit Schur product q = S v (v = d or x by the PCG mode), S = U + lambda diag - W V^-1 W^T:
+//   point pass   w_p = V^-1 sum_o J~_X,o^T J~_c,o v           (point chunks, point-major J)
+//   camera pass  P_c = sum_{o in c} J~_c,o^T (J~_c,o v - J~_X,o w_p)  (per camera, camera-major J)
+//   block pass   q_A = sum of the P rows of block A + lambda diag_A v_A
 template <class T>
-void launch_pcg_points(hipStream_t s, const Topo& tp, const PointElim<T> pe, const double* x, const double* d,
-                       double* w, const PcgState* st);
+void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const double* x,
+                       const double* d, double* w, const PcgState* st);
 template <class T>
-void launch_pcg_blocks(hipStream_t s, const Topo& tp, const PointElim<T> pe, const int32_t* diag_pair,
-                       const int32_t* edge_pair, const uint64_t* value_off, const double* u_vals,
-                       const double* x, const double* d, const double* w, double* q, const PcgState* st);
+void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const double* x, const double* d,
+                        const double* w, double* P, const PcgState* st);
+void launch_pcg_blocks(hipStream_t s, const Topo& tp, double lambda, const double* diag, const double* x,
+                       const double* d, const double* P, double* q, const PcgState* st);
+// Camera-major linearisation: the SoA camera-major Jacobian copy (lin.jcm).
+template <class T>
+void launch_linearize_cm(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
+                         const double* X, const double* uv, const GaugeDev* gauge, Lin<T> out);
 void launch_pcg_update(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x,
                        double* r, double* z, double* d, const double* q, PcgState* st);
 

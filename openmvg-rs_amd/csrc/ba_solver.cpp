@@ -167,6 +167,7 @@ class Engine {
             for (size_t c = 0; c < nc; ++c) cams[cur[L_.cam_group[c]]++] = (uint32_t)c;
             grp_cam_off_.upload(off, s_);
             grp_cams_.upload(cams, s_);
+            HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
         }
         uv_.upload(L_.uv, s_);
         for (int k = 0; k < 2; ++k) {
@@ -224,6 +225,10 @@ class Engine {
             diag_pair_.upload(dp, s_);
             edge_pair_.upload(ep, s_);
             factors_.alloc(54 * B_.n_blocks);
+            build_matvec_order();
+            jpm_.alloc(SOA_ROWS * no);
+            jcm_.alloc(SOA_ROWS * no);
+            pP_.alloc(9 * nc);
             px_.alloc(D);
             pr_.alloc(D);
             pz_.alloc(D);
@@ -253,12 +258,56 @@ class Engine {
         tp_.blk_obs = blk_obs_.p;
         tp_.grp_cam_off = grp_cam_off_.p;
         tp_.grp_cams = grp_cams_.p;
+        tp_.nchunks = (uint32_t)(chunk_pt_.n ? chunk_pt_.n - 1 : 0);
+        tp_.chunk_pt = chunk_pt_.p;
+        tp_.chunk_obs = chunk_obs_.p;
+        tp_.cm_off = cm_off_.p;
+        tp_.cm_obs = cm_obs_.p;
+        tp_.cm_pt = cm_pt_.p;
         tp_.loss = L_.loss;
         tp_.loss_scale = L_.loss_scale;
         HIP_OK(hipStreamSynchronize(s_));
     }
 
-    Lin<double> lin() { return Lin<double>{rj_.p, jc_.p, ocost_.p, osq_.p, valid_.p}; }
+    // Point chunks (at most NT observations of whole points, or one larger point; at most NT
+    // points) and the camera-major observation order of the matrix-free Schur product.
+    void build_matvec_order() {
+        const size_t np = L_.n_points(), no = L_.n_obs(), nc = L_.n_cameras();
+        const uint32_t cap = 256;
+        std::vector<uint32_t> cpt{0}, cob{0};
+        uint32_t obs_in = 0, pts_in = 0;
+        for (size_t p = 0; p < np; ++p) {
+            const uint32_t k = L_.point_off[p + 1] - L_.point_off[p];
+            if (pts_in > 0 && (obs_in + k > cap || pts_in == cap)) {
+                cpt.push_back((uint32_t)p);
+                cob.push_back(L_.point_off[p]);
+                obs_in = pts_in = 0;
+            }
+            obs_in += k;
+            pts_in += 1;
+        }
+        if (pts_in > 0) {
+            cpt.push_back((uint32_t)np);
+            cob.push_back((uint32_t)no);
+        }
+        chunk_pt_.upload(cpt, s_);
+        chunk_obs_.upload(cob, s_);
+        std::vector<uint32_t> off(nc + 1, 0), obs(no), pts(no);
+        for (size_t o = 0; o < no; ++o) off[L_.obs_cam[o] + 1]++;
+        for (size_t c = 0; c < nc; ++c) off[c + 1] += off[c];
+        std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+        for (size_t o = 0; o < no; ++o) {
+            const uint32_t j = cur[L_.obs_cam[o]]++;
+            obs[j] = (uint32_t)o;
+            pts[j] = L_.obs_point[o];
+        }
+        cm_off_.upload(off, s_);
+        cm_obs_.upload(obs, s_);
+        cm_pt_.upload(pts, s_);
+        HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
+    }
+
+    Lin<double> lin() { return Lin<double>{rj_.p, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p}; }
     PointElim<double> pe() { return PointElim<double>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, pscal_.p}; }
 
     // Use a caller-given gauge state instead of the one derived from the translations.
@@ -278,6 +327,10 @@ class Engine {
             launch_linearize<double>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
                                      lin());
         });
+        if (iterative_ && reduced_system)
+            launch_linearize_cm<double>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
+                                        lin());
+        lambda_ = lambda;
         timed(BA_PROFILE_POINT_ELIM, [&] { launch_point_elim<double>(s_, tp, lin(), lambda, lambda == 0.0, pe()); });
         launch_reduce_points(s_, pscal_.p, tp.np, partials_.p, asm_.p);
         launch_block_gradient<double>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
@@ -337,11 +390,10 @@ class Engine {
             if (h_pcg_.p->done) break;
             for (int k = 0; k < batch; ++k) {
                 timed(BA_PROFILE_PCG_POINTS,
-                      [&] { launch_pcg_points<double>(s_, tp_, pe(), px_.p, pd_.p, pw_.p, pcg_.p); });
-                timed(BA_PROFILE_PCG_BLOCKS, [&] {
-                    launch_pcg_blocks<double>(s_, tp_, pe(), diag_pair_.p, edge_pair_.p, value_off_.p, u_vals_.p,
-                                              px_.p, pd_.p, pw_.p, pq_.p, pcg_.p);
-                });
+                      [&] { launch_pcg_points<double>(s_, tp_, lin(), pe(), px_.p, pd_.p, pw_.p, pcg_.p); });
+                timed(BA_PROFILE_PCG_CAMERAS,
+                      [&] { launch_pcg_cameras<double>(s_, tp_, lin(), px_.p, pd_.p, pw_.p, pP_.p, pcg_.p); });
+                launch_pcg_blocks(s_, tp_, lambda_, diag_.p, px_.p, pd_.p, pP_.p, pq_.p, pcg_.p);
                 launch_pcg_update(s_, tp_, grad_.p, factors_.p, px_.p, pr_.p, pz_.p, pd_.p, pq_.p, pcg_.p);
             }
         }
@@ -412,13 +464,14 @@ class Engine {
     PairLists PL_;
     Topo tp_{};
     Buf<uint32_t> obs_cam_, obs_pt_, pt_off_, cam_group_, cam_off_, cam_dim_, grp_off_, grp_dim_, blk_off_, blk_dim_,
-        blk_obs_off_, blk_obs_, grp_cam_off_, grp_cams_, pair_a_, pair_b_, term_off_, term_i_, term_j_;
+        blk_obs_off_, blk_obs_, grp_cam_off_, grp_cams_, pair_a_, pair_b_, term_off_, term_i_, term_j_, chunk_pt_,
+        chunk_obs_, cm_off_, cm_obs_, cm_pt_;
     Buf<uint8_t> pt_fixed_, valid_;
     Buf<uint64_t> value_off_;
     Buf<int32_t> diag_pair_, edge_pair_;
     Buf<double> uv_, R_[2], t_[2], gp_[2], X_[2], rj_, jc_, ocost_, osq_, vinv_, vdamp_, y_, ddiag_, rrows_, pscal_,
         cscal_, grad_, fgrad_, diag_, step_, rhs_, partials_, u_vals_, s_vals_, S_, work_, factors_, px_, pr_, pz_, pd_,
-        pq_, pw_, dots_;
+        pq_, pw_, dots_, jpm_, jcm_, pP_;
     Buf<AsmScalars> asm_;
     Buf<CandScalars> cand_;
     Buf<GaugeDev> gauge_;
@@ -430,6 +483,7 @@ class Engine {
     Pinned<PcgState> h_pcg_;
     double h_dots_[4];
     bool fixed_gauge_ = false;
+    double lambda_ = 0.0;
     int profile_ = BA_PROFILE_NONE;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
     size_t used_ = 0;

@@ -44,7 +44,7 @@ PROFILE_NONE = 0
 PROFILE_LINEARIZE = 1
 PROFILE_POINT_ELIM = 2
 PROFILE_PCG_POINTS = 3
-PROFILE_PCG_BLOCKS = 4
+PROFILE_PCG_CAMERAS = 4
 PROFILE_PAIRS = 5
 PROFILE_CHOLESKY = 6
 PROFILE_CANDIDATE = 7
