@@ -50,25 +50,27 @@ def lowered_counts(problem) -> dict:
 
 
 def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
-    """Algorithmic HBM bytes of one launch of `kernel` in FP64 (each input read once, each output
-    written once; camera-indexed vectors counted once per launch).
+    """Algorithmic HBM bytes of one launch of `kernel` in FP64: each input read once, each output
+    written once; vectors indexed by camera or point are counted once per launch.
 
-    pcg_points (k_pcg_pass1, per point w_p = V^-1 sum_o R_o x_c): per observation the 27 R
-      entries (216 B) and its camera index (4 B); per point its offset (4 B), fixed flag (1 B),
-      packed V^-1 (48 B) and the written w_p (24 B); the direction vector once (8 D).
-    pcg_blocks (k_pcg_pass2, per camera/intrinsic block q_A = sum_o R_o,A^T w_p): per observation
-      its pose rows and intrinsic rows of R (216 B together), two block-list entries and two
-      point indices (16 B); w once per point (24 B); x/d read and q written (16 D).
-    linearize (k_linearize): per observation pixel (16 B), camera index (4 B), point index (4 B),
-      residual+point Jacobian record (64 B) and camera Jacobian (144 B); per point X (24 B).
+    pcg_points (k_pcg_points: w_p = V^-1 sum_o J~_X,o^T J~_c,o v, one workgroup per point chunk):
+      per observation J~_X (48 B) and J~_c (144 B) from the point-major copy and its camera index
+      (4 B); per point its offset (4 B), fixed flag (1 B), packed V^-1 (48 B) and the written w_p
+      (24 B); the vector v (8 D).
+    pcg_cameras (k_pcg_cameras: P_c = sum_o J~_c^T (J~_c v - J~_X w_p), one workgroup per camera):
+      per observation J~_X and J~_c from the camera-major copy (192 B) and its point index (4 B);
+      w once per point (24 B); P written (72 B per camera); v (8 D).
+    linearize (k_linearize): per observation pixel (16 B), camera and point index (8 B), the
+      structure-of-arrays record r~, J~_X, J~_c (208 B), cost, squared residual and flag (17 B);
+      per point X (24 B); per camera R, t, f, k1, k2 (120 B).
     """
     no, npt, nc = counts["obs"], counts["points"], counts["cameras"]
     if kernel == "pcg_points":
-        return no * (216 + 4) + npt * (4 + 1 + 48 + 24) + 8 * D
-    if kernel == "pcg_blocks":
-        return no * (216 + 16) + npt * 24 + 16 * D
+        return no * (192 + 4) + npt * (4 + 1 + 48 + 24) + 8 * D
+    if kernel == "pcg_cameras":
+        return no * (192 + 4) + npt * 24 + nc * 72 + 8 * D
     if kernel == "linearize":
-        return no * (16 + 4 + 4 + 64 + 144) + npt * 24 + nc * 8 * 15
+        return no * (16 + 8 + 208 + 17) + npt * 24 + nc * 120
     raise ValueError(kernel)
 
 
@@ -147,7 +149,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2, help="untimed LM iterations (a separate solve)")
     ap.add_argument("--points", type=int, default=1_000_000)
     ap.add_argument("--precision", choices=["fp64"], default="fp64")
-    ap.add_argument("--roofline-kernel", default="pcg_points", choices=["pcg_points", "pcg_blocks", "linearize"])
+    ap.add_argument("--roofline-kernel", default="pcg_points", choices=["pcg_points", "pcg_cameras", "linearize"])
     ap.add_argument("--converge-iterations", type=int, default=60,
                     help="iteration cap of the run that fixes the converged cost (0 skips time-to-cost)")
     ap.add_argument("--cpu-points", type=int, default=200_000, help="points of the CPU-baseline subsample")

@@ -78,6 +78,16 @@ __device__ __forceinline__ int row_param(const Topo& tp, uint32_t cam, int k) {
     return l < (int)tp.grp_dim[g] ? (int)(tp.grp_off[g] + l) : -1;
 }
 
+// Values of v on the 9 camera-Jacobian columns of camera c (0 beyond the free dimensions).
+__device__ __forceinline__ void camera_values(const Topo& tp, uint32_t c, const double* v, double vc[9]) {
+    const uint32_t pd = tp.cam_dim[c], po = tp.cam_off[c], g = tp.cam_group[c];
+    const uint32_t gd = tp.grp_dim[g], go = tp.grp_off[g];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) vc[k] = k < (int)pd ? v[po + k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vc[6 + k] = k < (int)gd ? v[go + k] : 0.0;
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------------------------------------
@@ -215,21 +225,25 @@ __global__ void __launch_bounds__(NT) k_linearize(Topo tp, const double* __restr
     out.valid[o] = ok ? 1 : 0;
     out.cost[o] = (T)cost;
     out.sq[o] = (T)sq;
-    T* rj = out.rj + 8 * (size_t)o;
-    T* jo = out.jc + 18 * (size_t)o;
-    rj[0] = (T)r[0];
-    rj[1] = (T)r[1];
+    if (out.rj) {
+        T* rj = out.rj + 8 * (size_t)o;
+        rj[0] = (T)r[0];
+        rj[1] = (T)r[1];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) rj[2 + k] = (T)jx[k];
-#pragma unroll
-    for (int k = 0; k < 18; ++k) jo[k] = (T)jc[k];
-    if (out.jpm) {
-        const size_t no = tp.no;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) out.jpm[(SOA_JX + k) * no + o] = (T)jx[k];
-#pragma unroll
-        for (int k = 0; k < 18; ++k) out.jpm[(SOA_JC + k) * no + o] = (T)jc[k];
+        for (int k = 0; k < 6; ++k) rj[2 + k] = (T)jx[k];
     }
+    if (out.jc) {
+        T* jo = out.jc + 18 * (size_t)o;
+#pragma unroll
+        for (int k = 0; k < 18; ++k) jo[k] = (T)jc[k];
+    }
+    const size_t no = tp.no;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) out.jpm[(SOA_R + k) * no + o] = (T)r[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out.jpm[(SOA_JX + k) * no + o] = (T)jx[k];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) out.jpm[(SOA_JC + k) * no + o] = (T)jc[k];
 }
 
 // Camera-major twin of k_linearize: lane j linearises observation cm_obs[j] and writes the
@@ -243,10 +257,13 @@ __global__ void __launch_bounds__(NT) k_linearize_cm(Topo tp, const double* __re
     if (j >= tp.no) return;
     double r[2], jx[6], jc[18], cost, sq;
     if (!linearize_obs(tp, tp.cm_obs[j], Rg, tg, gp, X, uv, gauge, r, jx, jc, &cost, &sq)) {
+        for (int k = 0; k < 2; ++k) r[k] = 0.0;
         for (int k = 0; k < 6; ++k) jx[k] = 0.0;
         for (int k = 0; k < 18; ++k) jc[k] = 0.0;
     }
     const size_t no = tp.no;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) out.jcm[(SOA_R + k) * no + j] = (T)r[k];
 #pragma unroll
     for (int k = 0; k < 6; ++k) out.jcm[(SOA_JX + k) * no + j] = (T)jx[k];
 #pragma unroll
@@ -254,139 +271,198 @@ __global__ void __launch_bounds__(NT) k_linearize_cm(Topo tp, const double* __re
 }
 
 // ----------------------------------------------------------------------------------------------
-// (b) k_point_elim: one lane per point.
+// (b) k_point_elim: one workgroup per point chunk.  Lanes read one observation each from the
+// point-major structure-of-arrays copy and form its contributions to V = J~_X^T J~_X,
+// g_p = J~_X^T r~, cost and squared residual; one lane per point then sums its observations'
+// contributions in observation order (a segmented reduction through LDS) and eliminates the point
+// (point_block.rs:85-168, schur/assembly.rs:893-1039).  With pe.rrows set, lanes finally write the
+// retained rows R_o = V^-1 W_o, W_o = J~_c,o^T J~_X,o, of the block-pair path.
 // ----------------------------------------------------------------------------------------------
+namespace {
+constexpr int PE_V = 12;  // V (6), g (3), cost, squared residual, invalid observations
+template <class T>
+__device__ __forceinline__ void point_contrib(const Lin<T>& lin, size_t no, uint32_t o, double c[PE_V]) {
+    const bool valid = lin.valid[o] != 0;
+    const double r0 = (double)lin.jpm[(SOA_R + 0) * no + o], r1 = (double)lin.jpm[(SOA_R + 1) * no + o];
+    double a[2][3];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a[r][k] = (double)lin.jpm[(SOA_JX + 3 * r + k) * no + o];
+    c[0] = a[0][0] * a[0][0] + a[1][0] * a[1][0];
+    c[1] = a[0][1] * a[0][0] + a[1][1] * a[1][0];
+    c[2] = a[0][1] * a[0][1] + a[1][1] * a[1][1];
+    c[3] = a[0][2] * a[0][0] + a[1][2] * a[1][0];
+    c[4] = a[0][2] * a[0][1] + a[1][2] * a[1][1];
+    c[5] = a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    c[6] = a[0][0] * r0 + a[1][0] * r1;
+    c[7] = a[0][1] * r0 + a[1][1] * r1;
+    c[8] = a[0][2] * r0 + a[1][2] * r1;
+    c[9] = (double)lin.cost[o];
+    c[10] = (double)lin.sq[o];
+    c[11] = valid ? 0.0 : 1.0;
+}
+}  // namespace
+
 template <class T>
 __global__ void __launch_bounds__(NT) k_point_elim(Topo tp, Lin<T> lin, T lambda, int want_rcond, PointElim<T> pe) {
-    uint32_t p = blockIdx.x * NT + threadIdx.x;
-    if (p >= tp.np) return;
-    const uint32_t b = tp.pt_off[p], e = tp.pt_off[p + 1];
-    T V[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-    T gv[3] = {T(0), T(0), T(0)};
-    double cost = 0.0, sq = 0.0;
-    bool lin_ok = true;
-    for (uint32_t o = b; o < e; ++o) {
-        if (!lin.valid[o]) {
-            lin_ok = false;
-            break;
-        }
-        const T* rj = lin.rj + 8 * (size_t)o;
-        cost += (double)lin.cost[o];
-        sq += (double)lin.sq[o];
+    __shared__ double sh[PE_V][NT];
+    __shared__ double shinv[6][NT];
+    const uint32_t k = blockIdx.x, t = threadIdx.x;
+    const uint32_t ob = tp.chunk_obs[k], oe = tp.chunk_obs[k + 1];
+    const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
+    const size_t no = tp.no;
+    const bool big = oe - ob > NT;
+    double acc[PE_V];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            T wr = rj[r];
-            T a0 = rj[2 + 3 * r], a1 = rj[3 + 3 * r], a2 = rj[4 + 3 * r];
-            gv[0] += a0 * wr;
-            gv[1] += a1 * wr;
-            gv[2] += a2 * wr;
-            V[0] += a0 * a0;
-            V[1] += a1 * a0;
-            V[2] += a1 * a1;
-            V[3] += a2 * a0;
-            V[4] += a2 * a1;
-            V[5] += a2 * a2;
+    for (int q = 0; q < PE_V; ++q) acc[q] = 0.0;
+    for (uint32_t o = ob + t; o < oe; o += NT) {
+        double c[PE_V];
+        point_contrib(lin, no, o, c);
+#pragma unroll
+        for (int q = 0; q < PE_V; ++q) acc[q] += c[q];
+    }
+#pragma unroll
+    for (int q = 0; q < PE_V; ++q) sh[q][t] = acc[q];
+    __syncthreads();
+    if (big) {
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if ((int)t < s)
+#pragma unroll
+                for (int q = 0; q < PE_V; ++q) sh[q][t] += sh[q][t + s];
+            __syncthreads();
         }
     }
-    double* ps = pe.pscal + 6 * (size_t)p;
-    T* yp = pe.y + 3 * (size_t)p;
-    if (!lin_ok) {
-        ps[0] = 0.0;
-        ps[1] = 0.0;
-        ps[2] = 0.0;
-        ps[3] = 0.0;
-        ps[4] = 1.0;
-        ps[5] = 0.0;  // neither linearised nor eliminated
-        yp[0] = yp[1] = yp[2] = T(0);
-        return;
+    double s_[PE_V];
+    bool have_point = t < pend - pb;
+    const uint32_t p = pb + t;
+    if (have_point) {
+#pragma unroll
+        for (int q = 0; q < PE_V; ++q) s_[q] = 0.0;
+        if (big) {
+#pragma unroll
+            for (int q = 0; q < PE_V; ++q) s_[q] = sh[q][0];
+        } else {
+            for (uint32_t o = tp.pt_off[p]; o < tp.pt_off[p + 1]; ++o)
+#pragma unroll
+                for (int q = 0; q < PE_V; ++q) s_[q] += sh[q][o - ob];
+        }
     }
-    ps[0] = cost;
-    ps[1] = sq;
-    if (tp.pt_fixed[p]) {
-        ps[2] = 0.0;
-        ps[3] = 0.0;
-        ps[4] = 1.0;
-        ps[5] = 3.0;  // linearised; nothing to eliminate
-        yp[0] = yp[1] = yp[2] = T(0);
-        return;
-    }
-    // scaled gradient norm of the point block (schur/assembly.rs:1081-1096)
-    double d3[3] = {(double)V[0], (double)V[2], (double)V[5]};
-    double g3[3] = {(double)gv[0], (double)gv[1], (double)gv[2]};
-    double dscale = fmax(fmax(fmax(0.0, d3[0]), d3[1]), d3[2]);
-    double sgin;
-    if (!(isfinite(dscale) && dscale > 0.0)) {
-        sgin = (g3[0] == 0.0 && g3[1] == 0.0 && g3[2] == 0.0) ? 0.0 : INFINITY;
-    } else {
-        double fl = 64.0 * DBL_EPSILON * dscale;
-        sgin = 0.0;
-        for (int k = 0; k < 3; ++k) sgin = fmax(sgin, fabs(g3[k]) / sqrt(fmax(d3[k], fl)));
-    }
-    ps[3] = sgin;
-    // damping diagonal and damped block (point_block.rs:90-110)
-    T tscale = fmax(fmax(fmax(T(0), V[0]), V[2]), V[5]);
-    bool elim = isfinite(tscale) && tscale > T(0);
-    T fl = T(64) * Eps<T>::value * tscale;
-    T dd[3] = {fmax(V[0], fl), fmax(V[2], fl), fmax(V[5], fl)};
-    T Vd[6] = {V[0] + lambda * dd[0], V[1], V[2] + lambda * dd[1], V[3], V[4], V[5] + lambda * dd[2]};
-    T inv[6];
-    elim = elim && point_block_inverse(Vd, inv);
-    if (!elim) {
-        ps[2] = 0.0;
-        ps[4] = 0.0;
-        ps[5] = (lambda == T(0)) ? 5.0 : 1.0;  // linearised, not eliminated, weak at zero damping
-        yp[0] = yp[1] = yp[2] = T(0);
-        return;
-    }
-    double rcond = 1.0;
-    bool weak = false;
-    if (want_rcond) {  // PointBlockFactor::reciprocal_condition_number (point_block.rs:170-183)
-        double m[9] = {(double)V[0], (double)V[1], (double)V[3], (double)V[1], (double)V[2],
-                       (double)V[4], (double)V[3], (double)V[4], (double)V[5]};
-        double iv[9] = {(double)inv[0], (double)inv[1], (double)inv[3], (double)inv[1], (double)inv[2],
-                        (double)inv[4], (double)inv[3], (double)inv[4], (double)inv[5]};
-        double nm = 0.0, ni = 0.0;
-        for (int r = 0; r < 3; ++r) {
-            double s1 = 0.0, s2 = 0.0;
-            for (int c = 0; c < 3; ++c) {
-                s1 += fabs(m[3 * r + c]);
-                s2 += fabs(iv[3 * r + c]);
+    // per-point elimination; shinv[.][t] receives V^-1 of point pb + t (zero when not eliminated)
+    bool eliminated = false;
+    T inv[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (have_point) {
+        double* ps = pe.pscal + 6 * (size_t)p;
+        T* yp = pe.y + 3 * (size_t)p;
+        const T V[6] = {(T)s_[0], (T)s_[1], (T)s_[2], (T)s_[3], (T)s_[4], (T)s_[5]};
+        const T gv[3] = {(T)s_[6], (T)s_[7], (T)s_[8]};
+        if (s_[11] != 0.0) {
+            ps[0] = 0.0;
+            ps[1] = 0.0;
+            ps[2] = 0.0;
+            ps[3] = 0.0;
+            ps[4] = 1.0;
+            ps[5] = 0.0;  // neither linearised nor eliminated
+            yp[0] = yp[1] = yp[2] = T(0);
+        } else {
+            ps[0] = s_[9];
+            ps[1] = s_[10];
+            if (tp.pt_fixed[p]) {
+                ps[2] = 0.0;
+                ps[3] = 0.0;
+                ps[4] = 1.0;
+                ps[5] = 3.0;  // linearised; nothing to eliminate
+                yp[0] = yp[1] = yp[2] = T(0);
+            } else {
+                // scaled gradient norm of the point block (schur/assembly.rs:1081-1096)
+                double d3[3] = {(double)V[0], (double)V[2], (double)V[5]};
+                double g3[3] = {(double)gv[0], (double)gv[1], (double)gv[2]};
+                double dscale = fmax(fmax(fmax(0.0, d3[0]), d3[1]), d3[2]);
+                double sgin;
+                if (!(isfinite(dscale) && dscale > 0.0)) {
+                    sgin = (g3[0] == 0.0 && g3[1] == 0.0 && g3[2] == 0.0) ? 0.0 : INFINITY;
+                } else {
+                    double fl = 64.0 * DBL_EPSILON * dscale;
+                    sgin = 0.0;
+                    for (int q = 0; q < 3; ++q) sgin = fmax(sgin, fabs(g3[q]) / sqrt(fmax(d3[q], fl)));
+                }
+                ps[3] = sgin;
+                // damping diagonal and damped block (point_block.rs:90-110)
+                T tscale = fmax(fmax(fmax(T(0), V[0]), V[2]), V[5]);
+                bool elim = isfinite(tscale) && tscale > T(0);
+                T fl = T(64) * Eps<T>::value * tscale;
+                T dd[3] = {fmax(V[0], fl), fmax(V[2], fl), fmax(V[5], fl)};
+                T Vd[6] = {V[0] + lambda * dd[0], V[1], V[2] + lambda * dd[1], V[3], V[4], V[5] + lambda * dd[2]};
+                elim = elim && point_block_inverse(Vd, inv);
+                if (!elim) {
+                    for (int q = 0; q < 6; ++q) inv[q] = T(0);
+                    ps[2] = 0.0;
+                    ps[4] = 0.0;
+                    ps[5] = (lambda == T(0)) ? 5.0 : 1.0;  // linearised, not eliminated, weak at zero damping
+                    yp[0] = yp[1] = yp[2] = T(0);
+                } else {
+                    eliminated = true;
+                    double rcond = 1.0;
+                    bool weak = false;
+                    if (want_rcond) {  // PointBlockFactor::reciprocal_condition_number (point_block.rs:170-183)
+                        double m[9] = {(double)V[0], (double)V[1], (double)V[3], (double)V[1], (double)V[2],
+                                       (double)V[4], (double)V[3], (double)V[4], (double)V[5]};
+                        double iv[9] = {(double)inv[0], (double)inv[1], (double)inv[3], (double)inv[1], (double)inv[2],
+                                        (double)inv[4], (double)inv[3], (double)inv[4], (double)inv[5]};
+                        double nm = 0.0, ni = 0.0;
+                        for (int r = 0; r < 3; ++r) {
+                            double s1 = 0.0, s2 = 0.0;
+                            for (int c = 0; c < 3; ++c) {
+                                s1 += fabs(m[3 * r + c]);
+                                s2 += fabs(iv[3 * r + c]);
+                            }
+                            nm = fmax(nm, s1);
+                            ni = fmax(ni, s2);
+                        }
+                        double den = nm * ni;
+                        rcond = (isfinite(den) && den > 0.0) ? 1.0 / den : 0.0;
+                        weak = rcond <= 64.0 * DBL_EPSILON * 3.0;
+                    }
+                    T y[3];
+                    sym3_mul(inv, gv, y);
+                    ps[2] = 0.5 * (g3[0] * (double)y[0] + g3[1] * (double)y[1] + g3[2] * (double)y[2]);
+                    ps[4] = rcond;
+                    ps[5] = weak ? 7.0 : 3.0;
+                    for (int q = 0; q < 3; ++q) {
+                        yp[q] = y[q];
+                        pe.ddiag[3 * (size_t)p + q] = dd[q];
+                    }
+                    for (int q = 0; q < 6; ++q) {
+                        pe.vinv[6 * (size_t)p + q] = inv[q];
+                        pe.vdamp[6 * (size_t)p + q] = Vd[q];
+                    }
+                }
             }
-            nm = fmax(nm, s1);
-            ni = fmax(ni, s2);
         }
-        double den = nm * ni;
-        rcond = (isfinite(den) && den > 0.0) ? 1.0 / den : 0.0;
-        weak = rcond <= 64.0 * DBL_EPSILON * 3.0;
     }
-    T y[3];
-    sym3_mul(inv, gv, y);
-    ps[2] = 0.5 * (g3[0] * (double)y[0] + g3[1] * (double)y[1] + g3[2] * (double)y[2]);
-    ps[4] = rcond;
-    ps[5] = weak ? 7.0 : 3.0;
-    for (int k = 0; k < 3; ++k) {
-        yp[k] = y[k];
-        pe.ddiag[3 * (size_t)p + k] = dd[k];
-    }
-    for (int k = 0; k < 6; ++k) {
-        pe.vinv[6 * (size_t)p + k] = inv[k];
-        pe.vdamp[6 * (size_t)p + k] = Vd[k];
-    }
-    // retained rows: W_o = J~_c^T J~_X per observation (9 x 3), R_o = V^-1 W_o
-    for (uint32_t o = b; o < e; ++o) {
-        const T* rj = lin.rj + 8 * (size_t)o;
-        const T* jc = lin.jc + 18 * (size_t)o;
+    if (!pe.rrows) return;  // block-uniform
+    for (int q = 0; q < 6; ++q) shinv[q][t] = eliminated ? (double)inv[q] : 0.0;
+    __syncthreads();
+    // retained rows of the block-pair path: R_o = V^-1 W_o (zero for points not eliminated)
+    for (uint32_t o = ob + t; o < oe; o += NT) {
+        const uint32_t lp = tp.obs_pt[o] - pb;
+        T vi[6];
+        for (int q = 0; q < 6; ++q) vi[q] = (T)shinv[q][lp];
+        T x0[3], x1[3];
+        for (int q = 0; q < 3; ++q) {
+            x0[q] = lin.jpm[(SOA_JX + q) * no + o];
+            x1[q] = lin.jpm[(SOA_JX + 3 + q) * no + o];
+        }
         T* rr = pe.rrows + 27 * (size_t)o;
-        T x0[3] = {rj[2], rj[3], rj[4]}, x1[3] = {rj[5], rj[6], rj[7]};
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            T c0 = jc[k], c1 = jc[9 + k];
+        for (int q = 0; q < 9; ++q) {
+            T c0 = lin.jpm[(SOA_JC + q) * no + o], c1 = lin.jpm[(SOA_JC + 9 + q) * no + o];
             T w[3] = {x0[0] * c0 + x1[0] * c1, x0[1] * c0 + x1[1] * c1, x0[2] * c0 + x1[2] * c1};
             T v[3];
-            sym3_mul(inv, w, v);
-            rr[3 * k] = v[0];
-            rr[3 * k + 1] = v[1];
-            rr[3 * k + 2] = v[2];
+            sym3_mul(vi, w, v);
+            rr[3 * q] = v[0];
+            rr[3 * q + 1] = v[1];
+            rr[3 * q + 2] = v[2];
         }
     }
 }
@@ -478,18 +554,21 @@ __global__ void __launch_bounds__(NT) k_block_gradient(Topo tp, Lin<T> lin, Poin
     if (ch < chunks) {
         uint32_t c0 = lb + (uint32_t)((uint64_t)len * ch / chunks);
         uint32_t c1 = lb + (uint32_t)((uint64_t)len * (ch + 1) / chunks);
+        const size_t no = tp.no;
         for (uint32_t k = c0; k < c1; ++k) {
             uint32_t i = tp.blk_obs[k];
-            const T* rj = lin.rj + 8 * (size_t)i;
-            const T* jc = lin.jc + 18 * (size_t)i;
-            double j0 = (double)jc[rowbase + r], j1 = (double)jc[9 + rowbase + r];
-            full += j0 * (double)rj[0] + j1 * (double)rj[1];
+            double j0 = (double)lin.jpm[(SOA_JC + rowbase + r) * no + i];
+            double j1 = (double)lin.jpm[(SOA_JC + 9 + rowbase + r) * no + i];
+            full += j0 * (double)lin.jpm[(SOA_R + 0) * no + i] + j1 * (double)lin.jpm[(SOA_R + 1) * no + i];
             dg += j0 * j0 + j1 * j1;
             uint32_t p = tp.obs_pt[i];
             if (!tp.pt_fixed[p]) {
                 const T* y = pe.y + 3 * (size_t)p;
-                double wy0 = (double)rj[2] * (double)y[0] + (double)rj[3] * (double)y[1] + (double)rj[4] * (double)y[2];
-                double wy1 = (double)rj[5] * (double)y[0] + (double)rj[6] * (double)y[1] + (double)rj[7] * (double)y[2];
+                double wy0 = 0.0, wy1 = 0.0;
+                for (int q = 0; q < 3; ++q) {
+                    wy0 += (double)lin.jpm[(SOA_JX + q) * no + i] * (double)y[q];
+                    wy1 += (double)lin.jpm[(SOA_JX + 3 + q) * no + i] * (double)y[q];
+                }
                 red += j0 * wy0 + j1 * wy1;
             }
         }
@@ -509,6 +588,141 @@ __global__ void __launch_bounds__(NT) k_block_gradient(Topo tp, Lin<T> lin, Poin
         full_grad[off] = a;
         grad[off] = a - bsum;
         diag[off] = c;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// k_camera_assembly: one workgroup per camera over its camera-major observations (every group has
+// one camera and no camera sees a point twice).  Per observation, with W_o = J~_c^T J~_X (9x3):
+//   full gradient J~_c^T r~, reduced part W_o y_p, diagonal sum of J~_c squares,
+//   U blocks J~_c^T J~_c and Schur blocks W_o V^-1 W_o^T on the pose-pose and intrinsic-intrinsic
+//   blocks (the diagonal blocks of the reduced system, schur/assembly.rs:893-1039).
+// Lane sums run in position order, then a fixed tree over the lanes (CA_V values in 3 rounds).
+// ----------------------------------------------------------------------------------------------
+namespace {
+constexpr int CA_V = 81;  // 9 full, 9 reduced, 9 diagonal, 21 U pose, 6 U intrinsic, 21 S pose, 6 S intr.
+__device__ __forceinline__ int tri(int r, int c) { return r * (r + 1) / 2 + c; }  // r >= c
+}  // namespace
+
+template <class T>
+__global__ void __launch_bounds__(NT) k_camera_assembly(Topo tp, Lin<T> lin, PointElim<T> pe,
+                                                        const int32_t* __restrict__ diag_pair,
+                                                        const uint64_t* __restrict__ value_off, double* grad,
+                                                        double* full_grad, double* diag, double* u_vals,
+                                                        double* s_vals) {
+    __shared__ double sh[27][NT];
+    const uint32_t c = blockIdx.x, t = threadIdx.x;
+    const uint32_t jb = tp.cm_off[c], je = tp.cm_off[c + 1];
+    const size_t no = tp.no;
+    double acc[CA_V];
+#pragma unroll
+    for (int q = 0; q < CA_V; ++q) acc[q] = 0.0;
+    for (uint32_t j = jb + t; j < je; j += NT) {
+        double r[2], jx[2][3], jc[2][9];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            r[a] = (double)lin.jcm[(SOA_R + a) * no + j];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) jx[a][q] = (double)lin.jcm[(SOA_JX + 3 * a + q) * no + j];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) jc[a][q] = (double)lin.jcm[(SOA_JC + 9 * a + q) * no + j];
+        }
+        const uint32_t p = tp.cm_pt[j];
+        const bool free_point = !tp.pt_fixed[p];
+        double W[9][3];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            acc[q] += jc[0][q] * r[0] + jc[1][q] * r[1];
+            acc[18 + q] += jc[0][q] * jc[0][q] + jc[1][q] * jc[1][q];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) W[q][m] = jc[0][q] * jx[0][m] + jc[1][q] * jx[1][m];
+        }
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+#pragma unroll
+            for (int b = 0; b <= a; ++b) acc[27 + tri(a, b)] += jc[0][a] * jc[0][b] + jc[1][a] * jc[1][b];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b <= a; ++b) acc[48 + tri(a, b)] += jc[0][6 + a] * jc[0][6 + b] + jc[1][6 + a] * jc[1][6 + b];
+        if (free_point) {
+            const T* yp = pe.y + 3 * (size_t)p;
+            const T* vp = pe.vinv + 6 * (size_t)p;
+            const double y[3] = {(double)yp[0], (double)yp[1], (double)yp[2]};
+            const double vi[6] = {(double)vp[0], (double)vp[1], (double)vp[2], (double)vp[3], (double)vp[4], (double)vp[5]};
+            double Z[9][3];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                acc[9 + q] += W[q][0] * y[0] + W[q][1] * y[1] + W[q][2] * y[2];
+                sym3_mul(vi, W[q], Z[q]);
+            }
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int b = 0; b <= a; ++b)
+                    acc[54 + tri(a, b)] += Z[a][0] * W[b][0] + Z[a][1] * W[b][1] + Z[a][2] * W[b][2];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b <= a; ++b)
+                    acc[75 + tri(a, b)] += Z[6 + a][0] * W[6 + b][0] + Z[6 + a][1] * W[6 + b][1] + Z[6 + a][2] * W[6 + b][2];
+        }
+    }
+    // fixed tree over the lanes, 27 values per round; results kept in registers of lane 0
+    double tot[CA_V];
+    for (int round = 0; round < 3; ++round) {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) sh[q][t] = acc[27 * round + q];
+        __syncthreads();
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if ((int)t < s)
+#pragma unroll
+                for (int q = 0; q < 27; ++q) sh[q][t] += sh[q][t + s];
+            __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < 27; ++q) tot[27 * round + q] = sh[q][0];
+        __syncthreads();
+    }
+    if (t != 0) return;
+    // pose block (camera c) and intrinsic block (its group)
+    const uint32_t pd = tp.cam_dim[c], po = tp.cam_off[c];
+    for (uint32_t a = 0; a < pd; ++a) {
+        full_grad[po + a] = tot[a];
+        grad[po + a] = tot[a] - tot[9 + a];
+        diag[po + a] = tot[18 + a];
+    }
+    if (pd > 0) {
+        const int32_t pr = diag_pair[c];
+        if (pr >= 0) {
+            double* U = u_vals + value_off[pr];
+            double* S = s_vals + value_off[pr];
+            for (uint32_t a = 0; a < pd; ++a)
+                for (uint32_t b = 0; b < pd; ++b) {
+                    const int ix = a >= b ? tri(a, b) : tri(b, a);
+                    U[a * pd + b] = tot[27 + ix];
+                    S[a * pd + b] = tot[54 + ix];
+                }
+        }
+    }
+    const uint32_t g = tp.cam_group[c], gd = tp.grp_dim[g], go = tp.grp_off[g];
+    for (uint32_t a = 0; a < gd; ++a) {
+        full_grad[go + a] = tot[6 + a];
+        grad[go + a] = tot[6 + a] - tot[15 + a];
+        diag[go + a] = tot[24 + a];
+    }
+    if (gd > 0) {
+        const int32_t pr = diag_pair[tp.nc + g];
+        if (pr >= 0) {
+            double* U = u_vals + value_off[pr];
+            double* S = s_vals + value_off[pr];
+            for (uint32_t a = 0; a < gd; ++a)
+                for (uint32_t b = 0; b < gd; ++b) {
+                    const int ix = a >= b ? tri(a, b) : tri(b, a);
+                    U[a * gd + b] = tot[48 + ix];
+                    S[a * gd + b] = tot[75 + ix];
+                }
+        }
     }
 }
 
@@ -733,62 +947,182 @@ __device__ bool admitted_cost(const Topo& tp, uint32_t b, uint32_t e, const doub
 }
 }  // namespace
 
+// One workgroup per point chunk:
+//   1. lanes (one per observation): e_o = J~_X,o^T J~_c,o dc from the point-major copy;
+//   2. lanes (one per point): dp = -(V^-1 g_p + V^-1 sum_o e_o) (the cached back-substitution of
+//      candidate.rs:358-383 through the Jacobians), the stepped point X + s dp;
+//   3. lanes (one per observation): residual, robust cost and camera-to-point offset at the
+//      stepped point and the candidate cameras;
+//   4. lanes (one per point): admitted_point_cost (schur/point.rs:706-728) in observation order and
+//      the candidate rule (point_geometry.rs:153-168); a point whose stepped position is not
+//      admitted is decided by the sequential hold/reject rules (admitted_cost).
+// A chunk of one point with more than NT observations runs the sequential rules on one lane.
 template <class T>
 __global__ void __launch_bounds__(NT) k_candidate(Topo tp, const double* R, const double* t, const double* Rc,
                                                   const double* tc, const double* gpc, const double* X,
-                                                  const double* uv, PointElim<T> pe, const double* step,
-                                                  const GaugeDev* gauge, double* Xc, double* cscal) {
-    uint32_t p = blockIdx.x * NT + threadIdx.x;
-    if (p >= tp.np) return;
-    const uint32_t b = tp.pt_off[p], e = tp.pt_off[p + 1];
+                                                  const double* uv, Lin<T> lin, PointElim<T> pe,
+                                                  const double* step, const GaugeDev* gauge, double* Xc,
+                                                  double* cscal) {
+    __shared__ double she[3][NT];
+    __shared__ double shx[3][NT];
+    __shared__ double shw[3][NT];
+    __shared__ double shc[NT];
+    __shared__ int shv[NT];
+    const uint32_t k = blockIdx.x, lane = threadIdx.x;
+    const uint32_t ob = tp.chunk_obs[k], oe = tp.chunk_obs[k + 1];
+    const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
+    const size_t no = tp.no;
+    const bool big = oe - ob > NT;
     const double scene = gauge->scene_scale;
-    double prev[3] = {X[3 * (size_t)p], X[3 * (size_t)p + 1], X[3 * (size_t)p + 2]};
-    double* cs = cscal + 4 * (size_t)p;
-    if (tp.pt_fixed[p]) {
-        for (int k = 0; k < 3; ++k) Xc[3 * (size_t)p + k] = prev[k];
-        double c;
-        bool ok = admitted_cost(tp, b, e, Rc, tc, gpc, prev, uv, 0.0, ADMIT_PROJECTION, &c);
-        cs[0] = ok ? c : 0.0;
-        cs[1] = 0.0;
-        cs[2] = 0.0;
-        cs[3] = ok ? 0.0 : 1.0;
-        return;
+    // 1. per-observation back-substitution terms
+    double e[3] = {0.0, 0.0, 0.0};
+    for (uint32_t o = ob + lane; o < oe; o += NT) {
+        double vc[9];
+        camera_values(tp, tp.obs_cam[o], step, vc);
+        double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            t0 += (double)lin.jpm[(SOA_JC + q) * no + o] * vc[q];
+            t1 += (double)lin.jpm[(SOA_JC + 9 + q) * no + o] * vc[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            e[q] += (double)lin.jpm[(SOA_JX + q) * no + o] * t0 + (double)lin.jpm[(SOA_JX + 3 + q) * no + o] * t1;
     }
-    // cached back-substitution (candidate.rs:358-383): dp = -(V^-1 g + sum R dc)
-    double s3[3] = {(double)pe.y[3 * (size_t)p], (double)pe.y[3 * (size_t)p + 1], (double)pe.y[3 * (size_t)p + 2]};
-    for (uint32_t o = b; o < e; ++o) {
-        const uint32_t c = tp.obs_cam[o];
-        const T* rr = pe.rrows + 27 * (size_t)o;
-        for (int k = 0; k < 9; ++k) {
-            int q = row_param(tp, c, k);
-            if (q < 0) continue;
-            double v = step[q];
-            s3[0] += (double)rr[3 * k] * v;
-            s3[1] += (double)rr[3 * k + 1] * v;
-            s3[2] += (double)rr[3 * k + 2] * v;
+    for (int q = 0; q < 3; ++q) she[q][lane] = e[q];
+    __syncthreads();
+    if (big) {
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if ((int)lane < s)
+                for (int q = 0; q < 3; ++q) she[q][lane] += she[q][lane + s];
+            __syncthreads();
         }
     }
-    double dp[3] = {-s3[0], -s3[1], -s3[2]};
-    const T* dd = pe.ddiag + 3 * (size_t)p;
-    double dn = (double)dd[0] * dp[0] * dp[0] + (double)dd[1] * dp[1] * dp[1] + (double)dd[2] * dp[2] * dp[2];
-    double stepped[3] = {prev[0] + scene * dp[0], prev[1] + scene * dp[1], prev[2] + scene * dp[2]};
-    // candidate_rule (point_geometry.rs:153-168)
-    double c;
-    if (admitted_cost(tp, b, e, Rc, tc, gpc, stepped, uv, scene, ADMIT_GEOMETRY, &c)) {
-        for (int k = 0; k < 3; ++k) Xc[3 * (size_t)p + k] = stepped[k];
-        cs[0] = c;
-        cs[1] = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-        cs[2] = dn;
+    // 2. stepped point per point
+    const bool have_point = lane < pend - pb;
+    const uint32_t p = pb + lane;
+    double prev[3] = {0.0, 0.0, 0.0}, stepped[3] = {0.0, 0.0, 0.0}, dp[3] = {0.0, 0.0, 0.0};
+    bool fixed = true;
+    if (have_point) {
+        for (int q = 0; q < 3; ++q) prev[q] = X[3 * (size_t)p + q];
+        fixed = tp.pt_fixed[p] != 0;
+        if (!fixed) {
+            double s3[3] = {0.0, 0.0, 0.0};
+            if (big) {
+                for (int q = 0; q < 3; ++q) s3[q] = she[q][0];
+            } else {
+                for (uint32_t o = tp.pt_off[p]; o < tp.pt_off[p + 1]; ++o)
+                    for (int q = 0; q < 3; ++q) s3[q] += she[q][o - ob];
+            }
+            const T* vi = pe.vinv + 6 * (size_t)p;
+            const double m[6] = {(double)vi[0], (double)vi[1], (double)vi[2], (double)vi[3], (double)vi[4], (double)vi[5]};
+            double ve[3];
+            sym3_mul(m, s3, ve);
+            for (int q = 0; q < 3; ++q) dp[q] = -((double)pe.y[3 * (size_t)p + q] + ve[q]);
+            for (int q = 0; q < 3; ++q) stepped[q] = prev[q] + scene * dp[q];
+        } else {
+            for (int q = 0; q < 3; ++q) stepped[q] = prev[q];
+        }
+    }
+    double* cs = have_point ? cscal + 4 * (size_t)p : nullptr;
+    if (big) {  // one point, sequential rules on lane 0
+        if (!have_point) return;
+        const uint32_t b = tp.pt_off[p], en = tp.pt_off[p + 1];
+        double c;
+        bool ok = admitted_cost(tp, b, en, Rc, tc, gpc, stepped, uv, fixed ? 0.0 : scene,
+                                fixed ? ADMIT_PROJECTION : ADMIT_GEOMETRY, &c);
+        if (ok) {
+            for (int q = 0; q < 3; ++q) Xc[3 * (size_t)p + q] = stepped[q];
+            const T* dd = pe.ddiag + 3 * (size_t)p;
+            cs[0] = c;
+            cs[1] = fixed ? 0.0 : dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+            cs[2] = fixed ? 0.0
+                          : (double)dd[0] * dp[0] * dp[0] + (double)dd[1] * dp[1] * dp[1] + (double)dd[2] * dp[2] * dp[2];
+            cs[3] = 0.0;
+            return;
+        }
+        bool held = false;
+        if (!fixed) {
+            held = admitted_cost(tp, b, en, Rc, tc, gpc, prev, uv, scene, ADMIT_COLLAPSE, &c);
+            if (!held) {
+                double dummy;
+                bool cur_ok = admitted_cost(tp, b, en, R, t, gpc, prev, uv, scene, ADMIT_COLLAPSE, &dummy);
+                held = !cur_ok && admitted_cost(tp, b, en, Rc, tc, gpc, prev, uv, scene, ADMIT_PROJECTION, &c);
+            }
+        }
+        for (int q = 0; q < 3; ++q) Xc[3 * (size_t)p + q] = prev[q];
+        cs[0] = held ? c : 0.0;
+        cs[1] = 0.0;
+        cs[2] = 0.0;
+        cs[3] = held ? 0.0 : 1.0;
+        return;
+    }
+    for (int q = 0; q < 3; ++q) shx[q][lane] = stepped[q];
+    __syncthreads();
+    // 3. residual and offset per observation at the stepped point and the candidate cameras
+    if (ob + lane < oe) {
+        const uint32_t o = ob + lane;
+        const uint32_t lp = tp.obs_pt[o] - pb;
+        const double Xs[3] = {shx[0][lp], shx[1][lp], shx[2][lp]};
+        const uint32_t c = tp.obs_cam[o], g = tp.cam_group[c];
+        const double* Rm = Rc + 9 * (size_t)c;
+        double rot[3];
+        rotate(Rm, Xs, rot);
+        double cp[3] = {rot[0] + tc[3 * c], rot[1] + tc[3 * c + 1], rot[2] + tc[3 * c + 2]};
+        const double f = gpc[3 * g], k1 = gpc[3 * g + 1], k2 = gpc[3 * g + 2];
+        double turn = 0.0;
+        bool has_turn = first_turn(k1, k2, &turn);
+        double res[2];
+        const bool ok = radial_residual<double>(f, k1, k2, turn, has_turn, cp, uv[2 * (size_t)o], uv[2 * (size_t)o + 1], res);
+        double w[3];
+        rotate_t(Rm, cp, w);
+        shv[lane] = ok ? 1 : 0;
+        shc[lane] = ok ? loss_cost<double>(tp.loss, tp.loss_scale, sqrt(res[0] * res[0] + res[1] * res[1])) : 0.0;
+        for (int q = 0; q < 3; ++q) shw[q][lane] = w[q];
+    }
+    __syncthreads();
+    if (!have_point) return;
+    // 4. admission of the stepped point, in observation order
+    const uint32_t b = tp.pt_off[p], en = tp.pt_off[p + 1];
+    TrackGeometry<double> geo(fixed ? 0.0 : scene);
+    double total = 0.0;
+    bool all_ok = true;
+    for (uint32_t o = b; o < en; ++o) {
+        const uint32_t j = o - ob;
+        if (!shv[j]) {
+            all_ok = false;
+            break;
+        }
+        const double w[3] = {shw[0][j], shw[1][j], shw[2][j]};
+        geo.push(w);
+        total += shc[j];
+    }
+    const bool admitted = all_ok && (fixed || !geo.degenerate());
+    if (admitted) {
+        for (int q = 0; q < 3; ++q) Xc[3 * (size_t)p + q] = stepped[q];
+        cs[0] = total;
+        if (fixed) {
+            cs[1] = 0.0;
+            cs[2] = 0.0;
+        } else {
+            const T* dd = pe.ddiag + 3 * (size_t)p;
+            cs[1] = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+            cs[2] = (double)dd[0] * dp[0] * dp[0] + (double)dd[1] * dp[1] * dp[1] + (double)dd[2] * dp[2] * dp[2];
+        }
         cs[3] = 0.0;
         return;
     }
-    bool held = admitted_cost(tp, b, e, Rc, tc, gpc, prev, uv, scene, ADMIT_COLLAPSE, &c);
-    if (!held) {
-        double dummy;
-        bool cur_ok = admitted_cost(tp, b, e, R, t, gpc, prev, uv, scene, ADMIT_COLLAPSE, &dummy);
-        held = !cur_ok && admitted_cost(tp, b, e, Rc, tc, gpc, prev, uv, scene, ADMIT_PROJECTION, &c);
+    bool held = false;
+    double c = 0.0;
+    if (!fixed) {
+        held = admitted_cost(tp, b, en, Rc, tc, gpc, prev, uv, scene, ADMIT_COLLAPSE, &c);
+        if (!held) {
+            double dummy;
+            bool cur_ok = admitted_cost(tp, b, en, R, t, gpc, prev, uv, scene, ADMIT_COLLAPSE, &dummy);
+            held = !cur_ok && admitted_cost(tp, b, en, Rc, tc, gpc, prev, uv, scene, ADMIT_PROJECTION, &c);
+        }
     }
-    for (int k = 0; k < 3; ++k) Xc[3 * (size_t)p + k] = prev[k];
+    for (int q = 0; q < 3; ++q) Xc[3 * (size_t)p + q] = prev[q];
     cs[0] = held ? c : 0.0;
     cs[1] = 0.0;
     cs[2] = 0.0;
@@ -958,7 +1292,7 @@ __global__ void __launch_bounds__(1024) k_pcg_init(Topo tp, const double* grad, 
     s.done = 0;
     s.accepted = 0;
     s.mode = PCG_NORMAL;
-    s.pad = 0;
+    s.products = 0;
     if (!isfinite(n0)) {
         s.done = 1;
     } else if (n0 == 0.0) {
@@ -981,17 +1315,6 @@ __global__ void __launch_bounds__(1024) k_pcg_init(Topo tp, const double* grad, 
 // Matrix-free S v, S = U + lambda diag - W V^-1 W^T with U = sum_o J~_c,o^T J~_c,o and
 // W_o = J~_c,o^T J~_X,o (the retained operator of self_calibration_retained/operator.rs:26-76,
 // evaluated from the Jacobians instead of stored block products).
-namespace {
-// Values of v on the 9 camera-Jacobian columns of camera c (0 beyond the free dimensions).
-__device__ __forceinline__ void camera_values(const Topo& tp, uint32_t c, const double* v, double vc[9]) {
-    const uint32_t pd = tp.cam_dim[c], po = tp.cam_off[c], g = tp.cam_group[c];
-    const uint32_t gd = tp.grp_dim[g], go = tp.grp_off[g];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) vc[k] = k < (int)pd ? v[po + k] : 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) vc[6 + k] = k < (int)gd ? v[go + k] : 0.0;
-}
-}  // namespace
 
 // Point pass: one workgroup per point chunk, one lane per observation; per-observation
 // e_o = J~_X,o^T (J~_c,o v) is summed per point in observation order, then w_p = V^-1 sum e_o.
@@ -1126,6 +1449,7 @@ __global__ void __launch_bounds__(1024) k_pcg_update(Topo tp, const double* grad
     const uint32_t D = tp.D;
     PcgState s = *st;
     if (s.done) return;
+    s.products += 1;
     if (s.mode == PCG_NORMAL) {
         double curv = dot_all(D, d, q, sh);
         double alpha = s.rho / curv;
@@ -1200,8 +1524,8 @@ void launch_linearize(hipStream_t s, const Topo& tp, const double* R, const doub
 }
 template <class T>
 void launch_point_elim(hipStream_t s, const Topo& tp, const Lin<T> lin, T lambda, bool want_rcond, PointElim<T> out) {
-    if (tp.np == 0) return;
-    hipLaunchKernelGGL(k_point_elim<T>, dim3(grid_for(tp.np, NT)), dim3(NT), 0, s, tp, lin, lambda, (int)want_rcond, out);
+    if (tp.nchunks == 0) return;
+    hipLaunchKernelGGL(k_point_elim<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, lin, lambda, (int)want_rcond, out);
 }
 void launch_reduce_points(hipStream_t s, const double* pscal, uint32_t np, double* partials, AsmScalars* out) {
     hipLaunchKernelGGL(k_reduce_points_1, dim3(RED_BLOCKS), dim3(NT), 0, s, pscal, np, partials);
@@ -1212,6 +1536,14 @@ void launch_block_gradient(hipStream_t s, const Topo& tp, const Lin<T> lin, cons
                            double* full_grad, double* diag) {
     if (tp.nblocks == 0) return;
     hipLaunchKernelGGL(k_block_gradient<T>, dim3(tp.nblocks), dim3(NT), 0, s, tp, lin, pe, grad, full_grad, diag);
+}
+template <class T>
+void launch_camera_assembly(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe,
+                            const int32_t* diag_pair, const uint64_t* value_off, double* grad, double* full_grad,
+                            double* diag, double* u_vals, double* s_vals) {
+    if (tp.nc == 0) return;
+    hipLaunchKernelGGL(k_camera_assembly<T>, dim3(tp.nc), dim3(NT), 0, s, tp, lin, pe, diag_pair, value_off, grad,
+                       full_grad, diag, u_vals, s_vals);
 }
 template <class T>
 void launch_pairs(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, uint32_t npairs,
@@ -1241,11 +1573,11 @@ void launch_retract(hipStream_t s, const Topo& tp, const double* R, const double
 }
 template <class T>
 void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* Rc,
-                      const double* tc, const double* gpc, const double* X, const double* uv, const PointElim<T> pe,
-                      const double* step, const GaugeDev* gauge, double* Xc, double* cscal) {
-    if (tp.np == 0) return;
-    hipLaunchKernelGGL(k_candidate<T>, dim3(grid_for(tp.np, NT)), dim3(NT), 0, s, tp, R, t, Rc, tc, gpc, X, uv, pe,
-                       step, gauge, Xc, cscal);
+                      const double* tc, const double* gpc, const double* X, const double* uv, const Lin<T> lin,
+                      const PointElim<T> pe, const double* step, const GaugeDev* gauge, double* Xc, double* cscal) {
+    if (tp.nchunks == 0) return;
+    hipLaunchKernelGGL(k_candidate<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, R, t, Rc, tc, gpc, X, uv, lin, pe, step,
+                       gauge, Xc, cscal);
 }
 void launch_reduce_candidates(hipStream_t s, const double* cscal, uint32_t np, double* partials, CandScalars* out) {
     hipLaunchKernelGGL(k_reduce_cand_1, dim3(RED_BLOCKS), dim3(NT), 0, s, cscal, np, partials);
@@ -1304,13 +1636,16 @@ template void launch_gauge<double>(hipStream_t, const double*, int, double, Gaug
 template void launch_point_elim<double>(hipStream_t, const Topo&, const Lin<double>, double, bool, PointElim<double>);
 template void launch_block_gradient<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>,
                                             double*, double*, double*);
+template void launch_camera_assembly<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>,
+                                             const int32_t*, const uint64_t*, double*, double*, double*, double*,
+                                             double*);
 template void launch_pairs<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>, uint32_t,
                                    const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,
                                    const uint64_t*, double*, double*);
 template void launch_linearize<double>(hipStream_t, const Topo&, const double*, const double*, const double*,
                                        const double*, const double*, const GaugeDev*, Lin<double>);
 template void launch_candidate<double>(hipStream_t, const Topo&, const double*, const double*, const double*,
-                                       const double*, const double*, const double*, const double*,
+                                       const double*, const double*, const double*, const double*, const Lin<double>,
                                        const PointElim<double>, const double*, const GaugeDev*, double*, double*);
 template void launch_pcg_points<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>,
                                         const double*, const double*, double*, const PcgState*);

@@ -55,18 +55,19 @@ struct Topo {
     double loss_scale;
 };
 
-// Rows of the structure-of-arrays Jacobian copies (each row holds one value per observation).
-enum : int { SOA_JX = 0, SOA_JC = 6, SOA_ROWS = 24 };
+// Rows of the structure-of-arrays linearisation copies (each row holds one value per observation):
+// r~ (2), J~_X (2x3 row-major), J~_c (2x9 row-major).
+enum : int { SOA_R = 0, SOA_JX = 2, SOA_JC = 8, SOA_ROWS = 26 };
 
 // Per-observation linearisation (kernel (a)) outputs; T-typed.
 template <class T>
 struct Lin {
-    T* rj;      // 8 per observation: r~ (2), J~_X (2x3 row-major)
-    T* jc;      // 18 per observation: J~_c (2x9), rows [pose 0..5 | intrinsic 6..8]
+    T* rj;      // 8 per observation: r~ (2), J~_X (2x3 row-major); or null (block-pair path only)
+    T* jc;      // 18 per observation: J~_c (2x9), rows [pose 0..5 | intrinsic 6..8]; or null
     T* cost;    // robust cost per observation
     T* sq;      // squared residual norm per observation
     uint8_t* valid;
-    T* jpm;     // SOA_ROWS x no, point-major: J~_X (2x3 row-major) then J~_c (2x9); or null
+    T* jpm;     // SOA_ROWS x no, point-major
     T* jcm;     // SOA_ROWS x no, camera-major (position j holds observation cm_obs[j]); or null
 };
 
@@ -77,7 +78,7 @@ struct PointElim {
     T* vdamp;    // 6 per point: packed damped block V^
     T* y;        // 3 per point: V^-1 g_p
     T* ddiag;    // 3 per point: damping diagonal max(V_ii, floor)
-    T* rrows;    // 27 per observation: R = V^-1 W rows (9 rows x 3)
+    T* rrows;    // 27 per observation: R = V^-1 W rows (9 rows x 3); or null (block-pair path only)
     double* pscal;  // 6 per point: cost, squared residual, point model reduction, scaled gradient
                     // norm, rcond, flags (bit0 linearised, bit1 eliminated, bit2 weak)
 };
@@ -107,6 +108,14 @@ void launch_point_elim(hipStream_t s, const Topo& tp, const Lin<T> lin, T lambda
 // Sum of per-point scalars in a fixed order -> AsmScalars fields cost..invalid.
 void launch_reduce_points(hipStream_t s, const double* pscal, uint32_t np, double* partials,
                           AsmScalars* out);
+// Camera-side assembly when every intrinsic group has one camera and no camera sees a point
+// twice (every Schur block term is then a single observation's): per camera, the pose and
+// intrinsic blocks' full and reduced gradients, unclamped diagonal, U and Schur diagonal blocks
+// (into the diagonal pairs' value space).
+template <class T>
+void launch_camera_assembly(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe,
+                            const int32_t* diag_pair, const uint64_t* value_off, double* grad,
+                            double* full_grad, double* diag, double* u_vals, double* s_vals);
 // Per block: full gradient J~^T r~, reduced gradient (full minus W V^-1 g), unclamped diagonal.
 template <class T>
 void launch_block_gradient(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe,
@@ -132,7 +141,7 @@ void launch_retract(hipStream_t s, const Topo& tp, const double* R, const double
                     const double* step, const GaugeDev* gauge, double* Rc, double* tc, double* gpc);
 template <class T>
 void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* Rc,
-                      const double* tc, const double* gpc, const double* X, const double* uv,
+                      const double* tc, const double* gpc, const double* X, const double* uv, const Lin<T> lin,
                       const PointElim<T> pe, const double* step, const GaugeDev* gauge, double* Xc,
                       double* cscal);
 void launch_reduce_candidates(hipStream_t s, const double* cscal, uint32_t np, double* partials,
@@ -144,7 +153,8 @@ void launch_step_dots(hipStream_t s, uint32_t D, const double* g, const double* 
 // ---- implicit-Schur PCG (kernel (c)) ----
 struct PcgState {
     double rho, n0, rel, tol;
-    int mode, iter, max_iter, done, accepted, pad;
+    int mode, iter, max_iter, done, accepted;
+    int products;  // Schur products applied (each update that ran follows one product)
 };
 // Factor the block-Jacobi preconditioner: U_AA(damped) - Schur_AA per block.
 void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pair, const uint64_t* value_off,

@@ -180,8 +180,10 @@ class Engine {
         HIP_OK(hipMemcpyAsync(t_[0].p, L_.t.data(), 3 * nc * 8, hipMemcpyHostToDevice, s_));
         HIP_OK(hipMemcpyAsync(gp_[0].p, L_.gp.data(), 3 * ng * 8, hipMemcpyHostToDevice, s_));
         HIP_OK(hipMemcpyAsync(X_[0].p, L_.X.data(), 3 * np * 8, hipMemcpyHostToDevice, s_));
-        rj_.alloc(8 * no);
-        jc_.alloc(18 * no);
+        fast_cam_ = iterative_ && simple_topology();
+        build_matvec_order(iterative_);
+        jpm_.alloc(SOA_ROWS * no);
+        if (!fast_cam_) jc_.alloc(18 * no);  // block-pair path
         ocost_.alloc(no);
         osq_.alloc(no);
         valid_.alloc(no);
@@ -189,7 +191,7 @@ class Engine {
         vdamp_.alloc(6 * np);
         y_.alloc(3 * np);
         ddiag_.alloc(3 * np);
-        rrows_.alloc(27 * no);
+        if (!fast_cam_) rrows_.alloc(27 * no);
         pscal_.alloc(6 * np);
         cscal_.alloc(4 * np);
         grad_.alloc(D);
@@ -225,8 +227,6 @@ class Engine {
             diag_pair_.upload(dp, s_);
             edge_pair_.upload(ep, s_);
             factors_.alloc(54 * B_.n_blocks);
-            build_matvec_order();
-            jpm_.alloc(SOA_ROWS * no);
             jcm_.alloc(SOA_ROWS * no);
             pP_.alloc(9 * nc);
             px_.alloc(D);
@@ -271,7 +271,25 @@ class Engine {
 
     // Point chunks (at most NT observations of whole points, or one larger point; at most NT
     // points) and the camera-major observation order of the matrix-free Schur product.
-    void build_matvec_order() {
+    // Every intrinsic group has one camera and no camera observes a point twice: each diagonal
+    // block term of the reduced system is then one observation's (the camera-side assembly).
+    bool simple_topology() const {
+        const size_t nc = L_.n_cameras(), ng = L_.n_groups(), np = L_.n_points();
+        std::vector<uint32_t> per_group(ng, 0);
+        for (size_t c = 0; c < nc; ++c) per_group[L_.cam_group[c]]++;
+        for (uint32_t k : per_group)
+            if (k != 1) return false;
+        std::vector<uint32_t> seen(nc, UINT32_MAX);
+        for (size_t p = 0; p < np; ++p)
+            for (uint32_t o = L_.point_off[p]; o < L_.point_off[p + 1]; ++o) {
+                const uint32_t c = L_.obs_cam[o];
+                if (seen[c] == (uint32_t)p) return false;
+                seen[c] = (uint32_t)p;
+            }
+        return true;
+    }
+
+    void build_matvec_order(bool camera_major) {
         const size_t np = L_.n_points(), no = L_.n_obs(), nc = L_.n_cameras();
         const uint32_t cap = 256;
         std::vector<uint32_t> cpt{0}, cob{0};
@@ -292,6 +310,10 @@ class Engine {
         }
         chunk_pt_.upload(cpt, s_);
         chunk_obs_.upload(cob, s_);
+        if (!camera_major) {
+            HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
+            return;
+        }
         std::vector<uint32_t> off(nc + 1, 0), obs(no), pts(no);
         for (size_t o = 0; o < no; ++o) off[L_.obs_cam[o] + 1]++;
         for (size_t c = 0; c < nc; ++c) off[c + 1] += off[c];
@@ -307,7 +329,7 @@ class Engine {
         HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
     }
 
-    Lin<double> lin() { return Lin<double>{rj_.p, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p}; }
+    Lin<double> lin() { return Lin<double>{nullptr, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p}; }
     PointElim<double> pe() { return PointElim<double>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, pscal_.p}; }
 
     // Use a caller-given gauge state instead of the one derived from the translations.
@@ -327,28 +349,35 @@ class Engine {
             launch_linearize<double>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
                                      lin());
         });
-        if (iterative_ && reduced_system)
+        if (iterative_ && (reduced_system || fast_cam_))
             launch_linearize_cm<double>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
                                         lin());
         lambda_ = lambda;
         timed(BA_PROFILE_POINT_ELIM, [&] { launch_point_elim<double>(s_, tp, lin(), lambda, lambda == 0.0, pe()); });
         launch_reduce_points(s_, pscal_.p, tp.np, partials_.p, asm_.p);
-        launch_block_gradient<double>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
-        if (reduced_system) {
+        if (fast_cam_) {
+            timed(BA_PROFILE_PAIRS, [&] {
+                launch_camera_assembly<double>(s_, tp, lin(), pe(), diag_pair_.p, value_off_.p, grad_.p, fgrad_.p,
+                                               diag_.p, u_vals_.p, s_vals_.p);
+            });
+        } else {
+            launch_block_gradient<double>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
+        }
+        if (reduced_system && !fast_cam_) {
             timed(BA_PROFILE_PAIRS, [&] {
                 launch_pairs<double>(s_, tp, lin(), pe(), (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p,
                                      term_off_.p, term_i_.p, term_j_.p, value_off_.p, u_vals_.p, s_vals_.p);
             });
-            if (!iterative_) {
-                HIP_OK(hipMemsetAsync(S_.p, 0, S_.n * sizeof(double), s_));
-                launch_scatter_dense(s_, tp, (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p, value_off_.p,
-                                     u_vals_.p, s_vals_.p, S_.p, tp.D);
-                launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, S_.p, tp.D, nullptr, value_off_.p,
-                                u_vals_.p, s_vals_.p, asm_.p);
-            } else {
-                launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, nullptr, tp.D, diag_pair_.p, value_off_.p,
-                                u_vals_.p, s_vals_.p, asm_.p);
-            }
+        }
+        if (reduced_system && !iterative_) {  // dense S with lambda diag on its diagonal
+            HIP_OK(hipMemsetAsync(S_.p, 0, S_.n * sizeof(double), s_));
+            launch_scatter_dense(s_, tp, (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p, value_off_.p, u_vals_.p,
+                                 s_vals_.p, S_.p, tp.D);
+            launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, S_.p, tp.D, nullptr, value_off_.p, u_vals_.p,
+                            s_vals_.p, asm_.p);
+        } else if (reduced_system) {  // lambda diag on the diagonal blocks of the preconditioner
+            launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, nullptr, tp.D, diag_pair_.p, value_off_.p,
+                            u_vals_.p, s_vals_.p, asm_.p);
         } else {
             launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, nullptr, tp.D, nullptr, value_off_.p, u_vals_.p,
                             s_vals_.p, asm_.p);
@@ -384,10 +413,17 @@ class Engine {
         HIP_OK(hipStreamSynchronize(s_));
         if (*h_int_.p) return false;
         const int batch = 4;
+        const bool profiling_pcg = profile_ == BA_PROFILE_PCG_POINTS || profile_ == BA_PROFILE_PCG_CAMERAS;
+        size_t events_before = used_;
+        int products_before = 0;
         for (;;) {
             HIP_OK(hipMemcpyAsync(h_pcg_.p, pcg_.p, sizeof(PcgState), hipMemcpyDeviceToHost, s_));
             HIP_OK(hipStreamSynchronize(s_));
+            // launches after the state turned done returned at once: drop their timings
+            if (profiling_pcg) used_ = events_before + (size_t)(h_pcg_.p->products - products_before);
             if (h_pcg_.p->done) break;
+            events_before = used_;
+            products_before = h_pcg_.p->products;
             for (int k = 0; k < batch; ++k) {
                 timed(BA_PROFILE_PCG_POINTS,
                       [&] { launch_pcg_points<double>(s_, tp_, lin(), pe(), px_.p, pd_.p, pw_.p, pcg_.p); });
@@ -416,7 +452,7 @@ class Engine {
         launch_retract(s_, tp, R_[state].p, t_[state].p, gp_[state].p, step_.p, gauge_.p, R_[cs].p, t_[cs].p, gp_[cs].p);
         timed(BA_PROFILE_CANDIDATE, [&] {
             launch_candidate<double>(s_, tp, R_[state].p, t_[state].p, R_[cs].p, t_[cs].p, gp_[cs].p, X_[state].p,
-                                     uv_.p, pe(), step_.p, gauge_.p, X_[cs].p, cscal_.p);
+                                     uv_.p, lin(), pe(), step_.p, gauge_.p, X_[cs].p, cscal_.p);
         });
         launch_reduce_candidates(s_, cscal_.p, tp.np, partials_.p, cand_.p);
         launch_step_dots(s_, tp.D, grad_.p, diag_.p, step_.p, with_product ? pq_.p : nullptr, dots_.p);
@@ -444,8 +480,7 @@ class Engine {
 
     const Topo& topo() const { return tp_; }
     bool iterative() const { return iterative_; }
-    const double* rj() const { return rj_.p; }
-    const double* jc() const { return jc_.p; }
+    const double* jpm() const { return jpm_.p; }
     const double* obs_cost() const { return ocost_.p; }
     const uint8_t* valid() const { return valid_.p; }
     const double* S() const { return S_.p; }
@@ -469,7 +504,7 @@ class Engine {
     Buf<uint8_t> pt_fixed_, valid_;
     Buf<uint64_t> value_off_;
     Buf<int32_t> diag_pair_, edge_pair_;
-    Buf<double> uv_, R_[2], t_[2], gp_[2], X_[2], rj_, jc_, ocost_, osq_, vinv_, vdamp_, y_, ddiag_, rrows_, pscal_,
+    Buf<double> uv_, R_[2], t_[2], gp_[2], X_[2], jc_, ocost_, osq_, vinv_, vdamp_, y_, ddiag_, rrows_, pscal_,
         cscal_, grad_, fgrad_, diag_, step_, rhs_, partials_, u_vals_, s_vals_, S_, work_, factors_, px_, pr_, pz_, pd_,
         pq_, pw_, dots_, jpm_, jcm_, pP_;
     Buf<AsmScalars> asm_;
@@ -484,6 +519,7 @@ class Engine {
     double h_dots_[4];
     bool fixed_gauge_ = false;
     double lambda_ = 0.0;
+    bool fast_cam_ = false;
     int profile_ = BA_PROFILE_NONE;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
     size_t used_ = 0;
@@ -927,9 +963,15 @@ int ba_hip_linearize(ba_hip_handle* h, const ba_device_problem* dp, double* resi
     const size_t no = L.n_obs();
     hipStream_t s = h->stream;
     if (no) {
-        HIP_OK(hipMemcpy2DAsync(residual, 16, E.rj(), 64, 16, no, hipMemcpyDeviceToDevice, s));
-        HIP_OK(hipMemcpy2DAsync(point_jacobian, 48, E.rj() + 2, 64, 48, no, hipMemcpyDeviceToDevice, s));
-        HIP_OK(hipMemcpyAsync(camera_jacobian, E.jc(), 18 * no * 8, hipMemcpyDeviceToDevice, s));
+        // structure-of-arrays rows -> per-observation records
+        auto rows = [&](double* dst, int first, int count) {
+            for (int k = 0; k < count; ++k)
+                HIP_OK(hipMemcpy2DAsync(dst + k, count * sizeof(double), E.jpm() + (size_t)(first + k) * no,
+                                        sizeof(double), sizeof(double), no, hipMemcpyDeviceToDevice, s));
+        };
+        rows(residual, SOA_R, 2);
+        rows(point_jacobian, SOA_JX, 6);
+        rows(camera_jacobian, SOA_JC, 18);
         HIP_OK(hipMemcpyAsync(cost, E.obs_cost(), no * 8, hipMemcpyDeviceToDevice, s));
         HIP_OK(hipMemcpyAsync(valid, E.valid(), no, hipMemcpyDeviceToDevice, s));
     }
