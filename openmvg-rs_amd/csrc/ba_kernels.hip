@@ -1248,51 +1248,51 @@ __device__ void precond_apply_block(const double* F, int n, const double* right,
 }
 enum { PCG_NORMAL = 0, PCG_CONFIRM = 1, PCG_FINAL = 2 };
 
-// z = M^-1 r over all blocks, then rho = r.z (single workgroup)
-__device__ double apply_precond_dot(const Topo& tp, const double* factors, const double* r, double* z, double* sh) {
-    double acc = 0.0;
-    for (uint32_t A = threadIdx.x; A < tp.nblocks; A += 1024) {
-        const int n = (int)tp.blk_dim[A];
-        if (n == 0) continue;
-        const uint32_t off = tp.blk_off[A];
-        double sol[9];
-        precond_apply_block(factors + FACT * (size_t)A, n, r + off, sol);
-        for (int k = 0; k < n; ++k) z[off + k] = sol[k];
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < tp.D; i += 1024) acc += r[i] * z[i];
-    return tree_sum<1024>(acc, sh);
-}
 __device__ double dot_all(uint32_t D, const double* a, const double* b, double* sh) {
     double acc = 0.0;
     for (uint32_t i = threadIdx.x; i < D; i += 1024) acc += a[i] * b[i];
     return tree_sum<1024>(acc, sh);
 }
+// Per-camera copy of the next product's input vector (d in NORMAL mode, else x).
+__device__ void write_vcam(const Topo& tp, const PcgState& s, const PcgVecs& V) {
+    const double* v = s.mode == PCG_NORMAL ? V.d : V.x;
+    for (uint32_t c = threadIdx.x; c < tp.nc; c += 1024) {
+        double vc[9];
+        camera_values(tp, c, v, vc);
+        for (int k = 0; k < 9; ++k) V.vcam[9 * (size_t)c + k] = vc[k];
+    }
+}
 }  // namespace
 
-__global__ void __launch_bounds__(1024) k_pcg_init(Topo tp, const double* grad, const double* factors, double* x,
-                                                   double* r, double* z, double* d, double* q, PcgState* st,
-                                                   double tol, int max_iter) {
+// PCG control (pcg.rs:23-119 with the retained confirmation of self_calibration_retained.rs:
+// 512-579), split so that the block-Jacobi application runs over many workgroups:
+//   k_pcg_init_a / k_pcg_update_a (one workgroup): the step along d or the true-residual check;
+//   k_pcg_precond (one lane per block): z = M^-1 r and per-workgroup partials of r.z;
+//   k_pcg_update_b (one workgroup): the new direction and the per-camera input of the next product.
+__global__ void __launch_bounds__(1024) k_pcg_init_a(Topo tp, PcgVecs V, double tol, int max_iter) {
     __shared__ double sh[1024];
     const uint32_t D = tp.D;
     for (uint32_t i = threadIdx.x; i < D; i += 1024) {
-        x[i] = 0.0;
-        q[i] = 0.0;
-        r[i] = -grad[i];
+        V.x[i] = 0.0;
+        V.q[i] = 0.0;
+        V.r[i] = -V.grad[i];
     }
     __syncthreads();
-    double n0 = sqrt(dot_all(D, r, r, sh));
+    double n0 = sqrt(dot_all(D, V.r, V.r, sh));
     PcgState s;
     s.tol = tol;
     s.max_iter = max_iter;
     s.iter = 0;
     s.n0 = n0;
     s.rel = NAN;
+    s.tr = NAN;
     s.rho = 0.0;
     s.done = 0;
     s.accepted = 0;
     s.mode = PCG_NORMAL;
     s.products = 0;
+    s.need_z = 0;
+    s.restart = 0;
     if (!isfinite(n0)) {
         s.done = 1;
     } else if (n0 == 0.0) {
@@ -1300,16 +1300,119 @@ __global__ void __launch_bounds__(1024) k_pcg_init(Topo tp, const double* grad, 
         s.accepted = 1;
         s.rel = 0.0;
     } else {
-        double rho = apply_precond_dot(tp, factors, r, z, sh);
-        for (uint32_t i = threadIdx.x; i < D; i += 1024) d[i] = z[i];
-        if (!(isfinite(rho) && rho > 0.0)) {
+        s.need_z = 1;  // z = M^-1 r, d = z, rho = r.z (restart form)
+        s.restart = 1;
+    }
+    if (threadIdx.x == 0) *V.st = s;
+}
+
+__global__ void __launch_bounds__(1024) k_pcg_update_a(Topo tp, PcgVecs V) {
+    __shared__ double sh[1024];
+    const uint32_t D = tp.D;
+    PcgState s = *V.st;
+    if (s.done) return;
+    s.products += 1;
+    if (s.mode == PCG_NORMAL) {
+        double curv = dot_all(D, V.d, V.q, sh);
+        double alpha = s.rho / curv;
+        if (!(isfinite(curv) && curv > 0.0) || !isfinite(alpha)) {
             s.done = 1;
         } else {
-            s.rho = rho;
-            s.mode = max_iter == 0 ? PCG_FINAL : PCG_NORMAL;
+            s.iter += 1;
+            for (uint32_t i = threadIdx.x; i < D; i += 1024) {
+                V.x[i] += alpha * V.d[i];
+                V.r[i] -= alpha * V.q[i];
+            }
+            __syncthreads();
+            double ratio = sqrt(dot_all(D, V.r, V.r, sh)) / s.n0;
+            if (!isfinite(ratio)) {
+                s.done = 1;
+            } else if (ratio <= s.tol) {
+                s.mode = PCG_CONFIRM;
+            } else {
+                s.need_z = 1;
+                s.restart = 0;
+            }
+        }
+    } else {
+        // q = S x: the true residual
+        for (uint32_t i = threadIdx.x; i < D; i += 1024) V.r[i] = -V.grad[i] - V.q[i];
+        __syncthreads();
+        double tr = sqrt(dot_all(D, V.r, V.r, sh)) / s.n0;
+        if (s.mode == PCG_FINAL) {
+            s.done = 1;
+            s.rel = isfinite(tr) ? tr : NAN;
+        } else if (isfinite(tr) && tr <= s.tol) {
+            s.done = 1;
+            s.accepted = 1;
+            s.rel = tr;
+        } else if (!isfinite(tr)) {
+            s.done = 1;
+        } else {
+            s.need_z = 1;
+            s.restart = 1;
+            s.tr = tr;
         }
     }
-    if (threadIdx.x == 0) *st = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *V.st = s;
+}
+
+__global__ void __launch_bounds__(NT) k_pcg_precond(Topo tp, PcgVecs V) {
+    __shared__ double sh[NT];
+    if (V.st->done || !V.st->need_z) return;
+    const uint32_t A = blockIdx.x * NT + threadIdx.x;
+    double acc = 0.0;
+    if (A < tp.nblocks) {
+        const int n = (int)tp.blk_dim[A];
+        if (n > 0) {
+            const uint32_t off = tp.blk_off[A];
+            double sol[9];
+            precond_apply_block(V.factors + FACT * (size_t)A, n, V.r + off, sol);
+            for (int k = 0; k < n; ++k) {
+                V.z[off + k] = sol[k];
+                acc += V.r[off + k] * sol[k];
+            }
+        }
+    }
+    const double v = tree_sum<NT>(acc, sh);
+    if (threadIdx.x == 0) V.part[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(1024) k_pcg_update_b(Topo tp, PcgVecs V, uint32_t nparts) {
+    __shared__ double sh[1024];
+    const uint32_t D = tp.D;
+    PcgState s = *V.st;
+    if (s.done) return;
+    if (s.need_z) {
+        double acc = 0.0;
+        for (uint32_t g = threadIdx.x; g < nparts; g += 1024) acc += V.part[g];
+        const double next = tree_sum<1024>(acc, sh);
+        if (!s.restart) {
+            const double beta = next / s.rho;
+            if (!(isfinite(next) && next > 0.0) || !isfinite(beta)) {
+                s.done = 1;
+            } else {
+                for (uint32_t i = threadIdx.x; i < D; i += 1024) V.d[i] = V.z[i] + beta * V.d[i];
+                s.rho = next;
+                if (s.iter >= s.max_iter) s.mode = PCG_FINAL;
+            }
+        } else {
+            for (uint32_t i = threadIdx.x; i < D; i += 1024) V.d[i] = V.z[i];
+            if (!(isfinite(next) && next > 0.0)) {
+                s.done = 1;
+                s.rel = s.tr;
+            } else {
+                s.rho = next;
+                s.mode = s.iter >= s.max_iter ? PCG_FINAL : PCG_NORMAL;
+            }
+        }
+        s.need_z = 0;
+        s.restart = 0;
+    }
+    __syncthreads();
+    if (!s.done) write_vcam(tp, s, V);
+    if (threadIdx.x == 0) *V.st = s;
 }
 
 // Matrix-free S v, S = U + lambda diag - W V^-1 W^T with U = sum_o J~_c,o^T J~_c,o and
@@ -1319,11 +1422,10 @@ __global__ void __launch_bounds__(1024) k_pcg_init(Topo tp, const double* grad, 
 // Point pass: one workgroup per point chunk, one lane per observation; per-observation
 // e_o = J~_X,o^T (J~_c,o v) is summed per point in observation order, then w_p = V^-1 sum e_o.
 template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointElim<T> pe, const double* x,
-                                                   const double* d, double* w, const PcgState* st) {
+__global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointElim<T> pe, const double* vcam, double* w,
+                                                   const PcgState* st) {
     __shared__ double sh[3][NT];
     if (st->done) return;
-    const double* v = st->mode == PCG_NORMAL ? d : x;
     const uint32_t k = blockIdx.x, t = threadIdx.x;
     const uint32_t ob = tp.chunk_obs[k], oe = tp.chunk_obs[k + 1];
     const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
@@ -1331,8 +1433,7 @@ __global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointEli
     double acc[3] = {0.0, 0.0, 0.0};
     // lanes take observations ob + t, ob + t + NT, ... (one pass unless one point has > NT)
     for (uint32_t o = ob + t; o < oe; o += NT) {
-        double vc[9];
-        camera_values(tp, tp.obs_cam[o], v, vc);
+        const double* vc = vcam + 9 * (size_t)tp.obs_cam[o];
         double t0 = 0.0, t1 = 0.0;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
@@ -1377,16 +1478,15 @@ __global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointEli
 // Camera pass: one workgroup per camera over its camera-major observations;
 // P_c = sum_o J~_c,o^T (J~_c,o v - J~_X,o w_p), a fixed-shape tree over the lanes.
 template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_cameras(Topo tp, Lin<T> lin, const double* x, const double* d,
-                                                    const double* w, double* P, const PcgState* st) {
+__global__ void __launch_bounds__(NT) k_pcg_cameras(Topo tp, Lin<T> lin, const double* vcam, const double* w, double* P,
+                                                    const PcgState* st) {
     __shared__ double sh[9][NT];
     if (st->done) return;
-    const double* v = st->mode == PCG_NORMAL ? d : x;
     const uint32_t c = blockIdx.x, t = threadIdx.x;
     const uint32_t jb = tp.cm_off[c], je = tp.cm_off[c + 1];
     const size_t no = tp.no;
     double vc[9];
-    camera_values(tp, c, v, vc);
+    for (int q = 0; q < 9; ++q) vc[q] = vcam[9 * (size_t)c + q];
     double acc[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) acc[q] = 0.0;
@@ -1441,72 +1541,6 @@ __global__ void __launch_bounds__(NT) k_pcg_blocks(Topo tp, double lambda, const
             q[off + k] = s + lambda * diag[off + k] * v[off + k];
         }
     }
-}
-
-__global__ void __launch_bounds__(1024) k_pcg_update(Topo tp, const double* grad, const double* factors, double* x,
-                                                     double* r, double* z, double* d, const double* q, PcgState* st) {
-    __shared__ double sh[1024];
-    const uint32_t D = tp.D;
-    PcgState s = *st;
-    if (s.done) return;
-    s.products += 1;
-    if (s.mode == PCG_NORMAL) {
-        double curv = dot_all(D, d, q, sh);
-        double alpha = s.rho / curv;
-        if (!(isfinite(curv) && curv > 0.0) || !isfinite(alpha)) {
-            s.done = 1;
-        } else {
-            s.iter += 1;
-            for (uint32_t i = threadIdx.x; i < D; i += 1024) {
-                x[i] += alpha * d[i];
-                r[i] -= alpha * q[i];
-            }
-            __syncthreads();
-            double ratio = sqrt(dot_all(D, r, r, sh)) / s.n0;
-            if (!isfinite(ratio)) {
-                s.done = 1;
-            } else if (ratio <= s.tol) {
-                s.mode = PCG_CONFIRM;
-            } else {
-                double next = apply_precond_dot(tp, factors, r, z, sh);
-                double beta = next / s.rho;
-                if (!(isfinite(next) && next > 0.0) || !isfinite(beta)) {
-                    s.done = 1;
-                } else {
-                    for (uint32_t i = threadIdx.x; i < D; i += 1024) d[i] = z[i] + beta * d[i];
-                    s.rho = next;
-                    if (s.iter >= s.max_iter) s.mode = PCG_FINAL;
-                }
-            }
-        }
-    } else {
-        // q = S x: the true residual
-        for (uint32_t i = threadIdx.x; i < D; i += 1024) r[i] = -grad[i] - q[i];
-        __syncthreads();
-        double tr = sqrt(dot_all(D, r, r, sh)) / s.n0;
-        if (s.mode == PCG_FINAL) {
-            s.done = 1;
-            s.rel = isfinite(tr) ? tr : NAN;
-        } else if (isfinite(tr) && tr <= s.tol) {
-            s.done = 1;
-            s.accepted = 1;
-            s.rel = tr;
-        } else if (!isfinite(tr)) {
-            s.done = 1;
-        } else {
-            double rho = apply_precond_dot(tp, factors, r, z, sh);
-            for (uint32_t i = threadIdx.x; i < D; i += 1024) d[i] = z[i];
-            if (!(isfinite(rho) && rho > 0.0)) {
-                s.done = 1;
-                s.rel = tr;
-            } else {
-                s.rho = rho;
-                s.mode = s.iter >= s.max_iter ? PCG_FINAL : PCG_NORMAL;
-            }
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) *st = s;
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1593,19 +1627,28 @@ void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pa
     hipLaunchKernelGGL(k_precond_factor, dim3(grid_for(tp.nblocks, NT)), dim3(NT), 0, s, tp, diag_pair, value_off,
                        u_vals, s_vals, factors, fail);
 }
-void launch_pcg_init(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x, double* r,
-                     double* z, double* d, double* q, PcgState* st, double tol, int max_iter) {
-    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(1024), 0, s, tp, grad, factors, x, r, z, d, q, st, tol, max_iter);
+uint32_t pcg_partials(const Topo& tp) { return grid_for(tp.nblocks, NT); }
+void launch_pcg_init(hipStream_t s, const Topo& tp, const PcgVecs& V, double tol, int max_iter) {
+    hipLaunchKernelGGL(k_pcg_init_a, dim3(1), dim3(1024), 0, s, tp, V, tol, max_iter);
+    if (tp.nblocks) hipLaunchKernelGGL(k_pcg_precond, dim3(pcg_partials(tp)), dim3(NT), 0, s, tp, V);
+    hipLaunchKernelGGL(k_pcg_update_b, dim3(1), dim3(1024), 0, s, tp, V, pcg_partials(tp));
+}
+void launch_pcg_update(hipStream_t s, const Topo& tp, const PcgVecs& V) {
+    hipLaunchKernelGGL(k_pcg_update_a, dim3(1), dim3(1024), 0, s, tp, V);
+    if (tp.nblocks) hipLaunchKernelGGL(k_pcg_precond, dim3(pcg_partials(tp)), dim3(NT), 0, s, tp, V);
+    hipLaunchKernelGGL(k_pcg_update_b, dim3(1), dim3(1024), 0, s, tp, V, pcg_partials(tp));
 }
 template <class T>
-void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const double* x,
-                       const double* d, double* w, const PcgState* st) {
-    if (tp.nchunks) hipLaunchKernelGGL(k_pcg_points<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, lin, pe, x, d, w, st);
+void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const PcgVecs& V) {
+    if (tp.nchunks)
+        hipLaunchKernelGGL(k_pcg_points<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, lin, pe, (const double*)V.vcam, V.w,
+                           (const PcgState*)V.st);
 }
 template <class T>
-void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const double* x, const double* d,
-                        const double* w, double* P, const PcgState* st) {
-    if (tp.nc) hipLaunchKernelGGL(k_pcg_cameras<T>, dim3(tp.nc), dim3(NT), 0, s, tp, lin, x, d, w, P, st);
+void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const PcgVecs& V) {
+    if (tp.nc)
+        hipLaunchKernelGGL(k_pcg_cameras<T>, dim3(tp.nc), dim3(NT), 0, s, tp, lin, (const double*)V.vcam,
+                           (const double*)V.w, V.P, (const PcgState*)V.st);
 }
 void launch_pcg_blocks(hipStream_t s, const Topo& tp, double lambda, const double* diag, const double* x,
                        const double* d, const double* P, double* q, const PcgState* st) {
@@ -1618,10 +1661,6 @@ void launch_linearize_cm(hipStream_t s, const Topo& tp, const double* R, const d
                          const double* X, const double* uv, const GaugeDev* gauge, Lin<T> out) {
     if (tp.no == 0) return;
     hipLaunchKernelGGL(k_linearize_cm<T>, dim3(grid_for(tp.no, NT)), dim3(NT), 0, s, tp, R, t, gp, X, uv, gauge, out);
-}
-void launch_pcg_update(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x, double* r,
-                       double* z, double* d, const double* q, PcgState* st) {
-    hipLaunchKernelGGL(k_pcg_update, dim3(1), dim3(1024), 0, s, tp, grad, factors, x, r, z, d, q, st);
 }
 
 __global__ void k_negate(uint32_t n, const double* in, double* out) {
@@ -1648,9 +1687,8 @@ template void launch_candidate<double>(hipStream_t, const Topo&, const double*, 
                                        const double*, const double*, const double*, const double*, const Lin<double>,
                                        const PointElim<double>, const double*, const GaugeDev*, double*, double*);
 template void launch_pcg_points<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>,
-                                        const double*, const double*, double*, const PcgState*);
-template void launch_pcg_cameras<double>(hipStream_t, const Topo&, const Lin<double>, const double*, const double*,
-                                         const double*, double*, const PcgState*);
+                                        const PcgVecs&);
+template void launch_pcg_cameras<double>(hipStream_t, const Topo&, const Lin<double>, const PcgVecs&);
 template void launch_linearize_cm<double>(hipStream_t, const Topo&, const double*, const double*, const double*,
                                           const double*, const double*, const GaugeDev*, Lin<double>);
 

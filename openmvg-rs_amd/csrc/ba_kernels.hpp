@@ -153,32 +153,44 @@ void launch_step_dots(hipStream_t s, uint32_t D, const double* g, const double* 
 // ---- implicit-Schur PCG (kernel (c)) ----
 struct PcgState {
     double rho, n0, rel, tol;
+    double tr;     // true residual ratio of a failed confirmation (restart)
     int mode, iter, max_iter, done, accepted;
     int products;  // Schur products applied (each update that ran follows one product)
+    int need_z;    // the preconditioner pass runs
+    int restart;   // ... and the direction restarts from z
 };
+// Vectors of one PCG solve (device).
+struct PcgVecs {
+    const double* grad;     // D, reduced gradient (the right-hand side is -grad)
+    const double* factors;  // 54 per block, block-Jacobi factors
+    double *x, *r, *z, *d, *q;  // D each
+    double* w;     // 3 per point, point pass output
+    double* P;     // 9 per camera, camera pass output
+    double* vcam;  // 9 per camera, the product's input on each camera's Jacobian columns
+    double* part;  // one per preconditioner workgroup
+    PcgState* st;
+};
+uint32_t pcg_partials(const Topo& tp);
 // Factor the block-Jacobi preconditioner: U_AA(damped) - Schur_AA per block.
 void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pair, const uint64_t* value_off,
                            const double* u_vals, const double* s_vals, double* factors, int* fail);
-void launch_pcg_init(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x,
-                     double* r, double* z, double* d, double* q, PcgState* st, double tol, int max_iter);
+void launch_pcg_init(hipStream_t s, const Topo& tp, const PcgVecs& V, double tol, int max_iter);
 // Implicit Schur product q = S v (v = d or x by the PCG mode), S = U + lambda diag - W V^-1 W^T:
 //   point pass   w_p = V^-1 sum_o J~_X,o^T J~_c,o v           (point chunks, point-major J)
 //   camera pass  P_c = sum_{o in c} J~_c,o^T (J~_c,o v - J~_X,o w_p)  (per camera, camera-major J)
 //   block pass   q_A = sum of the P rows of block A + lambda diag_A v_A
 template <class T>
-void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const double* x,
-                       const double* d, double* w, const PcgState* st);
+void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const PcgVecs& V);
 template <class T>
-void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const double* x, const double* d,
-                        const double* w, double* P, const PcgState* st);
+void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const PcgVecs& V);
 void launch_pcg_blocks(hipStream_t s, const Topo& tp, double lambda, const double* diag, const double* x,
                        const double* d, const double* P, double* q, const PcgState* st);
 // Camera-major linearisation: the SoA camera-major Jacobian copy (lin.jcm).
 template <class T>
 void launch_linearize_cm(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
                          const double* X, const double* uv, const GaugeDev* gauge, Lin<T> out);
-void launch_pcg_update(hipStream_t s, const Topo& tp, const double* grad, const double* factors, double* x,
-                       double* r, double* z, double* d, const double* q, PcgState* st);
+// The PCG step after a product (update, preconditioner pass, new direction).
+void launch_pcg_update(hipStream_t s, const Topo& tp, const PcgVecs& V);
 
 // ---- dense Cholesky (kernel (d)) ----
 // Factor the lower triangle of the n x n row-major matrix A in place (MFMA FP64 trailing

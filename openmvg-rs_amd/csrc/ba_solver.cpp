@@ -229,6 +229,8 @@ class Engine {
             factors_.alloc(54 * B_.n_blocks);
             jcm_.alloc(SOA_ROWS * no);
             pP_.alloc(9 * nc);
+            pvcam_.alloc(9 * nc);
+            ppart_.alloc(std::max<size_t>(1, (B_.n_blocks + 255) / 256));
             px_.alloc(D);
             pr_.alloc(D);
             pz_.alloc(D);
@@ -329,6 +331,23 @@ class Engine {
         HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
     }
 
+    PcgVecs pcg_vecs() {
+        PcgVecs V;
+        V.grad = grad_.p;
+        V.factors = factors_.p;
+        V.x = px_.p;
+        V.r = pr_.p;
+        V.z = pz_.p;
+        V.d = pd_.p;
+        V.q = pq_.p;
+        V.w = pw_.p;
+        V.P = pP_.p;
+        V.vcam = pvcam_.p;
+        V.part = ppart_.p;
+        V.st = pcg_.p;
+        return V;
+    }
+
     Lin<double> lin() { return Lin<double>{nullptr, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p}; }
     PointElim<double> pe() { return PointElim<double>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, pscal_.p}; }
 
@@ -409,7 +428,8 @@ class Engine {
         HIP_OK(hipMemsetAsync(fail_.p, 0, sizeof(int), s_));
         launch_precond_factor(s_, tp_, diag_pair_.p, value_off_.p, u_vals_.p, s_vals_.p, factors_.p, fail_.p);
         HIP_OK(hipMemcpyAsync(h_int_.p, fail_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
-        launch_pcg_init(s_, tp_, grad_.p, factors_.p, px_.p, pr_.p, pz_.p, pd_.p, pq_.p, pcg_.p, tol, limit);
+        const PcgVecs V = pcg_vecs();
+        launch_pcg_init(s_, tp_, V, tol, limit);
         HIP_OK(hipStreamSynchronize(s_));
         if (*h_int_.p) return false;
         const int batch = 4;
@@ -425,12 +445,10 @@ class Engine {
             events_before = used_;
             products_before = h_pcg_.p->products;
             for (int k = 0; k < batch; ++k) {
-                timed(BA_PROFILE_PCG_POINTS,
-                      [&] { launch_pcg_points<double>(s_, tp_, lin(), pe(), px_.p, pd_.p, pw_.p, pcg_.p); });
-                timed(BA_PROFILE_PCG_CAMERAS,
-                      [&] { launch_pcg_cameras<double>(s_, tp_, lin(), px_.p, pd_.p, pw_.p, pP_.p, pcg_.p); });
+                timed(BA_PROFILE_PCG_POINTS, [&] { launch_pcg_points<double>(s_, tp_, lin(), pe(), V); });
+                timed(BA_PROFILE_PCG_CAMERAS, [&] { launch_pcg_cameras<double>(s_, tp_, lin(), V); });
                 launch_pcg_blocks(s_, tp_, lambda_, diag_.p, px_.p, pd_.p, pP_.p, pq_.p, pcg_.p);
-                launch_pcg_update(s_, tp_, grad_.p, factors_.p, px_.p, pr_.p, pz_.p, pd_.p, pq_.p, pcg_.p);
+                launch_pcg_update(s_, tp_, V);
             }
         }
         *terms = (uint32_t)h_pcg_.p->iter;
@@ -506,7 +524,7 @@ class Engine {
     Buf<int32_t> diag_pair_, edge_pair_;
     Buf<double> uv_, R_[2], t_[2], gp_[2], X_[2], jc_, ocost_, osq_, vinv_, vdamp_, y_, ddiag_, rrows_, pscal_,
         cscal_, grad_, fgrad_, diag_, step_, rhs_, partials_, u_vals_, s_vals_, S_, work_, factors_, px_, pr_, pz_, pd_,
-        pq_, pw_, dots_, jpm_, jcm_, pP_;
+        pq_, pw_, dots_, jpm_, jcm_, pP_, pvcam_, ppart_;
     Buf<AsmScalars> asm_;
     Buf<CandScalars> cand_;
     Buf<GaugeDev> gauge_;
