@@ -105,6 +105,33 @@ def cpu_baseline(scene_fn, n_points_sample: int, n_obs_full: int, iterations: in
     }
 
 
+def bench_mixed(ba, prob, base, steps: int, ttc: dict | None) -> dict:
+    """The mixed-precision form on the same workload: LM iterations/s over `steps` iterations and
+    the loop time until an accepted state is within 1e-6 of the converged FP64 cost."""
+    from openmvg_ba import _abi as A
+
+    cfg = base.with_(precision=A.PRECISION_MIXED)
+    ba.solve(prob, cfg.with_(max_iterations=1))
+    sol = ba.solve(prob, cfg.with_(max_iterations=steps))
+    out = {"lm_iter_per_s": sol.report.iterations / sol.report.seconds_iterations,
+           "ms_per_step": 1e3 * sol.report.seconds_iterations / max(sol.report.iterations, 1),
+           "pcg_iterations": int(sol.report.pcg_iterations)}
+    if ttc is not None:
+        conv = ba.solve(prob, cfg.with_(max_iterations=ttc["fp64_iterations"] + 20))
+        target = ttc["converged_cost"] * (1.0 + 1e-6)
+        tr = conv.trace
+        hit = None
+        for i in range(len(tr["seconds"])):
+            if tr["accepted"][i] and tr["candidate_cost"][i] <= target:
+                hit = float(tr["seconds"][i])
+                break
+        out.update(time_to_cost_seconds=hit, iterations=int(conv.report.iterations),
+                   final_cost=float(conv.report.final_cost),
+                   final_cost_rel_error_vs_fp64=abs(conv.report.final_cost - ttc["converged_cost"]) / ttc["converged_cost"],
+                   speedup_vs_fp64=(ttc["fp64_seconds"] / hit) if (hit and ttc["fp64_seconds"]) else None)
+    return out
+
+
 def bench_differentiation(steps: int) -> dict:
     """Forward+backward observations/s of the differentiable reprojection cost: 1,000 cameras,
     200,000 points, 1,000,000 observations, Huber 1 px (seed 0xBA0005), FP32 and FP64, through
@@ -148,7 +175,6 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10, help="timed LM iterations")
     ap.add_argument("--warmup", type=int, default=2, help="untimed LM iterations (a separate solve)")
     ap.add_argument("--points", type=int, default=1_000_000)
-    ap.add_argument("--precision", choices=["fp64"], default="fp64")
     ap.add_argument("--roofline-kernel", default="pcg_points", choices=["pcg_points", "pcg_cameras", "linearize"])
     ap.add_argument("--converge-iterations", type=int, default=60,
                     help="iteration cap of the run that fixes the converged cost (0 skips time-to-cost)")
@@ -232,6 +258,8 @@ def main() -> int:
 
     forms = {}
     if not args.no_forms:
+        forms["mixed"] = bench_mixed(ba, prob, base, args.steps, ttc)
+        log(f"mixed: {forms['mixed']}")
         forms["differentiation"] = bench_differentiation(20)
         log(f"differentiation: {forms['differentiation']}")
 
@@ -260,7 +288,7 @@ def main() -> int:
                          f"{prob.n_observations} observations, geometric tracks, 2% outliers, Huber 1px, "
                          "LM + block-Jacobi PCG on implicit Schur (rtol 0.1), seed 0xBA0003"),
             "reduced_dimension": D,
-            "precision": args.precision,
+            "precision": "fp64",
             "pcg_iterations": int(rep.pcg_iterations),
         },
         "roofline": {

@@ -174,6 +174,10 @@ typedef struct ba_solve_report {
     double seconds_final;                 /* projection check, final zero-damping assembly, download */
     uint32_t profile_launches;            /* launches of the profiled kernel */
     double profile_seconds;               /* their summed HIP-event durations */
+    /* mixed form: direct steps solved with the FP64 factor because the FP32 factor broke down or
+     * its refinement stalled, and the relative residual of the last refined step */
+    uint32_t precision_fallbacks;
+    double refinement_residual;           /* NaN when no refined step ran */
 } ba_solve_report;
 
 /* Optional per-iteration record: the LM trajectory a parity test compares. */
@@ -326,11 +330,12 @@ int ba_hip_pcg(ba_hip_handle* handle, const ba_device_problem* problem, double l
                double relative_tolerance, uint32_t max_iterations, double* step /*D*/,
                uint32_t* iterations, double* relative_residual, int32_t* accepted);
 
-/* Kernel (d): dense Cholesky factor of the n x n SPD matrix `a` (device, row-major, lower
- * triangle read) followed by the solve a x = b; `x` (device, n) receives the solution.
+/* Kernel (d): dense Cholesky factor of the n x n SPD matrix `a` (device, row-major) followed by
+ * the solve a x = b; `x` (device, n) receives the solution.
  * Fails (BA_ERR_NUMERICAL) on a pivot <= 64 eps max|diag| (dense/linear_solver.rs:9-61).
- * precision: BA_PRECISION_FP64, or BA_PRECISION_MIXED for the FP32 factor with FP64
- * iterative refinement. */
+ * precision: BA_PRECISION_FP64 (lower triangle read), or BA_PRECISION_MIXED: FP32 factor on the
+ * matrix cores and FP64 iterative refinement against the full FP64 matrix (both triangles read),
+ * failing when the refined relative residual stays above 1e-10. */
 int ba_hip_cholesky_solve(ba_hip_handle* handle, size_t n, const double* a, const double* b,
                           double* x, int32_t precision);
 

@@ -5,25 +5,53 @@
 //                  against 64 eps max|diag(A)| as the reference's dense camera solve does
 //                  (kernel/multi_view/dense/linear_solver.rs:9-61);
 //   k_trsm_panel   one wave per row tile solves X L^T = A against the factored diagonal tile;
-//   k_syrk_update  the trailing update A_ij -= L_ik L_jk^T on the FP64 matrix cores
-//                  (v_mfma_f64_16x16x4f64), four waves per 64x64 tile;
-// then blocked forward and backward substitution in one workgroup each.
+//   k_syrk_update  the trailing update A_ij -= L_ik L_jk^T on the matrix cores
+//                  (v_mfma_f64_16x16x4f64, or v_mfma_f32_16x16x4f32 for the FP32 factor),
+//                  four waves per 64x64 tile;
+// then blocked forward and backward substitution in one workgroup each (FP64 vectors).
+// The mixed form factors an FP32 copy and recovers the FP64 solution by iterative refinement
+// against the FP64 matrix (residuals r = b - S x in FP64).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
 
 #include "ba_kernels.hpp"
 
+#define HIP_OK_OR_IGNORE(expr) (void)(expr)
+
 namespace ba {
 namespace {
 
 constexpr int TB = 64;  // tile edge
-typedef double d4 __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(1024) k_diag_tol(const double* A, uint32_t n, uint32_t ld, double* work, int* fail) {
+// Matrix-core step C += A B on 16x16x4 tiles: lane l holds A[l%16][l/16] and B[l/16][l%16];
+// element v of the accumulator holds D[row(l/16, v)][l%16] (layouts measured on gfx950).
+template <class T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+    static constexpr double eps = DBL_EPSILON;
+};
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+    static constexpr double eps = FLT_EPSILON;
+};
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_diag_tol(const T* A, uint32_t n, uint32_t ld, double* work, int* fail) {
     __shared__ double sh[1024];
     double m = 0.0;
-    for (uint32_t i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(A[(size_t)i * ld + i]));
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs((double)A[(size_t)i * ld + i]));
     sh[threadIdx.x] = m;
     __syncthreads();
     for (int s = 512; s > 0; s >>= 1) {
@@ -31,32 +59,33 @@ __global__ void __launch_bounds__(1024) k_diag_tol(const double* A, uint32_t n, 
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        work[0] = 64.0 * DBL_EPSILON * fmax(sh[0], DBL_MIN);
+        work[0] = 64.0 * Mfma<T>::eps * fmax(sh[0], DBL_MIN);
         *fail = 0;
     }
 }
 
 // Factor the diagonal tile at k0 (size nb <= 64) in LDS, right-looking, column by column.
-__global__ void __launch_bounds__(256) k_potrf_tile(double* A, uint32_t ld, uint32_t k0, uint32_t nb, const double* work,
+template <class T>
+__global__ void __launch_bounds__(256) k_potrf_tile(T* A, uint32_t ld, uint32_t k0, uint32_t nb, const double* work,
                                                    int* fail) {
-    __shared__ double a[TB][TB + 1];
+    __shared__ T a[TB][TB + 1];
     const int t = threadIdx.x;
     for (int idx = t; idx < TB * TB; idx += 256) {
         int r = idx / TB, c = idx % TB;
-        a[r][c] = (r < (int)nb && c <= r) ? A[(size_t)(k0 + r) * ld + k0 + c] : 0.0;
+        a[r][c] = (r < (int)nb && c <= r) ? A[(size_t)(k0 + r) * ld + k0 + c] : T(0);
     }
     __syncthreads();
-    const double tol = work[0];
+    const T tol = (T)work[0];
     __shared__ int bad;
     if (t == 0) bad = 0;
     __syncthreads();
     for (int j = 0; j < (int)nb; ++j) {
-        double piv = a[j][j];
+        T piv = a[j][j];
         if (!(isfinite(piv) && piv > tol)) {
             if (t == 0) bad = 1;
             break;  // uniform: every thread reads the same pivot
         }
-        double l = sqrt(piv);
+        T l = sqrt(piv);
         __syncthreads();
         if (t == 0) a[j][j] = l;
         for (int i = j + 1 + t; i < (int)nb; i += 256) a[i][j] /= l;
@@ -88,23 +117,24 @@ __global__ void __launch_bounds__(256) k_potrf_tile(double* A, uint32_t ld, uint
 
 // Rows of tile `ti` (ti > k tile) below the factored diagonal tile: X L_kk^T = A_ik.  One lane
 // per row; the substitution is unrolled over the 64 columns so the row stays in registers.
-__global__ void __launch_bounds__(64) k_trsm_panel(double* A, uint32_t n, uint32_t ld, uint32_t k0, uint32_t nb) {
-    __shared__ double L[TB][TB + 1];
+template <class T>
+__global__ void __launch_bounds__(64) k_trsm_panel(T* A, uint32_t n, uint32_t ld, uint32_t k0, uint32_t nb) {
+    __shared__ T L[TB][TB + 1];
     const int t = threadIdx.x;
     for (int idx = t; idx < TB * TB; idx += 64) {
         int r = idx / TB, c = idx % TB;
-        L[r][c] = (r < (int)nb && c <= r) ? A[(size_t)(k0 + r) * ld + k0 + c] : (r == c ? 1.0 : 0.0);
+        L[r][c] = (r < (int)nb && c <= r) ? A[(size_t)(k0 + r) * ld + k0 + c] : (r == c ? T(1) : T(0));
     }
     __syncthreads();
     const uint32_t row = k0 + nb + blockIdx.x * TB + t;
     if (row >= n) return;
-    double x[TB];
-    double* Ar = A + (size_t)row * ld + k0;
+    T x[TB];
+    T* Ar = A + (size_t)row * ld + k0;
 #pragma unroll
-    for (int c = 0; c < TB; ++c) x[c] = c < (int)nb ? Ar[c] : 0.0;
+    for (int c = 0; c < TB; ++c) x[c] = c < (int)nb ? Ar[c] : T(0);
 #pragma unroll
     for (int c = 0; c < TB; ++c) {
-        double v = x[c];
+        T v = x[c];
 #pragma unroll
         for (int m = 0; m < c; ++m) v -= x[m] * L[c][m];
         x[c] = v / L[c][c];
@@ -118,10 +148,12 @@ __global__ void __launch_bounds__(64) k_trsm_panel(double* A, uint32_t n, uint32
 // A_ij -= L_ik L_jk^T with K = nb, staged through LDS in K chunks of 32.  Four waves, each a
 // 32x32 quadrant of 2x2 MFMA 16x16x4 FP64 tiles.
 constexpr int KC = 32;
-__global__ void __launch_bounds__(256) k_syrk_update(double* A, uint32_t n, uint32_t ld, uint32_t k0, uint32_t nb,
+template <class T>
+__global__ void __launch_bounds__(256) k_syrk_update(T* A, uint32_t n, uint32_t ld, uint32_t k0, uint32_t nb,
                                                      uint32_t first_tile) {
-    __shared__ double Li[TB][KC + 1];
-    __shared__ double Lj[TB][KC + 1];
+    typedef typename Mfma<T>::v4 v4;
+    __shared__ T Li[TB][KC + 1];
+    __shared__ T Lj[TB][KC + 1];
     // blockIdx.x enumerates the lower-triangular tile pairs (ti, tj) of the trailing matrix
     const uint32_t idx = blockIdx.x;
     uint32_t ti = (uint32_t)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
@@ -133,45 +165,48 @@ __global__ void __launch_bounds__(256) k_syrk_update(double* A, uint32_t n, uint
     const int wave = t / 64, lane = t % 64;
     const int qr = (wave / 2) * 32, qc = (wave % 2) * 32;
     const int li = lane % 16, lk = lane / 16;
-    d4 acc[2][2];
+    v4 acc[2][2];
     for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4{T(0), T(0), T(0), T(0)};
     for (int kc = 0; kc < (int)nb; kc += KC) {
         __syncthreads();
         for (int e = t; e < TB * KC; e += 256) {
             int r = e / KC, c = e % KC;
             uint32_t ri = r0 + r, rj = c0 + r;
             bool kin = kc + c < (int)nb;
-            Li[r][c] = (ri < n && kin) ? A[(size_t)ri * ld + k0 + kc + c] : 0.0;
-            Lj[r][c] = (rj < n && kin) ? A[(size_t)rj * ld + k0 + kc + c] : 0.0;
+            Li[r][c] = (ri < n && kin) ? A[(size_t)ri * ld + k0 + kc + c] : T(0);
+            Lj[r][c] = (rj < n && kin) ? A[(size_t)rj * ld + k0 + kc + c] : T(0);
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < KC; k += 4) {
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
-                double av = Li[qr + 16 * a + li][k + lk];
+                T av = Li[qr + 16 * a + li][k + lk];
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    double bv = Lj[qc + 16 * b + li][k + lk];  // B[k][j] = L_jk[j][k]
-                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[a][b], 0, 0, 0);
+                    T bv = Lj[qc + 16 * b + li][k + lk];  // B[k][j] = L_jk[j][k]
+                    acc[a][b] = Mfma<T>::mac(av, bv, acc[a][b]);
                 }
             }
         }
     }
-    // v_mfma_f64_16x16x4f64 leaves D[lane/16 + 4v][lane%16] in accumulator element v
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
             for (int v = 0; v < 4; ++v) {
-                uint32_t r = r0 + qr + 16 * a + lk + 4 * v;
+                uint32_t r = r0 + qr + 16 * a + Mfma<T>::row(lk, v);
                 uint32_t c = c0 + qc + 16 * b + li;
                 if (r < n && c < n && c <= r) A[(size_t)r * ld + c] -= acc[a][b][v];
             }
 }
 
-// Forward substitution L y = b (one workgroup, tile by tile).
-__global__ void __launch_bounds__(1024) k_trsv_lower(const double* L, uint32_t n, uint32_t ld, const double* b, double* y) {
+// Forward substitution L y = b (one workgroup, tile by tile; FP64 vectors).  `skip` (optional)
+// turns the solve into a no-op once set.
+template <class T>
+__global__ void __launch_bounds__(1024) k_trsv_lower(const T* L, uint32_t n, uint32_t ld, const double* b, double* y,
+                                                     const int* skip) {
     __shared__ double yt[TB];
+    if (skip && *skip) return;
     const int t = threadIdx.x;
     for (uint32_t i = t; i < n; i += 1024) y[i] = b[i];
     __syncthreads();
@@ -180,9 +215,9 @@ __global__ void __launch_bounds__(1024) k_trsv_lower(const double* L, uint32_t n
         if (t < 64) {  // one wave solves the diagonal tile column by column
             double v = t < (int)nb ? y[k0 + t] : 0.0;
             for (int c = 0; c < (int)nb; ++c) {
-                double yc = __shfl(v, c) / L[(size_t)(k0 + c) * ld + k0 + c];
+                double yc = __shfl(v, c) / (double)L[(size_t)(k0 + c) * ld + k0 + c];
                 if (t == c) v = yc;
-                if (t > c && t < (int)nb) v -= L[(size_t)(k0 + t) * ld + k0 + c] * yc;
+                if (t > c && t < (int)nb) v -= (double)L[(size_t)(k0 + t) * ld + k0 + c] * yc;
             }
             if (t < (int)nb) {
                 yt[t] = v;
@@ -191,18 +226,21 @@ __global__ void __launch_bounds__(1024) k_trsv_lower(const double* L, uint32_t n
         }
         __syncthreads();
         for (uint32_t r = k0 + nb + t; r < n; r += 1024) {
-            const double* Lr = L + (size_t)r * ld + k0;
+            const T* Lr = L + (size_t)r * ld + k0;
             double s = 0.0;
-            for (int m = 0; m < (int)nb; ++m) s += Lr[m] * yt[m];
+            for (int m = 0; m < (int)nb; ++m) s += (double)Lr[m] * yt[m];
             y[r] -= s;
         }
         __syncthreads();
     }
 }
 
-// Backward substitution L^T x = y (one workgroup).
-__global__ void __launch_bounds__(1024) k_trsv_upper(const double* L, uint32_t n, uint32_t ld, const double* y, double* x) {
+// Backward substitution L^T x = y (one workgroup; FP64 vectors).
+template <class T>
+__global__ void __launch_bounds__(1024) k_trsv_upper(const T* L, uint32_t n, uint32_t ld, const double* y, double* x,
+                                                     const int* skip) {
     __shared__ double xt[TB];
+    if (skip && *skip) return;
     const int t = threadIdx.x;
     for (uint32_t i = t; i < n; i += 1024) x[i] = y[i];
     __syncthreads();
@@ -214,9 +252,9 @@ __global__ void __launch_bounds__(1024) k_trsv_upper(const double* L, uint32_t n
         if (t < 64) {
             double v = t < (int)nb ? x[k0 + t] : 0.0;
             for (int c = (int)nb - 1; c >= 0; --c) {
-                double xc = __shfl(v, c) / L[(size_t)(k0 + c) * ld + k0 + c];
+                double xc = __shfl(v, c) / (double)L[(size_t)(k0 + c) * ld + k0 + c];
                 if (t == c) v = xc;
-                if (t < c) v -= L[(size_t)(k0 + c) * ld + k0 + t] * xc;
+                if (t < c) v -= (double)L[(size_t)(k0 + c) * ld + k0 + t] * xc;
             }
             if (t < (int)nb) {
                 xt[t] = v;
@@ -226,35 +264,146 @@ __global__ void __launch_bounds__(1024) k_trsv_upper(const double* L, uint32_t n
         __syncthreads();
         for (uint32_t r = t; r < k0; r += 1024) {
             double s = 0.0;
-            for (int m = 0; m < (int)nb; ++m) s += L[(size_t)(k0 + m) * ld + r] * xt[m];
+            for (int m = 0; m < (int)nb; ++m) s += (double)L[(size_t)(k0 + m) * ld + r] * xt[m];
             x[r] -= s;
         }
         __syncthreads();
     }
 }
 
-}  // namespace
+// Iterative refinement: r = b - S x from the FP64 matrix (one wave per row, fixed lane order),
+// then the convergence test (one workgroup): stop when ||r|| <= 1e-14 ||b|| or when ||r|| no
+// longer halves; state[0] = done flag, state[1] = ||r||^2 of the last step, state[2] = ||b||^2.
+__global__ void __launch_bounds__(256) k_refine_residual(const double* S, uint32_t n, uint32_t ld, const double* b,
+                                                         const double* x, double* r, const int* done) {
+    if (*done) return;
+    const uint32_t row = blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (row >= n) return;
+    const double* Sr = S + (size_t)row * ld;
+    double s = 0.0;
+    for (uint32_t c = lane; c < n; c += 64) s += Sr[c] * x[c];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) r[row] = b[row] - s;
+}
+__global__ void __launch_bounds__(1024) k_refine_check(uint32_t n, const double* b, const double* r, double* state,
+                                                       int* done, int first) {
+    __shared__ double sh[2][1024];
+    if (*done) return;
+    double rr = 0.0, bb = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) {
+        rr += r[i] * r[i];
+        bb += b[i] * b[i];
+    }
+    sh[0][threadIdx.x] = rr;
+    sh[1][threadIdx.x] = bb;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double r2 = sh[0][0], b2 = sh[1][0];
+    const bool small = r2 <= 1e-28 * b2;
+    const bool stalled = !first && !(r2 < 0.25 * state[1]);
+    if (small || stalled || !isfinite(r2)) *done = 1;
+    if (!stalled || first) state[1] = r2;
+    state[2] = b2;
+}
+__global__ void __launch_bounds__(256) k_refine_axpy(uint32_t n, const double* dx, double* x, const int* done) {
+    if (*done) return;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) x[i] += dx[i];
+}
+// Jacobi scaling of the FP32 copy: F = D^-1/2 S D^-1/2 (unit diagonal), scale[i] = S_ii^-1/2 (1
+// when S_ii is not positive; the factor then reports the breakdown).  The FP32 factor of the
+// scaled matrix keeps the units of pose, intrinsic and gauge columns apart from its rounding.
+__global__ void __launch_bounds__(256) k_jacobi_scale(const double* S, uint32_t n, uint32_t ld, double* scale) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double d = S[(size_t)i * ld + i];
+    scale[i] = (isfinite(d) && d > 0.0) ? 1.0 / sqrt(d) : 1.0;
+}
+__global__ void __launch_bounds__(256) k_lower_to_f32(const double* S, uint32_t n, uint32_t ld, const double* scale,
+                                                      float* F) {
+    for (uint32_t row = blockIdx.y; row < n; row += gridDim.y)
+        for (uint32_t c = blockIdx.x * 256 + threadIdx.x; c <= row && c < n; c += 256 * gridDim.x)
+            F[(size_t)row * ld + c] = (float)(S[(size_t)row * ld + c] * scale[row] * scale[c]);
+}
+// y = scale * v (elementwise), skipped once `done` is set
+__global__ void __launch_bounds__(256) k_scale_vec(uint32_t n, const double* scale, const double* v, double* y,
+                                                   const int* done) {
+    if (done && *done) return;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = scale[i] * v[i];
+}
 
-size_t cholesky_work_doubles(uint32_t n) { return 8 + (size_t)n; }
-
-void launch_cholesky_solve(hipStream_t s, double* A, uint32_t n, uint32_t ld, const double* b, double* x, double* work,
-                           int* fail) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_diag_tol, dim3(1), dim3(1024), 0, s, A, n, ld, work, fail);
+template <class T>
+void factor(hipStream_t s, T* A, uint32_t n, uint32_t ld, double* work, int* fail) {
+    hipLaunchKernelGGL(k_diag_tol<T>, dim3(1), dim3(1024), 0, s, A, n, ld, work, fail);
     const uint32_t ntile = (n + TB - 1) / TB;
     for (uint32_t kt = 0; kt < ntile; ++kt) {
         const uint32_t k0 = kt * TB;
         const uint32_t nb = n - k0 < TB ? n - k0 : TB;
-        hipLaunchKernelGGL(k_potrf_tile, dim3(1), dim3(256), 0, s, A, ld, k0, nb, work, fail);
+        hipLaunchKernelGGL(k_potrf_tile<T>, dim3(1), dim3(256), 0, s, A, ld, k0, nb, work, fail);
         const uint32_t rest = ntile - kt - 1;
         if (rest == 0) break;
-        hipLaunchKernelGGL(k_trsm_panel, dim3(rest), dim3(64), 0, s, A, n, ld, k0, nb);
+        hipLaunchKernelGGL(k_trsm_panel<T>, dim3(rest), dim3(64), 0, s, A, n, ld, k0, nb);
         const uint32_t pairs = rest * (rest + 1) / 2;
-        hipLaunchKernelGGL(k_syrk_update, dim3(pairs), dim3(256), 0, s, A, n, ld, k0, nb, kt + 1);
+        hipLaunchKernelGGL(k_syrk_update<T>, dim3(pairs), dim3(256), 0, s, A, n, ld, k0, nb, kt + 1);
     }
-    double* y = work + 8;
-    hipLaunchKernelGGL(k_trsv_lower, dim3(1), dim3(1024), 0, s, A, n, ld, b, y);
-    hipLaunchKernelGGL(k_trsv_upper, dim3(1), dim3(1024), 0, s, A, n, ld, y, x);
+}
+template <class T>
+void apply(hipStream_t s, const T* L, uint32_t n, uint32_t ld, const double* b, double* x, double* tmp, const int* skip) {
+    hipLaunchKernelGGL(k_trsv_lower<T>, dim3(1), dim3(1024), 0, s, L, n, ld, b, tmp, skip);
+    hipLaunchKernelGGL(k_trsv_upper<T>, dim3(1), dim3(1024), 0, s, L, n, ld, (const double*)tmp, x, skip);
+}
+
+}  // namespace
+
+size_t cholesky_work_doubles(uint32_t n) { return 16 + 5 * (size_t)n; }
+
+void launch_cholesky_solve(hipStream_t s, double* A, uint32_t n, uint32_t ld, const double* b, double* x, double* work,
+                           int* fail) {
+    if (n == 0) return;
+    factor<double>(s, A, n, ld, work, fail);
+    apply<double>(s, A, n, ld, b, x, work + 16, nullptr);
+}
+
+void launch_cholesky_solve_mixed(hipStream_t s, const double* S, float* F, uint32_t n, uint32_t ld, const double* b,
+                                 double* x, double* work, int* fail, int* done, int max_steps) {
+    if (n == 0) return;
+    double* tmp = work + 16;
+    double* r = tmp + n;
+    double* dx = r + n;
+    double* scale = dx + n;
+    double* sv = scale + n;
+    const unsigned nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_jacobi_scale, dim3(nb), dim3(256), 0, s, S, n, ld, scale);
+    hipLaunchKernelGGL(k_lower_to_f32, dim3(nb, n < 65535u ? n : 65535u), dim3(256), 0, s, S, n, ld,
+                       (const double*)scale, F);
+    factor<float>(s, F, n, ld, work, fail);
+    HIP_OK_OR_IGNORE(hipMemsetAsync(done, 0, sizeof(int), s));
+    // x = D^-1/2 F^-1 D^-1/2 b
+    hipLaunchKernelGGL(k_scale_vec, dim3(nb), dim3(256), 0, s, n, (const double*)scale, b, sv, (const int*)nullptr);
+    apply<float>(s, F, n, ld, sv, dx, tmp, nullptr);
+    hipLaunchKernelGGL(k_scale_vec, dim3(nb), dim3(256), 0, s, n, (const double*)scale, (const double*)dx, x,
+                       (const int*)nullptr);
+    const unsigned rows = (n + 3) / 4;
+    for (int k = 0; k < max_steps; ++k) {
+        hipLaunchKernelGGL(k_refine_residual, dim3(rows), dim3(256), 0, s, S, n, ld, b, (const double*)x, r,
+                           (const int*)done);
+        hipLaunchKernelGGL(k_refine_check, dim3(1), dim3(1024), 0, s, n, b, (const double*)r, work + 8, done, (int)(k == 0));
+        hipLaunchKernelGGL(k_scale_vec, dim3(nb), dim3(256), 0, s, n, (const double*)scale, (const double*)r, sv,
+                           (const int*)done);
+        apply<float>(s, F, n, ld, sv, dx, tmp, done);
+        hipLaunchKernelGGL(k_scale_vec, dim3(nb), dim3(256), 0, s, n, (const double*)scale, (const double*)dx, dx,
+                           (const int*)done);
+        hipLaunchKernelGGL(k_refine_axpy, dim3(nb), dim3(256), 0, s, n, (const double*)dx, x, (const int*)done);
+    }
 }
 
 }  // namespace ba

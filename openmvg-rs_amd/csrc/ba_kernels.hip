@@ -223,8 +223,8 @@ __global__ void __launch_bounds__(NT) k_linearize(Topo tp, const double* __restr
         cost = sq = 0.0;
     }
     out.valid[o] = ok ? 1 : 0;
-    out.cost[o] = (T)cost;
-    out.sq[o] = (T)sq;
+    out.cost[o] = cost;
+    out.sq[o] = sq;
     if (out.rj) {
         T* rj = out.rj + 8 * (size_t)o;
         rj[0] = (T)r[0];
@@ -1672,24 +1672,28 @@ void launch_negate(hipStream_t s, uint32_t n, const double* in, double* out) {
 }
 
 template void launch_gauge<double>(hipStream_t, const double*, int, double, GaugeDev*);
-template void launch_point_elim<double>(hipStream_t, const Topo&, const Lin<double>, double, bool, PointElim<double>);
-template void launch_block_gradient<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>,
-                                            double*, double*, double*);
-template void launch_camera_assembly<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>,
-                                             const int32_t*, const uint64_t*, double*, double*, double*, double*,
-                                             double*);
-template void launch_pairs<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>, uint32_t,
-                                   const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,
-                                   const uint64_t*, double*, double*);
-template void launch_linearize<double>(hipStream_t, const Topo&, const double*, const double*, const double*,
-                                       const double*, const double*, const GaugeDev*, Lin<double>);
-template void launch_candidate<double>(hipStream_t, const Topo&, const double*, const double*, const double*,
-                                       const double*, const double*, const double*, const double*, const Lin<double>,
-                                       const PointElim<double>, const double*, const GaugeDev*, double*, double*);
-template void launch_pcg_points<double>(hipStream_t, const Topo&, const Lin<double>, const PointElim<double>,
-                                        const PcgVecs&);
-template void launch_pcg_cameras<double>(hipStream_t, const Topo&, const Lin<double>, const PcgVecs&);
-template void launch_linearize_cm<double>(hipStream_t, const Topo&, const double*, const double*, const double*,
-                                          const double*, const double*, const GaugeDev*, Lin<double>);
+
+#define BA_INSTANTIATE(T)                                                                                          \
+    template void launch_point_elim<T>(hipStream_t, const Topo&, const Lin<T>, T, bool, PointElim<T>);               \
+    template void launch_block_gradient<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>, double*,      \
+                                           double*, double*);                                                      \
+    template void launch_camera_assembly<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>,              \
+                                            const int32_t*, const uint64_t*, double*, double*, double*, double*,   \
+                                            double*);                                                              \
+    template void launch_pairs<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>, uint32_t,               \
+                                  const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,              \
+                                  const uint32_t*, const uint64_t*, double*, double*);                             \
+    template void launch_linearize<T>(hipStream_t, const Topo&, const double*, const double*, const double*,        \
+                                      const double*, const double*, const GaugeDev*, Lin<T>);                      \
+    template void launch_linearize_cm<T>(hipStream_t, const Topo&, const double*, const double*, const double*,     \
+                                         const double*, const double*, const GaugeDev*, Lin<T>);                   \
+    template void launch_candidate<T>(hipStream_t, const Topo&, const double*, const double*, const double*,        \
+                                      const double*, const double*, const double*, const double*, const Lin<T>,    \
+                                      const PointElim<T>, const double*, const GaugeDev*, double*, double*);       \
+    template void launch_pcg_points<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>, const PcgVecs&);  \
+    template void launch_pcg_cameras<T>(hipStream_t, const Topo&, const Lin<T>, const PcgVecs&);
+
+BA_INSTANTIATE(double)
+BA_INSTANTIATE(float)
 
 }  // namespace ba

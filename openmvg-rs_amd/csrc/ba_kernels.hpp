@@ -64,8 +64,8 @@ template <class T>
 struct Lin {
     T* rj;      // 8 per observation: r~ (2), J~_X (2x3 row-major); or null (block-pair path only)
     T* jc;      // 18 per observation: J~_c (2x9), rows [pose 0..5 | intrinsic 6..8]; or null
-    T* cost;    // robust cost per observation
-    T* sq;      // squared residual norm per observation
+    double* cost;  // robust cost per observation (FP64 in both forms)
+    double* sq;    // squared residual norm per observation
     uint8_t* valid;
     T* jpm;     // SOA_ROWS x no, point-major
     T* jcm;     // SOA_ROWS x no, camera-major (position j holds observation cm_obs[j]); or null
@@ -198,5 +198,11 @@ void launch_pcg_update(hipStream_t s, const Topo& tp, const PcgVecs& V);
 void launch_cholesky_solve(hipStream_t s, double* A, uint32_t n, uint32_t ld, const double* b, double* x,
                            double* work, int* fail);
 size_t cholesky_work_doubles(uint32_t n);
+// Mixed form: factor an FP32 copy F of the lower triangle of S (FP32 matrix cores), solve, then
+// refine x against the FP64 S for at most max_steps steps (r = b - S x in FP64).  work[8..10]
+// receive the refinement state (done flag in *done, last ||r||^2, ||b||^2); *fail != 0 after an
+// FP32 pivot at or below 64 eps_32 max|diag|.
+void launch_cholesky_solve_mixed(hipStream_t s, const double* S, float* F, uint32_t n, uint32_t ld, const double* b,
+                                 double* x, double* work, int* fail, int* done, int max_steps);
 void launch_negate(hipStream_t s, uint32_t n, const double* in, double* out);
 }  // namespace ba

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ba_hip.h"
@@ -99,7 +100,11 @@ struct Nielsen {  // numerics/refine/nonlinear/controller.rs:15-91
     }
 };
 
-// Device copy of a lowered problem and every buffer of the loop.
+// Device copy of a lowered problem and every buffer of the loop.  T is the storage and
+// arithmetic type of the per-observation Jacobians and per-point blocks: double for the plain
+// form, float for the mixed form (state, sums, reduced-system vectors and the LM control stay
+// FP64 in both).
+template <class T>
 class Engine {
    public:
     Engine(hipStream_t s, const Lowered& L, bool iterative, int profile = BA_PROFILE_NONE)
@@ -217,6 +222,10 @@ class Engine {
         if (!iterative_) {
             S_.alloc(D * D);
             work_.alloc(cholesky_work_doubles((uint32_t)D));
+            if (std::is_same<T, float>::value) {
+                S32_.alloc(D * D);
+                done_.alloc(1);
+            }
         } else {
             std::vector<int32_t> dp(B_.n_blocks, -1), ep(nc, -1);
             for (size_t k = 0; k < PL_.block_a.size(); ++k) {
@@ -348,8 +357,8 @@ class Engine {
         return V;
     }
 
-    Lin<double> lin() { return Lin<double>{nullptr, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p}; }
-    PointElim<double> pe() { return PointElim<double>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, pscal_.p}; }
+    Lin<T> lin() { return Lin<T>{nullptr, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p}; }
+    PointElim<T> pe() { return PointElim<T>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, pscal_.p}; }
 
     // Use a caller-given gauge state instead of the one derived from the translations.
     void set_gauge(const GaugeDev& g) {
@@ -365,26 +374,26 @@ class Engine {
         tp.loss_scale = loss_scale;
         if (!fixed_gauge_) launch_gauge<double>(s_, t_[state].p, L_.gauge_camera, L_.scene_scale, gauge_.p);
         timed(BA_PROFILE_LINEARIZE, [&] {
-            launch_linearize<double>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
+            launch_linearize<T>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
                                      lin());
         });
         if (iterative_ && (reduced_system || fast_cam_))
-            launch_linearize_cm<double>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
+            launch_linearize_cm<T>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
                                         lin());
         lambda_ = lambda;
-        timed(BA_PROFILE_POINT_ELIM, [&] { launch_point_elim<double>(s_, tp, lin(), lambda, lambda == 0.0, pe()); });
+        timed(BA_PROFILE_POINT_ELIM, [&] { launch_point_elim<T>(s_, tp, lin(), (T)lambda, lambda == 0.0, pe()); });
         launch_reduce_points(s_, pscal_.p, tp.np, partials_.p, asm_.p);
         if (fast_cam_) {
             timed(BA_PROFILE_PAIRS, [&] {
-                launch_camera_assembly<double>(s_, tp, lin(), pe(), diag_pair_.p, value_off_.p, grad_.p, fgrad_.p,
+                launch_camera_assembly<T>(s_, tp, lin(), pe(), diag_pair_.p, value_off_.p, grad_.p, fgrad_.p,
                                                diag_.p, u_vals_.p, s_vals_.p);
             });
         } else {
-            launch_block_gradient<double>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
+            launch_block_gradient<T>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
         }
         if (reduced_system && !fast_cam_) {
             timed(BA_PROFILE_PAIRS, [&] {
-                launch_pairs<double>(s_, tp, lin(), pe(), (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p,
+                launch_pairs<T>(s_, tp, lin(), pe(), (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p,
                                      term_off_.p, term_i_.p, term_j_.p, value_off_.p, u_vals_.p, s_vals_.p);
             });
         }
@@ -406,16 +415,33 @@ class Engine {
         return *h_asm_.p;
     }
 
-    // Direct camera step: S step = -g by dense Cholesky (camera_step.rs:122-128).
+    // Direct camera step: S step = -g by dense Cholesky (camera_step.rs:122-128).  The mixed form
+    // factors S in FP32 and refines against the FP64 S; when the FP32 factor breaks down or the
+    // refinement stalls above 1e-10 relative residual, the step is solved with the FP64 factor.
     bool solve_direct() {
         if (tp_.D == 0) return true;
         launch_negate(s_, tp_.D, grad_.p, rhs_.p);
+        if (std::is_same<T, float>::value) {
+            timed(BA_PROFILE_CHOLESKY, [&] {
+                launch_cholesky_solve_mixed(s_, S_.p, S32_.p, tp_.D, tp_.D, rhs_.p, step_.p, work_.p, fail_.p, done_.p,
+                                            8);
+            });
+            double st[3];
+            HIP_OK(hipMemcpyAsync(h_int_.p, fail_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+            HIP_OK(hipMemcpyAsync(st, work_.p + 8, 3 * sizeof(double), hipMemcpyDeviceToHost, s_));
+            HIP_OK(hipStreamSynchronize(s_));
+            refine_rel_ = std::sqrt(st[1] / st[2]);
+            if (*h_int_.p == 0 && std::isfinite(refine_rel_) && refine_rel_ <= 1e-10) return true;
+            mixed_fallbacks_ += 1;
+        }
         timed(BA_PROFILE_CHOLESKY,
               [&] { launch_cholesky_solve(s_, S_.p, tp_.D, tp_.D, rhs_.p, step_.p, work_.p, fail_.p); });
         HIP_OK(hipMemcpyAsync(h_int_.p, fail_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
         HIP_OK(hipStreamSynchronize(s_));
         return *h_int_.p == 0;
     }
+    uint32_t mixed_fallbacks() const { return mixed_fallbacks_; }
+    double refinement_residual() const { return refine_rel_; }
 
     // Retained PCG (self_calibration_retained.rs:512-579).  Returns accepted; terms/rel out.
     bool solve_pcg(double tol, int limit, uint32_t* terms, double* rel) {
@@ -445,8 +471,8 @@ class Engine {
             events_before = used_;
             products_before = h_pcg_.p->products;
             for (int k = 0; k < batch; ++k) {
-                timed(BA_PROFILE_PCG_POINTS, [&] { launch_pcg_points<double>(s_, tp_, lin(), pe(), V); });
-                timed(BA_PROFILE_PCG_CAMERAS, [&] { launch_pcg_cameras<double>(s_, tp_, lin(), V); });
+                timed(BA_PROFILE_PCG_POINTS, [&] { launch_pcg_points<T>(s_, tp_, lin(), pe(), V); });
+                timed(BA_PROFILE_PCG_CAMERAS, [&] { launch_pcg_cameras<T>(s_, tp_, lin(), V); });
                 launch_pcg_blocks(s_, tp_, lambda_, diag_.p, px_.p, pd_.p, pP_.p, pq_.p, pcg_.p);
                 launch_pcg_update(s_, tp_, V);
             }
@@ -469,7 +495,7 @@ class Engine {
         tp.loss_scale = loss_scale;
         launch_retract(s_, tp, R_[state].p, t_[state].p, gp_[state].p, step_.p, gauge_.p, R_[cs].p, t_[cs].p, gp_[cs].p);
         timed(BA_PROFILE_CANDIDATE, [&] {
-            launch_candidate<double>(s_, tp, R_[state].p, t_[state].p, R_[cs].p, t_[cs].p, gp_[cs].p, X_[state].p,
+            launch_candidate<T>(s_, tp, R_[state].p, t_[state].p, R_[cs].p, t_[cs].p, gp_[cs].p, X_[state].p,
                                      uv_.p, lin(), pe(), step_.p, gauge_.p, X_[cs].p, cscal_.p);
         });
         launch_reduce_candidates(s_, cscal_.p, tp.np, partials_.p, cand_.p);
@@ -498,7 +524,7 @@ class Engine {
 
     const Topo& topo() const { return tp_; }
     bool iterative() const { return iterative_; }
-    const double* jpm() const { return jpm_.p; }
+    const T* jpm() const { return jpm_.p; }
     const double* obs_cost() const { return ocost_.p; }
     const uint8_t* valid() const { return valid_.p; }
     const double* S() const { return S_.p; }
@@ -522,9 +548,11 @@ class Engine {
     Buf<uint8_t> pt_fixed_, valid_;
     Buf<uint64_t> value_off_;
     Buf<int32_t> diag_pair_, edge_pair_;
-    Buf<double> uv_, R_[2], t_[2], gp_[2], X_[2], jc_, ocost_, osq_, vinv_, vdamp_, y_, ddiag_, rrows_, pscal_,
-        cscal_, grad_, fgrad_, diag_, step_, rhs_, partials_, u_vals_, s_vals_, S_, work_, factors_, px_, pr_, pz_, pd_,
-        pq_, pw_, dots_, jpm_, jcm_, pP_, pvcam_, ppart_;
+    Buf<double> uv_, R_[2], t_[2], gp_[2], X_[2], ocost_, osq_, pscal_, cscal_, grad_, fgrad_, diag_, step_, rhs_,
+        partials_, u_vals_, s_vals_, S_, work_, factors_, px_, pr_, pz_, pd_, pq_, pw_, dots_, pP_, pvcam_, ppart_;
+    Buf<T> jc_, vinv_, vdamp_, y_, ddiag_, rrows_, jpm_, jcm_;
+    Buf<float> S32_;  // FP32 factor of the mixed direct form
+    Buf<int> done_;
     Buf<AsmScalars> asm_;
     Buf<CandScalars> cand_;
     Buf<GaugeDev> gauge_;
@@ -537,6 +565,8 @@ class Engine {
     double h_dots_[4];
     bool fixed_gauge_ = false;
     double lambda_ = 0.0;
+    double refine_rel_ = NAN;
+    uint32_t mixed_fallbacks_ = 0;
     bool fast_cam_ = false;
     int profile_ = BA_PROFILE_NONE;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
@@ -639,7 +669,8 @@ struct StageDef {
 // The LM loop (sparse/self_calibration/mod.rs:212-639) on the GPU engine.
 // `loop_seconds` receives the host time from the first iteration's start to the end of the last one;
 // every iteration ends in a synchronous scalar readback, so this brackets completed device work.
-Status refine(Engine& E, const Lowered& L, const ba_solve_config* cfg, ba_solve_report& rep,
+template <class T>
+Status refine(Engine<T>& E, const Lowered& L, const ba_solve_config* cfg, ba_solve_report& rep,
               ba_iteration_trace* trace_rows, int* final_state, double* loop_seconds) {
     const size_t D = L.D;
     const bool iterative = E.iterative();
@@ -740,7 +771,7 @@ Status refine(Engine& E, const Lowered& L, const ba_solve_config* cfg, ba_solve_
                 trace_push(trace, current, lambda, NAN, NAN, NAN, false, terms);
                 continue;
             }
-            Engine::Candidate c = E.candidate(state, sched[si].loss, sched[si].scale, iterative);
+            auto c = E.candidate(state, sched[si].loss, sched[si].scale, iterative);
             if (c.c.rejected != 0.0) {
                 rej[2]++;
                 damping.reject();
@@ -838,6 +869,34 @@ int finish(const Status& st, char* msg, size_t len) {
 }  // namespace
 }  // namespace ba
 
+namespace ba {
+namespace {
+template <class T>
+Status run_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* cfg, const Lowered& L,
+                 ba_solution* sol, ba_solve_report& r, ba_iteration_trace* trace, double t0) {
+    const bool iterative = cfg->linear_solver == BA_SOLVER_ITERATIVE;
+    Engine<T> E(h->stream, L, iterative, cfg->profile_kernel);
+    E.setup();
+    double t1 = now();
+    int state = 0;
+    double loop_seconds = 0.0;
+    Status st = refine(E, L, cfg, r, trace, &state, &loop_seconds);
+    if (!st.ok()) return st;
+    std::vector<double> R, t, gp, X;
+    E.download(state, R, t, gp, X);
+    raise_solution(p, L, R.data(), t.data(), gp.data(), X.data(), sol);
+    double t3 = now();
+    r.seconds_setup = t1 - t0;
+    r.seconds_iterations = loop_seconds;
+    r.seconds_final = (t3 - t1) - loop_seconds;
+    E.profile_result(&r.profile_launches, &r.profile_seconds);
+    r.precision_fallbacks = E.mixed_fallbacks();
+    r.refinement_residual = E.refinement_residual();
+    return st;
+}
+}  // namespace
+}  // namespace ba
+
 using namespace ba;
 
 extern "C" {
@@ -913,38 +972,24 @@ int ba_hip_prepare(const ba_problem* p, ba_native_sizes* sizes, ba_native_proble
     return BA_OK;
 }
 
+
 int ba_hip_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* cfg, ba_solution* sol,
                  ba_solve_report* rep, ba_iteration_trace* trace, char* msg, size_t len) {
     if (!h || !cfg || !sol) return finish(error(BA_ERR_INVALID_CONFIG, "null argument"), msg, len);
     Status st = validate_config(cfg);
     if (st.ok()) st = validate_problem(p);
     if (!st.ok()) return finish(st, msg, len);
-    if (cfg->precision != BA_PRECISION_FP64)
-        return finish(error(BA_ERR_UNSUPPORTED, "this build runs the FP64 form only"), msg, len);
     try {
         if (hipSetDevice(h->device) != hipSuccess) return finish(error(BA_ERR_INTERNAL, "hipSetDevice"), msg, len);
         double t0 = now();
         Lowered L;
         st = lower_problem(p, L, false);
         if (!st.ok()) return finish(st, msg, len);
-        const bool iterative = cfg->linear_solver == BA_SOLVER_ITERATIVE;
         if (trace) trace->count = 0;
-        Engine E(h->stream, L, iterative, cfg->profile_kernel);
-        E.setup();
-        double t1 = now();
         ba_solve_report r;
-        int state = 0;
-        double loop_seconds = 0.0;
-        st = refine(E, L, cfg, r, trace, &state, &loop_seconds);
+        st = cfg->precision == BA_PRECISION_MIXED ? run_solve<float>(h, p, cfg, L, sol, r, trace, t0)
+                                                  : run_solve<double>(h, p, cfg, L, sol, r, trace, t0);
         if (!st.ok()) return finish(st, msg, len);
-        std::vector<double> R, t, gp, X;
-        E.download(state, R, t, gp, X);
-        raise_solution(p, L, R.data(), t.data(), gp.data(), X.data(), sol);
-        double t3 = now();
-        r.seconds_setup = t1 - t0;
-        r.seconds_iterations = loop_seconds;
-        r.seconds_final = (t3 - t1) - loop_seconds;
-        E.profile_result(&r.profile_launches, &r.profile_seconds);
         if (rep) *rep = r;
         return BA_OK;
     } catch (const HipError& e) {
@@ -974,7 +1019,7 @@ int ba_hip_linearize(ba_hip_handle* h, const ba_device_problem* dp, double* resi
     Lowered L;
     Status st = lowered_from_device(dp, L, h->stream);
     if (!st.ok()) return st.code;
-    Engine E(h->stream, L, true);
+    Engine<double> E(h->stream, L, true);
     E.setup();
     E.set_gauge(gauge_from(dp));
     E.assemble(0, 0.0, L.loss, L.loss_scale, false);
@@ -1004,7 +1049,7 @@ int ba_hip_schur(ba_hip_handle* h, const ba_device_problem* dp, double lambda, d
     Lowered L;
     Status st = lowered_from_device(dp, L, h->stream);
     if (!st.ok()) return st.code;
-    Engine E(h->stream, L, false);
+    Engine<double> E(h->stream, L, false);
     E.setup();
     E.set_gauge(gauge_from(dp));
     AsmScalars a = E.assemble(0, lambda, L.loss, L.loss_scale, true);
@@ -1036,7 +1081,7 @@ int ba_hip_pcg(ba_hip_handle* h, const ba_device_problem* dp, double lambda, dou
     Lowered L;
     Status st = lowered_from_device(dp, L, h->stream);
     if (!st.ok()) return st.code;
-    Engine E(h->stream, L, true);
+    Engine<double> E(h->stream, L, true);
     E.setup();
     E.set_gauge(gauge_from(dp));
     AsmScalars a = E.assemble(0, lambda, L.loss, L.loss_scale, true);
@@ -1054,17 +1099,31 @@ int ba_hip_pcg(ba_hip_handle* h, const ba_device_problem* dp, double lambda, dou
 
 int ba_hip_cholesky_solve(ba_hip_handle* h, size_t n, const double* a, const double* b, double* x, int32_t precision) {
     HOOK_BEGIN
-    if (precision != BA_PRECISION_FP64) return BA_ERR_UNSUPPORTED;
+    if (precision != BA_PRECISION_FP64 && precision != BA_PRECISION_MIXED) return BA_ERR_INVALID_CONFIG;
     if (n == 0) return BA_OK;
     hipStream_t s = h->stream;
-    Buf<double> A, work;
+    Buf<double> work;
     Buf<int> fail;
-    A.alloc(n * n);
     work.alloc(cholesky_work_doubles((uint32_t)n));
     fail.alloc(1);
+    int f = 0;
+    if (precision == BA_PRECISION_MIXED) {  // FP32 factor, FP64 refinement against the full matrix
+        Buf<float> F;
+        Buf<int> done;
+        F.alloc(n * n);
+        done.alloc(1);
+        launch_cholesky_solve_mixed(s, a, F.p, (uint32_t)n, (uint32_t)n, b, x, work.p, fail.p, done.p, 8);
+        double st[3];
+        HIP_OK(hipMemcpyAsync(&f, fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(st, work.p + 8, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        const double rel = std::sqrt(st[1] / st[2]);
+        return (f || !(rel <= 1e-10)) ? BA_ERR_NUMERICAL : BA_OK;
+    }
+    Buf<double> A;
+    A.alloc(n * n);
     HIP_OK(hipMemcpyAsync(A.p, a, n * n * 8, hipMemcpyDeviceToDevice, s));
     launch_cholesky_solve(s, A.p, (uint32_t)n, (uint32_t)n, b, x, work.p, fail.p);
-    int f = 0;
     HIP_OK(hipMemcpyAsync(&f, fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     return f ? BA_ERR_NUMERICAL : BA_OK;

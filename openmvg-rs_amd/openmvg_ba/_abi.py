@@ -133,6 +133,7 @@ class SolveReportC(C.Structure):
         ("observation_count", C.c_uint64), ("point_count", C.c_uint64), ("pose_count", C.c_uint64),
         ("seconds_setup", C.c_double), ("seconds_iterations", C.c_double), ("seconds_final", C.c_double),
         ("profile_launches", C.c_uint32), ("profile_seconds", C.c_double),
+        ("precision_fallbacks", C.c_uint32), ("refinement_residual", C.c_double),
     ]
 
 
