@@ -339,6 +339,38 @@ int ba_hip_pcg(ba_hip_handle* handle, const ba_device_problem* problem, double l
 int ba_hip_cholesky_solve(ba_hip_handle* handle, size_t n, const double* a, const double* b,
                           double* x, int32_t precision);
 
+/* ---- Batched form -------------------------------------------------------------------------
+ * Many independent windows (local bundle adjustments) solved together with the direct camera
+ * step: the result of each window equals ba_hip_solve on that window alone (same per-window LM
+ * control: sparse/self_calibration/mod.rs:212-639 for squared or Huber loss).  All windows run in
+ * every launch; each window keeps its damping, acceptance and stop tests on the device, so one LM
+ * iteration is a fixed launch sequence, captured once in a HIP graph and replayed
+ * (BA_BATCH_GRAPH), or issued on the stream each iteration (BA_BATCH_STREAM).  The tile edge of
+ * the per-window dense Cholesky (32 or 64) is chosen by timing both on the batch's window sizes.
+ * precision MIXED factors Jacobi-scaled FP32 copies with FP64 iterative refinement.  No
+ * counterpart entry point in the reference, whose callers solve windows one by one through
+ * BundleAdjuster::solve (solver.rs:1562-1573). */
+enum ba_batch_mode { BA_BATCH_GRAPH = 0, BA_BATCH_STREAM = 1 };
+
+typedef struct ba_batch_trace {
+    uint32_t capacity;                    /* iterations per window the arrays hold (0: none) */
+    double* cost;                         /* [iteration * n_windows + window], cost at the linearisation */
+    double* damping;                      /* lambda used */
+    double* candidate_cost;               /* NaN when no candidate was formed */
+    uint8_t* accepted;
+    uint32_t tile;                        /* out: tile edge the autotuner chose */
+    double seconds_loop;                  /* out: wall time of the LM iterations of all windows */
+    double autotune_seconds[2];           /* out: measured per-window Cholesky time, tiles 32 and 64 */
+} ba_batch_trace;
+
+/* problems, solutions and reports are arrays of n_windows (reports may be NULL); the config
+ * (max_iterations, tolerances, precision) applies to every window; linear_solver ITERATIVE is
+ * unsupported.  seconds_iterations of each report is the loop time of the whole batch. */
+int ba_hip_solve_batched(ba_hip_handle* handle, size_t n_windows, const ba_problem* problems,
+                         const ba_solve_config* config, ba_solution* solutions,
+                         ba_solve_report* reports, ba_batch_trace* trace, int32_t mode,
+                         char* message, size_t message_len);
+
 /* ---- Differentiation form ------------------------------------------------------------------
  * Robust reprojection cost  C = sum_o rho(|r_o|)  over observations o = (camera c, point p, pixel m)
  * with the radial model of self_calibration/model.rs:26-129 (X_c = exp(w) X + t, n = X_c.xy / X_c.z,

@@ -1,0 +1,579 @@
+// ba_engine.hpp — device state and kernel sequencing of one LM solve (internal to the library).
+//
+// Engine<T> holds a lowered problem resident in HBM (topology, two copies of the state, the
+// per-observation linearisation, per-point blocks, the reduced system) and runs the passes of
+// one LM iteration: assemble (linearise, eliminate points, assemble the camera side), the camera
+// step (dense Cholesky or PCG), and the candidate pass.  T is the storage type of Jacobians and
+// point blocks (double: plain form; float: mixed form).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/ba_hip.h"
+#include "ba_handle.hpp"
+#include "ba_host.hpp"
+#include "ba_kernels.hpp"
+
+namespace ba {
+namespace {
+
+constexpr int REDUCE_BLOCKS = 512;
+
+struct HipError {
+    hipError_t code;
+    const char* what;
+};
+#define HIP_OK(expr)                                 \
+    do {                                             \
+        hipError_t _e = (expr);                      \
+        if (_e != hipSuccess) throw HipError{_e, #expr}; \
+    } while (0)
+
+template <class T>
+struct Buf {
+    T* p = nullptr;
+    size_t n = 0;
+    Buf() = default;
+    Buf(const Buf&) = delete;
+    Buf& operator=(const Buf&) = delete;
+    ~Buf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = count;
+        if (count) HIP_OK(hipMalloc(&p, count * sizeof(T)));
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) HIP_OK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+template <class T>
+struct Pinned {
+    T* p = nullptr;
+    Pinned() { HIP_OK(hipHostMalloc(&p, sizeof(T))); }
+    ~Pinned() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Nielsen {  // numerics/refine/nonlinear/controller.rs:15-91
+    double value, nu = 2.0, gain = NAN;
+    bool has_gain = false;
+    void increase() {
+        value = std::fmin(value * nu, 1e12);
+        nu = std::fmin(nu * 2.0, 1e12);
+    }
+    void reject() {
+        has_gain = false;
+        increase();
+    }
+    bool accept(double actual, double predicted) {
+        double g = (std::isfinite(actual) && std::isfinite(predicted) && predicted > 0.0) ? actual / predicted : NAN;
+        has_gain = std::isfinite(g);
+        gain = g;
+        if (!(std::isfinite(g) && g > 0.0)) {
+            increase();
+            return false;
+        }
+        double c = 2.0 * g - 1.0;
+        value = std::fmax(value * std::fmax(1.0 - c * c * c, 1.0 / 3.0), 1e-12);
+        nu = 2.0;
+        return true;
+    }
+};
+
+// Device copy of a lowered problem and every buffer of the loop.  T is the storage and
+// arithmetic type of the per-observation Jacobians and per-point blocks: double for the plain
+// form, float for the mixed form (state, sums, reduced-system vectors and the LM control stay
+// FP64 in both).
+template <class T>
+class Engine {
+   public:
+    Engine(hipStream_t s, const Lowered& L, bool iterative, int profile = BA_PROFILE_NONE)
+        : s_(s), L_(L), iterative_(iterative), profile_(profile) {}
+    ~Engine() {
+        for (auto& e : events_) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    }
+
+    // Run `launch`, bracketed by HIP events on this stream when `which` is the profiled kernel.
+    template <class F>
+    void timed(int which, F&& launch) {
+        if (which != profile_) {
+            launch();
+            return;
+        }
+        if (used_ == events_.size()) {
+            hipEvent_t a, b;
+            HIP_OK(hipEventCreate(&a));
+            HIP_OK(hipEventCreate(&b));
+            events_.emplace_back(a, b);
+        }
+        auto& e = events_[used_++];
+        HIP_OK(hipEventRecord(e.first, s_));
+        launch();
+        HIP_OK(hipEventRecord(e.second, s_));
+    }
+    // Launch count and summed duration of the profiled kernel so far.
+    void profile_result(uint32_t* launches, double* seconds) {
+        HIP_OK(hipStreamSynchronize(s_));
+        double total = 0.0;
+        for (size_t k = 0; k < used_; ++k) {
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, events_[k].first, events_[k].second));
+            total += ms * 1e-3;
+        }
+        *launches = (uint32_t)used_;
+        *seconds = total;
+    }
+
+    void setup() {
+        B_ = build_block_lists(L_);
+        PL_ = build_pair_lists(L_, B_, !iterative_);
+        const size_t nc = L_.n_cameras(), ng = L_.n_groups(), np = L_.n_points(), no = L_.n_obs(), D = L_.D;
+        obs_cam_.upload(L_.obs_cam, s_);
+        obs_pt_.upload(L_.obs_point, s_);
+        pt_off_.upload(L_.point_off, s_);
+        pt_fixed_.upload(L_.point_fixed, s_);
+        cam_group_.upload(L_.cam_group, s_);
+        cam_off_.upload(L_.cam_off, s_);
+        cam_dim_.upload(L_.cam_dim, s_);
+        grp_off_.upload(L_.grp_off, s_);
+        grp_dim_.upload(L_.grp_dim, s_);
+        blk_off_.upload(B_.block_off, s_);
+        blk_dim_.upload(B_.block_dim, s_);
+        blk_obs_off_.upload(B_.obs_off, s_);
+        blk_obs_.upload(B_.obs, s_);
+        {  // cameras of each group, ascending
+            std::vector<uint32_t> off(ng + 1, 0), cams(nc);
+            for (size_t c = 0; c < nc; ++c) off[L_.cam_group[c] + 1]++;
+            for (size_t g = 0; g < ng; ++g) off[g + 1] += off[g];
+            std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+            for (size_t c = 0; c < nc; ++c) cams[cur[L_.cam_group[c]]++] = (uint32_t)c;
+            grp_cam_off_.upload(off, s_);
+            grp_cams_.upload(cams, s_);
+            HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
+        }
+        uv_.upload(L_.uv, s_);
+        for (int k = 0; k < 2; ++k) {
+            R_[k].alloc(9 * nc);
+            t_[k].alloc(3 * nc);
+            gp_[k].alloc(3 * ng);
+            X_[k].alloc(3 * np);
+        }
+        HIP_OK(hipMemcpyAsync(R_[0].p, L_.R.data(), 9 * nc * 8, hipMemcpyHostToDevice, s_));
+        HIP_OK(hipMemcpyAsync(t_[0].p, L_.t.data(), 3 * nc * 8, hipMemcpyHostToDevice, s_));
+        HIP_OK(hipMemcpyAsync(gp_[0].p, L_.gp.data(), 3 * ng * 8, hipMemcpyHostToDevice, s_));
+        HIP_OK(hipMemcpyAsync(X_[0].p, L_.X.data(), 3 * np * 8, hipMemcpyHostToDevice, s_));
+        fast_cam_ = iterative_ && simple_topology();
+        build_matvec_order(iterative_);
+        jpm_.alloc(SOA_ROWS * no);
+        if (!fast_cam_) jc_.alloc(18 * no);  // block-pair path
+        ocost_.alloc(no);
+        osq_.alloc(no);
+        valid_.alloc(no);
+        vinv_.alloc(6 * np);
+        vdamp_.alloc(6 * np);
+        y_.alloc(3 * np);
+        ddiag_.alloc(3 * np);
+        if (!fast_cam_) rrows_.alloc(27 * no);
+        pscal_.alloc(6 * np);
+        cscal_.alloc(4 * np);
+        grad_.alloc(D);
+        fgrad_.alloc(D);
+        diag_.alloc(D);
+        step_.alloc(D);
+        rhs_.alloc(D);
+        partials_.alloc(8 * REDUCE_BLOCKS);
+        asm_.alloc(1);
+        cand_.alloc(1);
+        gauge_.alloc(1);
+        fail_.alloc(1);
+        dots_.alloc(4);
+        pair_a_.upload(PL_.block_a, s_);
+        pair_b_.upload(PL_.block_b, s_);
+        term_off_.upload(PL_.term_off, s_);
+        term_i_.upload(PL_.term_i, s_);
+        term_j_.upload(PL_.term_j, s_);
+        value_off_.upload(PL_.value_off, s_);
+        const size_t nvals = PL_.value_off.back();
+        u_vals_.alloc(nvals);
+        s_vals_.alloc(nvals);
+        if (!iterative_ && !batched_) {
+            S_.alloc(D * D);
+            work_.alloc(cholesky_work_doubles((uint32_t)D));
+            if (std::is_same<T, float>::value) {
+                S32_.alloc(D * D);
+                done_.alloc(1);
+            }
+        } else {
+            std::vector<int32_t> dp(B_.n_blocks, -1), ep(nc, -1);
+            for (size_t k = 0; k < PL_.block_a.size(); ++k) {
+                uint32_t a = PL_.block_a[k], b = PL_.block_b[k];
+                if (a == b) dp[a] = (int32_t)k;
+                else if (a >= nc && b < nc) ep[b] = (int32_t)k;
+            }
+            diag_pair_.upload(dp, s_);
+            edge_pair_.upload(ep, s_);
+            factors_.alloc(54 * B_.n_blocks);
+            jcm_.alloc(SOA_ROWS * no);
+            pP_.alloc(9 * nc);
+            pvcam_.alloc(9 * nc);
+            ppart_.alloc(std::max<size_t>(1, (B_.n_blocks + 255) / 256));
+            px_.alloc(D);
+            pr_.alloc(D);
+            pz_.alloc(D);
+            pd_.alloc(D);
+            pq_.alloc(D);
+            pw_.alloc(3 * np);
+            pcg_.alloc(1);
+        }
+        tp_.nc = (uint32_t)nc;
+        tp_.ng = (uint32_t)ng;
+        tp_.np = (uint32_t)np;
+        tp_.no = (uint32_t)no;
+        tp_.D = (uint32_t)D;
+        tp_.nblocks = (uint32_t)B_.n_blocks;
+        tp_.obs_cam = obs_cam_.p;
+        tp_.obs_pt = obs_pt_.p;
+        tp_.pt_off = pt_off_.p;
+        tp_.pt_fixed = pt_fixed_.p;
+        tp_.cam_group = cam_group_.p;
+        tp_.cam_off = cam_off_.p;
+        tp_.cam_dim = cam_dim_.p;
+        tp_.grp_off = grp_off_.p;
+        tp_.grp_dim = grp_dim_.p;
+        tp_.blk_off = blk_off_.p;
+        tp_.blk_dim = blk_dim_.p;
+        tp_.blk_obs_off = blk_obs_off_.p;
+        tp_.blk_obs = blk_obs_.p;
+        tp_.grp_cam_off = grp_cam_off_.p;
+        tp_.grp_cams = grp_cams_.p;
+        tp_.nchunks = (uint32_t)(chunk_pt_.n ? chunk_pt_.n - 1 : 0);
+        tp_.chunk_pt = chunk_pt_.p;
+        tp_.chunk_obs = chunk_obs_.p;
+        tp_.cm_off = cm_off_.p;
+        tp_.cm_obs = cm_obs_.p;
+        tp_.cm_pt = cm_pt_.p;
+        tp_.loss = L_.loss;
+        tp_.loss_scale = L_.loss_scale;
+        HIP_OK(hipStreamSynchronize(s_));
+    }
+
+    // Point chunks (at most NT observations of whole points, or one larger point; at most NT
+    // points) and the camera-major observation order of the matrix-free Schur product.
+    // Every intrinsic group has one camera and no camera observes a point twice: each diagonal
+    // block term of the reduced system is then one observation's (the camera-side assembly).
+    bool simple_topology() const {
+        const size_t nc = L_.n_cameras(), ng = L_.n_groups(), np = L_.n_points();
+        std::vector<uint32_t> per_group(ng, 0);
+        for (size_t c = 0; c < nc; ++c) per_group[L_.cam_group[c]]++;
+        for (uint32_t k : per_group)
+            if (k != 1) return false;
+        std::vector<uint32_t> seen(nc, UINT32_MAX);
+        for (size_t p = 0; p < np; ++p)
+            for (uint32_t o = L_.point_off[p]; o < L_.point_off[p + 1]; ++o) {
+                const uint32_t c = L_.obs_cam[o];
+                if (seen[c] == (uint32_t)p) return false;
+                seen[c] = (uint32_t)p;
+            }
+        return true;
+    }
+
+    void build_matvec_order(bool camera_major) {
+        const size_t np = L_.n_points(), no = L_.n_obs(), nc = L_.n_cameras();
+        const uint32_t cap = 256;
+        std::vector<uint32_t> cpt{0}, cob{0};
+        uint32_t obs_in = 0, pts_in = 0;
+        for (size_t p = 0; p < np; ++p) {
+            const uint32_t k = L_.point_off[p + 1] - L_.point_off[p];
+            if (pts_in > 0 && (obs_in + k > cap || pts_in == cap)) {
+                cpt.push_back((uint32_t)p);
+                cob.push_back(L_.point_off[p]);
+                obs_in = pts_in = 0;
+            }
+            obs_in += k;
+            pts_in += 1;
+        }
+        if (pts_in > 0) {
+            cpt.push_back((uint32_t)np);
+            cob.push_back((uint32_t)no);
+        }
+        chunk_pt_.upload(cpt, s_);
+        chunk_obs_.upload(cob, s_);
+        if (!camera_major) {
+            HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
+            return;
+        }
+        std::vector<uint32_t> off(nc + 1, 0), obs(no), pts(no);
+        for (size_t o = 0; o < no; ++o) off[L_.obs_cam[o] + 1]++;
+        for (size_t c = 0; c < nc; ++c) off[c + 1] += off[c];
+        std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+        for (size_t o = 0; o < no; ++o) {
+            const uint32_t j = cur[L_.obs_cam[o]]++;
+            obs[j] = (uint32_t)o;
+            pts[j] = L_.obs_point[o];
+        }
+        cm_off_.upload(off, s_);
+        cm_obs_.upload(obs, s_);
+        cm_pt_.upload(pts, s_);
+        HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
+    }
+
+    PcgVecs pcg_vecs() {
+        PcgVecs V;
+        V.grad = grad_.p;
+        V.factors = factors_.p;
+        V.x = px_.p;
+        V.r = pr_.p;
+        V.z = pz_.p;
+        V.d = pd_.p;
+        V.q = pq_.p;
+        V.w = pw_.p;
+        V.P = pP_.p;
+        V.vcam = pvcam_.p;
+        V.part = ppart_.p;
+        V.st = pcg_.p;
+        return V;
+    }
+
+    Lin<T> lin() { return Lin<T>{nullptr, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p}; }
+    PointElim<T> pe() { return PointElim<T>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, pscal_.p}; }
+
+    // Use a caller-given gauge state instead of the one derived from the translations.
+    void set_gauge(const GaugeDev& g) {
+        fixed_gauge_ = true;
+        HIP_OK(hipMemcpyAsync(gauge_.p, &g, sizeof(GaugeDev), hipMemcpyHostToDevice, s_));
+        HIP_OK(hipStreamSynchronize(s_));
+    }
+
+    // One assembly at damping lambda with loss `loss` (schur/assembly.rs:832-1069).
+    AsmScalars assemble(int state, double lambda, int loss, double loss_scale, bool reduced_system) {
+        Topo tp = tp_;
+        tp.loss = loss;
+        tp.loss_scale = loss_scale;
+        if (!fixed_gauge_) launch_gauge<double>(s_, t_[state].p, L_.gauge_camera, L_.scene_scale, gauge_.p);
+        timed(BA_PROFILE_LINEARIZE, [&] {
+            launch_linearize<T>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
+                                     lin());
+        });
+        if (iterative_ && (reduced_system || fast_cam_))
+            launch_linearize_cm<T>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
+                                        lin());
+        lambda_ = lambda;
+        timed(BA_PROFILE_POINT_ELIM, [&] { launch_point_elim<T>(s_, tp, lin(), (T)lambda, lambda == 0.0, pe()); });
+        launch_reduce_points(s_, pscal_.p, tp.np, partials_.p, asm_.p);
+        if (fast_cam_) {
+            timed(BA_PROFILE_PAIRS, [&] {
+                launch_camera_assembly<T>(s_, tp, lin(), pe(), diag_pair_.p, value_off_.p, grad_.p, fgrad_.p,
+                                               diag_.p, u_vals_.p, s_vals_.p);
+            });
+        } else {
+            launch_block_gradient<T>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
+        }
+        if (reduced_system && !fast_cam_) {
+            timed(BA_PROFILE_PAIRS, [&] {
+                launch_pairs<T>(s_, tp, lin(), pe(), (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p,
+                                     term_off_.p, term_i_.p, term_j_.p, value_off_.p, u_vals_.p, s_vals_.p);
+            });
+        }
+        if (reduced_system && !iterative_) {  // dense S with lambda diag on its diagonal
+            HIP_OK(hipMemsetAsync(S_.p, 0, S_.n * sizeof(double), s_));
+            launch_scatter_dense(s_, tp, (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p, value_off_.p, u_vals_.p,
+                                 s_vals_.p, S_.p, tp.D);
+            launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, S_.p, tp.D, nullptr, value_off_.p, u_vals_.p,
+                            s_vals_.p, asm_.p);
+        } else if (reduced_system) {  // lambda diag on the diagonal blocks of the preconditioner
+            launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, nullptr, tp.D, diag_pair_.p, value_off_.p,
+                            u_vals_.p, s_vals_.p, asm_.p);
+        } else {
+            launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, nullptr, tp.D, nullptr, value_off_.p, u_vals_.p,
+                            s_vals_.p, asm_.p);
+        }
+        HIP_OK(hipMemcpyAsync(h_asm_.p, asm_.p, sizeof(AsmScalars), hipMemcpyDeviceToHost, s_));
+        HIP_OK(hipStreamSynchronize(s_));
+        return *h_asm_.p;
+    }
+
+    // Direct camera step: S step = -g by dense Cholesky (camera_step.rs:122-128).  The mixed form
+    // factors S in FP32 and refines against the FP64 S; when the FP32 factor breaks down or the
+    // refinement stalls above 1e-10 relative residual, the step is solved with the FP64 factor.
+    bool solve_direct() {
+        if (tp_.D == 0) return true;
+        launch_negate(s_, tp_.D, grad_.p, rhs_.p);
+        if (std::is_same<T, float>::value) {
+            timed(BA_PROFILE_CHOLESKY, [&] {
+                launch_cholesky_solve_mixed(s_, S_.p, S32_.p, tp_.D, tp_.D, rhs_.p, step_.p, work_.p, fail_.p, done_.p,
+                                            5);
+            });
+            double st[3];
+            HIP_OK(hipMemcpyAsync(h_int_.p, fail_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+            HIP_OK(hipMemcpyAsync(st, work_.p + 8, 3 * sizeof(double), hipMemcpyDeviceToHost, s_));
+            HIP_OK(hipStreamSynchronize(s_));
+            refine_rel_ = std::sqrt(st[1] / st[2]);
+            if (*h_int_.p == 0 && std::isfinite(refine_rel_) && refine_rel_ <= 1e-10) return true;
+            mixed_fallbacks_ += 1;
+        }
+        timed(BA_PROFILE_CHOLESKY,
+              [&] { launch_cholesky_solve(s_, S_.p, tp_.D, tp_.D, rhs_.p, step_.p, work_.p, fail_.p); });
+        HIP_OK(hipMemcpyAsync(h_int_.p, fail_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+        HIP_OK(hipStreamSynchronize(s_));
+        return *h_int_.p == 0;
+    }
+    uint32_t mixed_fallbacks() const { return mixed_fallbacks_; }
+    double refinement_residual() const { return refine_rel_; }
+
+    // Retained PCG (self_calibration_retained.rs:512-579).  Returns accepted; terms/rel out.
+    bool solve_pcg(double tol, int limit, uint32_t* terms, double* rel) {
+        *terms = 0;
+        *rel = NAN;
+        if (tp_.D == 0) {
+            HIP_OK(hipMemsetAsync(step_.p, 0, 0, s_));
+            return true;
+        }
+        HIP_OK(hipMemsetAsync(fail_.p, 0, sizeof(int), s_));
+        launch_precond_factor(s_, tp_, diag_pair_.p, value_off_.p, u_vals_.p, s_vals_.p, factors_.p, fail_.p);
+        HIP_OK(hipMemcpyAsync(h_int_.p, fail_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+        const PcgVecs V = pcg_vecs();
+        launch_pcg_init(s_, tp_, V, tol, limit);
+        HIP_OK(hipStreamSynchronize(s_));
+        if (*h_int_.p) return false;
+        const int batch = 4;
+        const bool profiling_pcg = profile_ == BA_PROFILE_PCG_POINTS || profile_ == BA_PROFILE_PCG_CAMERAS;
+        size_t events_before = used_;
+        int products_before = 0;
+        for (;;) {
+            HIP_OK(hipMemcpyAsync(h_pcg_.p, pcg_.p, sizeof(PcgState), hipMemcpyDeviceToHost, s_));
+            HIP_OK(hipStreamSynchronize(s_));
+            // launches after the state turned done returned at once: drop their timings
+            if (profiling_pcg) used_ = events_before + (size_t)(h_pcg_.p->products - products_before);
+            if (h_pcg_.p->done) break;
+            events_before = used_;
+            products_before = h_pcg_.p->products;
+            for (int k = 0; k < batch; ++k) {
+                timed(BA_PROFILE_PCG_POINTS, [&] { launch_pcg_points<T>(s_, tp_, lin(), pe(), V); });
+                timed(BA_PROFILE_PCG_CAMERAS, [&] { launch_pcg_cameras<T>(s_, tp_, lin(), V); });
+                launch_pcg_blocks(s_, tp_, lambda_, diag_.p, px_.p, pd_.p, pP_.p, pq_.p, pcg_.p);
+                launch_pcg_update(s_, tp_, V);
+            }
+        }
+        *terms = (uint32_t)h_pcg_.p->iter;
+        *rel = h_pcg_.p->rel;
+        if (h_pcg_.p->accepted)
+            HIP_OK(hipMemcpyAsync(step_.p, px_.p, tp_.D * sizeof(double), hipMemcpyDeviceToDevice, s_));
+        return h_pcg_.p->accepted != 0;
+    }
+
+    struct Candidate {
+        CandScalars c;
+        double dots[4];
+    };
+    Candidate candidate(int state, int loss, double loss_scale, bool with_product) {
+        const int cs = 1 - state;
+        Topo tp = tp_;
+        tp.loss = loss;
+        tp.loss_scale = loss_scale;
+        launch_retract(s_, tp, R_[state].p, t_[state].p, gp_[state].p, step_.p, gauge_.p, R_[cs].p, t_[cs].p, gp_[cs].p);
+        timed(BA_PROFILE_CANDIDATE, [&] {
+            launch_candidate<T>(s_, tp, R_[state].p, t_[state].p, R_[cs].p, t_[cs].p, gp_[cs].p, X_[state].p,
+                                     uv_.p, lin(), pe(), step_.p, gauge_.p, X_[cs].p, cscal_.p);
+        });
+        launch_reduce_candidates(s_, cscal_.p, tp.np, partials_.p, cand_.p);
+        launch_step_dots(s_, tp.D, grad_.p, diag_.p, step_.p, with_product ? pq_.p : nullptr, dots_.p);
+        HIP_OK(hipMemcpyAsync(h_cand_.p, cand_.p, sizeof(CandScalars), hipMemcpyDeviceToHost, s_));
+        HIP_OK(hipMemcpyAsync(h_dots_, dots_.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s_));
+        HIP_OK(hipStreamSynchronize(s_));
+        Candidate out;
+        out.c = *h_cand_.p;
+        std::memcpy(out.dots, h_dots_, sizeof(out.dots));
+        return out;
+    }
+
+    void download(int state, std::vector<double>& R, std::vector<double>& t, std::vector<double>& gp,
+                  std::vector<double>& X) {
+        R.resize(9 * tp_.nc);
+        t.resize(3 * tp_.nc);
+        gp.resize(3 * tp_.ng);
+        X.resize(3 * tp_.np);
+        HIP_OK(hipMemcpyAsync(R.data(), R_[state].p, R.size() * 8, hipMemcpyDeviceToHost, s_));
+        HIP_OK(hipMemcpyAsync(t.data(), t_[state].p, t.size() * 8, hipMemcpyDeviceToHost, s_));
+        HIP_OK(hipMemcpyAsync(gp.data(), gp_[state].p, gp.size() * 8, hipMemcpyDeviceToHost, s_));
+        HIP_OK(hipMemcpyAsync(X.data(), X_[state].p, X.size() * 8, hipMemcpyDeviceToHost, s_));
+        HIP_OK(hipStreamSynchronize(s_));
+    }
+
+    const Topo& topo() const { return tp_; }
+    bool iterative() const { return iterative_; }
+    const T* jpm() const { return jpm_.p; }
+    const double* obs_cost() const { return ocost_.p; }
+    const uint8_t* valid() const { return valid_.p; }
+    const double* S() const { return S_.p; }
+    const double* grad() const { return grad_.p; }
+    const double* full_grad() const { return fgrad_.p; }
+    const double* diag() const { return diag_.p; }
+    const double* step() const { return step_.p; }
+    const double* pcg_x() const { return px_.p; }
+    hipStream_t stream() const { return s_; }
+
+   protected:
+    hipStream_t s_;
+    const Lowered& L_;
+    bool iterative_;
+    BlockLists B_;
+    PairLists PL_;
+    Topo tp_{};
+    Buf<uint32_t> obs_cam_, obs_pt_, pt_off_, cam_group_, cam_off_, cam_dim_, grp_off_, grp_dim_, blk_off_, blk_dim_,
+        blk_obs_off_, blk_obs_, grp_cam_off_, grp_cams_, pair_a_, pair_b_, term_off_, term_i_, term_j_, chunk_pt_,
+        chunk_obs_, cm_off_, cm_obs_, cm_pt_;
+    Buf<uint8_t> pt_fixed_, valid_;
+    Buf<uint64_t> value_off_;
+    Buf<int32_t> diag_pair_, edge_pair_;
+    Buf<double> uv_, R_[2], t_[2], gp_[2], X_[2], ocost_, osq_, pscal_, cscal_, grad_, fgrad_, diag_, step_, rhs_,
+        partials_, u_vals_, s_vals_, S_, work_, factors_, px_, pr_, pz_, pd_, pq_, pw_, dots_, pP_, pvcam_, ppart_;
+    Buf<T> jc_, vinv_, vdamp_, y_, ddiag_, rrows_, jpm_, jcm_;
+    Buf<float> S32_;  // FP32 factor of the mixed direct form
+    Buf<int> done_;
+    Buf<AsmScalars> asm_;
+    Buf<CandScalars> cand_;
+    Buf<GaugeDev> gauge_;
+    Buf<int> fail_;
+    Buf<PcgState> pcg_;
+    Pinned<AsmScalars> h_asm_;
+    Pinned<CandScalars> h_cand_;
+    Pinned<int> h_int_;
+    Pinned<PcgState> h_pcg_;
+    double h_dots_[4];
+    bool fixed_gauge_ = false;
+    double lambda_ = 0.0;
+    bool batched_ = false;  // independent windows: per-window dense systems instead of one S
+    double refine_rel_ = NAN;
+    uint32_t mixed_fallbacks_ = 0;
+    bool fast_cam_ = false;
+    int profile_ = BA_PROFILE_NONE;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
+    size_t used_ = 0;
+};
+
+}  // namespace
+}  // namespace ba

@@ -96,41 +96,10 @@ __device__ __forceinline__ void camera_values(const Topo& tp, uint32_t c, const 
 // ----------------------------------------------------------------------------------------------
 template <class T>
 __global__ void k_gauge(const T* cam_t, int camera, double scene_scale, GaugeDev* out) {
-    GaugeDev g;
-    g.camera = camera;
-    g.scene_scale = scene_scale;
-    g.baseline = 0.0;
-    for (int k = 0; k < 3; ++k) g.direction[k] = 0.0;
-    for (int k = 0; k < 6; ++k) g.basis[k] = 0.0;
-    if (camera >= 0) {
-        double t[3] = {(double)cam_t[3 * camera], (double)cam_t[3 * camera + 1], (double)cam_t[3 * camera + 2]};
-        double b = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-        g.baseline = b;
-        double dir[3] = {t[0] / b, t[1] / b, t[2] / b};
-        for (int k = 0; k < 3; ++k) g.direction[k] = dir[k];
-        // s2_tangent_basis (numerics/manifold.rs:129-141)
-        double n = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-        double tv[3] = {dir[0] / n, dir[1] / n, dir[2] / n};
-        double ref[3] = {0, 0, 0};
-        double a0 = fabs(tv[0]), a1 = fabs(tv[1]), a2 = fabs(tv[2]);
-        if (a0 <= a1 && a0 <= a2)
-            ref[0] = 1.0;
-        else if (a1 <= a2)
-            ref[1] = 1.0;
-        else
-            ref[2] = 1.0;
-        double c0[3] = {tv[1] * ref[2] - tv[2] * ref[1], tv[2] * ref[0] - tv[0] * ref[2],
-                        tv[0] * ref[1] - tv[1] * ref[0]};
-        double n0 = sqrt(c0[0] * c0[0] + c0[1] * c0[1] + c0[2] * c0[2]);
-        double b0[3] = {c0[0] / n0, c0[1] / n0, c0[2] / n0};
-        double c1[3] = {tv[1] * b0[2] - tv[2] * b0[1], tv[2] * b0[0] - tv[0] * b0[2], tv[0] * b0[1] - tv[1] * b0[0]};
-        double n1 = sqrt(c1[0] * c1[0] + c1[1] * c1[1] + c1[2] * c1[2]);
-        for (int k = 0; k < 3; ++k) {
-            g.basis[k] = b0[k];
-            g.basis[3 + k] = c1[k] / n1;
-        }
-    }
-    *out = g;
+    double t[3] = {0.0, 0.0, 0.0};
+    if (camera >= 0)
+        for (int k = 0; k < 3; ++k) t[k] = (double)cam_t[3 * camera + k];
+    *out = make_gauge(t, camera, scene_scale);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -146,9 +115,10 @@ namespace {
 __device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t o, const double* __restrict__ Rg,
                                               const double* __restrict__ tg, const double* __restrict__ gp,
                                               const double* __restrict__ X, const double* __restrict__ uv,
-                                              const GaugeDev* __restrict__ gauge, double r[2], double jx[6],
+                                              const GaugeDev* __restrict__ gauges, double r[2], double jx[6],
                                               double jc[18], double* cost, double* sq) {
     const uint32_t c = tp.obs_cam[o], p = tp.obs_pt[o], g = tp.cam_group[c];
+    const GaugeDev* gauge = gauges + (tp.cam_win ? tp.cam_win[c] : 0);
     const double* R = Rg + 9 * (size_t)c;
     double Xp[3] = {X[3 * (size_t)p], X[3 * (size_t)p + 1], X[3 * (size_t)p + 2]};
     double rot[3];
@@ -392,6 +362,7 @@ __global__ void __launch_bounds__(NT) k_point_elim(Topo tp, Lin<T> lin, T lambda
                 bool elim = isfinite(tscale) && tscale > T(0);
                 T fl = T(64) * Eps<T>::value * tscale;
                 T dd[3] = {fmax(V[0], fl), fmax(V[2], fl), fmax(V[5], fl)};
+                if (tp.win_lambda) lambda = (T)tp.win_lambda[tp.pt_win[p]];
                 T Vd[6] = {V[0] + lambda * dd[0], V[1], V[2] + lambda * dd[1], V[3], V[4], V[5] + lambda * dd[2]};
                 elim = elim && point_block_inverse(Vd, inv);
                 if (!elim) {
@@ -890,12 +861,13 @@ __global__ void __launch_bounds__(NT) k_retract(Topo tp, const double* R, const 
                     for (int c = 0; c < 3; ++c) s += up[3 * a + c] * R[9 * i + 3 * c + b];
                     Rc[9 * i + 3 * a + b] = s;
                 }
-            if ((int)i == gauge->camera) {
+            const GaugeDev* gw = gauge + (tp.cam_win ? tp.cam_win[i] : 0);
+            if ((int)i == gw->camera) {
                 double dir[3];
-                retract_s2(gauge->direction, gauge->basis, step[off + 3], step[off + 4], dir);
-                for (int k = 0; k < 3; ++k) tc[3 * i + k] = gauge->baseline * dir[k];
+                retract_s2(gw->direction, gw->basis, step[off + 3], step[off + 4], dir);
+                for (int k = 0; k < 3; ++k) tc[3 * i + k] = gw->baseline * dir[k];
             } else {
-                for (int k = 0; k < 3; ++k) tc[3 * i + k] = t[3 * i + k] + gauge->scene_scale * step[off + 3 + k];
+                for (int k = 0; k < 3; ++k) tc[3 * i + k] = t[3 * i + k] + gw->scene_scale * step[off + 3 + k];
             }
         }
     }
@@ -973,7 +945,6 @@ __global__ void __launch_bounds__(NT) k_candidate(Topo tp, const double* R, cons
     const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
     const size_t no = tp.no;
     const bool big = oe - ob > NT;
-    const double scene = gauge->scene_scale;
     // 1. per-observation back-substitution terms
     double e[3] = {0.0, 0.0, 0.0};
     for (uint32_t o = ob + lane; o < oe; o += NT) {
@@ -1001,6 +972,7 @@ __global__ void __launch_bounds__(NT) k_candidate(Topo tp, const double* R, cons
     // 2. stepped point per point
     const bool have_point = lane < pend - pb;
     const uint32_t p = pb + lane;
+    const double scene = have_point ? gauge[tp.pt_win ? tp.pt_win[p] : 0].scene_scale : 0.0;
     double prev[3] = {0.0, 0.0, 0.0}, stepped[3] = {0.0, 0.0, 0.0}, dp[3] = {0.0, 0.0, 0.0};
     bool fixed = true;
     if (have_point) {

@@ -51,6 +51,11 @@ struct Topo {
     const uint32_t* cm_off;
     const uint32_t* cm_obs;
     const uint32_t* cm_pt;
+    // Batched windows (null for one problem): window of each camera and point, and the damping of
+    // each window; the gauge argument of the kernels then holds one GaugeDev per window.
+    const uint32_t* cam_win;
+    const uint32_t* pt_win;
+    const double* win_lambda;
     int loss;
     double loss_scale;
 };
@@ -191,6 +196,106 @@ void launch_linearize_cm(hipStream_t s, const Topo& tp, const double* R, const d
                          const double* X, const double* uv, const GaugeDev* gauge, Lin<T> out);
 // The PCG step after a product (update, preconditioner pass, new direction).
 void launch_pcg_update(hipStream_t s, const Topo& tp, const PcgVecs& V);
+
+// ---- batched windows: independent problems concatenated, one LM control per window ----
+struct WinTopo {
+    uint32_t W;
+    const uint32_t* cam_off;    // W+1 camera ranges
+    const uint32_t* grp_off;    // W+1 group ranges
+    const uint32_t* pt_off;     // W+1 point ranges
+    const uint32_t* param_off;  // W+1 parameter ranges (D_w = difference)
+    const uint64_t* S_off;      // W+1 dense per-window matrix offsets (D_w^2 each)
+    const uint64_t* linv_off;   // W+1 offsets of the per-window diagonal-tile inverses
+    const uint32_t* grp_win;    // window of each group
+    const uint32_t* blk_win;    // window of each block
+    const int32_t* gauge_cam;   // W, global baseline camera or -1
+    const double* scene;        // W, scene scale
+};
+// LM control of one window (sparse/self_calibration/mod.rs:212-639 for a single-stage loss and
+// the direct camera step).
+struct LmWin {
+    double nu, gain, current, pmr, initial_cost, initial_sq, step_norm;
+    int iterations, accepted, target_accepted, status;
+    int done, go, accepted_now, has_gain, have_initial;
+    int rej[4];
+};
+struct LmParams {
+    uint32_t max_iterations;
+    double ftol, gtol, ptol;
+};
+struct WinTrace {  // [iteration * W + window], optional
+    double* cost;
+    double* lambda;
+    double* candidate_cost;
+    uint8_t* accepted;
+};
+void launch_win_gauge(hipStream_t s, const WinTopo& wt, const double* t, GaugeDev* out);
+void launch_win_reduce_points(hipStream_t s, const WinTopo& wt, const double* pscal, AsmScalars* out);
+void launch_win_zero(hipStream_t s, double* p, size_t n);
+void launch_win_scatter(hipStream_t s, const Topo& tp, const WinTopo& wt, uint32_t npairs, const uint32_t* pair_a,
+                        const uint32_t* pair_b, const uint64_t* value_off, const double* u_vals,
+                        const double* s_vals, double* S);
+// lambda: damping per window (Topo::win_lambda carries the same array to the point elimination).
+void launch_win_finalize(hipStream_t s, const Topo& tp, const WinTopo& wt, const double* lambda, double* diag,
+                         const double* full_grad, const double* grad, double* S, AsmScalars* out, bool damp);
+// Per window: dense Cholesky of S_w (FP64, T = double) or of a Jacobi-scaled FP32 copy with FP64
+// iterative refinement (T = float), solving S_w x = -g_w; one workgroup per window, tile edge tb
+// (32 or 64, chosen by the autotuner).  Windows with go == 0 are skipped.
+// The mixed form (T = float) then re-solves, with the FP64 factor of S_w, every window whose
+// refinement stayed above 1e-10 relative residual (linv64: FP64 tile inverses; fallbacks counted
+// per window).
+template <class T>
+void launch_win_cholesky(hipStream_t s, const WinTopo& wt, int tb, double* S, T* F, T* linv, const double* grad,
+                         double* x, double* vecs, const LmWin* lm, int* fail, double* refine_rel,
+                         double* linv64, uint32_t* fallbacks);
+void launch_win_step_dots(hipStream_t s, const WinTopo& wt, const double* g, const double* diag, const double* x,
+                          double* dots);
+void launch_win_reduce_cand(hipStream_t s, const WinTopo& wt, const double* cscal, CandScalars* out);
+void launch_lm_begin(hipStream_t s, const WinTopo& wt, LmWin* lm, double* lambda, const AsmScalars* a, LmParams P,
+                     WinTrace tr);
+void launch_lm_solved(hipStream_t s, const WinTopo& wt, LmWin* lm, double* lambda, const int* fail, LmParams P);
+void launch_lm_end(hipStream_t s, const WinTopo& wt, LmWin* lm, double* lambda, const CandScalars* c,
+                   const double* dots, LmParams P, WinTrace tr);
+void launch_win_accept(hipStream_t s, const Topo& tp, const WinTopo& wt, const LmWin* lm, double* R0, double* t0,
+                       double* gp0, double* X0, const double* R1, const double* t1, const double* gp1,
+                       const double* X1);
+// Gauge state of one camera's translation (schur/layout.rs:101-113).
+__host__ __device__ inline GaugeDev make_gauge(const double* t3, int camera, double scene_scale) {
+    GaugeDev g;
+    g.camera = camera;
+    g._pad = 0;
+    g.scene_scale = scene_scale;
+    g.baseline = 0.0;
+    for (int k = 0; k < 3; ++k) g.direction[k] = 0.0;
+    for (int k = 0; k < 6; ++k) g.basis[k] = 0.0;
+    if (camera < 0) return g;
+    const double t[3] = {t3[0], t3[1], t3[2]};
+    const double b = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    g.baseline = b;
+    const double dir[3] = {t[0] / b, t[1] / b, t[2] / b};
+    for (int k = 0; k < 3; ++k) g.direction[k] = dir[k];
+    // s2_tangent_basis (numerics/manifold.rs:129-141)
+    const double n = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    const double tv[3] = {dir[0] / n, dir[1] / n, dir[2] / n};
+    double ref[3] = {0, 0, 0};
+    const double a0 = fabs(tv[0]), a1 = fabs(tv[1]), a2 = fabs(tv[2]);
+    if (a0 <= a1 && a0 <= a2)
+        ref[0] = 1.0;
+    else if (a1 <= a2)
+        ref[1] = 1.0;
+    else
+        ref[2] = 1.0;
+    const double c0[3] = {tv[1] * ref[2] - tv[2] * ref[1], tv[2] * ref[0] - tv[0] * ref[2], tv[0] * ref[1] - tv[1] * ref[0]};
+    const double n0 = sqrt(c0[0] * c0[0] + c0[1] * c0[1] + c0[2] * c0[2]);
+    const double b0[3] = {c0[0] / n0, c0[1] / n0, c0[2] / n0};
+    const double c1[3] = {tv[1] * b0[2] - tv[2] * b0[1], tv[2] * b0[0] - tv[0] * b0[2], tv[0] * b0[1] - tv[1] * b0[0]};
+    const double n1 = sqrt(c1[0] * c1[0] + c1[1] * c1[1] + c1[2] * c1[2]);
+    for (int k = 0; k < 3; ++k) {
+        g.basis[k] = b0[k];
+        g.basis[3 + k] = c1[k] / n1;
+    }
+    return g;
+}
 
 // ---- dense Cholesky (kernel (d)) ----
 // Factor the lower triangle of the n x n row-major matrix A in place (MFMA FP64 trailing

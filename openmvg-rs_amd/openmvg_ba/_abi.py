@@ -222,3 +222,16 @@ def vptr(array) -> int | None:
 
 def message_buffer(size: int = 512):
     return C.create_string_buffer(size)
+
+
+BATCH_GRAPH = 0
+BATCH_STREAM = 1
+
+
+class BatchTrace(C.Structure):
+    _fields_ = [
+        ("capacity", C.c_uint32), ("cost", C.POINTER(C.c_double)), ("damping", C.POINTER(C.c_double)),
+        ("candidate_cost", C.POINTER(C.c_double)), ("accepted", C.POINTER(C.c_uint8)),
+        ("tile", C.c_uint32), ("seconds_loop", C.c_double), ("autotune_seconds", C.c_double * 2),
+    ]
+

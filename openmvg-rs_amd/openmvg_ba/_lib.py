@@ -42,6 +42,7 @@ def load() -> C.CDLL:
     lib.ba_hip_reprojection_topology_create.argtypes = [vp, sz, sz, sz, vp, vp, C.POINTER(vp), C.c_char_p, sz]
     lib.ba_hip_reprojection_topology_destroy.argtypes = [vp]
     lib.ba_hip_reprojection_topology_destroy.restype = None
+    lib.ba_hip_solve_batched.argtypes = [vp, sz, vp, vp, vp, vp, vp, C.c_int32, C.c_char_p, sz]
     lib.ba_hip_cost_grad.argtypes = [vp, vp, C.c_int32, vp, vp, vp, C.c_int32, C.c_double, vp, vp, vp, vp, vp, vp,
                                      C.c_char_p, sz]
     _lib = lib

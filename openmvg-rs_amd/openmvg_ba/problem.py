@@ -254,11 +254,8 @@ class Solution:
     trace: dict = field(default_factory=dict)
 
 
-def solve_with(lib, function: str, problem: BundleProblem, config: SolveConfig, *handle,
-               trace_capacity: int | None = None) -> Solution:
-    """Call a `ba_solve`-shaped C entry point of `lib` and wrap its outputs."""
-    cp, keep = problem.to_c()
-    cfg = config.to_c()
+def _solution_buffers(problem: BundleProblem, sol: A.SolutionC) -> dict:
+    """Caller-allocated solution arrays for `problem`, wired into `sol`."""
     P, N, G = problem.n_poses, problem.n_points, len(problem.intrinsics)
     out = dict(
         pose_rotation=np.zeros((P, 3, 3)), pose_center=np.zeros((P, 3)),
@@ -266,12 +263,26 @@ def solve_with(lib, function: str, problem: BundleProblem, config: SolveConfig, 
         point_xyz=np.zeros((N, 3)), point_quarantined=np.zeros(N, dtype=np.uint8),
         intrinsic_unconstrained=np.zeros(G, dtype=np.uint8),
     )
-    sol = A.SolutionC()
     for name in ("pose_rotation", "pose_center", "intrinsic_focal", "intrinsic_k1", "intrinsic_k2",
                  "point_xyz"):
         setattr(sol, name, A.ptr(out[name], C.c_double))
     sol.point_quarantined = A.ptr(out["point_quarantined"], C.c_uint8)
     sol.intrinsic_unconstrained = A.ptr(out["intrinsic_unconstrained"], C.c_uint8)
+    return out
+
+
+def _report(rep: A.SolveReportC) -> SolveReport:
+    fields = {name: getattr(rep, name) for name, _ in A.SolveReportC._fields_}
+    return SolveReport(status=A.STATUS_NAMES.get(rep.status, str(rep.status)), fields=fields)
+
+
+def solve_with(lib, function: str, problem: BundleProblem, config: SolveConfig, *handle,
+               trace_capacity: int | None = None) -> Solution:
+    """Call a `ba_solve`-shaped C entry point of `lib` and wrap its outputs."""
+    cp, keep = problem.to_c()
+    cfg = config.to_c()
+    sol = A.SolutionC()
+    out = _solution_buffers(problem, sol)
     rep = A.SolveReportC()
     cap = trace_capacity if trace_capacity is not None else max(config.max_iterations, 1)
     tr_arrays = dict(cost=np.full(cap, np.nan), damping=np.full(cap, np.nan),
@@ -290,8 +301,7 @@ def solve_with(lib, function: str, problem: BundleProblem, config: SolveConfig, 
     code = fn(*handle, C.byref(cp), C.byref(cfg), C.byref(sol), C.byref(rep), C.byref(tr), msg, len(msg))
     del keep
     raise_for(code, msg.value)
-    fields = {name: getattr(rep, name) for name, _ in A.SolveReportC._fields_}
-    report = SolveReport(status=A.STATUS_NAMES.get(rep.status, str(rep.status)), fields=fields)
+    report = _report(rep)
     n = tr.count
     trace = {k: v[:n].copy() for k, v in tr_arrays.items()}
     return Solution(out["pose_rotation"], out["pose_center"], out["intrinsic_focal"],
@@ -404,6 +414,40 @@ def obs_point_index(point_offsets: np.ndarray) -> np.ndarray:
     return np.repeat(np.arange(len(counts), dtype=np.uint32), counts)
 
 
+def solve_batched_with(lib, handle, problems: list, config: SolveConfig, mode: int = A.BATCH_GRAPH,
+                       trace_capacity: int | None = None):
+    """ba_hip_solve_batched over `problems`: a list of Solutions (each with its report) and a dict
+    with the batch trace ([iteration, window] arrays, the chosen tile and loop time)."""
+    W = len(problems)
+    cps = [p.to_c() for p in problems]
+    arr = (A.Problem * W)(*[c for c, _ in cps])
+    sols = (A.SolutionC * W)()
+    outs = [_solution_buffers(p, sols[w]) for w, p in enumerate(problems)]
+    reps = (A.SolveReportC * W)()
+    cap = trace_capacity if trace_capacity is not None else config.max_iterations
+    tr_arrays = dict(cost=np.full((cap, W), np.nan), damping=np.full((cap, W), np.nan),
+                     candidate_cost=np.full((cap, W), np.nan), accepted=np.zeros((cap, W), dtype=np.uint8))
+    tr = A.BatchTrace()
+    tr.capacity = cap
+    for name in ("cost", "damping", "candidate_cost"):
+        setattr(tr, name, A.ptr(tr_arrays[name], C.c_double))
+    tr.accepted = A.ptr(tr_arrays["accepted"], C.c_uint8)
+    cfg = config.to_c()
+    msg = A.message_buffer()
+    code = lib.ba_hip_solve_batched(handle, W, arr, C.byref(cfg), sols, reps, C.byref(tr), mode, msg, len(msg))
+    del cps
+    raise_for(code, msg.value)
+    solutions = []
+    for w in range(W):
+        o = outs[w]
+        iters = reps[w].iterations
+        trace = {k: v[:iters, w].copy() for k, v in tr_arrays.items()}
+        solutions.append(Solution(o["pose_rotation"], o["pose_center"], o["intrinsic_focal"], o["intrinsic_k1"],
+                                  o["intrinsic_k2"], o["point_xyz"], o["point_quarantined"], _report(reps[w]), trace))
+    info = dict(tile=tr.tile, seconds_loop=tr.seconds_loop, autotune_seconds=tuple(tr.autotune_seconds))
+    return solutions, info
+
+
 class HipBundleAdjuster:
     """BundleAdjuster (solver.rs:1562-1573) backed by the MI355X library.
 
@@ -426,6 +470,12 @@ class HipBundleAdjuster:
 
     def prepare(self, problem: BundleProblem) -> NativeProblem:
         return prepare_with(self._handle.lib, "ba_hip_prepare", problem)
+
+    def solve_batched(self, problems: list, config: SolveConfig | None = None, mode: int = A.BATCH_GRAPH,
+                      trace_capacity=None):
+        """Many independent windows in one batched solve (direct step): (solutions, info)."""
+        config = config or SolveConfig(linear_solver=A.SOLVER_DIRECT)
+        return solve_batched_with(self._handle.lib, self._handle.ptr, problems, config, mode, trace_capacity)
 
     @property
     def handle(self):

@@ -251,6 +251,8 @@ def grid_scene(seed: int = 0xBA0002, nx: int = 20, ny: int = 10, n_points: int =
                   rng.uniform(8.0, 15.0, n_points)], -1)
     if track == "geometric":
         cap = np.clip(rng.geometric(1.0 / 6.0, n_points), 2, 50)
+    elif track == "window":
+        cap = rng.integers(2, 9, n_points)
     else:
         cap = 2 + rng.binomial(10, 0.4, n_points)
     op, opt, pix, kept = _observe_grid(R, C, f, k1, k2, X, rng, cap, noise, outlier_fraction, nx, ny)
@@ -264,6 +266,16 @@ def grid_scene(seed: int = 0xBA0002, nx: int = 20, ny: int = 10, n_points: int =
 
     return _finish(f"grid{m}", R, C, f, k1, k2, X, op, opt, pix, kept, init, A.CAMERA_BROWN_K1K2, True,
                    loss=loss, loss_scale=loss_scale)
+
+
+def window_scenes(seed: int = 0xBA0004, n_windows: int = 512, cameras=(5, 40), points=(200, 5000)) -> list:
+    """Local bundle-adjustment windows: window w (seed + w) has U{5..40} cameras on a line
+    (spacing 1, looking along +z; a stretch of trajectory), U{200..5000} points at depth 8–15 and
+    2–8 views per point; free f, k1, k2; squared loss."""
+    rng = np.random.default_rng(seed)
+    sizes = [(int(rng.integers(cameras[0], cameras[1] + 1)), int(rng.integers(points[0], points[1] + 1)))
+             for _ in range(n_windows)]
+    return [grid_scene(seed + 1 + w, m, 1, npts, track="window") for w, (m, npts) in enumerate(sizes)]
 
 
 def medium_scene(seed: int = 0xBA0002) -> Scene:

@@ -1,0 +1,120 @@
+"""Batched form: many local windows in one batched solve, against the single-window solve and the
+oracle, through the C ABI, on an MI355X.
+
+A window's batched result must equal solving that window alone: FP64 per-iteration cost within
+1e-10 relative, the same accept/reject sequence and status, final parameters within 1e-8
+relative (the plain form's bar); graph-replayed and stream-issued iterations agree bit for bit;
+the mixed form states its error against the FP64 form (final cost within 1e-4 relative after ten
+iterations; the measured values are printed with -s).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from openmvg_ba import SolveConfig, synthetic
+from openmvg_ba import _abi as A
+from openmvg_ba.problem import HipBundleAdjuster, solve_with
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def adjuster(torch_cuda):
+    return HipBundleAdjuster(0)
+
+
+@pytest.fixture(scope="module")
+def windows():
+    return synthetic.window_scenes(0xBA0004, 24, cameras=(5, 40), points=(200, 1500))
+
+
+CFG = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT)
+
+
+def _same(a, b, tol_cost=1e-10, tol_param=1e-8):
+    ta, tb = a.trace, b.trace
+    assert len(ta["cost"]) == len(tb["cost"]), (ta["cost"], tb["cost"])
+    assert np.array_equal(ta["accepted"], tb["accepted"])
+    fin = np.isfinite(tb["cost"])
+    assert np.all(np.abs(ta["cost"][fin] - tb["cost"][fin]) <= tol_cost * np.abs(tb["cost"][fin]))
+    assert a.report.status == b.report.status
+    assert abs(a.report.final_cost - b.report.final_cost) <= tol_cost * abs(b.report.final_cost)
+    for name in ("pose_center", "point_xyz", "intrinsic_focal", "intrinsic_k1", "intrinsic_k2", "pose_rotation"):
+        x, y = getattr(a, name), getattr(b, name)
+        assert np.all(np.abs(x - y) <= tol_param * np.maximum(np.abs(y), 1.0)), name
+
+
+def test_batched_windows_match_single_solves(adjuster, windows):
+    got, info = adjuster.solve_batched([w.problem for w in windows], CFG)
+    print(f"tile {info['tile']}, autotune {info['autotune_seconds']}, loop {info['seconds_loop']:.4f} s")
+    for w, scene in enumerate(windows):
+        want = adjuster.solve(scene.problem, CFG)
+        _same(got[w], want)
+
+
+def test_batched_windows_match_the_oracle(adjuster, oracle, windows):
+    sub = windows[:3]
+    got, _ = adjuster.solve_batched([w.problem for w in sub], CFG)
+    for w, scene in enumerate(sub):
+        want = solve_with(oracle, "bao_solve", scene.problem, CFG)
+        _same(got[w], want)
+
+
+def test_graph_replay_equals_stream_launches(adjuster, windows):
+    probs = [w.problem for w in windows[:8]]
+    a, _ = adjuster.solve_batched(probs, CFG, mode=A.BATCH_GRAPH)
+    b, _ = adjuster.solve_batched(probs, CFG, mode=A.BATCH_STREAM)
+    for x, y in zip(a, b):
+        assert np.array_equal(x.trace["cost"], y.trace["cost"])
+        assert np.array_equal(x.point_xyz, y.point_xyz)
+        assert np.array_equal(x.pose_center, y.pose_center)
+
+
+def _cost_errors(got, want):
+    worst_it, worst_final = 0.0, 0.0
+    for g, w in zip(got, want):
+        n = min(len(g.trace["cost"]), len(w.trace["cost"]))
+        fin = np.isfinite(w.trace["cost"][:n]) & np.isfinite(g.trace["cost"][:n])
+        if fin.any():
+            worst_it = max(worst_it, float(np.max(np.abs(g.trace["cost"][:n][fin] - w.trace["cost"][:n][fin]) /
+                                                  np.abs(w.trace["cost"][:n][fin]))))
+        worst_final = max(worst_final, abs(g.report.final_cost - w.report.final_cost) / abs(w.report.final_cost))
+    return worst_it, worst_final
+
+
+def test_batched_mixed_precision_error(adjuster, windows):
+    """Stated error of the mixed form after 10 LM iterations of each window: against the FP64 form
+    the trajectories part where FP32 rounding flips an accept/reject decision, so the bound is on
+    the cost reached (final cost within 1e-4 relative); the batched mixed solve and the
+    single-window mixed solve agree to the same bound."""
+    probs = [w.problem for w in windows[:12]]
+    cfg = CFG.with_(precision=A.PRECISION_MIXED)
+    want, _ = adjuster.solve_batched(probs, CFG)
+    got, _ = adjuster.solve_batched(probs, cfg)
+    single = [adjuster.solve(p, cfg) for p in probs]
+    it_b, fin_b = _cost_errors(got, want)
+    it_s, fin_s = _cost_errors(single, want)
+    fallbacks = sum(g.report.precision_fallbacks for g in got)
+    steps = sum(g.report.iterations for g in got)
+    print(f"mixed batched vs FP64: per-iteration cost {it_b:.2e}, final cost {fin_b:.2e}; "
+          f"mixed single vs FP64: {it_s:.2e}, {fin_s:.2e}; FP64 fallbacks {fallbacks} of {steps} steps")
+    assert fin_b <= 1e-4 and fin_s <= 1e-4
+    assert max(abs(g.report.final_cost - s.report.final_cost) / abs(s.report.final_cost)
+               for g, s in zip(got, single)) <= 1e-4
+
+
+def test_batched_rejects_iterative_solver(adjuster, windows):
+    from openmvg_ba.problem import Unsupported
+
+    with pytest.raises(Unsupported):
+        adjuster.solve_batched([windows[0].problem], CFG.with_(linear_solver=A.SOLVER_ITERATIVE))
