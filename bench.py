@@ -132,6 +132,59 @@ def bench_mixed(ba, prob, base, steps: int, ttc: dict | None) -> dict:
     return out
 
 
+def bench_medium(ba) -> dict:
+    """Direct form on the 200-camera / 50,000-point scene (D = 1,793): LM iterations/s over ten
+    iterations in FP64 and mixed precision, and the dense Cholesky (kernel d) rate on the matrix
+    cores from HIP events around its launches (D^3/3 + 2 D^2 flops per factor and solve)."""
+    from openmvg_ba import SolveConfig, _abi as A, synthetic
+
+    scene = synthetic.medium_scene()
+    prob = scene.problem
+    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT)
+    out = {"workload": f"{prob.n_poses} cameras, {prob.n_points} points, {prob.n_observations} observations, "
+                       "free f/k1/k2, squared loss, direct Schur + dense Cholesky, seed 0xBA0002"}
+    ba.solve(prob, cfg.with_(max_iterations=1))
+    for name, prec in (("fp64", A.PRECISION_FP64), ("mixed", A.PRECISION_MIXED)):
+        sol = ba.solve(prob, cfg.with_(precision=prec, profile_kernel=A.PROFILE_CHOLESKY))
+        rep = sol.report
+        D = rep.reduced_dimension
+        avg = rep.profile_seconds / max(rep.profile_launches, 1)
+        flops = D ** 3 / 3.0 + 2.0 * D * D
+        out[name] = {"lm_iter_per_s": rep.iterations / rep.seconds_iterations,
+                     "ms_per_step": 1e3 * rep.seconds_iterations / max(rep.iterations, 1),
+                     "cholesky_ms": 1e3 * avg, "cholesky_tflops": flops / avg / 1e12,
+                     "cholesky_peak_tflops": 78.6 if name == "fp64" else 157.3,
+                     "final_cost": rep.final_cost, "precision_fallbacks": int(rep.precision_fallbacks)}
+    return out
+
+
+def bench_batched(ba, sample_windows: int = 64) -> dict:
+    """512 local windows (seed 0xBA0004: U{5..40} cameras, U{200..5000} points, 2-8 views), ten LM
+    iterations each with the direct step: one batched solve (graph-replayed iterations) in FP64 and
+    mixed precision, against one solve per window (the first `sample_windows` windows, scaled)."""
+    from openmvg_ba import SolveConfig, _abi as A, synthetic
+
+    probs = [w.problem for w in synthetic.window_scenes()]
+    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT)
+    out = {"workload": f"{len(probs)} windows, {sum(p.n_observations for p in probs)} observations, "
+                       "10 LM iterations each"}
+    for name, prec in (("fp64", A.PRECISION_FP64), ("mixed", A.PRECISION_MIXED)):
+        sols, info = ba.solve_batched(probs, cfg.with_(precision=prec), mode=A.BATCH_GRAPH)
+        its = sum(s.report.iterations for s in sols)
+        out[name] = {"loop_seconds": info["seconds_loop"], "window_iterations_per_s": its / info["seconds_loop"],
+                     "tile": info["tile"], "autotune_seconds": list(info["autotune_seconds"])}
+    loop = 0.0
+    its = 0
+    for p in probs[:sample_windows]:
+        s = ba.solve(p, cfg)
+        loop += s.report.seconds_iterations
+        its += s.report.iterations
+    out["per_window_launches"] = {"window_iterations_per_s": its / loop,
+                                  "sample": f"first {sample_windows} windows solved one by one (FP64)"}
+    out["speedup_batched_vs_per_window"] = out["fp64"]["window_iterations_per_s"] / (its / loop)
+    return out
+
+
 def bench_differentiation(steps: int) -> dict:
     """Forward+backward observations/s of the differentiable reprojection cost: 1,000 cameras,
     200,000 points, 1,000,000 observations, Huber 1 px (seed 0xBA0005), FP32 and FP64, through
@@ -182,6 +235,9 @@ def main() -> int:
     ap.add_argument("--cpu-iterations", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms' measurements")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first copy BYTES between two device buffers three times (a known-size kernel that "
+                         "scales HBM counters in a profiled run)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -196,6 +252,16 @@ def main() -> int:
         return 1
     from openmvg_ba import HipBundleAdjuster, SolveConfig, _abi as A
     from openmvg_ba import synthetic
+
+    if args.copy_calibration > 0:
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {4 * n} bytes")
+        del src, dst
 
     t0 = time.perf_counter()
     scene = synthetic.large_scene(n_points=args.points)
@@ -260,6 +326,10 @@ def main() -> int:
     if not args.no_forms:
         forms["mixed"] = bench_mixed(ba, prob, base, args.steps, ttc)
         log(f"mixed: {forms['mixed']}")
+        forms["medium_direct"] = bench_medium(ba)
+        log(f"medium direct: {forms['medium_direct']}")
+        forms["batched"] = bench_batched(ba)
+        log(f"batched: {forms['batched']}")
         forms["differentiation"] = bench_differentiation(20)
         log(f"differentiation: {forms['differentiation']}")
 
