@@ -34,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "openmvg-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
+PCG_SEG = 256  # observations per camera segment of the PCG camera pass
 
 
 def log(msg: str) -> None:
@@ -46,29 +47,34 @@ def lowered_counts(problem) -> dict:
     n_obs = problem.n_observations
     n_pts = problem.n_points
     n_cams = problem.n_poses
-    return {"obs": n_obs, "points": n_pts, "cameras": n_cams}
+    per_cam = np.bincount(problem.obs_pose, minlength=n_cams)
+    segments = int(np.sum((per_cam + PCG_SEG - 1) // PCG_SEG))
+    return {"obs": n_obs, "points": n_pts, "cameras": n_cams, "segments": segments}
 
 
 def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     """Algorithmic HBM bytes of one launch of `kernel` in FP64: each input read once, each output
     written once; vectors indexed by camera or point are counted once per launch.
 
+    The product passes read per observation only J~_X (2x3) and the intrinsic columns of J~_c
+    (2x3), 96 B; the pose columns are rebuilt from J~_X, the point and the camera rotation.
     pcg_points (k_pcg_points: w_p = V^-1 sum_o J~_X,o^T J~_c,o v, one workgroup per point chunk):
-      per observation J~_X (48 B) and J~_c (144 B) from the point-major copy and its camera index
-      (4 B); per point its offset (4 B), fixed flag (1 B), packed V^-1 (48 B) and the written w_p
-      (24 B); the vector v (8 D).
-    pcg_cameras (k_pcg_cameras: P_c = sum_o J~_c^T (J~_c v - J~_X w_p), one workgroup per camera):
-      per observation J~_X and J~_c from the camera-major copy (192 B) and its point index (4 B);
-      w once per point (24 B); P written (72 B per camera); v (8 D).
+      per observation the 12 rows (96 B) and its camera and point indices (8 B); per point its
+      offset (4 B), fixed flag (1 B), packed V^-1 (48 B), X (24 B) and the written record X, w_p
+      (48 B); per camera the product input (96 B).
+    pcg_cameras (k_pcg_cameras: per camera segment of 256 observations the sums of
+      J~_c^T (J~_c v - J~_X w_p)): per observation the 12 rows of the camera-major copy (96 B) and
+      its point index (4 B); per point the record X, w_p once (48 B); per segment its camera (4 B)
+      and the written sums (72 B); per camera offsets (12 B) and the product input (96 B).
     linearize (k_linearize): per observation pixel (16 B), camera and point index (8 B), the
       structure-of-arrays record r~, J~_X, J~_c (208 B), cost, squared residual and flag (17 B);
       per point X (24 B); per camera R, t, f, k1, k2 (120 B).
     """
     no, npt, nc = counts["obs"], counts["points"], counts["cameras"]
     if kernel == "pcg_points":
-        return no * (192 + 4) + npt * (4 + 1 + 48 + 24) + 8 * D
+        return no * (96 + 8) + npt * (4 + 1 + 48 + 24 + 48) + nc * 96
     if kernel == "pcg_cameras":
-        return no * (192 + 4) + npt * 24 + nc * 72 + 8 * D
+        return no * (96 + 4) + npt * 48 + counts["segments"] * (4 + 72) + nc * (12 + 96)
     if kernel == "linearize":
         return no * (16 + 8 + 208 + 17) + npt * 24 + nc * 120
     raise ValueError(kernel)

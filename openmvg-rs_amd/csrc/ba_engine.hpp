@@ -238,16 +238,19 @@ class Engine {
             edge_pair_.upload(ep, s_);
             factors_.alloc(54 * B_.n_blocks);
             jcm_.alloc(SOA_ROWS * no);
-            pP_.alloc(9 * nc);
-            pvcam_.alloc(9 * nc);
-            ppart_.alloc(std::max<size_t>(1, (B_.n_blocks + 255) / 256));
+            pseg_.alloc(9 * std::max<size_t>(1, nseg_));
+            pvcam_.alloc(PCG_VCAM * nc);
+            ppart_.alloc(std::max<size_t>(1, (B_.n_blocks + PCG_Q_WG / 64 - 1) / (PCG_Q_WG / 64)) +
+                         2 * std::max<size_t>(1, (B_.n_blocks + PCG_BLOCK_WG - 1) / PCG_BLOCK_WG));
             px_.alloc(D);
             pr_.alloc(D);
             pz_.alloc(D);
-            pd_.alloc(D);
+            pd_[0].alloc(D);
+            pd_[1].alloc(D);
             pq_.alloc(D);
-            pw_.alloc(3 * np);
+            pxw_.alloc(PCG_XW * np);
             pcg_.alloc(1);
+            pcg1_.alloc(1);
         }
         tp_.nc = (uint32_t)nc;
         tp_.ng = (uint32_t)ng;
@@ -338,6 +341,16 @@ class Engine {
         cm_off_.upload(off, s_);
         cm_obs_.upload(obs, s_);
         cm_pt_.upload(pts, s_);
+        // camera segments of the PCG camera pass
+        std::vector<uint32_t> cseg(nc + 1, 0), segc;
+        for (size_t c = 0; c < nc; ++c) {
+            const uint32_t n = off[c + 1] - off[c], k = (n + PCG_SEG - 1) / PCG_SEG;
+            cseg[c + 1] = cseg[c] + k;
+            for (uint32_t i = 0; i < k; ++i) segc.push_back((uint32_t)c);
+        }
+        nseg_ = (uint32_t)segc.size();
+        cseg_off_.upload(cseg, s_);
+        seg_cam_.upload(segc, s_);
         HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
     }
 
@@ -345,16 +358,28 @@ class Engine {
         PcgVecs V;
         V.grad = grad_.p;
         V.factors = factors_.p;
+        V.diag = diag_.p;
+        V.R = R_[lin_state_].p;
+        V.X = X_[lin_state_].p;
+        V.gauge = gauge_.p;
+        V.lambda = lambda_;
         V.x = px_.p;
         V.r = pr_.p;
         V.z = pz_.p;
-        V.d = pd_.p;
         V.q = pq_.p;
-        V.w = pw_.p;
-        V.P = pP_.p;
+        V.d[0] = pd_[0].p;
+        V.d[1] = pd_[1].p;
+        V.xw = pxw_.p;
+        V.seg = pseg_.p;
         V.vcam = pvcam_.p;
         V.part = ppart_.p;
+        V.nparts = pcg_partials(tp_);
+        V.nqparts = pcg_q_partials(tp_);
+        V.nseg = nseg_;
+        V.seg_cam = seg_cam_.p;
+        V.cseg_off = cseg_off_.p;
         V.st = pcg_.p;
+        V.st1 = pcg1_.p;
         return V;
     }
 
@@ -373,6 +398,7 @@ class Engine {
         Topo tp = tp_;
         tp.loss = loss;
         tp.loss_scale = loss_scale;
+        lin_state_ = state;
         if (!fixed_gauge_) launch_gauge<double>(s_, t_[state].p, L_.gauge_camera, L_.scene_scale, gauge_.p);
         timed(BA_PROFILE_LINEARIZE, [&] {
             launch_linearize<T>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
@@ -474,8 +500,7 @@ class Engine {
             for (int k = 0; k < batch; ++k) {
                 timed(BA_PROFILE_PCG_POINTS, [&] { launch_pcg_points<T>(s_, tp_, lin(), pe(), V); });
                 timed(BA_PROFILE_PCG_CAMERAS, [&] { launch_pcg_cameras<T>(s_, tp_, lin(), V); });
-                launch_pcg_blocks(s_, tp_, lambda_, diag_.p, px_.p, pd_.p, pP_.p, pq_.p, pcg_.p);
-                launch_pcg_update(s_, tp_, V);
+                launch_pcg_step(s_, tp_, V);
             }
         }
         *terms = (uint32_t)h_pcg_.p->iter;
@@ -545,12 +570,13 @@ class Engine {
     Topo tp_{};
     Buf<uint32_t> obs_cam_, obs_pt_, pt_off_, cam_group_, cam_off_, cam_dim_, grp_off_, grp_dim_, blk_off_, blk_dim_,
         blk_obs_off_, blk_obs_, grp_cam_off_, grp_cams_, pair_a_, pair_b_, term_off_, term_i_, term_j_, chunk_pt_,
-        chunk_obs_, cm_off_, cm_obs_, cm_pt_;
+        chunk_obs_, cm_off_, cm_obs_, cm_pt_, cseg_off_, seg_cam_;
+    uint32_t nseg_ = 0;
     Buf<uint8_t> pt_fixed_, valid_;
     Buf<uint64_t> value_off_;
     Buf<int32_t> diag_pair_, edge_pair_;
     Buf<double> uv_, R_[2], t_[2], gp_[2], X_[2], ocost_, osq_, pscal_, cscal_, grad_, fgrad_, diag_, step_, rhs_,
-        partials_, u_vals_, s_vals_, S_, work_, factors_, px_, pr_, pz_, pd_, pq_, pw_, dots_, pP_, pvcam_, ppart_;
+        partials_, u_vals_, s_vals_, S_, work_, factors_, px_, pr_, pz_, pd_[2], pq_, pxw_, dots_, pseg_, pvcam_, ppart_;
     Buf<T> jc_, vinv_, vdamp_, y_, ddiag_, rrows_, jpm_, jcm_;
     Buf<float> S32_;  // FP32 factor of the mixed direct form
     Buf<int> done_;
@@ -558,7 +584,8 @@ class Engine {
     Buf<CandScalars> cand_;
     Buf<GaugeDev> gauge_;
     Buf<int> fail_;
-    Buf<PcgState> pcg_;
+    Buf<PcgState> pcg_, pcg1_;
+    int lin_state_ = 0;  // state of the last assembly (the PCG's linearisation point)
     Pinned<AsmScalars> h_asm_;
     Pinned<CandScalars> h_cand_;
     Pinned<int> h_int_;

@@ -1154,16 +1154,16 @@ __global__ void __launch_bounds__(1024) k_step_dots(uint32_t D, const double* g,
 // (c) PCG.  Preconditioner blocks: Jacobi-scaled Cholesky of U_AA(damped) - Schur_AA
 // (retained/block.rs:171-250); stored per block as 45 packed lower values + 9 scales.
 // ----------------------------------------------------------------------------------------------
-constexpr int FACT = 54;
 __device__ __forceinline__ int pidx(int r, int c) { return r >= c ? r * (r + 1) / 2 + c : c * (c + 1) / 2 + r; }
 
+// Factors are stored structure-of-arrays: value i of block A at factors[i * nblocks + A], so
+// that lane-per-block passes read them coalesced.
 __global__ void __launch_bounds__(NT) k_precond_factor(Topo tp, const int32_t* diag_pair, const uint64_t* value_off,
                                                        const double* u_vals, const double* s_vals, double* factors,
                                                        int* fail) {
     uint32_t A = blockIdx.x * NT + threadIdx.x;
     if (A >= tp.nblocks) return;
     const int n = (int)tp.blk_dim[A];
-    double* F = factors + FACT * (size_t)A;
     if (n == 0) return;
     int32_t pr = diag_pair[A];
     if (pr < 0) {
@@ -1182,8 +1182,7 @@ __global__ void __launch_bounds__(NT) k_precond_factor(Topo tp, const int32_t* d
         return;
     }
     double floor = 64.0 * DBL_EPSILON * ds;
-    double* L = F;
-    double* sc = F + 45;
+    double L[45], sc[9];
     for (int i = 0; i < 45; ++i) L[i] = 0.0;
     for (int i = 0; i < n; ++i) sc[i] = sqrt(fmax(m[pidx(i, i)], floor));
     for (int r = 0; r < n; ++r)
@@ -1200,117 +1199,510 @@ __global__ void __launch_bounds__(NT) k_precond_factor(Topo tp, const int32_t* d
                 L[pidx(r, c)] = v / L[pidx(c, c)];
             }
         }
+    const size_t st = tp.nblocks;
+    for (int i = 0; i < n * (n + 1) / 2; ++i) factors[i * st + A] = L[i];
+    for (int i = 0; i < n; ++i) factors[(45 + i) * st + A] = sc[i];
 }
 
 namespace {
-__device__ void precond_apply_block(const double* F, int n, const double* right, double* sol) {
-    const double* L = F;
-    const double* sc = F + 45;
-    for (int r = 0; r < n; ++r) {
+// z = M^-1 r for block A of dimension n <= 9 (Jacobi-scaled Cholesky factor, forward then
+// backward substitution in the order of retained/block.rs:171-250).  The factor is read into
+// registers first so that its loads are issued together.
+__device__ __forceinline__ void precond_apply_block(const double* __restrict__ factors, size_t nblocks, uint32_t A,
+                                                    int n, const double right[9], double sol[9]) {
+    double L[45], sc[9];
+    const int nl = n * (n + 1) / 2;
+#pragma unroll
+    for (int i = 0; i < 45; ++i) L[i] = i < nl ? factors[i * nblocks + A] : 1.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) sc[i] = i < n ? factors[(45 + i) * nblocks + A] : 1.0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
         double v = right[r] / sc[r];
+#pragma unroll
         for (int c = 0; c < r; ++c) v -= L[pidx(r, c)] * sol[c];
-        sol[r] = v / L[pidx(r, r)];
+        sol[r] = r < n ? v / L[pidx(r, r)] : 0.0;
     }
-    for (int r = n - 1; r >= 0; --r) {
+#pragma unroll
+    for (int r = 8; r >= 0; --r) {
         double v = sol[r];
-        for (int c = r + 1; c < n; ++c) v -= L[pidx(c, r)] * sol[c];
-        sol[r] = v / L[pidx(r, r)];
+#pragma unroll
+        for (int c = r + 1; c < 9; ++c) v -= L[pidx(c, r)] * sol[c];
+        sol[r] = r < n ? v / L[pidx(r, r)] : 0.0;
     }
-    for (int i = 0; i < n; ++i) sol[i] /= sc[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) sol[i] = i < n ? sol[i] / sc[i] : 0.0;
 }
-enum { PCG_NORMAL = 0, PCG_CONFIRM = 1, PCG_FINAL = 2 };
+// PCG_INIT: the state after k_pcg_init (r = -g, z = M^-1 r), before the first direction.
+enum { PCG_NORMAL = 0, PCG_CONFIRM = 1, PCG_FINAL = 2, PCG_INIT = 3 };
+constexpr int NTS = (int)PCG_BLOCK_WG;  // one lane per block
+constexpr int NQ = (int)PCG_Q_WG;        // k_pcg_q: one wavefront per block
 
-__device__ double dot_all(uint32_t D, const double* a, const double* b, double* sh) {
+// Sum of the per-workgroup partials part[0..nparts) in a fixed order, the same in every
+// workgroup (so each one derives the same step lengths and decisions).  Lane t adds
+// part[t], part[t + NTS], ... in that order (four loads in flight), then a fixed tree.
+__device__ double sum_parts(const double* __restrict__ part, uint32_t nparts, double* sh) {
     double acc = 0.0;
-    for (uint32_t i = threadIdx.x; i < D; i += 1024) acc += a[i] * b[i];
-    return tree_sum<1024>(acc, sh);
+    for (uint32_t base = 0; base < nparts; base += 4 * NTS) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t i = base + u * NTS + threadIdx.x;
+            v[u] = i < nparts ? part[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += v[u];
+    }
+    return tree_sum<NTS>(acc, sh);
 }
-// Per-camera copy of the next product's input vector (d in NORMAL mode, else x).
-__device__ void write_vcam(const Topo& tp, const PcgState& s, const PcgVecs& V) {
-    const double* v = s.mode == PCG_NORMAL ? V.d : V.x;
-    for (uint32_t c = threadIdx.x; c < tp.nc; c += 1024) {
-        double vc[9];
-        camera_values(tp, c, v, vc);
-        for (int k = 0; k < 9; ++k) V.vcam[9 * (size_t)c + k] = vc[k];
+
+// Butterfly sum over the 64 lanes of a wavefront: every lane ends with the same total (a + b and
+// b + a round alike), in an order fixed by the lane numbering.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// World-frame input of the product on camera c's Jacobian columns, from the values v of its pose
+// block (pd of 6) and of its intrinsic group (3).  With J~_X = sqrt(w) J_p R s (the point
+// Jacobian of the linearisation, J_p its camera-frame factor), the pose columns of J~_c are
+//   rotation    sqrt(w) J_p (-[R X]x) = J~_X (-[X]x) R^T / s,
+//   translation sqrt(w) J_p s         = J~_X R^T      (baseline camera: J~_X R^T (b/s) B),
+// so J~_c,pose v = J~_X (Om x X + Tr) with Om = R^T omega / s and Tr = R^T dt (baseline camera:
+// (b/s) R^T B dt).  vcam holds [Om, Tr, intrinsic values, 0, 0, 0].
+__device__ void write_vcam(const PcgVecs& V, const Topo& tp, uint32_t c, const double vp[6], const double vi[3]) {
+    const uint32_t pd = tp.cam_dim[c];
+    const GaugeDev& gg = *V.gauge;
+    const double* R = V.R + 9 * (size_t)c;
+    double om[3] = {0.0, 0.0, 0.0}, tr[3] = {0.0, 0.0, 0.0};
+    if (pd > 0) {
+        const double inv_s = 1.0 / gg.scene_scale;
+        double w[3] = {vp[0], vp[1], vp[2]};
+        rotate_t(R, w, om);
+        for (int k = 0; k < 3; ++k) om[k] *= inv_s;
+        double dt[3];
+        if ((int)c == gg.camera) {
+            const double f = gg.baseline * inv_s;
+            for (int k = 0; k < 3; ++k) dt[k] = f * (gg.basis[k] * vp[3] + gg.basis[3 + k] * vp[4]);
+        } else {
+            for (int k = 0; k < 3; ++k) dt[k] = vp[3 + k];
+        }
+        rotate_t(R, dt, tr);
+    }
+    double2* out = reinterpret_cast<double2*>(V.vcam + PCG_VCAM * (size_t)c);
+    out[0] = make_double2(om[0], om[1]);
+    out[1] = make_double2(om[2], tr[0]);
+    out[2] = make_double2(tr[1], tr[2]);
+    out[3] = make_double2(vi[0], vi[1]);
+    out[4] = make_double2(vi[2], 0.0);
+    out[5] = make_double2(0.0, 0.0);
+}
+
+// y = J~_c v for one observation from its J~_X (jx, 2x3) and intrinsic columns (ji, 2x3), the
+// camera's world-frame input (om, tr, vi) and the point X.
+__device__ __forceinline__ void compact_product(const double jx[6], const double ji[6], const double om[3],
+                                                const double tr[3], const double vi[3], const double X[3],
+                                                double* y0, double* y1) {
+    const double u0 = om[1] * X[2] - om[2] * X[1] + tr[0];
+    const double u1 = om[2] * X[0] - om[0] * X[2] + tr[1];
+    const double u2 = om[0] * X[1] - om[1] * X[0] + tr[2];
+    *y0 = jx[0] * u0 + jx[1] * u1 + jx[2] * u2 + ji[0] * vi[0] + ji[1] * vi[1] + ji[2] * vi[2];
+    *y1 = jx[3] * u0 + jx[4] * u1 + jx[5] * u2 + ji[3] * vi[0] + ji[4] * vi[1] + ji[5] * vi[2];
+}
+
+__device__ __forceinline__ void load_vcam(const double* vcam, uint32_t c, double om[3], double tr[3], double vi[3]) {
+    const double2* v = reinterpret_cast<const double2*>(vcam + PCG_VCAM * (size_t)c);
+    const double2 a = v[0], b = v[1], e = v[2], f = v[3], g = v[4];
+    om[0] = a.x;
+    om[1] = a.y;
+    om[2] = b.x;
+    tr[0] = b.y;
+    tr[1] = e.x;
+    tr[2] = e.y;
+    vi[0] = f.x;
+    vi[1] = f.y;
+    vi[2] = g.x;
+}
+
+// The observation's 12 product rows: J~_X (SoA rows SOA_JX..+5) and the intrinsic columns of
+// J~_c (rows SOA_JC+6..8 and SOA_JC+15..17).
+template <class T>
+__device__ __forceinline__ void load_compact_rows(const T* __restrict__ rows, size_t no, uint32_t j, double jx[6],
+                                                  double ji[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) jx[k] = (double)rows[(SOA_JX + k) * no + j];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ji[k] = (double)rows[(SOA_JC + 6 + k) * no + j];
+        ji[3 + k] = (double)rows[(SOA_JC + 15 + k) * no + j];
     }
 }
 }  // namespace
 
-// PCG control (pcg.rs:23-119 with the retained confirmation of self_calibration_retained.rs:
-// 512-579), split so that the block-Jacobi application runs over many workgroups:
-//   k_pcg_init_a / k_pcg_update_a (one workgroup): the step along d or the true-residual check;
-//   k_pcg_precond (one lane per block): z = M^-1 r and per-workgroup partials of r.z;
-//   k_pcg_update_b (one workgroup): the new direction and the per-camera input of the next product.
-__global__ void __launch_bounds__(1024) k_pcg_init_a(Topo tp, PcgVecs V, double tol, int max_iter) {
-    __shared__ double sh[1024];
-    const uint32_t D = tp.D;
-    for (uint32_t i = threadIdx.x; i < D; i += 1024) {
-        V.x[i] = 0.0;
-        V.q[i] = 0.0;
-        V.r[i] = -V.grad[i];
+// PCG (pcg.rs:23-119 with the retained confirmation of self_calibration_retained.rs:512-579).
+// One iteration is five launches; the vector work runs one lane per block over many workgroups,
+// and every workgroup derives the step lengths and the control decisions itself from the
+// previous launch's per-workgroup partial dot products (summed in a fixed order):
+//   k_pcg_points / k_pcg_cameras   the implicit Schur product (below);
+//   k_pcg_q     q = S v per block, partials of d.q;
+//   k_pcg_xr    alpha; x += alpha d, r -= alpha q (or the true residual r = -g - S x of a
+//               confirmation); z = M^-1 r; partials of r.r and r.z;
+//   k_pcg_dir   the convergence tests, beta, the next direction (into the other of the two
+//               direction buffers) and the per-camera input of the next product.
+// State: st is read by the product passes, k_pcg_q and k_pcg_xr and written by k_pcg_dir;
+// st1 is written by k_pcg_xr (and k_pcg_init) and read by k_pcg_dir.  No kernel reads the state
+// it writes, so no workgroup sees another's update.
+__global__ void __launch_bounds__(NTS) k_pcg_init(Topo tp, PcgVecs V, double tol, int max_iter) {
+    __shared__ double sh[NTS];
+    const uint32_t A = blockIdx.x * NTS + threadIdx.x;
+    double rr = 0.0, rz = 0.0;
+    if (A < tp.nblocks) {
+        const int n = (int)tp.blk_dim[A];
+        const uint32_t off = tp.blk_off[A];
+        double rv[9], sol[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) rv[k] = k < n ? -V.grad[off + k] : 0.0;
+        precond_apply_block(V.factors, tp.nblocks, A, n, rv, sol);
+        for (int k = 0; k < n; ++k) {
+            V.x[off + k] = 0.0;
+            V.q[off + k] = 0.0;
+            V.r[off + k] = rv[k];
+            V.z[off + k] = sol[k];
+            rr += rv[k] * rv[k];
+            rz += rv[k] * sol[k];
+        }
     }
-    __syncthreads();
-    double n0 = sqrt(dot_all(D, V.r, V.r, sh));
-    PcgState s;
-    s.tol = tol;
-    s.max_iter = max_iter;
-    s.iter = 0;
-    s.n0 = n0;
-    s.rel = NAN;
-    s.tr = NAN;
-    s.rho = 0.0;
-    s.done = 0;
-    s.accepted = 0;
-    s.mode = PCG_NORMAL;
-    s.products = 0;
-    s.need_z = 0;
-    s.restart = 0;
-    if (!isfinite(n0)) {
-        s.done = 1;
-    } else if (n0 == 0.0) {
-        s.done = 1;
-        s.accepted = 1;
-        s.rel = 0.0;
-    } else {
-        s.need_z = 1;  // z = M^-1 r, d = z, rho = r.z (restart form)
-        s.restart = 1;
+    rr = tree_sum<NTS>(rr, sh);
+    rz = tree_sum<NTS>(rz, sh);
+    if (threadIdx.x == 0) {
+        V.part[V.nqparts + blockIdx.x] = rr;
+        V.part[V.nqparts + V.nparts + blockIdx.x] = rz;
     }
-    if (threadIdx.x == 0) *V.st = s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        PcgState s;
+        s.tol = tol;
+        s.max_iter = max_iter;
+        s.iter = 0;
+        s.n0 = NAN;
+        s.rel = NAN;
+        s.tr = NAN;
+        s.rho = 0.0;
+        s.done = 0;
+        s.accepted = 0;
+        s.mode = PCG_INIT;
+        s.products = 0;
+        s.par = 0;
+        s._pad = 0;
+        *V.st1 = s;
+    }
 }
 
-__global__ void __launch_bounds__(1024) k_pcg_update_a(Topo tp, PcgVecs V) {
-    __shared__ double sh[1024];
-    const uint32_t D = tp.D;
-    PcgState s = *V.st;
+// Point pass: one workgroup per point chunk, one lane per observation;
+// e_o = J~_X,o^T (J~_c,o v) is summed per point in observation order and w_p = V^-1 sum e_o is
+// written with the point into xw (read by the camera pass).
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointElim<T> pe, PcgVecs V) {
+    __shared__ double sh[3][NT];
+    if (V.st->done) return;
+    const uint32_t k = blockIdx.x, t = threadIdx.x;
+    const uint32_t ob = tp.chunk_obs[k], oe = tp.chunk_obs[k + 1];
+    const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
+    const size_t no = tp.no;
+    // the point lanes' inputs, loaded first so that they are in flight with the observation pass
+    const bool plane = t < pend - pb;
+    const uint32_t p = pb + (plane ? t : 0);
+    uint32_t po0 = 0, po1 = 0;
+    bool fixed = true;
+    double m[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, Xp[3] = {0.0, 0.0, 0.0};
+    if (plane) {
+        po0 = tp.pt_off[p];
+        po1 = tp.pt_off[p + 1];
+        fixed = tp.pt_fixed[p] != 0;
+        const T* vi = pe.vinv + 6 * (size_t)p;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) m[q] = (double)vi[q];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) Xp[q] = V.X[3 * (size_t)p + q];
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    // lanes take observations ob + t, ob + t + NT, ... (one pass unless one point has > NT)
+    for (uint32_t o = ob + t; o < oe; o += NT) {
+        const uint32_t c = tp.obs_cam[o], q = tp.obs_pt[o];
+        double om[3], tr[3], vi[3], jx[6], ji[6];
+        load_vcam(V.vcam, c, om, tr, vi);
+        load_compact_rows(lin.jpm, no, o, jx, ji);
+        const double X[3] = {V.X[3 * (size_t)q], V.X[3 * (size_t)q + 1], V.X[3 * (size_t)q + 2]};
+        double y0, y1;
+        compact_product(jx, ji, om, tr, vi, X, &y0, &y1);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[i] += jx[i] * y0 + jx[3 + i] * y1;
+    }
+    sh[0][t] = acc[0];
+    sh[1][t] = acc[1];
+    sh[2][t] = acc[2];
+    __syncthreads();
+    if (oe - ob > NT) {  // a single point: fixed-shape tree over the lanes
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if ((int)t < s)
+                for (int i = 0; i < 3; ++i) sh[i][t] += sh[i][t + s];
+            __syncthreads();
+        }
+    }
+    if (!plane) return;
+    double s3[3] = {0.0, 0.0, 0.0};
+    if (oe - ob > NT) {
+        for (int i = 0; i < 3; ++i) s3[i] = sh[i][0];
+    } else {
+        for (uint32_t o = po0; o < po1; ++o)
+            for (int i = 0; i < 3; ++i) s3[i] += sh[i][o - ob];
+    }
+    double out[3] = {0.0, 0.0, 0.0};
+    if (!fixed) sym3_mul(m, s3, out);
+    double2* rec = reinterpret_cast<double2*>(V.xw + PCG_XW * (size_t)p);
+    rec[0] = make_double2(Xp[0], Xp[1]);
+    rec[1] = make_double2(Xp[2], out[0]);
+    rec[2] = make_double2(out[1], out[2]);
+}
+
+// Camera pass: one wavefront per camera segment (PCG_SEG camera-major observations of one
+// camera).  Per observation z_o = J~_c,o v - J~_X,o w_p and e_o = J~_X,o^T z_o; the segment's
+// sums E = sum e_o, F = sum X_p x e_o (world frame) and I = sum J~_int,o^T z_o go to V.seg, from
+// which k_pcg_q forms J~_c^T z = [R F / s; R E; I] (baseline camera: (b/s) B^T R E).
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_cameras(Topo tp, Lin<T> lin, PcgVecs V) {
+    if (V.st->done) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t k = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (k >= V.nseg) return;  // whole wavefronts leave; no workgroup barrier follows
+    const uint32_t c = V.seg_cam[k];
+    const uint32_t jb = tp.cm_off[c] + (k - V.cseg_off[c]) * PCG_SEG;
+    const uint32_t je = min(jb + PCG_SEG, tp.cm_off[c + 1]);
+    const size_t no = tp.no;
+    double om[3], tr[3], vi[3];
+    load_vcam(V.vcam, c, om, tr, vi);
+    double acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+#pragma unroll
+    for (uint32_t it = 0; it < PCG_SEG / 64; ++it) {
+        const uint32_t j0 = jb + lane + 64 * it;
+        const bool live = j0 < je;
+        const uint32_t j = live ? j0 : jb;  // in-range stand-in, its terms zeroed below
+        const uint32_t p = tp.cm_pt[j];
+        const double2* rec = reinterpret_cast<const double2*>(V.xw + PCG_XW * (size_t)p);
+        const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+        const double X[3] = {r0.x, r0.y, r1.x};
+        const double w[3] = {r1.y, r2.x, r2.y};
+        double jx[6], ji[6];
+        load_compact_rows(lin.jcm, no, j, jx, ji);
+        double y0, y1;
+        compact_product(jx, ji, om, tr, vi, X, &y0, &y1);
+        double z0 = y0 - (jx[0] * w[0] + jx[1] * w[1] + jx[2] * w[2]);
+        double z1 = y1 - (jx[3] * w[0] + jx[4] * w[1] + jx[5] * w[2]);
+        if (!live) z0 = z1 = 0.0;
+        const double e0 = jx[0] * z0 + jx[3] * z1, e1 = jx[1] * z0 + jx[4] * z1, e2 = jx[2] * z0 + jx[5] * z1;
+        acc[0] += e0;
+        acc[1] += e1;
+        acc[2] += e2;
+        acc[3] += X[1] * e2 - X[2] * e1;
+        acc[4] += X[2] * e0 - X[0] * e2;
+        acc[5] += X[0] * e1 - X[1] * e0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[6 + q] += ji[q] * z0 + ji[3 + q] * z1;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = wave_sum(acc[q]);
+    if (lane == 0) {
+        double* out = V.seg + 9 * (size_t)k;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) out[q] = acc[q];
+    }
+}
+
+// Block pass, one wavefront per block: q_A = (J~^T z rows of block A: its camera's segment
+// sums, or for an intrinsic block those of the group's cameras, added lane-strided and then by
+// the wavefront butterfly) + lambda diag_A v_A; one partial of v.q per workgroup.
+__global__ void __launch_bounds__(NQ) k_pcg_q(Topo tp, PcgVecs V) {
+    __shared__ double sh[NQ / 64];
+    const PcgState s = *V.st;
     if (s.done) return;
+    const double* __restrict__ v = s.mode == PCG_NORMAL ? V.d[s.par] : V.x;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t A = blockIdx.x * (NQ / 64) + wv;
+    double vq = 0.0;
+    if (A < tp.nblocks) {  // uniform over the wavefront
+        const uint32_t n = tp.blk_dim[A], off = tp.blk_off[A];
+        double P[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (A < tp.nc && n > 0) {
+            double E[3] = {0.0, 0.0, 0.0}, F[3] = {0.0, 0.0, 0.0};
+            for (uint32_t g = V.cseg_off[A] + lane; g < V.cseg_off[A + 1]; g += 64) {
+                const double* sg = V.seg + 9 * (size_t)g;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    E[i] += sg[i];
+                    F[i] += sg[3 + i];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                E[i] = wave_sum(E[i]);
+                F[i] = wave_sum(F[i]);
+            }
+            const GaugeDev& gg = *V.gauge;
+            const double* R = V.R + 9 * (size_t)A;
+            const double inv_s = 1.0 / gg.scene_scale;
+            double RF[3], RE[3];
+            rotate(R, F, RF);
+            rotate(R, E, RE);
+            for (int i = 0; i < 3; ++i) P[i] = RF[i] * inv_s;
+            if ((int)A == gg.camera) {
+                const double f = gg.baseline * inv_s;
+                P[3] = f * (gg.basis[0] * RE[0] + gg.basis[1] * RE[1] + gg.basis[2] * RE[2]);
+                P[4] = f * (gg.basis[3] * RE[0] + gg.basis[4] * RE[1] + gg.basis[5] * RE[2]);
+            } else {
+                for (int i = 0; i < 3; ++i) P[3 + i] = RE[i];
+            }
+        } else if (A >= tp.nc && n > 0) {
+            const uint32_t gr = A - tp.nc;
+            double I[3] = {0.0, 0.0, 0.0};
+            for (uint32_t i = tp.grp_cam_off[gr]; i < tp.grp_cam_off[gr + 1]; ++i) {
+                const uint32_t c = tp.grp_cams[i];
+                for (uint32_t g = V.cseg_off[c] + lane; g < V.cseg_off[c + 1]; g += 64) {
+                    const double* sg = V.seg + 9 * (size_t)g;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) I[j] += sg[6 + j];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j) P[j] = wave_sum(I[j]);
+        }
+        double term = 0.0;
+        if (lane < n) {
+            double pk = 0.0;
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+                if ((int)lane == j) pk = P[j];
+            const double vk = v[off + lane];
+            const double qk = pk + V.lambda * V.diag[off + lane] * vk;
+            V.q[off + lane] = qk;
+            term = vk * qk;
+        }
+        vq = wave_sum(term);
+    }
+    if (lane == 0) sh[wv] = vq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int i = 0; i < NQ / 64; ++i) acc += sh[i];
+        V.part[blockIdx.x] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(NTS) k_pcg_xr(Topo tp, PcgVecs V) {
+    __shared__ double sh[NTS];
+    PcgState s = *V.st;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (s.done) {
+        if (lead) *V.st1 = s;
+        return;
+    }
+    const double curv = sum_parts(V.part, V.nqparts, sh);
     s.products += 1;
-    if (s.mode == PCG_NORMAL) {
-        double curv = dot_all(D, V.d, V.q, sh);
-        double alpha = s.rho / curv;
+    double alpha = 0.0;
+    const bool normal = s.mode == PCG_NORMAL;
+    if (normal) {
+        alpha = s.rho / curv;
         if (!(isfinite(curv) && curv > 0.0) || !isfinite(alpha)) {
             s.done = 1;
-        } else {
-            s.iter += 1;
-            for (uint32_t i = threadIdx.x; i < D; i += 1024) {
-                V.x[i] += alpha * V.d[i];
-                V.r[i] -= alpha * V.q[i];
+            if (lead) *V.st1 = s;
+            return;
+        }
+        s.iter += 1;
+    }
+    const uint32_t A = blockIdx.x * NTS + threadIdx.x;
+    double rr = 0.0, rz = 0.0;
+    if (A < tp.nblocks) {
+        const int n = (int)tp.blk_dim[A];
+        const uint32_t off = tp.blk_off[A];
+        const double* __restrict__ d = V.d[s.par];
+        // every load of the block first, then the preconditioner, then the stores
+        double qv[9], av[9], bv[9], rv[9], xv[9], sol[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            qv[k] = k < n ? V.q[off + k] : 0.0;
+            av[k] = k < n ? (normal ? V.r[off + k] : -V.grad[off + k]) : 0.0;
+            bv[k] = (k < n && normal) ? d[off + k] : 0.0;
+            xv[k] = (k < n && normal) ? V.x[off + k] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            if (normal) {
+                xv[k] += alpha * bv[k];
+                rv[k] = av[k] - alpha * qv[k];
+            } else {  // q = S x: the true residual
+                rv[k] = av[k] - qv[k];
             }
-            __syncthreads();
-            double ratio = sqrt(dot_all(D, V.r, V.r, sh)) / s.n0;
-            if (!isfinite(ratio)) {
+        }
+        precond_apply_block(V.factors, tp.nblocks, A, n, rv, sol);
+        for (int k = 0; k < n; ++k) {
+            if (normal) V.x[off + k] = xv[k];
+            V.r[off + k] = rv[k];
+            V.z[off + k] = sol[k];
+            rr += rv[k] * rv[k];
+            rz += rv[k] * sol[k];
+        }
+    }
+    rr = tree_sum<NTS>(rr, sh);
+    rz = tree_sum<NTS>(rz, sh);
+    if (threadIdx.x == 0) {
+        V.part[V.nqparts + blockIdx.x] = rr;
+        V.part[V.nqparts + V.nparts + blockIdx.x] = rz;
+    }
+    if (lead) *V.st1 = s;
+}
+
+__global__ void __launch_bounds__(NTS) k_pcg_dir(Topo tp, PcgVecs V) {
+    __shared__ double sh[NTS];
+    PcgState s = *V.st1;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (s.done) {
+        if (lead) *V.st = s;
+        return;
+    }
+    const double rr = sum_parts(V.part + V.nqparts, V.nparts, sh);
+    const double rz = sum_parts(V.part + V.nqparts + V.nparts, V.nparts, sh);
+    int upd = 0;  // 1: d = z + beta d, 2: d = z (restart)
+    double beta = 0.0;
+    if (s.mode == PCG_INIT) {
+        s.n0 = sqrt(rr);
+        if (!isfinite(s.n0)) {
+            s.done = 1;
+        } else if (s.n0 == 0.0) {
+            s.done = 1;
+            s.accepted = 1;
+            s.rel = 0.0;
+        } else {
+            upd = 2;
+        }
+    } else if (s.mode == PCG_NORMAL) {
+        const double ratio = sqrt(rr) / s.n0;
+        if (!isfinite(ratio)) {
+            s.done = 1;
+        } else if (ratio <= s.tol) {
+            s.mode = PCG_CONFIRM;
+        } else {
+            beta = rz / s.rho;
+            if (!(isfinite(rz) && rz > 0.0) || !isfinite(beta)) {
                 s.done = 1;
-            } else if (ratio <= s.tol) {
-                s.mode = PCG_CONFIRM;
             } else {
-                s.need_z = 1;
-                s.restart = 0;
+                upd = 1;
+                s.rho = rz;
+                if (s.iter >= s.max_iter) s.mode = PCG_FINAL;
             }
         }
     } else {
-        // q = S x: the true residual
-        for (uint32_t i = threadIdx.x; i < D; i += 1024) V.r[i] = -V.grad[i] - V.q[i];
-        __syncthreads();
-        double tr = sqrt(dot_all(D, V.r, V.r, sh)) / s.n0;
+        const double tr = sqrt(rr) / s.n0;
         if (s.mode == PCG_FINAL) {
             s.done = 1;
             s.rel = isfinite(tr) ? tr : NAN;
@@ -1321,198 +1713,61 @@ __global__ void __launch_bounds__(1024) k_pcg_update_a(Topo tp, PcgVecs V) {
         } else if (!isfinite(tr)) {
             s.done = 1;
         } else {
-            s.need_z = 1;
-            s.restart = 1;
             s.tr = tr;
+            upd = 2;
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) *V.st = s;
-}
-
-__global__ void __launch_bounds__(NT) k_pcg_precond(Topo tp, PcgVecs V) {
-    __shared__ double sh[NT];
-    if (V.st->done || !V.st->need_z) return;
-    const uint32_t A = blockIdx.x * NT + threadIdx.x;
-    double acc = 0.0;
-    if (A < tp.nblocks) {
-        const int n = (int)tp.blk_dim[A];
-        if (n > 0) {
-            const uint32_t off = tp.blk_off[A];
-            double sol[9];
-            precond_apply_block(V.factors + FACT * (size_t)A, n, V.r + off, sol);
-            for (int k = 0; k < n; ++k) {
-                V.z[off + k] = sol[k];
-                acc += V.r[off + k] * sol[k];
-            }
-        }
-    }
-    const double v = tree_sum<NT>(acc, sh);
-    if (threadIdx.x == 0) V.part[blockIdx.x] = v;
-}
-
-__global__ void __launch_bounds__(1024) k_pcg_update_b(Topo tp, PcgVecs V, uint32_t nparts) {
-    __shared__ double sh[1024];
-    const uint32_t D = tp.D;
-    PcgState s = *V.st;
-    if (s.done) return;
-    if (s.need_z) {
-        double acc = 0.0;
-        for (uint32_t g = threadIdx.x; g < nparts; g += 1024) acc += V.part[g];
-        const double next = tree_sum<1024>(acc, sh);
-        if (!s.restart) {
-            const double beta = next / s.rho;
-            if (!(isfinite(next) && next > 0.0) || !isfinite(beta)) {
-                s.done = 1;
-            } else {
-                for (uint32_t i = threadIdx.x; i < D; i += 1024) V.d[i] = V.z[i] + beta * V.d[i];
-                s.rho = next;
-                if (s.iter >= s.max_iter) s.mode = PCG_FINAL;
-            }
+    if (upd == 2) {
+        if (!(isfinite(rz) && rz > 0.0)) {
+            s.done = 1;
+            s.rel = s.tr;
         } else {
-            for (uint32_t i = threadIdx.x; i < D; i += 1024) V.d[i] = V.z[i];
-            if (!(isfinite(next) && next > 0.0)) {
-                s.done = 1;
-                s.rel = s.tr;
-            } else {
-                s.rho = next;
-                s.mode = s.iter >= s.max_iter ? PCG_FINAL : PCG_NORMAL;
+            s.rho = rz;
+            s.mode = s.iter >= s.max_iter ? PCG_FINAL : PCG_NORMAL;
+        }
+    }
+    if (!s.done) {
+        const double* __restrict__ dc = V.d[s.par];
+        double* __restrict__ dn = V.d[1 - s.par];
+        const bool use_d = s.mode == PCG_NORMAL;  // else the next product is S x
+        const uint32_t A = blockIdx.x * NTS + threadIdx.x;
+        // the next direction (or x) on index i: the same expression in every lane that needs it
+        auto value = [&](double z, double d, double x) { return use_d ? (upd == 1 ? z + beta * d : z) : x; };
+        if (A < tp.nblocks) {
+            const uint32_t n = tp.blk_dim[A], off = tp.blk_off[A];
+            double zv[9], dv[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                zv[k] = (upd && k < (int)n) ? V.z[off + k] : 0.0;
+                dv[k] = (upd == 1 && k < (int)n) ? dc[off + k] : 0.0;
             }
+            double vp[6], vi[3];
+            const bool cam = A < tp.nc;
+            if (cam) {
+                const uint32_t pd = tp.cam_dim[A], g = tp.cam_group[A];
+                const uint32_t gd = tp.grp_dim[g], go = tp.grp_off[g];
+                double xp[6], gz[3], gdv[3], gx[3];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) xp[k] = (!use_d && k < (int)pd) ? V.x[off + k] : 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const bool in = k < (int)gd;
+                    gz[k] = (use_d && in) ? V.z[go + k] : 0.0;
+                    gdv[k] = (use_d && upd == 1 && in) ? dc[go + k] : 0.0;
+                    gx[k] = (!use_d && in) ? V.x[go + k] : 0.0;
+                }
+#pragma unroll
+                for (int k = 0; k < 6; ++k) vp[k] = k < (int)pd ? value(zv[k], dv[k], xp[k]) : 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) vi[k] = k < (int)gd ? value(gz[k], gdv[k], gx[k]) : 0.0;
+            }
+            if (upd)
+                for (uint32_t k = 0; k < n; ++k) dn[off + k] = upd == 1 ? zv[k] + beta * dv[k] : zv[k];
+            if (cam) write_vcam(V, tp, A, vp, vi);
         }
-        s.need_z = 0;
-        s.restart = 0;
+        if (upd) s.par = 1 - s.par;
     }
-    __syncthreads();
-    if (!s.done) write_vcam(tp, s, V);
-    if (threadIdx.x == 0) *V.st = s;
-}
-
-// Matrix-free S v, S = U + lambda diag - W V^-1 W^T with U = sum_o J~_c,o^T J~_c,o and
-// W_o = J~_c,o^T J~_X,o (the retained operator of self_calibration_retained/operator.rs:26-76,
-// evaluated from the Jacobians instead of stored block products).
-
-// Point pass: one workgroup per point chunk, one lane per observation; per-observation
-// e_o = J~_X,o^T (J~_c,o v) is summed per point in observation order, then w_p = V^-1 sum e_o.
-template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointElim<T> pe, const double* vcam, double* w,
-                                                   const PcgState* st) {
-    __shared__ double sh[3][NT];
-    if (st->done) return;
-    const uint32_t k = blockIdx.x, t = threadIdx.x;
-    const uint32_t ob = tp.chunk_obs[k], oe = tp.chunk_obs[k + 1];
-    const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
-    const size_t no = tp.no;
-    double acc[3] = {0.0, 0.0, 0.0};
-    // lanes take observations ob + t, ob + t + NT, ... (one pass unless one point has > NT)
-    for (uint32_t o = ob + t; o < oe; o += NT) {
-        const double* vc = vcam + 9 * (size_t)tp.obs_cam[o];
-        double t0 = 0.0, t1 = 0.0;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            t0 += (double)lin.jpm[(SOA_JC + q) * no + o] * vc[q];
-            t1 += (double)lin.jpm[(SOA_JC + 9 + q) * no + o] * vc[q];
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-            acc[q] += (double)lin.jpm[(SOA_JX + q) * no + o] * t0 + (double)lin.jpm[(SOA_JX + 3 + q) * no + o] * t1;
-    }
-    sh[0][t] = acc[0];
-    sh[1][t] = acc[1];
-    sh[2][t] = acc[2];
-    __syncthreads();
-    if (oe - ob > NT) {  // a single point: fixed-shape tree over the lanes
-        for (int s = NT / 2; s > 0; s >>= 1) {
-            if ((int)t < s)
-                for (int q = 0; q < 3; ++q) sh[q][t] += sh[q][t + s];
-            __syncthreads();
-        }
-    }
-    if (t >= pend - pb) return;
-    const uint32_t p = pb + t;
-    double s3[3] = {0.0, 0.0, 0.0};
-    if (oe - ob > NT) {
-        for (int q = 0; q < 3; ++q) s3[q] = sh[q][0];
-    } else {
-        for (uint32_t o = tp.pt_off[p]; o < tp.pt_off[p + 1]; ++o)
-            for (int q = 0; q < 3; ++q) s3[q] += sh[q][o - ob];
-    }
-    double out[3] = {0.0, 0.0, 0.0};
-    if (!tp.pt_fixed[p]) {
-        const T* vi = pe.vinv + 6 * (size_t)p;
-        double m[6] = {(double)vi[0], (double)vi[1], (double)vi[2], (double)vi[3], (double)vi[4], (double)vi[5]};
-        sym3_mul(m, s3, out);
-    }
-    w[3 * (size_t)p] = out[0];
-    w[3 * (size_t)p + 1] = out[1];
-    w[3 * (size_t)p + 2] = out[2];
-}
-
-// Camera pass: one workgroup per camera over its camera-major observations;
-// P_c = sum_o J~_c,o^T (J~_c,o v - J~_X,o w_p), a fixed-shape tree over the lanes.
-template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_cameras(Topo tp, Lin<T> lin, const double* vcam, const double* w, double* P,
-                                                    const PcgState* st) {
-    __shared__ double sh[9][NT];
-    if (st->done) return;
-    const uint32_t c = blockIdx.x, t = threadIdx.x;
-    const uint32_t jb = tp.cm_off[c], je = tp.cm_off[c + 1];
-    const size_t no = tp.no;
-    double vc[9];
-    for (int q = 0; q < 9; ++q) vc[q] = vcam[9 * (size_t)c + q];
-    double acc[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) acc[q] = 0.0;
-    for (uint32_t j = jb + t; j < je; j += NT) {
-        const uint32_t p = tp.cm_pt[j];
-        const double w0 = w[3 * (size_t)p], w1 = w[3 * (size_t)p + 1], w2 = w[3 * (size_t)p + 2];
-        double jc0[9], jc1[9];
-        double u0 = 0.0, u1 = 0.0;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            jc0[q] = (double)lin.jcm[(SOA_JC + q) * no + j];
-            jc1[q] = (double)lin.jcm[(SOA_JC + 9 + q) * no + j];
-            u0 += jc0[q] * vc[q];
-            u1 += jc1[q] * vc[q];
-        }
-        u0 -= (double)lin.jcm[(SOA_JX + 0) * no + j] * w0 + (double)lin.jcm[(SOA_JX + 1) * no + j] * w1 +
-              (double)lin.jcm[(SOA_JX + 2) * no + j] * w2;
-        u1 -= (double)lin.jcm[(SOA_JX + 3) * no + j] * w0 + (double)lin.jcm[(SOA_JX + 4) * no + j] * w1 +
-              (double)lin.jcm[(SOA_JX + 5) * no + j] * w2;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) acc[q] += jc0[q] * u0 + jc1[q] * u1;
-    }
-#pragma unroll
-    for (int q = 0; q < 9; ++q) sh[q][t] = acc[q];
-    __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
-        if ((int)t < s)
-#pragma unroll
-            for (int q = 0; q < 9; ++q) sh[q][t] += sh[q][t + s];
-        __syncthreads();
-    }
-    if (t < 9) P[9 * (size_t)c + t] = sh[t][0];
-}
-
-// Block pass: q_A = (rows of P for block A, summed over the group's cameras in ascending order)
-// + lambda diag_A v_A; one lane per block.
-__global__ void __launch_bounds__(NT) k_pcg_blocks(Topo tp, double lambda, const double* diag, const double* x,
-                                                   const double* d, const double* P, double* q,
-                                                   const PcgState* st) {
-    if (st->done) return;
-    const double* v = st->mode == PCG_NORMAL ? d : x;
-    const uint32_t A = blockIdx.x * NT + threadIdx.x;
-    if (A >= tp.nblocks) return;
-    const uint32_t n = tp.blk_dim[A], off = tp.blk_off[A];
-    if (A < tp.nc) {
-        for (uint32_t k = 0; k < n; ++k) q[off + k] = P[9 * (size_t)A + k] + lambda * diag[off + k] * v[off + k];
-    } else {
-        const uint32_t g = A - tp.nc;
-        for (uint32_t k = 0; k < n; ++k) {
-            double s = 0.0;
-            for (uint32_t i = tp.grp_cam_off[g]; i < tp.grp_cam_off[g + 1]; ++i) s += P[9 * (size_t)tp.grp_cams[i] + 6 + k];
-            q[off + k] = s + lambda * diag[off + k] * v[off + k];
-        }
-    }
+    if (lead) *V.st = s;
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1599,34 +1854,26 @@ void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pa
     hipLaunchKernelGGL(k_precond_factor, dim3(grid_for(tp.nblocks, NT)), dim3(NT), 0, s, tp, diag_pair, value_off,
                        u_vals, s_vals, factors, fail);
 }
-uint32_t pcg_partials(const Topo& tp) { return grid_for(tp.nblocks, NT); }
+uint32_t pcg_partials(const Topo& tp) { return grid_for(tp.nblocks, NTS); }
 void launch_pcg_init(hipStream_t s, const Topo& tp, const PcgVecs& V, double tol, int max_iter) {
-    hipLaunchKernelGGL(k_pcg_init_a, dim3(1), dim3(1024), 0, s, tp, V, tol, max_iter);
-    if (tp.nblocks) hipLaunchKernelGGL(k_pcg_precond, dim3(pcg_partials(tp)), dim3(NT), 0, s, tp, V);
-    hipLaunchKernelGGL(k_pcg_update_b, dim3(1), dim3(1024), 0, s, tp, V, pcg_partials(tp));
+    const unsigned g = pcg_partials(tp);
+    hipLaunchKernelGGL(k_pcg_init, dim3(g), dim3(NTS), 0, s, tp, V, tol, max_iter);
+    hipLaunchKernelGGL(k_pcg_dir, dim3(g), dim3(NTS), 0, s, tp, V);
 }
-void launch_pcg_update(hipStream_t s, const Topo& tp, const PcgVecs& V) {
-    hipLaunchKernelGGL(k_pcg_update_a, dim3(1), dim3(1024), 0, s, tp, V);
-    if (tp.nblocks) hipLaunchKernelGGL(k_pcg_precond, dim3(pcg_partials(tp)), dim3(NT), 0, s, tp, V);
-    hipLaunchKernelGGL(k_pcg_update_b, dim3(1), dim3(1024), 0, s, tp, V, pcg_partials(tp));
+uint32_t pcg_q_partials(const Topo& tp) { return grid_for(tp.nblocks, NQ / 64); }
+void launch_pcg_step(hipStream_t s, const Topo& tp, const PcgVecs& V) {
+    const unsigned g = pcg_partials(tp);
+    hipLaunchKernelGGL(k_pcg_q, dim3(pcg_q_partials(tp)), dim3(NQ), 0, s, tp, V);
+    hipLaunchKernelGGL(k_pcg_xr, dim3(g), dim3(NTS), 0, s, tp, V);
+    hipLaunchKernelGGL(k_pcg_dir, dim3(g), dim3(NTS), 0, s, tp, V);
 }
 template <class T>
 void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const PcgVecs& V) {
-    if (tp.nchunks)
-        hipLaunchKernelGGL(k_pcg_points<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, lin, pe, (const double*)V.vcam, V.w,
-                           (const PcgState*)V.st);
+    if (tp.nchunks) hipLaunchKernelGGL(k_pcg_points<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, lin, pe, V);
 }
 template <class T>
 void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const PcgVecs& V) {
-    if (tp.nc)
-        hipLaunchKernelGGL(k_pcg_cameras<T>, dim3(tp.nc), dim3(NT), 0, s, tp, lin, (const double*)V.vcam,
-                           (const double*)V.w, V.P, (const PcgState*)V.st);
-}
-void launch_pcg_blocks(hipStream_t s, const Topo& tp, double lambda, const double* diag, const double* x,
-                       const double* d, const double* P, double* q, const PcgState* st) {
-    if (tp.nblocks)
-        hipLaunchKernelGGL(k_pcg_blocks, dim3(grid_for(tp.nblocks, NT)), dim3(NT), 0, s, tp, lambda, diag, x, d, P, q,
-                           st);
+    if (V.nseg) hipLaunchKernelGGL(k_pcg_cameras<T>, dim3(grid_for(V.nseg, NT / 64)), dim3(NT), 0, s, tp, lin, V);
 }
 template <class T>
 void launch_linearize_cm(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,

@@ -156,46 +156,66 @@ void launch_step_dots(hipStream_t s, uint32_t D, const double* g, const double* 
                       const double* product, double* dots);
 
 // ---- implicit-Schur PCG (kernel (c)) ----
+// The camera pass runs one wavefront per camera segment: camera c's camera-major observations
+// cm_off[c]..cm_off[c+1] are cut into pieces of PCG_SEG, segments cseg_off[c]..cseg_off[c+1].
+constexpr uint32_t PCG_SEG = 256;
+constexpr uint32_t PCG_VCAM = 12;  // doubles per camera of the product input (9 used)
+constexpr uint32_t PCG_XW = 6;     // doubles per point of the point record (X, w)
+constexpr uint32_t PCG_BLOCK_WG = 64;  // workgroup of the block-parallel PCG kernels
+constexpr uint32_t PCG_Q_WG = 1024;    // workgroup of the block product pass (a wavefront per block)
 struct PcgState {
     double rho, n0, rel, tol;
     double tr;     // true residual ratio of a failed confirmation (restart)
     int mode, iter, max_iter, done, accepted;
     int products;  // Schur products applied (each update that ran follows one product)
-    int need_z;    // the preconditioner pass runs
-    int restart;   // ... and the direction restarts from z
+    int par;       // direction buffer holding d
+    int _pad;
 };
 // Vectors of one PCG solve (device).
 struct PcgVecs {
     const double* grad;     // D, reduced gradient (the right-hand side is -grad)
     const double* factors;  // 54 per block, block-Jacobi factors
-    double *x, *r, *z, *d, *q;  // D each
-    double* w;     // 3 per point, point pass output
-    double* P;     // 9 per camera, camera pass output
-    double* vcam;  // 9 per camera, the product's input on each camera's Jacobian columns
-    double* part;  // one per preconditioner workgroup
-    PcgState* st;
+    const double* diag;     // D, clamped damping diagonal
+    const double* R;        // 9 per camera: rotations of the linearisation state
+    const double* X;        // 3 per point: points of the linearisation state
+    const GaugeDev* gauge;  // gauge of the linearisation
+    double lambda;
+    double *x, *r, *z, *q;  // D each
+    double* d[2];           // D each: the direction alternates between the two
+    double* xw;    // PCG_XW per point: X and the point pass output w
+    double* seg;   // 9 per camera segment: the camera pass sums (E, F, I)
+    double* vcam;  // PCG_VCAM per camera: the product input in the world frame
+    double* part;  // per-workgroup partial dot products: v.q (nqparts), r.r and r.z (nparts each)
+    uint32_t nqparts, nparts;
+    uint32_t nseg;
+    const uint32_t* seg_cam;   // camera of each segment
+    const uint32_t* cseg_off;  // nc + 1 segment offsets
+    PcgState* st;   // read by the product passes; written by the direction update
+    PcgState* st1;  // written by the step update; read by the direction update
 };
+// Per-workgroup partial counts of the block-parallel PCG kernels and of the block product pass.
 uint32_t pcg_partials(const Topo& tp);
+uint32_t pcg_q_partials(const Topo& tp);
 // Factor the block-Jacobi preconditioner: U_AA(damped) - Schur_AA per block.
 void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pair, const uint64_t* value_off,
                            const double* u_vals, const double* s_vals, double* factors, int* fail);
+// x = 0, r = -g, z = M^-1 r, the first direction and its per-camera input.
 void launch_pcg_init(hipStream_t s, const Topo& tp, const PcgVecs& V, double tol, int max_iter);
 // Implicit Schur product q = S v (v = d or x by the PCG mode), S = U + lambda diag - W V^-1 W^T:
-//   point pass   w_p = V^-1 sum_o J~_X,o^T J~_c,o v           (point chunks, point-major J)
-//   camera pass  P_c = sum_{o in c} J~_c,o^T (J~_c,o v - J~_X,o w_p)  (per camera, camera-major J)
-//   block pass   q_A = sum of the P rows of block A + lambda diag_A v_A
+//   point pass   w_p = V^-1 sum_o J~_X,o^T J~_c,o v                   (point chunks, point-major J)
+//   camera pass  per segment sums of J~_c,o^T (J~_c,o v - J~_X,o w_p)  (camera-major J)
+// Both read only J~_X and the intrinsic columns of J~_c per observation: the pose columns follow
+// from J~_X, the point and the camera rotation.
 template <class T>
 void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const PcgVecs& V);
 template <class T>
 void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const PcgVecs& V);
-void launch_pcg_blocks(hipStream_t s, const Topo& tp, double lambda, const double* diag, const double* x,
-                       const double* d, const double* P, double* q, const PcgState* st);
+// q = S v per block, then the step (x, r, z) and direction (d, per-camera input) updates.
+void launch_pcg_step(hipStream_t s, const Topo& tp, const PcgVecs& V);
 // Camera-major linearisation: the SoA camera-major Jacobian copy (lin.jcm).
 template <class T>
 void launch_linearize_cm(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
                          const double* X, const double* uv, const GaugeDev* gauge, Lin<T> out);
-// The PCG step after a product (update, preconditioner pass, new direction).
-void launch_pcg_update(hipStream_t s, const Topo& tp, const PcgVecs& V);
 
 // ---- batched windows: independent problems concatenated, one LM control per window ----
 struct WinTopo {
