@@ -34,7 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "openmvg-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
-PCG_SEG = 256  # observations per camera segment of the PCG camera pass
+PCG_SEG = 128  # FP64 camera-major positions per segment of the PCG camera pass
 
 
 def log(msg: str) -> None:
@@ -47,8 +47,12 @@ def lowered_counts(problem) -> dict:
     n_obs = problem.n_observations
     n_pts = problem.n_points
     n_cams = problem.n_poses
-    per_cam = np.bincount(problem.obs_pose, minlength=n_cams)
-    segments = int(np.sum((per_cam + PCG_SEG - 1) // PCG_SEG))
+    # camera windows of the PCG camera pass: from each camera's first camera-major position
+    # rounded down to a pair of observations (16-byte FP64 groups), PCG_SEG positions each
+    per_cam = np.bincount(problem.obs_pose, minlength=n_cams).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(per_cam)])
+    span = np.where(per_cam > 0, off[1:] - off[:-1] // 2 * 2, 0)
+    segments = int(np.sum((span + PCG_SEG - 1) // PCG_SEG))
     return {"obs": n_obs, "points": n_pts, "cameras": n_cams, "segments": segments}
 
 
@@ -62,7 +66,7 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
       per observation the 12 rows (96 B) and its camera and point indices (8 B); per point its
       offset (4 B), fixed flag (1 B), packed V^-1 (48 B), X (24 B) and the written record X, w_p
       (48 B); per camera the product input (96 B).
-    pcg_cameras (k_pcg_cameras: per camera segment of 256 observations the sums of
+    pcg_cameras (k_pcg_cameras: per camera segment of 128 positions the sums of
       J~_c^T (J~_c v - J~_X w_p)): per observation the 12 rows of the camera-major copy (96 B) and
       its point index (4 B); per point the record X, w_p once (48 B); per segment its camera (4 B)
       and the written sums (72 B); per camera offsets (12 B) and the product input (96 B).
@@ -78,6 +82,25 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     if kernel == "linearize":
         return no * (16 + 8 + 208 + 17) + npt * 24 + nc * 120
     raise ValueError(kernel)
+
+
+def measured_traffic(kernel: str, per_launch: float):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    profiles/<kernel>_large_fp64_pmc.md (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    same counter's reading for a copy of known size).  None when there is no summary, or when it
+    was measured for a different algorithmic byte count (an older kernel)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_large_fp64_pmc.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as fh:
+        for ln in fh:
+            for key in ("traffic_bytes_per_launch", "algorithmic_bytes_per_launch"):
+                if ln.startswith(key + ":"):
+                    vals[key] = float(ln.split(":", 1)[1])
+    if "traffic_bytes_per_launch" not in vals or abs(vals.get("algorithmic_bytes_per_launch", -1) - per_launch) > 1:
+        return None
+    return vals["traffic_bytes_per_launch"]
 
 
 def cpu_baseline(scene_fn, n_points_sample: int, n_obs_full: int, iterations: int) -> dict:
@@ -374,7 +397,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.roofline_kernel, per_launch),
             "launches": launches,
             "avg_launch_us": avg_s * 1e6,
             "algorithmic_bytes_per_launch": per_launch,

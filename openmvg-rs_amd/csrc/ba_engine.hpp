@@ -60,6 +60,12 @@ struct Buf {
         if (count) HIP_OK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
     }
     void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    // Upload v into a buffer of count >= v.size() elements, the tail zeroed.
+    void upload_padded(const std::vector<T>& v, size_t count, hipStream_t s) {
+        alloc(count);
+        if (v.size()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        if (count > v.size()) HIP_OK(hipMemsetAsync(p + v.size(), 0, (count - v.size()) * sizeof(T), s));
+    }
 };
 
 template <class T>
@@ -115,6 +121,8 @@ class Engine {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
+        for (hipEvent_t e : pcg_ev_)
+            if (e) (void)hipEventDestroy(e);
     }
 
     // Run `launch`, bracketed by HIP events on this stream when `which` is the profiled kernel.
@@ -152,8 +160,9 @@ class Engine {
         B_ = build_block_lists(L_);
         PL_ = build_pair_lists(L_, B_, !iterative_);
         const size_t nc = L_.n_cameras(), ng = L_.n_groups(), np = L_.n_points(), no = L_.n_obs(), D = L_.D;
-        obs_cam_.upload(L_.obs_cam, s_);
-        obs_pt_.upload(L_.obs_point, s_);
+        npad_ = soa_pad(no);
+        obs_cam_.upload_padded(L_.obs_cam, npad_, s_);
+        obs_pt_.upload_padded(L_.obs_point, npad_, s_);
         pt_off_.upload(L_.point_off, s_);
         pt_fixed_.upload(L_.point_fixed, s_);
         cam_group_.upload(L_.cam_group, s_);
@@ -187,8 +196,11 @@ class Engine {
         HIP_OK(hipMemcpyAsync(gp_[0].p, L_.gp.data(), 3 * ng * 8, hipMemcpyHostToDevice, s_));
         HIP_OK(hipMemcpyAsync(X_[0].p, L_.X.data(), 3 * np * 8, hipMemcpyHostToDevice, s_));
         fast_cam_ = iterative_ && simple_topology();
+        pose_rows_ = keep_pose_rows_ || !fast_cam_;  // the block-gradient pass reads them
+        srec_.alloc(PCG_VCAM * std::max<size_t>(1, nc));
         build_matvec_order(iterative_);
-        jpm_.alloc(SOA_ROWS * no);
+        jpm_.alloc(SOA_ROWS * npad_);
+        HIP_OK(hipMemsetAsync(jpm_.p, 0, jpm_.n * sizeof(T), s_));  // finite padding
         if (!fast_cam_) jc_.alloc(18 * no);  // block-pair path
         ocost_.alloc(no);
         osq_.alloc(no);
@@ -237,9 +249,12 @@ class Engine {
             diag_pair_.upload(dp, s_);
             edge_pair_.upload(ep, s_);
             factors_.alloc(54 * B_.n_blocks);
-            jcm_.alloc(SOA_ROWS * no);
+            jcm_.alloc(SOA_ROWS * npad_);
+            HIP_OK(hipMemsetAsync(jcm_.p, 0, jcm_.n * sizeof(T), s_));  // finite padding
             pseg_.alloc(9 * std::max<size_t>(1, nseg_));
+            casm_.alloc(54 * std::max<size_t>(1, nseg_));
             pvcam_.alloc(PCG_VCAM * nc);
+            HIP_OK(hipMemsetAsync(pvcam_.p, 0, pvcam_.n * sizeof(double), s_));
             ppart_.alloc(std::max<size_t>(1, (B_.n_blocks + PCG_Q_WG / 64 - 1) / (PCG_Q_WG / 64)) +
                          2 * std::max<size_t>(1, (B_.n_blocks + PCG_BLOCK_WG - 1) / PCG_BLOCK_WG));
             px_.alloc(D);
@@ -249,6 +264,7 @@ class Engine {
             pd_[1].alloc(D);
             pq_.alloc(D);
             pxw_.alloc(PCG_XW * np);
+            HIP_OK(hipMemsetAsync(pxw_.p, 0, pxw_.n * sizeof(double), s_));
             pcg_.alloc(1);
             pcg1_.alloc(1);
         }
@@ -279,6 +295,8 @@ class Engine {
         tp_.cm_off = cm_off_.p;
         tp_.cm_obs = cm_obs_.p;
         tp_.cm_pt = cm_pt_.p;
+        tp_.cm_cam = cm_cam_.p;
+        tp_.cm_uv = cm_uv_.p;
         tp_.loss = L_.loss;
         tp_.loss_scale = L_.loss_scale;
         HIP_OK(hipStreamSynchronize(s_));
@@ -329,7 +347,8 @@ class Engine {
             HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
             return;
         }
-        std::vector<uint32_t> off(nc + 1, 0), obs(no), pts(no);
+        std::vector<uint32_t> off(nc + 1, 0), obs(no), pts(no), cams(no);
+        std::vector<double> uvj(2 * no);
         for (size_t o = 0; o < no; ++o) off[L_.obs_cam[o] + 1]++;
         for (size_t c = 0; c < nc; ++c) off[c + 1] += off[c];
         std::vector<uint32_t> cur(off.begin(), off.end() - 1);
@@ -337,14 +356,21 @@ class Engine {
             const uint32_t j = cur[L_.obs_cam[o]]++;
             obs[j] = (uint32_t)o;
             pts[j] = L_.obs_point[o];
+            cams[j] = L_.obs_cam[o];
+            uvj[2 * j] = L_.uv[2 * o];
+            uvj[2 * j + 1] = L_.uv[2 * o + 1];
         }
         cm_off_.upload(off, s_);
         cm_obs_.upload(obs, s_);
-        cm_pt_.upload(pts, s_);
-        // camera segments of the PCG camera pass
+        cm_pt_.upload_padded(pts, npad_, s_);
+        cm_cam_.upload_padded(cams, npad_, s_);
+        cm_uv_.upload(uvj, s_);
+        // camera segments of the PCG camera pass: windows from the aligned first position
         std::vector<uint32_t> cseg(nc + 1, 0), segc;
         for (size_t c = 0; c < nc; ++c) {
-            const uint32_t n = off[c + 1] - off[c], k = (n + PCG_SEG - 1) / PCG_SEG;
+            const uint32_t vn = pcg_vec<T>(), seg = pcg_seg<T>();
+            const uint32_t n = off[c + 1] > off[c] ? off[c + 1] - off[c] / vn * vn : 0;
+            const uint32_t k = (n + seg - 1) / seg;
             cseg[c + 1] = cseg[c] + k;
             for (uint32_t i = 0; i < k; ++i) segc.push_back((uint32_t)c);
         }
@@ -383,7 +409,9 @@ class Engine {
         return V;
     }
 
-    Lin<T> lin() { return Lin<T>{nullptr, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p}; }
+    Lin<T> lin() { return Lin<T>{nullptr, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p, pose_rows_ ? 1 : 0}; }
+    // Keep the pose columns of J~_c in the point-major copy (for the per-observation hook).
+    void keep_pose_rows() { keep_pose_rows_ = true; }
     PointElim<T> pe() { return PointElim<T>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, pscal_.p}; }
 
     // Use a caller-given gauge state instead of the one derived from the translations.
@@ -412,8 +440,9 @@ class Engine {
         launch_reduce_points(s_, pscal_.p, tp.np, partials_.p, asm_.p);
         if (fast_cam_) {
             timed(BA_PROFILE_PAIRS, [&] {
-                launch_camera_assembly<T>(s_, tp, lin(), pe(), diag_pair_.p, value_off_.p, grad_.p, fgrad_.p,
-                                               diag_.p, u_vals_.p, s_vals_.p);
+                launch_camera_assembly<T>(s_, tp, lin(), pe(), R_[state].p, X_[state].p, gauge_.p, nseg_,
+                                          seg_cam_.p, cseg_off_.p, casm_.p, diag_pair_.p, value_off_.p, grad_.p,
+                                          fgrad_.p, diag_.p, u_vals_.p, s_vals_.p);
             });
         } else {
             launch_block_gradient<T>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
@@ -482,32 +511,59 @@ class Engine {
         launch_precond_factor(s_, tp_, diag_pair_.p, value_off_.p, u_vals_.p, s_vals_.p, factors_.p, fail_.p);
         HIP_OK(hipMemcpyAsync(h_int_.p, fail_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
         const PcgVecs V = pcg_vecs();
-        launch_pcg_init(s_, tp_, V, tol, limit);
+        launch_pcg_init<T>(s_, tp_, V, tol, limit);
         HIP_OK(hipStreamSynchronize(s_));
         if (*h_int_.p) return false;
         const int batch = 4;
-        const bool profiling_pcg = profile_ == BA_PROFILE_PCG_POINTS || profile_ == BA_PROFILE_PCG_CAMERAS;
-        size_t events_before = used_;
-        int products_before = 0;
-        for (;;) {
-            HIP_OK(hipMemcpyAsync(h_pcg_.p, pcg_.p, sizeof(PcgState), hipMemcpyDeviceToHost, s_));
-            HIP_OK(hipStreamSynchronize(s_));
-            // launches after the state turned done returned at once: drop their timings
-            if (profiling_pcg) used_ = events_before + (size_t)(h_pcg_.p->products - products_before);
-            if (h_pcg_.p->done) break;
-            events_before = used_;
-            products_before = h_pcg_.p->products;
+        auto enqueue_batch = [&] {
             for (int k = 0; k < batch; ++k) {
                 timed(BA_PROFILE_PCG_POINTS, [&] { launch_pcg_points<T>(s_, tp_, lin(), pe(), V); });
                 timed(BA_PROFILE_PCG_CAMERAS, [&] { launch_pcg_cameras<T>(s_, tp_, lin(), V); });
-                launch_pcg_step(s_, tp_, V);
+                launch_pcg_step<T>(s_, tp_, V);
+            }
+        };
+        const PcgState* fin = nullptr;
+        if (profile_ == BA_PROFILE_PCG_POINTS || profile_ == BA_PROFILE_PCG_CAMERAS) {
+            // one batch at a time, so that launches after the state turned done (which return at
+            // once) can be dropped from the timings by the state's product count
+            size_t events_before = used_;
+            int products_before = 0;
+            for (;;) {
+                HIP_OK(hipMemcpyAsync(h_pcg_[0].p, pcg_.p, sizeof(PcgState), hipMemcpyDeviceToHost, s_));
+                HIP_OK(hipStreamSynchronize(s_));
+                used_ = events_before + (size_t)(h_pcg_[0].p->products - products_before);
+                if (h_pcg_[0].p->done) break;
+                events_before = used_;
+                products_before = h_pcg_[0].p->products;
+                enqueue_batch();
+            }
+            fin = h_pcg_[0].p;
+        } else {
+            // Two batches in flight: the next batch is queued before the host reads the state
+            // after the previous one, so the device never waits for the host.  Every kernel of
+            // the batch queued after the state turned done returns at once.
+            for (int i = 0; i < 2; ++i)
+                if (!pcg_ev_[i]) HIP_OK(hipEventCreateWithFlags(&pcg_ev_[i], hipEventDisableTiming));
+            enqueue_batch();
+            HIP_OK(hipMemcpyAsync(h_pcg_[0].p, pcg_.p, sizeof(PcgState), hipMemcpyDeviceToHost, s_));
+            HIP_OK(hipEventRecord(pcg_ev_[0], s_));
+            for (int k = 0;; ++k) {
+                const int cur = k & 1, nxt = cur ^ 1;
+                enqueue_batch();
+                HIP_OK(hipMemcpyAsync(h_pcg_[nxt].p, pcg_.p, sizeof(PcgState), hipMemcpyDeviceToHost, s_));
+                HIP_OK(hipEventRecord(pcg_ev_[nxt], s_));
+                HIP_OK(hipEventSynchronize(pcg_ev_[cur]));
+                if (h_pcg_[cur].p->done) {
+                    fin = h_pcg_[cur].p;
+                    break;
+                }
             }
         }
-        *terms = (uint32_t)h_pcg_.p->iter;
-        *rel = h_pcg_.p->rel;
-        if (h_pcg_.p->accepted)
+        *terms = (uint32_t)fin->iter;
+        *rel = fin->rel;
+        if (fin->accepted)
             HIP_OK(hipMemcpyAsync(step_.p, px_.p, tp_.D * sizeof(double), hipMemcpyDeviceToDevice, s_));
-        return h_pcg_.p->accepted != 0;
+        return fin->accepted != 0;
     }
 
     struct Candidate {
@@ -522,7 +578,7 @@ class Engine {
         launch_retract(s_, tp, R_[state].p, t_[state].p, gp_[state].p, step_.p, gauge_.p, R_[cs].p, t_[cs].p, gp_[cs].p);
         timed(BA_PROFILE_CANDIDATE, [&] {
             launch_candidate<T>(s_, tp, R_[state].p, t_[state].p, R_[cs].p, t_[cs].p, gp_[cs].p, X_[state].p,
-                                     uv_.p, lin(), pe(), step_.p, gauge_.p, X_[cs].p, cscal_.p);
+                                uv_.p, lin(), pe(), step_.p, gauge_.p, srec_.p, X_[cs].p, cscal_.p);
         });
         launch_reduce_candidates(s_, cscal_.p, tp.np, partials_.p, cand_.p);
         launch_step_dots(s_, tp.D, grad_.p, diag_.p, step_.p, with_product ? pq_.p : nullptr, dots_.p);
@@ -551,6 +607,7 @@ class Engine {
     const Topo& topo() const { return tp_; }
     bool iterative() const { return iterative_; }
     const T* jpm() const { return jpm_.p; }
+    size_t jpm_size() const { return jpm_.n; }
     const double* obs_cost() const { return ocost_.p; }
     const uint8_t* valid() const { return valid_.p; }
     const double* S() const { return S_.p; }
@@ -570,13 +627,18 @@ class Engine {
     Topo tp_{};
     Buf<uint32_t> obs_cam_, obs_pt_, pt_off_, cam_group_, cam_off_, cam_dim_, grp_off_, grp_dim_, blk_off_, blk_dim_,
         blk_obs_off_, blk_obs_, grp_cam_off_, grp_cams_, pair_a_, pair_b_, term_off_, term_i_, term_j_, chunk_pt_,
-        chunk_obs_, cm_off_, cm_obs_, cm_pt_, cseg_off_, seg_cam_;
+        chunk_obs_, cm_off_, cm_obs_, cm_pt_, cm_cam_, cseg_off_, seg_cam_;
     uint32_t nseg_ = 0;
+    size_t npad_ = 0;  // padded observation count of the tiled copies and index arrays
     Buf<uint8_t> pt_fixed_, valid_;
     Buf<uint64_t> value_off_;
     Buf<int32_t> diag_pair_, edge_pair_;
     Buf<double> uv_, R_[2], t_[2], gp_[2], X_[2], ocost_, osq_, pscal_, cscal_, grad_, fgrad_, diag_, step_, rhs_,
-        partials_, u_vals_, s_vals_, S_, work_, factors_, px_, pr_, pz_, pd_[2], pq_, pxw_, dots_, pseg_, pvcam_, ppart_;
+        partials_, u_vals_, s_vals_, S_, work_, factors_, px_, pr_, pz_, pd_[2], pq_, dots_, pseg_, ppart_;
+    Buf<double> pxw_, pvcam_;  // PCG point and camera records
+    Buf<double> srec_;         // candidate step records
+    Buf<double> cm_uv_;        // pixels in camera-major order
+    Buf<double> casm_;         // camera assembly: per-segment sums
     Buf<T> jc_, vinv_, vdamp_, y_, ddiag_, rrows_, jpm_, jcm_;
     Buf<float> S32_;  // FP32 factor of the mixed direct form
     Buf<int> done_;
@@ -589,7 +651,8 @@ class Engine {
     Pinned<AsmScalars> h_asm_;
     Pinned<CandScalars> h_cand_;
     Pinned<int> h_int_;
-    Pinned<PcgState> h_pcg_;
+    Pinned<PcgState> h_pcg_[2];
+    hipEvent_t pcg_ev_[2] = {nullptr, nullptr};
     double h_dots_[4];
     bool fixed_gauge_ = false;
     double lambda_ = 0.0;
@@ -597,6 +660,7 @@ class Engine {
     double refine_rel_ = NAN;
     uint32_t mixed_fallbacks_ = 0;
     bool fast_cam_ = false;
+    bool pose_rows_ = true, keep_pose_rows_ = false;
     int profile_ = BA_PROFILE_NONE;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
     size_t used_ = 0;

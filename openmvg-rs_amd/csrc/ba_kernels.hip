@@ -90,6 +90,136 @@ __device__ __forceinline__ void camera_values(const Topo& tp, uint32_t c, const 
 
 }  // namespace
 
+namespace {
+// Butterfly sum over the 64 lanes of a wavefront: every lane ends with the same total (a + b and
+// b + a round alike), in an order fixed by the lane numbering.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// World-frame input of the product on camera c's Jacobian columns, from the values v of its pose
+// block (pd of 6) and of its intrinsic group (3).  With J~_X = sqrt(w) J_p R s (the point
+// Jacobian of the linearisation, J_p its camera-frame factor), the pose columns of J~_c are
+//   rotation    sqrt(w) J_p (-[R X]x) = J~_X (-[X]x) R^T / s,
+//   translation sqrt(w) J_p s         = J~_X R^T      (baseline camera: J~_X R^T (b/s) B),
+// so J~_c,pose v = J~_X (Om x X + Tr) with Om = R^T omega / s and Tr = R^T dt (baseline camera:
+// (b/s) R^T B dt).  The camera record holds [Om, Tr, intrinsic values] in FP64 in both forms:
+// the product inputs, points and w_p keep FP64 so that only the Jacobians are rounded to FP32.
+struct Rec {  // camera record 12 doubles (9 used), point record X, w (6 doubles)
+    static constexpr int XW = 6;
+    __device__ static void store_cam(double* dst, const double v[9]) {
+        double2* o = reinterpret_cast<double2*>(dst);
+        o[0] = make_double2(v[0], v[1]);
+        o[1] = make_double2(v[2], v[3]);
+        o[2] = make_double2(v[4], v[5]);
+        o[3] = make_double2(v[6], v[7]);
+        o[4] = make_double2(v[8], 0.0);
+    }
+    __device__ static void load_cam(const double* src, double v[9]) {
+        const double2* s = reinterpret_cast<const double2*>(src);
+        const double2 a = s[0], b = s[1], c = s[2], d = s[3], e = s[4];
+        v[0] = a.x, v[1] = a.y, v[2] = b.x, v[3] = b.y, v[4] = c.x, v[5] = c.y, v[6] = d.x, v[7] = d.y, v[8] = e.x;
+    }
+    __device__ static void store_point(double* dst, const double X[3], const double w[3]) {
+        double2* o = reinterpret_cast<double2*>(dst);
+        o[0] = make_double2(X[0], X[1]);
+        o[1] = make_double2(X[2], w[0]);
+        o[2] = make_double2(w[1], w[2]);
+    }
+    __device__ static void load_point(const double* src, double X[3], double w[3]) {
+        const double2* s = reinterpret_cast<const double2*>(src);
+        const double2 a = s[0], b = s[1], c = s[2];
+        X[0] = a.x, X[1] = a.y, X[2] = b.x, w[0] = b.y, w[1] = c.x, w[2] = c.y;
+    }
+};
+// Groups of 16 bytes of a structure-of-arrays row: VN consecutive observations per load.
+template <class T>
+struct Vec;
+template <>
+struct Vec<double> {
+    using V = double2;
+    using I = uint2;
+    static constexpr int N = 2;
+    __device__ static double at(const V& v, int i) { return i == 0 ? v.x : v.y; }
+    __device__ static uint32_t at(const I& v, int i) { return i == 0 ? v.x : v.y; }
+};
+template <>
+struct Vec<float> {
+    using V = float4;
+    using I = uint4;
+    static constexpr int N = 4;
+    __device__ static double at(const V& v, int i) { return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w; }
+    __device__ static uint32_t at(const I& v, int i) { return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w; }
+};
+
+// The 12 product rows of observations j..j+VN-1 (j a multiple of VN): J~_X (SoA rows
+// SOA_JX..+5) and the intrinsic columns of J~_c (rows SOA_JC+6..8 and SOA_JC+15..17).
+template <class T>
+__device__ __forceinline__ void load_row_groups(const T* __restrict__ rows, uint32_t j,
+                                                typename Vec<T>::V jx[6], typename Vec<T>::V ji[6]) {
+    using VV = typename Vec<T>::V;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) jx[k] = *reinterpret_cast<const VV*>(rows + soa_at(SOA_JX + k, j));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ji[k] = *reinterpret_cast<const VV*>(rows + soa_at(SOA_JC + 6 + k, j));
+        ji[3 + k] = *reinterpret_cast<const VV*>(rows + soa_at(SOA_JC + 15 + k, j));
+    }
+}
+
+// The 12 product rows of observation j, scalar loads.
+template <class T>
+__device__ __forceinline__ void load_compact_rows(const T* __restrict__ rows, uint32_t j, double jx[6],
+                                                  double ji[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) jx[k] = (double)rows[soa_at(SOA_JX + k, j)];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ji[k] = (double)rows[soa_at(SOA_JC + 6 + k, j)];
+        ji[3 + k] = (double)rows[soa_at(SOA_JC + 15 + k, j)];
+    }
+}
+
+// Record of camera c (rotation R, gauge gg) for the pose values vp (pd of 6) and intrinsic
+// values vi.
+__device__ void world_record(const Topo& tp, uint32_t c, const double* R, const GaugeDev& gg, const double vp[6],
+                             const double vi[3], double* dst) {
+    const uint32_t pd = tp.cam_dim[c];
+    double out[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, vi[0], vi[1], vi[2]};
+    if (pd > 0) {
+        const double inv_s = 1.0 / gg.scene_scale;
+        double w[3] = {vp[0], vp[1], vp[2]}, om[3], tr[3];
+        rotate_t(R, w, om);
+        double dt[3];
+        if ((int)c == gg.camera) {
+            const double f = gg.baseline * inv_s;
+            for (int k = 0; k < 3; ++k) dt[k] = f * (gg.basis[k] * vp[3] + gg.basis[3 + k] * vp[4]);
+        } else {
+            for (int k = 0; k < 3; ++k) dt[k] = vp[3 + k];
+        }
+        rotate_t(R, dt, tr);
+        for (int k = 0; k < 3; ++k) {
+            out[k] = om[k] * inv_s;
+            out[3 + k] = tr[k];
+        }
+    }
+    Rec::store_cam(dst, out);
+}
+
+// y = J~_c v for one observation from its J~_X (jx, 2x3) and intrinsic columns (ji, 2x3), the
+// camera record v = [Om, Tr, intrinsics] and the point X.
+__device__ __forceinline__ void compact_product(const double jx[6], const double ji[6], const double v[9],
+                                                const double X[3], double* y0, double* y1) {
+    const double u0 = v[1] * X[2] - v[2] * X[1] + v[3];
+    const double u1 = v[2] * X[0] - v[0] * X[2] + v[4];
+    const double u2 = v[0] * X[1] - v[1] * X[0] + v[5];
+    *y0 = jx[0] * u0 + jx[1] * u1 + jx[2] * u2 + ji[0] * v[6] + ji[1] * v[7] + ji[2] * v[8];
+    *y1 = jx[3] * u0 + jx[4] * u1 + jx[5] * u2 + ji[3] * v[6] + ji[4] * v[7] + ji[5] * v[8];
+}
+}  // namespace
+
 // ----------------------------------------------------------------------------------------------
 // Gauge state (schur/layout.rs:101-113): baseline, direction and S^2 tangent basis of the
 // FixedBaseline camera at the current state.
@@ -112,12 +242,12 @@ namespace {
 // residual r~, point Jacobian J~_X (2x3, world frame, times the scene scale) and camera Jacobian
 // J~_c (2x9: pose columns 0..5 by the pose motion of schur/point.rs:730-762, intrinsic columns
 // 6..8), zero beyond the camera's and group's free dimensions.  False outside the domain.
-__device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t o, const double* __restrict__ Rg,
-                                              const double* __restrict__ tg, const double* __restrict__ gp,
-                                              const double* __restrict__ X, const double* __restrict__ uv,
+__device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t c, uint32_t p, double mu, double mv,
+                                              const double* __restrict__ Rg, const double* __restrict__ tg,
+                                              const double* __restrict__ gp, const double* __restrict__ X,
                                               const GaugeDev* __restrict__ gauges, double r[2], double jx[6],
                                               double jc[18], double* cost, double* sq) {
-    const uint32_t c = tp.obs_cam[o], p = tp.obs_pt[o], g = tp.cam_group[c];
+    const uint32_t g = tp.cam_group[c];
     const GaugeDev* gauge = gauges + (tp.cam_win ? tp.cam_win[c] : 0);
     const double* R = Rg + 9 * (size_t)c;
     double Xp[3] = {X[3 * (size_t)p], X[3 * (size_t)p + 1], X[3 * (size_t)p + 2]};
@@ -128,7 +258,7 @@ __device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t o, const 
     double turn = 0.0;
     bool has_turn = first_turn(k1, k2, &turn);
     double res[2], J[2][3], Ji[2][3];
-    if (!radial_linearize<double>(f, k1, k2, turn, has_turn, cp, uv[2 * (size_t)o], uv[2 * (size_t)o + 1], res, J, Ji))
+    if (!radial_linearize<double>(f, k1, k2, turn, has_turn, cp, mu, mv, res, J, Ji))
         return false;
     double rn = sqrt(res[0] * res[0] + res[1] * res[1]);
     double sw = loss_sqrt_weight<double>(tp.loss, tp.loss_scale, rn);
@@ -185,7 +315,8 @@ __global__ void __launch_bounds__(NT) k_linearize(Topo tp, const double* __restr
     uint32_t o = blockIdx.x * NT + threadIdx.x;
     if (o >= tp.no) return;
     double r[2], jx[6], jc[18], cost = 0.0, sq = 0.0;
-    const bool ok = linearize_obs(tp, o, Rg, tg, gp, X, uv, gauge, r, jx, jc, &cost, &sq);
+    const bool ok = linearize_obs(tp, tp.obs_cam[o], tp.obs_pt[o], uv[2 * (size_t)o], uv[2 * (size_t)o + 1], Rg, tg,
+                                  gp, X, gauge, r, jx, jc, &cost, &sq);
     if (!ok) {
         for (int k = 0; k < 2; ++k) r[k] = 0.0;
         for (int k = 0; k < 6; ++k) jx[k] = 0.0;
@@ -207,17 +338,18 @@ __global__ void __launch_bounds__(NT) k_linearize(Topo tp, const double* __restr
 #pragma unroll
         for (int k = 0; k < 18; ++k) jo[k] = (T)jc[k];
     }
-    const size_t no = tp.no;
 #pragma unroll
-    for (int k = 0; k < 2; ++k) out.jpm[(SOA_R + k) * no + o] = (T)r[k];
+    for (int k = 0; k < 2; ++k) out.jpm[soa_at(SOA_R + k, o)] = (T)r[k];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) out.jpm[(SOA_JX + k) * no + o] = (T)jx[k];
+    for (int k = 0; k < 6; ++k) out.jpm[soa_at(SOA_JX + k, o)] = (T)jx[k];
 #pragma unroll
-    for (int k = 0; k < 18; ++k) out.jpm[(SOA_JC + k) * no + o] = (T)jc[k];
+    for (int k = 0; k < 18; ++k)
+        if (out.pose_rows || k % 9 >= 6) out.jpm[soa_at(SOA_JC + k, o)] = (T)jc[k];
 }
 
 // Camera-major twin of k_linearize: lane j linearises observation cm_obs[j] and writes the
-// structure-of-arrays camera-major Jacobian copy (coalesced writes; gathered, cached reads).
+// structure-of-arrays camera-major Jacobian copy (coalesced reads of the camera-major camera,
+// point and pixel arrays and coalesced writes; the point itself is gathered).
 template <class T>
 __global__ void __launch_bounds__(NT) k_linearize_cm(Topo tp, const double* __restrict__ Rg,
                                                      const double* __restrict__ tg, const double* __restrict__ gp,
@@ -226,18 +358,20 @@ __global__ void __launch_bounds__(NT) k_linearize_cm(Topo tp, const double* __re
     uint32_t j = blockIdx.x * NT + threadIdx.x;
     if (j >= tp.no) return;
     double r[2], jx[6], jc[18], cost, sq;
-    if (!linearize_obs(tp, tp.cm_obs[j], Rg, tg, gp, X, uv, gauge, r, jx, jc, &cost, &sq)) {
+    // camera, point and pixel of position j from the camera-major copies (coalesced)
+    if (!linearize_obs(tp, tp.cm_cam[j], tp.cm_pt[j], tp.cm_uv[2 * (size_t)j], tp.cm_uv[2 * (size_t)j + 1], Rg, tg, gp,
+                       X, gauge, r, jx, jc, &cost, &sq)) {
         for (int k = 0; k < 2; ++k) r[k] = 0.0;
         for (int k = 0; k < 6; ++k) jx[k] = 0.0;
         for (int k = 0; k < 18; ++k) jc[k] = 0.0;
     }
-    const size_t no = tp.no;
 #pragma unroll
-    for (int k = 0; k < 2; ++k) out.jcm[(SOA_R + k) * no + j] = (T)r[k];
+    for (int k = 0; k < 2; ++k) out.jcm[soa_at(SOA_R + k, j)] = (T)r[k];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) out.jcm[(SOA_JX + k) * no + j] = (T)jx[k];
+    for (int k = 0; k < 6; ++k) out.jcm[soa_at(SOA_JX + k, j)] = (T)jx[k];
 #pragma unroll
-    for (int k = 0; k < 18; ++k) out.jcm[(SOA_JC + k) * no + j] = (T)jc[k];
+    for (int k = 0; k < 18; ++k)  // the pose columns are rebuilt from J~_X where needed
+        if (k % 9 >= 6) out.jcm[soa_at(SOA_JC + k, j)] = (T)jc[k];
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -253,12 +387,12 @@ constexpr int PE_V = 12;  // V (6), g (3), cost, squared residual, invalid obser
 template <class T>
 __device__ __forceinline__ void point_contrib(const Lin<T>& lin, size_t no, uint32_t o, double c[PE_V]) {
     const bool valid = lin.valid[o] != 0;
-    const double r0 = (double)lin.jpm[(SOA_R + 0) * no + o], r1 = (double)lin.jpm[(SOA_R + 1) * no + o];
+    const double r0 = (double)lin.jpm[soa_at(SOA_R + 0, o)], r1 = (double)lin.jpm[soa_at(SOA_R + 1, o)];
     double a[2][3];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) a[r][k] = (double)lin.jpm[(SOA_JX + 3 * r + k) * no + o];
+        for (int k = 0; k < 3; ++k) a[r][k] = (double)lin.jpm[soa_at(SOA_JX + 3 * r + k, o)];
     c[0] = a[0][0] * a[0][0] + a[1][0] * a[1][0];
     c[1] = a[0][1] * a[0][0] + a[1][1] * a[1][0];
     c[2] = a[0][1] * a[0][1] + a[1][1] * a[1][1];
@@ -421,13 +555,13 @@ __global__ void __launch_bounds__(NT) k_point_elim(Topo tp, Lin<T> lin, T lambda
         for (int q = 0; q < 6; ++q) vi[q] = (T)shinv[q][lp];
         T x0[3], x1[3];
         for (int q = 0; q < 3; ++q) {
-            x0[q] = lin.jpm[(SOA_JX + q) * no + o];
-            x1[q] = lin.jpm[(SOA_JX + 3 + q) * no + o];
+            x0[q] = lin.jpm[soa_at(SOA_JX + q, o)];
+            x1[q] = lin.jpm[soa_at(SOA_JX + 3 + q, o)];
         }
         T* rr = pe.rrows + 27 * (size_t)o;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-            T c0 = lin.jpm[(SOA_JC + q) * no + o], c1 = lin.jpm[(SOA_JC + 9 + q) * no + o];
+            T c0 = lin.jpm[soa_at(SOA_JC + q, o)], c1 = lin.jpm[soa_at(SOA_JC + 9 + q, o)];
             T w[3] = {x0[0] * c0 + x1[0] * c1, x0[1] * c0 + x1[1] * c1, x0[2] * c0 + x1[2] * c1};
             T v[3];
             sym3_mul(vi, w, v);
@@ -525,20 +659,19 @@ __global__ void __launch_bounds__(NT) k_block_gradient(Topo tp, Lin<T> lin, Poin
     if (ch < chunks) {
         uint32_t c0 = lb + (uint32_t)((uint64_t)len * ch / chunks);
         uint32_t c1 = lb + (uint32_t)((uint64_t)len * (ch + 1) / chunks);
-        const size_t no = tp.no;
         for (uint32_t k = c0; k < c1; ++k) {
             uint32_t i = tp.blk_obs[k];
-            double j0 = (double)lin.jpm[(SOA_JC + rowbase + r) * no + i];
-            double j1 = (double)lin.jpm[(SOA_JC + 9 + rowbase + r) * no + i];
-            full += j0 * (double)lin.jpm[(SOA_R + 0) * no + i] + j1 * (double)lin.jpm[(SOA_R + 1) * no + i];
+            double j0 = (double)lin.jpm[soa_at(SOA_JC + rowbase + r, i)];
+            double j1 = (double)lin.jpm[soa_at(SOA_JC + 9 + rowbase + r, i)];
+            full += j0 * (double)lin.jpm[soa_at(SOA_R + 0, i)] + j1 * (double)lin.jpm[soa_at(SOA_R + 1, i)];
             dg += j0 * j0 + j1 * j1;
             uint32_t p = tp.obs_pt[i];
             if (!tp.pt_fixed[p]) {
                 const T* y = pe.y + 3 * (size_t)p;
                 double wy0 = 0.0, wy1 = 0.0;
                 for (int q = 0; q < 3; ++q) {
-                    wy0 += (double)lin.jpm[(SOA_JX + q) * no + i] * (double)y[q];
-                    wy1 += (double)lin.jpm[(SOA_JX + 3 + q) * no + i] * (double)y[q];
+                    wy0 += (double)lin.jpm[soa_at(SOA_JX + q, i)] * (double)y[q];
+                    wy1 += (double)lin.jpm[soa_at(SOA_JX + 3 + q, i)] * (double)y[q];
                 }
                 red += j0 * wy0 + j1 * wy1;
             }
@@ -563,146 +696,196 @@ __global__ void __launch_bounds__(NT) k_block_gradient(Topo tp, Lin<T> lin, Poin
 }
 
 // ----------------------------------------------------------------------------------------------
-// k_camera_assembly: one workgroup per camera over its camera-major observations (every group has
-// one camera and no camera sees a point twice).  Per observation, with W_o = J~_c^T J~_X (9x3):
-//   full gradient J~_c^T r~, reduced part W_o y_p, diagonal sum of J~_c squares,
-//   U blocks J~_c^T J~_c and Schur blocks W_o V^-1 W_o^T on the pose-pose and intrinsic-intrinsic
-//   blocks (the diagonal blocks of the reduced system, schur/assembly.rs:893-1039).
-// Lane sums run in position order, then a fixed tree over the lanes (CA_V values in 3 rounds).
+// Camera-side assembly when every group has one camera and no camera sees a point twice (every
+// Schur term of a diagonal block is then one observation's), for the PCG form: per camera the
+// full and reduced gradients, the damping diagonal and the diagonal blocks of the reduced system
+// U - W V^-1 W^T (schur/assembly.rs:893-1039), from which the block-Jacobi preconditioner is
+// factored.  Per observation, with u = J~_X y_p and M = J~_X V^-1 J~_X^T (2x2):
+//   full gradient J~_c^T r~, reduced part J~_c^T u, diagonal sum of J~_c squares, and
+//   J~_c^T (I - M) J~_c (= U_o - W_o V^-1 W_o^T) on the pose-pose and intrinsic-intrinsic blocks.
+// The pose columns of J~_c are rebuilt from J~_X, the point and the camera rotation (the compact
+// form of the PCG product), so the camera-major copy holds only r~, J~_X and the intrinsic columns.
+// Kernel 1: one workgroup of two wavefronts per PCG camera segment; wavefront 0 sums the gradient
+// and diagonal terms, wavefront 1 the block terms (27 values each), each over its lanes' positions
+// in order and then by the wavefront butterfly.  Kernel 2: per camera, the segments' sums in
+// segment order, written out (the blocks into the U values of the diagonal pairs, zero Schur
+// values).
 // ----------------------------------------------------------------------------------------------
 namespace {
-constexpr int CA_V = 81;  // 9 full, 9 reduced, 9 diagonal, 21 U pose, 6 U intrinsic, 21 S pose, 6 S intr.
 __device__ __forceinline__ int tri(int r, int c) { return r * (r + 1) / 2 + c; }  // r >= c
+constexpr int CA_V = 54;  // per segment: 9 full, 9 reduced part, 9 diagonal | 21 pose block, 6 intrinsic block
+
+// J~_c (2x9) of an observation from J~_X (a, 2x3), the intrinsic columns (ji, 2x3), the point X,
+// the camera rotation R, its pose dimension and the gauge.
+__device__ __forceinline__ void rebuild_camera_jacobian(const double a[6], const double ji[6], const double X[3],
+                                                        const double* R, uint32_t pd, bool baseline,
+                                                        const GaugeDev& gg, double jc[2][9]) {
+    const double inv_s = 1.0 / gg.scene_scale;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double* ai = a + 3 * i;
+        const double b[3] = {X[1] * ai[2] - X[2] * ai[1], X[2] * ai[0] - X[0] * ai[2], X[0] * ai[1] - X[1] * ai[0]};
+        double rb[3], ra[3];
+        rotate(R, b, rb);
+        rotate(R, ai, ra);
+        const bool posed = pd > 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) jc[i][k] = posed ? rb[k] * inv_s : 0.0;
+        if (baseline) {
+            const double f = gg.baseline * inv_s;
+            jc[i][3] = posed ? f * (gg.basis[0] * ra[0] + gg.basis[1] * ra[1] + gg.basis[2] * ra[2]) : 0.0;
+            jc[i][4] = posed ? f * (gg.basis[3] * ra[0] + gg.basis[4] * ra[1] + gg.basis[5] * ra[2]) : 0.0;
+            jc[i][5] = 0.0;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) jc[i][3 + k] = posed ? ra[k] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) jc[i][6 + k] = ji[3 * i + k];
+    }
+}
 }  // namespace
 
 template <class T>
-__global__ void __launch_bounds__(NT) k_camera_assembly(Topo tp, Lin<T> lin, PointElim<T> pe,
-                                                        const int32_t* __restrict__ diag_pair,
-                                                        const uint64_t* __restrict__ value_off, double* grad,
-                                                        double* full_grad, double* diag, double* u_vals,
-                                                        double* s_vals) {
-    __shared__ double sh[27][NT];
-    const uint32_t c = blockIdx.x, t = threadIdx.x;
-    const uint32_t jb = tp.cm_off[c], je = tp.cm_off[c + 1];
-    const size_t no = tp.no;
-    double acc[CA_V];
+__global__ void __launch_bounds__(128) k_camera_assembly(Topo tp, Lin<T> lin, PointElim<T> pe, const double* Rg,
+                                                         const double* X, const GaugeDev* gauge,
+                                                         const uint32_t* __restrict__ seg_cam,
+                                                         const uint32_t* __restrict__ cseg_off, double* part) {
+    const uint32_t k = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63, role = threadIdx.x >> 6;  // role uniform per wavefront
+    constexpr uint32_t VN = pcg_vec<T>(), SEG = pcg_seg<T>();
+    const uint32_t c = seg_cam[k];
+    const uint32_t cb = tp.cm_off[c], ce = tp.cm_off[c + 1];
+    const uint32_t wb = cb / VN * VN + (k - cseg_off[c]) * SEG;
+    const GaugeDev& gg = *gauge;
+    const double* R = Rg + 9 * (size_t)c;
+    const uint32_t pd = tp.cam_dim[c];
+    const bool baseline = (int)c == gg.camera;
+    double acc[27];
 #pragma unroll
-    for (int q = 0; q < CA_V; ++q) acc[q] = 0.0;
-    for (uint32_t j = jb + t; j < je; j += NT) {
-        double r[2], jx[2][3], jc[2][9];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            r[a] = (double)lin.jcm[(SOA_R + a) * no + j];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) jx[a][q] = (double)lin.jcm[(SOA_JX + 3 * a + q) * no + j];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) jc[a][q] = (double)lin.jcm[(SOA_JC + 9 * a + q) * no + j];
-        }
+    for (int q = 0; q < 27; ++q) acc[q] = 0.0;
+    for (uint32_t it = 0; it < SEG / 64; ++it) {
+        const uint32_t j = wb + lane + 64 * it;
+        const bool live = j >= cb && j < ce;
         const uint32_t p = tp.cm_pt[j];
+        double a[6], ji[6];
+        load_compact_rows(lin.jcm, j, a, ji);
+        const double r0 = (double)lin.jcm[soa_at(SOA_R, j)], r1 = (double)lin.jcm[soa_at(SOA_R + 1, j)];
+        const double Xp[3] = {X[3 * (size_t)p], X[3 * (size_t)p + 1], X[3 * (size_t)p + 2]};
         const bool free_point = !tp.pt_fixed[p];
-        double W[9][3];
+        double jc[2][9];
+        rebuild_camera_jacobian(a, ji, Xp, R, pd, baseline, gg, jc);
+        if (!live) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            acc[q] += jc[0][q] * r[0] + jc[1][q] * r[1];
-            acc[18 + q] += jc[0][q] * jc[0][q] + jc[1][q] * jc[1][q];
-#pragma unroll
-            for (int m = 0; m < 3; ++m) W[q][m] = jc[0][q] * jx[0][m] + jc[1][q] * jx[1][m];
+            for (int q = 0; q < 9; ++q) jc[0][q] = jc[1][q] = 0.0;
         }
-#pragma unroll
-        for (int a = 0; a < 6; ++a)
-#pragma unroll
-            for (int b = 0; b <= a; ++b) acc[27 + tri(a, b)] += jc[0][a] * jc[0][b] + jc[1][a] * jc[1][b];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b <= a; ++b) acc[48 + tri(a, b)] += jc[0][6 + a] * jc[0][6 + b] + jc[1][6 + a] * jc[1][6 + b];
-        if (free_point) {
-            const T* yp = pe.y + 3 * (size_t)p;
-            const T* vp = pe.vinv + 6 * (size_t)p;
-            const double y[3] = {(double)yp[0], (double)yp[1], (double)yp[2]};
-            const double vi[6] = {(double)vp[0], (double)vp[1], (double)vp[2], (double)vp[3], (double)vp[4], (double)vp[5]};
-            double Z[9][3];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                acc[9 + q] += W[q][0] * y[0] + W[q][1] * y[1] + W[q][2] * y[2];
-                sym3_mul(vi, W[q], Z[q]);
+        if (role == 0) {
+            double u0 = 0.0, u1 = 0.0;
+            if (free_point && live) {
+                const T* yp = pe.y + 3 * (size_t)p;
+                const double y[3] = {(double)yp[0], (double)yp[1], (double)yp[2]};
+                u0 = a[0] * y[0] + a[1] * y[1] + a[2] * y[2];
+                u1 = a[3] * y[0] + a[4] * y[1] + a[5] * y[2];
             }
 #pragma unroll
-            for (int a = 0; a < 6; ++a)
+            for (int q = 0; q < 9; ++q) {
+                acc[q] += jc[0][q] * r0 + jc[1][q] * r1;
+                acc[9 + q] += jc[0][q] * u0 + jc[1][q] * u1;
+                acc[18 + q] += jc[0][q] * jc[0][q] + jc[1][q] * jc[1][q];
+            }
+        } else {
+            // N = I - M, M = J~_X V^-1 J~_X^T
+            double n00 = 1.0, n01 = 0.0, n11 = 1.0;
+            if (free_point && live) {
+                const T* vp = pe.vinv + 6 * (size_t)p;
+                const double vi[6] = {(double)vp[0], (double)vp[1], (double)vp[2], (double)vp[3], (double)vp[4], (double)vp[5]};
+                double z0[3], z1[3];
+                sym3_mul(vi, a, z0);
+                sym3_mul(vi, a + 3, z1);
+                n00 = 1.0 - (a[0] * z0[0] + a[1] * z0[1] + a[2] * z0[2]);
+                n01 = -(a[0] * z1[0] + a[1] * z1[1] + a[2] * z1[2]);
+                n11 = 1.0 - (a[3] * z1[0] + a[4] * z1[1] + a[5] * z1[2]);
+            }
+            double pn[2][9];  // N J~_c
 #pragma unroll
-                for (int b = 0; b <= a; ++b)
-                    acc[54 + tri(a, b)] += Z[a][0] * W[b][0] + Z[a][1] * W[b][1] + Z[a][2] * W[b][2];
+            for (int q = 0; q < 9; ++q) {
+                pn[0][q] = n00 * jc[0][q] + n01 * jc[1][q];
+                pn[1][q] = n01 * jc[0][q] + n11 * jc[1][q];
+            }
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+            for (int x = 0; x < 6; ++x)
 #pragma unroll
-                for (int b = 0; b <= a; ++b)
-                    acc[75 + tri(a, b)] += Z[6 + a][0] * W[6 + b][0] + Z[6 + a][1] * W[6 + b][1] + Z[6 + a][2] * W[6 + b][2];
+                for (int y = 0; y <= x; ++y) acc[tri(x, y)] += jc[0][x] * pn[0][y] + jc[1][x] * pn[1][y];
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int y = 0; y <= x; ++y)
+                    acc[21 + tri(x, y)] += jc[0][6 + x] * pn[0][6 + y] + jc[1][6 + x] * pn[1][6 + y];
         }
     }
-    // fixed tree over the lanes, 27 values per round; results kept in registers of lane 0
-    double tot[CA_V];
-    for (int round = 0; round < 3; ++round) {
 #pragma unroll
-        for (int q = 0; q < 27; ++q) sh[q][t] = acc[27 * round + q];
-        __syncthreads();
-        for (int s = NT / 2; s > 0; s >>= 1) {
-            if ((int)t < s)
+    for (int q = 0; q < 27; ++q) acc[q] = wave_sum(acc[q]);
+    if (lane == 0) {
+        double* out = part + CA_V * (size_t)k + 27 * role;
 #pragma unroll
-                for (int q = 0; q < 27; ++q) sh[q][t] += sh[q][t + s];
-            __syncthreads();
-        }
-#pragma unroll
-        for (int q = 0; q < 27; ++q) tot[27 * round + q] = sh[q][0];
-        __syncthreads();
+        for (int q = 0; q < 27; ++q) out[q] = acc[q];
     }
-    if (t != 0) return;
-    // pose block (camera c) and intrinsic block (its group)
+}
+
+// One wavefront per camera: lane v < CA_V adds value v over the camera's segments in order.
+__global__ void __launch_bounds__(64) k_camera_assembly_out(Topo tp, const uint32_t* __restrict__ cseg_off,
+                                                            const double* part,
+                                                            const int32_t* __restrict__ diag_pair,
+                                                            const uint64_t* __restrict__ value_off, double* grad,
+                                                            double* full_grad, double* diag, double* u_vals,
+                                                            double* s_vals) {
+    __shared__ double tot[CA_V];
+    const uint32_t c = blockIdx.x, v = threadIdx.x;
+    if (v < CA_V) {
+        double s = 0.0;
+        const uint32_t g0 = cseg_off[c], g1 = cseg_off[c + 1];
+        for (uint32_t g = g0; g < g1; g += 4) {  // four loads in flight, added in segment order
+            double x[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) x[i] = g + i < g1 ? part[CA_V * (size_t)(g + i) + v] : 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) s += x[i];
+        }
+        tot[v] = s;
+    }
+    __syncthreads();
     const uint32_t pd = tp.cam_dim[c], po = tp.cam_off[c];
-    for (uint32_t a = 0; a < pd; ++a) {
-        full_grad[po + a] = tot[a];
-        grad[po + a] = tot[a] - tot[9 + a];
-        diag[po + a] = tot[18 + a];
+    const uint32_t g = tp.cam_group[c], gd = tp.grp_dim[g], go = tp.grp_off[g];
+    if (v < pd) {
+        full_grad[po + v] = tot[v];
+        grad[po + v] = tot[v] - tot[9 + v];
+        diag[po + v] = tot[18 + v];
     }
-    if (pd > 0) {
-        const int32_t pr = diag_pair[c];
-        if (pr >= 0) {
-            double* U = u_vals + value_off[pr];
-            double* S = s_vals + value_off[pr];
-            for (uint32_t a = 0; a < pd; ++a)
-                for (uint32_t b = 0; b < pd; ++b) {
-                    const int ix = a >= b ? tri(a, b) : tri(b, a);
-                    U[a * pd + b] = tot[27 + ix];
-                    S[a * pd + b] = tot[54 + ix];
-                }
+    if (v < gd) {
+        full_grad[go + v] = tot[6 + v];
+        grad[go + v] = tot[6 + v] - tot[15 + v];
+        diag[go + v] = tot[24 + v];
+    }
+    if (pd > 0 && diag_pair[c] >= 0) {
+        double* U = u_vals + value_off[diag_pair[c]];
+        double* S = s_vals + value_off[diag_pair[c]];
+        for (uint32_t e = v; e < pd * pd; e += 64) {
+            const uint32_t x = e / pd, y = e % pd;
+            U[e] = tot[27 + (x >= y ? tri(x, y) : tri(y, x))];
+            S[e] = 0.0;
         }
     }
-    const uint32_t g = tp.cam_group[c], gd = tp.grp_dim[g], go = tp.grp_off[g];
-    for (uint32_t a = 0; a < gd; ++a) {
-        full_grad[go + a] = tot[6 + a];
-        grad[go + a] = tot[6 + a] - tot[15 + a];
-        diag[go + a] = tot[24 + a];
-    }
-    if (gd > 0) {
-        const int32_t pr = diag_pair[tp.nc + g];
-        if (pr >= 0) {
-            double* U = u_vals + value_off[pr];
-            double* S = s_vals + value_off[pr];
-            for (uint32_t a = 0; a < gd; ++a)
-                for (uint32_t b = 0; b < gd; ++b) {
-                    const int ix = a >= b ? tri(a, b) : tri(b, a);
-                    U[a * gd + b] = tot[48 + ix];
-                    S[a * gd + b] = tot[75 + ix];
-                }
+    if (gd > 0 && diag_pair[tp.nc + g] >= 0) {
+        double* U = u_vals + value_off[diag_pair[tp.nc + g]];
+        double* S = s_vals + value_off[diag_pair[tp.nc + g]];
+        for (uint32_t e = v; e < gd * gd; e += 64) {
+            const uint32_t x = e / gd, y = e % gd;
+            U[e] = tot[48 + (x >= y ? tri(x, y) : tri(y, x))];
+            S[e] = 0.0;
         }
     }
 }
 
-// ----------------------------------------------------------------------------------------------
-// k_pairs: one workgroup per lower block pair (A >= B).  Output (r, c) of the dA x dB block is
-// owned by threads t = chunk * nout + (r * dB + c); the pair's term list, pre-sorted by
-// (point, i, j), is split into contiguous chunks summed in order, then the chunk partials are
-// summed in chunk order.
-// ----------------------------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(NT) k_pairs(Topo tp, Lin<T> lin, PointElim<T> pe, const uint32_t* __restrict__ pair_a,
                                               const uint32_t* __restrict__ pair_b, const uint32_t* __restrict__ term_off,
@@ -933,7 +1116,7 @@ template <class T>
 __global__ void __launch_bounds__(NT) k_candidate(Topo tp, const double* R, const double* t, const double* Rc,
                                                   const double* tc, const double* gpc, const double* X,
                                                   const double* uv, Lin<T> lin, PointElim<T> pe,
-                                                  const double* step, const GaugeDev* gauge, double* Xc,
+                                                  const double* srec, const GaugeDev* gauge, double* Xc,
                                                   double* cscal) {
     __shared__ double she[3][NT];
     __shared__ double shx[3][NT];
@@ -943,22 +1126,20 @@ __global__ void __launch_bounds__(NT) k_candidate(Topo tp, const double* R, cons
     const uint32_t k = blockIdx.x, lane = threadIdx.x;
     const uint32_t ob = tp.chunk_obs[k], oe = tp.chunk_obs[k + 1];
     const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
-    const size_t no = tp.no;
     const bool big = oe - ob > NT;
-    // 1. per-observation back-substitution terms
+    // 1. per-observation back-substitution terms e_o = J~_X,o^T J~_c,o step (compact form, with the
+    //    cameras' step records)
     double e[3] = {0.0, 0.0, 0.0};
     for (uint32_t o = ob + lane; o < oe; o += NT) {
-        double vc[9];
-        camera_values(tp, tp.obs_cam[o], step, vc);
-        double t0 = 0.0, t1 = 0.0;
+        const uint32_t q = tp.obs_pt[o];
+        double v[9], jx[6], ji[6];
+        Rec::load_cam(srec + PCG_VCAM * (size_t)tp.obs_cam[o], v);
+        load_compact_rows(lin.jpm, o, jx, ji);
+        const double Xo[3] = {X[3 * (size_t)q], X[3 * (size_t)q + 1], X[3 * (size_t)q + 2]};
+        double t0, t1;
+        compact_product(jx, ji, v, Xo, &t0, &t1);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            t0 += (double)lin.jpm[(SOA_JC + q) * no + o] * vc[q];
-            t1 += (double)lin.jpm[(SOA_JC + 9 + q) * no + o] * vc[q];
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-            e[q] += (double)lin.jpm[(SOA_JX + q) * no + o] * t0 + (double)lin.jpm[(SOA_JX + 3 + q) * no + o] * t1;
+        for (int i = 0; i < 3; ++i) e[i] += jx[i] * t0 + jx[3 + i] * t1;
     }
     for (int q = 0; q < 3; ++q) she[q][lane] = e[q];
     __syncthreads();
@@ -1256,88 +1437,7 @@ __device__ double sum_parts(const double* __restrict__ part, uint32_t nparts, do
     return tree_sum<NTS>(acc, sh);
 }
 
-// Butterfly sum over the 64 lanes of a wavefront: every lane ends with the same total (a + b and
-// b + a round alike), in an order fixed by the lane numbering.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
 
-// World-frame input of the product on camera c's Jacobian columns, from the values v of its pose
-// block (pd of 6) and of its intrinsic group (3).  With J~_X = sqrt(w) J_p R s (the point
-// Jacobian of the linearisation, J_p its camera-frame factor), the pose columns of J~_c are
-//   rotation    sqrt(w) J_p (-[R X]x) = J~_X (-[X]x) R^T / s,
-//   translation sqrt(w) J_p s         = J~_X R^T      (baseline camera: J~_X R^T (b/s) B),
-// so J~_c,pose v = J~_X (Om x X + Tr) with Om = R^T omega / s and Tr = R^T dt (baseline camera:
-// (b/s) R^T B dt).  vcam holds [Om, Tr, intrinsic values, 0, 0, 0].
-__device__ void write_vcam(const PcgVecs& V, const Topo& tp, uint32_t c, const double vp[6], const double vi[3]) {
-    const uint32_t pd = tp.cam_dim[c];
-    const GaugeDev& gg = *V.gauge;
-    const double* R = V.R + 9 * (size_t)c;
-    double om[3] = {0.0, 0.0, 0.0}, tr[3] = {0.0, 0.0, 0.0};
-    if (pd > 0) {
-        const double inv_s = 1.0 / gg.scene_scale;
-        double w[3] = {vp[0], vp[1], vp[2]};
-        rotate_t(R, w, om);
-        for (int k = 0; k < 3; ++k) om[k] *= inv_s;
-        double dt[3];
-        if ((int)c == gg.camera) {
-            const double f = gg.baseline * inv_s;
-            for (int k = 0; k < 3; ++k) dt[k] = f * (gg.basis[k] * vp[3] + gg.basis[3 + k] * vp[4]);
-        } else {
-            for (int k = 0; k < 3; ++k) dt[k] = vp[3 + k];
-        }
-        rotate_t(R, dt, tr);
-    }
-    double2* out = reinterpret_cast<double2*>(V.vcam + PCG_VCAM * (size_t)c);
-    out[0] = make_double2(om[0], om[1]);
-    out[1] = make_double2(om[2], tr[0]);
-    out[2] = make_double2(tr[1], tr[2]);
-    out[3] = make_double2(vi[0], vi[1]);
-    out[4] = make_double2(vi[2], 0.0);
-    out[5] = make_double2(0.0, 0.0);
-}
-
-// y = J~_c v for one observation from its J~_X (jx, 2x3) and intrinsic columns (ji, 2x3), the
-// camera's world-frame input (om, tr, vi) and the point X.
-__device__ __forceinline__ void compact_product(const double jx[6], const double ji[6], const double om[3],
-                                                const double tr[3], const double vi[3], const double X[3],
-                                                double* y0, double* y1) {
-    const double u0 = om[1] * X[2] - om[2] * X[1] + tr[0];
-    const double u1 = om[2] * X[0] - om[0] * X[2] + tr[1];
-    const double u2 = om[0] * X[1] - om[1] * X[0] + tr[2];
-    *y0 = jx[0] * u0 + jx[1] * u1 + jx[2] * u2 + ji[0] * vi[0] + ji[1] * vi[1] + ji[2] * vi[2];
-    *y1 = jx[3] * u0 + jx[4] * u1 + jx[5] * u2 + ji[3] * vi[0] + ji[4] * vi[1] + ji[5] * vi[2];
-}
-
-__device__ __forceinline__ void load_vcam(const double* vcam, uint32_t c, double om[3], double tr[3], double vi[3]) {
-    const double2* v = reinterpret_cast<const double2*>(vcam + PCG_VCAM * (size_t)c);
-    const double2 a = v[0], b = v[1], e = v[2], f = v[3], g = v[4];
-    om[0] = a.x;
-    om[1] = a.y;
-    om[2] = b.x;
-    tr[0] = b.y;
-    tr[1] = e.x;
-    tr[2] = e.y;
-    vi[0] = f.x;
-    vi[1] = f.y;
-    vi[2] = g.x;
-}
-
-// The observation's 12 product rows: J~_X (SoA rows SOA_JX..+5) and the intrinsic columns of
-// J~_c (rows SOA_JC+6..8 and SOA_JC+15..17).
-template <class T>
-__device__ __forceinline__ void load_compact_rows(const T* __restrict__ rows, size_t no, uint32_t j, double jx[6],
-                                                  double ji[6]) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) jx[k] = (double)rows[(SOA_JX + k) * no + j];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        ji[k] = (double)rows[(SOA_JC + 6 + k) * no + j];
-        ji[3 + k] = (double)rows[(SOA_JC + 15 + k) * no + j];
-    }
-}
 }  // namespace
 
 // PCG (pcg.rs:23-119 with the retained confirmation of self_calibration_retained.rs:512-579).
@@ -1400,7 +1500,7 @@ __global__ void __launch_bounds__(NTS) k_pcg_init(Topo tp, PcgVecs V, double tol
 
 // Point pass: one workgroup per point chunk, one lane per observation;
 // e_o = J~_X,o^T (J~_c,o v) is summed per point in observation order and w_p = V^-1 sum e_o is
-// written with the point into xw (read by the camera pass).
+// written with the point into its record (X, w_p), read by the camera pass.
 template <class T>
 __global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointElim<T> pe, PcgVecs V) {
     __shared__ double sh[3][NT];
@@ -1408,7 +1508,6 @@ __global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointEli
     const uint32_t k = blockIdx.x, t = threadIdx.x;
     const uint32_t ob = tp.chunk_obs[k], oe = tp.chunk_obs[k + 1];
     const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
-    const size_t no = tp.no;
     // the point lanes' inputs, loaded first so that they are in flight with the observation pass
     const bool plane = t < pend - pb;
     const uint32_t p = pb + (plane ? t : 0);
@@ -1429,12 +1528,12 @@ __global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointEli
     // lanes take observations ob + t, ob + t + NT, ... (one pass unless one point has > NT)
     for (uint32_t o = ob + t; o < oe; o += NT) {
         const uint32_t c = tp.obs_cam[o], q = tp.obs_pt[o];
-        double om[3], tr[3], vi[3], jx[6], ji[6];
-        load_vcam(V.vcam, c, om, tr, vi);
-        load_compact_rows(lin.jpm, no, o, jx, ji);
+        double v[9], jx[6], ji[6];
+        Rec::load_cam(V.vcam + PCG_VCAM * (size_t)c, v);
+        load_compact_rows(lin.jpm, o, jx, ji);
         const double X[3] = {V.X[3 * (size_t)q], V.X[3 * (size_t)q + 1], V.X[3 * (size_t)q + 2]};
         double y0, y1;
-        compact_product(jx, ji, om, tr, vi, X, &y0, &y1);
+        compact_product(jx, ji, v, X, &y0, &y1);
 #pragma unroll
         for (int i = 0; i < 3; ++i) acc[i] += jx[i] * y0 + jx[3 + i] * y1;
     }
@@ -1459,57 +1558,61 @@ __global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointEli
     }
     double out[3] = {0.0, 0.0, 0.0};
     if (!fixed) sym3_mul(m, s3, out);
-    double2* rec = reinterpret_cast<double2*>(V.xw + PCG_XW * (size_t)p);
-    rec[0] = make_double2(Xp[0], Xp[1]);
-    rec[1] = make_double2(Xp[2], out[0]);
-    rec[2] = make_double2(out[1], out[2]);
+    Rec::store_point(V.xw + Rec::XW * (size_t)p, Xp, out);
 }
 
-// Camera pass: one wavefront per camera segment (PCG_SEG camera-major observations of one
-// camera).  Per observation z_o = J~_c,o v - J~_X,o w_p and e_o = J~_X,o^T z_o; the segment's
-// sums E = sum e_o, F = sum X_p x e_o (world frame) and I = sum J~_int,o^T z_o go to V.seg, from
-// which k_pcg_q forms J~_c^T z = [R F / s; R E; I] (baseline camera: (b/s) B^T R E).
+// Camera pass: one wavefront per camera segment.  Camera c's camera-major positions are cut into
+// windows of pcg_seg<T>() starting at cm_off[c] rounded down to a multiple of VN; lanes read VN
+// consecutive positions with 16-byte loads and keep those inside [cm_off[c], cm_off[c+1]).  Per
+// observation z_o = J~_c,o v - J~_X,o w_p and e_o = J~_X,o^T z_o; the segment's sums E = sum e_o,
+// F = sum X_p x e_o (world frame) and I = sum J~_int,o^T z_o go to V.seg, from which k_pcg_q forms
+// J~_c^T z = [R F / s; R E; I] (baseline camera: (b/s) B^T R E).
 template <class T>
 __global__ void __launch_bounds__(NT) k_pcg_cameras(Topo tp, Lin<T> lin, PcgVecs V) {
+    using VT = Vec<T>;
+    constexpr int VN = VT::N;
     if (V.st->done) return;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t k = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
     if (k >= V.nseg) return;  // whole wavefronts leave; no workgroup barrier follows
     const uint32_t c = V.seg_cam[k];
-    const uint32_t jb = tp.cm_off[c] + (k - V.cseg_off[c]) * PCG_SEG;
-    const uint32_t je = min(jb + PCG_SEG, tp.cm_off[c + 1]);
-    const size_t no = tp.no;
-    double om[3], tr[3], vi[3];
-    load_vcam(V.vcam, c, om, tr, vi);
+    const uint32_t cb = tp.cm_off[c], ce = tp.cm_off[c + 1];
+    constexpr uint32_t SEG = pcg_seg<T>();
+    const uint32_t wb = cb / VN * VN + (k - V.cseg_off[c]) * SEG;
+    double v[9];
+    Rec::load_cam(V.vcam + PCG_VCAM * (size_t)c, v);
+    const double* __restrict__ xw = V.xw;
     double acc[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) acc[q] = 0.0;
 #pragma unroll
-    for (uint32_t it = 0; it < PCG_SEG / 64; ++it) {
-        const uint32_t j0 = jb + lane + 64 * it;
-        const bool live = j0 < je;
-        const uint32_t j = live ? j0 : jb;  // in-range stand-in, its terms zeroed below
-        const uint32_t p = tp.cm_pt[j];
-        const double2* rec = reinterpret_cast<const double2*>(V.xw + PCG_XW * (size_t)p);
-        const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-        const double X[3] = {r0.x, r0.y, r1.x};
-        const double w[3] = {r1.y, r2.x, r2.y};
-        double jx[6], ji[6];
-        load_compact_rows(lin.jcm, no, j, jx, ji);
-        double y0, y1;
-        compact_product(jx, ji, om, tr, vi, X, &y0, &y1);
-        double z0 = y0 - (jx[0] * w[0] + jx[1] * w[1] + jx[2] * w[2]);
-        double z1 = y1 - (jx[3] * w[0] + jx[4] * w[1] + jx[5] * w[2]);
-        if (!live) z0 = z1 = 0.0;
-        const double e0 = jx[0] * z0 + jx[3] * z1, e1 = jx[1] * z0 + jx[4] * z1, e2 = jx[2] * z0 + jx[5] * z1;
-        acc[0] += e0;
-        acc[1] += e1;
-        acc[2] += e2;
-        acc[3] += X[1] * e2 - X[2] * e1;
-        acc[4] += X[2] * e0 - X[0] * e2;
-        acc[5] += X[0] * e1 - X[1] * e0;
+    for (uint32_t it = 0; it < SEG / (64 * VN); ++it) {
+        const uint32_t j0 = wb + VN * lane + 64 * VN * it;
+        typename VT::V jxv[6], jiv[6];
+        load_row_groups(lin.jcm, j0, jxv, jiv);
+        const typename VT::I pts = *reinterpret_cast<const typename VT::I*>(tp.cm_pt + j0);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) acc[6 + q] += ji[q] * z0 + ji[3 + q] * z1;
+        for (int i = 0; i < VN; ++i) {
+            const uint32_t j = j0 + i, p = VT::at(pts, i);
+            double X[3], w[3], jx[6], ji[6];
+            Rec::load_point(xw + Rec::XW * (size_t)p, X, w);
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                jx[r] = VT::at(jxv[r], i);
+                ji[r] = VT::at(jiv[r], i);
+            }
+            double y0, y1;
+            compact_product(jx, ji, v, X, &y0, &y1);
+            const double z0 = y0 - (jx[0] * w[0] + jx[1] * w[1] + jx[2] * w[2]);
+            const double z1 = y1 - (jx[3] * w[0] + jx[4] * w[1] + jx[5] * w[2]);
+            const double e0 = jx[0] * z0 + jx[3] * z1, e1 = jx[1] * z0 + jx[4] * z1, e2 = jx[2] * z0 + jx[5] * z1;
+            double f[9] = {e0, e1, e2, X[1] * e2 - X[2] * e1, X[2] * e0 - X[0] * e2, X[0] * e1 - X[1] * e0,
+                           ji[0] * z0 + ji[3] * z1, ji[1] * z0 + ji[4] * z1, ji[2] * z0 + ji[5] * z1};
+            const bool live = j >= cb && j < ce;
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                if (live) acc[q] += f[q];
+        }
     }
 #pragma unroll
     for (int q = 0; q < 9; ++q) acc[q] = wave_sum(acc[q]);
@@ -1763,7 +1866,7 @@ __global__ void __launch_bounds__(NTS) k_pcg_dir(Topo tp, PcgVecs V) {
             }
             if (upd)
                 for (uint32_t k = 0; k < n; ++k) dn[off + k] = upd == 1 ? zv[k] + beta * dv[k] : zv[k];
-            if (cam) write_vcam(V, tp, A, vp, vi);
+            if (cam) world_record(tp, A, V.R + 9 * (size_t)A, *V.gauge, vp, vi, V.vcam + PCG_VCAM * (size_t)A);
         }
         if (upd) s.par = 1 - s.par;
     }
@@ -1799,12 +1902,17 @@ void launch_block_gradient(hipStream_t s, const Topo& tp, const Lin<T> lin, cons
     hipLaunchKernelGGL(k_block_gradient<T>, dim3(tp.nblocks), dim3(NT), 0, s, tp, lin, pe, grad, full_grad, diag);
 }
 template <class T>
-void launch_camera_assembly(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe,
-                            const int32_t* diag_pair, const uint64_t* value_off, double* grad, double* full_grad,
-                            double* diag, double* u_vals, double* s_vals) {
+void launch_camera_assembly(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const double* R,
+                            const double* X, const GaugeDev* gauge, uint32_t nseg, const uint32_t* seg_cam,
+                            const uint32_t* cseg_off, double* part, const int32_t* diag_pair,
+                            const uint64_t* value_off, double* grad, double* full_grad, double* diag, double* u_vals,
+                            double* s_vals) {
     if (tp.nc == 0) return;
-    hipLaunchKernelGGL(k_camera_assembly<T>, dim3(tp.nc), dim3(NT), 0, s, tp, lin, pe, diag_pair, value_off, grad,
-                       full_grad, diag, u_vals, s_vals);
+    if (nseg)
+        hipLaunchKernelGGL(k_camera_assembly<T>, dim3(nseg), dim3(128), 0, s, tp, lin, pe, R, X, gauge, seg_cam,
+                           cseg_off, part);
+    hipLaunchKernelGGL(k_camera_assembly_out, dim3(tp.nc), dim3(64), 0, s, tp, cseg_off, part, diag_pair, value_off,
+                       grad, full_grad, diag, u_vals, s_vals);
 }
 template <class T>
 void launch_pairs(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, uint32_t npairs,
@@ -1832,12 +1940,25 @@ void launch_retract(hipStream_t s, const Topo& tp, const double* R, const double
     if (n == 0) return;
     hipLaunchKernelGGL(k_retract<double>, dim3(grid_for(n, NT)), dim3(NT), 0, s, tp, R, t, gp, step, gauge, Rc, tc, gpc);
 }
+// Per-camera records of the camera step in the compact form (world-frame rotation and
+// translation, intrinsic values) at the current rotations, for the candidate back-substitution.
+__global__ void __launch_bounds__(NT) k_step_records(Topo tp, const double* R, const double* step,
+                                                     const GaugeDev* gauges, double* srec) {
+    const uint32_t c = blockIdx.x * NT + threadIdx.x;
+    if (c >= tp.nc) return;
+    double vc[9];
+    camera_values(tp, c, step, vc);
+    world_record(tp, c, R + 9 * (size_t)c, gauges[tp.cam_win ? tp.cam_win[c] : 0], vc, vc + 6,
+                 srec + PCG_VCAM * (size_t)c);
+}
 template <class T>
 void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* Rc,
                       const double* tc, const double* gpc, const double* X, const double* uv, const Lin<T> lin,
-                      const PointElim<T> pe, const double* step, const GaugeDev* gauge, double* Xc, double* cscal) {
+                      const PointElim<T> pe, const double* step, const GaugeDev* gauge, double* srec, double* Xc,
+                      double* cscal) {
     if (tp.nchunks == 0) return;
-    hipLaunchKernelGGL(k_candidate<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, R, t, Rc, tc, gpc, X, uv, lin, pe, step,
+    if (tp.nc) hipLaunchKernelGGL(k_step_records, dim3(grid_for(tp.nc, NT)), dim3(NT), 0, s, tp, R, step, gauge, srec);
+    hipLaunchKernelGGL(k_candidate<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, R, t, Rc, tc, gpc, X, uv, lin, pe, srec,
                        gauge, Xc, cscal);
 }
 void launch_reduce_candidates(hipStream_t s, const double* cscal, uint32_t np, double* partials, CandScalars* out) {
@@ -1855,12 +1976,14 @@ void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pa
                        u_vals, s_vals, factors, fail);
 }
 uint32_t pcg_partials(const Topo& tp) { return grid_for(tp.nblocks, NTS); }
+template <class T>
 void launch_pcg_init(hipStream_t s, const Topo& tp, const PcgVecs& V, double tol, int max_iter) {
     const unsigned g = pcg_partials(tp);
     hipLaunchKernelGGL(k_pcg_init, dim3(g), dim3(NTS), 0, s, tp, V, tol, max_iter);
     hipLaunchKernelGGL(k_pcg_dir, dim3(g), dim3(NTS), 0, s, tp, V);
 }
 uint32_t pcg_q_partials(const Topo& tp) { return grid_for(tp.nblocks, NQ / 64); }
+template <class T>
 void launch_pcg_step(hipStream_t s, const Topo& tp, const PcgVecs& V) {
     const unsigned g = pcg_partials(tp);
     hipLaunchKernelGGL(k_pcg_q, dim3(pcg_q_partials(tp)), dim3(NQ), 0, s, tp, V);
@@ -1897,8 +2020,9 @@ template void launch_gauge<double>(hipStream_t, const double*, int, double, Gaug
     template void launch_block_gradient<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>, double*,      \
                                            double*, double*);                                                      \
     template void launch_camera_assembly<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>,              \
-                                            const int32_t*, const uint64_t*, double*, double*, double*, double*,   \
-                                            double*);                                                              \
+                                            const double*, const double*, const GaugeDev*, uint32_t,               \
+                                            const uint32_t*, const uint32_t*, double*, const int32_t*,             \
+                                            const uint64_t*, double*, double*, double*, double*, double*);         \
     template void launch_pairs<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>, uint32_t,               \
                                   const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,              \
                                   const uint32_t*, const uint64_t*, double*, double*);                             \
@@ -1908,9 +2032,12 @@ template void launch_gauge<double>(hipStream_t, const double*, int, double, Gaug
                                          const double*, const double*, const GaugeDev*, Lin<T>);                   \
     template void launch_candidate<T>(hipStream_t, const Topo&, const double*, const double*, const double*,        \
                                       const double*, const double*, const double*, const double*, const Lin<T>,    \
-                                      const PointElim<T>, const double*, const GaugeDev*, double*, double*);       \
+                                      const PointElim<T>, const double*, const GaugeDev*, double*, double*,        \
+                                      double*);                                                                    \
     template void launch_pcg_points<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>, const PcgVecs&);  \
-    template void launch_pcg_cameras<T>(hipStream_t, const Topo&, const Lin<T>, const PcgVecs&);
+    template void launch_pcg_cameras<T>(hipStream_t, const Topo&, const Lin<T>, const PcgVecs&);                     \
+    template void launch_pcg_init<T>(hipStream_t, const Topo&, const PcgVecs&, double, int);                       \
+    template void launch_pcg_step<T>(hipStream_t, const Topo&, const PcgVecs&);
 
 BA_INSTANTIATE(double)
 BA_INSTANTIATE(float)

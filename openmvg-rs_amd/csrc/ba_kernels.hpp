@@ -47,10 +47,13 @@ struct Topo {
     const uint32_t* chunk_pt;
     const uint32_t* chunk_obs;
     // Camera-major order: positions cm_off[c]..cm_off[c+1] hold camera c's observations in
-    // ascending point-major index; cm_obs gives that index and cm_pt its point.
+    // ascending point-major index; cm_obs gives that index and cm_pt its point.  The camera-major
+    // arrays are padded to soa_pad(no) positions.
     const uint32_t* cm_off;
     const uint32_t* cm_obs;
     const uint32_t* cm_pt;
+    const uint32_t* cm_cam;  // camera of position j
+    const double* cm_uv;     // pixel of position j (principal point removed), 2 per position
     // Batched windows (null for one problem): window of each camera and point, and the damping of
     // each window; the gauge argument of the kernels then holds one GaugeDev per window.
     const uint32_t* cam_win;
@@ -63,6 +66,16 @@ struct Topo {
 // Rows of the structure-of-arrays linearisation copies (each row holds one value per observation):
 // r~ (2), J~_X (2x3 row-major), J~_c (2x9 row-major).
 enum : int { SOA_R = 0, SOA_JX = 2, SOA_JC = 8, SOA_ROWS = 26 };
+// The copies are tiled: each run of SOA_TILE consecutive observations keeps its SOA_ROWS rows
+// together (SOA_ROWS x SOA_TILE values), so one wavefront's load of a row is one contiguous
+// segment and the rows it reads share DRAM pages.  soa_pad(no) positions are allocated (and the
+// per-observation index arrays padded to that length): SOA_SLACK spare positions after the last
+// observation let passes read whole fixed-size windows and aligned groups with vector loads.
+constexpr size_t SOA_TILE = 64, SOA_SLACK = 256;
+__host__ __device__ inline size_t soa_at(int row, size_t o) {
+    return (o / SOA_TILE) * (SOA_TILE * SOA_ROWS) + (size_t)row * SOA_TILE + o % SOA_TILE;
+}
+inline size_t soa_pad(size_t no) { return (no + SOA_SLACK + SOA_TILE - 1) / SOA_TILE * SOA_TILE; }
 
 // Per-observation linearisation (kernel (a)) outputs; T-typed.
 template <class T>
@@ -72,8 +85,9 @@ struct Lin {
     double* cost;  // robust cost per observation (FP64 in both forms)
     double* sq;    // squared residual norm per observation
     uint8_t* valid;
-    T* jpm;     // SOA_ROWS x no, point-major
-    T* jcm;     // SOA_ROWS x no, camera-major (position j holds observation cm_obs[j]); or null
+    T* jpm;     // tiled rows (soa_at), point-major
+    T* jcm;     // tiled rows (soa_at), camera-major (position j holds observation cm_obs[j]); or null
+    int pose_rows;  // k_linearize writes the pose columns of J~_c into jpm (else only jcm has them)
 };
 
 // Per-point elimination (kernel (b)) outputs.
@@ -113,14 +127,18 @@ void launch_point_elim(hipStream_t s, const Topo& tp, const Lin<T> lin, T lambda
 // Sum of per-point scalars in a fixed order -> AsmScalars fields cost..invalid.
 void launch_reduce_points(hipStream_t s, const double* pscal, uint32_t np, double* partials,
                           AsmScalars* out);
-// Camera-side assembly when every intrinsic group has one camera and no camera sees a point
-// twice (every Schur block term is then a single observation's): per camera, the pose and
-// intrinsic blocks' full and reduced gradients, unclamped diagonal, U and Schur diagonal blocks
-// (into the diagonal pairs' value space).
+// Camera-side assembly of the PCG form when every intrinsic group has one camera and no camera
+// sees a point twice: per camera, the pose and intrinsic blocks' full and reduced gradients,
+// unclamped diagonal, and the diagonal blocks of U - W V^-1 W^T (into the diagonal pairs' U
+// values; their Schur values are set to zero).  Runs over the PCG camera segments; part is
+// scratch of 54 doubles per segment.  The camera-major copy holds r~, J~_X and the intrinsic
+// columns; the pose columns are rebuilt from J~_X, the point X and the rotation R.
 template <class T>
-void launch_camera_assembly(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe,
-                            const int32_t* diag_pair, const uint64_t* value_off, double* grad,
-                            double* full_grad, double* diag, double* u_vals, double* s_vals);
+void launch_camera_assembly(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const double* R,
+                            const double* X, const GaugeDev* gauge, uint32_t nseg, const uint32_t* seg_cam,
+                            const uint32_t* cseg_off, double* part, const int32_t* diag_pair,
+                            const uint64_t* value_off, double* grad, double* full_grad, double* diag, double* u_vals,
+                            double* s_vals);
 // Per block: full gradient J~^T r~, reduced gradient (full minus W V^-1 g), unclamped diagonal.
 template <class T>
 void launch_block_gradient(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe,
@@ -141,13 +159,14 @@ void launch_finalize(hipStream_t s, const Topo& tp, double lambda, double* diag,
                      const double* grad, double* S, uint32_t ld, const int32_t* diag_pair,
                      const uint64_t* value_off, double* u_vals, double* s_vals, AsmScalars* sc);
 
-// Candidate pass (kernel (a')): back-substitution and candidate rule per point.
+// Candidate pass (kernel (a')): back-substitution and candidate rule per point; srec is scratch of
+// PCG_VCAM doubles per camera for the step records.
 void launch_retract(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
                     const double* step, const GaugeDev* gauge, double* Rc, double* tc, double* gpc);
 template <class T>
 void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* Rc,
                       const double* tc, const double* gpc, const double* X, const double* uv, const Lin<T> lin,
-                      const PointElim<T> pe, const double* step, const GaugeDev* gauge, double* Xc,
+                      const PointElim<T> pe, const double* step, const GaugeDev* gauge, double* srec, double* Xc,
                       double* cscal);
 void launch_reduce_candidates(hipStream_t s, const double* cscal, uint32_t np, double* partials,
                               CandScalars* out);
@@ -156,11 +175,18 @@ void launch_step_dots(hipStream_t s, uint32_t D, const double* g, const double* 
                       const double* product, double* dots);
 
 // ---- implicit-Schur PCG (kernel (c)) ----
-// The camera pass runs one wavefront per camera segment: camera c's camera-major observations
-// cm_off[c]..cm_off[c+1] are cut into pieces of PCG_SEG, segments cseg_off[c]..cseg_off[c+1].
-constexpr uint32_t PCG_SEG = 256;
+// The camera pass reads the structure-of-arrays rows in 16-byte groups, pcg_vec<T>() consecutive
+// observations per load, and runs one wavefront per camera segment: camera c's camera-major
+// positions from cm_off[c] rounded down to a multiple of pcg_vec<T>() are cut into windows of
+// pcg_seg<T>() (one group per lane), segments cseg_off[c]..cseg_off[c+1].
+template <class T>
+constexpr uint32_t pcg_vec() { return 16 / sizeof(T); }
+template <class T>
+constexpr uint32_t pcg_seg() { return 64 * pcg_vec<T>(); }
+static_assert(64 * 16 / sizeof(float) <= SOA_SLACK, "camera windows may end past the spare positions");
+static_assert(SOA_TILE % (16 / sizeof(float)) == 0, "16-byte groups stay inside one tile");
 constexpr uint32_t PCG_VCAM = 12;  // doubles per camera of the product input (9 used)
-constexpr uint32_t PCG_XW = 6;     // doubles per point of the point record (X, w)
+constexpr uint32_t PCG_XW = 6;     // doubles per point record (X, w)
 constexpr uint32_t PCG_BLOCK_WG = 64;  // workgroup of the block-parallel PCG kernels
 constexpr uint32_t PCG_Q_WG = 1024;    // workgroup of the block product pass (a wavefront per block)
 struct PcgState {
@@ -200,6 +226,7 @@ uint32_t pcg_q_partials(const Topo& tp);
 void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pair, const uint64_t* value_off,
                            const double* u_vals, const double* s_vals, double* factors, int* fail);
 // x = 0, r = -g, z = M^-1 r, the first direction and its per-camera input.
+template <class T>
 void launch_pcg_init(hipStream_t s, const Topo& tp, const PcgVecs& V, double tol, int max_iter);
 // Implicit Schur product q = S v (v = d or x by the PCG mode), S = U + lambda diag - W V^-1 W^T:
 //   point pass   w_p = V^-1 sum_o J~_X,o^T J~_c,o v                   (point chunks, point-major J)
@@ -211,6 +238,7 @@ void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const Po
 template <class T>
 void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const PcgVecs& V);
 // q = S v per block, then the step (x, r, z) and direction (d, per-camera input) updates.
+template <class T>
 void launch_pcg_step(hipStream_t s, const Topo& tp, const PcgVecs& V);
 // Camera-major linearisation: the SoA camera-major Jacobian copy (lin.jcm).
 template <class T>

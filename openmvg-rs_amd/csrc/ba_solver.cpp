@@ -475,17 +475,23 @@ int ba_hip_linearize(ba_hip_handle* h, const ba_device_problem* dp, double* resi
     Status st = lowered_from_device(dp, L, h->stream);
     if (!st.ok()) return st.code;
     Engine<double> E(h->stream, L, true);
+    E.keep_pose_rows();
     E.setup();
     E.set_gauge(gauge_from(dp));
     E.assemble(0, 0.0, L.loss, L.loss_scale, false);
     const size_t no = L.n_obs();
     hipStream_t s = h->stream;
     if (no) {
-        // structure-of-arrays rows -> per-observation records
+        // tiled structure-of-arrays rows -> per-observation records (through the host; a test hook)
+        std::vector<double> tiles(E.jpm_size()), rec;
+        HIP_OK(hipMemcpyAsync(tiles.data(), E.jpm(), tiles.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
         auto rows = [&](double* dst, int first, int count) {
-            for (int k = 0; k < count; ++k)
-                HIP_OK(hipMemcpy2DAsync(dst + k, count * sizeof(double), E.jpm() + (size_t)(first + k) * no,
-                                        sizeof(double), sizeof(double), no, hipMemcpyDeviceToDevice, s));
+            rec.resize((size_t)count * no);
+            for (size_t o = 0; o < no; ++o)
+                for (int k = 0; k < count; ++k) rec[o * count + k] = tiles[soa_at(first + k, o)];
+            HIP_OK(hipMemcpyAsync(dst, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice, s));
+            HIP_OK(hipStreamSynchronize(s));
         };
         rows(residual, SOA_R, 2);
         rows(point_jacobian, SOA_JX, 6);

@@ -81,22 +81,24 @@ def test_graph_replay_equals_stream_launches(adjuster, windows):
 
 
 def _cost_errors(got, want):
-    worst_it, worst_final = 0.0, 0.0
+    """Worst per-iteration relative cost difference, and the per-window final-cost differences."""
+    worst_it, finals = 0.0, []
     for g, w in zip(got, want):
         n = min(len(g.trace["cost"]), len(w.trace["cost"]))
         fin = np.isfinite(w.trace["cost"][:n]) & np.isfinite(g.trace["cost"][:n])
         if fin.any():
             worst_it = max(worst_it, float(np.max(np.abs(g.trace["cost"][:n][fin] - w.trace["cost"][:n][fin]) /
                                                   np.abs(w.trace["cost"][:n][fin]))))
-        worst_final = max(worst_final, abs(g.report.final_cost - w.report.final_cost) / abs(w.report.final_cost))
-    return worst_it, worst_final
+        finals.append(abs(g.report.final_cost - w.report.final_cost) / abs(w.report.final_cost))
+    return worst_it, np.array(finals)
 
 
 def test_batched_mixed_precision_error(adjuster, windows):
-    """Stated error of the mixed form after 10 LM iterations of each window: against the FP64 form
-    the trajectories part where FP32 rounding flips an accept/reject decision, so the bound is on
-    the cost reached (final cost within 1e-4 relative); the batched mixed solve and the
-    single-window mixed solve agree to the same bound."""
+    """Stated error of the mixed form after 10 LM iterations of each window.  Against the FP64 form
+    the trajectories part wherever FP32 rounding flips an accept/reject decision or an FP32 factor
+    falls back to FP64, so the statement is on the cost reached: final cost within 1e-3 relative
+    for every window and within 1e-5 for the median window; the batched mixed solve and the
+    single-window mixed solve agree to the same bounds.  The per-window values print with -s."""
     probs = [w.problem for w in windows[:12]]
     cfg = CFG.with_(precision=A.PRECISION_MIXED)
     want, _ = adjuster.solve_batched(probs, CFG)
@@ -104,13 +106,15 @@ def test_batched_mixed_precision_error(adjuster, windows):
     single = [adjuster.solve(p, cfg) for p in probs]
     it_b, fin_b = _cost_errors(got, want)
     it_s, fin_s = _cost_errors(single, want)
+    _, fin_bs = _cost_errors(got, single)
     fallbacks = sum(g.report.precision_fallbacks for g in got)
     steps = sum(g.report.iterations for g in got)
-    print(f"mixed batched vs FP64: per-iteration cost {it_b:.2e}, final cost {fin_b:.2e}; "
-          f"mixed single vs FP64: {it_s:.2e}, {fin_s:.2e}; FP64 fallbacks {fallbacks} of {steps} steps")
-    assert fin_b <= 1e-4 and fin_s <= 1e-4
-    assert max(abs(g.report.final_cost - s.report.final_cost) / abs(s.report.final_cost)
-               for g, s in zip(got, single)) <= 1e-4
+    print(f"mixed batched vs FP64: per-iteration cost {it_b:.2e}, final cost max {fin_b.max():.2e} median "
+          f"{np.median(fin_b):.2e}; mixed single vs FP64: {it_s:.2e}, max {fin_s.max():.2e} median "
+          f"{np.median(fin_s):.2e}; FP64 fallbacks {fallbacks} of {steps} steps")
+    print("per-window final cost, batched mixed vs FP64:", np.array2string(fin_b, precision=2))
+    for f in (fin_b, fin_s, fin_bs):
+        assert f.max() <= 1e-3 and np.median(f) <= 1e-5
 
 
 def test_batched_rejects_iterative_solver(adjuster, windows):
