@@ -60,27 +60,29 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     """Algorithmic HBM bytes of one launch of `kernel` in FP64: each input read once, each output
     written once; vectors indexed by camera or point are counted once per launch.
 
-    The product passes read per observation only J~_X (2x3) and the intrinsic columns of J~_c
-    (2x3), 96 B; the pose columns are rebuilt from J~_X, the point and the camera rotation.
+    The product passes rebuild the Jacobians from each observation's compact linearisation state
+    (n = (x/z, y/z), sqrt(w)/z, sqrt(w): 32 B) and the camera records (rotation, intrinsics and the
+    product input, 192 B per camera).
     pcg_points (k_pcg_points: w_p = V^-1 sum_o J~_X,o^T J~_c,o v, one workgroup per point chunk):
-      per observation the 12 rows (96 B) and its camera and point indices (8 B); per point its
-      offset (4 B), fixed flag (1 B), packed V^-1 (48 B), X (24 B) and the written record X, w_p
-      (48 B); per camera the product input (96 B).
+      per observation the compact state (32 B) and its camera and point indices (8 B); per point its
+      offset (4 B), fixed flag (1 B), packed V^-1 (48 B), X (24 B) and the written w_p record
+      (32 B); per camera its record (192 B).
     pcg_cameras (k_pcg_cameras: per camera segment of 128 positions the sums of
-      J~_c^T (J~_c v - J~_X w_p)): per observation the 12 rows of the camera-major copy (96 B) and
-      its point index (4 B); per point the record X, w_p once (48 B); per segment its camera (4 B)
-      and the written sums (72 B); per camera offsets (12 B) and the product input (96 B).
+      J~_c^T (J~_c v - J~_X w_p)): per observation the compact state and rotated point of the
+      camera-major copy (56 B) and its point index (4 B); per point the w_p record once (32 B); per
+      segment its camera (4 B) and the written sums (72 B); per camera offsets (12 B) and its
+      record (192 B).
     linearize (k_linearize): per observation pixel (16 B), camera and point index (8 B), the
-      structure-of-arrays record r~, J~_X, J~_c (208 B), cost, squared residual and flag (17 B);
-      per point X (24 B); per camera R, t, f, k1, k2 (120 B).
+      written r~, compact state and J~_X (96 B), cost, squared residual and flag (17 B); per point
+      X (24 B); per camera R, t, f, k1, k2 (120 B).
     """
     no, npt, nc = counts["obs"], counts["points"], counts["cameras"]
     if kernel == "pcg_points":
-        return no * (96 + 8) + npt * (4 + 1 + 48 + 24 + 48) + nc * 96
+        return no * (32 + 8) + npt * (4 + 1 + 48 + 24 + 32) + nc * 192
     if kernel == "pcg_cameras":
-        return no * (96 + 4) + npt * 48 + counts["segments"] * (4 + 72) + nc * (12 + 96)
+        return no * (56 + 4) + npt * 32 + counts["segments"] * (4 + 72) + nc * (12 + 192)
     if kernel == "linearize":
-        return no * (16 + 8 + 208 + 17) + npt * 24 + nc * 120
+        return no * (16 + 8 + 96 + 17) + npt * 24 + nc * 120
     raise ValueError(kernel)
 
 

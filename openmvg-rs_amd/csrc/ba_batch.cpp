@@ -213,7 +213,8 @@ class BatchEngine : public Engine<T> {
         launch_lm_solved(s, wt_, lm_.p, lam_.p, fail_w_.p, params_);
         launch_retract(s, tp, this->R_[0].p, this->t_[0].p, this->gp_[0].p, this->step_.p, this->gauge_.p,
                        this->R_[1].p, this->t_[1].p, this->gp_[1].p);
-        launch_candidate<T>(s, tp, this->R_[0].p, this->t_[0].p, this->R_[1].p, this->t_[1].p, this->gp_[1].p,
+        launch_candidate<T>(s, tp, this->R_[0].p, this->t_[0].p, this->gp_[0].p, this->R_[1].p, this->t_[1].p,
+                            this->gp_[1].p,
                             this->X_[0].p, this->uv_.p, this->lin(), this->pe(), this->step_.p, this->gauge_.p,
                             this->srec_.p, this->X_[1].p, this->cscal_.p);
         launch_win_reduce_cand(s, wt_, this->cscal_.p, cand_w_.p);

@@ -197,7 +197,7 @@ class Engine {
         HIP_OK(hipMemcpyAsync(X_[0].p, L_.X.data(), 3 * np * 8, hipMemcpyHostToDevice, s_));
         fast_cam_ = iterative_ && simple_topology();
         pose_rows_ = keep_pose_rows_ || !fast_cam_;  // the block-gradient pass reads them
-        srec_.alloc(PCG_VCAM * std::max<size_t>(1, nc));
+        srec_.alloc(PCG_CAM_REC * std::max<size_t>(1, nc));
         build_matvec_order(iterative_);
         jpm_.alloc(SOA_ROWS * npad_);
         HIP_OK(hipMemsetAsync(jpm_.p, 0, jpm_.n * sizeof(T), s_));  // finite padding
@@ -253,7 +253,7 @@ class Engine {
             HIP_OK(hipMemsetAsync(jcm_.p, 0, jcm_.n * sizeof(T), s_));  // finite padding
             pseg_.alloc(9 * std::max<size_t>(1, nseg_));
             casm_.alloc(54 * std::max<size_t>(1, nseg_));
-            pvcam_.alloc(PCG_VCAM * nc);
+            pvcam_.alloc(PCG_CAM_REC * nc);
             HIP_OK(hipMemsetAsync(pvcam_.p, 0, pvcam_.n * sizeof(double), s_));
             ppart_.alloc(std::max<size_t>(1, (B_.n_blocks + PCG_Q_WG / 64 - 1) / (PCG_Q_WG / 64)) +
                          2 * std::max<size_t>(1, (B_.n_blocks + PCG_BLOCK_WG - 1) / PCG_BLOCK_WG));
@@ -387,6 +387,7 @@ class Engine {
         V.diag = diag_.p;
         V.R = R_[lin_state_].p;
         V.X = X_[lin_state_].p;
+        V.gp = gp_[lin_state_].p;
         V.gauge = gauge_.p;
         V.lambda = lambda_;
         V.x = px_.p;
@@ -440,7 +441,7 @@ class Engine {
         launch_reduce_points(s_, pscal_.p, tp.np, partials_.p, asm_.p);
         if (fast_cam_) {
             timed(BA_PROFILE_PAIRS, [&] {
-                launch_camera_assembly<T>(s_, tp, lin(), pe(), R_[state].p, X_[state].p, gauge_.p, nseg_,
+                launch_camera_assembly<T>(s_, tp, lin(), pe(), R_[state].p, gp_[state].p, gauge_.p, nseg_,
                                           seg_cam_.p, cseg_off_.p, casm_.p, diag_pair_.p, value_off_.p, grad_.p,
                                           fgrad_.p, diag_.p, u_vals_.p, s_vals_.p);
             });
@@ -577,7 +578,7 @@ class Engine {
         tp.loss_scale = loss_scale;
         launch_retract(s_, tp, R_[state].p, t_[state].p, gp_[state].p, step_.p, gauge_.p, R_[cs].p, t_[cs].p, gp_[cs].p);
         timed(BA_PROFILE_CANDIDATE, [&] {
-            launch_candidate<T>(s_, tp, R_[state].p, t_[state].p, R_[cs].p, t_[cs].p, gp_[cs].p, X_[state].p,
+            launch_candidate<T>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, R_[cs].p, t_[cs].p, gp_[cs].p, X_[state].p,
                                 uv_.p, lin(), pe(), step_.p, gauge_.p, srec_.p, X_[cs].p, cscal_.p);
         });
         launch_reduce_candidates(s_, cscal_.p, tp.np, partials_.p, cand_.p);

@@ -99,41 +99,129 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// World-frame input of the product on camera c's Jacobian columns, from the values v of its pose
-// block (pd of 6) and of its intrinsic group (3).  With J~_X = sqrt(w) J_p R s (the point
-// Jacobian of the linearisation, J_p its camera-frame factor), the pose columns of J~_c are
-//   rotation    sqrt(w) J_p (-[R X]x) = J~_X (-[X]x) R^T / s,
-//   translation sqrt(w) J_p s         = J~_X R^T      (baseline camera: J~_X R^T (b/s) B),
-// so J~_c,pose v = J~_X (Om x X + Tr) with Om = R^T omega / s and Tr = R^T dt (baseline camera:
-// (b/s) R^T B dt).  The camera record holds [Om, Tr, intrinsic values] in FP64 in both forms:
-// the product inputs, points and w_p keep FP64 so that only the Jacobians are rounded to FP32.
-struct Rec {  // camera record 12 doubles (9 used), point record X, w (6 doubles)
-    static constexpr int XW = 6;
-    __device__ static void store_cam(double* dst, const double v[9]) {
-        double2* o = reinterpret_cast<double2*>(dst);
-        o[0] = make_double2(v[0], v[1]);
-        o[1] = make_double2(v[2], v[3]);
-        o[2] = make_double2(v[4], v[5]);
-        o[3] = make_double2(v[6], v[7]);
-        o[4] = make_double2(v[8], 0.0);
-    }
-    __device__ static void load_cam(const double* src, double v[9]) {
-        const double2* s = reinterpret_cast<const double2*>(src);
-        const double2 a = s[0], b = s[1], c = s[2], d = s[3], e = s[4];
-        v[0] = a.x, v[1] = a.y, v[2] = b.x, v[3] = b.y, v[4] = c.x, v[5] = c.y, v[6] = d.x, v[7] = d.y, v[8] = e.x;
-    }
-    __device__ static void store_point(double* dst, const double X[3], const double w[3]) {
-        double2* o = reinterpret_cast<double2*>(dst);
-        o[0] = make_double2(X[0], X[1]);
-        o[1] = make_double2(X[2], w[0]);
-        o[2] = make_double2(w[1], w[2]);
-    }
-    __device__ static void load_point(const double* src, double X[3], double w[3]) {
-        const double2* s = reinterpret_cast<const double2*>(src);
-        const double2 a = s[0], b = s[1], c = s[2];
-        X[0] = a.x, X[1] = a.y, X[2] = b.x, w[0] = b.y, w[1] = c.x, w[2] = c.y;
-    }
+// ---- compact linearisation ----
+// Per observation the linearisation keeps n = (x/z, y/z) of the camera-frame point, h = sqrt(w)/z
+// and sqrt(w) (rows SOA_Q..+3).  With r2 = |n|^2, d = 1 + k1 r2 + k2 r2^2, rd = k1 + 2 k2 r2 and
+// A = f (d I + 2 rd n n^T) (the derivative of the distorted projection), the robust-weighted
+// camera-frame projection Jacobian is sqrt(w) J_p = h A [I | -n] (model.rs:26-129), and
+//   J~_X            = s h A [I | -n] R                  (point columns, scene scale s),
+//   J~_c rotation   = h A [I | -n] (-[R X]x)            (pose motion of schur/point.rs:730-762),
+//   J~_c translation = s h A [I | -n]                   (baseline camera: h A [I | -n] b B),
+//   J~_c intrinsic  = sqrt(w) f [d n, r2 n, r2^2 n]     (log f, k1, k2).
+// So a product with J~_c or J~_X needs, per observation, these four values, R X (the rotated
+// point) and the camera's R, f, k1, k2; the passes rebuild the Jacobians instead of reading them.
+
+// Camera record (CAM_REC doubles): R (9), f, k1, k2, then a camera-frame motion: omega (3),
+// tau (3) (the translation motion: s dt, or b B dt for the baseline camera, 0 without pose) and
+// the intrinsic values (3).
+constexpr int CAM_REC = (int)PCG_CAM_REC;
+struct Cam {
+    double R[9], f, k1, k2, om[3], tau[3], di[3];
 };
+__device__ __forceinline__ void load_cam(const double* __restrict__ src, Cam& c) {
+    const double2* s = reinterpret_cast<const double2*>(src);
+    double v[22];
+#pragma unroll
+    for (int i = 0; i < 11; ++i) {
+        const double2 x = s[i];
+        v[2 * i] = x.x;
+        v[2 * i + 1] = x.y;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) c.R[i] = v[i];
+    c.f = v[9], c.k1 = v[10], c.k2 = v[11];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        c.om[i] = v[12 + i];
+        c.tau[i] = v[15 + i];
+        c.di[i] = v[18 + i];
+    }
+}
+// Record of camera c for the pose values vp (cam_dim of 6: rotation, then translation) and the
+// intrinsic values vi; R the camera's rotation, g its group's (f, k1, k2), gg its gauge.
+__device__ void camera_record(const Topo& tp, uint32_t c, const double* R, const double* g, const GaugeDev& gg,
+                              const double vp[6], const double vi[3], double* dst) {
+    const uint32_t pd = tp.cam_dim[c];
+    double v[CAM_REC];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) v[i] = R[i];
+    v[9] = g[0], v[10] = g[1], v[11] = g[2];
+    for (int i = 0; i < 3; ++i) {
+        v[12 + i] = pd > 0 ? vp[i] : 0.0;
+        v[15 + i] = 0.0;
+        v[18 + i] = vi[i];
+    }
+    if (pd > 0) {
+        if ((int)c == gg.camera) {
+            for (int i = 0; i < 3; ++i) v[15 + i] = gg.baseline * (gg.basis[i] * vp[3] + gg.basis[3 + i] * vp[4]);
+        } else {
+            for (int i = 0; i < 3; ++i) v[15 + i] = gg.scene_scale * vp[3 + i];
+        }
+    }
+    v[21] = v[22] = v[23] = 0.0;
+    double2* o = reinterpret_cast<double2*>(dst);
+#pragma unroll
+    for (int i = 0; i < CAM_REC / 2; ++i) o[i] = make_double2(v[2 * i], v[2 * i + 1]);
+}
+
+// The distortion terms of one observation.
+struct Obs {
+    double n0, n1, h, sw, r2, d, rd;
+};
+__device__ __forceinline__ Obs make_obs(double n0, double n1, double h, double sw, const Cam& c) {
+    Obs o;
+    o.n0 = n0, o.n1 = n1, o.h = h, o.sw = sw;
+    o.r2 = n0 * n0 + n1 * n1;
+    o.d = 1.0 + c.k1 * o.r2 + c.k2 * (o.r2 * o.r2);
+    o.rd = c.k1 + 2.0 * (c.k2 * o.r2);
+    return o;
+}
+// a = A b for the 2-vector b.
+__device__ __forceinline__ void apply_A(const Obs& o, const Cam& c, double b0, double b1, double* a0, double* a1) {
+    const double t = 2.0 * o.rd * (o.n0 * b0 + o.n1 * b1);
+    *a0 = c.f * (o.d * b0 + t * o.n0);
+    *a1 = c.f * (o.d * b1 + t * o.n1);
+}
+// y = J~_c v for the camera's motion record (om, tau, di) and the rotated point RX.
+__device__ __forceinline__ void camera_product(const Obs& o, const Cam& c, const double RX[3], double* y0,
+                                               double* y1) {
+    const double q0 = c.om[1] * RX[2] - c.om[2] * RX[1] + c.tau[0];
+    const double q1 = c.om[2] * RX[0] - c.om[0] * RX[2] + c.tau[1];
+    const double q2 = c.om[0] * RX[1] - c.om[1] * RX[0] + c.tau[2];
+    double a0, a1;
+    apply_A(o, c, q0 - o.n0 * q2, q1 - o.n1 * q2, &a0, &a1);
+    const double si = o.sw * c.f * (o.d * c.di[0] + o.r2 * c.di[1] + (o.r2 * o.r2) * c.di[2]);
+    *y0 = o.h * a0 + si * o.n0;
+    *y1 = o.h * a1 + si * o.n1;
+}
+// m = sqrt(w) J_p^T z (camera frame, 3-vector).
+__device__ __forceinline__ void projection_transpose(const Obs& o, const Cam& c, double z0, double z1, double m[3]) {
+    double a0, a1;
+    apply_A(o, c, z0, z1, &a0, &a1);
+    m[0] = o.h * a0;
+    m[1] = o.h * a1;
+    m[2] = -o.h * (o.n0 * a0 + o.n1 * a1);
+}
+// e = J~_X^T y (world frame, scaled): s R^T (sqrt(w) J_p^T y).
+__device__ __forceinline__ void point_transpose(const Obs& o, const Cam& c, double s, double y0, double y1,
+                                                double e[3]) {
+    double m[3];
+    projection_transpose(o, c, y0, y1, m);
+    double rt[3];
+    rotate_t(c.R, m, rt);
+    for (int i = 0; i < 3; ++i) e[i] = s * rt[i];
+}
+// t = J~_X w for a point-space vector w (world frame).
+__device__ __forceinline__ void point_product(const Obs& o, const Cam& c, double s, const double w[3], double* t0,
+                                              double* t1) {
+    double rw[3];
+    rotate(c.R, w, rw);
+    double a0, a1;
+    apply_A(o, c, rw[0] - o.n0 * rw[2], rw[1] - o.n1 * rw[2], &a0, &a1);
+    *t0 = s * o.h * a0;
+    *t1 = s * o.h * a1;
+}
+
 // Groups of 16 bytes of a structure-of-arrays row: VN consecutive observations per load.
 template <class T>
 struct Vec;
@@ -153,70 +241,9 @@ struct Vec<float> {
     __device__ static double at(const V& v, int i) { return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w; }
     __device__ static uint32_t at(const I& v, int i) { return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w; }
 };
-
-// The 12 product rows of observations j..j+VN-1 (j a multiple of VN): J~_X (SoA rows
-// SOA_JX..+5) and the intrinsic columns of J~_c (rows SOA_JC+6..8 and SOA_JC+15..17).
 template <class T>
-__device__ __forceinline__ void load_row_groups(const T* __restrict__ rows, uint32_t j,
-                                                typename Vec<T>::V jx[6], typename Vec<T>::V ji[6]) {
-    using VV = typename Vec<T>::V;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) jx[k] = *reinterpret_cast<const VV*>(rows + soa_at(SOA_JX + k, j));
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        ji[k] = *reinterpret_cast<const VV*>(rows + soa_at(SOA_JC + 6 + k, j));
-        ji[3 + k] = *reinterpret_cast<const VV*>(rows + soa_at(SOA_JC + 15 + k, j));
-    }
-}
-
-// The 12 product rows of observation j, scalar loads.
-template <class T>
-__device__ __forceinline__ void load_compact_rows(const T* __restrict__ rows, uint32_t j, double jx[6],
-                                                  double ji[6]) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) jx[k] = (double)rows[soa_at(SOA_JX + k, j)];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        ji[k] = (double)rows[soa_at(SOA_JC + 6 + k, j)];
-        ji[3 + k] = (double)rows[soa_at(SOA_JC + 15 + k, j)];
-    }
-}
-
-// Record of camera c (rotation R, gauge gg) for the pose values vp (pd of 6) and intrinsic
-// values vi.
-__device__ void world_record(const Topo& tp, uint32_t c, const double* R, const GaugeDev& gg, const double vp[6],
-                             const double vi[3], double* dst) {
-    const uint32_t pd = tp.cam_dim[c];
-    double out[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, vi[0], vi[1], vi[2]};
-    if (pd > 0) {
-        const double inv_s = 1.0 / gg.scene_scale;
-        double w[3] = {vp[0], vp[1], vp[2]}, om[3], tr[3];
-        rotate_t(R, w, om);
-        double dt[3];
-        if ((int)c == gg.camera) {
-            const double f = gg.baseline * inv_s;
-            for (int k = 0; k < 3; ++k) dt[k] = f * (gg.basis[k] * vp[3] + gg.basis[3 + k] * vp[4]);
-        } else {
-            for (int k = 0; k < 3; ++k) dt[k] = vp[3 + k];
-        }
-        rotate_t(R, dt, tr);
-        for (int k = 0; k < 3; ++k) {
-            out[k] = om[k] * inv_s;
-            out[3 + k] = tr[k];
-        }
-    }
-    Rec::store_cam(dst, out);
-}
-
-// y = J~_c v for one observation from its J~_X (jx, 2x3) and intrinsic columns (ji, 2x3), the
-// camera record v = [Om, Tr, intrinsics] and the point X.
-__device__ __forceinline__ void compact_product(const double jx[6], const double ji[6], const double v[9],
-                                                const double X[3], double* y0, double* y1) {
-    const double u0 = v[1] * X[2] - v[2] * X[1] + v[3];
-    const double u1 = v[2] * X[0] - v[0] * X[2] + v[4];
-    const double u2 = v[0] * X[1] - v[1] * X[0] + v[5];
-    *y0 = jx[0] * u0 + jx[1] * u1 + jx[2] * u2 + ji[0] * v[6] + ji[1] * v[7] + ji[2] * v[8];
-    *y1 = jx[3] * u0 + jx[4] * u1 + jx[5] * u2 + ji[3] * v[6] + ji[4] * v[7] + ji[5] * v[8];
+__device__ __forceinline__ typename Vec<T>::V load_group(const T* __restrict__ rows, int row, uint32_t j) {
+    return *reinterpret_cast<const typename Vec<T>::V*>(rows + soa_at(row, j));
 }
 }  // namespace
 
@@ -246,7 +273,7 @@ __device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t c, uint32
                                               const double* __restrict__ Rg, const double* __restrict__ tg,
                                               const double* __restrict__ gp, const double* __restrict__ X,
                                               const GaugeDev* __restrict__ gauges, double r[2], double jx[6],
-                                              double jc[18], double* cost, double* sq) {
+                                              double jc[18], double q[4], double rx[3], double* cost, double* sq) {
     const uint32_t g = tp.cam_group[c];
     const GaugeDev* gauge = gauges + (tp.cam_win ? tp.cam_win[c] : 0);
     const double* R = Rg + 9 * (size_t)c;
@@ -267,6 +294,13 @@ __device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t c, uint32
     const double s = gauge->scene_scale;
     r[0] = sw * res[0];
     r[1] = sw * res[1];
+    // compact state: n, sqrt(w)/z, sqrt(w); and the rotated point
+    const double iz = 1.0 / cp[2];
+    q[0] = cp[0] * iz;
+    q[1] = cp[1] * iz;
+    q[2] = sw * iz;
+    q[3] = sw;
+    for (int k = 0; k < 3; ++k) rx[k] = rot[k];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -314,13 +348,14 @@ __global__ void __launch_bounds__(NT) k_linearize(Topo tp, const double* __restr
                                                   Lin<T> out) {
     uint32_t o = blockIdx.x * NT + threadIdx.x;
     if (o >= tp.no) return;
-    double r[2], jx[6], jc[18], cost = 0.0, sq = 0.0;
+    double r[2], jx[6], jc[18], q[4], rx[3], cost = 0.0, sq = 0.0;
     const bool ok = linearize_obs(tp, tp.obs_cam[o], tp.obs_pt[o], uv[2 * (size_t)o], uv[2 * (size_t)o + 1], Rg, tg,
-                                  gp, X, gauge, r, jx, jc, &cost, &sq);
+                                  gp, X, gauge, r, jx, jc, q, rx, &cost, &sq);
     if (!ok) {
         for (int k = 0; k < 2; ++k) r[k] = 0.0;
         for (int k = 0; k < 6; ++k) jx[k] = 0.0;
         for (int k = 0; k < 18; ++k) jc[k] = 0.0;
+        for (int k = 0; k < 4; ++k) q[k] = 0.0;
         cost = sq = 0.0;
     }
     out.valid[o] = ok ? 1 : 0;
@@ -341,37 +376,37 @@ __global__ void __launch_bounds__(NT) k_linearize(Topo tp, const double* __restr
 #pragma unroll
     for (int k = 0; k < 2; ++k) out.jpm[soa_at(SOA_R + k, o)] = (T)r[k];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) out.jpm[soa_at(SOA_JX + k, o)] = (T)jx[k];
+    for (int k = 0; k < 4; ++k) out.jpm[soa_at(SOA_Q + k, o)] = (T)q[k];
 #pragma unroll
-    for (int k = 0; k < 18; ++k)
-        if (out.pose_rows || k % 9 >= 6) out.jpm[soa_at(SOA_JC + k, o)] = (T)jc[k];
+    for (int k = 0; k < 6; ++k) out.jpm[soa_at(SOA_JX + k, o)] = (T)jx[k];
+    if (out.pose_rows)
+#pragma unroll
+        for (int k = 0; k < 18; ++k) out.jpm[soa_at(SOA_JC + k, o)] = (T)jc[k];
 }
 
-// Camera-major twin of k_linearize: lane j linearises observation cm_obs[j] and writes the
-// structure-of-arrays camera-major Jacobian copy (coalesced reads of the camera-major camera,
-// point and pixel arrays and coalesced writes; the point itself is gathered).
+// Camera-major twin of k_linearize: lane j linearises observation cm_obs[j] and writes r~, the
+// compact state and the rotated point R X of the camera-major copy (coalesced reads of the
+// camera-major camera, point and pixel arrays and coalesced writes; the point is gathered).
 template <class T>
 __global__ void __launch_bounds__(NT) k_linearize_cm(Topo tp, const double* __restrict__ Rg,
                                                      const double* __restrict__ tg, const double* __restrict__ gp,
-                                                     const double* __restrict__ X, const double* __restrict__ uv,
-                                                     const GaugeDev* __restrict__ gauge, Lin<T> out) {
+                                                     const double* __restrict__ X, const GaugeDev* __restrict__ gauge,
+                                                     Lin<T> out) {
     uint32_t j = blockIdx.x * NT + threadIdx.x;
     if (j >= tp.no) return;
-    double r[2], jx[6], jc[18], cost, sq;
+    double r[2], jx[6], jc[18], q[4], rx[3], cost, sq;
     // camera, point and pixel of position j from the camera-major copies (coalesced)
     if (!linearize_obs(tp, tp.cm_cam[j], tp.cm_pt[j], tp.cm_uv[2 * (size_t)j], tp.cm_uv[2 * (size_t)j + 1], Rg, tg, gp,
-                       X, gauge, r, jx, jc, &cost, &sq)) {
+                       X, gauge, r, jx, jc, q, rx, &cost, &sq)) {
         for (int k = 0; k < 2; ++k) r[k] = 0.0;
-        for (int k = 0; k < 6; ++k) jx[k] = 0.0;
-        for (int k = 0; k < 18; ++k) jc[k] = 0.0;
+        for (int k = 0; k < 4; ++k) q[k] = 0.0;
     }
 #pragma unroll
     for (int k = 0; k < 2; ++k) out.jcm[soa_at(SOA_R + k, j)] = (T)r[k];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) out.jcm[soa_at(SOA_JX + k, j)] = (T)jx[k];
+    for (int k = 0; k < 4; ++k) out.jcm[soa_at(SOA_Q + k, j)] = (T)q[k];
 #pragma unroll
-    for (int k = 0; k < 18; ++k)  // the pose columns are rebuilt from J~_X where needed
-        if (k % 9 >= 6) out.jcm[soa_at(SOA_JC + k, j)] = (T)jc[k];
+    for (int k = 0; k < 3; ++k) out.jcm[soa_at(SOA_RX + k, j)] = (T)rx[k];
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -715,40 +750,45 @@ namespace {
 __device__ __forceinline__ int tri(int r, int c) { return r * (r + 1) / 2 + c; }  // r >= c
 constexpr int CA_V = 54;  // per segment: 9 full, 9 reduced part, 9 diagonal | 21 pose block, 6 intrinsic block
 
-// J~_c (2x9) of an observation from J~_X (a, 2x3), the intrinsic columns (ji, 2x3), the point X,
-// the camera rotation R, its pose dimension and the gauge.
-__device__ __forceinline__ void rebuild_camera_jacobian(const double a[6], const double ji[6], const double X[3],
-                                                        const double* R, uint32_t pd, bool baseline,
-                                                        const GaugeDev& gg, double jc[2][9]) {
-    const double inv_s = 1.0 / gg.scene_scale;
+// J~_c (2x9, camera frame) and J~_X (2x3) of an observation rebuilt from its compact state (o),
+// its rotated point RX and the camera (R, f; pose dimension pd, baseline flag, gauge gg).
+__device__ __forceinline__ void rebuild_jacobians(const Obs& o, const Cam& cam, const double RX[3], uint32_t pd,
+                                                  bool baseline, const GaugeDev& gg, double jc[2][9],
+                                                  double jx[6]) {
+    const double nn[2] = {o.n0, o.n1};
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        const double* ai = a + 3 * i;
-        const double b[3] = {X[1] * ai[2] - X[2] * ai[1], X[2] * ai[0] - X[0] * ai[2], X[0] * ai[1] - X[1] * ai[0]};
-        double rb[3], ra[3];
-        rotate(R, b, rb);
-        rotate(R, ai, ra);
+        // g = sqrt(w) J_p row i = h (A [I | -n])_i
+        double a0, a1;
+        apply_A(o, cam, i == 0 ? 1.0 : 0.0, i == 0 ? 0.0 : 1.0, &a0, &a1);  // column i of A (A symmetric)
+        const double g[3] = {o.h * a0, o.h * a1, -o.h * (o.n0 * a0 + o.n1 * a1)};
         const bool posed = pd > 0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) jc[i][k] = posed ? rb[k] * inv_s : 0.0;
+        jc[i][0] = posed ? RX[1] * g[2] - RX[2] * g[1] : 0.0;
+        jc[i][1] = posed ? RX[2] * g[0] - RX[0] * g[2] : 0.0;
+        jc[i][2] = posed ? RX[0] * g[1] - RX[1] * g[0] : 0.0;
         if (baseline) {
-            const double f = gg.baseline * inv_s;
-            jc[i][3] = posed ? f * (gg.basis[0] * ra[0] + gg.basis[1] * ra[1] + gg.basis[2] * ra[2]) : 0.0;
-            jc[i][4] = posed ? f * (gg.basis[3] * ra[0] + gg.basis[4] * ra[1] + gg.basis[5] * ra[2]) : 0.0;
+            jc[i][3] = posed ? gg.baseline * (gg.basis[0] * g[0] + gg.basis[1] * g[1] + gg.basis[2] * g[2]) : 0.0;
+            jc[i][4] = posed ? gg.baseline * (gg.basis[3] * g[0] + gg.basis[4] * g[1] + gg.basis[5] * g[2]) : 0.0;
             jc[i][5] = 0.0;
         } else {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) jc[i][3 + k] = posed ? ra[k] : 0.0;
+            for (int k = 0; k < 3; ++k) jc[i][3 + k] = posed ? gg.scene_scale * g[k] : 0.0;
         }
+        const double si = o.sw * cam.f * nn[i];
+        jc[i][6] = si * o.d;
+        jc[i][7] = si * o.r2;
+        jc[i][8] = si * (o.r2 * o.r2);
+        double rt[3];
+        rotate_t(cam.R, g, rt);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) jc[i][6 + k] = ji[3 * i + k];
+        for (int k = 0; k < 3; ++k) jx[3 * i + k] = gg.scene_scale * rt[k];
     }
 }
 }  // namespace
 
 template <class T>
 __global__ void __launch_bounds__(128) k_camera_assembly(Topo tp, Lin<T> lin, PointElim<T> pe, const double* Rg,
-                                                         const double* X, const GaugeDev* gauge,
+                                                         const double* gp, const GaugeDev* gauge,
                                                          const uint32_t* __restrict__ seg_cam,
                                                          const uint32_t* __restrict__ cseg_off, double* part) {
     const uint32_t k = blockIdx.x;
@@ -758,7 +798,10 @@ __global__ void __launch_bounds__(128) k_camera_assembly(Topo tp, Lin<T> lin, Po
     const uint32_t cb = tp.cm_off[c], ce = tp.cm_off[c + 1];
     const uint32_t wb = cb / VN * VN + (k - cseg_off[c]) * SEG;
     const GaugeDev& gg = *gauge;
-    const double* R = Rg + 9 * (size_t)c;
+    Cam cam;
+    for (int i = 0; i < 9; ++i) cam.R[i] = Rg[9 * (size_t)c + i];
+    const double* g3 = gp + 3 * (size_t)tp.cam_group[c];
+    cam.f = g3[0], cam.k1 = g3[1], cam.k2 = g3[2];
     const uint32_t pd = tp.cam_dim[c];
     const bool baseline = (int)c == gg.camera;
     double acc[27];
@@ -768,13 +811,14 @@ __global__ void __launch_bounds__(128) k_camera_assembly(Topo tp, Lin<T> lin, Po
         const uint32_t j = wb + lane + 64 * it;
         const bool live = j >= cb && j < ce;
         const uint32_t p = tp.cm_pt[j];
-        double a[6], ji[6];
-        load_compact_rows(lin.jcm, j, a, ji);
         const double r0 = (double)lin.jcm[soa_at(SOA_R, j)], r1 = (double)lin.jcm[soa_at(SOA_R + 1, j)];
-        const double Xp[3] = {X[3 * (size_t)p], X[3 * (size_t)p + 1], X[3 * (size_t)p + 2]};
+        const Obs o = make_obs((double)lin.jcm[soa_at(SOA_Q, j)], (double)lin.jcm[soa_at(SOA_Q + 1, j)],
+                               (double)lin.jcm[soa_at(SOA_Q + 2, j)], (double)lin.jcm[soa_at(SOA_Q + 3, j)], cam);
+        const double RX[3] = {(double)lin.jcm[soa_at(SOA_RX, j)], (double)lin.jcm[soa_at(SOA_RX + 1, j)],
+                              (double)lin.jcm[soa_at(SOA_RX + 2, j)]};
         const bool free_point = !tp.pt_fixed[p];
-        double jc[2][9];
-        rebuild_camera_jacobian(a, ji, Xp, R, pd, baseline, gg, jc);
+        double jc[2][9], a[6];
+        rebuild_jacobians(o, cam, RX, pd, baseline, gg, jc, a);
         if (!live) {
 #pragma unroll
             for (int q = 0; q < 9; ++q) jc[0][q] = jc[1][q] = 0.0;
@@ -1127,19 +1171,22 @@ __global__ void __launch_bounds__(NT) k_candidate(Topo tp, const double* R, cons
     const uint32_t ob = tp.chunk_obs[k], oe = tp.chunk_obs[k + 1];
     const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
     const bool big = oe - ob > NT;
-    // 1. per-observation back-substitution terms e_o = J~_X,o^T J~_c,o step (compact form, with the
-    //    cameras' step records)
+    // 1. per-observation back-substitution terms e_o = J~_X,o^T J~_c,o step (Jacobians rebuilt from
+    //    the compact state, with the cameras' step records)
     double e[3] = {0.0, 0.0, 0.0};
     for (uint32_t o = ob + lane; o < oe; o += NT) {
-        const uint32_t q = tp.obs_pt[o];
-        double v[9], jx[6], ji[6];
-        Rec::load_cam(srec + PCG_VCAM * (size_t)tp.obs_cam[o], v);
-        load_compact_rows(lin.jpm, o, jx, ji);
+        const uint32_t q = tp.obs_pt[o], c = tp.obs_cam[o];
+        Cam cam;
+        load_cam(srec + CAM_REC * (size_t)c, cam);
+        const Obs ob_ = make_obs((double)lin.jpm[soa_at(SOA_Q, o)], (double)lin.jpm[soa_at(SOA_Q + 1, o)],
+                                 (double)lin.jpm[soa_at(SOA_Q + 2, o)], (double)lin.jpm[soa_at(SOA_Q + 3, o)], cam);
         const double Xo[3] = {X[3 * (size_t)q], X[3 * (size_t)q + 1], X[3 * (size_t)q + 2]};
-        double t0, t1;
-        compact_product(jx, ji, v, Xo, &t0, &t1);
+        double RX[3], y0, y1, eo[3];
+        rotate(cam.R, Xo, RX);
+        camera_product(ob_, cam, RX, &y0, &y1);
+        point_transpose(ob_, cam, gauge[tp.cam_win ? tp.cam_win[c] : 0].scene_scale, y0, y1, eo);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) e[i] += jx[i] * t0 + jx[3 + i] * t1;
+        for (int i = 0; i < 3; ++i) e[i] += eo[i];
     }
     for (int q = 0; q < 3; ++q) she[q][lane] = e[q];
     __syncthreads();
@@ -1498,9 +1545,9 @@ __global__ void __launch_bounds__(NTS) k_pcg_init(Topo tp, PcgVecs V, double tol
     }
 }
 
-// Point pass: one workgroup per point chunk, one lane per observation;
-// e_o = J~_X,o^T (J~_c,o v) is summed per point in observation order and w_p = V^-1 sum e_o is
-// written with the point into its record (X, w_p), read by the camera pass.
+// Point pass: one workgroup per point chunk, one lane per observation; e_o = J~_X,o^T (J~_c,o v),
+// with both Jacobians rebuilt from the compact state, is summed per point in observation order and
+// w_p = V^-1 sum e_o is written to the point's record (read by the camera pass).
 template <class T>
 __global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointElim<T> pe, PcgVecs V) {
     __shared__ double sh[3][NT];
@@ -1513,7 +1560,7 @@ __global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointEli
     const uint32_t p = pb + (plane ? t : 0);
     uint32_t po0 = 0, po1 = 0;
     bool fixed = true;
-    double m[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, Xp[3] = {0.0, 0.0, 0.0};
+    double m[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (plane) {
         po0 = tp.pt_off[p];
         po1 = tp.pt_off[p + 1];
@@ -1521,30 +1568,34 @@ __global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointEli
         const T* vi = pe.vinv + 6 * (size_t)p;
 #pragma unroll
         for (int q = 0; q < 6; ++q) m[q] = (double)vi[q];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) Xp[q] = V.X[3 * (size_t)p + q];
     }
+    const double s = V.gauge->scene_scale;
     double acc[3] = {0.0, 0.0, 0.0};
     // lanes take observations ob + t, ob + t + NT, ... (one pass unless one point has > NT)
     for (uint32_t o = ob + t; o < oe; o += NT) {
         const uint32_t c = tp.obs_cam[o], q = tp.obs_pt[o];
-        double v[9], jx[6], ji[6];
-        Rec::load_cam(V.vcam + PCG_VCAM * (size_t)c, v);
-        load_compact_rows(lin.jpm, o, jx, ji);
+        const double n0 = (double)lin.jpm[soa_at(SOA_Q, o)], n1 = (double)lin.jpm[soa_at(SOA_Q + 1, o)];
+        const double h = (double)lin.jpm[soa_at(SOA_Q + 2, o)], sw = (double)lin.jpm[soa_at(SOA_Q + 3, o)];
+        Cam cam;
+        load_cam(V.vcam + CAM_REC * (size_t)c, cam);
         const double X[3] = {V.X[3 * (size_t)q], V.X[3 * (size_t)q + 1], V.X[3 * (size_t)q + 2]};
-        double y0, y1;
-        compact_product(jx, ji, v, X, &y0, &y1);
+        double RX[3];
+        rotate(cam.R, X, RX);
+        const Obs ob_ = make_obs(n0, n1, h, sw, cam);
+        double y0, y1, e[3];
+        camera_product(ob_, cam, RX, &y0, &y1);
+        point_transpose(ob_, cam, s, y0, y1, e);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) acc[i] += jx[i] * y0 + jx[3 + i] * y1;
+        for (int i = 0; i < 3; ++i) acc[i] += e[i];
     }
     sh[0][t] = acc[0];
     sh[1][t] = acc[1];
     sh[2][t] = acc[2];
     __syncthreads();
     if (oe - ob > NT) {  // a single point: fixed-shape tree over the lanes
-        for (int s = NT / 2; s > 0; s >>= 1) {
-            if ((int)t < s)
-                for (int i = 0; i < 3; ++i) sh[i][t] += sh[i][t + s];
+        for (int w = NT / 2; w > 0; w >>= 1) {
+            if ((int)t < w)
+                for (int i = 0; i < 3; ++i) sh[i][t] += sh[i][t + w];
             __syncthreads();
         }
     }
@@ -1558,15 +1609,17 @@ __global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointEli
     }
     double out[3] = {0.0, 0.0, 0.0};
     if (!fixed) sym3_mul(m, s3, out);
-    Rec::store_point(V.xw + Rec::XW * (size_t)p, Xp, out);
+    double2* rec = reinterpret_cast<double2*>(V.xw + PCG_XW * (size_t)p);
+    rec[0] = make_double2(out[0], out[1]);
+    rec[1] = make_double2(out[2], 0.0);
 }
 
 // Camera pass: one wavefront per camera segment.  Camera c's camera-major positions are cut into
 // windows of pcg_seg<T>() starting at cm_off[c] rounded down to a multiple of VN; lanes read VN
 // consecutive positions with 16-byte loads and keep those inside [cm_off[c], cm_off[c+1]).  Per
-// observation z_o = J~_c,o v - J~_X,o w_p and e_o = J~_X,o^T z_o; the segment's sums E = sum e_o,
-// F = sum X_p x e_o (world frame) and I = sum J~_int,o^T z_o go to V.seg, from which k_pcg_q forms
-// J~_c^T z = [R F / s; R E; I] (baseline camera: (b/s) B^T R E).
+// observation z_o = J~_c,o v - J~_X,o w_p and m_o = sqrt(w) J_p^T z_o (camera frame); the
+// segment's sums F = sum (R X) x m_o, M = sum m_o and I = sum J~_int,o^T z_o go to V.seg, from
+// which k_pcg_q forms J~_c^T z = [F; s M; I] (baseline camera: b B^T M).
 template <class T>
 __global__ void __launch_bounds__(NT) k_pcg_cameras(Topo tp, Lin<T> lin, PcgVecs V) {
     using VT = Vec<T>;
@@ -1579,35 +1632,37 @@ __global__ void __launch_bounds__(NT) k_pcg_cameras(Topo tp, Lin<T> lin, PcgVecs
     const uint32_t cb = tp.cm_off[c], ce = tp.cm_off[c + 1];
     constexpr uint32_t SEG = pcg_seg<T>();
     const uint32_t wb = cb / VN * VN + (k - V.cseg_off[c]) * SEG;
-    double v[9];
-    Rec::load_cam(V.vcam + PCG_VCAM * (size_t)c, v);
-    const double* __restrict__ xw = V.xw;
+    Cam cam;
+    load_cam(V.vcam + CAM_REC * (size_t)c, cam);
+    const double s = V.gauge->scene_scale;
     double acc[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) acc[q] = 0.0;
 #pragma unroll
     for (uint32_t it = 0; it < SEG / (64 * VN); ++it) {
         const uint32_t j0 = wb + VN * lane + 64 * VN * it;
-        typename VT::V jxv[6], jiv[6];
-        load_row_groups(lin.jcm, j0, jxv, jiv);
+        typename VT::V qv[4], rv[3];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) qv[r] = load_group(lin.jcm, SOA_Q + r, j0);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) rv[r] = load_group(lin.jcm, SOA_RX + r, j0);
         const typename VT::I pts = *reinterpret_cast<const typename VT::I*>(tp.cm_pt + j0);
 #pragma unroll
         for (int i = 0; i < VN; ++i) {
             const uint32_t j = j0 + i, p = VT::at(pts, i);
-            double X[3], w[3], jx[6], ji[6];
-            Rec::load_point(xw + Rec::XW * (size_t)p, X, w);
-#pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                jx[r] = VT::at(jxv[r], i);
-                ji[r] = VT::at(jiv[r], i);
-            }
-            double y0, y1;
-            compact_product(jx, ji, v, X, &y0, &y1);
-            const double z0 = y0 - (jx[0] * w[0] + jx[1] * w[1] + jx[2] * w[2]);
-            const double z1 = y1 - (jx[3] * w[0] + jx[4] * w[1] + jx[5] * w[2]);
-            const double e0 = jx[0] * z0 + jx[3] * z1, e1 = jx[1] * z0 + jx[4] * z1, e2 = jx[2] * z0 + jx[5] * z1;
-            double f[9] = {e0, e1, e2, X[1] * e2 - X[2] * e1, X[2] * e0 - X[0] * e2, X[0] * e1 - X[1] * e0,
-                           ji[0] * z0 + ji[3] * z1, ji[1] * z0 + ji[4] * z1, ji[2] * z0 + ji[5] * z1};
+            const double2* wr = reinterpret_cast<const double2*>(V.xw + PCG_XW * (size_t)p);
+            const double2 w01 = wr[0], w2 = wr[1];
+            const double w[3] = {w01.x, w01.y, w2.x};
+            const double RX[3] = {VT::at(rv[0], i), VT::at(rv[1], i), VT::at(rv[2], i)};
+            const Obs o = make_obs(VT::at(qv[0], i), VT::at(qv[1], i), VT::at(qv[2], i), VT::at(qv[3], i), cam);
+            double y0, y1, t0, t1, mm[3];
+            camera_product(o, cam, RX, &y0, &y1);
+            point_product(o, cam, s, w, &t0, &t1);
+            const double z0 = y0 - t0, z1 = y1 - t1;
+            projection_transpose(o, cam, z0, z1, mm);
+            const double nz = o.sw * cam.f * (o.n0 * z0 + o.n1 * z1);
+            double f[9] = {RX[1] * mm[2] - RX[2] * mm[1], RX[2] * mm[0] - RX[0] * mm[2], RX[0] * mm[1] - RX[1] * mm[0],
+                           mm[0], mm[1], mm[2], nz * o.d, nz * o.r2, nz * (o.r2 * o.r2)};
             const bool live = j >= cb && j < ce;
 #pragma unroll
             for (int q = 0; q < 9; ++q)
@@ -1625,7 +1680,8 @@ __global__ void __launch_bounds__(NT) k_pcg_cameras(Topo tp, Lin<T> lin, PcgVecs
 
 // Block pass, one wavefront per block: q_A = (J~^T z rows of block A: its camera's segment
 // sums, or for an intrinsic block those of the group's cameras, added lane-strided and then by
-// the wavefront butterfly) + lambda diag_A v_A; one partial of v.q per workgroup.
+// the wavefront butterfly) + lambda diag_A v_A; one partial of v.q per workgroup.  A camera's
+// rows are [F; s M] (baseline camera: [F; b B^T M]) from its segments' camera-frame sums.
 __global__ void __launch_bounds__(NQ) k_pcg_q(Topo tp, PcgVecs V) {
     __shared__ double sh[NQ / 64];
     const PcgState s = *V.st;
@@ -1638,33 +1694,27 @@ __global__ void __launch_bounds__(NQ) k_pcg_q(Topo tp, PcgVecs V) {
         const uint32_t n = tp.blk_dim[A], off = tp.blk_off[A];
         double P[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (A < tp.nc && n > 0) {
-            double E[3] = {0.0, 0.0, 0.0}, F[3] = {0.0, 0.0, 0.0};
+            double F[3] = {0.0, 0.0, 0.0}, M[3] = {0.0, 0.0, 0.0};
             for (uint32_t g = V.cseg_off[A] + lane; g < V.cseg_off[A + 1]; g += 64) {
                 const double* sg = V.seg + 9 * (size_t)g;
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    E[i] += sg[i];
-                    F[i] += sg[3 + i];
+                    F[i] += sg[i];
+                    M[i] += sg[3 + i];
                 }
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                E[i] = wave_sum(E[i]);
                 F[i] = wave_sum(F[i]);
+                M[i] = wave_sum(M[i]);
             }
             const GaugeDev& gg = *V.gauge;
-            const double* R = V.R + 9 * (size_t)A;
-            const double inv_s = 1.0 / gg.scene_scale;
-            double RF[3], RE[3];
-            rotate(R, F, RF);
-            rotate(R, E, RE);
-            for (int i = 0; i < 3; ++i) P[i] = RF[i] * inv_s;
+            for (int i = 0; i < 3; ++i) P[i] = F[i];
             if ((int)A == gg.camera) {
-                const double f = gg.baseline * inv_s;
-                P[3] = f * (gg.basis[0] * RE[0] + gg.basis[1] * RE[1] + gg.basis[2] * RE[2]);
-                P[4] = f * (gg.basis[3] * RE[0] + gg.basis[4] * RE[1] + gg.basis[5] * RE[2]);
+                P[3] = gg.baseline * (gg.basis[0] * M[0] + gg.basis[1] * M[1] + gg.basis[2] * M[2]);
+                P[4] = gg.baseline * (gg.basis[3] * M[0] + gg.basis[4] * M[1] + gg.basis[5] * M[2]);
             } else {
-                for (int i = 0; i < 3; ++i) P[3 + i] = RE[i];
+                for (int i = 0; i < 3; ++i) P[3 + i] = gg.scene_scale * M[i];
             }
         } else if (A >= tp.nc && n > 0) {
             const uint32_t gr = A - tp.nc;
@@ -1866,7 +1916,9 @@ __global__ void __launch_bounds__(NTS) k_pcg_dir(Topo tp, PcgVecs V) {
             }
             if (upd)
                 for (uint32_t k = 0; k < n; ++k) dn[off + k] = upd == 1 ? zv[k] + beta * dv[k] : zv[k];
-            if (cam) world_record(tp, A, V.R + 9 * (size_t)A, *V.gauge, vp, vi, V.vcam + PCG_VCAM * (size_t)A);
+            if (cam)
+                camera_record(tp, A, V.R + 9 * (size_t)A, V.gp + 3 * (size_t)tp.cam_group[A], *V.gauge, vp, vi,
+                              V.vcam + CAM_REC * (size_t)A);
         }
         if (upd) s.par = 1 - s.par;
     }
@@ -1903,13 +1955,13 @@ void launch_block_gradient(hipStream_t s, const Topo& tp, const Lin<T> lin, cons
 }
 template <class T>
 void launch_camera_assembly(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const double* R,
-                            const double* X, const GaugeDev* gauge, uint32_t nseg, const uint32_t* seg_cam,
+                            const double* gp, const GaugeDev* gauge, uint32_t nseg, const uint32_t* seg_cam,
                             const uint32_t* cseg_off, double* part, const int32_t* diag_pair,
                             const uint64_t* value_off, double* grad, double* full_grad, double* diag, double* u_vals,
                             double* s_vals) {
     if (tp.nc == 0) return;
     if (nseg)
-        hipLaunchKernelGGL(k_camera_assembly<T>, dim3(nseg), dim3(128), 0, s, tp, lin, pe, R, X, gauge, seg_cam,
+        hipLaunchKernelGGL(k_camera_assembly<T>, dim3(nseg), dim3(128), 0, s, tp, lin, pe, R, gp, gauge, seg_cam,
                            cseg_off, part);
     hipLaunchKernelGGL(k_camera_assembly_out, dim3(tp.nc), dim3(64), 0, s, tp, cseg_off, part, diag_pair, value_off,
                        grad, full_grad, diag, u_vals, s_vals);
@@ -1940,24 +1992,25 @@ void launch_retract(hipStream_t s, const Topo& tp, const double* R, const double
     if (n == 0) return;
     hipLaunchKernelGGL(k_retract<double>, dim3(grid_for(n, NT)), dim3(NT), 0, s, tp, R, t, gp, step, gauge, Rc, tc, gpc);
 }
-// Per-camera records of the camera step in the compact form (world-frame rotation and
-// translation, intrinsic values) at the current rotations, for the candidate back-substitution.
-__global__ void __launch_bounds__(NT) k_step_records(Topo tp, const double* R, const double* step,
+// Per-camera records of the camera step (camera-frame motion, intrinsic values, with the current
+// rotation and intrinsics) for the candidate back-substitution.
+__global__ void __launch_bounds__(NT) k_step_records(Topo tp, const double* R, const double* gp, const double* step,
                                                      const GaugeDev* gauges, double* srec) {
     const uint32_t c = blockIdx.x * NT + threadIdx.x;
     if (c >= tp.nc) return;
     double vc[9];
     camera_values(tp, c, step, vc);
-    world_record(tp, c, R + 9 * (size_t)c, gauges[tp.cam_win ? tp.cam_win[c] : 0], vc, vc + 6,
-                 srec + PCG_VCAM * (size_t)c);
+    camera_record(tp, c, R + 9 * (size_t)c, gp + 3 * (size_t)tp.cam_group[c], gauges[tp.cam_win ? tp.cam_win[c] : 0],
+                  vc, vc + 6, srec + CAM_REC * (size_t)c);
 }
 template <class T>
-void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* Rc,
-                      const double* tc, const double* gpc, const double* X, const double* uv, const Lin<T> lin,
-                      const PointElim<T> pe, const double* step, const GaugeDev* gauge, double* srec, double* Xc,
-                      double* cscal) {
+void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
+                      const double* Rc, const double* tc, const double* gpc, const double* X, const double* uv,
+                      const Lin<T> lin, const PointElim<T> pe, const double* step, const GaugeDev* gauge, double* srec,
+                      double* Xc, double* cscal) {
     if (tp.nchunks == 0) return;
-    if (tp.nc) hipLaunchKernelGGL(k_step_records, dim3(grid_for(tp.nc, NT)), dim3(NT), 0, s, tp, R, step, gauge, srec);
+    if (tp.nc)
+        hipLaunchKernelGGL(k_step_records, dim3(grid_for(tp.nc, NT)), dim3(NT), 0, s, tp, R, gp, step, gauge, srec);
     hipLaunchKernelGGL(k_candidate<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, R, t, Rc, tc, gpc, X, uv, lin, pe, srec,
                        gauge, Xc, cscal);
 }
@@ -2002,7 +2055,8 @@ template <class T>
 void launch_linearize_cm(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
                          const double* X, const double* uv, const GaugeDev* gauge, Lin<T> out) {
     if (tp.no == 0) return;
-    hipLaunchKernelGGL(k_linearize_cm<T>, dim3(grid_for(tp.no, NT)), dim3(NT), 0, s, tp, R, t, gp, X, uv, gauge, out);
+    (void)uv;  // the camera-major pixels are in tp.cm_uv
+    hipLaunchKernelGGL(k_linearize_cm<T>, dim3(grid_for(tp.no, NT)), dim3(NT), 0, s, tp, R, t, gp, X, gauge, out);
 }
 
 __global__ void k_negate(uint32_t n, const double* in, double* out) {
@@ -2031,6 +2085,7 @@ template void launch_gauge<double>(hipStream_t, const double*, int, double, Gaug
     template void launch_linearize_cm<T>(hipStream_t, const Topo&, const double*, const double*, const double*,     \
                                          const double*, const double*, const GaugeDev*, Lin<T>);                   \
     template void launch_candidate<T>(hipStream_t, const Topo&, const double*, const double*, const double*,        \
+                                      const double*,                                                               \
                                       const double*, const double*, const double*, const double*, const Lin<T>,    \
                                       const PointElim<T>, const double*, const GaugeDev*, double*, double*,        \
                                       double*);                                                                    \

@@ -64,8 +64,10 @@ struct Topo {
 };
 
 // Rows of the structure-of-arrays linearisation copies (each row holds one value per observation):
-// r~ (2), J~_X (2x3 row-major), J~_c (2x9 row-major).
-enum : int { SOA_R = 0, SOA_JX = 2, SOA_JC = 8, SOA_ROWS = 26 };
+// r~ (2); the compact state n = (x/z, y/z), sqrt(w)/z, sqrt(w) (4); the rotated point R X (3,
+// camera-major copy only); J~_X (2x3 row-major, point-major copy); J~_c (2x9 row-major, point-major
+// copy, written only for the block-gradient path and the per-observation hook).
+enum : int { SOA_R = 0, SOA_Q = 2, SOA_RX = 6, SOA_JX = 9, SOA_JC = 15, SOA_ROWS = 33 };
 // The copies are tiled: each run of SOA_TILE consecutive observations keeps its SOA_ROWS rows
 // together (SOA_ROWS x SOA_TILE values), so one wavefront's load of a row is one contiguous
 // segment and the rows it reads share DRAM pages.  soa_pad(no) positions are allocated (and the
@@ -131,11 +133,11 @@ void launch_reduce_points(hipStream_t s, const double* pscal, uint32_t np, doubl
 // sees a point twice: per camera, the pose and intrinsic blocks' full and reduced gradients,
 // unclamped diagonal, and the diagonal blocks of U - W V^-1 W^T (into the diagonal pairs' U
 // values; their Schur values are set to zero).  Runs over the PCG camera segments; part is
-// scratch of 54 doubles per segment.  The camera-major copy holds r~, J~_X and the intrinsic
-// columns; the pose columns are rebuilt from J~_X, the point X and the rotation R.
+// scratch of 54 doubles per segment.  The Jacobians are rebuilt from the camera-major compact state
+// and rotated points with the rotations R and group intrinsics gp.
 template <class T>
 void launch_camera_assembly(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const double* R,
-                            const double* X, const GaugeDev* gauge, uint32_t nseg, const uint32_t* seg_cam,
+                            const double* gp, const GaugeDev* gauge, uint32_t nseg, const uint32_t* seg_cam,
                             const uint32_t* cseg_off, double* part, const int32_t* diag_pair,
                             const uint64_t* value_off, double* grad, double* full_grad, double* diag, double* u_vals,
                             double* s_vals);
@@ -159,15 +161,16 @@ void launch_finalize(hipStream_t s, const Topo& tp, double lambda, double* diag,
                      const double* grad, double* S, uint32_t ld, const int32_t* diag_pair,
                      const uint64_t* value_off, double* u_vals, double* s_vals, AsmScalars* sc);
 
-// Candidate pass (kernel (a')): back-substitution and candidate rule per point; srec is scratch of
-// PCG_VCAM doubles per camera for the step records.
+// Candidate pass (kernel (a')): back-substitution and candidate rule per point; R, t, gp the
+// current state, Rc, tc, gpc the candidate cameras; srec is scratch of PCG_CAM_REC doubles per
+// camera for the step records.
 void launch_retract(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
                     const double* step, const GaugeDev* gauge, double* Rc, double* tc, double* gpc);
 template <class T>
-void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* Rc,
-                      const double* tc, const double* gpc, const double* X, const double* uv, const Lin<T> lin,
-                      const PointElim<T> pe, const double* step, const GaugeDev* gauge, double* srec, double* Xc,
-                      double* cscal);
+void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
+                      const double* Rc, const double* tc, const double* gpc, const double* X, const double* uv,
+                      const Lin<T> lin, const PointElim<T> pe, const double* step, const GaugeDev* gauge, double* srec,
+                      double* Xc, double* cscal);
 void launch_reduce_candidates(hipStream_t s, const double* cscal, uint32_t np, double* partials,
                               CandScalars* out);
 // dots[0] = g.x, dots[1] = diag.x^2, dots[2] = x.x, dots[3] = x.product
@@ -185,8 +188,8 @@ template <class T>
 constexpr uint32_t pcg_seg() { return 64 * pcg_vec<T>(); }
 static_assert(64 * 16 / sizeof(float) <= SOA_SLACK, "camera windows may end past the spare positions");
 static_assert(SOA_TILE % (16 / sizeof(float)) == 0, "16-byte groups stay inside one tile");
-constexpr uint32_t PCG_VCAM = 12;  // doubles per camera of the product input (9 used)
-constexpr uint32_t PCG_XW = 6;     // doubles per point record (X, w)
+constexpr uint32_t PCG_CAM_REC = 24;  // doubles per camera record: R, f, k1, k2, motion (21 used)
+constexpr uint32_t PCG_XW = 4;        // doubles per point record: the point pass output w (3 used)
 constexpr uint32_t PCG_BLOCK_WG = 64;  // workgroup of the block-parallel PCG kernels
 constexpr uint32_t PCG_Q_WG = 1024;    // workgroup of the block product pass (a wavefront per block)
 struct PcgState {
@@ -204,13 +207,14 @@ struct PcgVecs {
     const double* diag;     // D, clamped damping diagonal
     const double* R;        // 9 per camera: rotations of the linearisation state
     const double* X;        // 3 per point: points of the linearisation state
+    const double* gp;       // 3 per group: intrinsics (f, k1, k2) of the linearisation state
     const GaugeDev* gauge;  // gauge of the linearisation
     double lambda;
     double *x, *r, *z, *q;  // D each
     double* d[2];           // D each: the direction alternates between the two
-    double* xw;    // PCG_XW per point: X and the point pass output w
+    double* xw;    // PCG_XW per point: the point pass output w
     double* seg;   // 9 per camera segment: the camera pass sums (E, F, I)
-    double* vcam;  // PCG_VCAM per camera: the product input in the world frame
+    double* vcam;  // PCG_CAM_REC per camera: rotation, intrinsics and the product input
     double* part;  // per-workgroup partial dot products: v.q (nqparts), r.r and r.z (nparts each)
     uint32_t nqparts, nparts;
     uint32_t nseg;
