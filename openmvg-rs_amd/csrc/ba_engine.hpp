@@ -254,6 +254,13 @@ class Engine {
             pseg_.alloc(9 * std::max<size_t>(1, nseg_));
             casm_.alloc(54 * std::max<size_t>(1, nseg_));
             pvcam_.alloc(PCG_CAM_REC * nc);
+            pwcam_.alloc(PCG_WREC * std::max<size_t>(1, nc));
+            {
+                int dev = 0, cu = 0;
+                HIP_OK(hipGetDevice(&dev));
+                HIP_OK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+                ncu_ = (uint32_t)std::max(cu, 1);
+            }
             HIP_OK(hipMemsetAsync(pvcam_.p, 0, pvcam_.n * sizeof(double), s_));
             ppart_.alloc(std::max<size_t>(1, (B_.n_blocks + PCG_Q_WG / 64 - 1) / (PCG_Q_WG / 64)) +
                          2 * std::max<size_t>(1, (B_.n_blocks + PCG_BLOCK_WG - 1) / PCG_BLOCK_WG));
@@ -274,6 +281,8 @@ class Engine {
         tp_.no = (uint32_t)no;
         tp_.D = (uint32_t)D;
         tp_.nblocks = (uint32_t)B_.n_blocks;
+        gturn_.alloc(std::max<size_t>(1, ng));
+        tp_.gturn = gturn_.p;
         tp_.obs_cam = obs_cam_.p;
         tp_.obs_pt = obs_pt_.p;
         tp_.pt_off = pt_off_.p;
@@ -343,6 +352,27 @@ class Engine {
         }
         chunk_pt_.upload(cpt, s_);
         chunk_obs_.upload(cob, s_);
+        if (camera_major) {  // chunks of the PCG point pass: whole points in one wavefront's lanes
+            std::vector<uint32_t> ppt{0}, pob{0};
+            uint32_t first = 0, pts = 0;
+            for (size_t p = 0; p < np; ++p) {
+                const uint32_t end = L_.point_off[p + 1];
+                if (pts > 0 && end - first > PCG_WAVE_OBS) {
+                    ppt.push_back((uint32_t)p);
+                    pob.push_back(L_.point_off[p]);
+                    first = L_.point_off[p];
+                    pts = 0;
+                }
+                pts += 1;
+            }
+            if (pts > 0) {
+                ppt.push_back((uint32_t)np);
+                pob.push_back((uint32_t)no);
+            }
+            npchunks_ = (uint32_t)(ppt.size() - 1);
+            pchunk_pt_.upload(ppt, s_);
+            pchunk_obs_.upload(pob, s_);
+        }
         if (!camera_major) {
             HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
             return;
@@ -403,6 +433,11 @@ class Engine {
         V.nparts = pcg_partials(tp_);
         V.nqparts = pcg_q_partials(tp_);
         V.nseg = nseg_;
+        V.npchunks = npchunks_;
+        V.pchunk_pt = pchunk_pt_.p;
+        V.pchunk_obs = pchunk_obs_.p;
+        V.ncu = ncu_;
+        V.wcam = pwcam_.p;
         V.seg_cam = seg_cam_.p;
         V.cseg_off = cseg_off_.p;
         V.st = pcg_.p;
@@ -628,16 +663,17 @@ class Engine {
     Topo tp_{};
     Buf<uint32_t> obs_cam_, obs_pt_, pt_off_, cam_group_, cam_off_, cam_dim_, grp_off_, grp_dim_, blk_off_, blk_dim_,
         blk_obs_off_, blk_obs_, grp_cam_off_, grp_cams_, pair_a_, pair_b_, term_off_, term_i_, term_j_, chunk_pt_,
-        chunk_obs_, cm_off_, cm_obs_, cm_pt_, cm_cam_, cseg_off_, seg_cam_;
-    uint32_t nseg_ = 0;
+        chunk_obs_, cm_off_, cm_obs_, cm_pt_, cm_cam_, cseg_off_, seg_cam_, pchunk_pt_, pchunk_obs_;
+    uint32_t nseg_ = 0, npchunks_ = 0, ncu_ = 1;
     size_t npad_ = 0;  // padded observation count of the tiled copies and index arrays
     Buf<uint8_t> pt_fixed_, valid_;
     Buf<uint64_t> value_off_;
     Buf<int32_t> diag_pair_, edge_pair_;
     Buf<double> uv_, R_[2], t_[2], gp_[2], X_[2], ocost_, osq_, pscal_, cscal_, grad_, fgrad_, diag_, step_, rhs_,
         partials_, u_vals_, s_vals_, S_, work_, factors_, px_, pr_, pz_, pd_[2], pq_, dots_, pseg_, ppart_;
-    Buf<double> pxw_, pvcam_;  // PCG point and camera records
+    Buf<double> pxw_, pvcam_, pwcam_;  // PCG point, camera and world camera records
     Buf<double> srec_;         // candidate step records
+    Buf<double> gturn_;        // first radial turn per group
     Buf<double> cm_uv_;        // pixels in camera-major order
     Buf<double> casm_;         // camera assembly: per-segment sums
     Buf<T> jc_, vinv_, vdamp_, y_, ddiag_, rrows_, jpm_, jcm_;

@@ -164,6 +164,40 @@ __device__ void camera_record(const Topo& tp, uint32_t c, const double* R, const
     for (int i = 0; i < CAM_REC / 2; ++i) o[i] = make_double2(v[2 * i], v[2 * i + 1]);
 }
 
+// World-frame record of a camera motion (WREC doubles): Om = R^T omega / s, T = R^T tau / s and
+// the intrinsic terms A = f d0, B = f (k1 d0 + d1), C = f (k2 d0 + d2), so that with the
+// point-major J~_X and the compact state (n, sqrt(w)):
+//   J~_c v = J~_X (Om x X + T) + sqrt(w) n (A + r2 B + r2^2 C).
+// Built from a camera record (camera frame) and the scene scale.
+constexpr int WREC = (int)PCG_WREC;
+__device__ __forceinline__ void world_from_camera(const Cam& c, double s, double w[WREC]) {
+    double om[3], tr[3];
+    rotate_t(c.R, c.om, om);
+    rotate_t(c.R, c.tau, tr);
+    const double is = 1.0 / s;
+    for (int i = 0; i < 3; ++i) {
+        w[i] = om[i] * is;
+        w[3 + i] = tr[i] * is;
+    }
+    w[6] = c.f * c.di[0];
+    w[7] = c.f * (c.k1 * c.di[0] + c.di[1]);
+    w[8] = c.f * (c.k2 * c.di[0] + c.di[2]);
+}
+// e = J~_X^T (J~_c v) for one observation from J~_X (jx), n, sqrt(w), its point X and the
+// camera's world record w.
+__device__ __forceinline__ void world_product(const double jx[6], double n0, double n1, double sw, const double X[3],
+                                              const double* w, double e[3]) {
+    const double u0 = w[1] * X[2] - w[2] * X[1] + w[3];
+    const double u1 = w[2] * X[0] - w[0] * X[2] + w[4];
+    const double u2 = w[0] * X[1] - w[1] * X[0] + w[5];
+    const double r2 = n0 * n0 + n1 * n1;
+    const double si = sw * (w[6] + r2 * (w[7] + r2 * w[8]));
+    const double y0 = jx[0] * u0 + jx[1] * u1 + jx[2] * u2 + si * n0;
+    const double y1 = jx[3] * u0 + jx[4] * u1 + jx[5] * u2 + si * n1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) e[i] = jx[i] * y0 + jx[3 + i] * y1;
+}
+
 // The distortion terms of one observation.
 struct Obs {
     double n0, n1, h, sw, r2, d, rd;
@@ -282,8 +316,8 @@ __device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t c, uint32
     rotate(R, Xp, rot);
     double cp[3] = {rot[0] + tg[3 * c], rot[1] + tg[3 * c + 1], rot[2] + tg[3 * c + 2]};
     const double f = gp[3 * g], k1 = gp[3 * g + 1], k2 = gp[3 * g + 2];
-    double turn = 0.0;
-    bool has_turn = first_turn(k1, k2, &turn);
+    const double turn = tp.gturn[g];  // first radial turn of the group (+inf: none)
+    const bool has_turn = turn < INFINITY;
     double res[2], J[2][3], Ji[2][3];
     if (!radial_linearize<double>(f, k1, k2, turn, has_turn, cp, mu, mv, res, J, Ji))
         return false;
@@ -1130,8 +1164,8 @@ __device__ bool admitted_cost(const Topo& tp, uint32_t b, uint32_t e, const doub
         rotate(Rc, X, rot);
         double cp[3] = {rot[0] + t[3 * c], rot[1] + t[3 * c + 1], rot[2] + t[3 * c + 2]};
         const double f = gpc[3 * g], k1 = gpc[3 * g + 1], k2 = gpc[3 * g + 2];
-        double turn = 0.0;
-        bool has_turn = first_turn(k1, k2, &turn);
+        const double turn = tp.gturn[g];  // first radial turn of the group (+inf: none)
+        const bool has_turn = turn < INFINITY;
         double res[2];
         if (!radial_residual<double>(f, k1, k2, turn, has_turn, cp, uv[2 * (size_t)o], uv[2 * (size_t)o + 1], res))
             return false;
@@ -1270,8 +1304,8 @@ __global__ void __launch_bounds__(NT) k_candidate(Topo tp, const double* R, cons
         rotate(Rm, Xs, rot);
         double cp[3] = {rot[0] + tc[3 * c], rot[1] + tc[3 * c + 1], rot[2] + tc[3 * c + 2]};
         const double f = gpc[3 * g], k1 = gpc[3 * g + 1], k2 = gpc[3 * g + 2];
-        double turn = 0.0;
-        bool has_turn = first_turn(k1, k2, &turn);
+        const double turn = tp.gturn[g];  // first radial turn of the group (+inf: none)
+        const bool has_turn = turn < INFINITY;
         double res[2];
         const bool ok = radial_residual<double>(f, k1, k2, turn, has_turn, cp, uv[2 * (size_t)o], uv[2 * (size_t)o + 1], res);
         double w[3];
@@ -1545,73 +1579,85 @@ __global__ void __launch_bounds__(NTS) k_pcg_init(Topo tp, PcgVecs V, double tol
     }
 }
 
-// Point pass: one workgroup per point chunk, one lane per observation; e_o = J~_X,o^T (J~_c,o v),
-// with both Jacobians rebuilt from the compact state, is summed per point in observation order and
-// w_p = V^-1 sum e_o is written to the point's record (read by the camera pass).
-template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_points(Topo tp, Lin<T> lin, PointElim<T> pe, PcgVecs V) {
-    __shared__ double sh[3][NT];
+// Point pass: wavefronts of a persistent grid take point chunks (whole points whose observations
+// fit the 64 lanes, or one point with more) in turn, one lane per observation.  The cameras'
+// world records sit in LDS when they fit (CAM_LDS), else they are read from global memory.
+// e_o = J~_X,o^T (J~_c,o v) is summed per point by a segmented scan over the lanes (a fixed
+// shuffle order), and the lane of each point's last observation forms w_p = V^-1 sum e_o and
+// writes the point's record (read by the camera pass).
+constexpr int PT_NT = 1024;
+constexpr int PT_TABLE = 2048 * WREC;  // doubles of the LDS camera table
+template <class T, bool CAM_LDS>
+__global__ void __launch_bounds__(PT_NT) k_pcg_points(Topo tp, Lin<T> lin, PointElim<T> pe, PcgVecs V) {
+    __shared__ double tab[CAM_LDS ? PT_TABLE : 1];
     if (V.st->done) return;
-    const uint32_t k = blockIdx.x, t = threadIdx.x;
-    const uint32_t ob = tp.chunk_obs[k], oe = tp.chunk_obs[k + 1];
-    const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
-    // the point lanes' inputs, loaded first so that they are in flight with the observation pass
-    const bool plane = t < pend - pb;
-    const uint32_t p = pb + (plane ? t : 0);
-    uint32_t po0 = 0, po1 = 0;
-    bool fixed = true;
-    double m[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (plane) {
-        po0 = tp.pt_off[p];
-        po1 = tp.pt_off[p + 1];
-        fixed = tp.pt_fixed[p] != 0;
-        const T* vi = pe.vinv + 6 * (size_t)p;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) m[q] = (double)vi[q];
+    if (CAM_LDS) {
+        for (uint32_t i = threadIdx.x; i < WREC * tp.nc; i += PT_NT) tab[i] = V.wcam[i];
+        __syncthreads();
     }
-    const double s = V.gauge->scene_scale;
-    double acc[3] = {0.0, 0.0, 0.0};
-    // lanes take observations ob + t, ob + t + NT, ... (one pass unless one point has > NT)
-    for (uint32_t o = ob + t; o < oe; o += NT) {
-        const uint32_t c = tp.obs_cam[o], q = tp.obs_pt[o];
-        const double n0 = (double)lin.jpm[soa_at(SOA_Q, o)], n1 = (double)lin.jpm[soa_at(SOA_Q + 1, o)];
-        const double h = (double)lin.jpm[soa_at(SOA_Q + 2, o)], sw = (double)lin.jpm[soa_at(SOA_Q + 3, o)];
-        Cam cam;
-        load_cam(V.vcam + CAM_REC * (size_t)c, cam);
-        const double X[3] = {V.X[3 * (size_t)q], V.X[3 * (size_t)q + 1], V.X[3 * (size_t)q + 2]};
-        double RX[3];
-        rotate(cam.R, X, RX);
-        const Obs ob_ = make_obs(n0, n1, h, sw, cam);
-        double y0, y1, e[3];
-        camera_product(ob_, cam, RX, &y0, &y1);
-        point_transpose(ob_, cam, s, y0, y1, e);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nw = gridDim.x * (PT_NT / 64);
+    for (uint32_t k = blockIdx.x * (PT_NT / 64) + (threadIdx.x >> 6); k < V.npchunks; k += nw) {
+        const uint32_t ob = V.pchunk_obs[k], oe = V.pchunk_obs[k + 1];
+        const bool big = oe - ob > 64;
+        double acc[3] = {0.0, 0.0, 0.0};
+        uint32_t q = 0, pend = 0, head = lane;
+        for (uint32_t o = ob + lane; o < (big ? oe : ob + 64); o += 64) {
+            const bool live = o < oe;
+            const uint32_t oo = live ? o : ob;  // in-range stand-in, its terms dropped below
+            const uint32_t c = tp.obs_cam[oo];
+            q = tp.obs_pt[oo];
+            double jx[6];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) acc[i] += e[i];
-    }
-    sh[0][t] = acc[0];
-    sh[1][t] = acc[1];
-    sh[2][t] = acc[2];
-    __syncthreads();
-    if (oe - ob > NT) {  // a single point: fixed-shape tree over the lanes
-        for (int w = NT / 2; w > 0; w >>= 1) {
-            if ((int)t < w)
-                for (int i = 0; i < 3; ++i) sh[i][t] += sh[i][t + w];
-            __syncthreads();
+            for (int i = 0; i < 6; ++i) jx[i] = (double)lin.jpm[soa_at(SOA_JX + i, oo)];
+            const double n0 = (double)lin.jpm[soa_at(SOA_Q, oo)], n1 = (double)lin.jpm[soa_at(SOA_Q + 1, oo)];
+            const double sw = (double)lin.jpm[soa_at(SOA_Q + 3, oo)];
+            const double X[3] = {V.X[3 * (size_t)q], V.X[3 * (size_t)q + 1], V.X[3 * (size_t)q + 2]};
+            if (!big) {
+                head = live ? tp.pt_off[q] - ob : lane;
+                pend = tp.pt_off[q + 1];
+            }
+            double w[WREC];
+            const double* src = CAM_LDS ? tab + WREC * c : V.wcam + WREC * (size_t)c;
+#pragma unroll
+            for (int i = 0; i < WREC; ++i) w[i] = src[i];
+            double e[3];
+            world_product(jx, n0, n1, sw, X, w, e);
+            if (live)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) acc[i] += e[i];
+        }
+        bool last;
+        if (big) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) acc[i] = wave_sum(acc[i]);
+            q = tp.obs_pt[ob];
+            last = lane == 0;
+        } else {
+            // inclusive segmented scan: lane l ends with the sum over lanes head..l of its point
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                double sv[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) sv[i] = __shfl_up(acc[i], d, 64);
+                if ((int)lane - d >= (int)head)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) acc[i] += sv[i];
+            }
+            last = ob + lane < oe && ob + lane + 1 == pend;
+        }
+        if (last) {
+            double out[3] = {0.0, 0.0, 0.0};
+            if (!tp.pt_fixed[q]) {
+                const T* vi = pe.vinv + 6 * (size_t)q;
+                const double m[6] = {(double)vi[0], (double)vi[1], (double)vi[2], (double)vi[3], (double)vi[4], (double)vi[5]};
+                sym3_mul(m, acc, out);
+            }
+            double2* rec = reinterpret_cast<double2*>(V.xw + PCG_XW * (size_t)q);
+            rec[0] = make_double2(out[0], out[1]);
+            rec[1] = make_double2(out[2], 0.0);
         }
     }
-    if (!plane) return;
-    double s3[3] = {0.0, 0.0, 0.0};
-    if (oe - ob > NT) {
-        for (int i = 0; i < 3; ++i) s3[i] = sh[i][0];
-    } else {
-        for (uint32_t o = po0; o < po1; ++o)
-            for (int i = 0; i < 3; ++i) s3[i] += sh[i][o - ob];
-    }
-    double out[3] = {0.0, 0.0, 0.0};
-    if (!fixed) sym3_mul(m, s3, out);
-    double2* rec = reinterpret_cast<double2*>(V.xw + PCG_XW * (size_t)p);
-    rec[0] = make_double2(out[0], out[1]);
-    rec[1] = make_double2(out[2], 0.0);
 }
 
 // Camera pass: one wavefront per camera segment.  Camera c's camera-major positions are cut into
@@ -1916,13 +1962,31 @@ __global__ void __launch_bounds__(NTS) k_pcg_dir(Topo tp, PcgVecs V) {
             }
             if (upd)
                 for (uint32_t k = 0; k < n; ++k) dn[off + k] = upd == 1 ? zv[k] + beta * dv[k] : zv[k];
-            if (cam)
-                camera_record(tp, A, V.R + 9 * (size_t)A, V.gp + 3 * (size_t)tp.cam_group[A], *V.gauge, vp, vi,
-                              V.vcam + CAM_REC * (size_t)A);
+            if (cam) {
+                double* rec = V.vcam + CAM_REC * (size_t)A;
+                camera_record(tp, A, V.R + 9 * (size_t)A, V.gp + 3 * (size_t)tp.cam_group[A], *V.gauge, vp, vi, rec);
+                Cam cr;
+                load_cam(rec, cr);
+                double w[WREC];
+                world_from_camera(cr, V.gauge->scene_scale, w);
+                for (int i = 0; i < WREC; ++i) V.wcam[WREC * (size_t)A + i] = w[i];
+            }
         }
         if (upd) s.par = 1 - s.par;
     }
     if (lead) *V.st = s;
+}
+
+// First radial turn of each group's (k1, k2) (model.rs:131-161), +inf where there is none; read by
+// the kernels that project with these intrinsics.
+__global__ void __launch_bounds__(NT) k_group_turns(uint32_t ng, const double* gp, double* turn) {
+    const uint32_t g = blockIdx.x * NT + threadIdx.x;
+    if (g >= ng) return;
+    double r = 0.0;
+    turn[g] = first_turn(gp[3 * g + 1], gp[3 * g + 2], &r) ? r : INFINITY;
+}
+void launch_group_turns(hipStream_t s, const Topo& tp, const double* gp) {
+    if (tp.ng) hipLaunchKernelGGL(k_group_turns, dim3(grid_for(tp.ng, NT)), dim3(NT), 0, s, tp.ng, gp, tp.gturn);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1936,6 +2000,7 @@ template <class T>
 void launch_linearize(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
                       const double* X, const double* uv, const GaugeDev* gauge, Lin<T> out) {
     if (tp.no == 0) return;
+    launch_group_turns(s, tp, gp);
     hipLaunchKernelGGL(k_linearize<T>, dim3(grid_for(tp.no, NT)), dim3(NT), 0, s, tp, R, t, gp, X, uv, gauge, out);
 }
 template <class T>
@@ -2011,6 +2076,7 @@ void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const doub
     if (tp.nchunks == 0) return;
     if (tp.nc)
         hipLaunchKernelGGL(k_step_records, dim3(grid_for(tp.nc, NT)), dim3(NT), 0, s, tp, R, gp, step, gauge, srec);
+    launch_group_turns(s, tp, gpc);
     hipLaunchKernelGGL(k_candidate<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, R, t, Rc, tc, gpc, X, uv, lin, pe, srec,
                        gauge, Xc, cscal);
 }
@@ -2045,7 +2111,15 @@ void launch_pcg_step(hipStream_t s, const Topo& tp, const PcgVecs& V) {
 }
 template <class T>
 void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const PcgVecs& V) {
-    if (tp.nchunks) hipLaunchKernelGGL(k_pcg_points<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, lin, pe, V);
+    if (V.npchunks == 0) return;
+    const unsigned waves = PT_NT / 64, need = grid_for(V.npchunks, waves);
+    if ((size_t)WREC * tp.nc <= (size_t)PT_TABLE) {  // one resident workgroup per compute unit
+        const unsigned g = need < V.ncu ? need : V.ncu;
+        hipLaunchKernelGGL((k_pcg_points<T, true>), dim3(g), dim3(PT_NT), 0, s, tp, lin, pe, V);
+    } else {
+        const unsigned g = need < 8 * V.ncu ? need : 8 * V.ncu;
+        hipLaunchKernelGGL((k_pcg_points<T, false>), dim3(g), dim3(PT_NT), 0, s, tp, lin, pe, V);
+    }
 }
 template <class T>
 void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const PcgVecs& V) {
@@ -2056,6 +2130,7 @@ void launch_linearize_cm(hipStream_t s, const Topo& tp, const double* R, const d
                          const double* X, const double* uv, const GaugeDev* gauge, Lin<T> out) {
     if (tp.no == 0) return;
     (void)uv;  // the camera-major pixels are in tp.cm_uv
+    launch_group_turns(s, tp, gp);
     hipLaunchKernelGGL(k_linearize_cm<T>, dim3(grid_for(tp.no, NT)), dim3(NT), 0, s, tp, R, t, gp, X, gauge, out);
 }
 

@@ -61,6 +61,7 @@ struct Topo {
     const double* win_lambda;
     int loss;
     double loss_scale;
+    double* gturn;  // ng scratch: first radial turn per group of the intrinsics being projected with
 };
 
 // Rows of the structure-of-arrays linearisation copies (each row holds one value per observation):
@@ -190,6 +191,8 @@ static_assert(64 * 16 / sizeof(float) <= SOA_SLACK, "camera windows may end past
 static_assert(SOA_TILE % (16 / sizeof(float)) == 0, "16-byte groups stay inside one tile");
 constexpr uint32_t PCG_CAM_REC = 24;  // doubles per camera record: R, f, k1, k2, motion (21 used)
 constexpr uint32_t PCG_XW = 4;        // doubles per point record: the point pass output w (3 used)
+constexpr uint32_t PCG_WREC = 9;      // doubles per camera world record
+constexpr uint32_t PCG_WAVE_OBS = 64; // observations per point-pass chunk (whole points)
 constexpr uint32_t PCG_BLOCK_WG = 64;  // workgroup of the block-parallel PCG kernels
 constexpr uint32_t PCG_Q_WG = 1024;    // workgroup of the block product pass (a wavefront per block)
 struct PcgState {
@@ -215,11 +218,16 @@ struct PcgVecs {
     double* xw;    // PCG_XW per point: the point pass output w
     double* seg;   // 9 per camera segment: the camera pass sums (E, F, I)
     double* vcam;  // PCG_CAM_REC per camera: rotation, intrinsics and the product input
+    double* wcam;  // PCG_WREC per camera: the product input in the world frame (point pass)
     double* part;  // per-workgroup partial dot products: v.q (nqparts), r.r and r.z (nparts each)
     uint32_t nqparts, nparts;
     uint32_t nseg;
     const uint32_t* seg_cam;   // camera of each segment
     const uint32_t* cseg_off;  // nc + 1 segment offsets
+    uint32_t npchunks;           // point-pass chunks: whole points in at most PCG_WAVE_OBS lanes
+    const uint32_t* pchunk_pt;   // npchunks + 1 point offsets
+    const uint32_t* pchunk_obs;  // npchunks + 1 observation offsets
+    uint32_t ncu;                // compute units (persistent point-pass grid)
     PcgState* st;   // read by the product passes; written by the direction update
     PcgState* st1;  // written by the step update; read by the direction update
 };
