@@ -60,13 +60,13 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     """Algorithmic HBM bytes of one launch of `kernel` in FP64: each input read once, each output
     written once; vectors indexed by camera or point are counted once per launch.
 
-    The product passes rebuild the Jacobians from each observation's compact linearisation state
+    The camera pass rebuilds the Jacobians from each observation's compact linearisation state
     (n = (x/z, y/z), sqrt(w)/z, sqrt(w): 32 B) and the camera records (rotation, intrinsics and the
-    product input, 192 B per camera).
-    pcg_points (k_pcg_points: w_p = V^-1 sum_o J~_X,o^T J~_c,o v, one workgroup per point chunk):
-      per observation the compact state (32 B) and its camera and point indices (8 B); per point its
-      offset (4 B), fixed flag (1 B), packed V^-1 (48 B), X (24 B) and the written w_p record
-      (32 B); per camera its record (192 B).
+    product input, 192 B per camera); the point pass uses J~_X with n and sqrt(w).
+    pcg_points (k_pcg_points: w_p = V^-1 sum_o J~_X,o^T J~_c,o v, persistent wavefronts over
+      64-observation point chunks): per observation J~_X (48 B), n and sqrt(w) (24 B) and its camera
+      and point indices (8 B); per point its offset (4 B), fixed flag (1 B), packed V^-1 (48 B), X
+      (24 B) and the written w_p record (32 B); per camera its world-frame record (72 B), read once.
     pcg_cameras (k_pcg_cameras: per camera segment of 128 positions the sums of
       J~_c^T (J~_c v - J~_X w_p)): per observation the compact state and rotated point of the
       camera-major copy (56 B) and its point index (4 B); per point the w_p record once (32 B); per
@@ -78,7 +78,7 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     """
     no, npt, nc = counts["obs"], counts["points"], counts["cameras"]
     if kernel == "pcg_points":
-        return no * (32 + 8) + npt * (4 + 1 + 48 + 24 + 32) + nc * 192
+        return no * (48 + 24 + 8) + npt * (4 + 1 + 48 + 24 + 32) + nc * 72
     if kernel == "pcg_cameras":
         return no * (56 + 4) + npt * 32 + counts["segments"] * (4 + 72) + nc * (12 + 192)
     if kernel == "linearize":

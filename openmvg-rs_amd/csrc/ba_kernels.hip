@@ -170,6 +170,7 @@ __device__ void camera_record(const Topo& tp, uint32_t c, const double* R, const
 //   J~_c v = J~_X (Om x X + T) + sqrt(w) n (A + r2 B + r2^2 C).
 // Built from a camera record (camera frame) and the scene scale.
 constexpr int WREC = (int)PCG_WREC;
+constexpr int WREC_PAD = 10;  // a world record padded to 16-byte groups (step records)
 __device__ __forceinline__ void world_from_camera(const Cam& c, double s, double w[WREC]) {
     double om[3], tr[3];
     rotate_t(c.R, c.om, om);
@@ -1205,20 +1206,25 @@ __global__ void __launch_bounds__(NT) k_candidate(Topo tp, const double* R, cons
     const uint32_t ob = tp.chunk_obs[k], oe = tp.chunk_obs[k + 1];
     const uint32_t pb = tp.chunk_pt[k], pend = tp.chunk_pt[k + 1];
     const bool big = oe - ob > NT;
-    // 1. per-observation back-substitution terms e_o = J~_X,o^T J~_c,o step (Jacobians rebuilt from
-    //    the compact state, with the cameras' step records)
+    // 1. per-observation back-substitution terms e_o = J~_X,o^T J~_c,o step, with the cameras'
+    //    world-frame step records
     double e[3] = {0.0, 0.0, 0.0};
     for (uint32_t o = ob + lane; o < oe; o += NT) {
         const uint32_t q = tp.obs_pt[o], c = tp.obs_cam[o];
-        Cam cam;
-        load_cam(srec + CAM_REC * (size_t)c, cam);
-        const Obs ob_ = make_obs((double)lin.jpm[soa_at(SOA_Q, o)], (double)lin.jpm[soa_at(SOA_Q + 1, o)],
-                                 (double)lin.jpm[soa_at(SOA_Q + 2, o)], (double)lin.jpm[soa_at(SOA_Q + 3, o)], cam);
+        double jx[6], w[WREC], eo[3];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) jx[i] = (double)lin.jpm[soa_at(SOA_JX + i, o)];
+        const double2* wr = reinterpret_cast<const double2*>(srec + WREC_PAD * (size_t)c);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double2 x = wr[i];
+            w[2 * i] = x.x;
+            w[2 * i + 1] = x.y;
+        }
+        w[8] = wr[4].x;
         const double Xo[3] = {X[3 * (size_t)q], X[3 * (size_t)q + 1], X[3 * (size_t)q + 2]};
-        double RX[3], y0, y1, eo[3];
-        rotate(cam.R, Xo, RX);
-        camera_product(ob_, cam, RX, &y0, &y1);
-        point_transpose(ob_, cam, gauge[tp.cam_win ? tp.cam_win[c] : 0].scene_scale, y0, y1, eo);
+        world_product(jx, (double)lin.jpm[soa_at(SOA_Q, o)], (double)lin.jpm[soa_at(SOA_Q + 1, o)],
+                      (double)lin.jpm[soa_at(SOA_Q + 3, o)], Xo, w, eo);
 #pragma unroll
         for (int i = 0; i < 3; ++i) e[i] += eo[i];
     }
@@ -1580,13 +1586,38 @@ __global__ void __launch_bounds__(NTS) k_pcg_init(Topo tp, PcgVecs V, double tol
 }
 
 // Point pass: wavefronts of a persistent grid take point chunks (whole points whose observations
-// fit the 64 lanes, or one point with more) in turn, one lane per observation.  The cameras'
-// world records sit in LDS when they fit (CAM_LDS), else they are read from global memory.
+// fit the 64 lanes, or one point with more) in turn, one lane per observation; each wavefront
+// loads its next chunk's observations while it works on the current one.  The cameras' world
+// records sit in LDS when they fit (CAM_LDS), else they are read from global memory.
 // e_o = J~_X,o^T (J~_c,o v) is summed per point by a segmented scan over the lanes (a fixed
 // shuffle order), and the lane of each point's last observation forms w_p = V^-1 sum e_o and
 // writes the point's record (read by the camera pass).
 constexpr int PT_NT = 1024;
 constexpr int PT_TABLE = 2048 * WREC;  // doubles of the LDS camera table
+namespace {
+struct PtLoad {  // one lane's observation of a chunk: indices and compact rows
+    uint32_t ob, oe, c, q;
+    bool live;
+    double jx[6], n0, n1, sw;
+};
+template <class T>
+__device__ __forceinline__ void pt_load(const Topo& tp, const Lin<T>& lin, const PcgVecs& V, uint32_t k, uint32_t lane,
+                                        PtLoad& L) {
+    L.ob = V.pchunk_obs[k];
+    L.oe = V.pchunk_obs[k + 1];
+    const uint32_t o = L.ob + lane;
+    L.live = o < L.oe;
+    const uint32_t oo = L.live ? o : L.ob;  // in-range stand-in, its terms dropped
+    L.c = tp.obs_cam[oo];
+    L.q = tp.obs_pt[oo];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) L.jx[i] = (double)lin.jpm[soa_at(SOA_JX + i, oo)];
+    L.n0 = (double)lin.jpm[soa_at(SOA_Q, oo)];
+    L.n1 = (double)lin.jpm[soa_at(SOA_Q + 1, oo)];
+    L.sw = (double)lin.jpm[soa_at(SOA_Q + 3, oo)];
+}
+}  // namespace
+
 template <class T, bool CAM_LDS>
 __global__ void __launch_bounds__(PT_NT) k_pcg_points(Topo tp, Lin<T> lin, PointElim<T> pe, PcgVecs V) {
     __shared__ double tab[CAM_LDS ? PT_TABLE : 1];
@@ -1597,43 +1628,37 @@ __global__ void __launch_bounds__(PT_NT) k_pcg_points(Topo tp, Lin<T> lin, Point
     }
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nw = gridDim.x * (PT_NT / 64);
-    for (uint32_t k = blockIdx.x * (PT_NT / 64) + (threadIdx.x >> 6); k < V.npchunks; k += nw) {
-        const uint32_t ob = V.pchunk_obs[k], oe = V.pchunk_obs[k + 1];
+    auto record = [&](uint32_t c, double w[WREC]) {
+        const double* src = CAM_LDS ? tab + WREC * c : V.wcam + WREC * (size_t)c;
+#pragma unroll
+        for (int i = 0; i < WREC; ++i) w[i] = src[i];
+    };
+    uint32_t k = blockIdx.x * (PT_NT / 64) + (threadIdx.x >> 6);
+    PtLoad cur;
+    double X[3];
+    uint32_t h0 = 0, h1 = 0;
+    if (k < V.npchunks) {
+        pt_load(tp, lin, V, k, lane, cur);
+        for (int i = 0; i < 3; ++i) X[i] = V.X[3 * (size_t)cur.q + i];
+        h0 = tp.pt_off[cur.q];
+        h1 = tp.pt_off[cur.q + 1];
+    }
+    for (; k < V.npchunks; k += nw) {
+        const bool more = k + nw < V.npchunks;
+        PtLoad nxt;
+        if (more) pt_load(tp, lin, V, k + nw, lane, nxt);  // in flight while this chunk is worked on
+        const uint32_t ob = cur.ob, oe = cur.oe;
         const bool big = oe - ob > 64;
         double acc[3] = {0.0, 0.0, 0.0};
-        uint32_t q = 0, pend = 0, head = lane;
-        for (uint32_t o = ob + lane; o < (big ? oe : ob + 64); o += 64) {
-            const bool live = o < oe;
-            const uint32_t oo = live ? o : ob;  // in-range stand-in, its terms dropped below
-            const uint32_t c = tp.obs_cam[oo];
-            q = tp.obs_pt[oo];
-            double jx[6];
+        uint32_t q = cur.q;
+        if (!big) {
+            double w[WREC], e[3];
+            record(cur.c, w);
+            world_product(cur.jx, cur.n0, cur.n1, cur.sw, X, w, e);
+            if (cur.live)
 #pragma unroll
-            for (int i = 0; i < 6; ++i) jx[i] = (double)lin.jpm[soa_at(SOA_JX + i, oo)];
-            const double n0 = (double)lin.jpm[soa_at(SOA_Q, oo)], n1 = (double)lin.jpm[soa_at(SOA_Q + 1, oo)];
-            const double sw = (double)lin.jpm[soa_at(SOA_Q + 3, oo)];
-            const double X[3] = {V.X[3 * (size_t)q], V.X[3 * (size_t)q + 1], V.X[3 * (size_t)q + 2]};
-            if (!big) {
-                head = live ? tp.pt_off[q] - ob : lane;
-                pend = tp.pt_off[q + 1];
-            }
-            double w[WREC];
-            const double* src = CAM_LDS ? tab + WREC * c : V.wcam + WREC * (size_t)c;
-#pragma unroll
-            for (int i = 0; i < WREC; ++i) w[i] = src[i];
-            double e[3];
-            world_product(jx, n0, n1, sw, X, w, e);
-            if (live)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) acc[i] += e[i];
-        }
-        bool last;
-        if (big) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) acc[i] = wave_sum(acc[i]);
-            q = tp.obs_pt[ob];
-            last = lane == 0;
-        } else {
+                for (int i = 0; i < 3; ++i) acc[i] = e[i];
+            const uint32_t head = cur.live ? h0 - ob : lane;
             // inclusive segmented scan: lane l ends with the sum over lanes head..l of its point
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
@@ -1644,8 +1669,23 @@ __global__ void __launch_bounds__(PT_NT) k_pcg_points(Topo tp, Lin<T> lin, Point
 #pragma unroll
                     for (int i = 0; i < 3; ++i) acc[i] += sv[i];
             }
-            last = ob + lane < oe && ob + lane + 1 == pend;
+        } else {  // one point: lanes accumulate over passes, then the wavefront butterfly
+            for (uint32_t o = ob + lane; o < oe; o += 64) {
+                double jx[6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) jx[i] = (double)lin.jpm[soa_at(SOA_JX + i, o)];
+                double w[WREC], e[3];
+                record(tp.obs_cam[o], w);
+                world_product(jx, (double)lin.jpm[soa_at(SOA_Q, o)], (double)lin.jpm[soa_at(SOA_Q + 1, o)],
+                              (double)lin.jpm[soa_at(SOA_Q + 3, o)], X, w, e);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) acc[i] += e[i];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) acc[i] = wave_sum(acc[i]);
+            q = tp.obs_pt[ob];
         }
+        const bool last = big ? lane == 0 : (cur.live && ob + lane + 1 == h1);
         if (last) {
             double out[3] = {0.0, 0.0, 0.0};
             if (!tp.pt_fixed[q]) {
@@ -1656,6 +1696,12 @@ __global__ void __launch_bounds__(PT_NT) k_pcg_points(Topo tp, Lin<T> lin, Point
             double2* rec = reinterpret_cast<double2*>(V.xw + PCG_XW * (size_t)q);
             rec[0] = make_double2(out[0], out[1]);
             rec[1] = make_double2(out[2], 0.0);
+        }
+        if (more) {
+            cur = nxt;
+            for (int i = 0; i < 3; ++i) X[i] = V.X[3 * (size_t)cur.q + i];
+            h0 = tp.pt_off[cur.q];
+            h1 = tp.pt_off[cur.q + 1];
         }
     }
 }
@@ -2057,16 +2103,22 @@ void launch_retract(hipStream_t s, const Topo& tp, const double* R, const double
     if (n == 0) return;
     hipLaunchKernelGGL(k_retract<double>, dim3(grid_for(n, NT)), dim3(NT), 0, s, tp, R, t, gp, step, gauge, Rc, tc, gpc);
 }
-// Per-camera records of the camera step (camera-frame motion, intrinsic values, with the current
-// rotation and intrinsics) for the candidate back-substitution.
+// Per-camera world-frame records of the camera step (at the current rotations and intrinsics) for
+// the candidate back-substitution.
 __global__ void __launch_bounds__(NT) k_step_records(Topo tp, const double* R, const double* gp, const double* step,
                                                      const GaugeDev* gauges, double* srec) {
     const uint32_t c = blockIdx.x * NT + threadIdx.x;
     if (c >= tp.nc) return;
-    double vc[9];
+    double vc[9], rec[CAM_REC], w[WREC];
     camera_values(tp, c, step, vc);
-    camera_record(tp, c, R + 9 * (size_t)c, gp + 3 * (size_t)tp.cam_group[c], gauges[tp.cam_win ? tp.cam_win[c] : 0],
-                  vc, vc + 6, srec + CAM_REC * (size_t)c);
+    const GaugeDev& gg = gauges[tp.cam_win ? tp.cam_win[c] : 0];
+    camera_record(tp, c, R + 9 * (size_t)c, gp + 3 * (size_t)tp.cam_group[c], gg, vc, vc + 6, rec);
+    Cam cr;
+    load_cam(rec, cr);
+    world_from_camera(cr, gg.scene_scale, w);
+    double2* o = reinterpret_cast<double2*>(srec + WREC_PAD * (size_t)c);
+    for (int i = 0; i < 4; ++i) o[i] = make_double2(w[2 * i], w[2 * i + 1]);
+    o[4] = make_double2(w[8], 0.0);
 }
 template <class T>
 void launch_candidate(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
