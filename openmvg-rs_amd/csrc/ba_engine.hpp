@@ -197,6 +197,7 @@ class Engine {
         HIP_OK(hipMemcpyAsync(X_[0].p, L_.X.data(), 3 * np * 8, hipMemcpyHostToDevice, s_));
         fast_cam_ = iterative_ && simple_topology();
         pose_rows_ = keep_pose_rows_ || !fast_cam_;  // the block-gradient pass reads them
+        if (fast_cam_) prec_.alloc(PE_REC * np);          // read by the camera assembly
         srec_.alloc(PCG_CAM_REC * std::max<size_t>(1, nc));
         build_matvec_order(iterative_);
         jpm_.alloc(SOA_ROWS * npad_);
@@ -448,7 +449,7 @@ class Engine {
     Lin<T> lin() { return Lin<T>{nullptr, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p, pose_rows_ ? 1 : 0}; }
     // Keep the pose columns of J~_c in the point-major copy (for the per-observation hook).
     void keep_pose_rows() { keep_pose_rows_ = true; }
-    PointElim<T> pe() { return PointElim<T>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, pscal_.p}; }
+    PointElim<T> pe() { return PointElim<T>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, prec_.p, pscal_.p}; }
 
     // Use a caller-given gauge state instead of the one derived from the translations.
     void set_gauge(const GaugeDev& g) {
@@ -676,7 +677,7 @@ class Engine {
     Buf<double> gturn_;        // first radial turn per group
     Buf<double> cm_uv_;        // pixels in camera-major order
     Buf<double> casm_;         // camera assembly: per-segment sums
-    Buf<T> jc_, vinv_, vdamp_, y_, ddiag_, rrows_, jpm_, jcm_;
+    Buf<T> jc_, vinv_, vdamp_, y_, ddiag_, rrows_, prec_, jpm_, jcm_;
     Buf<float> S32_;  // FP32 factor of the mixed direct form
     Buf<int> done_;
     Buf<AsmScalars> asm_;

@@ -525,6 +525,7 @@ __global__ void __launch_bounds__(NT) k_point_elim(Topo tp, Lin<T> lin, T lambda
     // per-point elimination; shinv[.][t] receives V^-1 of point pb + t (zero when not eliminated)
     bool eliminated = false;
     T inv[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    T ysave[3] = {T(0), T(0), T(0)};
     if (have_point) {
         double* ps = pe.pscal + 6 * (size_t)p;
         T* yp = pe.y + 3 * (size_t)p;
@@ -605,14 +606,20 @@ __global__ void __launch_bounds__(NT) k_point_elim(Topo tp, Lin<T> lin, T lambda
                     ps[5] = weak ? 7.0 : 3.0;
                     for (int q = 0; q < 3; ++q) {
                         yp[q] = y[q];
+                        ysave[q] = y[q];
                         pe.ddiag[3 * (size_t)p + q] = dd[q];
                     }
-                    for (int q = 0; q < 6; ++q) {
-                        pe.vinv[6 * (size_t)p + q] = inv[q];
-                        pe.vdamp[6 * (size_t)p + q] = Vd[q];
-                    }
+                    for (int q = 0; q < 6; ++q) pe.vdamp[6 * (size_t)p + q] = Vd[q];
                 }
             }
+        }
+        // V^-1 (zero for a point that is not eliminated: it then adds nothing to the reduced
+        // system) and the packed record [V^-1, y] read by the camera-side passes
+        for (int q = 0; q < 6; ++q) pe.vinv[6 * (size_t)p + q] = inv[q];
+        if (pe.prec) {
+            T* rec = pe.prec + PE_REC * (size_t)p;
+            for (int q = 0; q < 6; ++q) rec[q] = inv[q];
+            for (int q = 0; q < 3; ++q) rec[6 + q] = ysave[q];
         }
     }
     if (!pe.rrows) return;  // block-uniform
@@ -851,7 +858,11 @@ __global__ void __launch_bounds__(128) k_camera_assembly(Topo tp, Lin<T> lin, Po
                                (double)lin.jcm[soa_at(SOA_Q + 2, j)], (double)lin.jcm[soa_at(SOA_Q + 3, j)], cam);
         const double RX[3] = {(double)lin.jcm[soa_at(SOA_RX, j)], (double)lin.jcm[soa_at(SOA_RX + 1, j)],
                               (double)lin.jcm[soa_at(SOA_RX + 2, j)]};
-        const bool free_point = !tp.pt_fixed[p];
+        // the point's packed [V^-1, y]: zero for points that are fixed or not eliminated
+        const T* prec = pe.prec + PE_REC * (size_t)p;
+        double pv[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) pv[q] = (double)prec[q];
         double jc[2][9], a[6];
         rebuild_jacobians(o, cam, RX, pd, baseline, gg, jc, a);
         if (!live) {
@@ -860,9 +871,8 @@ __global__ void __launch_bounds__(128) k_camera_assembly(Topo tp, Lin<T> lin, Po
         }
         if (role == 0) {
             double u0 = 0.0, u1 = 0.0;
-            if (free_point && live) {
-                const T* yp = pe.y + 3 * (size_t)p;
-                const double y[3] = {(double)yp[0], (double)yp[1], (double)yp[2]};
+            if (live) {
+                const double* y = pv + 6;
                 u0 = a[0] * y[0] + a[1] * y[1] + a[2] * y[2];
                 u1 = a[3] * y[0] + a[4] * y[1] + a[5] * y[2];
             }
@@ -875,9 +885,8 @@ __global__ void __launch_bounds__(128) k_camera_assembly(Topo tp, Lin<T> lin, Po
         } else {
             // N = I - M, M = J~_X V^-1 J~_X^T
             double n00 = 1.0, n01 = 0.0, n11 = 1.0;
-            if (free_point && live) {
-                const T* vp = pe.vinv + 6 * (size_t)p;
-                const double vi[6] = {(double)vp[0], (double)vp[1], (double)vp[2], (double)vp[3], (double)vp[4], (double)vp[5]};
+            if (live) {
+                const double* vi = pv;
                 double z0[3], z1[3];
                 sym3_mul(vi, a, z0);
                 sym3_mul(vi, a + 3, z1);
@@ -1635,13 +1644,20 @@ __global__ void __launch_bounds__(PT_NT) k_pcg_points(Topo tp, Lin<T> lin, Point
     };
     uint32_t k = blockIdx.x * (PT_NT / 64) + (threadIdx.x >> 6);
     PtLoad cur;
-    double X[3];
+    double X[3], vm[6];
     uint32_t h0 = 0, h1 = 0;
+    bool fx = true;
+    auto point_data = [&](uint32_t q) {
+        for (int i = 0; i < 3; ++i) X[i] = V.X[3 * (size_t)q + i];
+        h0 = tp.pt_off[q];
+        h1 = tp.pt_off[q + 1];
+        fx = tp.pt_fixed[q] != 0;
+        const T* vi = pe.vinv + 6 * (size_t)q;
+        for (int i = 0; i < 6; ++i) vm[i] = (double)vi[i];
+    };
     if (k < V.npchunks) {
         pt_load(tp, lin, V, k, lane, cur);
-        for (int i = 0; i < 3; ++i) X[i] = V.X[3 * (size_t)cur.q + i];
-        h0 = tp.pt_off[cur.q];
-        h1 = tp.pt_off[cur.q + 1];
+        point_data(cur.q);
     }
     for (; k < V.npchunks; k += nw) {
         const bool more = k + nw < V.npchunks;
@@ -1688,20 +1704,15 @@ __global__ void __launch_bounds__(PT_NT) k_pcg_points(Topo tp, Lin<T> lin, Point
         const bool last = big ? lane == 0 : (cur.live && ob + lane + 1 == h1);
         if (last) {
             double out[3] = {0.0, 0.0, 0.0};
-            if (!tp.pt_fixed[q]) {
-                const T* vi = pe.vinv + 6 * (size_t)q;
-                const double m[6] = {(double)vi[0], (double)vi[1], (double)vi[2], (double)vi[3], (double)vi[4], (double)vi[5]};
-                sym3_mul(m, acc, out);
-            }
+            if (big) point_data(q);
+            if (!fx) sym3_mul(vm, acc, out);
             double2* rec = reinterpret_cast<double2*>(V.xw + PCG_XW * (size_t)q);
             rec[0] = make_double2(out[0], out[1]);
             rec[1] = make_double2(out[2], 0.0);
         }
         if (more) {
             cur = nxt;
-            for (int i = 0; i < 3; ++i) X[i] = V.X[3 * (size_t)cur.q + i];
-            h0 = tp.pt_off[cur.q];
-            h1 = tp.pt_off[cur.q + 1];
+            point_data(cur.q);
         }
     }
 }

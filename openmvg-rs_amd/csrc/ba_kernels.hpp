@@ -93,7 +93,9 @@ struct Lin {
     int pose_rows;  // k_linearize writes the pose columns of J~_c into jpm (else only jcm has them)
 };
 
-// Per-point elimination (kernel (b)) outputs.
+// Per-point elimination (kernel (b)) outputs.  V^-1 and y are zero for points that are fixed or
+// not eliminated.  PE_REC values per point make the packed record one 128-byte line in FP64.
+constexpr int PE_REC = 16;
 template <class T>
 struct PointElim {
     T* vinv;     // 6 per point: packed inverse of the damped block V^
@@ -101,6 +103,7 @@ struct PointElim {
     T* y;        // 3 per point: V^-1 g_p
     T* ddiag;    // 3 per point: damping diagonal max(V_ii, floor)
     T* rrows;    // 27 per observation: R = V^-1 W rows (9 rows x 3); or null (block-pair path only)
+    T* prec;     // PE_REC per point: [V^-1 (6), y (3)] packed for camera-side gathers; or null
     double* pscal;  // 6 per point: cost, squared residual, point model reduction, scaled gradient
                     // norm, rcond, flags (bit0 linearised, bit1 eliminated, bit2 weak)
 };

@@ -45,7 +45,8 @@ def test_linearize_matches_golden(torch_cuda, adjuster, ring, loss, scale, key):
     _, nat, arrays = ring
     g = load("ring_linearize")
     n, no = g["n_observations"], nat.sizes["n_observations"]
-    dv = device_view(nat, to_device(arrays), arrays, loss, scale)
+    dev = to_device(arrays)  # the view holds raw pointers: keep the tensors alive
+    dv = device_view(nat, dev, arrays, loss, scale)
     out = [torch.zeros(m, dtype=torch.float64, device="cuda") for m in (2 * no, 6 * no, 18 * no, no)]
     valid = torch.zeros(no, dtype=torch.uint8, device="cuda")
     code = adjuster.handle.lib.ba_hip_linearize(adjuster.handle.ptr, C.byref(dv), *(t.data_ptr() for t in out),
@@ -62,7 +63,8 @@ def test_schur_matches_golden(torch_cuda, adjuster, ring):
     _, nat, arrays = ring
     g = load("ring_schur")
     D = g["dimension"]
-    dv = device_view(nat, to_device(arrays), arrays, A.LOSS_SQUARED, 0.0)
+    dev = to_device(arrays)
+    dv = device_view(nat, dev, arrays, A.LOSS_SQUARED, 0.0)
     S = torch.zeros(D * D, dtype=torch.float64, device="cuda")
     gr, fg, dg = (torch.zeros(D, dtype=torch.float64, device="cuda") for _ in range(3))
     s1 = A.SchurScalars()
