@@ -1,15 +1,16 @@
 // ba_cholesky.hip — kernel (d): dense blocked Cholesky of the reduced camera system on gfx950.
 //
-// Right-looking blocked LL^T on a row-major lower triangle with 64x64 tiles, per diagonal tile k:
-//   k_potrf_tile   one wave factors the diagonal tile in LDS, checking every pivot against
-//                  64 eps max|diag(A)| as the reference's dense camera solve does
-//                  (kernel/multi_view/dense/linear_solver.rs:9-61), and forms L_kk^-1;
-//   k_trsm_gemm    the panel below, X = A_ik L_kk^-T, as a 64x64x64 product per tile on the
-//                  matrix cores;
-//   k_syrk_update  the trailing update A_ij -= L_ik L_jk^T on the matrix cores;
+// Right-looking blocked LL^T on a row-major lower triangle with 64x64 tiles, two launches per
+// diagonal tile k:
+//   k_panel        every workgroup's first wave factors the diagonal tile in registers, checking
+//                  every pivot against 64 eps max|diag(A)| as the reference's dense camera solve
+//                  does (kernel/multi_view/dense/linear_solver.rs:9-61), and forms L_kk^-1;
+//                  workgroup b > 0 then forms its panel tile L_ik = A_ik L_kk^-T on the matrix
+//                  cores, and advances the forward solve L y = b by one tile;
+//   k_syrk_update  the trailing update A_ij -= L_ik L_jk^T on the matrix cores
 // (v_mfma_f64_16x16x4f64, or v_mfma_f32_16x16x4f32 for the FP32 factor; four waves per 64x64
-// tile).  The solves L y = b and L^T x = y run tile by tile in one workgroup: each step is a
-// product with the already solved part (coalesced row reads) and one with L_kk^-1.
+// tile).  Only the off-diagonal tiles of L stay in A; the solves use the stored L_kk^-1.  Further
+// solves (iterative refinement) and L^T x = y run one launch per tile step across workgroups.
 // The mixed form factors a Jacobi-scaled FP32 copy and recovers the FP64 solution by iterative
 // refinement against the FP64 matrix (residuals r = b - S x in FP64).
 #include <hip/hip_runtime.h>
@@ -66,79 +67,183 @@ __global__ void __launch_bounds__(1024) k_diag_tol(const T* A, uint32_t n, uint3
     }
 }
 
-// Factor the diagonal tile at k0 (size nb <= 64) in LDS with four waves: thread (i, q) owns the
-// entries of row i in columns c = q (mod 4).  Right-looking, one column per step; every pivot is
-// checked against 64 eps max|diag(A)|.  L_kk is written back, then inverted in place column by
-// column from the last (X L = I: X_ij = -(sum_{j<k<=i} X_ik L_kj) / L_jj, the four partial sums
-// of a row combined by lane shuffles in a fixed order) and L_kk^-1 written (identity beyond nb).
-// Once a pivot fails, *fail is set and later tiles return at once.
-template <class T>
-__global__ void __launch_bounds__(256) k_potrf_tile(T* A, uint32_t ld, uint32_t k0, uint32_t nb, const double* work,
-                                                   int* fail, T* Linv) {
-    __shared__ T a[TB][TB + 1];
-    if (*fail) return;
-    const int t = threadIdx.x, i = t / 4, q = t % 4;
-    for (int c = q; c < TB; c += 4)
-        a[i][c] = (i < (int)nb && c <= i) ? A[(size_t)(k0 + i) * ld + k0 + c] : (i == c ? T(1) : T(0));
-    __syncthreads();
-    const T tol = (T)work[0];
-    const bool row_in = i < (int)nb;
-    for (int j = 0; j < (int)nb; ++j) {
-        const T piv = a[j][j];
-        if (!(isfinite(piv) && piv > tol)) {
-            if (t == 0) *fail = 1;
-            return;  // uniform: every thread reads the same pivot
-        }
-        const T l = sqrt(piv);
-        __syncthreads();
-        if (q == 0 && row_in && i > j) a[i][j] /= l;
-        if (t == 0) a[j][j] = l;
-        __syncthreads();
-        if (row_in && i > j) {
-            const T lij = a[i][j];
-            // columns c in (j, i] with c = q (mod 4)
-            int c = j + 1 + ((q - (j + 1)) % 4 + 4) % 4;
-            for (; c <= i; c += 4) a[i][c] -= lij * a[c][j];
-        }
-        __syncthreads();
-    }
-    if (row_in)
-        for (int c = q; c <= i; c += 4) A[(size_t)(k0 + i) * ld + k0 + c] = a[i][c];
-    for (int j = (int)nb - 1; j >= 0; --j) {
-        const T xjj = T(1) / a[j][j];
-        T s = T(0);
-        if (row_in && i > j) {
-            int k = j + 1 + ((q - (j + 1)) % 4 + 4) % 4;
-            for (; k <= i; k += 4) s += a[i][k] * a[k][j];
-        }
-        // lanes 4i .. 4i+3 are consecutive in one wave: fixed-order sum of the four partials
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        __syncthreads();
-        if (q == 0 && row_in && i > j) a[i][j] = -s * xjj;
-        if (t == 0) a[j][j] = xjj;
-        __syncthreads();
-    }
-    for (int c = q; c < TB; c += 4)
-        Linv[i * TB + c] = (row_in && c < (int)nb) ? (c <= i ? a[i][c] : T(0)) : (i == c ? T(1) : T(0));
+// Orders the LDS accesses of one wavefront around this point (lanes exchange data through LDS
+// without a workgroup barrier; a wavefront's LDS instructions complete in issue order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Panel tile ti (row tile below k0): X = A_ik L_kk^-T on the matrix cores (K = 64), in place.
+// Copies the 64x64 tile at src (row-major, leading dimension ld) into LDS; entries in rows >= nr
+// or columns >= nc (or above the diagonal, with `lower`) read as zero.  Every thread issues all
+// of its loads before the first LDS store, so the tile arrives in one memory round trip.
 template <class T>
-__global__ void __launch_bounds__(256) k_trsm_gemm(T* A, uint32_t n, uint32_t ld, uint32_t k0, const T* Linv,
-                                                   const int* fail) {
+__device__ __forceinline__ void load_tile(T (*dst)[TB + 1], const T* src, size_t ld, uint32_t nr, uint32_t nc,
+                                          bool lower = false) {
+    constexpr int PER = TT / 256;
+    T v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = (int)threadIdx.x + 256 * q, r = e / TB, c = e % TB;
+        v[q] = ((uint32_t)r < nr && (uint32_t)c < nc && (!lower || c <= r)) ? src[(size_t)r * ld + c] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = (int)threadIdx.x + 256 * q;
+        dst[e / TB][e % TB] = v[q];
+    }
+}
+
+// One wavefront factors the diagonal tile (size nb <= 64; identity beyond nb) held in `t` (lower
+// triangle) and forms X = L_kk^-1 into `xs` (row-major).  Lane i keeps row i in registers.
+// Column j: the pivot a_jj is checked against 64 eps max|diag(A)| as the reference's dense camera
+// solve does (kernel/multi_view/dense/linear_solver.rs:9-61), L_ij = a_ij / sqrt(a_jj), and the
+// rows below update a_ic -= (a_ij / a_jj) a_cj with column j broadcast through LDS (`col`).
+// Entries above the diagonal of a row are never read.  Then L goes to `t` and lane j solves
+// column j of L X = I by forward substitution, x_k = (delta_kj - sum_{m<k} L_km x_m) / L_kk, with
+// row k of L read from `t`.  Returns false, on every lane, when a pivot fails.
+template <class T>
+__device__ bool factor_diag_wave(T (*t)[TB + 1], T* col, T (*xs)[TB + 1], uint32_t nb, T tol) {
+    const int i = threadIdx.x & 63;
+    T r[TB];
+#pragma unroll
+    for (int c = 0; c < TB; ++c) r[c] = (i < (int)nb) ? (c <= i ? t[i][c] : T(0)) : (i == c ? T(1) : T(0));
+#pragma unroll
+    for (int j = 0; j < TB; ++j) {
+        wave_sync();
+        col[i] = r[j];
+        wave_sync();
+        const T piv = col[j];
+        if (!(isfinite(piv) && piv > tol)) return false;  // uniform: every lane read the same pivot
+        const T l = sqrt(piv);
+        const T sj = r[j] / piv;
+        r[j] = i > j ? r[j] / l : (i == j ? l : r[j]);
+#pragma unroll
+        for (int c = j + 1; c < TB; ++c) r[c] -= sj * col[c];
+    }
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < TB; ++c) t[i][c] = c <= i ? r[c] : T(0);
+    wave_sync();
+    const int j = i;  // lane j: column j of X, in r
+#pragma unroll
+    for (int k = 0; k < TB; ++k) {
+        T s = k == j ? T(1) : T(0);
+#pragma unroll
+        for (int m = 0; m < k; ++m) s -= t[k][m] * r[m];
+        r[k] = s / t[k][k];
+    }
+#pragma unroll
+    for (int k = 0; k < TB; ++k) xs[k][j] = r[k];
+    return true;
+}
+
+// Tile step kt of the factor, one launch (grid 1 + the row tiles below kt, 256 threads):
+// every workgroup's first wave factors the diagonal tile A_kk (read-only here) and forms
+// X = L_kk^-1 (the same arithmetic in every workgroup); workgroup 0 stores X, workgroup b > 0 the
+// panel tile L_ik = A_ik X^T (i = kt + b) in place, as a 64x64x64 matrix-core product.  With `rhs`
+// set the forward solve L y = rhs advances with the factor: every workgroup forms
+// y_k = X rhs_k, workgroup 0 stores it, workgroup b updates rhs_i -= L_ik y_k.  A failing pivot
+// sets *fail, and later steps return at once.
+template <class T>
+__global__ void __launch_bounds__(256) k_panel(T* A, uint32_t n, uint32_t ld, uint32_t kt, const double* work,
+                                               int* fail, T* Linv, double* rhs, double* y) {
     typedef typename Mfma<T>::v4 v4;
-    __shared__ T X[TB][TB + 1];
-    __shared__ T Li[TB][TB + 1];
+    __shared__ T t[TB][TB + 1];
+    __shared__ T xs[TB][TB + 1];
+    __shared__ T col[TB];
+    __shared__ double yk[TB];
+    __shared__ int ok_flag;
     if (*fail) return;
-    const uint32_t r0 = k0 + TB * (1 + blockIdx.x);
-    const int t = threadIdx.x;
-    for (int e = t; e < TT; e += 256) {
-        const int r = e / TB, c = e % TB;
-        X[r][c] = (r0 + r < n && k0 + c < n) ? A[(size_t)(r0 + r) * ld + k0 + c] : T(0);
-        Li[r][c] = Linv[e];
+    const uint32_t k0 = kt * TB;
+    const uint32_t nb = n - k0 < TB ? n - k0 : TB;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    load_tile<T>(t, A + (size_t)k0 * ld + k0, ld, nb, nb, true);
+    __syncthreads();
+    if (wave == 0) {
+        const bool ok = factor_diag_wave<T>(t, col, xs, nb, (T)work[0]);
+        if (lane == 0) ok_flag = ok ? 1 : 0;
     }
     __syncthreads();
+    if (!ok_flag) {
+        if (tid == 0) *fail = 1;
+        return;
+    }
+    if (rhs && wave == 0) {  // y_k = X rhs_k, rows in a fixed order
+        yk[lane] = lane < (int)nb ? rhs[k0 + lane] : 0.0;
+        wave_sync();
+        double v = 0.0;
+        for (int c = 0; c <= lane; ++c) v += (double)xs[lane][c] * yk[c];
+        wave_sync();
+        yk[lane] = v;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        for (int e = tid; e < TT; e += 256) Linv[(size_t)kt * TT + e] = xs[e / TB][e % TB];
+        if (rhs && tid < (int)nb) y[k0 + tid] = yk[tid];
+        return;
+    }
+    const uint32_t r0 = k0 + TB * blockIdx.x;
+    load_tile<T>(t, A + (size_t)r0 * ld + k0, ld, n - r0, nb);
+    __syncthreads();
+    const int qr = (wave / 2) * 32, qc = (wave % 2) * 32;
+    const int li = lane % 16, lk = lane / 16;
+    v4 acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int k = 0; k < TB; k += 4) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const T av = t[qr + 16 * a + li][k + lk];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const T bv = xs[qc + 16 * b + li][k + lk];  // B[k][j] = X[j][k]
+                acc[a][b] = Mfma<T>::mac(av, bv, acc[a][b]);
+            }
+        }
+    }
+    __syncthreads();  // every wave has read t: overwrite it with L_ik
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int v = 0; v < 4; ++v) {
+                const int r = qr + 16 * a + Mfma<T>::row(lk, v), c = qc + 16 * b + li;
+                t[r][c] = acc[a][b][v];
+                if (r0 + r < n && c < (int)nb) A[(size_t)(r0 + r) * ld + k0 + c] = acc[a][b][v];
+            }
+    if (!rhs) return;
+    __syncthreads();
+    // rhs_i -= L_ik y_k: thread (row, quarter) sums 16 columns, quarters combined in lane order
+    const int row = tid / 4, q = tid % 4;
+    double s = 0.0;
+    for (int c = 16 * q; c < 16 * q + 16; ++c) s += (double)t[row][c] * yk[c];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q == 0 && r0 + row < n) rhs[r0 + row] -= s;
+}
+
+// Trailing update of tile pair (ti, tj), tj <= ti, both after the panel at k0:
+// A_ij -= L_ik L_jk^T with K = nb, both panel tiles staged whole in LDS.  Four waves, each a
+// 32x32 quadrant of 2x2 16x16x4 matrix-core tiles.
+template <class T>
+__global__ void __launch_bounds__(256) k_syrk_update(T* A, uint32_t n, uint32_t ld, uint32_t k0, uint32_t nb,
+                                                     uint32_t first_tile, const int* fail) {
+    typedef typename Mfma<T>::v4 v4;
+    __shared__ T Li[TB][TB + 1];
+    __shared__ T Lj[TB][TB + 1];
+    if (*fail) return;
+    // blockIdx.x enumerates the lower-triangular tile pairs (ti, tj) of the trailing matrix
+    const uint32_t idx = blockIdx.x;
+    uint32_t ti = (uint32_t)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+    while ((ti + 1) * (ti + 2) / 2 <= idx) ++ti;
+    while (ti * (ti + 1) / 2 > idx) --ti;
+    const uint32_t tj = idx - ti * (ti + 1) / 2;
+    const uint32_t r0 = (first_tile + ti) * TB, c0 = (first_tile + tj) * TB;
+    load_tile<T>(Li, A + (size_t)r0 * ld + k0, ld, n - r0, nb);
+    load_tile<T>(Lj, A + (size_t)c0 * ld + k0, ld, n - c0, nb);
+    __syncthreads();
+    const int t = threadIdx.x;
     const int wave = t / 64, lane = t % 64;
     const int qr = (wave / 2) * 32, qc = (wave % 2) * 32;
     const int li = lane % 16, lk = lane / 16;
@@ -149,172 +254,114 @@ __global__ void __launch_bounds__(256) k_trsm_gemm(T* A, uint32_t n, uint32_t ld
     for (int k = 0; k < TB; k += 4) {
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            const T av = X[qr + 16 * a + li][k + lk];
+            T av = Li[qr + 16 * a + li][k + lk];
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                const T bv = Li[qc + 16 * b + li][k + lk];  // B[k][j] = Linv[j][k]
+                T bv = Lj[qc + 16 * b + li][k + lk];  // B[k][j] = L_jk[j][k]
                 acc[a][b] = Mfma<T>::mac(av, bv, acc[a][b]);
             }
         }
     }
+    T old[2][2][4];
+#pragma unroll
     for (int a = 0; a < 2; ++a)
+#pragma unroll
         for (int b = 0; b < 2; ++b)
+#pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const uint32_t r = r0 + qr + 16 * a + Mfma<T>::row(lk, v);
-                const uint32_t c = k0 + qc + 16 * b + li;
-                if (r < n && c < n) A[(size_t)r * ld + c] = acc[a][b][v];
+                const uint32_t r = r0 + qr + 16 * a + Mfma<T>::row(lk, v), c = c0 + qc + 16 * b + li;
+                old[a][b][v] = (r < n && c <= r) ? A[(size_t)r * ld + c] : T(0);
             }
-}
-
-// Trailing update of tile pair (ti, tj), tj <= ti, both after the panel at k0:
-// A_ij -= L_ik L_jk^T with K = nb, staged through LDS in K chunks of 32.  Four waves, each a
-// 32x32 quadrant of 2x2 16x16x4 matrix-core tiles.
-constexpr int KC = 32;
-template <class T>
-__global__ void __launch_bounds__(256) k_syrk_update(T* A, uint32_t n, uint32_t ld, uint32_t k0, uint32_t nb,
-                                                     uint32_t first_tile, const int* fail) {
-    typedef typename Mfma<T>::v4 v4;
-    __shared__ T Li[TB][KC + 1];
-    __shared__ T Lj[TB][KC + 1];
-    if (*fail) return;
-    // blockIdx.x enumerates the lower-triangular tile pairs (ti, tj) of the trailing matrix
-    const uint32_t idx = blockIdx.x;
-    uint32_t ti = (uint32_t)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
-    while ((ti + 1) * (ti + 2) / 2 <= idx) ++ti;
-    while (ti * (ti + 1) / 2 > idx) --ti;
-    const uint32_t tj = idx - ti * (ti + 1) / 2;
-    const uint32_t r0 = (first_tile + ti) * TB, c0 = (first_tile + tj) * TB;
-    const int t = threadIdx.x;
-    const int wave = t / 64, lane = t % 64;
-    const int qr = (wave / 2) * 32, qc = (wave % 2) * 32;
-    const int li = lane % 16, lk = lane / 16;
-    v4 acc[2][2];
+#pragma unroll
     for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) acc[a][b] = v4{T(0), T(0), T(0), T(0)};
-    for (int kc = 0; kc < (int)nb; kc += KC) {
-        __syncthreads();
-        for (int e = t; e < TB * KC; e += 256) {
-            int r = e / KC, c = e % KC;
-            uint32_t ri = r0 + r, rj = c0 + r;
-            bool kin = kc + c < (int)nb;
-            Li[r][c] = (ri < n && kin) ? A[(size_t)ri * ld + k0 + kc + c] : T(0);
-            Lj[r][c] = (rj < n && kin) ? A[(size_t)rj * ld + k0 + kc + c] : T(0);
-        }
-        __syncthreads();
 #pragma unroll
-        for (int k = 0; k < KC; k += 4) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                T av = Li[qr + 16 * a + li][k + lk];
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    T bv = Lj[qc + 16 * b + li][k + lk];  // B[k][j] = L_jk[j][k]
-                    acc[a][b] = Mfma<T>::mac(av, bv, acc[a][b]);
-                }
-            }
-        }
-    }
-    for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
+#pragma unroll
             for (int v = 0; v < 4; ++v) {
-                uint32_t r = r0 + qr + 16 * a + Mfma<T>::row(lk, v);
-                uint32_t c = c0 + qc + 16 * b + li;
-                if (r < n && c < n && c <= r) A[(size_t)r * ld + c] -= acc[a][b][v];
+                const uint32_t r = r0 + qr + 16 * a + Mfma<T>::row(lk, v), c = c0 + qc + 16 * b + li;
+                if (r < n && c <= r) A[(size_t)r * ld + c] = old[a][b][v] - acc[a][b][v];
             }
 }
 
-// Forward solve L y = b (one workgroup; FP64 vectors), left-looking by tiles:
-//   y_k = L_kk^-1 (b_k - L_k,<k y_<k).
-// Thread (r, g) of 64 x 16 sums row r of tile k against the solved entries j = g, g + 16, ...
-// (16 consecutive lanes read one row); the 16 partials are summed in g order.  `skip` (optional)
-// turns the solve into a no-op.
+// Forward solve step kt of L y = r (FP64 vectors; grid 1 + the row tiles below kt): every
+// workgroup forms y_k = X_k r_k (X_k = L_kk^-1, rows in a fixed order), workgroup 0 stores it and
+// workgroup b updates r_i -= L_ik y_k (i = kt + b).  `skip` (optional) makes the step a no-op.
 template <class T>
-__global__ void __launch_bounds__(1024) k_trsv_lower(const T* L, const T* Linv, uint32_t n, uint32_t ld, const double* b,
-                                                     double* y, const int* skip) {
-    __shared__ double part[16][TB];
-    __shared__ double rhs[TB];
+__global__ void __launch_bounds__(256) k_fwd_step(const T* L, const T* Linv, uint32_t n, uint32_t ld, uint32_t kt,
+                                                  double* r, double* y, const int* skip) {
+    __shared__ T xs[TB][TB + 1];
+    __shared__ T lt[TB][TB + 1];
+    __shared__ double rk[TB];
+    __shared__ double yk[TB];
     if (skip && *skip) return;
-    const int t = threadIdx.x;
-    const int sr = t / 16, sg = t % 16;  // summation: 16 consecutive lanes share a row
-    const uint32_t ntile = (n + TB - 1) / TB;
-    for (uint32_t kt = 0; kt < ntile; ++kt) {
-        const uint32_t k0 = kt * TB;
-        double s = 0.0;
-        if (k0 + sr < n) {  // four independent chains, combined in a fixed order
-            const T* Lr = L + (size_t)(k0 + sr) * ld;
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-            uint32_t j = sg;
-            for (; j + 48 < k0; j += 64) {
-                s0 += (double)Lr[j] * y[j];
-                s1 += (double)Lr[j + 16] * y[j + 16];
-                s2 += (double)Lr[j + 32] * y[j + 32];
-                s3 += (double)Lr[j + 48] * y[j + 48];
-            }
-            for (; j < k0; j += 16) s0 += (double)Lr[j] * y[j];
-            s = (s0 + s1) + (s2 + s3);
-        }
-        part[sg][sr] = s;
-        __syncthreads();
-        const uint32_t row = k0 + t;
-        if (t < TB) {
-            double v = row < n ? b[row] : 0.0;
-            for (int q = 0; q < 16; ++q) v -= part[q][t];
-            rhs[t] = v;
-        }
-        __syncthreads();
-        if (t < TB && row < n) {
-            const T* Li = Linv + (size_t)kt * TT + t * TB;
-            double v = 0.0;
-            for (int c = 0; c <= t; ++c) v += (double)Li[c] * rhs[c];
-            y[row] = v;
-        }
-        __syncthreads();
+    const uint32_t k0 = kt * TB;
+    const uint32_t nb = n - k0 < TB ? n - k0 : TB;
+    const uint32_t r0 = k0 + TB * blockIdx.x;
+    const int tid = threadIdx.x;
+    load_tile<T>(xs, Linv + (size_t)kt * TT, TB, TB, TB);
+    if (blockIdx.x) load_tile<T>(lt, L + (size_t)r0 * ld + k0, ld, n - r0, nb);
+    if (tid < TB) rk[tid] = tid < (int)nb ? r[k0 + tid] : 0.0;
+    __syncthreads();
+    if (tid < TB) {
+        double v = 0.0;
+        for (int c = 0; c <= tid; ++c) v += (double)xs[tid][c] * rk[c];
+        yk[tid] = v;
     }
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        if (tid < (int)nb) y[k0 + tid] = yk[tid];
+        return;
+    }
+    const int row = tid / 4, q = tid % 4;
+    double s = 0.0;
+    for (int c = 16 * q; c < 16 * q + 16; ++c) s += (double)lt[row][c] * yk[c];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q == 0 && r0 + row < n) r[r0 + row] -= s;
 }
 
-// Backward solve L^T x = y (one workgroup; FP64 vectors), by tiles from the last:
-//   x_k = L_kk^-T (y_k - L_>k,k^T x_>k).
-// Thread (g, c) of 16 x 64 sums column c of tile k over the solved rows i = k0 + 64 + g + 16 m.
+// Backward solve step kt of L^T x = r (grid kt + 1, from the last tile down): every workgroup
+// forms x_k = X_k^T r_k, workgroup 0 stores it and workgroup b updates r_j -= L_kj^T x_k
+// (j = b - 1 < kt).
 template <class T>
-__global__ void __launch_bounds__(1024) k_trsv_upper(const T* L, const T* Linv, uint32_t n, uint32_t ld, const double* y,
-                                                     double* x, const int* skip) {
-    __shared__ double part[16][TB];
-    __shared__ double rhs[TB];
+__global__ void __launch_bounds__(256) k_bwd_step(const T* L, const T* Linv, uint32_t n, uint32_t ld, uint32_t kt,
+                                                  double* r, double* x, const int* skip) {
+    __shared__ T xs[TB][TB + 1];
+    __shared__ T lt[TB][TB + 1];
+    __shared__ double rk[TB];
+    __shared__ double xk[TB];
+    __shared__ double part[4][TB];
     if (skip && *skip) return;
-    const int t = threadIdx.x, c = t % TB, g = t / TB;
-    const uint32_t ntile = (n + TB - 1) / TB;
-    for (int kt = (int)ntile - 1; kt >= 0; --kt) {
-        const uint32_t k0 = (uint32_t)kt * TB;
-        const uint32_t col = k0 + c;
-        double s = 0.0;
-        if (col < n) {  // four independent chains, combined in a fixed order
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-            uint32_t i = k0 + TB + g;
-            for (; i + 48 < n; i += 64) {
-                s0 += (double)L[(size_t)i * ld + col] * x[i];
-                s1 += (double)L[(size_t)(i + 16) * ld + col] * x[i + 16];
-                s2 += (double)L[(size_t)(i + 32) * ld + col] * x[i + 32];
-                s3 += (double)L[(size_t)(i + 48) * ld + col] * x[i + 48];
-            }
-            for (; i < n; i += 16) s0 += (double)L[(size_t)i * ld + col] * x[i];
-            s = (s0 + s1) + (s2 + s3);
-        }
-        part[g][c] = s;
-        __syncthreads();
-        if (t < TB) {
-            double v = col < n ? y[col] : 0.0;
-            for (int q = 0; q < 16; ++q) v -= part[q][c];
-            rhs[c] = v;
-        }
-        __syncthreads();
-        if (t < TB && col < n) {  // x_c = sum_r Linv[r][c] rhs[r], r >= c
-            const T* Li = Linv + (size_t)kt * TT;
-            double v = 0.0;
-            for (int rr = c; rr < TB; ++rr) v += (double)Li[rr * TB + c] * rhs[rr];
-            x[col] = v;
-        }
-        __syncthreads();
+    const uint32_t k0 = kt * TB;
+    const uint32_t nb = n - k0 < TB ? n - k0 : TB;
+    const int tid = threadIdx.x;
+    const uint32_t j0 = blockIdx.x ? (blockIdx.x - 1) * TB : 0;
+    load_tile<T>(xs, Linv + (size_t)kt * TT, TB, TB, TB);
+    if (blockIdx.x) load_tile<T>(lt, L + (size_t)k0 * ld + j0, ld, nb, TB);
+    if (tid < TB) rk[tid] = tid < (int)nb ? r[k0 + tid] : 0.0;
+    __syncthreads();
+    if (tid < TB) {  // x_c = sum_{m >= c} X[m][c] r_m
+        double v = 0.0;
+        for (int m = tid; m < TB; ++m) v += (double)xs[m][tid] * rk[m];
+        xk[tid] = v;
     }
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        if (tid < (int)nb) x[k0 + tid] = xk[tid];
+        return;
+    }
+    const int c = tid % TB, g = tid / TB;  // column c of tile (kt, j), rows m = g (mod 4)
+    double s = 0.0;
+    for (int m = g; m < TB; m += 4) s += (double)lt[m][c] * xk[m];
+    part[g][c] = s;
+    __syncthreads();
+    if (tid < TB) r[j0 + tid] -= ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]));
+}
+
+__global__ void __launch_bounds__(256) k_copy_vec(uint32_t n, const double* a, double* b, const int* skip) {
+    if (skip && *skip) return;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = a[i];
 }
 
 // Iterative refinement: r = b - S x from the FP64 matrix (one wave per row, fixed lane order),
@@ -327,8 +374,16 @@ __global__ void __launch_bounds__(256) k_refine_residual(const double* S, uint32
     const int lane = threadIdx.x % 64;
     if (row >= n) return;
     const double* Sr = S + (size_t)row * ld;
-    double s = 0.0;
-    for (uint32_t c = lane; c < n; c += 64) s += Sr[c] * x[c];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // four chains: four loads in flight
+    uint32_t c = lane;
+    for (; c + 192 < n; c += 256) {
+        s0 += Sr[c] * x[c];
+        s1 += Sr[c + 64] * x[c + 64];
+        s2 += Sr[c + 128] * x[c + 128];
+        s3 += Sr[c + 192] * x[c + 192];
+    }
+    for (; c < n; c += 64) s0 += Sr[c] * x[c];
+    double s = (s0 + s1) + (s2 + s3);
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane == 0) r[row] = b[row] - s;
 }
@@ -387,33 +442,51 @@ __global__ void __launch_bounds__(256) k_scale_vec(uint32_t n, const double* sca
     if (i < n) y[i] = scale[i] * v[i];
 }
 
-// Workspace (doubles): [0, 16) scalars | tmp n | r n | dx n | scale n | sv n | L_kk^-1 tiles.
-inline size_t linv_offset(uint32_t n) { return 16 + 5 * (size_t)n; }
+// Workspace (doubles): [0, 16) scalars | y n | r n | dx n | scale n | sv n | w n | L_kk^-1 tiles.
+inline size_t linv_offset(uint32_t n) { return 16 + 6 * (size_t)n; }
+inline double* work_y(double* work) { return work + 16; }
+inline double* work_w(double* work, uint32_t n) { return work + 16 + 5 * (size_t)n; }
 
+// Factor A (lower triangle, in place) tile step by tile step; with `rhs` the forward solve
+// L y = rhs advances with it (rhs is consumed).
 template <class T>
-void factor(hipStream_t s, T* A, uint32_t n, uint32_t ld, double* work, int* fail) {
+void factor(hipStream_t s, T* A, uint32_t n, uint32_t ld, double* work, int* fail, double* rhs, double* y) {
     hipLaunchKernelGGL(k_diag_tol<T>, dim3(1), dim3(1024), 0, s, A, n, ld, work, fail);
     T* Linv = reinterpret_cast<T*>(work + linv_offset(n));
     const uint32_t ntile = (n + TB - 1) / TB;
     for (uint32_t kt = 0; kt < ntile; ++kt) {
         const uint32_t k0 = kt * TB;
         const uint32_t nb = n - k0 < TB ? n - k0 : TB;
-        hipLaunchKernelGGL(k_potrf_tile<T>, dim3(1), dim3(256), 0, s, A, ld, k0, nb, (const double*)work, fail,
-                           Linv + (size_t)kt * TT);
         const uint32_t rest = ntile - kt - 1;
+        hipLaunchKernelGGL(k_panel<T>, dim3(1 + rest), dim3(256), 0, s, A, n, ld, kt, (const double*)work, fail, Linv,
+                           rhs, y);
         if (rest == 0) break;
-        hipLaunchKernelGGL(k_trsm_gemm<T>, dim3(rest), dim3(256), 0, s, A, n, ld, k0, (const T*)(Linv + (size_t)kt * TT),
-                           (const int*)fail);
         const uint32_t pairs = rest * (rest + 1) / 2;
         hipLaunchKernelGGL(k_syrk_update<T>, dim3(pairs), dim3(256), 0, s, A, n, ld, k0, nb, kt + 1, (const int*)fail);
     }
 }
+// x = L^-T y (y is left unchanged).
+template <class T>
+void backward(hipStream_t s, const T* L, uint32_t n, uint32_t ld, const double* y, double* x, double* work,
+              const int* skip) {
+    const T* Linv = reinterpret_cast<const T*>(work + linv_offset(n));
+    double* w = work_w(work, n);
+    const uint32_t ntile = (n + TB - 1) / TB;
+    hipLaunchKernelGGL(k_copy_vec, dim3((n + 255) / 256), dim3(256), 0, s, n, y, w, skip);
+    for (int kt = (int)ntile - 1; kt >= 0; --kt)
+        hipLaunchKernelGGL(k_bwd_step<T>, dim3(kt + 1), dim3(256), 0, s, L, Linv, n, ld, (uint32_t)kt, w, x, skip);
+}
+// x = L^-T L^-1 b with a factored L.
 template <class T>
 void apply(hipStream_t s, const T* L, uint32_t n, uint32_t ld, const double* b, double* x, double* work, const int* skip) {
     const T* Linv = reinterpret_cast<const T*>(work + linv_offset(n));
-    double* tmp = work + 16;
-    hipLaunchKernelGGL(k_trsv_lower<T>, dim3(1), dim3(1024), 0, s, L, Linv, n, ld, b, tmp, skip);
-    hipLaunchKernelGGL(k_trsv_upper<T>, dim3(1), dim3(1024), 0, s, L, Linv, n, ld, (const double*)tmp, x, skip);
+    double* y = work_y(work);
+    double* w = work_w(work, n);
+    const uint32_t ntile = (n + TB - 1) / TB;
+    hipLaunchKernelGGL(k_copy_vec, dim3((n + 255) / 256), dim3(256), 0, s, n, b, w, skip);
+    for (uint32_t kt = 0; kt < ntile; ++kt)
+        hipLaunchKernelGGL(k_fwd_step<T>, dim3(ntile - kt), dim3(256), 0, s, L, Linv, n, ld, kt, w, y, skip);
+    backward<T>(s, L, n, ld, y, x, work, skip);
 }
 
 }  // namespace
@@ -426,8 +499,10 @@ size_t cholesky_work_doubles(uint32_t n) {
 void launch_cholesky_solve(hipStream_t s, double* A, uint32_t n, uint32_t ld, const double* b, double* x, double* work,
                            int* fail) {
     if (n == 0) return;
-    factor<double>(s, A, n, ld, work, fail);
-    apply<double>(s, A, n, ld, b, x, work, (const int*)fail);
+    double* w = work_w(work, n);
+    hipLaunchKernelGGL(k_copy_vec, dim3((n + 255) / 256), dim3(256), 0, s, n, b, w, (const int*)nullptr);
+    factor<double>(s, A, n, ld, work, fail, w, work_y(work));
+    backward<double>(s, A, n, ld, work_y(work), x, work, (const int*)fail);
 }
 
 void launch_cholesky_solve_mixed(hipStream_t s, const double* S, float* F, uint32_t n, uint32_t ld, const double* b,
@@ -441,11 +516,12 @@ void launch_cholesky_solve_mixed(hipStream_t s, const double* S, float* F, uint3
     hipLaunchKernelGGL(k_jacobi_scale, dim3(nb), dim3(256), 0, s, S, n, ld, scale);
     hipLaunchKernelGGL(k_lower_to_f32, dim3(nb, n < 65535u ? n : 65535u), dim3(256), 0, s, S, n, ld,
                        (const double*)scale, F);
-    factor<float>(s, F, n, ld, work, fail);
     HIP_OK_OR_IGNORE(hipMemsetAsync(done, 0, sizeof(int), s));
-    // x = D^-1/2 F^-1 D^-1/2 b
-    hipLaunchKernelGGL(k_scale_vec, dim3(nb), dim3(256), 0, s, n, (const double*)scale, b, sv, (const int*)nullptr);
-    apply<float>(s, F, n, ld, sv, dx, work, (const int*)fail);
+    // x = D^-1/2 F^-1 D^-1/2 b, the forward solve advancing with the factor
+    hipLaunchKernelGGL(k_scale_vec, dim3(nb), dim3(256), 0, s, n, (const double*)scale, b, work_w(work, n),
+                       (const int*)nullptr);
+    factor<float>(s, F, n, ld, work, fail, work_w(work, n), work_y(work));
+    backward<float>(s, F, n, ld, work_y(work), dx, work, (const int*)fail);
     hipLaunchKernelGGL(k_scale_vec, dim3(nb), dim3(256), 0, s, n, (const double*)scale, (const double*)dx, x,
                        (const int*)nullptr);
     const unsigned rows = (n + 3) / 4;

@@ -51,7 +51,7 @@ def test_name_matches_contract(adjuster):
     assert adjuster.name() == "openmvg-ba-hip"
 
 
-@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 257])
+@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 257, 1000])
 def test_cholesky_solve_matches_lapack_and_oracle(torch_cuda, adjuster, oracle, n):
     torch = torch_cuda
     rng = np.random.default_rng(n)
