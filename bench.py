@@ -56,6 +56,28 @@ def lowered_counts(problem) -> dict:
     return {"obs": n_obs, "points": n_pts, "cameras": n_cams, "segments": segments}
 
 
+def schur_slots(problem) -> int:
+    """Neighbour slots of the explicit reduced system: per camera, itself and every camera it
+    shares a point with (every point of the synthetic scenes is free)."""
+    order = np.lexsort((problem.obs_pose, problem.obs_point))
+    cam = problem.obs_pose[order].astype(np.int64)
+    pt = problem.obs_point[order]
+    starts = np.flatnonzero(np.r_[True, pt[1:] != pt[:-1]])
+    lens = np.diff(np.r_[starts, len(pt)])
+    nc = int(problem.n_poses)
+    keys = []
+    for k in range(2, int(lens.max()) + 1 if len(lens) else 2):
+        s = starts[lens == k]
+        if len(s) == 0:
+            continue
+        c = cam[s[:, None] + np.arange(k)[None, :]]
+        a, b = np.triu_indices(k, 1)
+        lo, hi = np.minimum(c[:, a], c[:, b]), np.maximum(c[:, a], c[:, b])
+        keys.append((lo * nc + hi).ravel())
+    pairs = len(np.unique(np.concatenate(keys))) if keys else 0
+    return nc + 2 * pairs
+
+
 def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     """Algorithmic HBM bytes of one launch of `kernel` in FP64: each input read once, each output
     written once; vectors indexed by camera or point are counted once per launch.
@@ -75,8 +97,14 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     linearize (k_linearize): per observation pixel (16 B), camera and point index (8 B), the
       written r~, compact state and J~_X (96 B), cost, squared residual and flag (17 B); per point
       X (24 B); per camera R, t, f, k1, k2 (120 B).
+    pcg_product (k_pcg_product: q = S v on the explicit reduced system, one wavefront per camera
+      row): per neighbour slot the 9 x 9 block (648 B) and, once per row, the neighbour index
+      (9 x 4 B); per camera the 9 input values (72 B), its offsets and dimensions (16 B) and the
+      written q and read damping diagonal of its 9 rows (144 B).
     """
     no, npt, nc = counts["obs"], counts["points"], counts["cameras"]
+    if kernel == "pcg_product":
+        return counts["slots"] * (648 + 36) + nc * (72 + 16 + 144)
     if kernel == "pcg_points":
         return no * (48 + 24 + 8) + npt * (4 + 1 + 48 + 24 + 32) + nc * 72
     if kernel == "pcg_cameras":
@@ -259,7 +287,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10, help="timed LM iterations")
     ap.add_argument("--warmup", type=int, default=2, help="untimed LM iterations (a separate solve)")
     ap.add_argument("--points", type=int, default=1_000_000)
-    ap.add_argument("--roofline-kernel", default="pcg_points", choices=["pcg_points", "pcg_cameras", "linearize"])
+    ap.add_argument("--roofline-kernel", default="auto",
+                    choices=["auto", "pcg_product", "pcg_points", "pcg_cameras", "linearize"],
+                    help="auto: the PCG product kernel of the form the solve uses")
     ap.add_argument("--converge-iterations", type=int, default=60,
                     help="iteration cap of the run that fixes the converged cost (0 skips time-to-cost)")
     ap.add_argument("--cpu-points", type=int, default=200_000, help="points of the CPU-baseline subsample")
@@ -322,14 +352,21 @@ def main() -> int:
         f"call {call_s:.1f} s, PCG iterations {rep.pcg_iterations}, cost {rep.initial_cost:.6e} -> "
         f"{rep.final_cost:.6e}, status {rep.status}")
 
+    form = {A.SCHUR_EXPLICIT: "explicit", A.SCHUR_IMPLICIT: "implicit"}.get(
+        ba.handle.lib.ba_hip_last_schur_product(ba.handle.ptr), "unknown")
+    if args.roofline_kernel == "auto":
+        args.roofline_kernel = "pcg_product" if form == "explicit" else "pcg_points"
     # roofline of the dominant kernel: HIP events around each of its launches
     kernel_ids = {"pcg_points": A.PROFILE_PCG_POINTS, "pcg_cameras": A.PROFILE_PCG_CAMERAS,
-                  "linearize": A.PROFILE_LINEARIZE}
+                  "linearize": A.PROFILE_LINEARIZE, "pcg_product": A.PROFILE_PCG_PRODUCT}
     prof = ba.solve(prob, base.with_(max_iterations=max(2, min(args.steps, 4)),
                                      profile_kernel=kernel_ids[args.roofline_kernel]))
     launches = int(prof.report.profile_launches)
     avg_s = float(prof.report.profile_seconds) / max(launches, 1)
-    per_launch = algorithmic_bytes(args.roofline_kernel, lowered_counts(prob), D)
+    counts = lowered_counts(prob)
+    if args.roofline_kernel == "pcg_product":
+        counts["slots"] = schur_slots(prob)
+    per_launch = algorithmic_bytes(args.roofline_kernel, counts, D)
     achieved = per_launch / avg_s / 1e9
     log(f"roofline {args.roofline_kernel}: {launches} launches, avg {avg_s * 1e6:.1f} us, "
         f"{per_launch / 1e6:.1f} MB/launch, {achieved:.1f} GB/s")
@@ -387,9 +424,10 @@ def main() -> int:
         "config": {
             "workload": (f"large BA: {prob.n_poses} cameras (f,k1,k2 free), {prob.n_points} points, "
                          f"{prob.n_observations} observations, geometric tracks, 2% outliers, Huber 1px, "
-                         "LM + block-Jacobi PCG on implicit Schur (rtol 0.1), seed 0xBA0003"),
+                         f"LM + block-Jacobi PCG on the {form} Schur system (rtol 0.1), seed 0xBA0003"),
             "reduced_dimension": D,
             "precision": "fp64",
+            "schur_product": form,
             "pcg_iterations": int(rep.pcg_iterations),
         },
         "roofline": {

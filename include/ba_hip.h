@@ -131,7 +131,9 @@ enum ba_profile_kernel {
     BA_PROFILE_PCG_CAMERAS = 4,    /* kernel (c), camera pass of the implicit Schur product */
     BA_PROFILE_PAIRS = 5,          /* kernel (b), block-pair segmented reduction */
     BA_PROFILE_CHOLESKY = 6,       /* kernel (d), factor and solve */
-    BA_PROFILE_CANDIDATE = 7       /* candidate pass */
+    BA_PROFILE_CANDIDATE = 7,      /* candidate pass */
+    BA_PROFILE_SCHUR_PAIRS = 8,    /* kernel (b), off-diagonal blocks of the explicit S */
+    BA_PROFILE_PCG_PRODUCT = 9     /* kernel (c), explicit block-sparse product with S */
 };
 
 /* SolveStatus (solver.rs:172-185) with ConvergenceReason folded in. */
@@ -216,6 +218,18 @@ const char* ba_hip_name(void);
  * the adjuster object of the reference (NativeBundleAdjuster, native/mod.rs:117-118). */
 int ba_hip_create(int device, ba_hip_handle** out);
 void ba_hip_destroy(ba_hip_handle* handle);
+
+/* Form of the PCG product with S = U - W V^-1 W^T (no counterpart in the reference, whose
+ * retained operator is matrix-free: self_calibration_retained/operator.rs:26-76).  IMPLICIT
+ * multiplies matrix-free from the per-observation Jacobians (two passes over the observations
+ * per product); EXPLICIT assembles S once per LM iteration in 9 x 9 camera blocks and multiplies
+ * block-sparse.  AUTO (the default) takes EXPLICIT when every intrinsic group has one camera, no
+ * camera observes a point twice and the explicit buffers fit in 40 % of the free device memory;
+ * EXPLICIT on another topology falls back to IMPLICIT.  The forms agree to rounding. */
+enum ba_schur_product { BA_SCHUR_AUTO = 0, BA_SCHUR_IMPLICIT = 1, BA_SCHUR_EXPLICIT = 2 };
+int ba_hip_set_schur_product(ba_hip_handle* handle, int32_t mode);
+/* The form of the handle's last iterative solve (IMPLICIT or EXPLICIT; AUTO before any). */
+int32_t ba_hip_last_schur_product(const ba_hip_handle* handle);
 
 /* BundleAdjuster::solve (solver.rs:1572) for the self-calibrating pixel path
  * (native/mod.rs:260-367, native/self_calibration.rs:223-249).  Host buffers.

@@ -253,7 +253,7 @@ class Engine {
             jcm_.alloc(SOA_ROWS * npad_);
             HIP_OK(hipMemsetAsync(jcm_.p, 0, jcm_.n * sizeof(T), s_));  // finite padding
             pseg_.alloc(9 * std::max<size_t>(1, nseg_));
-            casm_.alloc(54 * std::max<size_t>(1, nseg_));
+            casm_.alloc((size_t)CAM_ASM_VALUES * std::max<size_t>(1, nseg_));
             pvcam_.alloc(PCG_CAM_REC * nc);
             pwcam_.alloc(PCG_WREC * std::max<size_t>(1, nc));
             {
@@ -263,7 +263,10 @@ class Engine {
                 ncu_ = (uint32_t)std::max(cu, 1);
             }
             HIP_OK(hipMemsetAsync(pvcam_.p, 0, pvcam_.n * sizeof(double), s_));
-            ppart_.alloc(std::max<size_t>(1, (B_.n_blocks + PCG_Q_WG / 64 - 1) / (PCG_Q_WG / 64)) +
+            setup_explicit();
+            const size_t qparts = std::max<size_t>((B_.n_blocks + PCG_Q_WG / 64 - 1) / (PCG_Q_WG / 64),
+                                                   (9 * nc + 3) / 4);  // implicit or explicit product
+            ppart_.alloc(std::max<size_t>(1, qparts) +
                          2 * std::max<size_t>(1, (B_.n_blocks + PCG_BLOCK_WG - 1) / PCG_BLOCK_WG));
             px_.alloc(D);
             pr_.alloc(D);
@@ -310,6 +313,50 @@ class Engine {
         tp_.loss = L_.loss;
         tp_.loss_scale = L_.loss_scale;
         HIP_OK(hipStreamSynchronize(s_));
+    }
+
+    // The explicit reduced camera system: chosen (see ba_schur_product) after the camera-major
+    // order exists; builds the neighbour rows and the camera pairs' observation pairs on the host.
+    void setup_explicit() {
+        explicit_ = false;
+        if (!fast_cam_ || schur_mode_ == BA_SCHUR_IMPLICIT) return;
+        const size_t nc = L_.n_cameras(), no = L_.n_obs();
+        const uint64_t terms = count_schur_terms(L_);
+        const uint64_t pairs = std::min<uint64_t>(terms, (uint64_t)nc * (nc > 0 ? nc - 1 : 0) / 2);
+        const double bytes = (double)sizeof(T) * ((double)ZREC * no + 81.0 * ((double)nc + 2.0 * (double)pairs)) +
+                             8.0 * (double)terms + 16.0 * (double)pairs;
+        if (schur_mode_ != BA_SCHUR_EXPLICIT) {
+            size_t fr = 0, tot = 0;
+            HIP_OK(hipMemGetInfo(&fr, &tot));
+            if (bytes > 0.4 * (double)fr) return;
+        }
+        if (terms >= (uint64_t)UINT32_MAX) return;
+        std::vector<uint32_t> pos(no), cur(nc + 1, 0);
+        for (size_t o = 0; o < no; ++o) cur[L_.obs_cam[o] + 1]++;
+        for (size_t c = 0; c < nc; ++c) cur[c + 1] += cur[c];
+        for (size_t o = 0; o < no; ++o) pos[o] = cur[L_.obs_cam[o]]++;
+        const SchurLists SL = build_schur_lists(L_, pos);
+        snbr_off_.upload(SL.nbr_off, s_);
+        snbr_.upload(SL.nbr, s_);
+        sdiag_slot_.upload(SL.diag_slot, s_);
+        srow_base_.upload(SL.row_base, s_);
+        spair_cam_.upload(SL.pair_cam, s_);
+        spair_slot_.upload(SL.pair_slot, s_);
+        sterm_off_.upload(SL.term_off, s_);
+        sterm_.upload(SL.term, s_);
+        npairs_ = (uint32_t)(SL.pair_cam.size() / 2);
+        zrec_.alloc((size_t)ZREC * std::max<size_t>(1, no));
+        svals_.alloc(SL.n_values);
+        pvexp_.alloc(9 * std::max<size_t>(1, nc));
+        HIP_OK(hipMemsetAsync(pvexp_.p, 0, pvexp_.n * sizeof(double), s_));
+        HIP_OK(hipStreamSynchronize(s_));  // the host lists end with this scope
+        explicit_ = true;
+    }
+    SchurRows<T> schur_rows() {
+        return explicit_ ? SchurRows<T>{svals_.p, srow_base_.p, snbr_off_.p, snbr_.p, sdiag_slot_.p} : SchurRows<T>{};
+    }
+    SchurPairs schur_pairs() {
+        return SchurPairs{npairs_, spair_cam_.p, spair_slot_.p, sterm_off_.p, sterm_.p};
     }
 
     // Point chunks (at most NT observations of whole points, or one larger point; at most NT
@@ -432,7 +479,8 @@ class Engine {
         V.vcam = pvcam_.p;
         V.part = ppart_.p;
         V.nparts = pcg_partials(tp_);
-        V.nqparts = pcg_q_partials(tp_);
+        V.nqparts = explicit_ ? pcg_product_partials(tp_) : pcg_q_partials(tp_);
+        V.vexp = explicit_ ? pvexp_.p : nullptr;
         V.nseg = nseg_;
         V.npchunks = npchunks_;
         V.pchunk_pt = pchunk_pt_.p;
@@ -447,6 +495,8 @@ class Engine {
     }
 
     Lin<T> lin() { return Lin<T>{nullptr, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p, pose_rows_ ? 1 : 0}; }
+    void set_schur_product(int mode) { schur_mode_ = mode; }
+    bool explicit_product() const { return explicit_; }
     // Keep the pose columns of J~_c in the point-major copy (for the per-observation hook).
     void keep_pose_rows() { keep_pose_rows_ = true; }
     PointElim<T> pe() { return PointElim<T>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, prec_.p, pscal_.p}; }
@@ -479,8 +529,11 @@ class Engine {
             timed(BA_PROFILE_PAIRS, [&] {
                 launch_camera_assembly<T>(s_, tp, lin(), pe(), R_[state].p, gp_[state].p, gauge_.p, nseg_,
                                           seg_cam_.p, cseg_off_.p, casm_.p, diag_pair_.p, value_off_.p, grad_.p,
-                                          fgrad_.p, diag_.p, u_vals_.p, s_vals_.p);
+                                          fgrad_.p, diag_.p, u_vals_.p, s_vals_.p, explicit_ ? zrec_.p : nullptr,
+                                          schur_rows());
             });
+            if (explicit_)
+                timed(BA_PROFILE_SCHUR_PAIRS, [&] { launch_schur_pairs<T>(s_, schur_pairs(), zrec_.p, schur_rows()); });
         } else {
             launch_block_gradient<T>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
         }
@@ -552,15 +605,23 @@ class Engine {
         HIP_OK(hipStreamSynchronize(s_));
         if (*h_int_.p) return false;
         const int batch = 4;
+        const SchurRows<T> SR = schur_rows();
         auto enqueue_batch = [&] {
             for (int k = 0; k < batch; ++k) {
-                timed(BA_PROFILE_PCG_POINTS, [&] { launch_pcg_points<T>(s_, tp_, lin(), pe(), V); });
-                timed(BA_PROFILE_PCG_CAMERAS, [&] { launch_pcg_cameras<T>(s_, tp_, lin(), V); });
-                launch_pcg_step<T>(s_, tp_, V);
+                if (explicit_) {
+                    timed(BA_PROFILE_PCG_PRODUCT, [&] { launch_pcg_product<T>(s_, tp_, SR, V); });
+                    launch_pcg_step_explicit<T>(s_, tp_, V);
+                } else {
+                    timed(BA_PROFILE_PCG_POINTS, [&] { launch_pcg_points<T>(s_, tp_, lin(), pe(), V); });
+                    timed(BA_PROFILE_PCG_CAMERAS, [&] { launch_pcg_cameras<T>(s_, tp_, lin(), V); });
+                    launch_pcg_step<T>(s_, tp_, V);
+                }
             }
         };
         const PcgState* fin = nullptr;
-        if (profile_ == BA_PROFILE_PCG_POINTS || profile_ == BA_PROFILE_PCG_CAMERAS) {
+        const bool profiled_product = explicit_ ? profile_ == BA_PROFILE_PCG_PRODUCT
+                                                : (profile_ == BA_PROFILE_PCG_POINTS || profile_ == BA_PROFILE_PCG_CAMERAS);
+        if (profiled_product) {
             // one batch at a time, so that launches after the state turned done (which return at
             // once) can be dropped from the timings by the state's product count
             size_t events_before = used_;
@@ -698,6 +759,13 @@ class Engine {
     double refine_rel_ = NAN;
     uint32_t mixed_fallbacks_ = 0;
     bool fast_cam_ = false;
+    int schur_mode_ = BA_SCHUR_AUTO;
+    bool explicit_ = false;  // PCG product with the assembled S
+    uint32_t npairs_ = 0;
+    Buf<T> zrec_, svals_;
+    Buf<uint64_t> srow_base_;
+    Buf<uint32_t> snbr_off_, snbr_, sdiag_slot_, spair_cam_, spair_slot_, sterm_off_, sterm_;
+    Buf<double> pvexp_;
     bool pose_rows_ = true, keep_pose_rows_ = false;
     int profile_ = BA_PROFILE_NONE;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;

@@ -6,4 +6,6 @@
 struct ba_hip_handle {
     int device = 0;
     hipStream_t stream = nullptr;
+    int schur_product = 0;       // ba_schur_product requested for iterative solves
+    int last_schur_product = 0;  // the form the last iterative solve used
 };

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 namespace ba {
 namespace {
@@ -610,6 +611,130 @@ PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all) {
         out.term_j[at] = terms[k].j;
     }
     return out;
+}
+
+uint64_t count_schur_terms(const Lowered& L) {
+    uint64_t n = 0;
+    for (size_t p = 0; p < L.n_points(); ++p) {
+        if (L.point_fixed[p]) continue;
+        const uint64_t k = L.point_off[p + 1] - L.point_off[p];
+        n += k * (k - 1) / 2;
+    }
+    return n;
+}
+
+SchurLists build_schur_lists(const Lowered& L, const std::vector<uint32_t>& pos) {
+    const size_t nc = L.n_cameras(), np = L.n_points();
+    SchurLists S;
+    // 1. observation pairs bucketed by their lower camera, in point order
+    std::vector<uint64_t> boff(nc + 1, 0);
+    for (size_t p = 0; p < np; ++p) {
+        if (L.point_fixed[p]) continue;
+        for (uint32_t a = L.point_off[p]; a < L.point_off[p + 1]; ++a)
+            for (uint32_t b = a + 1; b < L.point_off[p + 1]; ++b)
+                boff[std::min(L.obs_cam[a], L.obs_cam[b]) + 1]++;
+    }
+    for (size_t c = 0; c < nc; ++c) boff[c + 1] += boff[c];
+    struct Entry {
+        uint32_t j, pc, pj;
+    };
+    std::vector<Entry> ent(boff[nc]);
+    {
+        std::vector<uint64_t> cur(boff.begin(), boff.end() - 1);
+        for (size_t p = 0; p < np; ++p) {
+            if (L.point_fixed[p]) continue;
+            for (uint32_t a = L.point_off[p]; a < L.point_off[p + 1]; ++a)
+                for (uint32_t b = a + 1; b < L.point_off[p + 1]; ++b) {
+                    const uint32_t ca = L.obs_cam[a], cb = L.obs_cam[b];
+                    if (ca < cb) ent[cur[ca]++] = Entry{cb, pos[a], pos[b]};
+                    else ent[cur[cb]++] = Entry{ca, pos[b], pos[a]};
+                }
+        }
+    }
+    // 2. per camera: its partner cameras ascending, the entries grouped by partner (stable)
+    std::vector<std::vector<uint32_t>> js(nc), jn(nc);
+    auto group = [&](size_t c0, size_t c1) {
+        std::vector<Entry> tmp;
+        std::vector<uint32_t> all;
+        for (size_t c = c0; c < c1; ++c) {
+            const uint64_t b = boff[c], e = boff[c + 1];
+            if (b == e) continue;
+            all.clear();
+            for (uint64_t i = b; i < e; ++i) all.push_back(ent[i].j);
+            std::sort(all.begin(), all.end());
+            all.erase(std::unique(all.begin(), all.end()), all.end());
+            std::vector<uint64_t> at(all.size() + 1, 0);
+            for (uint64_t i = b; i < e; ++i) {
+                const size_t k = std::lower_bound(all.begin(), all.end(), ent[i].j) - all.begin();
+                at[k + 1]++;
+            }
+            jn[c].resize(all.size());
+            for (size_t k = 0; k < all.size(); ++k) {
+                jn[c][k] = (uint32_t)at[k + 1];
+                at[k + 1] += at[k];
+            }
+            tmp.resize(e - b);
+            for (uint64_t i = b; i < e; ++i) {
+                const size_t k = std::lower_bound(all.begin(), all.end(), ent[i].j) - all.begin();
+                tmp[at[k]++] = ent[i];
+            }
+            std::copy(tmp.begin(), tmp.end(), ent.begin() + b);
+            js[c] = all;
+        }
+    };
+    {
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        const size_t per = (nc + hw - 1) / hw;
+        for (unsigned t = 0; t < hw; ++t) {
+            const size_t c0 = std::min(nc, t * per), c1 = std::min(nc, c0 + per);
+            if (c0 < c1) th.emplace_back(group, c0, c1);
+        }
+        for (auto& x : th) x.join();
+    }
+    // 3. neighbour rows: lower partners (ascending), the camera, upper partners
+    std::vector<std::vector<uint32_t>> rev(nc);
+    std::vector<uint32_t> rev_slot;  // per pair: slot of c in row j
+    for (size_t c = 0; c < nc; ++c)
+        for (uint32_t j : js[c]) {
+            rev_slot.push_back((uint32_t)rev[j].size());
+            rev[j].push_back((uint32_t)c);
+        }
+    S.nbr_off.assign(nc + 1, 0);
+    S.diag_slot.resize(nc);
+    S.row_base.resize(nc);
+    for (size_t c = 0; c < nc; ++c) {
+        S.diag_slot[c] = (uint32_t)rev[c].size();
+        S.nbr_off[c + 1] = S.nbr_off[c] + (uint32_t)(rev[c].size() + 1 + js[c].size());
+        S.row_base[c] = 81ull * S.nbr_off[c];
+    }
+    S.n_values = 81ull * S.nbr_off[nc];
+    S.nbr.resize(S.nbr_off[nc]);
+    for (size_t c = 0; c < nc; ++c) {
+        uint32_t* out = S.nbr.data() + S.nbr_off[c];
+        for (uint32_t i : rev[c]) *out++ = i;
+        *out++ = (uint32_t)c;
+        for (uint32_t j : js[c]) *out++ = j;
+    }
+    // 4. pairs in (c, j) order and their terms
+    S.term_off.push_back(0);
+    S.term.reserve(2 * ent.size());
+    size_t k = 0;
+    for (size_t c = 0; c < nc; ++c) {
+        uint64_t i = boff[c];
+        for (size_t m = 0; m < js[c].size(); ++m, ++k) {
+            S.pair_cam.push_back((uint32_t)c);
+            S.pair_cam.push_back(js[c][m]);
+            S.pair_slot.push_back((uint32_t)(rev[c].size() + 1 + m));
+            S.pair_slot.push_back(rev_slot[k]);
+            for (uint32_t t = 0; t < jn[c][m]; ++t, ++i) {
+                S.term.push_back(ent[i].pc);
+                S.term.push_back(ent[i].pj);
+            }
+            S.term_off.push_back((uint32_t)(S.term.size() / 2));
+        }
+    }
+    return S;
 }
 
 void raise_solution(const ba_problem* p, const Lowered& L, const double* R, const double* t, const double* gp,

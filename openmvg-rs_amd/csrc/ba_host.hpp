@@ -81,6 +81,22 @@ struct PairLists {
 // pose-intrinsic edges of each camera (the retained operator and its preconditioner).
 PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all);
 
+// Lists of the explicit reduced camera system, for topologies with one camera per intrinsic
+// group and at most one observation per point and camera.  Row c of S holds its neighbour cameras
+// (those sharing a free point with c, and c itself) ascending; the camera pairs c < j carry the
+// observation pairs (camera-major positions pos[o] of the observations by c and by j) of their
+// common free points, in point order.
+struct SchurLists {
+    std::vector<uint32_t> nbr_off, nbr, diag_slot;  // nc + 1, neighbours, nc
+    std::vector<uint64_t> row_base;                 // nc: first value of row c (81 per slot)
+    std::vector<uint32_t> pair_cam, pair_slot;      // 2 per pair: (c, j), (slot of j in c, of c in j)
+    std::vector<uint32_t> term_off, term;           // n_pairs + 1, 2 per term
+    uint64_t n_values = 0;
+};
+// Number of observation pairs (sum of k(k-1)/2 over free points with k observations).
+uint64_t count_schur_terms(const Lowered& L);
+SchurLists build_schur_lists(const Lowered& L, const std::vector<uint32_t>& pos);
+
 // Raise the native state back into the caller's frame (native/self_calibration.rs:375-436,
 // native/world_rebase.rs:61-86).
 void raise_solution(const ba_problem* p, const Lowered& L, const double* R, const double* t,

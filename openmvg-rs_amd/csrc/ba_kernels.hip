@@ -790,7 +790,10 @@ __global__ void __launch_bounds__(NT) k_block_gradient(Topo tp, Lin<T> lin, Poin
 // ----------------------------------------------------------------------------------------------
 namespace {
 __device__ __forceinline__ int tri(int r, int c) { return r * (r + 1) / 2 + c; }  // r >= c
-constexpr int CA_V = 54;  // per segment: 9 full, 9 reduced part, 9 diagonal | 21 pose block, 6 intrinsic block
+// per segment: 9 full, 9 reduced part, 9 diagonal | 21 pose block, 6 intrinsic block, 18 pose x
+// intrinsic cross terms
+constexpr int CA_V = CAM_ASM_VALUES;
+static_assert(CA_V == 27 + 45, "camera assembly values");
 
 // J~_c (2x9, camera frame) and J~_X (2x3) of an observation rebuilt from its compact state (o),
 // its rotated point RX and the camera (R, f; pose dimension pd, baseline flag, gauge gg).
@@ -828,11 +831,21 @@ __device__ __forceinline__ void rebuild_jacobians(const Obs& o, const Cam& cam, 
 }
 }  // namespace
 
+constexpr int ZS_STRIDE = ZREC + 1;  // LDS row of a staged Z record (odd: fewer bank conflicts)
+// Orders one wavefront's LDS accesses around this point (its lanes exchange data through LDS
+// without a workgroup barrier; a wavefront's LDS instructions complete in issue order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class T>
-__global__ void __launch_bounds__(128) k_camera_assembly(Topo tp, Lin<T> lin, PointElim<T> pe, const double* Rg,
+__global__ void __launch_bounds__(192) k_camera_assembly(Topo tp, Lin<T> lin, PointElim<T> pe, const double* Rg,
                                                          const double* gp, const GaugeDev* gauge,
                                                          const uint32_t* __restrict__ seg_cam,
-                                                         const uint32_t* __restrict__ cseg_off, double* part) {
+                                                         const uint32_t* __restrict__ cseg_off, double* part,
+                                                         T* __restrict__ zrec) {
     const uint32_t k = blockIdx.x;
     const uint32_t lane = threadIdx.x & 63, role = threadIdx.x >> 6;  // role uniform per wavefront
     constexpr uint32_t VN = pcg_vec<T>(), SEG = pcg_seg<T>();
@@ -846,6 +859,7 @@ __global__ void __launch_bounds__(128) k_camera_assembly(Topo tp, Lin<T> lin, Po
     cam.f = g3[0], cam.k1 = g3[1], cam.k2 = g3[2];
     const uint32_t pd = tp.cam_dim[c];
     const bool baseline = (int)c == gg.camera;
+    __shared__ T zs[64 * ZS_STRIDE];  // role 0's Z records of one pass (padded rows)
     double acc[27];
 #pragma unroll
     for (int q = 0; q < 27; ++q) acc[q] = 0.0;
@@ -882,6 +896,42 @@ __global__ void __launch_bounds__(128) k_camera_assembly(Topo tp, Lin<T> lin, Po
                 acc[9 + q] += jc[0][q] * u0 + jc[1][q] * u1;
                 acc[18 + q] += jc[0][q] * jc[0][q] + jc[1][q] * jc[1][q];
             }
+            if (zrec) {
+                // M M^T = V^-1 (lower Cholesky; a non-positive pivot drops its column, as for the
+                // zero V^-1 of fixed or not eliminated points), Z = M^T (J~_X^T J~_c); staged in LDS
+                // so that the wavefront's 64 consecutive records are written coalesced
+                const double* vi = pv;
+                const double m00 = vi[0] > 0.0 ? sqrt(vi[0]) : 0.0;
+                const double i00 = m00 > 0.0 ? 1.0 / m00 : 0.0;
+                const double m10 = vi[1] * i00, m20 = vi[3] * i00;
+                const double p11 = vi[2] - m10 * m10;
+                const double m11 = p11 > 0.0 ? sqrt(p11) : 0.0;
+                const double m21 = m11 > 0.0 ? (vi[4] - m20 * m10) / m11 : 0.0;
+                const double p22 = vi[5] - m20 * m20 - m21 * m21;
+                const double m22 = p22 > 0.0 ? sqrt(p22) : 0.0;
+                T* zo = zs + ZS_STRIDE * lane;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    const double w0 = a[0] * jc[0][q] + a[3] * jc[1][q];
+                    const double w1 = a[1] * jc[0][q] + a[4] * jc[1][q];
+                    const double w2 = a[2] * jc[0][q] + a[5] * jc[1][q];
+                    zo[q] = (T)(m00 * w0 + m10 * w1 + m20 * w2);
+                    zo[9 + q] = (T)(m11 * w1 + m21 * w2);
+                    zo[18 + q] = (T)(m22 * w2);
+                }
+                zo[27] = T(0);
+                wave_lds_sync();
+                // the records of positions wb + 64 it .. + 63 are contiguous: element e of that span
+                // is value e % ZREC of record e / ZREC; only live records are written
+                T* dst = zrec + (size_t)ZREC * (wb + 64 * it);
+#pragma unroll 2
+                for (int i = 0; i < ZREC; ++i) {
+                    const uint32_t e = lane + 64 * i, rec = e / ZREC;
+                    const uint32_t jr = wb + 64 * it + rec;
+                    if (jr >= cb && jr < ce) dst[e] = zs[ZS_STRIDE * rec + e % ZREC];
+                }
+                wave_lds_sync();
+            }
         } else {
             // N = I - M, M = J~_X V^-1 J~_X^T
             double n00 = 1.0, n01 = 0.0, n11 = 1.0;
@@ -900,46 +950,57 @@ __global__ void __launch_bounds__(128) k_camera_assembly(Topo tp, Lin<T> lin, Po
                 pn[0][q] = n00 * jc[0][q] + n01 * jc[1][q];
                 pn[1][q] = n01 * jc[0][q] + n11 * jc[1][q];
             }
+            if (role == 1) {
 #pragma unroll
-            for (int x = 0; x < 6; ++x)
+                for (int x = 0; x < 6; ++x)
 #pragma unroll
-                for (int y = 0; y <= x; ++y) acc[tri(x, y)] += jc[0][x] * pn[0][y] + jc[1][x] * pn[1][y];
+                    for (int y = 0; y <= x; ++y) acc[tri(x, y)] += jc[0][x] * pn[0][y] + jc[1][x] * pn[1][y];
 #pragma unroll
-            for (int x = 0; x < 3; ++x)
+                for (int x = 0; x < 3; ++x)
 #pragma unroll
-                for (int y = 0; y <= x; ++y)
-                    acc[21 + tri(x, y)] += jc[0][6 + x] * pn[0][6 + y] + jc[1][6 + x] * pn[1][6 + y];
+                    for (int y = 0; y <= x; ++y)
+                        acc[21 + tri(x, y)] += jc[0][6 + x] * pn[0][6 + y] + jc[1][6 + x] * pn[1][6 + y];
+            } else {  // role 2: pose x intrinsic cross terms
+#pragma unroll
+                for (int x = 0; x < 6; ++x)
+#pragma unroll
+                    for (int y = 0; y < 3; ++y) acc[3 * x + y] += jc[0][x] * pn[0][6 + y] + jc[1][x] * pn[1][6 + y];
+            }
         }
     }
+    const int nv = role == 2 ? 18 : 27;
 #pragma unroll
-    for (int q = 0; q < 27; ++q) acc[q] = wave_sum(acc[q]);
+    for (int q = 0; q < 27; ++q)
+        if (q < nv) acc[q] = wave_sum(acc[q]);
     if (lane == 0) {
         double* out = part + CA_V * (size_t)k + 27 * role;
 #pragma unroll
-        for (int q = 0; q < 27; ++q) out[q] = acc[q];
+        for (int q = 0; q < 27; ++q)
+            if (q < nv) out[q] = acc[q];
     }
 }
 
-// One wavefront per camera: lane v < CA_V adds value v over the camera's segments in order.
+// One wavefront per camera: lane v adds values v, v + 64, ... over the camera's segments in order.
+template <class T>
 __global__ void __launch_bounds__(64) k_camera_assembly_out(Topo tp, const uint32_t* __restrict__ cseg_off,
                                                             const double* part,
                                                             const int32_t* __restrict__ diag_pair,
                                                             const uint64_t* __restrict__ value_off, double* grad,
                                                             double* full_grad, double* diag, double* u_vals,
-                                                            double* s_vals) {
+                                                            double* s_vals, SchurRows<T> S) {
     __shared__ double tot[CA_V];
     const uint32_t c = blockIdx.x, v = threadIdx.x;
-    if (v < CA_V) {
+    for (uint32_t u = v; u < (uint32_t)CA_V; u += 64) {
         double s = 0.0;
         const uint32_t g0 = cseg_off[c], g1 = cseg_off[c + 1];
         for (uint32_t g = g0; g < g1; g += 4) {  // four loads in flight, added in segment order
             double x[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) x[i] = g + i < g1 ? part[CA_V * (size_t)(g + i) + v] : 0.0;
+            for (int i = 0; i < 4; ++i) x[i] = g + i < g1 ? part[CA_V * (size_t)(g + i) + u] : 0.0;
 #pragma unroll
             for (int i = 0; i < 4; ++i) s += x[i];
         }
-        tot[v] = s;
+        tot[u] = s;
     }
     __syncthreads();
     const uint32_t pd = tp.cam_dim[c], po = tp.cam_off[c];
@@ -956,20 +1017,35 @@ __global__ void __launch_bounds__(64) k_camera_assembly_out(Topo tp, const uint3
     }
     if (pd > 0 && diag_pair[c] >= 0) {
         double* U = u_vals + value_off[diag_pair[c]];
-        double* S = s_vals + value_off[diag_pair[c]];
+        double* Sv = s_vals + value_off[diag_pair[c]];
         for (uint32_t e = v; e < pd * pd; e += 64) {
             const uint32_t x = e / pd, y = e % pd;
             U[e] = tot[27 + (x >= y ? tri(x, y) : tri(y, x))];
-            S[e] = 0.0;
+            Sv[e] = 0.0;
         }
     }
     if (gd > 0 && diag_pair[tp.nc + g] >= 0) {
         double* U = u_vals + value_off[diag_pair[tp.nc + g]];
-        double* S = s_vals + value_off[diag_pair[tp.nc + g]];
+        double* Sv = s_vals + value_off[diag_pair[tp.nc + g]];
         for (uint32_t e = v; e < gd * gd; e += 64) {
             const uint32_t x = e / gd, y = e % gd;
             U[e] = tot[48 + (x >= y ? tri(x, y) : tri(y, x))];
-            S[e] = 0.0;
+            Sv[e] = 0.0;
+        }
+    }
+    if (S.vals) {  // the camera's diagonal block of S, 9 x 9 (zero beyond the free dimensions)
+        const uint32_t nn = S.nbr_off[c + 1] - S.nbr_off[c];
+        T* row = S.vals + S.row_base[c] + 9 * (size_t)S.diag_slot[c];
+        for (uint32_t e = v; e < 81; e += 64) {
+            const uint32_t x = e / 9, y = e % 9;
+            const bool in = (x < 6 ? x < pd : x - 6 < gd) && (y < 6 ? y < pd : y - 6 < gd);
+            double val = 0.0;
+            if (in) {
+                if (x < 6 && y < 6) val = tot[27 + (x >= y ? tri(x, y) : tri(y, x))];
+                else if (x >= 6 && y >= 6) val = tot[48 + (x >= y ? tri(x - 6, y - 6) : tri(y - 6, x - 6))];
+                else val = x < 6 ? tot[54 + 3 * x + (y - 6)] : tot[54 + 3 * y + (x - 6)];
+            }
+            row[(size_t)x * 9 * nn + y] = (T)val;
         }
     }
 }
@@ -2019,7 +2095,13 @@ __global__ void __launch_bounds__(NTS) k_pcg_dir(Topo tp, PcgVecs V) {
             }
             if (upd)
                 for (uint32_t k = 0; k < n; ++k) dn[off + k] = upd == 1 ? zv[k] + beta * dv[k] : zv[k];
-            if (cam) {
+            if (cam && V.vexp) {  // explicit product: the input in camera coordinates
+                double* ve = V.vexp + 9 * (size_t)A;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) ve[k] = vp[k];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) ve[6 + k] = vi[k];
+            } else if (cam) {
                 double* rec = V.vcam + CAM_REC * (size_t)A;
                 camera_record(tp, A, V.R + 9 * (size_t)A, V.gp + 3 * (size_t)tp.cam_group[A], *V.gauge, vp, vi, rec);
                 Cam cr;
@@ -2032,6 +2114,135 @@ __global__ void __launch_bounds__(NTS) k_pcg_dir(Topo tp, PcgVecs V) {
         if (upd) s.par = 1 - s.par;
     }
     if (lead) *V.st = s;
+}
+
+// ---- explicit reduced camera system ----
+namespace {
+template <class T>
+struct PairMfma;
+template <>
+struct PairMfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+};
+template <>
+struct PairMfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+};
+}  // namespace
+
+// Off-diagonal blocks, one wavefront per camera pair (c, j): S_cj = -sum_t Z_o(t)^T Z_o'(t) as one
+// 16x16x4 matrix-core step per term (lane l supplies Z_o[l/16][l%16] and Z_o'[l/16][l%16]; rows
+// and columns 9..15 and k = 3 are zero; terms past the end add zero).  Terms go in batches of
+// SP_DEPTH whose loads are issued one batch ahead.  Pairs go in (c, j) order over the whole chip,
+// so at any time every XCD gathers from the same band of cameras' records, which the shared
+// last-level cache then holds (dealing each XCD a contiguous pair range measured 1.5x slower).
+// The block goes to row c and, transposed, to row j.
+constexpr int SP_WAVES = 4;
+constexpr int SP_DEPTH = 4;
+template <class T>
+__global__ void __launch_bounds__(64 * SP_WAVES) k_schur_pairs(SchurPairs P, const T* __restrict__ z, SchurRows<T> S) {
+    typedef typename PairMfma<T>::v4 v4;
+    const uint32_t k = blockIdx.x * SP_WAVES + (threadIdx.x >> 6);
+    if (k >= P.npairs) return;  // whole wavefronts leave; no barrier follows
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const bool in = li < 9 && lk < 3;
+    const uint32_t off = lk * 9 + li;
+    const uint32_t t0 = P.term_off[k], t1 = P.term_off[k + 1];
+    auto load = [&](uint32_t t, T* x, T* y) {
+#pragma unroll
+        for (int u = 0; u < SP_DEPTH; ++u) {
+            const bool live = in && t + u < t1;
+            const uint32_t tc = live ? P.term[2 * (t + u)] : 0, tj = live ? P.term[2 * (t + u) + 1] : 0;
+            x[u] = live ? z[(size_t)ZREC * tc + off] : T(0);
+            y[u] = live ? z[(size_t)ZREC * tj + off] : T(0);
+        }
+    };
+    v4 acc = v4{T(0), T(0), T(0), T(0)};
+    T xa[SP_DEPTH], ya[SP_DEPTH];
+    load(t0, xa, ya);
+    for (uint32_t t = t0; t < t1; t += SP_DEPTH) {
+        T xn[SP_DEPTH], yn[SP_DEPTH];
+        load(t + SP_DEPTH, xn, yn);
+#pragma unroll
+        for (int u = 0; u < SP_DEPTH; ++u) acc = PairMfma<T>::mac(xa[u], ya[u], acc);
+#pragma unroll
+        for (int u = 0; u < SP_DEPTH; ++u) {
+            xa[u] = xn[u];
+            ya[u] = yn[u];
+        }
+    }
+    const uint32_t c = P.cam[2 * k], j = P.cam[2 * k + 1];
+    const uint32_t nnc = S.nbr_off[c + 1] - S.nbr_off[c], nnj = S.nbr_off[j + 1] - S.nbr_off[j];
+    T* rc = S.vals + S.row_base[c] + 9 * (size_t)P.slot[2 * k];
+    T* rj = S.vals + S.row_base[j] + 9 * (size_t)P.slot[2 * k + 1];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int a = PairMfma<T>::row(lk, v), b = li;
+        if (a < 9 && b < 9) {
+            const T val = -acc[v];
+            rc[(size_t)a * 9 * nnc + b] = val;
+            rj[(size_t)b * 9 * nnj + a] = val;
+        }
+    }
+}
+
+// Explicit product, one wavefront per camera row (c, a): q_a = sum over the row's neighbour slots
+// s and columns b of S[c][a][s][b] v_nbr(s),b (lane-strided, then the wavefront butterfly), plus
+// lambda diag v on the row's parameter; one partial of v.q per workgroup of four rows, in order.
+constexpr int PP_WAVES = 16;
+template <class T>
+__global__ void __launch_bounds__(64 * PP_WAVES) k_pcg_product(Topo tp, SchurRows<T> S, PcgVecs V) {
+    __shared__ double sh[PP_WAVES];
+    const PcgState st = *V.st;
+    if (st.done) return;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t row = blockIdx.x * PP_WAVES + wv;
+    double vq = 0.0;
+    if (row < 9 * tp.nc) {  // uniform over the wavefront
+        const uint32_t c = row / 9, a = row % 9;
+        const uint32_t pd = tp.cam_dim[c], g = tp.cam_group[c], gd = tp.grp_dim[g];
+        const bool live = a < 6 ? a < pd : a - 6 < gd;
+        if (live) {
+            const uint32_t nb = S.nbr_off[c], nn = S.nbr_off[c + 1] - nb;
+            const T* __restrict__ sv = S.vals + S.row_base[c] + (size_t)a * 9 * nn;
+            const uint32_t len = 9 * nn;
+            double sacc[4] = {0.0, 0.0, 0.0, 0.0};
+            uint32_t e = lane;
+            for (; e + 192 < len; e += 256) {  // four elements in flight per lane
+                T m[4];
+                uint32_t jj[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    m[u] = sv[e + 64 * u];
+                    jj[u] = S.nbr[nb + (e + 64 * u) / 9];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) sacc[u] += (double)m[u] * V.vexp[9 * (size_t)jj[u] + (e + 64 * u) % 9];
+            }
+            for (; e < len; e += 64) sacc[0] += (double)sv[e] * V.vexp[9 * (size_t)S.nbr[nb + e / 9] + e % 9];
+            const double qa = wave_sum((sacc[0] + sacc[1]) + (sacc[2] + sacc[3]));
+            const uint32_t par = a < 6 ? tp.cam_off[c] + a : tp.grp_off[g] + (a - 6);
+            const double va = V.vexp[9 * (size_t)c + a];
+            const double qv = qa + V.lambda * V.diag[par] * va;
+            if (lane == 0) V.q[par] = qv;
+            vq = va * qv;
+        }
+    }
+    if (lane == 0) sh[wv] = vq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int i = 0; i < PP_WAVES; ++i) acc += sh[i];
+        V.part[blockIdx.x] = acc;
+    }
 }
 
 // First radial turn of each group's (k1, k2) (model.rs:131-161), +inf where there is none; read by
@@ -2080,13 +2291,13 @@ void launch_camera_assembly(hipStream_t s, const Topo& tp, const Lin<T> lin, con
                             const double* gp, const GaugeDev* gauge, uint32_t nseg, const uint32_t* seg_cam,
                             const uint32_t* cseg_off, double* part, const int32_t* diag_pair,
                             const uint64_t* value_off, double* grad, double* full_grad, double* diag, double* u_vals,
-                            double* s_vals) {
+                            double* s_vals, T* z, SchurRows<T> S) {
     if (tp.nc == 0) return;
-    if (nseg)
-        hipLaunchKernelGGL(k_camera_assembly<T>, dim3(nseg), dim3(128), 0, s, tp, lin, pe, R, gp, gauge, seg_cam,
-                           cseg_off, part);
-    hipLaunchKernelGGL(k_camera_assembly_out, dim3(tp.nc), dim3(64), 0, s, tp, cseg_off, part, diag_pair, value_off,
-                       grad, full_grad, diag, u_vals, s_vals);
+    if (nseg)  // the third wavefront (cross terms) only for the explicit system
+        hipLaunchKernelGGL(k_camera_assembly<T>, dim3(nseg), dim3(S.vals ? 192 : 128), 0, s, tp, lin, pe, R, gp, gauge,
+                           seg_cam, cseg_off, part, z);
+    hipLaunchKernelGGL(k_camera_assembly_out<T>, dim3(tp.nc), dim3(64), 0, s, tp, cseg_off, part, diag_pair, value_off,
+                       grad, full_grad, diag, u_vals, s_vals, S);
 }
 template <class T>
 void launch_pairs(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, uint32_t npairs,
@@ -2184,6 +2395,22 @@ void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const Po
         hipLaunchKernelGGL((k_pcg_points<T, false>), dim3(g), dim3(PT_NT), 0, s, tp, lin, pe, V);
     }
 }
+uint32_t pcg_product_partials(const Topo& tp) { return grid_for(9 * (size_t)tp.nc, PP_WAVES); }
+template <class T>
+void launch_pcg_product(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V) {
+    if (tp.nc) hipLaunchKernelGGL(k_pcg_product<T>, dim3(pcg_product_partials(tp)), dim3(64 * PP_WAVES), 0, s, tp, S, V);
+}
+template <class T>
+void launch_pcg_step_explicit(hipStream_t s, const Topo& tp, const PcgVecs& V) {
+    const unsigned g = pcg_partials(tp);
+    hipLaunchKernelGGL(k_pcg_xr, dim3(g), dim3(NTS), 0, s, tp, V);
+    hipLaunchKernelGGL(k_pcg_dir, dim3(g), dim3(NTS), 0, s, tp, V);
+}
+template <class T>
+void launch_schur_pairs(hipStream_t s, const SchurPairs& P, const T* z, SchurRows<T> S) {
+    if (P.npairs)
+        hipLaunchKernelGGL(k_schur_pairs<T>, dim3(grid_for(P.npairs, SP_WAVES)), dim3(64 * SP_WAVES), 0, s, P, z, S);
+}
 template <class T>
 void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const PcgVecs& V) {
     if (V.nseg) hipLaunchKernelGGL(k_pcg_cameras<T>, dim3(grid_for(V.nseg, NT / 64)), dim3(NT), 0, s, tp, lin, V);
@@ -2214,7 +2441,11 @@ template void launch_gauge<double>(hipStream_t, const double*, int, double, Gaug
     template void launch_camera_assembly<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>,              \
                                             const double*, const double*, const GaugeDev*, uint32_t,               \
                                             const uint32_t*, const uint32_t*, double*, const int32_t*,             \
-                                            const uint64_t*, double*, double*, double*, double*, double*);         \
+                                            const uint64_t*, double*, double*, double*, double*, double*, T*,      \
+                                            SchurRows<T>);                                                         \
+    template void launch_schur_pairs<T>(hipStream_t, const SchurPairs&, const T*, SchurRows<T>);                    \
+    template void launch_pcg_product<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&);              \
+    template void launch_pcg_step_explicit<T>(hipStream_t, const Topo&, const PcgVecs&);                            \
     template void launch_pairs<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>, uint32_t,               \
                                   const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,              \
                                   const uint32_t*, const uint64_t*, double*, double*);                             \

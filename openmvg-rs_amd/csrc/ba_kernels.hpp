@@ -133,18 +133,49 @@ void launch_point_elim(hipStream_t s, const Topo& tp, const Lin<T> lin, T lambda
 // Sum of per-point scalars in a fixed order -> AsmScalars fields cost..invalid.
 void launch_reduce_points(hipStream_t s, const double* pscal, uint32_t np, double* partials,
                           AsmScalars* out);
+// ---- explicit reduced camera system (same topologies as the camera-side assembly) ----
+// S = U - W V^-1 W^T in camera blocks of 9 rows (pose 0..5, the camera's intrinsics 6..8; rows
+// and columns beyond the free dimensions are zero).  Row-slice layout: row a of camera c holds 9
+// values per neighbour slot s (neighbour camera nbr[nbr_off[c] + s], ascending, c itself
+// included): vals[row_base[c] + a * 9 * nn_c + 9 * s + b], nn_c = nbr_off[c + 1] - nbr_off[c].
+template <class T>
+struct SchurRows {
+    T* vals;
+    const uint64_t* row_base;   // nc
+    const uint32_t* nbr_off;    // nc + 1
+    const uint32_t* nbr;        // neighbour cameras
+    const uint32_t* diag_slot;  // nc: the camera's own slot
+};
+// Camera pairs c < j that see common points, with the observation pairs (camera-major positions
+// of the observations by c and by j) of those points in point order.
+struct SchurPairs {
+    uint32_t npairs;
+    const uint32_t* cam;       // 2 per pair: c, j
+    const uint32_t* slot;      // 2 per pair: slot of j in row c, slot of c in row j
+    const uint32_t* term_off;  // npairs + 1
+    const uint32_t* term;      // 2 per term
+};
+// Per camera-major observation: Z = M^T J~_X^T J~_c (3 x 9, row-major, padded to ZREC), where
+// M M^T = V^-1 of the observation's point; Z_o^T Z_o' = W_o V^-1 W_o'^T.
+constexpr int ZREC = 28;
+// Off-diagonal blocks S_cj = -sum Z_o^T Z_o' over each pair's terms, stored in rows c and j.
+template <class T>
+void launch_schur_pairs(hipStream_t s, const SchurPairs& P, const T* z, SchurRows<T> S);
+
 // Camera-side assembly of the PCG form when every intrinsic group has one camera and no camera
 // sees a point twice: per camera, the pose and intrinsic blocks' full and reduced gradients,
 // unclamped diagonal, and the diagonal blocks of U - W V^-1 W^T (into the diagonal pairs' U
 // values; their Schur values are set to zero).  Runs over the PCG camera segments; part is
-// scratch of 54 doubles per segment.  The Jacobians are rebuilt from the camera-major compact state
+// scratch of CAM_ASM_VALUES doubles per segment.  With z set it also writes each camera-major
+// observation's Z record and, with S.vals set, each camera's full diagonal block of S.  The Jacobians are rebuilt from the camera-major compact state
 // and rotated points with the rotations R and group intrinsics gp.
+constexpr int CAM_ASM_VALUES = 72;
 template <class T>
 void launch_camera_assembly(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe, const double* R,
                             const double* gp, const GaugeDev* gauge, uint32_t nseg, const uint32_t* seg_cam,
                             const uint32_t* cseg_off, double* part, const int32_t* diag_pair,
                             const uint64_t* value_off, double* grad, double* full_grad, double* diag, double* u_vals,
-                            double* s_vals);
+                            double* s_vals, T* z = nullptr, SchurRows<T> S = SchurRows<T>{});
 // Per block: full gradient J~^T r~, reduced gradient (full minus W V^-1 g), unclamped diagonal.
 template <class T>
 void launch_block_gradient(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe,
@@ -222,6 +253,7 @@ struct PcgVecs {
     double* seg;   // 9 per camera segment: the camera pass sums (E, F, I)
     double* vcam;  // PCG_CAM_REC per camera: rotation, intrinsics and the product input
     double* wcam;  // PCG_WREC per camera: the product input in the world frame (point pass)
+    double* vexp;  // 9 per camera: the product input in camera coordinates (explicit product), or null
     double* part;  // per-workgroup partial dot products: v.q (nqparts), r.r and r.z (nparts each)
     uint32_t nqparts, nparts;
     uint32_t nseg;
@@ -255,6 +287,14 @@ void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const P
 // q = S v per block, then the step (x, r, z) and direction (d, per-camera input) updates.
 template <class T>
 void launch_pcg_step(hipStream_t s, const Topo& tp, const PcgVecs& V);
+// Explicit product q = S v + lambda diag v (v = d or x by the PCG mode, from V.vexp) with the
+// partials of v.q, then the step and direction updates.
+template <class T>
+void launch_pcg_product(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V);
+template <class T>
+void launch_pcg_step_explicit(hipStream_t s, const Topo& tp, const PcgVecs& V);
+// Partials of the explicit product (one per workgroup of four camera rows).
+uint32_t pcg_product_partials(const Topo& tp);
 // Camera-major linearisation: the SoA camera-major Jacobian copy (lin.jcm).
 template <class T>
 void launch_linearize_cm(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,

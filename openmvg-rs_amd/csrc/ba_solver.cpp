@@ -331,7 +331,9 @@ Status run_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* c
                  ba_solution* sol, ba_solve_report& r, ba_iteration_trace* trace, double t0) {
     const bool iterative = cfg->linear_solver == BA_SOLVER_ITERATIVE;
     Engine<T> E(h->stream, L, iterative, cfg->profile_kernel);
+    E.set_schur_product(h->schur_product);
     E.setup();
+    if (iterative) h->last_schur_product = E.explicit_product() ? BA_SCHUR_EXPLICIT : BA_SCHUR_IMPLICIT;
     double t1 = now();
     int state = 0;
     double loop_seconds = 0.0;
@@ -373,6 +375,14 @@ int ba_hip_create(int device, ba_hip_handle** out) {
     *out = h;
     return BA_OK;
 }
+
+int ba_hip_set_schur_product(ba_hip_handle* h, int32_t mode) {
+    if (!h || mode < BA_SCHUR_AUTO || mode > BA_SCHUR_EXPLICIT) return BA_ERR_INVALID_CONFIG;
+    h->schur_product = mode;
+    return BA_OK;
+}
+
+int32_t ba_hip_last_schur_product(const ba_hip_handle* h) { return h ? h->last_schur_product : BA_SCHUR_AUTO; }
 
 void ba_hip_destroy(ba_hip_handle* h) {
     if (!h) return;
@@ -543,7 +553,9 @@ int ba_hip_pcg(ba_hip_handle* h, const ba_device_problem* dp, double lambda, dou
     Status st = lowered_from_device(dp, L, h->stream);
     if (!st.ok()) return st.code;
     Engine<double> E(h->stream, L, true);
+    E.set_schur_product(h->schur_product);
     E.setup();
+    h->last_schur_product = E.explicit_product() ? BA_SCHUR_EXPLICIT : BA_SCHUR_IMPLICIT;
     E.set_gauge(gauge_from(dp));
     AsmScalars a = E.assemble(0, lambda, L.loss, L.loss_scale, true);
     uint32_t terms = 0;

@@ -48,6 +48,13 @@ PROFILE_PCG_CAMERAS = 4
 PROFILE_PAIRS = 5
 PROFILE_CHOLESKY = 6
 PROFILE_CANDIDATE = 7
+PROFILE_SCHUR_PAIRS = 8
+PROFILE_PCG_PRODUCT = 9
+
+# ba_schur_product
+SCHUR_AUTO = 0
+SCHUR_IMPLICIT = 1
+SCHUR_EXPLICIT = 2
 
 STATUS_CONVERGED_GRADIENT = 0
 STATUS_CONVERGED_STEP = 1

@@ -38,6 +38,9 @@ def load() -> C.CDLL:
     lib.ba_hip_schur.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp]
     lib.ba_hip_pcg.argtypes = [vp, vp, C.c_double, C.c_double, C.c_uint32, vp, vp, vp, vp]
     lib.ba_hip_cholesky_solve.argtypes = [vp, C.c_size_t, vp, vp, vp, C.c_int32]
+    lib.ba_hip_set_schur_product.argtypes = [vp, C.c_int32]
+    lib.ba_hip_last_schur_product.argtypes = [vp]
+    lib.ba_hip_last_schur_product.restype = C.c_int32
     sz = C.c_size_t
     lib.ba_hip_reprojection_topology_create.argtypes = [vp, sz, sz, sz, vp, vp, C.POINTER(vp), C.c_char_p, sz]
     lib.ba_hip_reprojection_topology_destroy.argtypes = [vp]

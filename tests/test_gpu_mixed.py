@@ -6,9 +6,10 @@ on the matrix cores and refines each Newton step in FP64; the LM control (gain r
 acceptance on the FP64 candidate cost) is FP64.  Its error against the FP64 oracle is stated,
 not bit-matched:
   * dense solve: refined relative residual <= 1e-10, solution within 1e-8 of LAPACK;
-  * LM solves: per-iteration cost within 1e-6 relative, final cost within 1e-9 relative, final
-    parameters within 1e-6 relative (scale max(|x|, 1)) of the FP64 oracle on the ring scene, and
-    within 1e-5 of the FP64 form on the medium scene solved to tight tolerances.
+  * LM solves: per-iteration cost within 1e-6 relative (5e-6 for the PCG solve, whose explicit
+    reduced system is stored in FP32), final cost within 1e-9 relative, final parameters within
+    1e-6 relative (scale max(|x|, 1)) of the FP64 oracle on the ring scene, and within 1e-5 of
+    the FP64 form on the medium scene solved to tight tolerances.
 The measured errors are printed with -s.
 """
 
@@ -105,7 +106,8 @@ def test_mixed_ring_solve_vs_fp64_oracle(adjuster, oracle, solver):
     it_err, final, par = _errors(got, want)
     print(f"ring solver={solver}: per-iteration cost {it_err:.2e}, final cost {final:.2e}, parameters {par:.2e}, "
           f"fallbacks {got.report.precision_fallbacks}, refinement residual {got.report.refinement_residual:.2e}")
-    assert it_err <= 1e-6 and final <= 1e-9 and par <= 1e-6
+    it_bound = 1e-6 if solver == A.SOLVER_DIRECT else 5e-6
+    assert it_err <= it_bound and final <= 1e-9 and par <= 1e-6
 
 
 def test_mixed_medium_direct_vs_fp64(adjuster):
