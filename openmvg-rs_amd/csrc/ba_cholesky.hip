@@ -109,19 +109,24 @@ __device__ bool factor_diag_wave(T (*t)[TB + 1], T* col, T (*xs)[TB + 1], uint32
     T r[TB];
 #pragma unroll
     for (int c = 0; c < TB; ++c) r[c] = (i < (int)nb) ? (c <= i ? t[i][c] : T(0)) : (i == c ? T(1) : T(0));
+    bool ok = true;  // (a return inside the unrolled loop would keep r in scratch memory)
 #pragma unroll
     for (int j = 0; j < TB; ++j) {
         wave_sync();
         col[i] = r[j];
         wave_sync();
         const T piv = col[j];
-        if (!(isfinite(piv) && piv > tol)) return false;  // uniform: every lane read the same pivot
+        if (!(isfinite(piv) && piv > tol)) {
+            ok = false;  // uniform: every lane read the same pivot
+            break;
+        }
         const T l = sqrt(piv);
         const T sj = r[j] / piv;
         r[j] = i > j ? r[j] / l : (i == j ? l : r[j]);
 #pragma unroll
         for (int c = j + 1; c < TB; ++c) r[c] -= sj * col[c];
     }
+    if (!ok) return false;
     wave_sync();
 #pragma unroll
     for (int c = 0; c < TB; ++c) t[i][c] = c <= i ? r[c] : T(0);

@@ -191,3 +191,26 @@ def test_golden_initial_cost_of_the_self_calibration_contract_fixture(oracle):
     oracle.bao_native_cost.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]
     cost = oracle.bao_native_cost(C.byref(view), 1, grp.ctypes.data, mean.ctypes.data, sqrt_info.ctypes.data)
     assert abs(cost - 863.644_893_653_298_3) <= 1e-8, cost
+
+
+def test_dense_factor_kernels_keep_their_registers(tmp_path):
+    """The one-wavefront diagonal factor keeps a 64-entry row per lane in registers through fully
+    unrolled loops; a change that defeats the unrolling (an early return inside the loop did)
+    moves the row to scratch memory and slows the factor several times over.  Compiles the dense
+    factor for gfx950 and requires zero scratch in every kernel of the file."""
+    import shutil
+
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not pathlib.Path(hipcc).exists():
+        pytest.skip("hipcc not available")
+    src = ROOT / "openmvg-rs_amd" / "csrc" / "ba_cholesky.hip"
+    r = subprocess.run([hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops", "-fPIC", "--cuda-device-only", "-c", str(src), "-o",
+                        str(tmp_path / "chol.o"), "-Rpass-analysis=kernel-resource-usage"],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    names = re.findall(r"Function Name: (\S+)", r.stderr)
+    scratch = [int(x) for x in re.findall(r"ScratchSize \[bytes/lane\]: (\d+)", r.stderr)]
+    assert names and len(names) == len(scratch)
+    spilled = [(n, s) for n, s in zip(names, scratch) if s]
+    assert not spilled, spilled
