@@ -78,6 +78,14 @@ def schur_slots(problem) -> int:
     return nc + 2 * pairs
 
 
+def schur_counts(problem) -> dict:
+    """Camera pairs and observation pairs of the explicit reduced system."""
+    slots = schur_slots(problem)
+    pt = np.sort(problem.obs_point)
+    k = np.diff(np.r_[np.flatnonzero(np.r_[True, pt[1:] != pt[:-1]]), len(pt)]).astype(np.int64)
+    return {"slots": slots, "pairs": (slots - int(problem.n_poses)) // 2, "terms": int(np.sum(k * (k - 1) // 2))}
+
+
 def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     """Algorithmic HBM bytes of one launch of `kernel` in FP64: each input read once, each output
     written once; vectors indexed by camera or point are counted once per launch.
@@ -101,8 +109,19 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
       row): per neighbour slot the 9 x 9 block (648 B) and, once per row, the neighbour index
       (9 x 4 B); per camera the 9 input values (72 B), its offsets and dimensions (16 B) and the
       written q and read damping diagonal of its 9 rows (144 B).
+    schur_pairs (k_schur_pairs: the off-diagonal blocks of the explicit S): each observation's Z
+      record once (27 values, 216 B), 8 B per observation pair, and per camera pair the two
+      written 9 x 9 blocks and its indices (1,316 B).
+    camera_assembly (k_camera_assembly with the explicit system): per observation the
+      camera-major rows read (72 B) and point index (4 B) and the Z record written (216 B); per
+      point its packed record (128 B); per segment its camera (4 B) and written sums (576 B); per
+      camera R, f, k1, k2 (120 B).
     """
     no, npt, nc = counts["obs"], counts["points"], counts["cameras"]
+    if kernel == "schur_pairs":
+        return no * 216 + counts["terms"] * 8 + counts["pairs"] * (2 * 648 + 20)
+    if kernel == "camera_assembly":
+        return no * (72 + 4 + 216) + npt * 128 + counts["segments"] * (4 + 576) + nc * 120
     if kernel == "pcg_product":
         return counts["slots"] * (648 + 36) + nc * (72 + 16 + 144)
     if kernel == "pcg_points":
@@ -299,6 +318,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES between two device buffers three times (a known-size kernel that "
                          "scales HBM counters in a profiled run)")
+    ap.add_argument("--only-form", choices=["medium_direct", "batched", "differentiation"],
+                    help="run only this secondary measurement and print its JSON (no headline line)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -313,6 +334,13 @@ def main() -> int:
         return 1
     from openmvg_ba import HipBundleAdjuster, SolveConfig, _abi as A
     from openmvg_ba import synthetic
+
+    if args.only_form:
+        fn = {"medium_direct": lambda: bench_medium(HipBundleAdjuster(0)),
+              "batched": lambda: bench_batched(HipBundleAdjuster(0)),
+              "differentiation": lambda: bench_differentiation(20)}[args.only_form]
+        print(json.dumps({args.only_form: fn()}))
+        return 0
 
     if args.copy_calibration > 0:
         n = args.copy_calibration // 4
@@ -365,7 +393,7 @@ def main() -> int:
     avg_s = float(prof.report.profile_seconds) / max(launches, 1)
     counts = lowered_counts(prob)
     if args.roofline_kernel == "pcg_product":
-        counts["slots"] = schur_slots(prob)
+        counts.update(schur_counts(prob))
     per_launch = algorithmic_bytes(args.roofline_kernel, counts, D)
     achieved = per_launch / avg_s / 1e9
     log(f"roofline {args.roofline_kernel}: {launches} launches, avg {avg_s * 1e6:.1f} us, "
