@@ -265,7 +265,7 @@ class Engine {
             HIP_OK(hipMemsetAsync(pvcam_.p, 0, pvcam_.n * sizeof(double), s_));
             setup_explicit();
             const size_t qparts = std::max<size_t>((B_.n_blocks + PCG_Q_WG / 64 - 1) / (PCG_Q_WG / 64),
-                                                   (9 * nc + 3) / 4);  // implicit or explicit product
+                                                   nc);  // implicit or explicit product
             ppart_.alloc(std::max<size_t>(1, qparts) +
                          2 * std::max<size_t>(1, (B_.n_blocks + PCG_BLOCK_WG - 1) / PCG_BLOCK_WG));
             px_.alloc(D);
@@ -345,6 +345,15 @@ class Engine {
         sterm_off_.upload(SL.term_off, s_);
         sterm_.upload(SL.term, s_);
         npairs_ = (uint32_t)(SL.pair_cam.size() / 2);
+        smax_slots_ = 0;
+        for (size_t c = 0; c < nc; ++c) smax_slots_ = std::max(smax_slots_, SL.nbr_off[c + 1] - SL.nbr_off[c]);
+        {
+            std::vector<uint32_t> grp;
+            std::vector<uint8_t> grn;
+            build_pair_groups(SL.pair_cam, (uint32_t)nc, grp, grn);
+            sgroup_.upload(grp, s_);
+            sgroup_n_.upload(grn, s_);
+        }
         zrec_.alloc((size_t)ZREC * std::max<size_t>(1, no));
         svals_.alloc(SL.n_values);
         pvexp_.alloc(9 * std::max<size_t>(1, nc));
@@ -356,7 +365,8 @@ class Engine {
         return explicit_ ? SchurRows<T>{svals_.p, srow_base_.p, snbr_off_.p, snbr_.p, sdiag_slot_.p} : SchurRows<T>{};
     }
     SchurPairs schur_pairs() {
-        return SchurPairs{npairs_, spair_cam_.p, spair_slot_.p, sterm_off_.p, sterm_.p};
+        return SchurPairs{npairs_, spair_cam_.p, spair_slot_.p, sterm_off_.p, sterm_.p, (uint32_t)sgroup_.n, sgroup_.p,
+                          sgroup_n_.p};
     }
 
     // Point chunks (at most NT observations of whole points, or one larger point; at most NT
@@ -609,7 +619,7 @@ class Engine {
         auto enqueue_batch = [&] {
             for (int k = 0; k < batch; ++k) {
                 if (explicit_) {
-                    timed(BA_PROFILE_PCG_PRODUCT, [&] { launch_pcg_product<T>(s_, tp_, SR, V); });
+                    timed(BA_PROFILE_PCG_PRODUCT, [&] { launch_pcg_product<T>(s_, tp_, SR, V, smax_slots_); });
                     launch_pcg_step_explicit<T>(s_, tp_, V);
                 } else {
                     timed(BA_PROFILE_PCG_POINTS, [&] { launch_pcg_points<T>(s_, tp_, lin(), pe(), V); });
@@ -761,10 +771,11 @@ class Engine {
     bool fast_cam_ = false;
     int schur_mode_ = BA_SCHUR_AUTO;
     bool explicit_ = false;  // PCG product with the assembled S
-    uint32_t npairs_ = 0;
+    uint32_t npairs_ = 0, smax_slots_ = 0;
     Buf<T> zrec_, svals_;
     Buf<uint64_t> srow_base_;
-    Buf<uint32_t> snbr_off_, snbr_, sdiag_slot_, spair_cam_, spair_slot_, sterm_off_, sterm_;
+    Buf<uint32_t> snbr_off_, snbr_, sdiag_slot_, spair_cam_, spair_slot_, sterm_off_, sterm_, sgroup_;
+    Buf<uint8_t> sgroup_n_;
     Buf<double> pvexp_;
     bool pose_rows_ = true, keep_pose_rows_ = false;
     int profile_ = BA_PROFILE_NONE;

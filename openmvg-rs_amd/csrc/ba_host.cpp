@@ -12,6 +12,7 @@
 //   kernel/multi_view/sparse/workspace/topology.rs:351-397   ReducedLayout::try_radial
 //   kernel/multi_view/sparse/workspace/pattern.rs:306-468    point parameters, block pattern
 #include "ba_host.hpp"
+#include "ba_kernels.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -613,6 +614,32 @@ PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all) {
     return out;
 }
 
+void build_pair_groups(const std::vector<uint32_t>& pair_cam, uint32_t nc, std::vector<uint32_t>& group,
+                       std::vector<uint8_t>& group_n) {
+    constexpr uint32_t NX = 8;
+    const size_t np = pair_cam.size() / 2;
+    std::vector<std::vector<uint32_t>> qs(NX), qn(NX);  // per residue: group starts and counts
+    size_t k = 0;
+    for (uint32_t c = 0; c < nc; ++c) {
+        size_t e = k;
+        while (e < np && pair_cam[2 * e] == c) ++e;
+        for (size_t g = k; g < e; g += SP_GROUP) {
+            qs[c % NX].push_back((uint32_t)g);
+            qn[c % NX].push_back((uint32_t)std::min<size_t>(SP_GROUP, e - g));
+        }
+        k = e;
+    }
+    size_t depth = 0;
+    for (auto& q : qs) depth = std::max(depth, q.size());
+    group.assign(NX * depth, 0);
+    group_n.assign(NX * depth, 0);
+    for (uint32_t x = 0; x < NX; ++x)
+        for (size_t i = 0; i < qs[x].size(); ++i) {
+            group[NX * i + x] = qs[x][i];
+            group_n[NX * i + x] = (uint8_t)qn[x][i];
+        }
+}
+
 uint64_t count_schur_terms(const Lowered& L) {
     uint64_t n = 0;
     for (size_t p = 0; p < L.n_points(); ++p) {
@@ -703,12 +730,15 @@ SchurLists build_schur_lists(const Lowered& L, const std::vector<uint32_t>& pos)
     S.nbr_off.assign(nc + 1, 0);
     S.diag_slot.resize(nc);
     S.row_base.resize(nc);
+    uint64_t base = 0;
     for (size_t c = 0; c < nc; ++c) {
         S.diag_slot[c] = (uint32_t)rev[c].size();
-        S.nbr_off[c + 1] = S.nbr_off[c] + (uint32_t)(rev[c].size() + 1 + js[c].size());
-        S.row_base[c] = 81ull * S.nbr_off[c];
+        const uint32_t nn = (uint32_t)(rev[c].size() + 1 + js[c].size());
+        S.nbr_off[c + 1] = S.nbr_off[c] + nn;
+        S.row_base[c] = base;
+        base += 9ull * schur_row_len(nn);
     }
-    S.n_values = 81ull * S.nbr_off[nc];
+    S.n_values = base;
     S.nbr.resize(S.nbr_off[nc]);
     for (size_t c = 0; c < nc; ++c) {
         uint32_t* out = S.nbr.data() + S.nbr_off[c];

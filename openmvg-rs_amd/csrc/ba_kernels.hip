@@ -1045,7 +1045,7 @@ __global__ void __launch_bounds__(64) k_camera_assembly_out(Topo tp, const uint3
                 else if (x >= 6 && y >= 6) val = tot[48 + (x >= y ? tri(x - 6, y - 6) : tri(y - 6, x - 6))];
                 else val = x < 6 ? tot[54 + 3 * x + (y - 6)] : tot[54 + 3 * y + (x - 6)];
             }
-            row[(size_t)x * 9 * nn + y] = (T)val;
+            row[(size_t)x * schur_row_len(nn) + y] = (T)val;
         }
     }
 }
@@ -2141,17 +2141,20 @@ struct PairMfma<float> {
 // Off-diagonal blocks, one wavefront per camera pair (c, j): S_cj = -sum_t Z_o(t)^T Z_o'(t) as one
 // 16x16x4 matrix-core step per term (lane l supplies Z_o[l/16][l%16] and Z_o'[l/16][l%16]; rows
 // and columns 9..15 and k = 3 are zero; terms past the end add zero).  Terms go in batches of
-// SP_DEPTH whose loads are issued one batch ahead.  Pairs go in (c, j) order over the whole chip,
-// so at any time every XCD gathers from the same band of cameras' records, which the shared
-// last-level cache then holds (dealing each XCD a contiguous pair range measured 1.5x slower).
+// SP_DEPTH whose loads are issued one batch ahead.  Workgroup slot b holds up to SP_GROUP pairs of
+// one camera row (P.group): rows sweep in ascending order over the whole chip, so at any time
+// every XCD gathers from the same band of cameras' records, which the shared last-level cache
+// holds (dealing each XCD a contiguous pair range measured 1.5x slower), while each row stays on
+// one XCD, whose L2 then serves the row's own records to all of its pairs.
 // The block goes to row c and, transposed, to row j.
-constexpr int SP_WAVES = 4;
+constexpr int SP_WAVES = (int)SP_GROUP;
 constexpr int SP_DEPTH = 4;
 template <class T>
 __global__ void __launch_bounds__(64 * SP_WAVES) k_schur_pairs(SchurPairs P, const T* __restrict__ z, SchurRows<T> S) {
     typedef typename PairMfma<T>::v4 v4;
-    const uint32_t k = blockIdx.x * SP_WAVES + (threadIdx.x >> 6);
-    if (k >= P.npairs) return;  // whole wavefronts leave; no barrier follows
+    const uint32_t w = threadIdx.x >> 6;
+    if (w >= P.group_n[blockIdx.x]) return;  // whole wavefronts leave; no barrier follows
+    const uint32_t k = P.group[blockIdx.x] + w;
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const bool in = li < 9 && lk < 3;
     const uint32_t off = lk * 9 + li;
@@ -2188,60 +2191,104 @@ __global__ void __launch_bounds__(64 * SP_WAVES) k_schur_pairs(SchurPairs P, con
         const int a = PairMfma<T>::row(lk, v), b = li;
         if (a < 9 && b < 9) {
             const T val = -acc[v];
-            rc[(size_t)a * 9 * nnc + b] = val;
-            rj[(size_t)b * 9 * nnj + a] = val;
+            rc[(size_t)a * schur_row_len(nnc) + b] = val;
+            rj[(size_t)b * schur_row_len(nnj) + a] = val;
         }
     }
 }
 
-// Explicit product, one wavefront per camera row (c, a): q_a = sum over the row's neighbour slots
-// s and columns b of S[c][a][s][b] v_nbr(s),b (lane-strided, then the wavefront butterfly), plus
-// lambda diag v on the row's parameter; one partial of v.q per workgroup of four rows, in order.
-constexpr int PP_WAVES = 16;
+// Explicit product, one workgroup per camera c and one wavefront per row slice a: the inputs of the
+// camera's neighbours, vin[9 s + b] = v_nbr(s),b, are gathered once into LDS (the row slices share
+// them), then each wavefront streams its slice in 16-byte groups (four in flight per lane) against
+// vin and the butterfly gives q_a = sum_e S[c][a][e] vin[e]; lambda diag v is added on the row's
+// parameter.  One partial of v.q per camera (its rows in order).  PP_MAX_SLOTS bounds the LDS
+// table; rows with more neighbours use k_pcg_product_gather, which reads the inputs per element.
+constexpr int PP_ROWS = 9;
+constexpr uint32_t PP_MAX_SLOTS = 800;  // LDS table of at most 9 x 800 x 8 B = 57.6 KB, sized per launch
 template <class T>
-__global__ void __launch_bounds__(64 * PP_WAVES) k_pcg_product(Topo tp, SchurRows<T> S, PcgVecs V) {
-    __shared__ double sh[PP_WAVES];
+__device__ __forceinline__ void product_row(const Topo& tp, const SchurRows<T>& S, const PcgVecs& V, uint32_t c,
+                                            uint32_t a, uint32_t lane, const double* vin, double* vq_out) {
+    const uint32_t pd = tp.cam_dim[c], g = tp.cam_group[c], gd = tp.grp_dim[g];
+    const bool live = a < 6 ? a < pd : a - 6 < gd;
+    *vq_out = 0.0;
+    if (!live) return;  // uniform over the wavefront
+    const uint32_t nb = S.nbr_off[c], nn = S.nbr_off[c + 1] - nb;
+    const T* __restrict__ sv = S.vals + S.row_base[c] + (size_t)a * schur_row_len(nn);
+    const uint32_t len = 9 * nn;
+    using G = typename Vec<T>::V;
+    constexpr uint32_t VN = Vec<T>::N, STEP = 64 * VN;
+    double sacc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t e0 = VN * lane; e0 < len; e0 += 4 * STEP) {
+        G m[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t e = e0 + u * STEP;
+            m[u] = e < len ? *reinterpret_cast<const G*>(sv + e) : G{};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (uint32_t i = 0; i < VN; ++i) {
+                const uint32_t e = e0 + u * STEP + i;
+                if (e < len) sacc[u] += (double)Vec<T>::at(m[u], i) * vin[e];
+            }
+    }
+    const double qa = wave_sum((sacc[0] + sacc[1]) + (sacc[2] + sacc[3]));
+    const uint32_t par = a < 6 ? tp.cam_off[c] + a : tp.grp_off[g] + (a - 6);
+    const double va = V.vexp[9 * (size_t)c + a];
+    const double qv = qa + V.lambda * V.diag[par] * va;
+    if (lane == 0) V.q[par] = qv;
+    *vq_out = va * qv;
+}
+template <class T>
+__global__ void __launch_bounds__(64 * PP_ROWS) k_pcg_product(Topo tp, SchurRows<T> S, PcgVecs V) {
+    extern __shared__ double vin[];  // 9 x (the largest neighbour count) values
+    __shared__ double sh[PP_ROWS];
     const PcgState st = *V.st;
     if (st.done) return;
-    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t row = blockIdx.x * PP_WAVES + wv;
-    double vq = 0.0;
-    if (row < 9 * tp.nc) {  // uniform over the wavefront
-        const uint32_t c = row / 9, a = row % 9;
-        const uint32_t pd = tp.cam_dim[c], g = tp.cam_group[c], gd = tp.grp_dim[g];
-        const bool live = a < 6 ? a < pd : a - 6 < gd;
-        if (live) {
-            const uint32_t nb = S.nbr_off[c], nn = S.nbr_off[c + 1] - nb;
-            const T* __restrict__ sv = S.vals + S.row_base[c] + (size_t)a * 9 * nn;
-            const uint32_t len = 9 * nn;
-            double sacc[4] = {0.0, 0.0, 0.0, 0.0};
-            uint32_t e = lane;
-            for (; e + 192 < len; e += 256) {  // four elements in flight per lane
-                T m[4];
-                uint32_t jj[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    m[u] = sv[e + 64 * u];
-                    jj[u] = S.nbr[nb + (e + 64 * u) / 9];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) sacc[u] += (double)m[u] * V.vexp[9 * (size_t)jj[u] + (e + 64 * u) % 9];
-            }
-            for (; e < len; e += 64) sacc[0] += (double)sv[e] * V.vexp[9 * (size_t)S.nbr[nb + e / 9] + e % 9];
-            const double qa = wave_sum((sacc[0] + sacc[1]) + (sacc[2] + sacc[3]));
-            const uint32_t par = a < 6 ? tp.cam_off[c] + a : tp.grp_off[g] + (a - 6);
-            const double va = V.vexp[9 * (size_t)c + a];
-            const double qv = qa + V.lambda * V.diag[par] * va;
-            if (lane == 0) V.q[par] = qv;
-            vq = va * qv;
-        }
-    }
-    if (lane == 0) sh[wv] = vq;
+    const uint32_t c = blockIdx.x, lane = threadIdx.x & 63, a = threadIdx.x >> 6;
+    const uint32_t nb = S.nbr_off[c], len = 9 * (S.nbr_off[c + 1] - nb);
+    for (uint32_t e = threadIdx.x; e < len; e += 64 * PP_ROWS) vin[e] = V.vexp[9 * (size_t)S.nbr[nb + e / 9] + e % 9];
+    __syncthreads();
+    double vq;
+    product_row<T>(tp, S, V, c, a, lane, vin, &vq);
+    if (lane == 0) sh[a] = vq;
     __syncthreads();
     if (threadIdx.x == 0) {
         double acc = 0.0;
-        for (int i = 0; i < PP_WAVES; ++i) acc += sh[i];
-        V.part[blockIdx.x] = acc;
+        for (int i = 0; i < PP_ROWS; ++i) acc += sh[i];
+        V.part[c] = acc;
+    }
+}
+// The same product for cameras with more than PP_MAX_SLOTS neighbours: the inputs are read from
+// global memory per element (one wavefront per row slice, as above).
+template <class T>
+__global__ void __launch_bounds__(64 * PP_ROWS) k_pcg_product_gather(Topo tp, SchurRows<T> S, PcgVecs V) {
+    __shared__ double sh[PP_ROWS];
+    const PcgState st = *V.st;
+    if (st.done) return;
+    const uint32_t c = blockIdx.x, lane = threadIdx.x & 63, a = threadIdx.x >> 6;
+    const uint32_t pd = tp.cam_dim[c], g = tp.cam_group[c], gd = tp.grp_dim[g];
+    const bool live = a < 6 ? a < pd : a - 6 < gd;
+    double vq = 0.0;
+    if (live) {
+        const uint32_t nb = S.nbr_off[c], nn = S.nbr_off[c + 1] - nb;
+        const T* __restrict__ sv = S.vals + S.row_base[c] + (size_t)a * schur_row_len(nn);
+        double s0 = 0.0;
+        for (uint32_t e = lane; e < 9 * nn; e += 64) s0 += (double)sv[e] * V.vexp[9 * (size_t)S.nbr[nb + e / 9] + e % 9];
+        const double qa = wave_sum(s0);
+        const uint32_t par = a < 6 ? tp.cam_off[c] + a : tp.grp_off[g] + (a - 6);
+        const double va = V.vexp[9 * (size_t)c + a];
+        const double qv = qa + V.lambda * V.diag[par] * va;
+        if (lane == 0) V.q[par] = qv;
+        vq = va * qv;
+    }
+    if (lane == 0) sh[a] = vq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int i = 0; i < PP_ROWS; ++i) acc += sh[i];
+        V.part[c] = acc;
     }
 }
 
@@ -2395,10 +2442,15 @@ void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const Po
         hipLaunchKernelGGL((k_pcg_points<T, false>), dim3(g), dim3(PT_NT), 0, s, tp, lin, pe, V);
     }
 }
-uint32_t pcg_product_partials(const Topo& tp) { return grid_for(9 * (size_t)tp.nc, PP_WAVES); }
+uint32_t pcg_product_partials(const Topo& tp) { return tp.nc; }
 template <class T>
-void launch_pcg_product(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V) {
-    if (tp.nc) hipLaunchKernelGGL(k_pcg_product<T>, dim3(pcg_product_partials(tp)), dim3(64 * PP_WAVES), 0, s, tp, S, V);
+void launch_pcg_product(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V, uint32_t max_slots) {
+    if (tp.nc == 0) return;
+    if (max_slots <= PP_MAX_SLOTS)
+        hipLaunchKernelGGL(k_pcg_product<T>, dim3(tp.nc), dim3(64 * PP_ROWS), 9 * max_slots * sizeof(double), s, tp, S,
+                           V);
+    else
+        hipLaunchKernelGGL(k_pcg_product_gather<T>, dim3(tp.nc), dim3(64 * PP_ROWS), 0, s, tp, S, V);
 }
 template <class T>
 void launch_pcg_step_explicit(hipStream_t s, const Topo& tp, const PcgVecs& V) {
@@ -2408,8 +2460,8 @@ void launch_pcg_step_explicit(hipStream_t s, const Topo& tp, const PcgVecs& V) {
 }
 template <class T>
 void launch_schur_pairs(hipStream_t s, const SchurPairs& P, const T* z, SchurRows<T> S) {
-    if (P.npairs)
-        hipLaunchKernelGGL(k_schur_pairs<T>, dim3(grid_for(P.npairs, SP_WAVES)), dim3(64 * SP_WAVES), 0, s, P, z, S);
+    if (P.npairs && P.ngroups)
+        hipLaunchKernelGGL(k_schur_pairs<T>, dim3(P.ngroups), dim3(64 * SP_WAVES), 0, s, P, z, S);
 }
 template <class T>
 void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const PcgVecs& V) {
@@ -2444,7 +2496,7 @@ template void launch_gauge<double>(hipStream_t, const double*, int, double, Gaug
                                             const uint64_t*, double*, double*, double*, double*, double*, T*,      \
                                             SchurRows<T>);                                                         \
     template void launch_schur_pairs<T>(hipStream_t, const SchurPairs&, const T*, SchurRows<T>);                    \
-    template void launch_pcg_product<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&);              \
+    template void launch_pcg_product<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&, uint32_t);              \
     template void launch_pcg_step_explicit<T>(hipStream_t, const Topo&, const PcgVecs&);                            \
     template void launch_pairs<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>, uint32_t,               \
                                   const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,              \

@@ -137,7 +137,10 @@ void launch_reduce_points(hipStream_t s, const double* pscal, uint32_t np, doubl
 // S = U - W V^-1 W^T in camera blocks of 9 rows (pose 0..5, the camera's intrinsics 6..8; rows
 // and columns beyond the free dimensions are zero).  Row-slice layout: row a of camera c holds 9
 // values per neighbour slot s (neighbour camera nbr[nbr_off[c] + s], ascending, c itself
-// included): vals[row_base[c] + a * 9 * nn_c + 9 * s + b], nn_c = nbr_off[c + 1] - nbr_off[c].
+// included): vals[row_base[c] + a * schur_row_len(nn_c) + 9 * s + b], nn_c = nbr_off[c + 1] -
+// nbr_off[c]; each row slice is padded to a multiple of 4 values (16-byte groups in both precisions;
+// the padding is never read as data).
+__host__ __device__ inline uint32_t schur_row_len(uint32_t nn) { return (9 * nn + 3) & ~3u; }
 template <class T>
 struct SchurRows {
     T* vals;
@@ -154,7 +157,18 @@ struct SchurPairs {
     const uint32_t* slot;      // 2 per pair: slot of j in row c, slot of c in row j
     const uint32_t* term_off;  // npairs + 1
     const uint32_t* term;      // 2 per term
+    uint32_t ngroups;
+    const uint32_t* group;     // per workgroup: first pair (SP_GROUP pairs of one camera row at most)
+    const uint8_t* group_n;    // per workgroup: its pair count (0: an empty slot)
 };
+// Pairs per workgroup of the pair assembly.
+constexpr uint32_t SP_GROUP = 4;
+// Workgroup slots of the pair assembly: camera row c's pairs go, in groups of SP_GROUP, to slots
+// b = 8 i + (c mod 8), rows in ascending order within each residue, so that with workgroups dealt
+// round-robin to the eight XCDs a row stays on one XCD's L2 while all XCDs sweep the same band of
+// rows.
+void build_pair_groups(const std::vector<uint32_t>& pair_cam, uint32_t nc, std::vector<uint32_t>& group,
+                       std::vector<uint8_t>& group_n);
 // Per camera-major observation: Z = M^T J~_X^T J~_c (3 x 9, row-major, padded to ZREC), where
 // M M^T = V^-1 of the observation's point; Z_o^T Z_o' = W_o V^-1 W_o'^T.
 constexpr int ZREC = 28;
@@ -290,10 +304,11 @@ void launch_pcg_step(hipStream_t s, const Topo& tp, const PcgVecs& V);
 // Explicit product q = S v + lambda diag v (v = d or x by the PCG mode, from V.vexp) with the
 // partials of v.q, then the step and direction updates.
 template <class T>
-void launch_pcg_product(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V);
+void launch_pcg_product(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V, uint32_t max_slots);
 template <class T>
 void launch_pcg_step_explicit(hipStream_t s, const Topo& tp, const PcgVecs& V);
-// Partials of the explicit product (one per workgroup of four camera rows).
+// Partials of the explicit product (one per camera).  max_slots: the largest neighbour count of a
+// camera row (rows up to 800 neighbours keep their inputs in LDS).
 uint32_t pcg_product_partials(const Topo& tp);
 // Camera-major linearisation: the SoA camera-major Jacobian copy (lin.jcm).
 template <class T>
