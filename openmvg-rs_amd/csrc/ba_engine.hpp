@@ -558,13 +558,13 @@ class Engine {
             launch_scatter_dense(s_, tp, (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p, value_off_.p, u_vals_.p,
                                  s_vals_.p, S_.p, tp.D);
             launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, S_.p, tp.D, nullptr, value_off_.p, u_vals_.p,
-                            s_vals_.p, asm_.p);
+                            partials_.p, asm_.p);
         } else if (reduced_system) {  // lambda diag on the diagonal blocks of the preconditioner
             launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, nullptr, tp.D, diag_pair_.p, value_off_.p,
-                            u_vals_.p, s_vals_.p, asm_.p);
+                            u_vals_.p, partials_.p, asm_.p);
         } else {
             launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, nullptr, tp.D, nullptr, value_off_.p, u_vals_.p,
-                            s_vals_.p, asm_.p);
+                            partials_.p, asm_.p);
         }
         HIP_OK(hipMemcpyAsync(h_asm_.p, asm_.p, sizeof(AsmScalars), hipMemcpyDeviceToHost, s_));
         HIP_OK(hipStreamSynchronize(s_));

@@ -14,6 +14,8 @@
 // Every reduction runs over a fixed partition in a fixed order; no floating-point atomics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ba_kernels.hpp"
 #include "ba_math.hpp"
 
@@ -1128,55 +1130,82 @@ __global__ void __launch_bounds__(NT) k_scatter_dense(Topo tp, uint32_t npairs, 
 // k_finalize (single workgroup): damping-diagonal clamp per block (schur/assembly.rs:729-766),
 // lambda * diag on the reduced diagonal, scaled gradient norm (:1098-1112), finiteness of g.
 // ----------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_finalize(Topo tp, double lambda, double* diag, const double* full_grad,
-                                                   const double* grad, double* S, uint32_t ld,
-                                                   const int32_t* diag_pair, const uint64_t* value_off,
-                                                   double* u_vals, AsmScalars* sc) {
-    __shared__ double sh[1024];
-    const int t = threadIdx.x;
-    // min/max of the unclamped diagonal
-    double mn = INFINITY, mx = 0.0;
-    for (uint32_t i = t; i < tp.D; i += 1024) {
-        mn = fmin(mn, diag[i]);
-        mx = fmax(mx, diag[i]);
-    }
-    mn = tree_min<1024>(mn, sh);
-    mx = tree_max<1024>(mx, sh);
-    __syncthreads();
-    double clamped = 0.0;
-    for (uint32_t A = t; A < tp.nblocks; A += 1024) {
+// Lane per block (grid-stride over a fixed grid): the block's unclamped diagonal enters min/max, is
+// clamped to 64 eps of the block's largest entry (unit for an empty block scale), lambda diag
+// goes onto S or the diagonal pair's U values, and the scaled gradient max and the non-finite
+// gradient count are formed on the clamped diagonal.  Per workgroup: [min, max, clamped, sg,
+// nonfinite]; min/max are order-free and the counts exact, so the result is deterministic.
+constexpr int FIN_WG = 256;
+constexpr int FIN_GRID_MAX = 512;
+__global__ void __launch_bounds__(FIN_WG) k_finalize_blocks(Topo tp, double lambda, double* diag,
+                                                            const double* full_grad, const double* grad, double* S,
+                                                            uint32_t ld, const int32_t* diag_pair,
+                                                            const uint64_t* value_off, double* u_vals, double* part) {
+    __shared__ double sh[FIN_WG];
+    double mn = INFINITY, mx = 0.0, clamped = 0.0, sg = 0.0, nonfinite = 0.0;
+    for (uint32_t A = blockIdx.x * FIN_WG + threadIdx.x; A < tp.nblocks; A += gridDim.x * FIN_WG) {
         const uint32_t n = tp.blk_dim[A], off = tp.blk_off[A];
         if (n == 0) continue;
-        double scale = 0.0;
-        for (uint32_t k = 0; k < n; ++k) scale = fmax(scale, diag[off + k]);
-        double floor = scale > 0.0 ? 64.0 * DBL_EPSILON * scale : 1.0;
+        double d[9], scale = 0.0;
+        for (uint32_t k = 0; k < n; ++k) {
+            d[k] = diag[off + k];
+            mn = fmin(mn, d[k]);
+            mx = fmax(mx, d[k]);
+            scale = fmax(scale, d[k]);
+        }
+        const double floor = scale > 0.0 ? 64.0 * DBL_EPSILON * scale : 1.0;
         for (uint32_t k = 0; k < n; ++k)
-            if (diag[off + k] < floor) {
+            if (d[k] < floor) {
+                d[k] = floor;
                 diag[off + k] = floor;
                 clamped += 1.0;
             }
         if (S) {
-            for (uint32_t k = 0; k < n; ++k) S[(size_t)(off + k) * ld + off + k] += lambda * diag[off + k];
+            for (uint32_t k = 0; k < n; ++k) S[(size_t)(off + k) * ld + off + k] += lambda * d[k];
         } else if (diag_pair) {
-            int32_t pr = diag_pair[A];
+            const int32_t pr = diag_pair[A];
             if (pr >= 0)
-                for (uint32_t k = 0; k < n; ++k) u_vals[value_off[pr] + k * n + k] += lambda * diag[off + k];
+                for (uint32_t k = 0; k < n; ++k) u_vals[value_off[pr] + k * n + k] += lambda * d[k];
+        }
+        for (uint32_t k = 0; k < n; ++k) {
+            const double gv = full_grad[off + k];
+            const double v = (isfinite(d[k]) && d[k] > 0.0) ? fabs(gv) / sqrt(d[k]) : (gv == 0.0 ? 0.0 : INFINITY);
+            sg = fmax(sg, v);
+            if (!isfinite(grad[off + k])) nonfinite += 1.0;
         }
     }
-    clamped = tree_sum<1024>(clamped, sh);
-    double sg = 0.0, nonfinite = 0.0;
-    for (uint32_t i = t; i < tp.D; i += 1024) {
-        double cv = diag[i], gv = full_grad[i], v;
-        if (isfinite(cv) && cv > 0.0)
-            v = fabs(gv) / sqrt(cv);
-        else
-            v = gv == 0.0 ? 0.0 : INFINITY;
-        sg = fmax(sg, v);
-        if (!isfinite(grad[i])) nonfinite += 1.0;
+    mn = tree_min<FIN_WG>(mn, sh);
+    mx = tree_max<FIN_WG>(mx, sh);
+    clamped = tree_sum<FIN_WG>(clamped, sh);
+    sg = tree_max<FIN_WG>(sg, sh);
+    nonfinite = tree_sum<FIN_WG>(nonfinite, sh);
+    if (threadIdx.x == 0) {
+        double* o = part + 5 * (size_t)blockIdx.x;
+        o[0] = mn;
+        o[1] = mx;
+        o[2] = clamped;
+        o[3] = sg;
+        o[4] = nonfinite;
     }
-    sg = tree_max<1024>(sg, sh);
-    nonfinite = tree_sum<1024>(nonfinite, sh);
-    if (t == 0) {
+}
+__global__ void __launch_bounds__(FIN_WG) k_finalize_reduce(Topo tp, uint32_t nparts, const double* part,
+                                                            AsmScalars* sc) {
+    __shared__ double sh[FIN_WG];
+    double mn = INFINITY, mx = 0.0, clamped = 0.0, sg = 0.0, nonfinite = 0.0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += FIN_WG) {
+        const double* o = part + 5 * (size_t)i;
+        mn = fmin(mn, o[0]);
+        mx = fmax(mx, o[1]);
+        clamped += o[2];
+        sg = fmax(sg, o[3]);
+        nonfinite += o[4];
+    }
+    mn = tree_min<FIN_WG>(mn, sh);
+    mx = tree_max<FIN_WG>(mx, sh);
+    clamped = tree_sum<FIN_WG>(clamped, sh);
+    sg = tree_max<FIN_WG>(sg, sh);
+    nonfinite = tree_sum<FIN_WG>(nonfinite, sh);
+    if (threadIdx.x == 0) {
         sc->sgin_prepared = sg;
         sc->min_diag = tp.D == 0 ? 0.0 : mn;
         sc->max_diag = mx;
@@ -2362,9 +2391,11 @@ void launch_scatter_dense(hipStream_t s, const Topo& tp, uint32_t npairs, const 
 }
 void launch_finalize(hipStream_t s, const Topo& tp, double lambda, double* diag, const double* full_grad,
                      const double* grad, double* S, uint32_t ld, const int32_t* diag_pair, const uint64_t* value_off,
-                     double* u_vals, double* /*s_vals*/, AsmScalars* sc) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, tp, lambda, diag, full_grad, grad, S, ld, diag_pair,
-                       value_off, u_vals, sc);
+                     double* u_vals, double* partials, AsmScalars* sc) {
+    const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(grid_for(tp.nblocks, FIN_WG), FIN_GRID_MAX));
+    hipLaunchKernelGGL(k_finalize_blocks, dim3(g), dim3(FIN_WG), 0, s, tp, lambda, diag, full_grad, grad, S, ld,
+                       diag_pair, value_off, u_vals, partials);
+    hipLaunchKernelGGL(k_finalize_reduce, dim3(1), dim3(FIN_WG), 0, s, tp, g, (const double*)partials, sc);
 }
 void launch_retract(hipStream_t s, const Topo& tp, const double* R, const double* t, const double* gp,
                     const double* step, const GaugeDev* gauge, double* Rc, double* tc, double* gpc) {

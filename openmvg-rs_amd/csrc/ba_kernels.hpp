@@ -208,7 +208,7 @@ void launch_scatter_dense(hipStream_t s, const Topo& tp, uint32_t npairs, const 
 // S != null, else the diagonal pairs' U and Schur-block values), and finish the scaled gradient.
 void launch_finalize(hipStream_t s, const Topo& tp, double lambda, double* diag, const double* full_grad,
                      const double* grad, double* S, uint32_t ld, const int32_t* diag_pair,
-                     const uint64_t* value_off, double* u_vals, double* s_vals, AsmScalars* sc);
+                     const uint64_t* value_off, double* u_vals, double* partials, AsmScalars* sc);
 
 // Candidate pass (kernel (a')): back-substitution and candidate rule per point; R, t, gp the
 // current state, Rc, tc, gpc the candidate cameras; srec is scratch of PCG_CAM_REC doubles per
