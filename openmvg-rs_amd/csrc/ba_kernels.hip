@@ -2181,34 +2181,46 @@ constexpr int SP_DEPTH = 4;
 template <class T>
 __global__ void __launch_bounds__(64 * SP_WAVES) k_schur_pairs(SchurPairs P, const T* __restrict__ z, SchurRows<T> S) {
     typedef typename PairMfma<T>::v4 v4;
-    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar loads below
     if (w >= P.group_n[blockIdx.x]) return;  // whole wavefronts leave; no barrier follows
     const uint32_t k = P.group[blockIdx.x] + w;
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const bool in = li < 9 && lk < 3;
-    const uint32_t off = lk * 9 + li;
+    const uint32_t off = in ? lk * 9 + li : 0;  // every lane reads inside the record
     const uint32_t t0 = P.term_off[k], t1 = P.term_off[k + 1];
+    // One batch: the term indices (clamped to the last term, uniform: scalar loads), then all of
+    // the batch's record loads back to back; terms past the end are zeroed after the loads.
     auto load = [&](uint32_t t, T* x, T* y) {
+        uint32_t tc[SP_DEPTH], tj[SP_DEPTH];
 #pragma unroll
         for (int u = 0; u < SP_DEPTH; ++u) {
-            const bool live = in && t + u < t1;
-            const uint32_t tc = live ? P.term[2 * (t + u)] : 0, tj = live ? P.term[2 * (t + u) + 1] : 0;
-            x[u] = live ? z[(size_t)ZREC * tc + off] : T(0);
-            y[u] = live ? z[(size_t)ZREC * tj + off] : T(0);
+            const uint32_t tt = t + u < t1 ? t + u : t1 - 1;
+            tc[u] = P.term[2 * tt];
+            tj[u] = P.term[2 * tt + 1];
         }
+#pragma unroll
+        for (int u = 0; u < SP_DEPTH; ++u) {
+            x[u] = z[(size_t)ZREC * tc[u] + off];
+            y[u] = z[(size_t)ZREC * tj[u] + off];
+        }
+#pragma unroll
+        for (int u = 0; u < SP_DEPTH; ++u)
+            if (!(in && t + u < t1)) x[u] = y[u] = T(0);
     };
     v4 acc = v4{T(0), T(0), T(0), T(0)};
-    T xa[SP_DEPTH], ya[SP_DEPTH];
-    load(t0, xa, ya);
-    for (uint32_t t = t0; t < t1; t += SP_DEPTH) {
-        T xn[SP_DEPTH], yn[SP_DEPTH];
-        load(t + SP_DEPTH, xn, yn);
+    if (t0 < t1) {
+        T xa[SP_DEPTH], ya[SP_DEPTH];
+        load(t0, xa, ya);
+        for (uint32_t t = t0; t < t1; t += SP_DEPTH) {
+            T xn[SP_DEPTH], yn[SP_DEPTH];
+            if (t + SP_DEPTH < t1) load(t + SP_DEPTH, xn, yn);
 #pragma unroll
-        for (int u = 0; u < SP_DEPTH; ++u) acc = PairMfma<T>::mac(xa[u], ya[u], acc);
+            for (int u = 0; u < SP_DEPTH; ++u) acc = PairMfma<T>::mac(xa[u], ya[u], acc);
 #pragma unroll
-        for (int u = 0; u < SP_DEPTH; ++u) {
-            xa[u] = xn[u];
-            ya[u] = yn[u];
+            for (int u = 0; u < SP_DEPTH; ++u) {
+                xa[u] = xn[u];
+                ya[u] = yn[u];
+            }
         }
     }
     const uint32_t c = P.cam[2 * k], j = P.cam[2 * k + 1];
