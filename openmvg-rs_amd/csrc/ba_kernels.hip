@@ -313,6 +313,11 @@ __device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t c, uint32
                                               double jc[18], double q[4], double rx[3], double* cost, double* sq) {
     const uint32_t g = tp.cam_group[c];
     const GaugeDev* gauge = gauges + (tp.cam_win ? tp.cam_win[c] : 0);
+    // everything read after the domain test is loaded here, with the camera and the point, so
+    // that one memory round trip serves the whole observation
+    const uint32_t pd = tp.cam_dim[c], idim = tp.grp_dim[g];
+    const double s = gauge->scene_scale;
+    const int gauge_cam = gauge->camera;
     const double* R = Rg + 9 * (size_t)c;
     double Xp[3] = {X[3 * (size_t)p], X[3 * (size_t)p + 1], X[3 * (size_t)p + 2]};
     double rot[3];
@@ -328,7 +333,6 @@ __device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t c, uint32
     double sw = loss_sqrt_weight<double>(tp.loss, tp.loss_scale, rn);
     *cost = loss_cost<double>(tp.loss, tp.loss_scale, rn);
     *sq = rn * rn;
-    const double s = gauge->scene_scale;
     r[0] = sw * res[0];
     r[1] = sw * res[1];
     // compact state: n, sqrt(w)/z, sqrt(w); and the rotated point
@@ -347,9 +351,8 @@ __device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t c, uint32
         }
     // pose motion: rotation columns -[RX]_x, translation columns (scene scale, or the baseline
     // camera's S^2 tangent basis times the baseline)
-    const uint32_t pd = tp.cam_dim[c];
     double mot[3][6] = {{0.0, rot[2], -rot[1], 0, 0, 0}, {-rot[2], 0.0, rot[0], 0, 0, 0}, {rot[1], -rot[0], 0.0, 0, 0, 0}};
-    if ((int)c == gauge->camera) {
+    if ((int)c == gauge_cam) {
         for (int q = 0; q < 3; ++q) {
             mot[q][3] = gauge->baseline * gauge->basis[q];
             mot[q][4] = gauge->baseline * gauge->basis[3 + q];
@@ -357,7 +360,6 @@ __device__ __forceinline__ bool linearize_obs(const Topo& tp, uint32_t c, uint32
     } else {
         for (int q = 0; q < 3; ++q) mot[q][3 + q] = s;
     }
-    const uint32_t idim = tp.grp_dim[g];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
 #pragma unroll
