@@ -56,9 +56,9 @@ def lowered_counts(problem) -> dict:
     return {"obs": n_obs, "points": n_pts, "cameras": n_cams, "segments": segments}
 
 
-def schur_slots(problem) -> int:
-    """Neighbour slots of the explicit reduced system: per camera, itself and every camera it
-    shares a point with (every point of the synthetic scenes is free)."""
+def schur_pairs(problem) -> int:
+    """Camera pairs c < j of the explicit reduced system: cameras that share a point (every point
+    of the synthetic scenes is free)."""
     order = np.lexsort((problem.obs_pose, problem.obs_point))
     cam = problem.obs_pose[order].astype(np.int64)
     pt = problem.obs_point[order]
@@ -74,16 +74,16 @@ def schur_slots(problem) -> int:
         a, b = np.triu_indices(k, 1)
         lo, hi = np.minimum(c[:, a], c[:, b]), np.maximum(c[:, a], c[:, b])
         keys.append((lo * nc + hi).ravel())
-    pairs = len(np.unique(np.concatenate(keys))) if keys else 0
-    return nc + 2 * pairs
+    return len(np.unique(np.concatenate(keys))) if keys else 0
 
 
 def schur_counts(problem) -> dict:
-    """Camera pairs and observation pairs of the explicit reduced system."""
-    slots = schur_slots(problem)
+    """Stored slots (the diagonal and upper blocks), camera pairs and observation pairs of the
+    explicit reduced system."""
+    pairs = schur_pairs(problem)
     pt = np.sort(problem.obs_point)
     k = np.diff(np.r_[np.flatnonzero(np.r_[True, pt[1:] != pt[:-1]]), len(pt)]).astype(np.int64)
-    return {"slots": slots, "pairs": (slots - int(problem.n_poses)) // 2, "terms": int(np.sum(k * (k - 1) // 2))}
+    return {"slots": int(problem.n_poses) + pairs, "pairs": pairs, "terms": int(np.sum(k * (k - 1) // 2))}
 
 
 def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
@@ -105,13 +105,14 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     linearize (k_linearize): per observation pixel (16 B), camera and point index (8 B), the
       written r~, compact state and J~_X (96 B), cost, squared residual and flag (17 B); per point
       X (24 B); per camera R, t, f, k1, k2 (120 B).
-    pcg_product (k_pcg_product: q = S v on the explicit reduced system, one wavefront per camera
-      row): per neighbour slot the 9 x 9 block (648 B) and, once per row, the neighbour index
-      (9 x 4 B); per camera the 9 input values (72 B), its offsets and dimensions (16 B) and the
-      written q and read damping diagonal of its 9 rows (144 B).
+    pcg_product (k_pcg_product_upper: the first pass of q = S v over the stored diagonal and
+      upper blocks of the explicit reduced system, one workgroup per camera row): per stored slot
+      the 9 x 9 block (648 B) and its neighbour index (4 B); per camera pair its reverse position
+      (4 B) and the written column sums (72 B); per camera the 9 input values (72 B), its offsets
+      (16 B) and the written row sums (72 B).
     schur_pairs (k_schur_pairs: the off-diagonal blocks of the explicit S): each observation's Z
-      record once (27 values, 216 B), 8 B per observation pair, and per camera pair the two
-      written 9 x 9 blocks and its indices (1,316 B).
+      record once (27 values, 216 B), 8 B per observation pair, and per camera pair the written
+      9 x 9 block and its indices (656 B).
     camera_assembly (k_camera_assembly with the explicit system): per observation the
       camera-major rows read (72 B) and point index (4 B) and the Z record written (216 B); per
       point its packed record (128 B); per segment its camera (4 B) and written sums (576 B); per
@@ -119,11 +120,11 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     """
     no, npt, nc = counts["obs"], counts["points"], counts["cameras"]
     if kernel == "schur_pairs":
-        return no * 216 + counts["terms"] * 8 + counts["pairs"] * (2 * 648 + 20)
+        return no * 216 + counts["terms"] * 8 + counts["pairs"] * (648 + 8)
     if kernel == "camera_assembly":
         return no * (72 + 4 + 216) + npt * 128 + counts["segments"] * (4 + 576) + nc * 120
     if kernel == "pcg_product":
-        return counts["slots"] * (648 + 36) + nc * (72 + 16 + 144)
+        return counts["slots"] * (648 + 4) + counts["pairs"] * (4 + 72) + nc * (72 + 16 + 72)
     if kernel == "pcg_points":
         return no * (48 + 24 + 8) + npt * (4 + 1 + 48 + 24 + 32) + nc * 72
     if kernel == "pcg_cameras":

@@ -323,8 +323,8 @@ class Engine {
         const size_t nc = L_.n_cameras(), no = L_.n_obs();
         const uint64_t terms = count_schur_terms(L_);
         const uint64_t pairs = std::min<uint64_t>(terms, (uint64_t)nc * (nc > 0 ? nc - 1 : 0) / 2);
-        const double bytes = (double)sizeof(T) * ((double)ZREC * no + 81.0 * ((double)nc + 2.0 * (double)pairs)) +
-                             8.0 * (double)terms + 16.0 * (double)pairs;
+        const double bytes = (double)sizeof(T) * ((double)ZREC * no + 81.0 * ((double)nc + (double)pairs)) +
+                             8.0 * (double)terms + 88.0 * (double)pairs;
         if (schur_mode_ != BA_SCHUR_EXPLICIT) {
             size_t fr = 0, tot = 0;
             HIP_OK(hipMemGetInfo(&fr, &tot));
@@ -338,15 +338,14 @@ class Engine {
         const SchurLists SL = build_schur_lists(L_, pos);
         snbr_off_.upload(SL.nbr_off, s_);
         snbr_.upload(SL.nbr, s_);
-        sdiag_slot_.upload(SL.diag_slot, s_);
         srow_base_.upload(SL.row_base, s_);
+        spair_off_.upload(SL.pair_off, s_);
+        srev_off_.upload(SL.rev_off, s_);
+        srev_pos_.upload(SL.rev_pos, s_);
         spair_cam_.upload(SL.pair_cam, s_);
-        spair_slot_.upload(SL.pair_slot, s_);
         sterm_off_.upload(SL.term_off, s_);
         sterm_.upload(SL.term, s_);
         npairs_ = (uint32_t)(SL.pair_cam.size() / 2);
-        smax_slots_ = 0;
-        for (size_t c = 0; c < nc; ++c) smax_slots_ = std::max(smax_slots_, SL.nbr_off[c + 1] - SL.nbr_off[c]);
         {
             std::vector<uint32_t> grp;
             std::vector<uint8_t> grn;
@@ -357,16 +356,19 @@ class Engine {
         zrec_.alloc((size_t)ZREC * std::max<size_t>(1, no));
         svals_.alloc(SL.n_values);
         pvexp_.alloc(9 * std::max<size_t>(1, nc));
+        stpair_.alloc(9 * std::max<size_t>(1, npairs_));
+        sqrow_.alloc(9 * std::max<size_t>(1, nc));
         HIP_OK(hipMemsetAsync(pvexp_.p, 0, pvexp_.n * sizeof(double), s_));
         HIP_OK(hipStreamSynchronize(s_));  // the host lists end with this scope
         explicit_ = true;
     }
     SchurRows<T> schur_rows() {
-        return explicit_ ? SchurRows<T>{svals_.p, srow_base_.p, snbr_off_.p, snbr_.p, sdiag_slot_.p} : SchurRows<T>{};
+        return explicit_ ? SchurRows<T>{svals_.p, srow_base_.p, snbr_off_.p, snbr_.p, spair_off_.p,
+                                        srev_off_.p, srev_pos_.p, stpair_.p, sqrow_.p}
+                         : SchurRows<T>{};
     }
     SchurPairs schur_pairs() {
-        return SchurPairs{npairs_, spair_cam_.p, spair_slot_.p, sterm_off_.p, sterm_.p, (uint32_t)sgroup_.n, sgroup_.p,
-                          sgroup_n_.p};
+        return SchurPairs{npairs_, spair_cam_.p, sterm_off_.p, sterm_.p, (uint32_t)sgroup_.n, sgroup_.p, sgroup_n_.p};
     }
 
     // Point chunks (at most NT observations of whole points, or one larger point; at most NT
@@ -619,7 +621,8 @@ class Engine {
         auto enqueue_batch = [&] {
             for (int k = 0; k < batch; ++k) {
                 if (explicit_) {
-                    timed(BA_PROFILE_PCG_PRODUCT, [&] { launch_pcg_product<T>(s_, tp_, SR, V, smax_slots_); });
+                    timed(BA_PROFILE_PCG_PRODUCT, [&] { launch_pcg_product_upper<T>(s_, tp_, SR, V); });
+                    launch_pcg_product_lower<T>(s_, tp_, SR, V);
                     launch_pcg_step_explicit<T>(s_, tp_, V);
                 } else {
                     timed(BA_PROFILE_PCG_POINTS, [&] { launch_pcg_points<T>(s_, tp_, lin(), pe(), V); });
@@ -771,12 +774,12 @@ class Engine {
     bool fast_cam_ = false;
     int schur_mode_ = BA_SCHUR_AUTO;
     bool explicit_ = false;  // PCG product with the assembled S
-    uint32_t npairs_ = 0, smax_slots_ = 0;
+    uint32_t npairs_ = 0;
     Buf<T> zrec_, svals_;
     Buf<uint64_t> srow_base_;
-    Buf<uint32_t> snbr_off_, snbr_, sdiag_slot_, spair_cam_, spair_slot_, sterm_off_, sterm_, sgroup_;
+    Buf<uint32_t> snbr_off_, snbr_, spair_off_, srev_off_, srev_pos_, spair_cam_, sterm_off_, sterm_, sgroup_;
     Buf<uint8_t> sgroup_n_;
-    Buf<double> pvexp_;
+    Buf<double> pvexp_, stpair_, sqrow_;
     bool pose_rows_ = true, keep_pose_rows_ = false;
     int profile_ = BA_PROFILE_NONE;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;

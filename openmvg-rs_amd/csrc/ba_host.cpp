@@ -719,22 +719,15 @@ SchurLists build_schur_lists(const Lowered& L, const std::vector<uint32_t>& pos)
         }
         for (auto& x : th) x.join();
     }
-    // 3. neighbour rows: lower partners (ascending), the camera, upper partners
-    std::vector<std::vector<uint32_t>> rev(nc);
-    std::vector<uint32_t> rev_slot;  // per pair: slot of c in row j
-    for (size_t c = 0; c < nc; ++c)
-        for (uint32_t j : js[c]) {
-            rev_slot.push_back((uint32_t)rev[j].size());
-            rev[j].push_back((uint32_t)c);
-        }
+    // 3. upper rows: the camera, then its partners j > c; pairs numbered in (c, j) order
     S.nbr_off.assign(nc + 1, 0);
-    S.diag_slot.resize(nc);
+    S.pair_off.assign(nc + 1, 0);
     S.row_base.resize(nc);
     uint64_t base = 0;
     for (size_t c = 0; c < nc; ++c) {
-        S.diag_slot[c] = (uint32_t)rev[c].size();
-        const uint32_t nn = (uint32_t)(rev[c].size() + 1 + js[c].size());
+        const uint32_t nn = (uint32_t)(1 + js[c].size());
         S.nbr_off[c + 1] = S.nbr_off[c] + nn;
+        S.pair_off[c + 1] = S.pair_off[c] + (uint32_t)js[c].size();
         S.row_base[c] = base;
         base += 9ull * schur_row_len(nn);
     }
@@ -742,21 +735,27 @@ SchurLists build_schur_lists(const Lowered& L, const std::vector<uint32_t>& pos)
     S.nbr.resize(S.nbr_off[nc]);
     for (size_t c = 0; c < nc; ++c) {
         uint32_t* out = S.nbr.data() + S.nbr_off[c];
-        for (uint32_t i : rev[c]) *out++ = i;
         *out++ = (uint32_t)c;
         for (uint32_t j : js[c]) *out++ = j;
     }
-    // 4. pairs in (c, j) order and their terms
+    // 4. pairs and their terms; per camera j the pairs (c < j, j) in ascending c
+    const uint32_t npairs = S.pair_off[nc];
+    S.rev_off.assign(nc + 1, 0);
+    for (size_t c = 0; c < nc; ++c)
+        for (uint32_t j : js[c]) S.rev_off[j + 1]++;
+    for (size_t c = 0; c < nc; ++c) S.rev_off[c + 1] += S.rev_off[c];
+    S.rev_pos.resize(npairs);
+    std::vector<uint32_t> rcur(S.rev_off.begin(), S.rev_off.end() - 1);
     S.term_off.push_back(0);
     S.term.reserve(2 * ent.size());
-    size_t k = 0;
+    S.pair_cam.reserve(2 * (size_t)npairs);
+    uint32_t k = 0;
     for (size_t c = 0; c < nc; ++c) {
         uint64_t i = boff[c];
         for (size_t m = 0; m < js[c].size(); ++m, ++k) {
             S.pair_cam.push_back((uint32_t)c);
             S.pair_cam.push_back(js[c][m]);
-            S.pair_slot.push_back((uint32_t)(rev[c].size() + 1 + m));
-            S.pair_slot.push_back(rev_slot[k]);
+            S.rev_pos[k] = rcur[js[c][m]]++;
             for (uint32_t t = 0; t < jn[c][m]; ++t, ++i) {
                 S.term.push_back(ent[i].pc);
                 S.term.push_back(ent[i].pj);

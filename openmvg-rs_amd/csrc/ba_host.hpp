@@ -82,15 +82,19 @@ struct PairLists {
 PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all);
 
 // Lists of the explicit reduced camera system, for topologies with one camera per intrinsic
-// group and at most one observation per point and camera.  Row c of S holds its neighbour cameras
-// (those sharing a free point with c, and c itself) ascending; the camera pairs c < j carry the
-// observation pairs (camera-major positions pos[o] of the observations by c and by j) of their
-// common free points, in point order.
+// group and at most one observation per point and camera.  S is symmetric and only its diagonal
+// and upper blocks are stored: row c holds the camera itself (slot 0) and then its partners j > c
+// (cameras sharing a free point with c) ascending.  The camera pairs c < j, numbered in (c, j)
+// order, carry the observation pairs (camera-major positions pos[o] of the observations by c and
+// by j) of their common free points, in point order.  The pairs (c < j, j) of each camera j,
+// in ascending c, form its reverse range rev_off[j]..rev_off[j + 1]; rev_pos[p] is pair p's
+// position in that order.
 struct SchurLists {
-    std::vector<uint32_t> nbr_off, nbr, diag_slot;  // nc + 1, neighbours, nc
-    std::vector<uint64_t> row_base;                 // nc: first value of row c (81 per slot)
-    std::vector<uint32_t> pair_cam, pair_slot;      // 2 per pair: (c, j), (slot of j in c, of c in j)
-    std::vector<uint32_t> term_off, term;           // n_pairs + 1, 2 per term
+    std::vector<uint32_t> nbr_off, nbr;        // nc + 1, upper neighbours (the camera first)
+    std::vector<uint64_t> row_base;            // nc: first value of row c
+    std::vector<uint32_t> pair_off, pair_cam;  // nc + 1 (pairs of row c), 2 per pair: (c, j)
+    std::vector<uint32_t> rev_off, rev_pos;    // nc + 1, per pair
+    std::vector<uint32_t> term_off, term;      // n_pairs + 1, 2 per term
     uint64_t n_values = 0;
 };
 // Number of observation pairs (sum of k(k-1)/2 over free points with k observations).

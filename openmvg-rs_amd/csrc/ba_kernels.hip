@@ -1037,9 +1037,9 @@ __global__ void __launch_bounds__(64) k_camera_assembly_out(Topo tp, const uint3
             Sv[e] = 0.0;
         }
     }
-    if (S.vals) {  // the camera's diagonal block of S, 9 x 9 (zero beyond the free dimensions)
+    if (S.vals) {  // the camera's diagonal block of S in slot 0, 9 x 9 (zero beyond the free dimensions)
         const uint32_t nn = S.nbr_off[c + 1] - S.nbr_off[c];
-        T* row = S.vals + S.row_base[c] + 9 * (size_t)S.diag_slot[c];
+        T* row = S.vals + S.row_base[c];
         for (uint32_t e = v; e < 81; e += 64) {
             const uint32_t x = e / 9, y = e % 9;
             const bool in = (x < 6 ? x < pd : x - 6 < gd) && (y < 6 ? y < pd : y - 6 < gd);
@@ -2177,7 +2177,7 @@ struct PairMfma<float> {
 // every XCD gathers from the same band of cameras' records, which the shared last-level cache
 // holds (dealing each XCD a contiguous pair range measured 1.5x slower), while each row stays on
 // one XCD, whose L2 then serves the row's own records to all of its pairs.
-// The block goes to row c and, transposed, to row j.
+// The block goes to its slot in row c.
 constexpr int SP_WAVES = (int)SP_GROUP;
 constexpr int SP_DEPTH = 4;
 template <class T>
@@ -2225,113 +2225,134 @@ __global__ void __launch_bounds__(64 * SP_WAVES) k_schur_pairs(SchurPairs P, con
             }
         }
     }
-    const uint32_t c = P.cam[2 * k], j = P.cam[2 * k + 1];
-    const uint32_t nnc = S.nbr_off[c + 1] - S.nbr_off[c], nnj = S.nbr_off[j + 1] - S.nbr_off[j];
-    T* rc = S.vals + S.row_base[c] + 9 * (size_t)P.slot[2 * k];
-    T* rj = S.vals + S.row_base[j] + 9 * (size_t)P.slot[2 * k + 1];
+    const uint32_t c = P.cam[2 * k];
+    const uint32_t nnc = S.nbr_off[c + 1] - S.nbr_off[c];
+    T* rc = S.vals + S.row_base[c] + 9 * (size_t)(1 + k - S.pair_off[c]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int a = PairMfma<T>::row(lk, v), b = li;
-        if (a < 9 && b < 9) {
-            const T val = -acc[v];
-            rc[(size_t)a * schur_row_len(nnc) + b] = val;
-            rj[(size_t)b * schur_row_len(nnj) + a] = val;
-        }
+        if (a < 9 && b < 9) rc[(size_t)a * schur_row_len(nnc) + b] = -acc[v];
     }
 }
 
-// Explicit product, one workgroup per camera c and one wavefront per row slice a: the inputs of the
-// camera's neighbours, vin[9 s + b] = v_nbr(s),b, are gathered once into LDS (the row slices share
-// them), then each wavefront streams its slice in 16-byte groups (four in flight per lane) against
-// vin and the butterfly gives q_a = sum_e S[c][a][e] vin[e]; lambda diag v is added on the row's
-// parameter.  One partial of v.q per camera (its rows in order).  PP_MAX_SLOTS bounds the LDS
-// table; rows with more neighbours use k_pcg_product_gather, which reads the inputs per element.
-constexpr int PP_ROWS = 9;
-constexpr uint32_t PP_MAX_SLOTS = 800;  // LDS table of at most 9 x 800 x 8 B = 57.6 KB, sized per launch
+// Explicit product, first pass: one workgroup per camera c over its stored upper row.  Thread t
+// takes the 16-byte column groups t, t + PU_NT, ... of all 9 row slices at once, so each stored
+// value is read once and serves both of its uses: the row sums q_a += S[c][a][e] v_nbr(e) (the
+// inputs gathered from V.vexp per column) and, for the columns e >= 9 of the off-diagonal blocks
+// S_cj, the column sums (S_cj^T v_c)_b = sum_a S[c][a][e] v_c,a, written to S.tpair at the pair's
+// reverse position.  The row sums are reduced over the workgroup (per thread, per wavefront by the
+// butterfly, over the wavefronts in order) into S.qrow.
+constexpr int PU_NT = 256;
 template <class T>
-__device__ __forceinline__ void product_row(const Topo& tp, const SchurRows<T>& S, const PcgVecs& V, uint32_t c,
-                                            uint32_t a, uint32_t lane, const double* vin, double* vq_out) {
-    const uint32_t pd = tp.cam_dim[c], g = tp.cam_group[c], gd = tp.grp_dim[g];
-    const bool live = a < 6 ? a < pd : a - 6 < gd;
-    *vq_out = 0.0;
-    if (!live) return;  // uniform over the wavefront
+__global__ void __launch_bounds__(PU_NT) k_pcg_product_upper(SchurRows<T> S, PcgVecs V) {
+    __shared__ double red[PU_NT / 64][9];
+    const PcgState st = *V.st;
+    if (st.done) return;
+    const uint32_t c = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint32_t nb = S.nbr_off[c], nn = S.nbr_off[c + 1] - nb;
-    const T* __restrict__ sv = S.vals + S.row_base[c] + (size_t)a * schur_row_len(nn);
-    const uint32_t len = 9 * nn;
-    using G = typename Vec<T>::V;
-    constexpr uint32_t VN = Vec<T>::N, STEP = 64 * VN;
-    double sacc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t e0 = VN * lane; e0 < len; e0 += 4 * STEP) {
-        G m[4];
+    const uint32_t rl = schur_row_len(nn), len = 9 * nn;
+    const T* __restrict__ sv = S.vals + S.row_base[c];
+    const uint32_t p0 = S.pair_off[c];
+    double vc[9], ra[9];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint32_t e = e0 + u * STEP;
-            m[u] = e < len ? *reinterpret_cast<const G*>(sv + e) : G{};
+    for (int a = 0; a < 9; ++a) {
+        vc[a] = V.vexp[9 * (size_t)c + a];  // zero beyond the camera's free dimensions
+        ra[a] = 0.0;
+    }
+    using G = typename Vec<T>::V;
+    constexpr uint32_t VN = Vec<T>::N;
+    for (uint32_t e0 = VN * tid; e0 < len; e0 += VN * PU_NT) {
+        // e0 is a multiple of VN below len, and each slice holds schur_row_len(nn) >= e0 + VN values
+        G m[9];
+#pragma unroll
+        for (int a = 0; a < 9; ++a) m[a] = *reinterpret_cast<const G*>(sv + (size_t)a * rl + e0);
+        double vin[VN];
+#pragma unroll
+        for (uint32_t i = 0; i < VN; ++i) {
+            const uint32_t e = e0 + i;
+            vin[i] = e < len ? V.vexp[9 * (size_t)S.nbr[nb + e / 9] + e % 9] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (uint32_t i = 0; i < VN; ++i) {
+            const uint32_t e = e0 + i;
+            if (e >= len) continue;  // padding: never read as data
+            double t = 0.0;
 #pragma unroll
-            for (uint32_t i = 0; i < VN; ++i) {
-                const uint32_t e = e0 + u * STEP + i;
-                if (e < len) sacc[u] += (double)Vec<T>::at(m[u], i) * vin[e];
+            for (int a = 0; a < 9; ++a) {
+                const double x = (double)Vec<T>::at(m[a], i);
+                ra[a] += x * vin[i];
+                t += x * vc[a];
             }
+            if (e >= 9) {
+                const uint32_t s = e / 9, p = p0 + s - 1;
+                S.tpair[9 * (size_t)S.rev_pos[p] + (e - 9 * s)] = t;
+            }
+        }
     }
-    const double qa = wave_sum((sacc[0] + sacc[1]) + (sacc[2] + sacc[3]));
-    const uint32_t par = a < 6 ? tp.cam_off[c] + a : tp.grp_off[g] + (a - 6);
-    const double va = V.vexp[9 * (size_t)c + a];
-    const double qv = qa + V.lambda * V.diag[par] * va;
-    if (lane == 0) V.q[par] = qv;
-    *vq_out = va * qv;
-}
-template <class T>
-__global__ void __launch_bounds__(64 * PP_ROWS) k_pcg_product(Topo tp, SchurRows<T> S, PcgVecs V) {
-    extern __shared__ double vin[];  // 9 x (the largest neighbour count) values
-    __shared__ double sh[PP_ROWS];
-    const PcgState st = *V.st;
-    if (st.done) return;
-    const uint32_t c = blockIdx.x, lane = threadIdx.x & 63, a = threadIdx.x >> 6;
-    const uint32_t nb = S.nbr_off[c], len = 9 * (S.nbr_off[c + 1] - nb);
-    for (uint32_t e = threadIdx.x; e < len; e += 64 * PP_ROWS) vin[e] = V.vexp[9 * (size_t)S.nbr[nb + e / 9] + e % 9];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+        const double x = wave_sum(ra[a]);
+        if (lane == 0) red[w][a] = x;
+    }
     __syncthreads();
-    double vq;
-    product_row<T>(tp, S, V, c, a, lane, vin, &vq);
-    if (lane == 0) sh[a] = vq;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double acc = 0.0;
-        for (int i = 0; i < PP_ROWS; ++i) acc += sh[i];
-        V.part[c] = acc;
+    if (tid < 9) {
+        double q = 0.0;
+#pragma unroll
+        for (int i = 0; i < PU_NT / 64; ++i) q += red[i][tid];
+        S.qrow[9 * (size_t)c + tid] = q;
     }
 }
-// The same product for cameras with more than PP_MAX_SLOTS neighbours: the inputs are read from
-// global memory per element (one wavefront per row slice, as above).
+
+// Second pass: one workgroup per camera j.  Its reverse range of S.tpair (9 values per pair
+// (c < j, j)) is contiguous; thread L < PL_LANES reads values L, L + PL_LANES, ..., all of one
+// component b = L mod 9 (PL_LANES is a multiple of 9), and the 9 components are summed over the
+// threads in order.  q_j,b = qrow_j,b + that sum + lambda diag v_j,b on the free parameters, and
+// one partial of v.q per camera (its components in order).
+constexpr int PL_NT = 256;
+constexpr uint32_t PL_LANES = 63 * (PL_NT / 64);
 template <class T>
-__global__ void __launch_bounds__(64 * PP_ROWS) k_pcg_product_gather(Topo tp, SchurRows<T> S, PcgVecs V) {
-    __shared__ double sh[PP_ROWS];
+__global__ void __launch_bounds__(PL_NT) k_pcg_product_lower(Topo tp, SchurRows<T> S, PcgVecs V) {
+    __shared__ double red[PL_LANES];
+    __shared__ double vq[9];
     const PcgState st = *V.st;
     if (st.done) return;
-    const uint32_t c = blockIdx.x, lane = threadIdx.x & 63, a = threadIdx.x >> 6;
-    const uint32_t pd = tp.cam_dim[c], g = tp.cam_group[c], gd = tp.grp_dim[g];
-    const bool live = a < 6 ? a < pd : a - 6 < gd;
-    double vq = 0.0;
-    if (live) {
-        const uint32_t nb = S.nbr_off[c], nn = S.nbr_off[c + 1] - nb;
-        const T* __restrict__ sv = S.vals + S.row_base[c] + (size_t)a * schur_row_len(nn);
-        double s0 = 0.0;
-        for (uint32_t e = lane; e < 9 * nn; e += 64) s0 += (double)sv[e] * V.vexp[9 * (size_t)S.nbr[nb + e / 9] + e % 9];
-        const double qa = wave_sum(s0);
-        const uint32_t par = a < 6 ? tp.cam_off[c] + a : tp.grp_off[g] + (a - 6);
-        const double va = V.vexp[9 * (size_t)c + a];
-        const double qv = qa + V.lambda * V.diag[par] * va;
-        if (lane == 0) V.q[par] = qv;
-        vq = va * qv;
+    const uint32_t j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t L = 63 * w + lane;
+    const uint32_t r0 = 9 * S.rev_off[j], r1 = 9 * S.rev_off[j + 1];
+    if (lane < 63) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t e = r0 + L; e < r1; e += 4 * PL_LANES) {
+            double x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = e + u * PL_LANES < r1 ? S.tpair[e + u * PL_LANES] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] += x[u];
+        }
+        red[L] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     }
-    if (lane == 0) sh[a] = vq;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid < 9) {
+        const uint32_t b = tid;
+        double q = S.qrow[9 * (size_t)j + b];
+        for (uint32_t i = b; i < PL_LANES; i += 9) q += red[i];
+        const uint32_t pd = tp.cam_dim[j], g = tp.cam_group[j], gd = tp.grp_dim[g];
+        const bool live = b < 6 ? b < pd : b - 6 < gd;
+        double x = 0.0;
+        if (live) {
+            const uint32_t par = b < 6 ? tp.cam_off[j] + b : tp.grp_off[g] + (b - 6);
+            const double vb = V.vexp[9 * (size_t)j + b];
+            q += V.lambda * V.diag[par] * vb;
+            V.q[par] = q;
+            x = vb * q;
+        }
+        vq[b] = x;
+    }
+    __syncthreads();
+    if (tid == 0) {
         double acc = 0.0;
-        for (int i = 0; i < PP_ROWS; ++i) acc += sh[i];
-        V.part[c] = acc;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc += vq[i];
+        V.part[j] = acc;
     }
 }
 
@@ -2489,13 +2510,12 @@ void launch_pcg_points(hipStream_t s, const Topo& tp, const Lin<T> lin, const Po
 }
 uint32_t pcg_product_partials(const Topo& tp) { return tp.nc; }
 template <class T>
-void launch_pcg_product(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V, uint32_t max_slots) {
-    if (tp.nc == 0) return;
-    if (max_slots <= PP_MAX_SLOTS)
-        hipLaunchKernelGGL(k_pcg_product<T>, dim3(tp.nc), dim3(64 * PP_ROWS), 9 * max_slots * sizeof(double), s, tp, S,
-                           V);
-    else
-        hipLaunchKernelGGL(k_pcg_product_gather<T>, dim3(tp.nc), dim3(64 * PP_ROWS), 0, s, tp, S, V);
+void launch_pcg_product_upper(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V) {
+    if (tp.nc) hipLaunchKernelGGL(k_pcg_product_upper<T>, dim3(tp.nc), dim3(PU_NT), 0, s, S, V);
+}
+template <class T>
+void launch_pcg_product_lower(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V) {
+    if (tp.nc) hipLaunchKernelGGL(k_pcg_product_lower<T>, dim3(tp.nc), dim3(PL_NT), 0, s, tp, S, V);
 }
 template <class T>
 void launch_pcg_step_explicit(hipStream_t s, const Topo& tp, const PcgVecs& V) {
@@ -2541,7 +2561,8 @@ template void launch_gauge<double>(hipStream_t, const double*, int, double, Gaug
                                             const uint64_t*, double*, double*, double*, double*, double*, T*,      \
                                             SchurRows<T>);                                                         \
     template void launch_schur_pairs<T>(hipStream_t, const SchurPairs&, const T*, SchurRows<T>);                    \
-    template void launch_pcg_product<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&, uint32_t);              \
+    template void launch_pcg_product_upper<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&);                  \
+    template void launch_pcg_product_lower<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&);                  \
     template void launch_pcg_step_explicit<T>(hipStream_t, const Topo&, const PcgVecs&);                            \
     template void launch_pairs<T>(hipStream_t, const Topo&, const Lin<T>, const PointElim<T>, uint32_t,               \
                                   const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,              \

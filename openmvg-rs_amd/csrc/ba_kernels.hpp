@@ -135,32 +135,42 @@ void launch_reduce_points(hipStream_t s, const double* pscal, uint32_t np, doubl
                           AsmScalars* out);
 // ---- explicit reduced camera system (same topologies as the camera-side assembly) ----
 // S = U - W V^-1 W^T in camera blocks of 9 rows (pose 0..5, the camera's intrinsics 6..8; rows
-// and columns beyond the free dimensions are zero).  Row-slice layout: row a of camera c holds 9
-// values per neighbour slot s (neighbour camera nbr[nbr_off[c] + s], ascending, c itself
-// included): vals[row_base[c] + a * schur_row_len(nn_c) + 9 * s + b], nn_c = nbr_off[c + 1] -
-// nbr_off[c]; each row slice is padded to a multiple of 4 values (16-byte groups in both precisions;
-// the padding is never read as data).
+// and columns beyond the free dimensions are zero).  S is symmetric; its diagonal and upper blocks
+// are stored as row slices: row a of camera c holds 9 values per slot s of its upper row (slot 0
+// the camera itself, then its partners j > c ascending, nbr[nbr_off[c] + s]):
+// vals[row_base[c] + a * schur_row_len(nn_c) + 9 * s + b], nn_c = nbr_off[c + 1] - nbr_off[c];
+// each row slice is padded to a multiple of 4 values (16-byte groups in both precisions; the
+// padding is never read as data).  Pair number pair_off[c] + s - 1 is the block in slot s >= 1.
 __host__ __device__ inline uint32_t schur_row_len(uint32_t nn) { return (9 * nn + 3) & ~3u; }
 template <class T>
 struct SchurRows {
     T* vals;
     const uint64_t* row_base;   // nc
     const uint32_t* nbr_off;    // nc + 1
-    const uint32_t* nbr;        // neighbour cameras
-    const uint32_t* diag_slot;  // nc: the camera's own slot
+    const uint32_t* nbr;        // upper neighbour cameras (the camera first)
+    const uint32_t* pair_off;   // nc + 1: pairs (c, j > c) of row c
+    const uint32_t* rev_off;    // nc + 1: per camera j, the range of its pairs (c < j, j), ascending c
+    const uint32_t* rev_pos;    // per pair: its position in that order
+    double* tpair;              // 9 per pair, in reverse order: S_cj^T v_c (the product's first pass)
+    double* qrow;               // 9 per camera: sum over the camera's upper row of S v
 };
 // Camera pairs c < j that see common points, with the observation pairs (camera-major positions
 // of the observations by c and by j) of those points in point order.
 struct SchurPairs {
     uint32_t npairs;
     const uint32_t* cam;       // 2 per pair: c, j
-    const uint32_t* slot;      // 2 per pair: slot of j in row c, slot of c in row j
     const uint32_t* term_off;  // npairs + 1
     const uint32_t* term;      // 2 per term
     uint32_t ngroups;
     const uint32_t* group;     // per workgroup: first pair (SP_GROUP pairs of one camera row at most)
     const uint8_t* group_n;    // per workgroup: its pair count (0: an empty slot)
 };
+// Per camera-major observation: Z = M^T J~_X^T J~_c (3 x 9, row-major, padded to ZREC), where
+// M M^T = V^-1 of the observation's point; Z_o^T Z_o' = W_o V^-1 W_o'^T.
+constexpr int ZREC = 28;
+// Off-diagonal blocks S_cj = -sum Z_o^T Z_o' over each pair's terms, stored in row c.
+template <class T>
+void launch_schur_pairs(hipStream_t s, const SchurPairs& P, const T* z, SchurRows<T> S);
 // Pairs per workgroup of the pair assembly.
 constexpr uint32_t SP_GROUP = 4;
 // Workgroup slots of the pair assembly: camera row c's pairs go, in groups of SP_GROUP, to slots
@@ -169,12 +179,6 @@ constexpr uint32_t SP_GROUP = 4;
 // rows.
 void build_pair_groups(const std::vector<uint32_t>& pair_cam, uint32_t nc, std::vector<uint32_t>& group,
                        std::vector<uint8_t>& group_n);
-// Per camera-major observation: Z = M^T J~_X^T J~_c (3 x 9, row-major, padded to ZREC), where
-// M M^T = V^-1 of the observation's point; Z_o^T Z_o' = W_o V^-1 W_o'^T.
-constexpr int ZREC = 28;
-// Off-diagonal blocks S_cj = -sum Z_o^T Z_o' over each pair's terms, stored in rows c and j.
-template <class T>
-void launch_schur_pairs(hipStream_t s, const SchurPairs& P, const T* z, SchurRows<T> S);
 
 // Camera-side assembly of the PCG form when every intrinsic group has one camera and no camera
 // sees a point twice: per camera, the pose and intrinsic blocks' full and reduced gradients,
@@ -302,13 +306,16 @@ void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const P
 template <class T>
 void launch_pcg_step(hipStream_t s, const Topo& tp, const PcgVecs& V);
 // Explicit product q = S v + lambda diag v (v = d or x by the PCG mode, from V.vexp) with the
-// partials of v.q, then the step and direction updates.
+// partials of v.q, in two launches: the stored upper rows (row sums into S.qrow, column sums of
+// the off-diagonal blocks into S.tpair), then per camera the row part, its blocks' transposed
+// parts and the damping.  Then the step and direction updates.
 template <class T>
-void launch_pcg_product(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V, uint32_t max_slots);
+void launch_pcg_product_upper(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V);
+template <class T>
+void launch_pcg_product_lower(hipStream_t s, const Topo& tp, const SchurRows<T> S, const PcgVecs& V);
 template <class T>
 void launch_pcg_step_explicit(hipStream_t s, const Topo& tp, const PcgVecs& V);
-// Partials of the explicit product (one per camera).  max_slots: the largest neighbour count of a
-// camera row (rows up to 800 neighbours keep their inputs in LDS).
+// Partials of the explicit product (one per camera).
 uint32_t pcg_product_partials(const Topo& tp);
 // Camera-major linearisation: the SoA camera-major Jacobian copy (lin.jcm).
 template <class T>
