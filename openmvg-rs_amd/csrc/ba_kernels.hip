@@ -28,9 +28,18 @@ constexpr int NT = 256;
 
 inline unsigned grid_for(size_t n, int nt) { return (unsigned)((n + nt - 1) / nt); }
 
-// Fixed-shape tree sum over the NT threads of a block.
+// Butterfly sum over the 64 lanes of a wavefront: every lane ends with the same total (a + b and
+// b + a round alike), in an order fixed by the lane numbering.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Fixed-shape tree sum over the N threads of a block (one wavefront: the butterfly, no barrier).
 template <int N>
 __device__ double tree_sum(double v, double* sh) {
+    if constexpr (N == 64) return wave_sum(v);
     int t = threadIdx.x;
     sh[t] = v;
     __syncthreads();
@@ -93,14 +102,6 @@ __device__ __forceinline__ void camera_values(const Topo& tp, uint32_t c, const 
 }  // namespace
 
 namespace {
-// Butterfly sum over the 64 lanes of a wavefront: every lane ends with the same total (a + b and
-// b + a round alike), in an order fixed by the lane numbering.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
-
 // ---- compact linearisation ----
 // Per observation the linearisation keeps n = (x/z, y/z) of the camera-frame point, h = sqrt(w)/z
 // and sqrt(w) (rows SOA_Q..+3).  With r2 = |n|^2, d = 1 + k1 r2 + k2 r2^2, rd = k1 + 2 k2 r2 and
@@ -1624,18 +1625,20 @@ constexpr int NQ = (int)PCG_Q_WG;        // k_pcg_q: one wavefront per block
 
 // Sum of the per-workgroup partials part[0..nparts) in a fixed order, the same in every
 // workgroup (so each one derives the same step lengths and decisions).  Lane t adds
-// part[t], part[t + NTS], ... in that order (four loads in flight), then a fixed tree.
+// part[t], part[t + NTS], ... in that order (sixteen loads in flight: a few thousand partials
+// take one or two rounds), then a fixed tree.
 __device__ double sum_parts(const double* __restrict__ part, uint32_t nparts, double* sh) {
+    constexpr int U = 16;
     double acc = 0.0;
-    for (uint32_t base = 0; base < nparts; base += 4 * NTS) {
-        double v[4];
+    for (uint32_t base = 0; base < nparts; base += U * NTS) {
+        double v[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
             const uint32_t i = base + u * NTS + threadIdx.x;
             v[u] = i < nparts ? part[i] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc += v[u];
+        for (int u = 0; u < U; ++u) acc += v[u];
     }
     return tree_sum<NTS>(acc, sh);
 }
@@ -2169,17 +2172,19 @@ struct PairMfma<float> {
 };
 }  // namespace
 
-// Off-diagonal blocks, one wavefront per camera pair (c, j): S_cj = -sum_t Z_o(t)^T Z_o'(t) as one
-// 16x16x4 matrix-core step per term (lane l supplies Z_o[l/16][l%16] and Z_o'[l/16][l%16]; rows
-// and columns 9..15 and k = 3 are zero; terms past the end add zero).  Terms go in batches of
-// SP_DEPTH whose loads are issued one batch ahead.  Workgroup slot b holds up to SP_GROUP pairs of
-// one camera row (P.group): rows sweep in ascending order over the whole chip, so at any time
-// every XCD gathers from the same band of cameras' records, which the shared last-level cache
-// holds (dealing each XCD a contiguous pair range measured 1.5x slower), while each row stays on
-// one XCD, whose L2 then serves the row's own records to all of its pairs.
+// Off-diagonal blocks, one wavefront per camera pair (c, j): S_cj = -sum_t Z_o(t)^T Z_o'(t), the
+// sum over the 3 rows of every term's records of their outer products, four rows per 16x16x4
+// matrix-core step (lane l supplies value l%16 of row l/16 of the step's rows of Z_o and Z_o';
+// columns 9..15 and rows past the last term are zero; packing four rows per step instead of one
+// term's three takes a quarter of the steps and loads away).  Terms go in batches of SP_DEPTH = 4
+// (12 rows, three steps) whose loads are issued one batch ahead.  Workgroup slot b holds up to
+// SP_GROUP pairs of one camera row (P.group): rows sweep in ascending order over the whole chip,
+// so at any time every XCD gathers from the same band of cameras' records, which the shared
+// last-level cache holds (dealing each XCD a contiguous pair range measured 1.5x slower), while
+// each row stays on one XCD, whose L2 then serves the row's own records to all of its pairs.
 // The block goes to its slot in row c.
 constexpr int SP_WAVES = (int)SP_GROUP;
-constexpr int SP_DEPTH = 4;
+constexpr int SP_DEPTH = 4;  // terms per batch: 12 record rows, three whole steps
 template <class T>
 __global__ void __launch_bounds__(64 * SP_WAVES) k_schur_pairs(SchurPairs P, const T* __restrict__ z, SchurRows<T> S) {
     typedef typename PairMfma<T>::v4 v4;
@@ -2187,11 +2192,18 @@ __global__ void __launch_bounds__(64 * SP_WAVES) k_schur_pairs(SchurPairs P, con
     if (w >= P.group_n[blockIdx.x]) return;  // whole wavefronts leave; no barrier follows
     const uint32_t k = P.group[blockIdx.x] + w;
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-    const bool in = li < 9 && lk < 3;
-    const uint32_t off = in ? lk * 9 + li : 0;  // every lane reads inside the record
+    // A batch of SP_DEPTH = 4 terms is 12 rows (term u, record row r % 3 for row r = 3u + r % 3),
+    // four per matrix-core step: step s takes rows 4s + lk, i.e. term (4s + lk) / 3, which is
+    // one of the two terms (4s) / 3 and (4s) / 3 + 1.
+    uint32_t off[3];
+    bool hi[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int r = 4 * s + lk;
+        off[s] = li < 9 ? (uint32_t)((r % 3) * 9 + li) : 0u;  // every lane reads inside the record
+        hi[s] = r / 3 != (4 * s) / 3;
+    }
     const uint32_t t0 = P.term_off[k], t1 = P.term_off[k + 1];
-    // One batch: the term indices (clamped to the last term, uniform: scalar loads), then all of
-    // the batch's record loads back to back; terms past the end are zeroed after the loads.
     auto load = [&](uint32_t t, T* x, T* y) {
         uint32_t tc[SP_DEPTH], tj[SP_DEPTH];
 #pragma unroll
@@ -2201,27 +2213,32 @@ __global__ void __launch_bounds__(64 * SP_WAVES) k_schur_pairs(SchurPairs P, con
             tj[u] = P.term[2 * tt + 1];
         }
 #pragma unroll
-        for (int u = 0; u < SP_DEPTH; ++u) {
-            x[u] = z[(size_t)ZREC * tc[u] + off];
-            y[u] = z[(size_t)ZREC * tj[u] + off];
+        for (int s = 0; s < 3; ++s) {
+            const int u0 = (4 * s) / 3;
+            const uint32_t pc = hi[s] ? tc[u0 + 1] : tc[u0];
+            const uint32_t pj = hi[s] ? tj[u0 + 1] : tj[u0];
+            x[s] = z[(size_t)ZREC * pc + off[s]];
+            y[s] = z[(size_t)ZREC * pj + off[s]];
         }
 #pragma unroll
-        for (int u = 0; u < SP_DEPTH; ++u)
-            if (!(in && t + u < t1)) x[u] = y[u] = T(0);
+        for (int s = 0; s < 3; ++s) {
+            const uint32_t u = (uint32_t)((4 * s + lk) / 3);
+            if (!(li < 9 && t + u < t1)) x[s] = y[s] = T(0);
+        }
     };
     v4 acc = v4{T(0), T(0), T(0), T(0)};
     if (t0 < t1) {
-        T xa[SP_DEPTH], ya[SP_DEPTH];
+        T xa[3], ya[3];
         load(t0, xa, ya);
         for (uint32_t t = t0; t < t1; t += SP_DEPTH) {
-            T xn[SP_DEPTH], yn[SP_DEPTH];
+            T xn[3], yn[3];
             if (t + SP_DEPTH < t1) load(t + SP_DEPTH, xn, yn);
 #pragma unroll
-            for (int u = 0; u < SP_DEPTH; ++u) acc = PairMfma<T>::mac(xa[u], ya[u], acc);
+            for (int s = 0; s < 3; ++s) acc = PairMfma<T>::mac(xa[s], ya[s], acc);
 #pragma unroll
-            for (int u = 0; u < SP_DEPTH; ++u) {
-                xa[u] = xn[u];
-                ya[u] = yn[u];
+            for (int s = 0; s < 3; ++s) {
+                xa[s] = xn[s];
+                ya[s] = yn[s];
             }
         }
     }
