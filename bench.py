@@ -107,9 +107,9 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
       X (24 B); per camera R, t, f, k1, k2 (120 B).
     pcg_product (k_pcg_product_upper: the first pass of q = S v over the stored diagonal and
       upper blocks of the explicit reduced system, one workgroup per camera row): per stored slot
-      the 9 x 9 block (648 B) and its neighbour index (4 B); per camera pair its reverse position
-      (4 B) and the written column sums (72 B); per camera the 9 input values (72 B), its offsets
-      (16 B) and the written row sums (72 B).
+      the 9 x 9 block (648 B) and its neighbour index (4 B); per camera pair the written column
+      sums (72 B); per camera the 9 input values (72 B), its offsets (16 B) and the written row
+      sums (72 B).
     schur_pairs (k_schur_pairs: the off-diagonal blocks of the explicit S): each observation's Z
       record once (27 values, 216 B), 8 B per observation pair, and per camera pair the written
       9 x 9 block and its indices (656 B).
@@ -124,7 +124,7 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     if kernel == "camera_assembly":
         return no * (72 + 4 + 216) + npt * 128 + counts["segments"] * (4 + 576) + nc * 120
     if kernel == "pcg_product":
-        return counts["slots"] * (648 + 4) + counts["pairs"] * (4 + 72) + nc * (72 + 16 + 72)
+        return counts["slots"] * (648 + 4) + counts["pairs"] * 72 + nc * (72 + 16 + 72)
     if kernel == "pcg_points":
         return no * (48 + 24 + 8) + npt * (4 + 1 + 48 + 24 + 32) + nc * 72
     if kernel == "pcg_cameras":

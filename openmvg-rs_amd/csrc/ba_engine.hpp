@@ -341,7 +341,7 @@ class Engine {
         srow_base_.upload(SL.row_base, s_);
         spair_off_.upload(SL.pair_off, s_);
         srev_off_.upload(SL.rev_off, s_);
-        srev_pos_.upload(SL.rev_pos, s_);
+        srev_pair_.upload(SL.rev_pair, s_);
         spair_cam_.upload(SL.pair_cam, s_);
         sterm_off_.upload(SL.term_off, s_);
         sterm_.upload(SL.term, s_);
@@ -364,7 +364,7 @@ class Engine {
     }
     SchurRows<T> schur_rows() {
         return explicit_ ? SchurRows<T>{svals_.p, srow_base_.p, snbr_off_.p, snbr_.p, spair_off_.p,
-                                        srev_off_.p, srev_pos_.p, stpair_.p, sqrow_.p}
+                                        srev_off_.p, srev_pair_.p, stpair_.p, sqrow_.p}
                          : SchurRows<T>{};
     }
     SchurPairs schur_pairs() {
@@ -777,7 +777,7 @@ class Engine {
     uint32_t npairs_ = 0;
     Buf<T> zrec_, svals_;
     Buf<uint64_t> srow_base_;
-    Buf<uint32_t> snbr_off_, snbr_, spair_off_, srev_off_, srev_pos_, spair_cam_, sterm_off_, sterm_, sgroup_;
+    Buf<uint32_t> snbr_off_, snbr_, spair_off_, srev_off_, srev_pair_, spair_cam_, sterm_off_, sterm_, sgroup_;
     Buf<uint8_t> sgroup_n_;
     Buf<double> pvexp_, stpair_, sqrow_;
     bool pose_rows_ = true, keep_pose_rows_ = false;

@@ -744,7 +744,7 @@ SchurLists build_schur_lists(const Lowered& L, const std::vector<uint32_t>& pos)
     for (size_t c = 0; c < nc; ++c)
         for (uint32_t j : js[c]) S.rev_off[j + 1]++;
     for (size_t c = 0; c < nc; ++c) S.rev_off[c + 1] += S.rev_off[c];
-    S.rev_pos.resize(npairs);
+    S.rev_pair.resize(npairs);
     std::vector<uint32_t> rcur(S.rev_off.begin(), S.rev_off.end() - 1);
     S.term_off.push_back(0);
     S.term.reserve(2 * ent.size());
@@ -755,7 +755,7 @@ SchurLists build_schur_lists(const Lowered& L, const std::vector<uint32_t>& pos)
         for (size_t m = 0; m < js[c].size(); ++m, ++k) {
             S.pair_cam.push_back((uint32_t)c);
             S.pair_cam.push_back(js[c][m]);
-            S.rev_pos[k] = rcur[js[c][m]]++;
+            S.rev_pair[rcur[js[c][m]]++] = k;
             for (uint32_t t = 0; t < jn[c][m]; ++t, ++i) {
                 S.term.push_back(ent[i].pc);
                 S.term.push_back(ent[i].pj);

@@ -87,13 +87,12 @@ PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all);
 // (cameras sharing a free point with c) ascending.  The camera pairs c < j, numbered in (c, j)
 // order, carry the observation pairs (camera-major positions pos[o] of the observations by c and
 // by j) of their common free points, in point order.  The pairs (c < j, j) of each camera j,
-// in ascending c, form its reverse range rev_off[j]..rev_off[j + 1]; rev_pos[p] is pair p's
-// position in that order.
+// in ascending c, form its reverse range rev_off[j]..rev_off[j + 1] of rev_pair.
 struct SchurLists {
     std::vector<uint32_t> nbr_off, nbr;        // nc + 1, upper neighbours (the camera first)
     std::vector<uint64_t> row_base;            // nc: first value of row c
     std::vector<uint32_t> pair_off, pair_cam;  // nc + 1 (pairs of row c), 2 per pair: (c, j)
-    std::vector<uint32_t> rev_off, rev_pos;    // nc + 1, per pair
+    std::vector<uint32_t> rev_off, rev_pair;   // nc + 1, pair numbers
     std::vector<uint32_t> term_off, term;      // n_pairs + 1, 2 per term
     uint64_t n_values = 0;
 };

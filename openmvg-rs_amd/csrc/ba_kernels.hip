@@ -2256,8 +2256,9 @@ __global__ void __launch_bounds__(64 * SP_WAVES) k_schur_pairs(SchurPairs P, con
 // takes the 16-byte column groups t, t + PU_NT, ... of all 9 row slices at once, so each stored
 // value is read once and serves both of its uses: the row sums q_a += S[c][a][e] v_nbr(e) (the
 // inputs gathered from V.vexp per column) and, for the columns e >= 9 of the off-diagonal blocks
-// S_cj, the column sums (S_cj^T v_c)_b = sum_a S[c][a][e] v_c,a, written to S.tpair at the pair's
-// reverse position.  The row sums are reduced over the workgroup (per thread, per wavefront by the
+// S_cj, the column sums (S_cj^T v_c)_b = sum_a S[c][a][e] v_c,a, written to S.tpair in pair
+// order (a row's pairs are consecutive, so the stores are contiguous: scattering them to the
+// partners' ranges instead measured 17 us slower per launch on large scenes).  The row sums are reduced over the workgroup (per thread, per wavefront by the
 // butterfly, over the wavefronts in order) into S.qrow.
 constexpr int PU_NT = 256;
 template <class T>
@@ -2300,10 +2301,7 @@ __global__ void __launch_bounds__(PU_NT) k_pcg_product_upper(SchurRows<T> S, Pcg
                 ra[a] += x * vin[i];
                 t += x * vc[a];
             }
-            if (e >= 9) {
-                const uint32_t s = e / 9, p = p0 + s - 1;
-                S.tpair[9 * (size_t)S.rev_pos[p] + (e - 9 * s)] = t;
-            }
+            if (e >= 9) S.tpair[9 * (size_t)p0 + (e - 9)] = t;  // pair p0 + e / 9 - 1, component e % 9
         }
     }
 #pragma unroll
@@ -2320,38 +2318,42 @@ __global__ void __launch_bounds__(PU_NT) k_pcg_product_upper(SchurRows<T> S, Pcg
     }
 }
 
-// Second pass: one workgroup per camera j.  Its reverse range of S.tpair (9 values per pair
-// (c < j, j)) is contiguous; thread L < PL_LANES reads values L, L + PL_LANES, ..., all of one
-// component b = L mod 9 (PL_LANES is a multiple of 9), and the 9 components are summed over the
-// threads in order.  q_j,b = qrow_j,b + that sum + lambda diag v_j,b on the free parameters, and
-// one partial of v.q per camera (its components in order).
+// Second pass: one workgroup per camera j over the column sums of its pairs (c < j, j): thread t
+// adds the 9 values of pairs rev_pair[rev_off[j] + t], + PL_NT, ... (one 72-byte record each), and
+// the 9 components are reduced over the workgroup (per wavefront by the butterfly, then over the
+// wavefronts in order).  q_j,b = qrow_j,b + that sum + lambda diag v_j,b on the free parameters,
+// and one partial of v.q per camera (its components in order).
 constexpr int PL_NT = 256;
-constexpr uint32_t PL_LANES = 63 * (PL_NT / 64);
 template <class T>
 __global__ void __launch_bounds__(PL_NT) k_pcg_product_lower(Topo tp, SchurRows<T> S, PcgVecs V) {
-    __shared__ double red[PL_LANES];
+    __shared__ double red[PL_NT / 64][9];
     __shared__ double vq[9];
     const PcgState st = *V.st;
     if (st.done) return;
     const uint32_t j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t L = 63 * w + lane;
-    const uint32_t r0 = 9 * S.rev_off[j], r1 = 9 * S.rev_off[j + 1];
-    if (lane < 63) {
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (uint32_t e = r0 + L; e < r1; e += 4 * PL_LANES) {
-            double x[4];
+    const uint32_t r0 = S.rev_off[j], r1 = S.rev_off[j + 1];
+    double acc[9];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = e + u * PL_LANES < r1 ? S.tpair[e + u * PL_LANES] : 0.0;
+    for (int b = 0; b < 9; ++b) acc[b] = 0.0;
+    for (uint32_t r = r0 + tid; r < r1; r += PL_NT) {
+        const double* __restrict__ t = S.tpair + 9 * (size_t)S.rev_pair[r];
+        double x[9];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc[u] += x[u];
-        }
-        red[L] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        for (int b = 0; b < 9; ++b) x[b] = t[b];
+#pragma unroll
+        for (int b = 0; b < 9; ++b) acc[b] += x[b];
+    }
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+        const double x = wave_sum(acc[b]);
+        if (lane == 0) red[w][b] = x;
     }
     __syncthreads();
     if (tid < 9) {
         const uint32_t b = tid;
         double q = S.qrow[9 * (size_t)j + b];
-        for (uint32_t i = b; i < PL_LANES; i += 9) q += red[i];
+#pragma unroll
+        for (int i = 0; i < PL_NT / 64; ++i) q += red[i][b];
         const uint32_t pd = tp.cam_dim[j], g = tp.cam_group[j], gd = tp.grp_dim[g];
         const bool live = b < 6 ? b < pd : b - 6 < gd;
         double x = 0.0;

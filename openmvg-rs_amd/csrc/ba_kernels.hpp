@@ -149,9 +149,9 @@ struct SchurRows {
     const uint32_t* nbr_off;    // nc + 1
     const uint32_t* nbr;        // upper neighbour cameras (the camera first)
     const uint32_t* pair_off;   // nc + 1: pairs (c, j > c) of row c
-    const uint32_t* rev_off;    // nc + 1: per camera j, the range of its pairs (c < j, j), ascending c
-    const uint32_t* rev_pos;    // per pair: its position in that order
-    double* tpair;              // 9 per pair, in reverse order: S_cj^T v_c (the product's first pass)
+    const uint32_t* rev_off;    // nc + 1: per camera j, the range of its pairs (c < j, j) in rev_pair
+    const uint32_t* rev_pair;   // the pair numbers, ascending c per camera j
+    double* tpair;              // 9 per pair: S_cj^T v_c (the product's first pass)
     double* qrow;               // 9 per camera: sum over the camera's upper row of S v
 };
 // Camera pairs c < j that see common points, with the observation pairs (camera-major positions
