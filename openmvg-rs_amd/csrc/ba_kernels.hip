@@ -1593,14 +1593,16 @@ namespace {
 // z = M^-1 r for block A of dimension n <= 9 (Jacobi-scaled Cholesky factor, forward then
 // backward substitution in the order of retained/block.rs:171-250).  The factor is read into
 // registers first so that its loads are issued together.
-__device__ __forceinline__ void precond_apply_block(const double* __restrict__ factors, size_t nblocks, uint32_t A,
-                                                    int n, const double right[9], double sol[9]) {
-    double L[45], sc[9];
+__device__ __forceinline__ void precond_load(const double* __restrict__ factors, size_t nblocks, uint32_t A, int n,
+                                             double L[45], double sc[9]) {
     const int nl = n * (n + 1) / 2;
 #pragma unroll
     for (int i = 0; i < 45; ++i) L[i] = i < nl ? factors[i * nblocks + A] : 1.0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) sc[i] = i < n ? factors[(45 + i) * nblocks + A] : 1.0;
+}
+__device__ __forceinline__ void precond_solve(const double L[45], const double sc[9], int n, const double right[9],
+                                              double sol[9]) {
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         double v = right[r] / sc[r];
@@ -1617,6 +1619,12 @@ __device__ __forceinline__ void precond_apply_block(const double* __restrict__ f
     }
 #pragma unroll
     for (int i = 0; i < 9; ++i) sol[i] = i < n ? sol[i] / sc[i] : 0.0;
+}
+__device__ __forceinline__ void precond_apply_block(const double* __restrict__ factors, size_t nblocks, uint32_t A,
+                                                    int n, const double right[9], double sol[9]) {
+    double L[45], sc[9];
+    precond_load(factors, nblocks, A, n, L, sc);
+    precond_solve(L, sc, n, right, sol);
 }
 // PCG_INIT: the state after k_pcg_init (r = -g, z = M^-1 r), before the first direction.
 enum { PCG_NORMAL = 0, PCG_CONFIRM = 1, PCG_FINAL = 2, PCG_INIT = 3 };
@@ -1973,10 +1981,27 @@ __global__ void __launch_bounds__(NTS) k_pcg_xr(Topo tp, PcgVecs V) {
         if (lead) *V.st1 = s;
         return;
     }
+    // every load of the block first (none depends on the step length), then the partial sums
+    const bool normal = s.mode == PCG_NORMAL;
+    const uint32_t A = blockIdx.x * NTS + threadIdx.x;
+    const bool has = A < tp.nblocks;
+    const int n = has ? (int)tp.blk_dim[A] : 0;
+    const uint32_t off = has ? tp.blk_off[A] : 0u;
+    double qv[9], av[9], bv[9], xv[9], L[45], sc[9];
+    {
+        const double* __restrict__ d = V.d[s.par];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            qv[k] = k < n ? V.q[off + k] : 0.0;
+            av[k] = k < n ? (normal ? V.r[off + k] : -V.grad[off + k]) : 0.0;
+            bv[k] = (k < n && normal) ? d[off + k] : 0.0;
+            xv[k] = (k < n && normal) ? V.x[off + k] : 0.0;
+        }
+        precond_load(V.factors, tp.nblocks, has ? A : 0u, n, L, sc);
+    }
     const double curv = sum_parts(V.part, V.nqparts, sh);
     s.products += 1;
     double alpha = 0.0;
-    const bool normal = s.mode == PCG_NORMAL;
     if (normal) {
         alpha = s.rho / curv;
         if (!(isfinite(curv) && curv > 0.0) || !isfinite(alpha)) {
@@ -1986,21 +2011,9 @@ __global__ void __launch_bounds__(NTS) k_pcg_xr(Topo tp, PcgVecs V) {
         }
         s.iter += 1;
     }
-    const uint32_t A = blockIdx.x * NTS + threadIdx.x;
     double rr = 0.0, rz = 0.0;
-    if (A < tp.nblocks) {
-        const int n = (int)tp.blk_dim[A];
-        const uint32_t off = tp.blk_off[A];
-        const double* __restrict__ d = V.d[s.par];
-        // every load of the block first, then the preconditioner, then the stores
-        double qv[9], av[9], bv[9], rv[9], xv[9], sol[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            qv[k] = k < n ? V.q[off + k] : 0.0;
-            av[k] = k < n ? (normal ? V.r[off + k] : -V.grad[off + k]) : 0.0;
-            bv[k] = (k < n && normal) ? d[off + k] : 0.0;
-            xv[k] = (k < n && normal) ? V.x[off + k] : 0.0;
-        }
+    if (has) {
+        double rv[9], sol[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             if (normal) {
@@ -2010,7 +2023,7 @@ __global__ void __launch_bounds__(NTS) k_pcg_xr(Topo tp, PcgVecs V) {
                 rv[k] = av[k] - qv[k];
             }
         }
-        precond_apply_block(V.factors, tp.nblocks, A, n, rv, sol);
+        precond_solve(L, sc, n, rv, sol);
         for (int k = 0; k < n; ++k) {
             if (normal) V.x[off + k] = xv[k];
             V.r[off + k] = rv[k];
@@ -2035,6 +2048,32 @@ __global__ void __launch_bounds__(NTS) k_pcg_dir(Topo tp, PcgVecs V) {
     if (s.done) {
         if (lead) *V.st = s;
         return;
+    }
+    // the block's loads first (which of them count is decided below), then the partial sums
+    const uint32_t A = blockIdx.x * NTS + threadIdx.x;
+    const bool has = A < tp.nblocks, cam = A < tp.nc;
+    const uint32_t n = has ? tp.blk_dim[A] : 0u, off = has ? tp.blk_off[A] : 0u;
+    const double* __restrict__ dc = V.d[s.par];
+    double zr[9], dr[9], xr[6], gzr[3], gdr[3], gxr[3];
+    uint32_t pd = 0, gd = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        zr[k] = k < (int)n ? V.z[off + k] : 0.0;
+        dr[k] = k < (int)n ? dc[off + k] : 0.0;
+    }
+    if (cam) {
+        const uint32_t g = tp.cam_group[A], go = tp.grp_off[g];
+        pd = tp.cam_dim[A];
+        gd = tp.grp_dim[g];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) xr[k] = k < (int)pd ? V.x[off + k] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const bool in = k < (int)gd;
+            gzr[k] = in ? V.z[go + k] : 0.0;
+            gdr[k] = in ? dc[go + k] : 0.0;
+            gxr[k] = in ? V.x[go + k] : 0.0;
+        }
     }
     const double rr = sum_parts(V.part + V.nqparts, V.nparts, sh);
     const double rz = sum_parts(V.part + V.nqparts + V.nparts, V.nparts, sh);
@@ -2093,34 +2132,27 @@ __global__ void __launch_bounds__(NTS) k_pcg_dir(Topo tp, PcgVecs V) {
         }
     }
     if (!s.done) {
-        const double* __restrict__ dc = V.d[s.par];
         double* __restrict__ dn = V.d[1 - s.par];
         const bool use_d = s.mode == PCG_NORMAL;  // else the next product is S x
-        const uint32_t A = blockIdx.x * NTS + threadIdx.x;
         // the next direction (or x) on index i: the same expression in every lane that needs it
         auto value = [&](double z, double d, double x) { return use_d ? (upd == 1 ? z + beta * d : z) : x; };
-        if (A < tp.nblocks) {
-            const uint32_t n = tp.blk_dim[A], off = tp.blk_off[A];
+        if (has) {
             double zv[9], dv[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                zv[k] = (upd && k < (int)n) ? V.z[off + k] : 0.0;
-                dv[k] = (upd == 1 && k < (int)n) ? dc[off + k] : 0.0;
+                zv[k] = upd ? zr[k] : 0.0;
+                dv[k] = upd == 1 ? dr[k] : 0.0;
             }
             double vp[6], vi[3];
-            const bool cam = A < tp.nc;
             if (cam) {
-                const uint32_t pd = tp.cam_dim[A], g = tp.cam_group[A];
-                const uint32_t gd = tp.grp_dim[g], go = tp.grp_off[g];
                 double xp[6], gz[3], gdv[3], gx[3];
 #pragma unroll
-                for (int k = 0; k < 6; ++k) xp[k] = (!use_d && k < (int)pd) ? V.x[off + k] : 0.0;
+                for (int k = 0; k < 6; ++k) xp[k] = !use_d ? xr[k] : 0.0;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const bool in = k < (int)gd;
-                    gz[k] = (use_d && in) ? V.z[go + k] : 0.0;
-                    gdv[k] = (use_d && upd == 1 && in) ? dc[go + k] : 0.0;
-                    gx[k] = (!use_d && in) ? V.x[go + k] : 0.0;
+                    gz[k] = use_d ? gzr[k] : 0.0;
+                    gdv[k] = (use_d && upd == 1) ? gdr[k] : 0.0;
+                    gx[k] = !use_d ? gxr[k] : 0.0;
                 }
 #pragma unroll
                 for (int k = 0; k < 6; ++k) vp[k] = k < (int)pd ? value(zv[k], dv[k], xp[k]) : 0.0;
