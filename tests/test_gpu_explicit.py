@@ -137,3 +137,25 @@ def test_medium_scene_forms_agree(adjusters):
     _compare_solutions(ex, im, tol_param=1e-7)
     print(f"medium iterative: explicit loop {ex.report.seconds_iterations:.3f} s, "
           f"implicit {im.report.seconds_iterations:.3f} s, {ex.report.iterations} iterations")
+
+
+def test_dense_covisibility_forms_agree(adjusters):
+    """300 cameras on a ring that all see one cloud: most camera pairs share points, so camera rows
+    hold up to ~300 blocks and a camera's pairs with the cameras below it exceed one pass of the
+    second product launch (256 threads, one pair each).  The two products give the same trajectory
+    (per-iteration cost within 1e-10) and final parameters within 1e-7 (as on the medium scene)."""
+    sc = synthetic.ring_scene(seed=0xBA0011, n_cameras=300, n_points=20_000)
+    p = sc.problem
+    cov = np.zeros((p.n_poses, p.n_poses), dtype=bool)
+    order = np.argsort(p.obs_point, kind="stable")
+    pts, cams = p.obs_point[order], p.obs_pose[order]
+    starts = np.flatnonzero(np.r_[True, pts[1:] != pts[:-1]])
+    for a, b in zip(starts, np.r_[starts[1:], len(pts)]):
+        cs = cams[a:b]
+        cov[np.ix_(cs, cs)] = True
+    below = np.triu(cov, 1).sum(axis=0)  # per camera j: cameras c < j sharing a point with it
+    assert below.max() > 256, below.max()
+    cfg = SolveConfig(max_iterations=5, linear_solver=A.SOLVER_ITERATIVE)
+    ex, im = (ba.solve(p, cfg) for ba in adjusters)
+    assert _last(adjusters[0]) == A.SCHUR_EXPLICIT
+    _compare_solutions(ex, im, tol_param=1e-7)
