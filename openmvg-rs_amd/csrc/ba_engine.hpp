@@ -521,19 +521,29 @@ class Engine {
     }
 
     // One assembly at damping lambda with loss `loss` (schur/assembly.rs:832-1069).
-    AsmScalars assemble(int state, double lambda, int loss, double loss_scale, bool reduced_system) {
+    // same_state: the caller has not written state `state` since the last assembly (an LM step
+    // was rejected); the linearisation of that assembly is then kept when it was taken of this
+    // state with this loss (it is a function of them alone).
+    AsmScalars assemble(int state, double lambda, int loss, double loss_scale, bool reduced_system,
+                        bool same_state = false) {
         Topo tp = tp_;
         tp.loss = loss;
         tp.loss_scale = loss_scale;
         lin_state_ = state;
-        if (!fixed_gauge_) launch_gauge<double>(s_, t_[state].p, L_.gauge_camera, L_.scene_scale, gauge_.p);
-        timed(BA_PROFILE_LINEARIZE, [&] {
-            launch_linearize<T>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
-                                     lin());
-        });
-        if (iterative_ && (reduced_system || fast_cam_))
-            launch_linearize_cm<T>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
-                                        lin());
+        const bool cm = iterative_ && (reduced_system || fast_cam_);
+        const bool keep = same_state && lin_kept_.valid && lin_kept_.state == state && lin_kept_.loss == loss &&
+                          lin_kept_.scale == loss_scale && (lin_kept_.cm || !cm);
+        if (!keep) {
+            if (!fixed_gauge_) launch_gauge<double>(s_, t_[state].p, L_.gauge_camera, L_.scene_scale, gauge_.p);
+            timed(BA_PROFILE_LINEARIZE, [&] {
+                launch_linearize<T>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
+                                    lin());
+            });
+            if (cm)
+                launch_linearize_cm<T>(s_, tp, R_[state].p, t_[state].p, gp_[state].p, X_[state].p, uv_.p, gauge_.p,
+                                       lin());
+            lin_kept_ = {true, state, loss, loss_scale, cm};
+        }
         lambda_ = lambda;
         timed(BA_PROFILE_POINT_ELIM, [&] { launch_point_elim<T>(s_, tp, lin(), (T)lambda, lambda == 0.0, pe()); });
         launch_reduce_points(s_, pscal_.p, tp.np, partials_.p, asm_.p);
@@ -767,6 +777,12 @@ class Engine {
     hipEvent_t pcg_ev_[2] = {nullptr, nullptr};
     double h_dots_[4];
     bool fixed_gauge_ = false;
+    struct {
+        bool valid;
+        int state, loss;
+        double scale;
+        bool cm;
+    } lin_kept_ = {false, 0, 0, 0.0, false};  // the linearisation held in lin() (see assemble)
     double lambda_ = 0.0;
     bool batched_ = false;  // independent windows: per-window dense systems instead of one S
     double refine_rel_ = NAN;

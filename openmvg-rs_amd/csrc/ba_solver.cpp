@@ -181,12 +181,14 @@ Status refine(Engine<T>& E, const Lowered& L, const ba_solve_config* cfg, ba_sol
     uint32_t pcg_total = 0;
     double pcg_max_rel = NAN;
     bool stop = false;
+    bool same_state = true;  // state `state` unchanged since the last assembly
     for (int si = 0; si < 3 && !stop; ++si) {
         bool confirm = false;
         for (uint32_t it = 0; it < sched[si].iters; ++it) {
             iterations += 1;
             const double lambda = damping.value;
-            AsmScalars a = E.assemble(state, lambda, sched[si].loss, sched[si].scale, true);
+            AsmScalars a = E.assemble(state, lambda, sched[si].loss, sched[si].scale, true, same_state);
+            same_state = true;
             if (!have_initial && std::isfinite(a.cost) && std::isfinite(a.sq) && no > 0) {
                 initial_cost = a.cost;
                 initial_rmse = std::sqrt(a.sq / (2.0 * no));
@@ -250,6 +252,7 @@ Status refine(Engine<T>& E, const Lowered& L, const ba_solve_config* cfg, ba_sol
                 continue;
             }
             state = 1 - state;
+            same_state = false;
             accepted += 1;
             const double rms = std::sqrt(xx / (double)std::max<size_t>(D, 1));
             const double sn = std::sqrt(rms * rms + point_rms * point_rms);
