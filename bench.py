@@ -388,8 +388,14 @@ def main() -> int:
     # roofline of the dominant kernel: HIP events around each of its launches
     kernel_ids = {"pcg_points": A.PROFILE_PCG_POINTS, "pcg_cameras": A.PROFILE_PCG_CAMERAS,
                   "linearize": A.PROFILE_LINEARIZE, "pcg_product": A.PROFILE_PCG_PRODUCT}
+    # the profiled kernel's form for every iteration of that solve (AUTO keeps the first,
+    # few-PCG-iteration LM iterations matrix-free)
+    prof_mode = {"pcg_product": A.SCHUR_EXPLICIT, "pcg_points": A.SCHUR_IMPLICIT,
+                 "pcg_cameras": A.SCHUR_IMPLICIT}.get(args.roofline_kernel, A.SCHUR_AUTO)
+    ba.handle.lib.ba_hip_set_schur_product(ba.handle.ptr, prof_mode)
     prof = ba.solve(prob, base.with_(max_iterations=max(2, min(args.steps, 4)),
                                      profile_kernel=kernel_ids[args.roofline_kernel]))
+    ba.handle.lib.ba_hip_set_schur_product(ba.handle.ptr, A.SCHUR_AUTO)
     launches = int(prof.report.profile_launches)
     avg_s = float(prof.report.profile_seconds) / max(launches, 1)
     counts = lowered_counts(prob)
@@ -453,10 +459,11 @@ def main() -> int:
         "config": {
             "workload": (f"large BA: {prob.n_poses} cameras (f,k1,k2 free), {prob.n_points} points, "
                          f"{prob.n_observations} observations, geometric tracks, 2% outliers, Huber 1px, "
-                         f"LM + block-Jacobi PCG on the {form} Schur system (rtol 0.1), seed 0xBA0003"),
+                         "LM + block-Jacobi PCG (rtol 0.1) on the Schur system: explicit in the LM iterations "
+                         "after a PCG solve of 10 or more iterations, matrix-free otherwise (AUTO), seed 0xBA0003"),
             "reduced_dimension": D,
             "precision": "fp64",
-            "schur_product": form,
+            "schur_product": "auto" + (" (explicit system used)" if form == "explicit" else " (matrix-free only)"),
             "pcg_iterations": int(rep.pcg_iterations),
         },
         "roofline": {

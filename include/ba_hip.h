@@ -222,13 +222,17 @@ void ba_hip_destroy(ba_hip_handle* handle);
 /* Form of the PCG product with S = U - W V^-1 W^T (no counterpart in the reference, whose
  * retained operator is matrix-free: self_calibration_retained/operator.rs:26-76).  IMPLICIT
  * multiplies matrix-free from the per-observation Jacobians (two passes over the observations
- * per product); EXPLICIT assembles S once per LM iteration in 9 x 9 camera blocks and multiplies
- * block-sparse.  AUTO (the default) takes EXPLICIT when every intrinsic group has one camera, no
- * camera observes a point twice and the explicit buffers fit in 40 % of the free device memory;
- * EXPLICIT on another topology falls back to IMPLICIT.  The forms agree to rounding. */
+ * per product); EXPLICIT assembles S once per LM iteration in 9 x 9 camera blocks (its diagonal
+ * and upper blocks) and multiplies block-sparse.  The explicit system is set up when every
+ * intrinsic group has one camera, no camera observes a point twice and (under AUTO) its buffers
+ * fit in 40 % of the free device memory; EXPLICIT then uses it in every LM iteration, AUTO (the
+ * default) in the iterations that follow one whose PCG took at least 10 iterations (the
+ * assembly costs about as much as ten matrix-free products save), and IMPLICIT never.  On
+ * another topology every mode is IMPLICIT.  The forms agree to rounding. */
 enum ba_schur_product { BA_SCHUR_AUTO = 0, BA_SCHUR_IMPLICIT = 1, BA_SCHUR_EXPLICIT = 2 };
 int ba_hip_set_schur_product(ba_hip_handle* handle, int32_t mode);
-/* The form of the handle's last iterative solve (IMPLICIT or EXPLICIT; AUTO before any). */
+/* The form of the handle's last iterative solve: EXPLICIT when any of its PCG solves used the
+ * explicit system, else IMPLICIT (AUTO before any). */
 int32_t ba_hip_last_schur_product(const ba_hip_handle* handle);
 
 /* BundleAdjuster::solve (solver.rs:1572) for the self-calibrating pixel path

@@ -361,9 +361,10 @@ class Engine {
         HIP_OK(hipMemsetAsync(pvexp_.p, 0, pvexp_.n * sizeof(double), s_));
         HIP_OK(hipStreamSynchronize(s_));  // the host lists end with this scope
         explicit_ = true;
+        use_explicit_ = true;
     }
     SchurRows<T> schur_rows() {
-        return explicit_ ? SchurRows<T>{svals_.p, srow_base_.p, snbr_off_.p, snbr_.p, spair_off_.p,
+        return use_explicit_ ? SchurRows<T>{svals_.p, srow_base_.p, snbr_off_.p, snbr_.p, spair_off_.p,
                                         srev_off_.p, srev_pair_.p, stpair_.p, sqrow_.p}
                          : SchurRows<T>{};
     }
@@ -491,8 +492,8 @@ class Engine {
         V.vcam = pvcam_.p;
         V.part = ppart_.p;
         V.nparts = pcg_partials(tp_);
-        V.nqparts = explicit_ ? pcg_product_partials(tp_) : pcg_q_partials(tp_);
-        V.vexp = explicit_ ? pvexp_.p : nullptr;
+        V.nqparts = use_explicit_ ? pcg_product_partials(tp_) : pcg_q_partials(tp_);
+        V.vexp = use_explicit_ ? pvexp_.p : nullptr;
         V.nseg = nseg_;
         V.npchunks = npchunks_;
         V.pchunk_pt = pchunk_pt_.p;
@@ -508,7 +509,17 @@ class Engine {
 
     Lin<T> lin() { return Lin<T>{nullptr, jc_.p, ocost_.p, osq_.p, valid_.p, jpm_.p, jcm_.p, pose_rows_ ? 1 : 0}; }
     void set_schur_product(int mode) { schur_mode_ = mode; }
-    bool explicit_product() const { return explicit_; }
+    // Whether a PCG solve of this engine has used the explicit system.
+    bool explicit_product() const { return used_explicit_; }
+    // The product of the next LM iteration (before its assembly): with the explicit system set
+    // up, EXPLICIT takes it always, AUTO when the PCG is expected to take at least
+    // EXPLICIT_MIN_PCG iterations (the previous iteration's count).  Its assembly costs about as
+    // much as ten matrix-free products save (2.1 ms against 0.21 ms per PCG iteration at the
+    // 2,000-camera scene), so solves that need only a few PCG iterations stay matrix-free.
+    static constexpr uint32_t EXPLICIT_MIN_PCG = 10;
+    void plan_product(uint32_t expected_pcg) {
+        use_explicit_ = explicit_ && (schur_mode_ == BA_SCHUR_EXPLICIT || expected_pcg >= EXPLICIT_MIN_PCG);
+    }
     // Keep the pose columns of J~_c in the point-major copy (for the per-observation hook).
     void keep_pose_rows() { keep_pose_rows_ = true; }
     PointElim<T> pe() { return PointElim<T>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, prec_.p, pscal_.p}; }
@@ -551,10 +562,10 @@ class Engine {
             timed(BA_PROFILE_PAIRS, [&] {
                 launch_camera_assembly<T>(s_, tp, lin(), pe(), R_[state].p, gp_[state].p, gauge_.p, nseg_,
                                           seg_cam_.p, cseg_off_.p, casm_.p, diag_pair_.p, value_off_.p, grad_.p,
-                                          fgrad_.p, diag_.p, u_vals_.p, s_vals_.p, explicit_ ? zrec_.p : nullptr,
+                                          fgrad_.p, diag_.p, u_vals_.p, s_vals_.p, use_explicit_ ? zrec_.p : nullptr,
                                           schur_rows());
             });
-            if (explicit_)
+            if (use_explicit_)
                 timed(BA_PROFILE_SCHUR_PAIRS, [&] { launch_schur_pairs<T>(s_, schur_pairs(), zrec_.p, schur_rows()); });
         } else {
             launch_block_gradient<T>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
@@ -626,11 +637,12 @@ class Engine {
         launch_pcg_init<T>(s_, tp_, V, tol, limit);
         HIP_OK(hipStreamSynchronize(s_));
         if (*h_int_.p) return false;
+        used_explicit_ = used_explicit_ || use_explicit_;
         const int batch = 4;
         const SchurRows<T> SR = schur_rows();
         auto enqueue_batch = [&] {
             for (int k = 0; k < batch; ++k) {
-                if (explicit_) {
+                if (use_explicit_) {
                     timed(BA_PROFILE_PCG_PRODUCT, [&] { launch_pcg_product_upper<T>(s_, tp_, SR, V); });
                     launch_pcg_product_lower<T>(s_, tp_, SR, V);
                     launch_pcg_step_explicit<T>(s_, tp_, V);
@@ -642,7 +654,7 @@ class Engine {
             }
         };
         const PcgState* fin = nullptr;
-        const bool profiled_product = explicit_ ? profile_ == BA_PROFILE_PCG_PRODUCT
+        const bool profiled_product = use_explicit_ ? profile_ == BA_PROFILE_PCG_PRODUCT
                                                 : (profile_ == BA_PROFILE_PCG_POINTS || profile_ == BA_PROFILE_PCG_CAMERAS);
         if (profiled_product) {
             // one batch at a time, so that launches after the state turned done (which return at
@@ -789,7 +801,9 @@ class Engine {
     uint32_t mixed_fallbacks_ = 0;
     bool fast_cam_ = false;
     int schur_mode_ = BA_SCHUR_AUTO;
-    bool explicit_ = false;  // PCG product with the assembled S
+    bool explicit_ = false;      // the explicit system is set up
+    bool use_explicit_ = false;  // ... and the current LM iteration's PCG uses it
+    bool used_explicit_ = false;
     uint32_t npairs_ = 0;
     Buf<T> zrec_, svals_;
     Buf<uint64_t> srow_base_;

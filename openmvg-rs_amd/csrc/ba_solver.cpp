@@ -182,11 +182,13 @@ Status refine(Engine<T>& E, const Lowered& L, const ba_solve_config* cfg, ba_sol
     double pcg_max_rel = NAN;
     bool stop = false;
     bool same_state = true;  // state `state` unchanged since the last assembly
+    uint32_t last_terms = 0;  // PCG iterations of the last solve: picks the next product form
     for (int si = 0; si < 3 && !stop; ++si) {
         bool confirm = false;
         for (uint32_t it = 0; it < sched[si].iters; ++it) {
             iterations += 1;
             const double lambda = damping.value;
+            if (iterative) E.plan_product(last_terms);
             AsmScalars a = E.assemble(state, lambda, sched[si].loss, sched[si].scale, true, same_state);
             same_state = true;
             if (!have_initial && std::isfinite(a.cost) && std::isfinite(a.sq) && no > 0) {
@@ -220,6 +222,7 @@ Status refine(Engine<T>& E, const Lowered& L, const ba_solve_config* cfg, ba_sol
                 uint32_t limit = confirm ? pcg_cap : std::min<uint32_t>(pcg_cap, (uint32_t)D);
                 solved = E.solve_pcg(tol, (int)limit, &terms, &rel);
                 pcg_total += terms;
+                last_terms = terms;
                 if (std::isfinite(rel)) pcg_max_rel = std::isfinite(pcg_max_rel) ? std::fmax(pcg_max_rel, rel) : rel;
             }
             if (!solved) {
@@ -336,11 +339,11 @@ Status run_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* c
     Engine<T> E(h->stream, L, iterative, cfg->profile_kernel);
     E.set_schur_product(h->schur_product);
     E.setup();
-    if (iterative) h->last_schur_product = E.explicit_product() ? BA_SCHUR_EXPLICIT : BA_SCHUR_IMPLICIT;
     double t1 = now();
     int state = 0;
     double loop_seconds = 0.0;
     Status st = refine(E, L, cfg, r, trace, &state, &loop_seconds);
+    if (iterative) h->last_schur_product = E.explicit_product() ? BA_SCHUR_EXPLICIT : BA_SCHUR_IMPLICIT;
     if (!st.ok()) return st;
     std::vector<double> R, t, gp, X;
     E.download(state, R, t, gp, X);
@@ -558,12 +561,12 @@ int ba_hip_pcg(ba_hip_handle* h, const ba_device_problem* dp, double lambda, dou
     Engine<double> E(h->stream, L, true);
     E.set_schur_product(h->schur_product);
     E.setup();
-    h->last_schur_product = E.explicit_product() ? BA_SCHUR_EXPLICIT : BA_SCHUR_IMPLICIT;
     E.set_gauge(gauge_from(dp));
-    AsmScalars a = E.assemble(0, lambda, L.loss, L.loss_scale, true);
+    AsmScalars a = E.assemble(0, lambda, L.loss, L.loss_scale, true);  // the explicit system when set up
     uint32_t terms = 0;
     double rel = NAN;
     bool ok = (a.invalid == 0.0 && a.nonfinite == 0.0) && E.solve_pcg(tol, (int)max_iterations, &terms, &rel);
+    h->last_schur_product = E.explicit_product() ? BA_SCHUR_EXPLICIT : BA_SCHUR_IMPLICIT;
     if (L.D) HIP_OK(hipMemcpyAsync(step, E.pcg_x(), L.D * 8, hipMemcpyDeviceToDevice, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     if (iterations) *iterations = terms;
