@@ -557,15 +557,16 @@ class Engine {
         }
         lambda_ = lambda;
         timed(BA_PROFILE_POINT_ELIM, [&] { launch_point_elim<T>(s_, tp, lin(), (T)lambda, lambda == 0.0, pe()); });
+        const bool ex = use_explicit_ && reduced_system;  // the explicit system only for a step
         launch_reduce_points(s_, pscal_.p, tp.np, partials_.p, asm_.p);
         if (fast_cam_) {
             timed(BA_PROFILE_PAIRS, [&] {
                 launch_camera_assembly<T>(s_, tp, lin(), pe(), R_[state].p, gp_[state].p, gauge_.p, nseg_,
                                           seg_cam_.p, cseg_off_.p, casm_.p, diag_pair_.p, value_off_.p, grad_.p,
-                                          fgrad_.p, diag_.p, u_vals_.p, s_vals_.p, use_explicit_ ? zrec_.p : nullptr,
-                                          schur_rows());
+                                          fgrad_.p, diag_.p, u_vals_.p, s_vals_.p, ex ? zrec_.p : nullptr,
+                                          ex ? schur_rows() : SchurRows<T>{});
             });
-            if (use_explicit_)
+            if (ex)
                 timed(BA_PROFILE_SCHUR_PAIRS, [&] { launch_schur_pairs<T>(s_, schur_pairs(), zrec_.p, schur_rows()); });
         } else {
             launch_block_gradient<T>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
