@@ -319,7 +319,7 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES between two device buffers three times (a known-size kernel that "
                          "scales HBM counters in a profiled run)")
-    ap.add_argument("--only-form", choices=["medium_direct", "batched", "differentiation"],
+    ap.add_argument("--only-form", choices=["medium_direct", "batched", "differentiation", "mixed_large"],
                     help="run only this secondary measurement and print its JSON (no headline line)")
     args = ap.parse_args()
 
@@ -338,6 +338,8 @@ def main() -> int:
 
     if args.only_form:
         fn = {"medium_direct": lambda: bench_medium(HipBundleAdjuster(0)),
+              "mixed_large": lambda: bench_mixed(HipBundleAdjuster(0), synthetic.large_scene(n_points=args.points).problem,
+                                                 SolveConfig(linear_solver=A.SOLVER_ITERATIVE), args.steps, None),
               "batched": lambda: bench_batched(HipBundleAdjuster(0)),
               "differentiation": lambda: bench_differentiation(20)}[args.only_form]
         print(json.dumps({args.only_form: fn()}))

@@ -115,7 +115,9 @@ typedef struct ba_solve_config {
     double gradient_tolerance;            /* default 1e-10 */
     double parameter_tolerance;           /* default 1e-8 */
     int32_t linear_solver;                /* enum ba_linear_solver */
-    uint64_t workspace_budget_bytes;      /* 0 = automatic (1 GiB planning budget) */
+    uint64_t workspace_budget_bytes;      /* 0 = automatic: a 1 GiB planning budget for AUTO and no
+                                             limit; else the planning budget of AUTO and a limit on
+                                             the solve's device memory (BA_ERR_RESOURCE_EXHAUSTED) */
     int32_t precision;                    /* enum ba_precision */
     uint32_t pcg_max_iterations;          /* 0 = the native cap min(600, D) */
     double pcg_relative_tolerance;        /* 0 = the native forcing tolerance 1e-1 */
@@ -180,7 +182,48 @@ typedef struct ba_solve_report {
      * its refinement stalled, and the relative residual of the last refined step */
     uint32_t precision_fallbacks;
     double refinement_residual;           /* NaN when no refined step ran */
+    /* camera-step choice (SparseExecutionDecision, kernel/multi_view/sparse/api.rs) */
+    int32_t plan_reason;                  /* enum ba_plan_reason */
+    uint64_t workspace_bytes;             /* device memory the solve allocated (checked against an
+                                             explicit workspace_budget_bytes, native/mod.rs:557-570) */
+    /* SolveReport::inlier_count / final_inlier_rmse (solver.rs:585-608, 1353-1371): final residual
+     * norms within the robust-loss scale (every retained observation under squared loss) */
+    uint64_t inlier_count;
+    double final_inlier_rmse;             /* NaN when no observation is within the scale */
 } ba_solve_report;
+
+/* SparseExecutionReason (kernel/multi_view/sparse/api.rs) of the camera-step choice. */
+enum ba_plan_reason {
+    BA_PLAN_EXPLICIT_SOLVER = 0,          /* ExplicitBackend: linear_solver DIRECT or ITERATIVE */
+    BA_PLAN_DIRECT_FITS_BUDGET = 1,       /* DirectFitsBudget: AUTO picks the direct step */
+    BA_PLAN_DIRECT_EXCEEDS_BUDGET = 2,    /* DirectExceedsBudget: AUTO picks PCG, which fits */
+    BA_PLAN_LOWEST_MEMORY = 3             /* LowestMemoryBackend: neither fits; AUTO picks PCG */
+};
+
+/* The camera-step choice of LinearSolver::Auto (execution_policy.rs:190-227, estimates of
+ * workspace/self_calibration.rs:466-535): the direct step when the reference's direct-workspace
+ * estimate is at most 4/5 of the planning budget (workspace_budget_bytes, or 1 GiB when it is 0:
+ * SparseWorkspaceMemoryBudget::AUTO_BYTES, api.rs:43), else PCG.  The estimates restate
+ * estimated_direct_bytes (execution_policy.rs:98-120) with serial direct assembly and
+ * estimated_self_calibrating_retained_bytes (self_calibration_retained.rs:233-290) without its
+ * thread-count-dependent worker term; they size the reference's CPU workspace, which is what the
+ * policy compares, not this library's device memory (ba_solve_report.workspace_bytes). */
+typedef struct ba_execution_plan {
+    int32_t linear_solver;                /* BA_SOLVER_DIRECT or BA_SOLVER_ITERATIVE */
+    int32_t reason;                       /* enum ba_plan_reason */
+    uint64_t budget_bytes;                /* planning budget (0 for an explicit solver) */
+    uint64_t estimated_direct_bytes;      /* 0 for an explicit solver */
+    uint64_t estimated_retained_bytes;    /* 0 for an explicit solver */
+    uint64_t reduced_nonzeros;            /* lower-triangle nonzeros of S; 0 when the bounds
+                                             D(D+1)/2 and the diagonal blocks decided alone */
+    uint32_t reduced_dimension;
+    uint32_t _pad;
+} ba_execution_plan;
+
+/* The camera step ba_hip_solve would take for `problem` under `config` (host only, no device
+ * work).  Same errors as ba_hip_solve's validation and lowering. */
+int ba_hip_plan(const ba_problem* problem, const ba_solve_config* config, ba_execution_plan* plan,
+                char* message, size_t message_len);
 
 /* Optional per-iteration record: the LM trajectory a parity test compares. */
 typedef struct ba_iteration_trace {

@@ -23,6 +23,8 @@
 #include "ba_handle.hpp"
 #include "ba_host.hpp"
 #include "ba_kernels.hpp"
+#include "ba_ledger.hpp"
+#include "ba_policy.hpp"
 
 namespace ba {
 namespace {
@@ -39,6 +41,10 @@ struct HipError {
         if (_e != hipSuccess) throw HipError{_e, #expr}; \
     } while (0)
 
+// While a solve sets its engine up, the device bytes that Buf allocates are added here (the
+// workspace size that an explicit budget limits).
+inline thread_local uint64_t* g_device_tally = nullptr;
+
 template <class T>
 struct Buf {
     T* p = nullptr;
@@ -53,7 +59,10 @@ struct Buf {
         if (p) (void)hipFree(p);
         p = nullptr;
         n = count;
-        if (count) HIP_OK(hipMalloc(&p, count * sizeof(T)));
+        if (count) {
+            HIP_OK(hipMalloc(&p, count * sizeof(T)));
+            if (g_device_tally) *g_device_tally += count * sizeof(T);
+        }
     }
     void upload(const T* h, size_t count, hipStream_t s) {
         alloc(count);
@@ -65,6 +74,13 @@ struct Buf {
         alloc(count);
         if (v.size()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
         if (count > v.size()) HIP_OK(hipMemsetAsync(p + v.size(), 0, (count - v.size()) * sizeof(T), s));
+    }
+};
+
+struct HipEventOps {
+    void create(hipEvent_t& e) { HIP_OK(hipEventCreate(&e)); }
+    void destroy(hipEvent_t e) {
+        if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -117,10 +133,6 @@ class Engine {
     Engine(hipStream_t s, const Lowered& L, bool iterative, int profile = BA_PROFILE_NONE)
         : s_(s), L_(L), iterative_(iterative), profile_(profile) {}
     ~Engine() {
-        for (auto& e : events_) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
-        }
         for (hipEvent_t e : pcg_ev_)
             if (e) (void)hipEventDestroy(e);
     }
@@ -132,13 +144,7 @@ class Engine {
             launch();
             return;
         }
-        if (used_ == events_.size()) {
-            hipEvent_t a, b;
-            HIP_OK(hipEventCreate(&a));
-            HIP_OK(hipEventCreate(&b));
-            events_.emplace_back(a, b);
-        }
-        auto& e = events_[used_++];
+        auto& e = ledger_.next();
         HIP_OK(hipEventRecord(e.first, s_));
         launch();
         HIP_OK(hipEventRecord(e.second, s_));
@@ -147,16 +153,25 @@ class Engine {
     void profile_result(uint32_t* launches, double* seconds) {
         HIP_OK(hipStreamSynchronize(s_));
         double total = 0.0;
-        for (size_t k = 0; k < used_; ++k) {
+        for (size_t k = 0; k < ledger_.count(); ++k) {
             float ms = 0.0f;
-            HIP_OK(hipEventElapsedTime(&ms, events_[k].first, events_[k].second));
+            HIP_OK(hipEventElapsedTime(&ms, ledger_[k].first, ledger_[k].second));
             total += ms * 1e-3;
         }
-        *launches = (uint32_t)used_;
+        *launches = (uint32_t)ledger_.count();
         *seconds = total;
     }
 
+    // Device memory allocated by setup().
+    uint64_t device_bytes() const { return device_bytes_; }
+    // An explicit workspace budget: the optional explicit reduced system is set up only within it.
+    void set_budget(uint64_t bytes) { budget_ = bytes; }
+
     void setup() {
+        struct Tally {
+            explicit Tally(uint64_t* t) { g_device_tally = t; }
+            ~Tally() { g_device_tally = nullptr; }
+        } tally(&device_bytes_);
         B_ = build_block_lists(L_);
         PL_ = build_pair_lists(L_, B_, !iterative_);
         const size_t nc = L_.n_cameras(), ng = L_.n_groups(), np = L_.n_points(), no = L_.n_obs(), D = L_.D;
@@ -263,7 +278,6 @@ class Engine {
                 ncu_ = (uint32_t)std::max(cu, 1);
             }
             HIP_OK(hipMemsetAsync(pvcam_.p, 0, pvcam_.n * sizeof(double), s_));
-            setup_explicit();
             const size_t qparts = std::max<size_t>((B_.n_blocks + PCG_Q_WG / 64 - 1) / (PCG_Q_WG / 64),
                                                    nc);  // implicit or explicit product
             ppart_.alloc(std::max<size_t>(1, qparts) +
@@ -278,6 +292,7 @@ class Engine {
             HIP_OK(hipMemsetAsync(pxw_.p, 0, pxw_.n * sizeof(double), s_));
             pcg_.alloc(1);
             pcg1_.alloc(1);
+            setup_explicit();  // last: it is optional and fits in what the budget leaves
         }
         tp_.nc = (uint32_t)nc;
         tp_.ng = (uint32_t)ng;
@@ -330,6 +345,8 @@ class Engine {
             HIP_OK(hipMemGetInfo(&fr, &tot));
             if (bytes > 0.4 * (double)fr) return;
         }
+        // the rest of the workspace is allocated before this point
+        if (budget_ && (double)device_bytes_ + bytes > (double)budget_) return;
         if (terms >= (uint64_t)UINT32_MAX) return;
         std::vector<uint32_t> pos(no), cur(nc + 1, 0);
         for (size_t o = 0; o < no; ++o) cur[L_.obs_cam[o] + 1]++;
@@ -511,15 +528,8 @@ class Engine {
     void set_schur_product(int mode) { schur_mode_ = mode; }
     // Whether a PCG solve of this engine has used the explicit system.
     bool explicit_product() const { return used_explicit_; }
-    // The product of the next LM iteration (before its assembly): with the explicit system set
-    // up, EXPLICIT takes it always, AUTO when the PCG is expected to take at least
-    // EXPLICIT_MIN_PCG iterations (the previous iteration's count).  Its assembly costs about as
-    // much as ten matrix-free products save (2.1 ms against 0.21 ms per PCG iteration at the
-    // 2,000-camera scene), so solves that need only a few PCG iterations stay matrix-free.
-    static constexpr uint32_t EXPLICIT_MIN_PCG = 10;
-    void plan_product(uint32_t expected_pcg) {
-        use_explicit_ = explicit_ && (schur_mode_ == BA_SCHUR_EXPLICIT || expected_pcg >= EXPLICIT_MIN_PCG);
-    }
+    // The product of the next LM iteration (before its assembly): use_explicit_product.
+    void plan_product(uint32_t expected_pcg) { use_explicit_ = use_explicit_product(explicit_, schur_mode_, expected_pcg); }
     // Keep the pose columns of J~_c in the point-major copy (for the per-observation hook).
     void keep_pose_rows() { keep_pose_rows_ = true; }
     PointElim<T> pe() { return PointElim<T>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, prec_.p, pscal_.p}; }
@@ -660,14 +670,15 @@ class Engine {
         if (profiled_product) {
             // one batch at a time, so that launches after the state turned done (which return at
             // once) can be dropped from the timings by the state's product count
-            size_t events_before = used_;
+            size_t events_before = ledger_.count();
             int products_before = 0;
             for (;;) {
                 HIP_OK(hipMemcpyAsync(h_pcg_[0].p, pcg_.p, sizeof(PcgState), hipMemcpyDeviceToHost, s_));
                 HIP_OK(hipStreamSynchronize(s_));
-                used_ = events_before + (size_t)(h_pcg_[0].p->products - products_before);
+                const int applied = h_pcg_[0].p->products - products_before;
+                ledger_.keep(events_before + (size_t)std::max(applied, 0));
                 if (h_pcg_[0].p->done) break;
-                events_before = used_;
+                events_before = ledger_.count();
                 products_before = h_pcg_[0].p->products;
                 enqueue_batch();
             }
@@ -723,6 +734,20 @@ class Engine {
         out.c = *h_cand_.p;
         std::memcpy(out.dots, h_dots_, sizeof(out.dots));
         return out;
+    }
+
+    // Inlier count and RMSE of the residual norms within `limit` at the last linearisation
+    // (solver.rs:1353-1371); the RMSE is NaN when none is within it.
+    void inliers(double limit, uint64_t* count, double* rmse) {
+        double v[2] = {0.0, 0.0};
+        if (tp_.no) {
+            launch_inliers(s_, osq_.p, valid_.p, tp_.no, limit, partials_.p, dots_.p);
+            HIP_OK(hipMemcpyAsync(v, dots_.p, sizeof(v), hipMemcpyDeviceToHost, s_));
+            HIP_OK(hipStreamSynchronize(s_));
+        }
+        *count = (uint64_t)v[0];
+        *rmse = v[0] > 0.0 ? std::sqrt(v[1] / v[0]) : NAN;
+        if (!std::isfinite(*rmse)) *rmse = NAN;
     }
 
     void download(int state, std::vector<double>& R, std::vector<double>& t, std::vector<double>& gp,
@@ -813,8 +838,8 @@ class Engine {
     Buf<double> pvexp_, stpair_, sqrow_;
     bool pose_rows_ = true, keep_pose_rows_ = false;
     int profile_ = BA_PROFILE_NONE;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
-    size_t used_ = 0;
+    uint64_t device_bytes_ = 0, budget_ = 0;
+    LaunchLedger<hipEvent_t, HipEventOps> ledger_;  // event pairs of the profiled kernel's launches
 };
 
 }  // namespace

@@ -807,4 +807,111 @@ void raise_solution(const ba_problem* p, const Lowered& L, const double* R, cons
     }
 }
 
+// ---- LinearSolver::Auto (kernel/multi_view/sparse/execution_policy.rs:98-227,
+//      workspace/self_calibration.rs:466-535, self_calibration_retained.rs:233-290) ----
+namespace {
+// SparseWorkspaceMemoryBudget::AUTO_BYTES (kernel/multi_view/sparse/api.rs:43)
+constexpr uint64_t AUTO_BUDGET_BYTES = 1024ull * 1024ull * 1024ull;
+// direct_fits_budget: at most 4/5 of the limit, without overflow (execution_policy.rs:280-286)
+bool direct_fits(uint64_t estimate, uint64_t limit) {
+    const uint64_t guarded = limit / 5 * 4 + limit % 5 * 4 / 5;
+    return estimate <= guarded;
+}
+// Lower-triangle nonzeros of the reduced pattern (pattern.rs:381-491): every block pair that a
+// free point couples, plus each camera's pose-intrinsic pair for fixed points, counted element
+// by element.  Block pairs are marked in a bitmap of (blocks x blocks) bits.
+uint64_t reduced_nonzeros(const Lowered& L) {
+    const size_t P = L.n_cameras(), G = L.n_groups(), NB = P + G, NP = L.n_points();
+    auto bdim = [&](size_t b) -> uint64_t { return b < P ? L.cam_dim[b] : L.grp_dim[b - P]; };
+    const size_t words = (NB + 63) / 64;
+    std::vector<uint64_t> bits(words * NB, 0);
+    auto mark = [&](size_t row, size_t col) { bits[col * words + row / 64] |= 1ull << (row % 64); };
+    for (size_t b = 0; b < NB; ++b)
+        if (bdim(b)) mark(b, b);
+    std::vector<uint32_t> active;
+    for (size_t q = 0; q < NP; ++q) {
+        const uint32_t o0 = L.point_off[q], o1 = L.point_off[q + 1];
+        if (L.point_fixed[q]) {
+            for (uint32_t o = o0; o < o1; ++o) {
+                const uint32_t c = L.obs_cam[o], g = L.cam_group[c];
+                if (L.cam_dim[c] && L.grp_dim[g]) mark(P + g, c);
+            }
+            continue;
+        }
+        active.clear();
+        for (uint32_t o = o0; o < o1; ++o) {
+            const uint32_t c = L.obs_cam[o], g = L.cam_group[c];
+            if (L.cam_dim[c]) active.push_back(c);
+            if (L.grp_dim[g]) active.push_back((uint32_t)(P + g));
+        }
+        std::sort(active.begin(), active.end());
+        active.erase(std::unique(active.begin(), active.end()), active.end());
+        for (size_t i = 0; i < active.size(); ++i)
+            for (size_t j = 0; j <= i; ++j) mark(active[i], active[j]);
+    }
+    uint64_t count = 0;
+    for (size_t col = 0; col < NB; ++col) {
+        const uint64_t dc = bdim(col);
+        if (!dc) continue;
+        for (size_t w = 0; w < words; ++w) {
+            uint64_t m = bits[col * words + w];
+            while (m) {
+                const size_t row = w * 64 + (size_t)__builtin_ctzll(m);
+                m &= m - 1;
+                count += row == col ? dc * (dc + 1) / 2 : bdim(row) * dc;
+            }
+        }
+    }
+    return count;
+}
+}  // namespace
+
+void plan_execution(const Lowered& L, const ba_solve_config* c, ba_execution_plan& out) {
+    std::memset(&out, 0, sizeof(out));
+    out.reduced_dimension = (uint32_t)L.D;
+    if (c->linear_solver != BA_SOLVER_AUTO) {  // explicit_decision (execution_policy.rs:252-268)
+        out.linear_solver = c->linear_solver;
+        out.reason = BA_PLAN_EXPLICIT_SOLVER;
+        return;
+    }
+    const uint64_t P = L.n_cameras(), G = L.n_groups(), NP = L.n_points(), D = L.D;
+    const uint64_t cross = L.params.size();
+    const uint64_t limit = c->workspace_budget_bytes ? c->workspace_budget_bytes : AUTO_BUDGET_BYTES;
+    out.budget_bytes = limit;
+    // estimated_point_cache_bytes (execution_policy.rs:68-96): usize offsets, no retained point
+    // steps (serial direct assembly), one [f64; 3] row per point parameter
+    const uint64_t direct_points = (NP + 1) * 8 + cross * 24;
+    // estimated_direct_bytes (execution_policy.rs:98-120): 4 x 8 bytes per lower nonzero
+    auto direct_bytes = [&](uint64_t nnz) { return nnz * 8 * 4 + direct_points; };
+    uint64_t diagonal_nnz = 0;
+    for (size_t k = 0; k < P; ++k) diagonal_nnz += (uint64_t)L.cam_dim[k] * (L.cam_dim[k] + 1) / 2;
+    for (size_t g = 0; g < G; ++g) diagonal_nnz += (uint64_t)L.grp_dim[g] * (L.grp_dim[g] + 1) / 2;
+    const uint64_t lower = direct_bytes(diagonal_nnz), upper = direct_bytes(D * (D + 1) / 2);
+    if (!direct_fits(lower, limit)) {
+        out.estimated_direct_bytes = lower;  // at least this much: the exact count cannot fit
+    } else if (direct_fits(upper, limit)) {
+        out.estimated_direct_bytes = upper;  // at most this much: the exact count fits
+    } else {
+        out.reduced_nonzeros = reduced_nonzeros(L);
+        out.estimated_direct_bytes = direct_bytes(out.reduced_nonzeros);
+    }
+    // estimated_self_calibrating_retained_bytes (self_calibration_retained.rs:233-290) without the
+    // worker term: per block two packed 6x6 blocks and a factor (2*168 + 216 bytes) and three
+    // usize; per pose-intrinsic edge 160 bytes; per point its packed 3x3 block (48 bytes); per
+    // point parameter a u32; per dimension five f64 and a usize; and the retained point cache
+    // (offsets, point steps of 1 + 48 bytes, [f64; 3] rows)
+    uint64_t edges = 0;
+    for (size_t k = 0; k < P; ++k) edges += (L.cam_dim[k] > 0 && L.grp_dim[L.cam_group[k]] > 0) ? 1 : 0;
+    const uint64_t NB = P + G;
+    out.estimated_retained_bytes = NB * (2 * 168 + 216) + edges * 160 + NB * 3 * 8 + NP * 48 + cross * 4 +
+                                   D * (5 * 8 + 8) + (NP + 1) * 8 + NP * 49 + cross * 24;
+    if (direct_fits(out.estimated_direct_bytes, limit)) {
+        out.linear_solver = BA_SOLVER_DIRECT;
+        out.reason = BA_PLAN_DIRECT_FITS_BUDGET;
+    } else {
+        out.linear_solver = BA_SOLVER_ITERATIVE;
+        out.reason = out.estimated_retained_bytes <= limit ? BA_PLAN_DIRECT_EXCEEDS_BUDGET : BA_PLAN_LOWEST_MEMORY;
+    }
+}
+
 }  // namespace ba

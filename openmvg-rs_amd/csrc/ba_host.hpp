@@ -55,6 +55,9 @@ Status validate_problem(const ba_problem* p);
 Status validate_config(const ba_solve_config* c);
 Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern);
 
+// LinearSolver::Auto's camera-step choice for a lowered problem (execution_policy.rs:190-227).
+void plan_execution(const Lowered& L, const ba_solve_config* c, ba_execution_plan& out);
+
 // Block b < n_cameras is the pose block of camera b; block n_cameras + g the intrinsic block
 // of group g.  Rows of a per-observation 9-row camera Jacobian: pose rows 0..5, intrinsic 6..8.
 struct BlockLists {

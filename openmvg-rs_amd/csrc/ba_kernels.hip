@@ -691,6 +691,45 @@ __global__ void __launch_bounds__(NT) k_reduce_points_1(const double* __restrict
     if (threadIdx.x == 0)
         for (int k = 0; k < 8; ++k) partials[8 * blockIdx.x + k] = v[k];
 }
+// Inlier statistics of the final state (solver.rs:1353-1371): per observation with a valid
+// linearisation, its residual norm |r| counts when |r| <= limit; the squared norms of those are
+// summed.  Fixed partition into RED_BLOCKS ranges, each summed in a fixed order.
+__global__ void __launch_bounds__(NT) k_inliers_1(const double* __restrict__ sq, const uint8_t* __restrict__ valid,
+                                                  uint32_t no, double limit, double* partials) {
+    __shared__ double sh[NT];
+    const uint32_t b0 = (uint32_t)((uint64_t)no * blockIdx.x / RED_BLOCKS);
+    const uint32_t b1 = (uint32_t)((uint64_t)no * (blockIdx.x + 1) / RED_BLOCKS);
+    double count = 0.0, sum = 0.0;
+    for (uint32_t o = b0 + threadIdx.x; o < b1; o += NT) {
+        if (!valid[o]) continue;
+        const double norm = sqrt(sq[o]);
+        if (norm <= limit) {
+            count += 1.0;
+            sum += norm * norm;
+        }
+    }
+    count = tree_sum<NT>(count, sh);
+    sum = tree_sum<NT>(sum, sh);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = count;
+        partials[2 * blockIdx.x + 1] = sum;
+    }
+}
+__global__ void __launch_bounds__(NT) k_inliers_2(const double* __restrict__ partials, double* out) {
+    __shared__ double sh[NT];
+    double count = 0.0, sum = 0.0;
+    for (int b = threadIdx.x; b < RED_BLOCKS; b += NT) {
+        count += partials[2 * b];
+        sum += partials[2 * b + 1];
+    }
+    count = tree_sum<NT>(count, sh);
+    sum = tree_sum<NT>(sum, sh);
+    if (threadIdx.x == 0) {
+        out[0] = count;
+        out[1] = sum;
+    }
+}
+
 __global__ void __launch_bounds__(NT) k_reduce_points_2(const double* __restrict__ partials, AsmScalars* out) {
     __shared__ double sh[NT];
     double a[8] = {0, 0, 0, 0, 1.0, 0, 0, 0};
@@ -2437,6 +2476,11 @@ template <class T>
 void launch_point_elim(hipStream_t s, const Topo& tp, const Lin<T> lin, T lambda, bool want_rcond, PointElim<T> out) {
     if (tp.nchunks == 0) return;
     hipLaunchKernelGGL(k_point_elim<T>, dim3(tp.nchunks), dim3(NT), 0, s, tp, lin, lambda, (int)want_rcond, out);
+}
+void launch_inliers(hipStream_t s, const double* sq, const uint8_t* valid, uint32_t no, double limit, double* partials,
+                    double* out) {
+    hipLaunchKernelGGL(k_inliers_1, dim3(RED_BLOCKS), dim3(NT), 0, s, sq, valid, no, limit, partials);
+    hipLaunchKernelGGL(k_inliers_2, dim3(1), dim3(NT), 0, s, partials, out);
 }
 void launch_reduce_points(hipStream_t s, const double* pscal, uint32_t np, double* partials, AsmScalars* out) {
     hipLaunchKernelGGL(k_reduce_points_1, dim3(RED_BLOCKS), dim3(NT), 0, s, pscal, np, partials);

@@ -131,6 +131,10 @@ template <class T>
 void launch_point_elim(hipStream_t s, const Topo& tp, const Lin<T> lin, T lambda, bool want_rcond,
                        PointElim<T> out);
 // Sum of per-point scalars in a fixed order -> AsmScalars fields cost..invalid.
+// Inlier count and squared-norm sum of the residuals within `limit` (out: 2 doubles); partials
+// holds 2 * 512 doubles.
+void launch_inliers(hipStream_t s, const double* sq, const uint8_t* valid, uint32_t no, double limit, double* partials,
+                    double* out);
 void launch_reduce_points(hipStream_t s, const double* pscal, uint32_t np, double* partials,
                           AsmScalars* out);
 // ---- explicit reduced camera system (same topologies as the camera-side assembly) ----

@@ -334,17 +334,31 @@ namespace ba {
 namespace {
 template <class T>
 Status run_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* cfg, const Lowered& L,
-                 ba_solution* sol, ba_solve_report& r, ba_iteration_trace* trace, double t0) {
-    const bool iterative = cfg->linear_solver == BA_SOLVER_ITERATIVE;
+                 const ba_execution_plan& plan, ba_solution* sol, ba_solve_report& r, ba_iteration_trace* trace,
+                 double t0) {
+    const bool iterative = plan.linear_solver == BA_SOLVER_ITERATIVE;
     Engine<T> E(h->stream, L, iterative, cfg->profile_kernel);
     E.set_schur_product(h->schur_product);
+    E.set_budget(cfg->workspace_budget_bytes);
     E.setup();
+    // check_workspace_budget (native/mod.rs:557-570): an explicit budget refuses a larger
+    // workspace before the first iteration; the automatic budget only guides the plan
+    if (cfg->workspace_budget_bytes && E.device_bytes() > cfg->workspace_budget_bytes)
+        return error(BA_ERR_RESOURCE_EXHAUSTED,
+                     "native self-calibration workspace " + std::to_string(E.device_bytes()) +
+                         " bytes exceeds the workspace budget " + std::to_string(cfg->workspace_budget_bytes));
     double t1 = now();
     int state = 0;
     double loop_seconds = 0.0;
     Status st = refine(E, L, cfg, r, trace, &state, &loop_seconds);
     if (iterative) h->last_schur_product = E.explicit_product() ? BA_SCHUR_EXPLICIT : BA_SCHUR_IMPLICIT;
     if (!st.ok()) return st;
+    {  // inlier statistics at the final linearisation (refine's zero-damping assembly)
+        const double limit = p->robust_loss == BA_LOSS_SQUARED ? INFINITY : p->loss_scale;
+        E.inliers(limit, &r.inlier_count, &r.final_inlier_rmse);
+    }
+    r.plan_reason = plan.reason;
+    r.workspace_bytes = E.device_bytes();
     std::vector<double> R, t, gp, X;
     E.download(state, R, t, gp, X);
     raise_solution(p, L, R.data(), t.data(), gp.data(), X.data(), sol);
@@ -444,6 +458,22 @@ int ba_hip_prepare(const ba_problem* p, ba_native_sizes* sizes, ba_native_proble
 }
 
 
+int ba_hip_plan(const ba_problem* p, const ba_solve_config* cfg, ba_execution_plan* plan, char* msg, size_t len) {
+    if (!cfg || !plan) return finish(error(BA_ERR_INVALID_CONFIG, "null argument"), msg, len);
+    Status st = validate_config(cfg);
+    if (st.ok()) st = validate_problem(p);
+    if (!st.ok()) return finish(st, msg, len);
+    try {
+        Lowered L;
+        st = lower_problem(p, L, false);
+        if (!st.ok()) return finish(st, msg, len);
+        plan_execution(L, cfg, *plan);
+        return BA_OK;
+    } catch (const std::bad_alloc&) {
+        return finish(error(BA_ERR_RESOURCE_EXHAUSTED, "host allocation failed"), msg, len);
+    }
+}
+
 int ba_hip_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* cfg, ba_solution* sol,
                  ba_solve_report* rep, ba_iteration_trace* trace, char* msg, size_t len) {
     if (!h || !cfg || !sol) return finish(error(BA_ERR_INVALID_CONFIG, "null argument"), msg, len);
@@ -457,9 +487,11 @@ int ba_hip_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* c
         st = lower_problem(p, L, false);
         if (!st.ok()) return finish(st, msg, len);
         if (trace) trace->count = 0;
+        ba_execution_plan plan;
+        plan_execution(L, cfg, plan);
         ba_solve_report r;
-        st = cfg->precision == BA_PRECISION_MIXED ? run_solve<float>(h, p, cfg, L, sol, r, trace, t0)
-                                                  : run_solve<double>(h, p, cfg, L, sol, r, trace, t0);
+        st = cfg->precision == BA_PRECISION_MIXED ? run_solve<float>(h, p, cfg, L, plan, sol, r, trace, t0)
+                                                  : run_solve<double>(h, p, cfg, L, plan, sol, r, trace, t0);
         if (!st.ok()) return finish(st, msg, len);
         if (rep) *rep = r;
         return BA_OK;
@@ -471,10 +503,14 @@ int ba_hip_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* c
     }
 }
 
+// The hooks read and write the caller's device buffers on the handle's stream, which does not
+// synchronise with other streams: they first wait for all work queued on the device (the caller's
+// writes to their inputs and outputs), and end with the handle's stream drained.
 #define HOOK_BEGIN                                                       \
     if (!h) return BA_ERR_INVALID_CONFIG;                                \
     try {                                                                \
-        if (hipSetDevice(h->device) != hipSuccess) return BA_ERR_INTERNAL;
+        if (hipSetDevice(h->device) != hipSuccess) return BA_ERR_INTERNAL; \
+        if (hipDeviceSynchronize() != hipSuccess) return BA_ERR_INTERNAL;
 #define HOOK_END                                                         \
     }                                                                    \
     catch (const HipError&) {                                            \

@@ -141,6 +141,24 @@ class SolveReportC(C.Structure):
         ("seconds_setup", C.c_double), ("seconds_iterations", C.c_double), ("seconds_final", C.c_double),
         ("profile_launches", C.c_uint32), ("profile_seconds", C.c_double),
         ("precision_fallbacks", C.c_uint32), ("refinement_residual", C.c_double),
+        ("plan_reason", C.c_int32), ("workspace_bytes", C.c_uint64),
+        ("inlier_count", C.c_uint64), ("final_inlier_rmse", C.c_double),
+    ]
+
+
+# ba_plan_reason (SparseExecutionReason)
+PLAN_EXPLICIT_SOLVER = 0
+PLAN_DIRECT_FITS_BUDGET = 1
+PLAN_DIRECT_EXCEEDS_BUDGET = 2
+PLAN_LOWEST_MEMORY = 3
+
+
+class ExecutionPlanC(C.Structure):
+    _fields_ = [
+        ("linear_solver", C.c_int32), ("reason", C.c_int32),
+        ("budget_bytes", C.c_uint64), ("estimated_direct_bytes", C.c_uint64),
+        ("estimated_retained_bytes", C.c_uint64), ("reduced_nonzeros", C.c_uint64),
+        ("reduced_dimension", C.c_uint32), ("_pad", C.c_uint32),
     ]
 
 

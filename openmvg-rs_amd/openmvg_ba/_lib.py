@@ -34,6 +34,7 @@ def load() -> C.CDLL:
     lib.ba_hip_destroy.restype = None
     lib.ba_hip_solve.argtypes = [vp] + [vp] * 5 + [C.c_char_p, C.c_size_t]
     lib.ba_hip_prepare.argtypes = [vp, vp, vp, C.c_char_p, C.c_size_t]
+    lib.ba_hip_plan.argtypes = [vp, vp, vp, C.c_char_p, C.c_size_t]
     lib.ba_hip_linearize.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     lib.ba_hip_schur.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp]
     lib.ba_hip_pcg.argtypes = [vp, vp, C.c_double, C.c_double, C.c_uint32, vp, vp, vp, vp]

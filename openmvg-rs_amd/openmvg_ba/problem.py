@@ -309,6 +309,19 @@ def solve_with(lib, function: str, problem: BundleProblem, config: SolveConfig, 
                     out["point_quarantined"], report, trace)
 
 
+def plan_with(lib, function: str, problem: BundleProblem, config: SolveConfig) -> dict:
+    """The camera-step choice of a `ba_hip_plan`-shaped entry point (LinearSolver::Auto's
+    execution_policy.rs:190-227) as a dict of the ba_execution_plan fields."""
+    cp, keep = problem.to_c()
+    cfg = config.to_c()
+    plan = A.ExecutionPlanC()
+    msg = A.message_buffer()
+    code = getattr(lib, function)(C.byref(cp), C.byref(cfg), C.byref(plan), msg, len(msg))
+    del keep
+    raise_for(code, msg.value)
+    return {name: getattr(plan, name) for name, _ in A.ExecutionPlanC._fields_ if name != "_pad"}
+
+
 @dataclass
 class NativeProblem:
     """The lowered problem and index maps that ba_hip_prepare / bao_prepare return."""
@@ -470,6 +483,10 @@ class HipBundleAdjuster:
 
     def prepare(self, problem: BundleProblem) -> NativeProblem:
         return prepare_with(self._handle.lib, "ba_hip_prepare", problem)
+
+    def plan(self, problem: BundleProblem, config: SolveConfig | None = None) -> dict:
+        """The camera step `solve` would take (host only)."""
+        return plan_with(self._handle.lib, "ba_hip_plan", problem, config or SolveConfig())
 
     def solve_batched(self, problems: list, config: SolveConfig | None = None, mode: int = A.BATCH_GRAPH,
                       trace_capacity=None):

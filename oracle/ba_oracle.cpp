@@ -2313,6 +2313,77 @@ static void raise_solution(const ba_problem* p, const Lowered& L, const State& s
     }
 }
 
+// LinearSolver::Auto: kernel/multi_view/sparse/execution_policy.rs:190-227 and 280-286, with the
+// estimates of workspace/self_calibration.rs:466-535 (estimated_direct_bytes with a serial direct
+// assembly: execution_policy.rs:68-120; estimated_self_calibrating_retained_bytes without its
+// worker term: self_calibration_retained.rs:233-290).  The lower nonzero count is the length of
+// the reduced pattern (pattern.rs:441-468), which lower() builds.
+static void plan_solver(const Lowered& L, const ba_solve_config* c, ba_execution_plan& out) {
+    std::memset(&out, 0, sizeof(out));
+    out.reduced_dimension = (uint32_t)L.D;
+    if (c->linear_solver != BA_SOLVER_AUTO) {
+        out.linear_solver = c->linear_solver;
+        out.reason = BA_PLAN_EXPLICIT_SOLVER;
+        return;
+    }
+    const uint64_t limit = c->workspace_budget_bytes ? c->workspace_budget_bytes : (uint64_t)1 << 30;
+    const uint64_t np = L.kept_points.size(), cross = L.params.size(), D = L.D;
+    const uint64_t nnz = L.row_idx.size();
+    const uint64_t point_cache = (np + 1) * sizeof(uint64_t) + cross * 3 * sizeof(double);
+    out.budget_bytes = limit;
+    out.reduced_nonzeros = nnz;
+    out.estimated_direct_bytes = nnz * sizeof(double) * 4 + point_cache;
+    const uint64_t blocks = L.cam_dim.size() + L.grp_dim.size();
+    uint64_t edges = 0;
+    for (size_t k = 0; k < L.cam_dim.size(); ++k) edges += L.cam_dim[k] > 0 && L.grp_dim[L.cam_group[k]] > 0;
+    const uint64_t packed_block = 21 * 8, block_factor = (21 + 6) * 8, edge = 2 * 8 + 18 * 8;
+    out.estimated_retained_bytes = blocks * (2 * packed_block + block_factor) + edges * edge + blocks * 3 * 8 +
+                                   np * 6 * 8 + cross * 4 + D * (5 * 8 + 8) + (np + 1) * 8 + np * (1 + 6 * 8) +
+                                   cross * 3 * 8;
+    const uint64_t guarded = limit / 5 * 4 + limit % 5 * 4 / 5;
+    if (out.estimated_direct_bytes <= guarded) {
+        out.linear_solver = BA_SOLVER_DIRECT;
+        out.reason = BA_PLAN_DIRECT_FITS_BUDGET;
+    } else {
+        out.linear_solver = BA_SOLVER_ITERATIVE;
+        out.reason = out.estimated_retained_bytes <= limit ? BA_PLAN_DIRECT_EXCEEDS_BUDGET : BA_PLAN_LOWEST_MEMORY;
+    }
+}
+
+// inlier_metrics / visit_residual_norms / residual_norm (solver.rs:1353-1430): pixel residual
+// norms of the returned state in the caller's frame, over the observations of kept points.
+static void inlier_metrics(const ba_problem* p, const Lowered& L, const ba_solution* s, uint64_t* count,
+                           double* rmse) {
+    const double limit = p->robust_loss == BA_LOSS_SQUARED ? INFINITY : p->loss_scale;
+    uint64_t n = 0;
+    double sum = 0.0;
+    for (size_t o = 0; o < p->n_observations; ++o) {
+        const size_t q = p->obs_point[o], i = p->obs_pose[o], g = p->obs_intrinsic[o];
+        if (L.quarantined[q]) continue;
+        const double* R = s->pose_rotation + 9 * i;
+        const double* C = s->pose_center + 3 * i;
+        const double* X = s->point_xyz + 3 * q;
+        const double d[3] = {X[0] - C[0], X[1] - C[1], X[2] - C[2]};
+        double xc[3];
+        for (int r = 0; r < 3; ++r) xc[r] = R[3 * r] * d[0] + R[3 * r + 1] * d[1] + R[3 * r + 2] * d[2];
+        if (!(xc[2] > 0.0)) continue;
+        const double nx = xc[0] / xc[2], ny = xc[1] / xc[2], r2 = nx * nx + ny * ny;
+        const ba_intrinsic& in = p->intrinsics[g];
+        const double dist = 1.0 + s->intrinsic_k1[g] * r2 + s->intrinsic_k2[g] * r2 * r2;
+        const double f = s->intrinsic_focal[g];
+        const double ex = f * dist * nx + in.cx - p->obs_pixel[2 * o];
+        const double ey = f * dist * ny + in.cy - p->obs_pixel[2 * o + 1];
+        const double norm = std::sqrt(ex * ex + ey * ey);
+        if (norm <= limit) {
+            n += 1;
+            sum += norm * norm;
+        }
+    }
+    *count = n;
+    *rmse = n > 0 ? std::sqrt(sum / (double)n) : NaN;
+    if (!std::isfinite(*rmse)) *rmse = NaN;
+}
+
 static void copy_msg(const Err& e, char* msg, size_t len) {
     if (msg && len) {
         std::snprintf(msg, len, "%s", e.msg.c_str());
@@ -2402,8 +2473,12 @@ int bao_solve(const ba_problem* p, const ba_solve_config* cfg, ba_solution* sol,
     }
     if (trace) trace->count = 0;
     State st{L.R, L.t, L.gp, L.X};
+    ba_execution_plan plan;
+    plan_solver(L, cfg, plan);
+    ba_solve_config planned = *cfg;
+    planned.linear_solver = plan.linear_solver;
     ba_solve_report r;
-    e = refine(L, cfg, st, r, trace);
+    e = refine(L, &planned, st, r, trace);
     if (e.code) {
         copy_msg(e, msg, len);
         return e.code;
@@ -2411,8 +2486,23 @@ int bao_solve(const ba_problem* p, const ba_solve_config* cfg, ba_solution* sol,
     size_t nq = 0;
     for (auto q : L.quarantined) nq += q;
     r.quarantined_points = (uint32_t)nq;
-    if (rep) *rep = r;
+    r.plan_reason = plan.reason;
     raise_solution(p, L, st, sol);
+    inlier_metrics(p, L, sol, &r.inlier_count, &r.final_inlier_rmse);
+    if (rep) *rep = r;
+    return BA_OK;
+}
+
+int bao_plan(const ba_problem* p, const ba_solve_config* cfg, ba_execution_plan* plan, char* msg, size_t len) {
+    Err e = validate_config(cfg);
+    if (!e.code) e = validate(p);
+    Lowered L;
+    if (!e.code) e = lower(p, L);
+    if (e.code) {
+        copy_msg(e, msg, len);
+        return e.code;
+    }
+    plan_solver(L, cfg, *plan);
     return BA_OK;
 }
 

@@ -38,6 +38,10 @@ int bao_solve(const ba_problem* problem, const ba_solve_config* config, ba_solut
               ba_solve_report* report, ba_iteration_trace* trace, char* message,
               size_t message_len);
 
+/* LinearSolver::Auto's camera-step choice, same contract as ba_hip_plan. */
+int bao_plan(const ba_problem* problem, const ba_solve_config* config, ba_execution_plan* plan,
+             char* message, size_t message_len);
+
 /* Lowering and index maps, same contract as ba_hip_prepare. */
 int bao_prepare(const ba_problem* problem, ba_native_sizes* sizes, ba_native_problem* native,
                 char* message, size_t message_len);

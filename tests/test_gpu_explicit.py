@@ -64,8 +64,10 @@ def test_mode_is_validated_and_reported(adjusters, ring):
     assert (_last(ex), _last(im)) == (A.SCHUR_EXPLICIT, A.SCHUR_IMPLICIT)
     auto = HipBundleAdjuster(0)
     assert _last(auto) == A.SCHUR_AUTO
-    auto.solve(ring[0].problem, cfg)
-    assert _last(auto) in (A.SCHUR_EXPLICIT, A.SCHUR_IMPLICIT)  # by the PCG iteration counts
+    got = auto.solve(ring[0].problem, cfg)
+    # AUTO: explicit in an iteration that follows a PCG solve of 10 or more iterations
+    pcg = got.trace["pcg_iterations"]
+    assert _last(auto) == (A.SCHUR_EXPLICIT if np.any(pcg[:-1] >= 10) else A.SCHUR_IMPLICIT)
 
 
 @pytest.mark.parametrize("lam", [1e-4, 0.1])
@@ -166,11 +168,14 @@ def test_auto_switches_per_iteration(adjusters):
     iterations and the matrix-free product otherwise; its trajectory matches the forced forms'
     (per-iteration cost within 1e-10, parameters within 1e-7 as on the medium scene)."""
     sc = synthetic.ring_scene(seed=0xBA0011, n_cameras=300, n_points=20_000)
-    cfg = SolveConfig(max_iterations=6, linear_solver=A.SOLVER_ITERATIVE)
+    cfg = SolveConfig(max_iterations=8, linear_solver=A.SOLVER_ITERATIVE)
     auto = HipBundleAdjuster(0)
     got = auto.solve(sc.problem, cfg)
     pcg = got.trace["pcg_iterations"]
-    assert _last(auto) == (A.SCHUR_EXPLICIT if np.any(pcg[:-1] >= 10) else A.SCHUR_IMPLICIT)
+    # the first iteration is matrix-free (no solve yet); some later one follows a solve of 10 or
+    # more PCG iterations, so both forms ran
+    assert np.any(pcg[:-1] >= 10), list(pcg)
+    assert _last(auto) == A.SCHUR_EXPLICIT
     print(f"AUTO: PCG iterations per LM iteration {list(pcg)}, reported form {_last(auto)}")
     for ba in adjusters:
         _compare_solutions(got, ba.solve(sc.problem, cfg), tol_param=1e-7)

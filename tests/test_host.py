@@ -50,7 +50,7 @@ STRUCTS = {
     "ba_intrinsic": A.Intrinsic, "ba_problem": A.Problem, "ba_solve_config": A.SolveConfigC,
     "ba_solve_report": A.SolveReportC, "ba_iteration_trace": A.IterationTrace, "ba_solution": A.SolutionC,
     "ba_native_sizes": A.NativeSizes, "ba_native_problem": A.NativeProblem, "ba_device_problem": A.DeviceProblem,
-    "ba_schur_scalars": A.SchurScalars, "ba_batch_trace": A.BatchTrace,
+    "ba_schur_scalars": A.SchurScalars, "ba_batch_trace": A.BatchTrace, "ba_execution_plan": A.ExecutionPlanC,
 }
 
 
