@@ -278,6 +278,62 @@ def window_scenes(seed: int = 0xBA0004, n_windows: int = 512, cameras=(5, 40), p
     return [grid_scene(seed + 1 + w, m, 1, npts, track="window") for w, (m, npts) in enumerate(sizes)]
 
 
+def candidate_rule_scene(seed: int = 0xBA0001):
+    """The ring scene plus two points that drive the candidate rule of
+    crates/openmvg-ba/src/kernel/point_geometry.rs:153-168 into its other branches.
+
+    * A point 0.01 behind camera 3's true centre on its optical axis, seen by camera 3 at the
+      principal point and by cameras 7, 8 and 9, which are fixed at their true poses.  Camera 3
+      starts 0.05 behind its true centre (the point 0.04 in front of it); a step that carries the
+      camera past the point leaves the point behind the candidate camera both at its stepped and
+      at its previous position while the current cameras still project it, so the step is
+      rejected; otherwise the stepped point is degenerate (nearly on a camera centre) and the
+      point is held at its previous position.
+    * A far point seen by cameras 0 and 1 at the direction of their bisecting axis at infinity,
+      placed where the parallax sine is 1.75e-5 (just above TrackGeometry's 1.7e-5).
+
+    Returns (problem, index of the near point, index of the far point)."""
+    sc = ring_scene(seed)
+    p = sc.problem
+    Rt, Ct = sc.truth_rotation, sc.truth_center
+    rot, cen = p.pose_rotation.copy(), p.pose_center.copy()
+    pts, op, opt, pix = list(p.point_xyz), list(p.obs_pose), list(p.obs_point), list(p.obs_pixel)
+    var = np.ones(len(rot), dtype=np.uint8)
+    c, opposite = 3, (7, 8, 9)
+    axis = Rt[c][2]
+    near = Ct[c] - 0.01 * axis
+    cen[c], rot[c] = Ct[c] - 0.05 * axis, Rt[c].copy()
+    for k in opposite:
+        var[k] = 0
+        rot[k], cen[k] = Rt[k], Ct[k]
+    q_near = len(pts)
+    pts.append(near.copy())
+    for cam in (c,) + opposite:
+        if cam == c:
+            uv = np.array([CX, CY])
+        else:
+            pc = Rt[cam] @ (near - Ct[cam])
+            uv = np.array([1000.0 * pc[0] / pc[2] + CX, 1000.0 * pc[1] / pc[2] + CY])
+        op.append(cam)
+        opt.append(q_near)
+        pix.append(uv)
+    a, b = 0, 1
+    u = Rt[a][2] + Rt[b][2]
+    u /= np.linalg.norm(u)
+    far = 0.5 * (Ct[a] + Ct[b]) + np.linalg.norm(Ct[a] - Ct[b]) / 1.75e-5 * u
+    q_far = len(pts)
+    pts.append(far)
+    for cam in (a, b):
+        pc = Rt[cam] @ u
+        op.append(cam)
+        opt.append(q_far)
+        pix.append(np.array([1000.0 * pc[0] / pc[2] + CX, 1000.0 * pc[1] / pc[2] + CY]))
+    prob = BundleProblem(pose_rotation=rot, pose_center=cen, intrinsics=p.intrinsics, point_xyz=np.array(pts),
+                         obs_pose=np.array(op), obs_intrinsic=np.array(op), obs_point=np.array(opt),
+                         obs_pixel=np.array(pix), pose_var_rotation=var, pose_var_translation=var.copy())
+    return prob, q_near, q_far
+
+
 def medium_scene(seed: int = 0xBA0002) -> Scene:
     """200 cameras, 50,000 points, ~300k observations, (f, k1, k2) free, squared loss."""
     return grid_scene(seed, 20, 10, 50_000)

@@ -9,20 +9,32 @@
  * The problem, solution and report structs are the product's boundary types
  * (include/ba_hip.h) so that one set of caller buffers feeds both implementations.
  *
- * Parity pins (what fixes each function independently of this file; see tests/):
+ * Parity pins (what fixes each function independently of this file; the test that does it):
  *   radial model      closed form; central differences and a dual-number oracle
- *                     (self_calibration/model.rs:396-467, 598-634); golden initial cost
- *                     863.6448936532983 of tests/self_calibration.rs:135
- *   robust losses     derivative identities (numerics/robust.rs:50-131)
+ *                     (self_calibration/model.rs:396-467, 598-634): test_oracle_pins.py
+ *   cost evaluation   golden initial cost 863.6448936532983 of tests/self_calibration.rs:135:
+ *                     test_host.py
+ *   robust losses     derivative identities (numerics/robust.rs:50-131): test_oracle_pins.py
  *   point block       known inverse Q diag(1e6,1,1e-3) Q^T (point_block.rs:237-266)
  *   Nielsen/run state tables of controller.rs:93-115 and run_state.rs:187-350
  *   stages            continuation.rs:81-111
- *   index maps        topology example (workspace/topology.rs:611-629) and brute force
- *   Schur assembly    brute-force dense normal equations of the full Jacobian
- *   camera solve      LAPACK (numpy/scipy) Cholesky of the same S
- *   PCG               direct solve of the materialised S to the PCG tolerance
- *   LM driver         truth recovery on the noise-free fixtures of
+ *   index maps        topology example (workspace/topology.rs:611-629), product vs oracle
+ *                     lowering bit for bit: test_host.py
+ *   Jacobians         central differences of the residuals along the reference's retraction
+ *                     coordinates, the FixedBaseline S^2 chart included:
+ *                     test_oracle_assembly.py
+ *   Schur assembly    dense normal equations of the full Jacobian in numpy with the reference's
+ *                     damping clamps and numpy's point-block inverses (S, g^, gradient,
+ *                     diagonal to 1e-10): test_oracle_assembly.py
+ *   camera solve      LAPACK (numpy/scipy) Cholesky: test_oracle_pins.py
+ *   PCG               at rtol 1e-10 equals numpy's solve of the dense-normal-equation S on scenes
+ *                     scaled by 1e-6, 1, 1e6: test_oracle_assembly.py
+ *   candidate rule    a scene that holds a point and rejects steps: test_candidate_scene.py
+ *                     (its branch decisions are otherwise checked only against the product)
+ *   LM driver         truth recovery on the noise-free fixture of
  *                     tests/sparse_self_calibration/main.rs:48-135 (final rmse < 1e-6)
+ *   Auto policy       bao_plan: the reference's rule and estimate formulas, and the pattern
+ *                     count against the product's independent count: test_plan.py
  */
 #ifndef OPENMVG_BA_ORACLE_H
 #define OPENMVG_BA_ORACLE_H

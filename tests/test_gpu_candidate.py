@@ -1,0 +1,64 @@
+"""HIP path against the oracle on scenes that exercise the candidate rule's held and reject-step
+branches (synthetic.candidate_rule_scene: point_geometry.rs:153-168, candidate.rs:25-233) and the
+Cauchy loss's continuation stages (continuation.rs:26-74: min(max_iterations/4, 5) squared-loss
+iterations, then Cauchy), on an MI355X.
+
+Bars as the plain form's: the same accept/reject sequence, the same number of candidate
+rejections, per-iteration cost within 1e-10 relative, final parameters within 1e-8."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from openmvg_ba import SolveConfig, _abi as A, synthetic
+from openmvg_ba.problem import HipBundleAdjuster, solve_with
+from oracle_lib import load_oracle
+from test_gpu_parity import _compare_solutions
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def adjuster():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return HipBundleAdjuster(0)
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return load_oracle()
+
+
+@pytest.mark.parametrize("solver", [A.SOLVER_DIRECT, A.SOLVER_ITERATIVE])
+@pytest.mark.parametrize("loss", [A.LOSS_SQUARED, A.LOSS_HUBER])
+def test_candidate_rule_branches_match_oracle(adjuster, oracle, solver, loss):
+    prob, q_near, _ = synthetic.candidate_rule_scene()
+    prob.robust_loss, prob.loss_scale = loss, (1.0 if loss == A.LOSS_HUBER else 0.0)
+    cfg = SolveConfig(max_iterations=30, linear_solver=solver)
+    want = solve_with(oracle, "bao_solve", prob, cfg)
+    got = adjuster.solve(prob, cfg)
+    assert want.report.rejected_candidate >= 2
+    assert got.report.rejected_candidate == want.report.rejected_candidate
+    assert got.report.accepted_steps == want.report.accepted_steps
+    assert np.abs(got.point_xyz[q_near] - prob.point_xyz[q_near]).max() <= 1e-11
+    _compare_solutions(got, want)
+
+
+@pytest.mark.parametrize("solver", [A.SOLVER_DIRECT, A.SOLVER_ITERATIVE])
+def test_cauchy_stages_match_oracle(adjuster, oracle, solver):
+    """Ring scene with 3 % of the observations replaced by uniform pixels, Cauchy loss of scale
+    2 px: five squared-loss iterations, then the Cauchy stage."""
+    sc = synthetic.ring_scene(seed=0xBA0021)
+    prob = sc.problem
+    rng = np.random.default_rng(5)
+    bad = rng.random(prob.n_observations) < 0.03
+    prob.obs_pixel[bad] = rng.uniform([0.0, 0.0], [1024.0, 768.0], (int(bad.sum()), 2))
+    prob.robust_loss, prob.loss_scale = A.LOSS_CAUCHY, 2.0
+    cfg = SolveConfig(max_iterations=40, linear_solver=solver)
+    want = solve_with(oracle, "bao_solve", prob, cfg)
+    got = adjuster.solve(prob, cfg)
+    assert want.report.iterations > 5  # the Cauchy stage ran
+    _compare_solutions(got, want)

