@@ -97,7 +97,7 @@ __device__ __forceinline__ void load_tile(T (*dst)[TB + 1], const T* src, size_t
 
 // One wavefront factors the diagonal tile (size nb <= 64; identity beyond nb) held in `t` (lower
 // triangle) and forms X = L_kk^-1 into `xs` (row-major).  Lane i keeps row i in registers.
-// Column j: the pivot a_jj is checked against 64 eps max|diag(A)| as the reference's dense camera
+// Column j < nb: the pivot a_jj is checked against 64 eps max|diag(A)| as the reference's dense camera
 // solve does (kernel/multi_view/dense/linear_solver.rs:9-61), L_ij = a_ij / sqrt(a_jj), and the
 // rows below update a_ic -= (a_ij / a_jj) a_cj with column j broadcast through LDS (`col`).
 // Entries above the diagonal of a row are never read.  Then L goes to `t` and lane j solves
@@ -116,7 +116,9 @@ __device__ bool factor_diag_wave(T (*t)[TB + 1], T* col, T (*xs)[TB + 1], uint32
         col[i] = r[j];
         wave_sync();
         const T piv = col[j];
-        if (!(isfinite(piv) && piv > tol)) {
+        // rows at and beyond nb are the identity padding of a partial tile: their unit pivots are
+        // not the matrix's and are not held to its (possibly much larger) tolerance
+        if (j < (int)nb && !(isfinite(piv) && piv > tol)) {
             ok = false;  // uniform: every lane read the same pivot
             break;
         }

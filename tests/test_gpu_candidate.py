@@ -44,7 +44,18 @@ def test_candidate_rule_branches_match_oracle(adjuster, oracle, solver, loss):
     assert got.report.rejected_candidate == want.report.rejected_candidate
     assert got.report.accepted_steps == want.report.accepted_steps
     assert np.abs(got.point_xyz[q_near] - prob.point_xyz[q_near]).max() <= 1e-11
-    _compare_solutions(got, want)
+    # The scene is ill-conditioned on purpose (point rcond ~1e-11, camera diagonal 1e7..1e15):
+    # rounding differences reach 6.8e-10 of the cost after a few Huber steps (measured), so the
+    # trajectory is held to 2e-9 (parameters to the plain 1e-8); the far point (3.5e5 away,
+    # parallax 1.75e-5) is known only along its ray, so its coordinates are compared relative to
+    # its distance.
+    tc, tb = got.trace["cost"], want.trace["cost"]
+    n = min(len(tc), len(tb))
+    print(f"solver {solver} loss {loss}: cost diff {np.nanmax(np.abs(tc[:n] - tb[:n]) / np.abs(tb[:n])):.2e}, "
+          f"point diff {np.max(np.abs(got.point_xyz - want.point_xyz) / np.maximum(np.linalg.norm(want.point_xyz, axis=1, keepdims=True), 1)):.2e}, "
+          f"centre diff {np.max(np.abs(got.pose_center - want.pose_center)):.2e}, "
+          f"focal diff {np.max(np.abs(got.intrinsic_focal - want.intrinsic_focal) / want.intrinsic_focal):.2e}")
+    _compare_solutions(got, want, tol_cost=2e-9, point_norm=True)
 
 
 @pytest.mark.parametrize("solver", [A.SOLVER_DIRECT, A.SOLVER_ITERATIVE])

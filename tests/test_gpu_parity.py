@@ -73,6 +73,33 @@ def test_cholesky_solve_matches_lapack_and_oracle(torch_cuda, adjuster, oracle, 
     assert rel_err(x, xo) < 1e-11
 
 
+@pytest.mark.parametrize("n", [5, 52, 64, 65, 130])
+@pytest.mark.parametrize("precision", [A.PRECISION_FP64, A.PRECISION_MIXED])
+def test_cholesky_badly_scaled(torch_cuda, adjuster, oracle, n, precision):
+    """Diagonal scales from 1e7 to 1e15 (a camera system whose translation columns carry a large
+    scene scale): the pivot floor 64 eps max|diag| then exceeds 1, which the unit pivots of a
+    partial tile's identity padding must not be held to."""
+    torch = torch_cuda
+    rng = np.random.default_rng(100 + n)
+    M = rng.normal(size=(n, n))
+    d = 10.0 ** rng.uniform(7.0, 15.0, n)
+    d[0] = 1e15
+    A_ = (M @ M.T + n * np.eye(n)) * np.sqrt(np.outer(d, d))
+    b = rng.normal(size=n) * np.sqrt(d)
+    x_ref = np.linalg.solve(A_, b)
+    dA = torch.from_numpy(A_).cuda()
+    db = torch.from_numpy(b).cuda()
+    dx = torch.zeros(n, dtype=torch.float64, device="cuda")
+    code = adjuster.handle.lib.ba_hip_cholesky_solve(adjuster.handle.ptr, n, dA.data_ptr(), db.data_ptr(),
+                                                     dx.data_ptr(), precision)
+    assert code == A.BA_OK
+    xo = np.zeros(n)
+    assert oracle.bao_cholesky_solve(n, A_.ctypes.data, b.ctypes.data, xo.ctypes.data) == 1
+    x = dx.cpu().numpy()
+    # componentwise in the scaled unknowns sqrt(d) x (the system is well conditioned after scaling)
+    assert np.abs((x - x_ref) * np.sqrt(d)).max() <= 1e-9 * np.abs(x_ref * np.sqrt(d)).max()
+
+
 def test_cholesky_rejects_indefinite(torch_cuda, adjuster):
     torch = torch_cuda
     n = 70
@@ -158,10 +185,11 @@ def test_pcg_step_matches_direct_solve(torch_cuda, adjuster, oracle, ring):
     assert rel_err(step.cpu().numpy(), so) < 1e-9
 
 
-def _compare_solutions(a, b, tol_cost=1e-10, tol_param=1e-8, cost_floor=0.0, tol_final=None):
+def _compare_solutions(a, b, tol_cost=1e-10, tol_param=1e-8, cost_floor=0.0, tol_final=None, point_norm=False):
     """Same LM trajectory (accept/reject sequence, per-iteration cost within tol_cost relative
     to max(|cost|, cost_floor)), same status, final cost within tol_final (default tol_cost) and
-    parameters within tol_param."""
+    parameters within tol_param (relative to max(|value|, 1); with point_norm, a point's three
+    coordinates relative to max(|X|, 1))."""
     tol_final = tol_cost if tol_final is None else tol_final
     ta, tb = a.trace, b.trace
     n = min(len(ta["cost"]), len(tb["cost"]))
@@ -176,6 +204,8 @@ def _compare_solutions(a, b, tol_cost=1e-10, tol_param=1e-8, cost_floor=0.0, tol
     for name in ("pose_center", "point_xyz", "intrinsic_focal", "intrinsic_k1", "intrinsic_k2", "pose_rotation"):
         x, y = getattr(a, name), getattr(b, name)
         scale = np.maximum(np.abs(y), 1.0)
+        if point_norm and name == "point_xyz":
+            scale = np.maximum(np.linalg.norm(y, axis=1, keepdims=True), 1.0)
         assert np.all(np.abs(x - y) <= tol_param * scale), name
 
 
