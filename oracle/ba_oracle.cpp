@@ -447,7 +447,7 @@ constexpr double MIN_NEAR_FAR_RATIO = 1e-4;
 constexpr double MIN_PARALLAX_SINE = 1.7e-5;
 struct TrackGeometry {
     double near2, far2, scene2;
-    double first[3];
+    double first[3] = {0.0, 0.0, 0.0};
     bool has_first = false, parallax = false, invalid = false;
     size_t count = 0;
     explicit TrackGeometry(double scene_length) {
@@ -712,6 +712,21 @@ struct HStore {
         if (br >= nc && bc < nc && cam_group[bc] == br - nc) return edges[bc][plocal[c] * 3 + plocal[r]];
         return 0.0;
     }
+    // this += o, entry by entry (the same layout)
+    void add_from(const HStore& o) {
+        valid = valid && o.valid;
+        for (size_t i = 0; i < full.size(); ++i) full[i] += o.full[i];
+        for (size_t b = 0; b < blocks.size(); ++b)
+            for (size_t k = 0; k < blocks[b].size(); ++k) blocks[b][k] += o.blocks[b][k];
+        for (size_t c = 0; c < edges.size(); ++c)
+            for (size_t k = 0; k < edges[c].size(); ++k) edges[c][k] += o.edges[c][k];
+    }
+    size_t bytes() const {
+        size_t n = full.size();
+        for (auto& b : blocks) n += b.size();
+        for (auto& e : edges) n += e.size();
+        return 8 * n;
+    }
     bool finite() const {
         if (!valid) return false;
         if (dense) {
@@ -771,6 +786,43 @@ static size_t clamp_block(double* d, size_t n) {  // :744-766
     return clamped;
 }
 
+// ---------------------------------------------------------------------------------------
+// Deterministic point shards (kernel/multi_view/sparse/retained/parallel.rs:9-31, 94-122).
+// A pass over points runs in contiguous point shards balanced by observations.  Their number
+// depends on the problem alone (the observation count, and the private accumulator bytes each
+// shard needs), never on the thread count: each shard sums in its own storage in point order,
+// and the shards are added in shard order, so any number of OpenMP threads gives the same bits.
+// Below 32,768 observations there is one shard: the serial sums.
+// ---------------------------------------------------------------------------------------
+static std::vector<size_t> point_shards(const uint32_t* off, size_t np, size_t worker_bytes) {
+    const size_t no = np ? off[np] : 0;
+    size_t want = 1;
+    if (no >= 32768) {
+        want = std::min<size_t>(std::max<size_t>((no + 8191) / 8192, 1), 32);
+        const size_t mem = worker_bytes ? ((size_t)128 << 20) / worker_bytes : want;
+        want = std::max<size_t>(1, std::min(want, mem));
+    }
+    std::vector<size_t> r{0};
+    if (np == 0) return r;
+    const size_t workers = std::min(want, np);
+    size_t previous = 0;
+    for (size_t part = 1; part < workers; ++part) {
+        const size_t lo = previous + 1, hi = np - (workers - part);
+        size_t boundary;
+        if (no == 0) {
+            boundary = np * part / workers;
+        } else {
+            const size_t target = (size_t)((unsigned __int128)no * part / workers);
+            boundary = (size_t)(std::lower_bound(off, off + np + 1, (uint32_t)std::min<size_t>(target, UINT32_MAX)) - off);
+        }
+        boundary = std::min(std::max(boundary, lo), hi);
+        r.push_back(boundary);
+        previous = boundary;
+    }
+    r.push_back(np);
+    return r;
+}
+
 // assemble_with_hessian_problem_impl + finalize_assembly; `materialize` forms S, else U.
 static Assembly assemble(const View& v, double lambda, bool materialize, const uint32_t* cross_off) {
     const size_t D = v.D;
@@ -788,11 +840,37 @@ static Assembly assemble(const View& v, double lambda, bool materialize, const u
     a.damped_packed.assign(6 * v.np, 0.0);
     a.inv_packed.assign(6 * v.np, 0.0);
     if (cross_off) a.inv_cross.assign(3 * (size_t)cross_off[v.np], 0.0);
+    // per-shard sums: the Hessian store, the three D-vectors and the scalars
+    struct Acc {
+        HStore H;
+        std::vector<double> g, fg, diag;
+        double cost = 0.0, sq = 0.0, pmr = 0.0, sgin = 0.0, min_rcond = 1.0;
+        size_t weak = 0;
+        bool valid = true;
+    };
+    const std::vector<size_t> shards = point_shards(v.off, v.np, a.H.bytes() + 6 * 8 * D);
+    const size_t nshards = shards.size() - 1;
+    std::vector<Acc> accs(nshards);
+    for (size_t sh = 0; sh < nshards; ++sh) {
+        accs[sh].H = sh == 0 ? std::move(a.H) : HStore{};
+        if (sh > 0) {
+            if (materialize)
+                accs[sh].H.init_dense(D);
+            else
+                accs[sh].H.init_blocks(v.cam_off, v.cam_dim, v.grp_off, v.grp_dim, v.cam_group, v.nc, v.ng, D);
+        }
+        accs[sh].g.assign(D, 0.0);
+        accs[sh].fg.assign(D, 0.0);
+        accs[sh].diag.assign(D, 0.0);
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t sh = 0; sh < nshards; ++sh) {
+    Acc& acc = accs[sh];
     std::vector<double> cross(3 * D, 0.0);
     std::vector<uint8_t> marks(D, 0);
     std::vector<size_t> active;
     active.reserve(D);
-    for (size_t p = 0; p < v.np; ++p) {
+    for (size_t p = shards[sh]; p < shards[sh + 1]; ++p) {
         Mat3 H{};
         double gp[3] = {0, 0, 0}, pcost = 0.0, psq = 0.0;
         bool ok = true;
@@ -852,12 +930,12 @@ static Assembly assemble(const View& v, double lambda, bool materialize, const u
             double wres[2] = {sw * o.res[0], sw * o.res[1]};
             for (int k = 0; k < cd; ++k) {
                 size_t q = param[k];
-                a.g[q] += first[k] * wres[0];
-                a.g[q] += second[k] * wres[1];
-                a.fg[q] += first[k] * wres[0];
-                a.fg[q] += second[k] * wres[1];
-                a.diag[q] += first[k] * first[k];
-                a.diag[q] += second[k] * second[k];
+                acc.g[q] += first[k] * wres[0];
+                acc.g[q] += second[k] * wres[1];
+                acc.fg[q] += first[k] * wres[0];
+                acc.fg[q] += second[k] * wres[1];
+                acc.diag[q] += first[k] * first[k];
+                acc.diag[q] += second[k] * second[k];
             }
             // Gram: intrinsic rows against the pose run, then each run's lower triangle
             int run_begin[2] = {0, o.pose_dim}, run_end[2] = {o.pose_dim, cd};
@@ -867,40 +945,40 @@ static Assembly assemble(const View& v, double lambda, bool materialize, const u
                     if (run_end[ci] == run_begin[ci]) continue;
                     for (int i = run_begin[ri]; i < run_end[ri]; ++i)
                         for (int j = run_begin[ci]; j < run_end[ci]; ++j) {
-                            add_sym(a.H, D, param[i], param[j], first[i] * first[j]);
-                            add_sym(a.H, D, param[i], param[j], second[i] * second[j]);
+                            add_sym(acc.H, D, param[i], param[j], first[i] * first[j]);
+                            add_sym(acc.H, D, param[i], param[j], second[i] * second[j]);
                         }
                 }
                 for (int i = run_begin[ri]; i < run_end[ri]; ++i)
                     for (int j = run_begin[ri]; j <= i; ++j) {
-                        add_sym(a.H, D, param[i], param[j], first[i] * first[j]);
-                        add_sym(a.H, D, param[i], param[j], second[i] * second[j]);
+                        add_sym(acc.H, D, param[i], param[j], first[i] * first[j]);
+                        add_sym(acc.H, D, param[i], param[j], second[i] * second[j]);
                     }
             }
         }
         for (size_t q : active) marks[q] = 0;
         if (!ok) {
-            a.valid = false;
+            acc.valid = false;
             continue;
         }
-        a.cost += pcost;
-        a.sq += psq;
+        acc.cost += pcost;
+        acc.sq += psq;
         if (v.point_fixed && v.point_fixed[p]) continue;
         double pd[3] = {H[0][0], H[1][1], H[2][2]};
-        a.sgin = fmax_(a.sgin, scaled_gradient_inf_norm(gp, pd));
+        acc.sgin = fmax_(acc.sgin, scaled_gradient_inf_norm(gp, pd));
         PointFactor f;
         if (!point_factor_new(H, lambda, f)) {
-            a.valid = false;
+            acc.valid = false;
             if (lambda == 0.0) {
-                a.weak += 1;
-                a.min_rcond = 0.0;
+                acc.weak += 1;
+                acc.min_rcond = 0.0;
             }
             continue;
         }
-        double dd[3];
+        double dd[3] = {0.0, 0.0, 0.0};
         damping_diagonal(H, dd);
         for (int k = 0; k < 3; ++k) a.damp_diag[3 * p + k] = dd[k];
-        Mat3 dm;
+        Mat3 dm{};
         damped_matrix(H, lambda, dm);
         double packed[6] = {dm[0][0], dm[1][0], dm[1][1], dm[2][0], dm[2][1], dm[2][2]};
         for (int k = 0; k < 6; ++k) {
@@ -909,14 +987,14 @@ static Assembly assemble(const View& v, double lambda, bool materialize, const u
         }
         if (lambda == 0.0) {
             double rc = point_rcond(f, H);
-            a.min_rcond = fmin_(a.min_rcond, rc);
-            if (rc <= POINT_BLOCK_RCOND_THRESHOLD) a.weak += 1;
+            acc.min_rcond = fmin_(acc.min_rcond, rc);
+            if (rc <= POINT_BLOCK_RCOND_THRESHOLD) acc.weak += 1;
         }
         double ig[3];
         f.solve(gp, ig);
-        a.pmr += 0.5 * (gp[0] * ig[0] + gp[1] * ig[1] + gp[2] * ig[2]);
+        acc.pmr += 0.5 * (gp[0] * ig[0] + gp[1] * ig[1] + gp[2] * ig[2]);
         if (cross_off && cross_off[p + 1] - cross_off[p] != active.size()) {
-            a.valid = false;
+            acc.valid = false;
             continue;
         }
         a.cache_valid[p] = 1;
@@ -924,7 +1002,7 @@ static Assembly assemble(const View& v, double lambda, bool materialize, const u
         std::vector<std::array<double, 3>> inv_rows(active.size());
         for (size_t i = 0; i < active.size(); ++i) {
             const double* w = &cross[3 * active[i]];
-            a.g[active[i]] -= w[0] * ig[0] + w[1] * ig[1] + w[2] * ig[2];
+            acc.g[active[i]] -= w[0] * ig[0] + w[1] * ig[1] + w[2] * ig[2];
             f.solve(w, inv_rows[i].data());
             if (cross_off)
                 for (int c = 0; c < 3; ++c) a.inv_cross[3 * (cross_off[p] + i) + c] = inv_rows[i][c];
@@ -934,9 +1012,41 @@ static Assembly assemble(const View& v, double lambda, bool materialize, const u
                 for (size_t j = 0; j <= i; ++j) {
                     const double* w = &cross[3 * active[j]];
                     double val = -(inv_rows[i][0] * w[0] + inv_rows[i][1] * w[1] + inv_rows[i][2] * w[2]);
-                    add_sym(a.H, D, active[i], active[j], val);
+                    add_sym(acc.H, D, active[i], active[j], val);
                 }
         }
+    }
+    }
+    // shard sums in shard order
+    {
+        Acc& first = accs[0];
+        for (size_t sh = 1; sh < nshards; ++sh) {
+            const Acc& o = accs[sh];
+            first.H.add_from(o.H);
+            for (size_t i = 0; i < D; ++i) {
+                first.g[i] += o.g[i];
+                first.fg[i] += o.fg[i];
+                first.diag[i] += o.diag[i];
+            }
+            first.cost += o.cost;
+            first.sq += o.sq;
+            first.pmr += o.pmr;
+            first.sgin = fmax_(first.sgin, o.sgin);
+            first.min_rcond = fmin_(first.min_rcond, o.min_rcond);
+            first.weak += o.weak;
+            first.valid = first.valid && o.valid;
+        }
+        a.H = std::move(first.H);
+        a.g = std::move(first.g);
+        a.fg = std::move(first.fg);
+        a.diag = std::move(first.diag);
+        a.cost = first.cost;
+        a.sq = first.sq;
+        a.pmr = first.pmr;
+        a.sgin = first.sgin;
+        a.min_rcond = first.min_rcond;
+        a.weak = first.weak;
+        a.valid = first.valid;
     }
     // finalize_assembly, assembly.rs:691-727
     double scale = 0.0, mn = INF;
@@ -986,17 +1096,16 @@ static bool dense_cholesky_solve(std::vector<double>& F, size_t n, const double*
     for (size_t i = 0; i < n; ++i) scale = fmax_(scale, std::fabs(F[i * n + i]));
     double tol = 64.0 * EPS * fmax_(scale, MINPOS);
     for (size_t c = 0; c < n; ++c) {
-        // Row c's pivot first, then rows below: identical arithmetic to the row-by-row loop.
-        for (size_t r = c; r < n; ++r) {
-            double dot = (r == c) ? dot4(&F[r * n], &F[r * n], c) : dot4(&F[r * n], &F[c * n], c);
-            double val = F[r * n + c] - dot;
-            if (r == c) {
-                if (!(std::isfinite(val) && val > tol)) return false;
-                F[r * n + c] = std::sqrt(val);
-            } else {
-                F[r * n + c] = val / F[c * n + c];
-            }
+        // Row c's pivot first, then rows below: identical arithmetic to the row-by-row loop (each
+        // row's entry depends only on rows c and r, so the rows below run in parallel).
+        {
+            double val = F[c * n + c] - dot4(&F[c * n], &F[c * n], c);
+            if (!(std::isfinite(val) && val > tol)) return false;
+            F[c * n + c] = std::sqrt(val);
         }
+        const double pivot = F[c * n + c];
+#pragma omp parallel for schedule(static) if (n - c > 256)
+        for (size_t r = c + 1; r < n; ++r) F[r * n + c] = (F[r * n + c] - dot4(&F[r * n], &F[c * n], c)) / pivot;
     }
     for (size_t r = 0; r < n; ++r) {
         double v = rhs[r];
@@ -1078,6 +1187,8 @@ struct Retained {
     const uint32_t* params;
     size_t np;
     const uint8_t* point_fixed;
+    std::vector<size_t> shards;  // point shards of the coupling (one private D-vector each)
+    mutable std::vector<std::vector<double>> shard_out;
 
     // apply_base (operator.rs:563-594) then the fused point coupling (operator.rs:597-630)
     void apply(const double* in, double* out) const {
@@ -1101,25 +1212,39 @@ struct Retained {
                     out[io + il] += val * in[po + pl];
                 }
         }
-        for (size_t p = 0; p < np; ++p) {
-            uint32_t b = cross_off[p], en = cross_off[p + 1];
-            double sum[3] = {0, 0, 0};
-            for (uint32_t k = b; k < en; ++k) {
-                double val = in[params[k]];
-                const double* rc = &asmb->inv_cross[3 * k];
-                sum[0] += rc[0] * val;
-                sum[1] += rc[1] * val;
-                sum[2] += rc[2] * val;
+        // shard 0 adds into `out` after the base terms, shard s > 0 into its own vector; the
+        // shard vectors are then added in shard order
+        const size_t nshards = shards.size() - 1;
+        if (shard_out.size() < nshards) shard_out.resize(nshards);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (size_t sh = 0; sh < nshards; ++sh) {
+            double* dst = out;
+            if (sh > 0) {
+                shard_out[sh].assign(D, 0.0);
+                dst = shard_out[sh].data();
             }
-            const double* h = &asmb->damped_packed[6 * p];
-            double w[3] = {h[0] * sum[0] + h[1] * sum[1] + h[3] * sum[2],
-                           h[1] * sum[0] + h[2] * sum[1] + h[4] * sum[2],
-                           h[3] * sum[0] + h[4] * sum[1] + h[5] * sum[2]};
-            for (uint32_t k = b; k < en; ++k) {
-                const double* rc = &asmb->inv_cross[3 * k];
-                out[params[k]] -= rc[0] * w[0] + rc[1] * w[1] + rc[2] * w[2];
+            for (size_t p = shards[sh]; p < shards[sh + 1]; ++p) {
+                uint32_t b = cross_off[p], en = cross_off[p + 1];
+                double sum[3] = {0, 0, 0};
+                for (uint32_t k = b; k < en; ++k) {
+                    double val = in[params[k]];
+                    const double* rc = &asmb->inv_cross[3 * k];
+                    sum[0] += rc[0] * val;
+                    sum[1] += rc[1] * val;
+                    sum[2] += rc[2] * val;
+                }
+                const double* h = &asmb->damped_packed[6 * p];
+                double w[3] = {h[0] * sum[0] + h[1] * sum[1] + h[3] * sum[2],
+                               h[1] * sum[0] + h[2] * sum[1] + h[4] * sum[2],
+                               h[3] * sum[0] + h[4] * sum[1] + h[5] * sum[2]};
+                for (uint32_t k = b; k < en; ++k) {
+                    const double* rc = &asmb->inv_cross[3 * k];
+                    dst[params[k]] -= rc[0] * w[0] + rc[1] * w[1] + rc[2] * w[2];
+                }
             }
         }
+        for (size_t sh = 1; sh < nshards; ++sh)
+            for (size_t i = 0; i < D; ++i) out[i] += shard_out[sh][i];
     }
 };
 
@@ -1237,6 +1362,7 @@ static bool build_retained(const View& v, const Assembly& a, const uint32_t* cro
     ret.params = params;
     ret.np = v.np;
     ret.point_fixed = v.point_fixed;
+    ret.shards = point_shards(v.off, v.np, v.D * sizeof(double));
     // accumulate_point_coupling_range (self_calibration_retained.rs:827-868)
     std::vector<Packed> coupling(ret.nblocks);
     for (auto& p : coupling) std::memset(p.v, 0, sizeof(p.v));
@@ -2106,9 +2232,17 @@ static Err refine(const Lowered& L, const ba_solve_config* cfg, State& st, ba_so
             // candidate_points, cached back-substitution (candidate.rs:25-233, 358-391)
             View cv = make_view(L, cand, sched[si].loss, L.gauge_camera, scene_scale);
             cv.gauge = gs;  // the candidate uses the linearisation's gauge state
+            // per point shard: its sums in point order, stopping at its first failing point; the
+            // shards' sums are added in shard order (a failing shard rejects the step)
+            const std::vector<size_t> cshards = point_shards(L.off.data(), np, 0);
+            const size_t ncs = cshards.size() - 1;
+            std::vector<double> csq(ncs, 0.0), cpdn(ncs, 0.0), ccost_s(ncs, 0.0);
+            std::vector<uint8_t> cok(ncs, 1);
+#pragma omp parallel for schedule(dynamic, 1)
+            for (size_t sh = 0; sh < ncs; ++sh) {
             double sq = 0.0, pdn = 0.0, ccost = 0.0;
             bool cand_ok = true;
-            for (size_t q = 0; q < np && cand_ok; ++q) {
+            for (size_t q = cshards[sh]; q < cshards[sh + 1] && cand_ok; ++q) {
                 const double* prev = &st.X[3 * q];
                 if (L.point_fixed[q]) {
                     for (int k = 0; k < 3; ++k) cand.X[3 * q + k] = prev[k];
@@ -2155,6 +2289,19 @@ static Err refine(const Lowered& L, const ba_solve_config* cfg, State& st, ba_so
                     pdn += dn;
                     sq += dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
                 }
+            }
+            csq[sh] = sq;
+            cpdn[sh] = pdn;
+            ccost_s[sh] = ccost;
+            cok[sh] = cand_ok;
+            }
+            double sq = csq[0], pdn = cpdn[0], ccost = ccost_s[0];
+            bool cand_ok = cok[0] != 0;
+            for (size_t sh = 1; sh < ncs; ++sh) {
+                sq += csq[sh];
+                pdn += cpdn[sh];
+                ccost += ccost_s[sh];
+                cand_ok = cand_ok && cok[sh] != 0;
             }
             if (!cand_ok) {
                 rej[2]++;
