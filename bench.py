@@ -3,21 +3,25 @@
 
 Workload: the large synthetic scene (seed 0xBA0003): 2,000 cameras with free f, k1, k2; 1M points;
 ~6.1M observations with geometric track lengths (mean 6, clipped to [2, 50]); 2 % outliers; Huber
-loss with delta 1 px; inexact LM with block-Jacobi PCG on the implicit Schur system (forcing
-tolerance 0.1). One step is one LM iteration: linearization, point elimination, block assembly,
-the PCG solve and the candidate pass (sparse/self_calibration/mod.rs:212-639).
+loss with delta 1 px; inexact LM with block-Jacobi PCG on the Schur system (forcing tolerance 0.1,
+the native cap of min(600, D) PCG iterations per solve). One step is one LM iteration:
+linearization, point elimination, block assembly, the PCG solve and the candidate pass
+(sparse/self_calibration/mod.rs:212-639).
 
 The timed region is the LM loop as the library reports it (`seconds_iterations`): it starts after
 the problem has been lowered and uploaded, so every input is resident in HBM, and every iteration
 ends in a synchronous scalar readback. The printed JSON line carries:
 
-* ``roofline`` for the dominant kernel, from HIP events the library records around each of that
-  kernel's launches on the solve's stream, and that kernel's algorithmic bytes per launch;
-* ``cpu_baseline``: the oracle (a single-threaded C++ restatement of the same LM path, used here
-  only as the CPU reference timing) on a bounded subsample of the same scene generator, scaled
-  to the full scene by observation count;
-* ``time_to_cost``: seconds until the loop holds a state within 1e-6 relative of the converged
-  FP64 cost of a full run.
+* ``kernels``: every hot kernel's launches, average duration (HIP events the library records
+  around that kernel's launches on the solve's stream, one profiled solve per kernel with the same
+  iterations and product choice as the timed one), total time, algorithmic bytes and fraction of
+  the HBM peak; ``roofline`` is the kernel with the largest total time;
+* ``cpu_baseline``: the oracle (the C++ restatement of the same LM path, OpenMP over point shards
+  fixed by the problem, so the result does not depend on the thread count) on the full scene on
+  all host cores: LM iterations/s over the same first iterations, the time to reach its own
+  converged cost, and one iteration on one thread;
+* ``time_to_cost``: seconds until an accepted state is within 1e-6 relative of the oracle's
+  converged cost C* (from the CPU leg, or the committed oracle fixture when it is skipped).
 """
 
 from __future__ import annotations
@@ -105,6 +109,14 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
     linearize (k_linearize): per observation pixel (16 B), camera and point index (8 B), the
       written r~, compact state and J~_X (96 B), cost, squared residual and flag (17 B); per point
       X (24 B); per camera R, t, f, k1, k2 (120 B).
+    point_elim (k_point_elim): per observation r~ and J~_X (64 B), cost and squared residual
+      (16 B) and the validity flag (1 B); per point its offset and fixed flag (5 B) and the written
+      six scalars (48 B), y = V^-1 g_p and the damping diagonal (24 B each), V^ and V^-1 (48 B
+      each) and the packed 128-byte record of the camera-side assembly: 325 B.
+    candidate (k_candidate): per observation the camera and point index (8 B), pixel (16 B),
+      J~_X and three values of the compact state (72 B); per point its offset and fixed flag
+      (5 B), X read and the candidate written (24 B each), V^-1 (48 B), y and the damping
+      diagonal (24 B each) and the written scalars (32 B): 181 B.
     pcg_product (k_pcg_product_upper: the first pass of q = S v over the stored diagonal and
       upper blocks of the explicit reduced system, one workgroup per camera row): per stored slot
       the 9 x 9 block (648 B) and its neighbour index (4 B); per camera pair the written column
@@ -131,6 +143,10 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
         return no * (56 + 4) + npt * 32 + counts["segments"] * (4 + 72) + nc * (12 + 192)
     if kernel == "linearize":
         return no * (16 + 8 + 96 + 17) + npt * 24 + nc * 120
+    if kernel == "point_elim":
+        return no * 81 + npt * 325
+    if kernel == "candidate":
+        return no * 96 + npt * 181
     raise ValueError(kernel)
 
 
@@ -153,35 +169,120 @@ def measured_traffic(kernel: str, per_launch: float):
     return vals["traffic_bytes_per_launch"]
 
 
-def cpu_baseline(scene_fn, n_points_sample: int, n_obs_full: int, iterations: int) -> dict:
-    """The oracle's LM iterations/s on a subsample, scaled to the full scene by observations."""
+def host_cpu() -> dict:
+    """The host's logical CPUs and model name (the CPU baseline's cores)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count() or 1, "model": model}
+
+
+def omp_threads(n: int | None = None) -> int:
+    """Set (when n is given) and return the OpenMP thread count of this process's runtime, the one
+    the oracle library uses."""
+    import ctypes
+
+    gomp = ctypes.CDLL("libgomp.so.1")
+    if n is not None:
+        gomp.omp_set_num_threads(int(n))
+    return int(gomp.omp_get_max_threads())
+
+
+def load_oracle_lib():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import load_oracle
 
+    return load_oracle()
+
+
+def first_hit(trace: dict, target: float):
+    """Loop seconds at the first accepted iteration whose candidate cost is at most `target`."""
+    for i in range(len(trace["seconds"])):
+        if trace["accepted"][i] and trace["candidate_cost"][i] <= target:
+            return float(trace["seconds"][i]), i + 1
+    return None, None
+
+
+def cpu_baseline(prob, steps: int, converge_iterations: int) -> dict:
+    """The oracle on the full scene: the run to convergence on all host cores (LM iterations/s over
+    the first `steps` iterations, the same ones the GPU line times; its converged cost C*; the
+    time to within 1e-6 of C*) and the first iteration on one thread."""
     from openmvg_ba import SolveConfig, _abi as A
     from openmvg_ba.problem import solve_with
 
-    oracle = load_oracle()
-    scene = scene_fn(n_points=n_points_sample)
-    cfg = SolveConfig(max_iterations=iterations, linear_solver=A.SOLVER_ITERATIVE)
+    oracle = load_oracle_lib()
+    cpu = host_cpu()
+    threads = omp_threads(cpu["nproc"]) if "OMP_NUM_THREADS" not in os.environ else omp_threads()
+    cfg = SolveConfig(max_iterations=converge_iterations, linear_solver=A.SOLVER_ITERATIVE)
     t0 = time.perf_counter()
-    sol = solve_with(oracle, "bao_solve", scene.problem, cfg)
+    sol = solve_with(oracle, "bao_solve", prob, cfg)
     wall = time.perf_counter() - t0
     tr = sol.trace
-    n_it = len(tr["seconds"])
-    loop_s = float(tr["seconds"][-1]) if n_it else float("nan")
-    rate_sample = n_it / loop_s
-    n_obs = scene.problem.n_observations
+    n = min(steps, len(tr["seconds"]))
+    c_star = float(sol.report.final_cost)
+    ttc, ttc_it = first_hit(tr, c_star * (1.0 + 1e-6))
+    omp_threads(1)
+    one = solve_with(oracle, "bao_solve", prob, SolveConfig(max_iterations=1, linear_solver=A.SOLVER_ITERATIVE))
+    omp_threads(threads)
+    one_s = float(one.trace["seconds"][0])
     return {
-        "value": rate_sample * n_obs / n_obs_full,
+        "value": n / float(tr["seconds"][n - 1]),
         "unit": "LM iter/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"oracle (single-threaded C++ restatement), same generator with {n_points_sample} points / "
-                   f"{n_obs} observations, {n_it} LM iterations in {loop_s:.2f} s "
-                   f"({rate_sample:.3f} iter/s, {int(tr['pcg_iterations'].sum())} PCG iterations), scaled by "
-                   f"{n_obs}/{n_obs_full} observations; whole call {wall:.1f} s"),
+        "sample": (f"oracle (C++ FP64 restatement of the same LM path, OpenMP over point shards fixed by the "
+                   f"problem: same result at any thread count) on the full scene, {threads} threads on "
+                   f"{cpu['nproc']} logical CPUs ({cpu['model']}); value = the first {n} LM iterations of its run to "
+                   f"convergence ({int(tr['pcg_iterations'][:n].sum())} PCG iterations); whole call {wall:.1f} s"),
+        "nproc": cpu["nproc"],
+        "cpu_model": cpu["model"],
+        "converged_cost": c_star,
+        "converge_iterations": int(sol.report.iterations),
+        "status": str(sol.report.status),
+        "time_to_cost_seconds": ttc,
+        "time_to_cost_iterations": ttc_it,
+        "loop_seconds": float(tr["seconds"][-1]),
+        "one_thread_first_iteration_seconds": one_s,
+        "all_threads_first_iteration_seconds": float(tr["seconds"][0]),
     }
+
+
+def kernel_table(ba, prob, base, steps: int, D: int) -> list:
+    """Per hot kernel: a solve like the timed one (same iterations, AUTO product choice) with HIP
+    events around that kernel's launches; its launches, average and total time, algorithmic bytes
+    per launch and achieved fraction of the HBM peak.  PCG launches issued after the PCG state
+    turned done return at once and are not counted."""
+    from openmvg_ba import _abi as A
+
+    counts = lowered_counts(prob)
+    counts.update(schur_counts(prob))
+    ids = {"schur_pairs": A.PROFILE_SCHUR_PAIRS, "camera_assembly": A.PROFILE_PAIRS,
+           "pcg_product": A.PROFILE_PCG_PRODUCT, "pcg_points": A.PROFILE_PCG_POINTS,
+           "pcg_cameras": A.PROFILE_PCG_CAMERAS, "linearize": A.PROFILE_LINEARIZE,
+           "point_elim": A.PROFILE_POINT_ELIM, "candidate": A.PROFILE_CANDIDATE}
+    rows = []
+    for name, kid in ids.items():
+        sol = ba.solve(prob, base.with_(max_iterations=steps, profile_kernel=kid))
+        launches = int(sol.report.profile_launches)
+        total = float(sol.report.profile_seconds)
+        if launches == 0:
+            continue
+        avg = total / launches
+        per_launch = algorithmic_bytes(name, counts, D)
+        achieved = per_launch / avg / 1e9
+        rows.append({"kernel": name, "launches": launches, "avg_launch_us": avg * 1e6, "total_ms": total * 1e3,
+                     "algorithmic_bytes_per_launch": per_launch, "achieved_gbs": achieved,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name, per_launch)})
+        log(f"kernel {name}: {launches} launches, avg {avg * 1e6:.1f} us, total {total * 1e3:.2f} ms, "
+            f"{achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.3f})")
+    rows.sort(key=lambda r: -r["total_ms"])
+    return rows
 
 
 def bench_mixed(ba, prob, base, steps: int, ttc: dict | None) -> dict:
@@ -197,21 +298,16 @@ def bench_mixed(ba, prob, base, steps: int, ttc: dict | None) -> dict:
            "pcg_iterations": int(sol.report.pcg_iterations)}
     if ttc is not None:
         conv = ba.solve(prob, cfg.with_(max_iterations=ttc["fp64_iterations"] + 20))
-        target = ttc["converged_cost"] * (1.0 + 1e-6)
-        tr = conv.trace
-        hit = None
-        for i in range(len(tr["seconds"])):
-            if tr["accepted"][i] and tr["candidate_cost"][i] <= target:
-                hit = float(tr["seconds"][i])
-                break
-        out.update(time_to_cost_seconds=hit, iterations=int(conv.report.iterations),
+        c_star = ttc["oracle_converged_cost"]
+        hit, hit_it = first_hit(conv.trace, c_star * (1.0 + 1e-6))
+        out.update(time_to_cost_seconds=hit, time_to_cost_iterations=hit_it, iterations=int(conv.report.iterations),
                    final_cost=float(conv.report.final_cost),
-                   final_cost_rel_error_vs_fp64=abs(conv.report.final_cost - ttc["converged_cost"]) / ttc["converged_cost"],
+                   final_cost_rel_error_vs_oracle=abs(conv.report.final_cost - c_star) / c_star,
                    speedup_vs_fp64=(ttc["fp64_seconds"] / hit) if (hit and ttc["fp64_seconds"]) else None)
     return out
 
 
-def bench_medium(ba) -> dict:
+def bench_medium(ba, cpu: bool = False) -> dict:
     """Direct form on the 200-camera / 50,000-point scene (D = 1,793): LM iterations/s over ten
     iterations in FP64 and mixed precision, and the dense Cholesky (kernel d) rate on the matrix
     cores from HIP events around its launches (D^3/3 + 2 D^2 flops per factor and solve)."""
@@ -234,10 +330,18 @@ def bench_medium(ba) -> dict:
                      "cholesky_ms": 1e3 * avg, "cholesky_tflops": flops / avg / 1e12,
                      "cholesky_peak_tflops": 78.6 if name == "fp64" else 157.3,
                      "final_cost": rep.final_cost, "precision_fallbacks": int(rep.precision_fallbacks)}
+    if cpu:
+        from openmvg_ba.problem import solve_with
+
+        sol = solve_with(load_oracle_lib(), "bao_solve", prob, cfg)
+        loop = float(sol.trace["seconds"][-1])
+        out["cpu"] = {"lm_iter_per_s": sol.report.iterations / loop, "threads": omp_threads(),
+                      "final_cost": float(sol.report.final_cost), "kind": "port",
+                      "sample": "oracle, the same 10-iteration direct solve, all host cores"}
     return out
 
 
-def bench_batched(ba, sample_windows: int = 64) -> dict:
+def bench_batched(ba, sample_windows: int = 64, cpu: bool = False) -> dict:
     """512 local windows (seed 0xBA0004: U{5..40} cameras, U{200..5000} points, 2-8 views), ten LM
     iterations each with the direct step: one batched solve (graph-replayed iterations) in FP64 and
     mixed precision, against one solve per window (the first `sample_windows` windows, scaled)."""
@@ -261,10 +365,27 @@ def bench_batched(ba, sample_windows: int = 64) -> dict:
     out["per_window_launches"] = {"window_iterations_per_s": its / loop,
                                   "sample": f"first {sample_windows} windows solved one by one (FP64)"}
     out["speedup_batched_vs_per_window"] = out["fp64"]["window_iterations_per_s"] / (its / loop)
+    if cpu:
+        # the oracle solves the windows one by one, one window per host thread (each window's solve
+        # is sequential: fewer than 32,768 observations make one point shard)
+        from concurrent.futures import ThreadPoolExecutor
+
+        from openmvg_ba.problem import solve_with
+
+        oracle = load_oracle_lib()
+        workers = os.cpu_count() or 1
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(workers) as pool:
+            sols = list(pool.map(lambda p: solve_with(oracle, "bao_solve", p, cfg), probs))
+        wall = time.perf_counter() - t0
+        its = sum(x.report.iterations for x in sols)
+        out["cpu"] = {"window_iterations_per_s": its / wall, "wall_seconds": wall, "threads": workers,
+                      "kind": "port", "sample": "oracle, every window's 10-iteration direct solve, one window per "
+                                                f"host thread ({workers} threads), whole calls"}
     return out
 
 
-def bench_differentiation(steps: int) -> dict:
+def bench_differentiation(steps: int, cpu: bool = False) -> dict:
     """Forward+backward observations/s of the differentiable reprojection cost: 1,000 cameras,
     200,000 points, 1,000,000 observations, Huber 1 px (seed 0xBA0005), FP32 and FP64, through
     the torch.autograd.Function (forward call + backward call) on torch's current stream."""
@@ -298,6 +419,23 @@ def bench_differentiation(steps: int) -> dict:
         sec = e0.elapsed_time(e1) / 1e3 / steps
         out[name] = {"obs_per_s": len(oc) / sec, "ms_per_forward_backward": sec * 1e3}
     topo.close()
+    if cpu:
+        import ctypes as C
+
+        oracle = load_oracle_lib()
+        c64, p64, uv64 = (np.ascontiguousarray(x, dtype=np.float64) for x in (cams, pts, uv))
+        oc32, op32 = np.ascontiguousarray(oc, dtype=np.uint32), np.ascontiguousarray(op, dtype=np.uint32)
+        gc, gp = np.zeros_like(c64), np.zeros_like(p64)
+        cost, inv = C.c_double(), C.c_uint32()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            oracle.bao_cost_grad(len(c64), len(p64), len(oc32), c64.ctypes.data, p64.ctypes.data, oc32.ctypes.data,
+                                 op32.ctypes.data, uv64.ctypes.data, A.LOSS_HUBER, 1.0, C.byref(cost), C.byref(inv),
+                                 gc.ctypes.data, gp.ctypes.data)
+        sec = (time.perf_counter() - t0) / reps
+        out["cpu"] = {"obs_per_s": len(oc) / sec, "threads": 1, "kind": "port",
+                      "sample": "oracle cost and analytic gradient (FP64), one thread, mean of 3 calls"}
     return out
 
 
@@ -307,14 +445,16 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10, help="timed LM iterations")
     ap.add_argument("--warmup", type=int, default=2, help="untimed LM iterations (a separate solve)")
     ap.add_argument("--points", type=int, default=1_000_000)
-    ap.add_argument("--roofline-kernel", default="auto",
-                    choices=["auto", "pcg_product", "pcg_points", "pcg_cameras", "linearize"],
-                    help="auto: the PCG product kernel of the form the solve uses")
+    ap.add_argument("--profile-kernel", default="all",
+                    choices=["all", "none", "schur_pairs", "camera_assembly", "pcg_product", "pcg_points",
+                             "pcg_cameras", "linearize", "point_elim", "candidate"],
+                    help="kernels to time with HIP events (all: the per-kernel table; the roofline is the one "
+                         "with the largest total time)")
+    ap.add_argument("--schur-product", default="auto", choices=["auto", "explicit", "implicit"],
+                    help="PCG product form of every solve (auto: the library's per-iteration choice)")
     ap.add_argument("--converge-iterations", type=int, default=60,
-                    help="iteration cap of the run that fixes the converged cost (0 skips time-to-cost)")
-    ap.add_argument("--cpu-points", type=int, default=200_000, help="points of the CPU-baseline subsample")
-    ap.add_argument("--cpu-iterations", type=int, default=6)
-    ap.add_argument("--no-cpu", action="store_true")
+                    help="iteration cap of the runs to convergence (0 skips time-to-cost)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle's CPU legs")
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms' measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES between two device buffers three times (a known-size kernel that "
@@ -336,12 +476,19 @@ def main() -> int:
     from openmvg_ba import HipBundleAdjuster, SolveConfig, _abi as A
     from openmvg_ba import synthetic
 
+    product = {"auto": A.SCHUR_AUTO, "explicit": A.SCHUR_EXPLICIT, "implicit": A.SCHUR_IMPLICIT}[args.schur_product]
+
+    def adjuster():
+        b = HipBundleAdjuster(0)
+        b.handle.lib.ba_hip_set_schur_product(b.handle.ptr, product)
+        return b
+
     if args.only_form:
-        fn = {"medium_direct": lambda: bench_medium(HipBundleAdjuster(0)),
-              "mixed_large": lambda: bench_mixed(HipBundleAdjuster(0), synthetic.large_scene(n_points=args.points).problem,
+        fn = {"medium_direct": lambda: bench_medium(adjuster(), cpu=not args.no_cpu),
+              "mixed_large": lambda: bench_mixed(adjuster(), synthetic.large_scene(n_points=args.points).problem,
                                                  SolveConfig(linear_solver=A.SOLVER_ITERATIVE), args.steps, None),
-              "batched": lambda: bench_batched(HipBundleAdjuster(0)),
-              "differentiation": lambda: bench_differentiation(20)}[args.only_form]
+              "batched": lambda: bench_batched(adjuster(), cpu=not args.no_cpu),
+              "differentiation": lambda: bench_differentiation(20, cpu=not args.no_cpu)}[args.only_form]
         print(json.dumps({args.only_form: fn()}))
         return 0
 
@@ -360,7 +507,7 @@ def main() -> int:
     prob = scene.problem
     log(f"scene: {prob.n_poses} cameras, {prob.n_points} points, {prob.n_observations} observations "
         f"({time.perf_counter() - t0:.1f} s)")
-    ba = HipBundleAdjuster(0)
+    ba = adjuster()
     base = SolveConfig(linear_solver=A.SOLVER_ITERATIVE)
 
     # warmup: a separate short solve (module load, first-touch allocations)
@@ -382,69 +529,67 @@ def main() -> int:
     log(f"timed: {steps} iterations in {loop_s:.3f} s ({steps / loop_s:.2f} iter/s), setup {rep.seconds_setup:.2f} s, "
         f"call {call_s:.1f} s, PCG iterations {rep.pcg_iterations}, cost {rep.initial_cost:.6e} -> "
         f"{rep.final_cost:.6e}, status {rep.status}")
-
     form = {A.SCHUR_EXPLICIT: "explicit", A.SCHUR_IMPLICIT: "implicit"}.get(
         ba.handle.lib.ba_hip_last_schur_product(ba.handle.ptr), "unknown")
-    if args.roofline_kernel == "auto":
-        args.roofline_kernel = "pcg_product" if form == "explicit" else "pcg_points"
-    # roofline of the dominant kernel: HIP events around each of its launches
-    kernel_ids = {"pcg_points": A.PROFILE_PCG_POINTS, "pcg_cameras": A.PROFILE_PCG_CAMERAS,
-                  "linearize": A.PROFILE_LINEARIZE, "pcg_product": A.PROFILE_PCG_PRODUCT}
-    # the profiled kernel's form for every iteration of that solve (AUTO keeps the first,
-    # few-PCG-iteration LM iterations matrix-free)
-    prof_mode = {"pcg_product": A.SCHUR_EXPLICIT, "pcg_points": A.SCHUR_IMPLICIT,
-                 "pcg_cameras": A.SCHUR_IMPLICIT}.get(args.roofline_kernel, A.SCHUR_AUTO)
-    ba.handle.lib.ba_hip_set_schur_product(ba.handle.ptr, prof_mode)
-    prof = ba.solve(prob, base.with_(max_iterations=max(2, min(args.steps, 4)),
-                                     profile_kernel=kernel_ids[args.roofline_kernel]))
-    ba.handle.lib.ba_hip_set_schur_product(ba.handle.ptr, A.SCHUR_AUTO)
-    launches = int(prof.report.profile_launches)
-    avg_s = float(prof.report.profile_seconds) / max(launches, 1)
-    counts = lowered_counts(prob)
-    if args.roofline_kernel == "pcg_product":
-        counts.update(schur_counts(prob))
-    per_launch = algorithmic_bytes(args.roofline_kernel, counts, D)
-    achieved = per_launch / avg_s / 1e9
-    log(f"roofline {args.roofline_kernel}: {launches} launches, avg {avg_s * 1e6:.1f} us, "
-        f"{per_launch / 1e6:.1f} MB/launch, {achieved:.1f} GB/s")
 
-    # time-to-cost: first point at which an accepted state is within 1e-6 of the converged cost
+    # per-kernel table (HIP events) and the roofline of the kernel with the largest total time
+    table = []
+    if args.profile_kernel == "all":
+        table = kernel_table(ba, prob, base, steps, D)
+    elif args.profile_kernel != "none":
+        table = [r for r in kernel_table(ba, prob, base, steps, D) if r["kernel"] == args.profile_kernel]
+    roofline = None
+    if table:
+        top = table[0]
+        roofline = {"kernel": top["kernel"], "bound": "hbm", "achieved": top["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": top["frac"], "traffic": top["traffic"], "launches": top["launches"],
+                    "avg_launch_us": top["avg_launch_us"],
+                    "algorithmic_bytes_per_launch": top["algorithmic_bytes_per_launch"],
+                    "total_ms_in_timed_iterations": top["total_ms"]}
+
+    # the oracle's run on all host cores: CPU baseline and the converged cost C*
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(prob, steps, args.converge_iterations or 60)
+        log(f"cpu baseline: {cpu['value']:.4f} iter/s, C* {cpu['converged_cost']:.10e} in {cpu['converge_iterations']} "
+            f"iterations, time-to-cost {cpu['time_to_cost_seconds']} s ({time.perf_counter() - t0:.1f} s)")
+    if cpu is not None:
+        c_star, c_src = cpu["converged_cost"], "oracle run to convergence in this bench (CPU leg)"
+    else:
+        with open(os.path.join(ROOT, "tests", "golden", "large_pcg.json")) as fh:
+            c_star = float(json.load(fh)["converged"]["final_cost"])
+        c_src = "oracle run to convergence, committed fixture tests/golden/large_pcg.json"
+
+    # time-to-cost: first accepted state within 1e-6 of the oracle's converged cost
     ttc = None
     if args.converge_iterations > 0:
         conv = ba.solve(prob, base.with_(max_iterations=args.converge_iterations))
-        c_star = float(conv.report.final_cost)
-        tr = conv.trace
-        target = c_star * (1.0 + 1e-6)
-        hit = None
-        for i in range(len(tr["seconds"])):
-            if tr["accepted"][i] and tr["candidate_cost"][i] <= target:
-                hit = float(tr["seconds"][i])
-                break
-        ttc = {"fp64_seconds": hit, "fp64_iterations": int(conv.report.iterations),
-               "converged_cost": c_star, "status": conv.report.status,
-               "loop_seconds": float(conv.report.seconds_iterations), "mixed_seconds": None,
-               "reference_cost": "FP64 GPU run to convergence (the FP64 path matches the oracle per iteration)"}
-        log(f"converge: {conv.report.iterations} iterations, {conv.report.status}, cost {c_star:.9e}, "
-            f"time-to-cost {hit}")
+        hit, hit_it = first_hit(conv.trace, c_star * (1.0 + 1e-6))
+        ttc = {"fp64_seconds": hit, "fp64_iterations_to_cost": hit_it, "fp64_iterations": int(conv.report.iterations),
+               "fp64_final_cost": float(conv.report.final_cost), "oracle_converged_cost": c_star,
+               "fp64_final_cost_rel_error_vs_oracle": abs(conv.report.final_cost - c_star) / c_star,
+               "status": conv.report.status, "loop_seconds": float(conv.report.seconds_iterations),
+               "reference_cost": c_src,
+               "cpu_seconds": cpu["time_to_cost_seconds"] if cpu else None}
+        log(f"converge: {conv.report.iterations} iterations, {conv.report.status}, cost "
+            f"{conv.report.final_cost:.10e} (C* {c_star:.10e}), time-to-cost {hit}")
 
     forms = {}
     if not args.no_forms:
         forms["mixed"] = bench_mixed(ba, prob, base, args.steps, ttc)
         log(f"mixed: {forms['mixed']}")
-        forms["medium_direct"] = bench_medium(ba)
+        if ttc is not None:
+            ttc["mixed_seconds"] = forms["mixed"].get("time_to_cost_seconds")
+        forms["medium_direct"] = bench_medium(ba, cpu=not args.no_cpu)
         log(f"medium direct: {forms['medium_direct']}")
-        forms["batched"] = bench_batched(ba)
+        forms["batched"] = bench_batched(ba, cpu=not args.no_cpu)
         log(f"batched: {forms['batched']}")
-        forms["differentiation"] = bench_differentiation(20)
+        forms["differentiation"] = bench_differentiation(20, cpu=not args.no_cpu)
         log(f"differentiation: {forms['differentiation']}")
 
-    cpu = None
-    if not args.no_cpu:
-        t0 = time.perf_counter()
-        cpu = cpu_baseline(synthetic.large_scene, args.cpu_points, prob.n_observations, args.cpu_iterations)
-        log(f"cpu baseline: {cpu['value']:.4f} iter/s ({time.perf_counter() - t0:.1f} s)")
-
     value = steps / loop_s
+    pcg_cap = min(600, D)
     line = {
         "metric": "LM iter/s (2k-cam/1M-pt BA, FP64 GPU)",
         "value": value,
@@ -461,25 +606,18 @@ def main() -> int:
         "config": {
             "workload": (f"large BA: {prob.n_poses} cameras (f,k1,k2 free), {prob.n_points} points, "
                          f"{prob.n_observations} observations, geometric tracks, 2% outliers, Huber 1px, "
-                         "LM + block-Jacobi PCG (rtol 0.1) on the Schur system: explicit in the LM iterations "
-                         "after a PCG solve of 10 or more iterations, matrix-free otherwise (AUTO), seed 0xBA0003"),
+                         "LM + block-Jacobi PCG (forcing tolerance 0.1) on the Schur system, seed 0xBA0003"),
             "reduced_dimension": D,
             "precision": "fp64",
-            "schur_product": "auto" + (" (explicit system used)" if form == "explicit" else " (matrix-free only)"),
+            "pcg_cap": f"min(600, D) = {pcg_cap} PCG iterations per solve (the native cap; "
+                       "self_calibration_retained.rs:527-531)",
+            "schur_product": f"{args.schur_product} (" + ("explicit system used" if form == "explicit"
+                                                         else "matrix-free only") + ")",
             "pcg_iterations": int(rep.pcg_iterations),
+            "seconds_setup": float(rep.seconds_setup),
         },
-        "roofline": {
-            "kernel": args.roofline_kernel,
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.roofline_kernel, per_launch),
-            "launches": launches,
-            "avg_launch_us": avg_s * 1e6,
-            "algorithmic_bytes_per_launch": per_launch,
-        },
+        "roofline": roofline,
+        "kernels": table,
         "cpu_baseline": cpu,
         "time_to_cost": ttc,
         "forms": forms,

@@ -19,7 +19,6 @@
 
 #include "ba_kernels.hpp"
 
-#define HIP_OK_OR_IGNORE(expr) (void)(expr)
 
 namespace ba {
 namespace {
@@ -441,6 +440,7 @@ __global__ void __launch_bounds__(256) k_lower_to_f32(const double* S, uint32_t 
         for (uint32_t c = blockIdx.x * 256 + threadIdx.x; c <= row && c < n; c += 256 * gridDim.x)
             F[(size_t)row * ld + c] = (float)(S[(size_t)row * ld + c] * scale[row] * scale[c]);
 }
+__global__ void k_clear_flag(int* flag) { *flag = 0; }
 // y = scale * v (elementwise), skipped once `done` is set
 __global__ void __launch_bounds__(256) k_scale_vec(uint32_t n, const double* scale, const double* v, double* y,
                                                    const int* done) {
@@ -523,7 +523,7 @@ void launch_cholesky_solve_mixed(hipStream_t s, const double* S, float* F, uint3
     hipLaunchKernelGGL(k_jacobi_scale, dim3(nb), dim3(256), 0, s, S, n, ld, scale);
     hipLaunchKernelGGL(k_lower_to_f32, dim3(nb, n < 65535u ? n : 65535u), dim3(256), 0, s, S, n, ld,
                        (const double*)scale, F);
-    HIP_OK_OR_IGNORE(hipMemsetAsync(done, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, done);  // a kernel: no memset node in a captured graph
     // x = D^-1/2 F^-1 D^-1/2 b, the forward solve advancing with the factor
     hipLaunchKernelGGL(k_scale_vec, dim3(nb), dim3(256), 0, s, n, (const double*)scale, b, work_w(work, n),
                        (const int*)nullptr);

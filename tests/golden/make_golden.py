@@ -1,6 +1,8 @@
 """Writes the golden fixtures in tests/golden/ from the oracle.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # the ring-scene fixtures (seconds)
+    python tests/golden/make_golden.py --scale    # the medium, large and window fixtures (minutes;
+                                                  # OMP_NUM_THREADS threads, same bits at any count)
 
 Every fixture is oracle-generated (the oracle restates the reference algorithm; its header lists
 the reference files it follows) on the seeded 10-camera ring scene, and records a SHA-256 of the
@@ -13,6 +15,15 @@ scene's input arrays so that a test regenerating the scene can tell it is the sa
                            assembly scalars at lambda = 1e-4 (squared loss)
   ring_solve.json          LM trajectories (cost, damping, accepted) and final parameters of the
                            direct and the PCG solve, 50 iterations (squared loss)
+  medium_direct.json       200-camera / 50,000-point scene (synthetic.medium_scene), direct step,
+                           10 LM iterations: trajectory, final cost, final camera parameters and
+                           every 97th point; and the run to convergence (at most 50 iterations)
+  large_pcg.json           2,000-camera / 1M-point scene (synthetic.large_scene), PCG step: the
+                           first 5 LM iterations (trajectory, PCG iterations, final cost) and the
+                           run to convergence (at most 60 iterations, the bench's converged-cost
+                           run): its trajectory and converged cost C*
+  windows.json             the 512-window batch (synthetic.window_scenes): direct-step solves of 10
+                           iterations of the 4 largest windows and 4 seeded others
 """
 
 from __future__ import annotations
@@ -101,5 +112,82 @@ def main() -> None:
         print(name, os.path.getsize(os.path.join(HERE, name + ".json")), "bytes")
 
 
+def trajectory(sol) -> dict:
+    tr = sol.trace
+    return dict(cost=floats(tr["cost"]), damping=floats(tr["damping"]), candidate_cost=floats(tr["candidate_cost"]),
+                accepted=ints(tr["accepted"]), pcg_iterations=ints(tr["pcg_iterations"]),
+                seconds=floats(tr["seconds"]), final_cost=float(sol.report.final_cost),
+                status=str(sol.report.status), iterations=int(sol.report.iterations),
+                rejected_candidate=int(sol.report.rejected_candidate))
+
+
+def scale_fixtures() -> None:
+    import time
+
+    oracle = load_oracle()
+    threads = os.environ.get("OMP_NUM_THREADS", "default")
+    out = {}
+    # medium: direct step, 10 iterations
+    scene = synthetic.medium_scene()
+    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT)
+    t0 = time.time()
+    sol = solve_with(oracle, "bao_solve", scene.problem, cfg)
+    obj = dict(scene="synthetic.medium_scene()", scene_sha256=scene_digest(scene.problem),
+               made_by="tests/golden/make_golden.py --scale", source="oracle (C++ FP64 restatement)",
+               config=dict(max_iterations=10, linear_solver="direct"), omp_threads=threads,
+               wall_seconds=time.time() - t0, **trajectory(sol),
+               pose_rotation=floats(sol.pose_rotation), pose_center=floats(sol.pose_center),
+               intrinsic_focal=floats(sol.intrinsic_focal), intrinsic_k1=floats(sol.intrinsic_k1),
+               intrinsic_k2=floats(sol.intrinsic_k2), point_stride=97, point_xyz=floats(sol.point_xyz[::97]))
+    t0 = time.time()
+    conv = solve_with(oracle, "bao_solve", scene.problem, SolveConfig(max_iterations=50, linear_solver=A.SOLVER_DIRECT))
+    obj["converged"] = dict(config=dict(max_iterations=50, linear_solver="direct"), wall_seconds=time.time() - t0,
+                            **trajectory(conv), pose_center=floats(conv.pose_center),
+                            intrinsic_focal=floats(conv.intrinsic_focal), intrinsic_k1=floats(conv.intrinsic_k1),
+                            intrinsic_k2=floats(conv.intrinsic_k2), point_xyz=floats(conv.point_xyz[::97]))
+    out["medium_direct"] = obj
+    print("medium", obj["final_cost"], obj["wall_seconds"], "converged", conv.report.final_cost,
+          conv.report.iterations, flush=True)
+    # large: PCG step, 5 iterations, then the converged run
+    scene = synthetic.large_scene()
+    head = dict(scene="synthetic.large_scene()", scene_sha256=scene_digest(scene.problem),
+                made_by="tests/golden/make_golden.py --scale", source="oracle (C++ FP64 restatement)",
+                omp_threads=threads)
+    t0 = time.time()
+    five = solve_with(oracle, "bao_solve", scene.problem, SolveConfig(max_iterations=5, linear_solver=A.SOLVER_ITERATIVE))
+    t1 = time.time()
+    print("large 5", five.report.final_cost, t1 - t0, flush=True)
+    conv = solve_with(oracle, "bao_solve", scene.problem, SolveConfig(max_iterations=60, linear_solver=A.SOLVER_ITERATIVE))
+    t2 = time.time()
+    print("large conv", conv.report.final_cost, conv.report.iterations, t2 - t1, flush=True)
+    out["large_pcg"] = dict(head, first=dict(config=dict(max_iterations=5, linear_solver="iterative"),
+                                             wall_seconds=t1 - t0, **trajectory(five)),
+                            converged=dict(config=dict(max_iterations=60, linear_solver="iterative"),
+                                           wall_seconds=t2 - t1, **trajectory(conv)))
+    # windows: the 4 largest and 4 seeded others
+    wins = synthetic.window_scenes()
+    sizes = np.array([w.problem.n_observations for w in wins])
+    pick = list(np.argsort(-sizes, kind="stable")[:4])
+    rng = np.random.default_rng(0xBA0004)
+    pick += [int(i) for i in rng.choice(np.setdiff1d(np.arange(len(wins)), pick), 4, replace=False)]
+    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT)
+    wl = []
+    for i in pick:
+        sol = solve_with(oracle, "bao_solve", wins[i].problem, cfg)
+        wl.append(dict(window=int(i), scene_sha256=scene_digest(wins[i].problem),
+                       n_observations=int(wins[i].problem.n_observations), **trajectory(sol),
+                       point_xyz_digest=float(np.sum(sol.point_xyz * sol.point_xyz))))
+    out["windows"] = dict(scene="synthetic.window_scenes()", made_by="tests/golden/make_golden.py --scale",
+                          source="oracle (C++ FP64 restatement)", config=dict(max_iterations=10, linear_solver="direct"),
+                          windows=wl)
+    for name, obj in out.items():
+        with open(os.path.join(HERE, name + ".json"), "w") as fh:
+            json.dump(obj, fh, separators=(",", ":"))
+        print(name, os.path.getsize(os.path.join(HERE, name + ".json")), "bytes")
+
+
 if __name__ == "__main__":
-    main()
+    if "--scale" in sys.argv:
+        scale_fixtures()
+    else:
+        main()
