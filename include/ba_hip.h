@@ -400,6 +400,18 @@ int ba_hip_pcg(ba_hip_handle* handle, const ba_device_problem* problem, double l
 int ba_hip_cholesky_solve(ba_hip_handle* handle, size_t n, const double* a, const double* b,
                           double* x, int32_t precision);
 
+/* Allocation-free form of ba_hip_cholesky_solve (same kernels, same results, same errors) for
+ * callers that cannot allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  No counterpart in the reference.  Its workspace (the matrix
+ * copy or FP32 factor, the factor's scratch and two flags) lives in a fixed device array that each
+ * device holds for the life of the process (about 33 MiB), resolved per device by ba_hip_create;
+ * the call makes no hipMalloc, hipFree or other allocating call.  n above
+ * BA_NOALLOC_MAX_DIMENSION returns BA_ERR_RESOURCE_EXHAUSTED.  Calls on one device share its
+ * array: calls that overlap on one device (from several handles or threads) race. */
+#define BA_NOALLOC_MAX_DIMENSION 2048
+int ba_hip_cholesky_solve_noalloc(ba_hip_handle* handle, size_t n, const double* a, const double* b,
+                                  double* x, int32_t precision);
+
 /* ---- Batched form -------------------------------------------------------------------------
  * Many independent windows (local bundle adjustments) solved together with the direct camera
  * step: the result of each window equals ba_hip_solve on that window alone (same per-window LM

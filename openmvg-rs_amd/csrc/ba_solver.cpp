@@ -392,6 +392,13 @@ int ba_hip_create(int device, ba_hip_handle** out) {
         delete h;
         return BA_ERR_INTERNAL;
     }
+    if (hipHostMalloc((void**)&h->pinned, 4 * sizeof(double)) != hipSuccess) {
+        h->pinned = nullptr;
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return BA_ERR_INTERNAL;
+    }
+    ba::noalloc_prepare_device(device);  // with `device` current
     *out = h;
     return BA_OK;
 }
@@ -408,6 +415,7 @@ void ba_hip_destroy(ba_hip_handle* h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->pinned) (void)hipHostFree(h->pinned);
     delete h;
 }
 

@@ -39,6 +39,7 @@ def load() -> C.CDLL:
     lib.ba_hip_schur.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp]
     lib.ba_hip_pcg.argtypes = [vp, vp, C.c_double, C.c_double, C.c_uint32, vp, vp, vp, vp]
     lib.ba_hip_cholesky_solve.argtypes = [vp, C.c_size_t, vp, vp, vp, C.c_int32]
+    lib.ba_hip_cholesky_solve_noalloc.argtypes = [vp, C.c_size_t, vp, vp, vp, C.c_int32]
     lib.ba_hip_set_schur_product.argtypes = [vp, C.c_int32]
     lib.ba_hip_last_schur_product.argtypes = [vp]
     lib.ba_hip_last_schur_product.restype = C.c_int32
