@@ -122,19 +122,19 @@ def algorithmic_bytes(kernel: str, counts: dict, D: int) -> float:
       the 9 x 9 block (648 B) and its neighbour index (4 B); per camera pair the written column
       sums (72 B); per camera the 9 input values (72 B), its offsets (16 B) and the written row
       sums (72 B).
-    schur_pairs (k_schur_pairs: the off-diagonal blocks of the explicit S): each observation's Z
-      record once (27 values, 216 B), 8 B per observation pair, and per camera pair the written
-      9 x 9 block and its indices (656 B).
+    schur_pairs (k_schur_pairs: the off-diagonal blocks of the explicit S, rebuilt from compact
+      records): each observation's 16-value record once (128 B), 8 B per observation pair, and per
+      camera pair the written 9 x 9 block and its indices (656 B).
     camera_assembly (k_camera_assembly with the explicit system): per observation the
-      camera-major rows read (72 B) and point index (4 B) and the Z record written (216 B); per
-      point its packed record (128 B); per segment its camera (4 B) and written sums (576 B); per
-      camera R, f, k1, k2 (120 B).
+      camera-major rows read (72 B) and point index (4 B) and the compact record written (128 B);
+      per point its packed record (128 B); per segment its camera (4 B) and written sums (576 B);
+      per camera R, f, k1, k2 (120 B).
     """
     no, npt, nc = counts["obs"], counts["points"], counts["cameras"]
     if kernel == "schur_pairs":
-        return no * 216 + counts["terms"] * 8 + counts["pairs"] * (648 + 8)
+        return no * 128 + counts["terms"] * 8 + counts["pairs"] * (648 + 8)
     if kernel == "camera_assembly":
-        return no * (72 + 4 + 216) + npt * 128 + counts["segments"] * (4 + 576) + nc * 120
+        return no * (72 + 4 + 128) + npt * 128 + counts["segments"] * (4 + 576) + nc * 120
     if kernel == "pcg_product":
         return counts["slots"] * (648 + 4) + counts["pairs"] * 72 + nc * (72 + 16 + 72)
     if kernel == "pcg_points":

@@ -338,7 +338,7 @@ class Engine {
         const size_t nc = L_.n_cameras(), no = L_.n_obs();
         const uint64_t terms = count_schur_terms(L_);
         const uint64_t pairs = std::min<uint64_t>(terms, (uint64_t)nc * (nc > 0 ? nc - 1 : 0) / 2);
-        const double bytes = (double)sizeof(T) * ((double)ZREC * no + 81.0 * ((double)nc + (double)pairs)) +
+        const double bytes = (double)sizeof(T) * ((double)CREC * no + 81.0 * ((double)nc + (double)pairs)) +
                              8.0 * (double)terms + 88.0 * (double)pairs;
         if (schur_mode_ != BA_SCHUR_EXPLICIT) {
             size_t fr = 0, tot = 0;
@@ -370,7 +370,7 @@ class Engine {
             sgroup_.upload(grp, s_);
             sgroup_n_.upload(grn, s_);
         }
-        zrec_.alloc((size_t)ZREC * std::max<size_t>(1, no));
+        crec_.alloc((size_t)CREC * std::max<size_t>(1, no));
         svals_.alloc(SL.n_values);
         pvexp_.alloc(9 * std::max<size_t>(1, nc));
         stpair_.alloc(9 * std::max<size_t>(1, npairs_));
@@ -573,11 +573,14 @@ class Engine {
             timed(BA_PROFILE_PAIRS, [&] {
                 launch_camera_assembly<T>(s_, tp, lin(), pe(), R_[state].p, gp_[state].p, gauge_.p, nseg_,
                                           seg_cam_.p, cseg_off_.p, casm_.p, diag_pair_.p, value_off_.p, grad_.p,
-                                          fgrad_.p, diag_.p, u_vals_.p, s_vals_.p, ex ? zrec_.p : nullptr,
+                                          fgrad_.p, diag_.p, u_vals_.p, s_vals_.p, ex ? crec_.p : nullptr,
                                           ex ? schur_rows() : SchurRows<T>{});
             });
             if (ex)
-                timed(BA_PROFILE_SCHUR_PAIRS, [&] { launch_schur_pairs<T>(s_, schur_pairs(), zrec_.p, schur_rows()); });
+                timed(BA_PROFILE_SCHUR_PAIRS, [&] {
+                    launch_schur_pairs<T>(s_, schur_pairs(), crec_.p, R_[state].p, gp_[state].p, cam_group_.p,
+                                          cam_dim_.p, gauge_.p, schur_rows());
+                });
         } else {
             launch_block_gradient<T>(s_, tp, lin(), pe(), grad_.p, fgrad_.p, diag_.p);
         }
@@ -831,7 +834,7 @@ class Engine {
     bool use_explicit_ = false;  // ... and the current LM iteration's PCG uses it
     bool used_explicit_ = false;
     uint32_t npairs_ = 0;
-    Buf<T> zrec_, svals_;
+    Buf<T> crec_, svals_;
     Buf<uint64_t> srow_base_;
     Buf<uint32_t> snbr_off_, snbr_, spair_off_, srev_off_, srev_pair_, spair_cam_, sterm_off_, sterm_, sgroup_;
     Buf<uint8_t> sgroup_n_;

@@ -875,7 +875,6 @@ __device__ __forceinline__ void rebuild_jacobians(const Obs& o, const Cam& cam, 
 }
 }  // namespace
 
-constexpr int ZS_STRIDE = ZREC + 1;  // LDS row of a staged Z record (odd: fewer bank conflicts)
 // Orders one wavefront's LDS accesses around this point (its lanes exchange data through LDS
 // without a workgroup barrier; a wavefront's LDS instructions complete in issue order).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -889,7 +888,7 @@ __global__ void __launch_bounds__(192) k_camera_assembly(Topo tp, Lin<T> lin, Po
                                                          const double* gp, const GaugeDev* gauge,
                                                          const uint32_t* __restrict__ seg_cam,
                                                          const uint32_t* __restrict__ cseg_off, double* part,
-                                                         T* __restrict__ zrec) {
+                                                         T* __restrict__ crec) {
     const uint32_t k = blockIdx.x;
     const uint32_t lane = threadIdx.x & 63, role = threadIdx.x >> 6;  // role uniform per wavefront
     constexpr uint32_t VN = pcg_vec<T>(), SEG = pcg_seg<T>();
@@ -903,7 +902,7 @@ __global__ void __launch_bounds__(192) k_camera_assembly(Topo tp, Lin<T> lin, Po
     cam.f = g3[0], cam.k1 = g3[1], cam.k2 = g3[2];
     const uint32_t pd = tp.cam_dim[c];
     const bool baseline = (int)c == gg.camera;
-    __shared__ T zs[64 * ZS_STRIDE];  // role 0's Z records of one pass (padded rows)
+    __shared__ __align__(16) T cs[64 * CREC];  // role 0's compact records of one pass
     double acc[27];
 #pragma unroll
     for (int q = 0; q < 27; ++q) acc[q] = 0.0;
@@ -940,39 +939,31 @@ __global__ void __launch_bounds__(192) k_camera_assembly(Topo tp, Lin<T> lin, Po
                 acc[9 + q] += jc[0][q] * u0 + jc[1][q] * u1;
                 acc[18 + q] += jc[0][q] * jc[0][q] + jc[1][q] * jc[1][q];
             }
-            if (zrec) {
-                // M M^T = V^-1 (lower Cholesky; a non-positive pivot drops its column, as for the
-                // zero V^-1 of fixed or not eliminated points), Z = M^T (J~_X^T J~_c); staged in LDS
-                // so that the wavefront's 64 consecutive records are written coalesced
-                const double* vi = pv;
-                const double m00 = vi[0] > 0.0 ? sqrt(vi[0]) : 0.0;
-                const double i00 = m00 > 0.0 ? 1.0 / m00 : 0.0;
-                const double m10 = vi[1] * i00, m20 = vi[3] * i00;
-                const double p11 = vi[2] - m10 * m10;
-                const double m11 = p11 > 0.0 ? sqrt(p11) : 0.0;
-                const double m21 = m11 > 0.0 ? (vi[4] - m20 * m10) / m11 : 0.0;
-                const double p22 = vi[5] - m20 * m20 - m21 * m21;
-                const double m22 = p22 > 0.0 ? sqrt(p22) : 0.0;
-                T* zo = zs + ZS_STRIDE * lane;
+            if (crec) {
+                // the compact record of this position for the pair assembly, staged in LDS so that
+                // the wavefront's 64 consecutive records are written as one contiguous run of
+                // 16-byte stores (only live positions)
+                T* mine = cs + CREC * lane;
+                mine[0] = lin.jcm[soa_at(SOA_Q, j)];
+                mine[1] = lin.jcm[soa_at(SOA_Q + 1, j)];
+                mine[2] = lin.jcm[soa_at(SOA_Q + 2, j)];
+                mine[3] = lin.jcm[soa_at(SOA_Q + 3, j)];
+                mine[4] = lin.jcm[soa_at(SOA_RX, j)];
+                mine[5] = lin.jcm[soa_at(SOA_RX + 1, j)];
+                mine[6] = lin.jcm[soa_at(SOA_RX + 2, j)];
+                mine[7] = T(0);
 #pragma unroll
-                for (int q = 0; q < 9; ++q) {
-                    const double w0 = a[0] * jc[0][q] + a[3] * jc[1][q];
-                    const double w1 = a[1] * jc[0][q] + a[4] * jc[1][q];
-                    const double w2 = a[2] * jc[0][q] + a[5] * jc[1][q];
-                    zo[q] = (T)(m00 * w0 + m10 * w1 + m20 * w2);
-                    zo[9 + q] = (T)(m11 * w1 + m21 * w2);
-                    zo[18 + q] = (T)(m22 * w2);
-                }
-                zo[27] = T(0);
+                for (int q = 0; q < 6; ++q) mine[8 + q] = (T)pv[q];
+                mine[14] = T(0);
+                mine[15] = T(0);
                 wave_lds_sync();
-                // the records of positions wb + 64 it .. + 63 are contiguous: element e of that span
-                // is value e % ZREC of record e / ZREC; only live records are written
-                T* dst = zrec + (size_t)ZREC * (wb + 64 * it);
-#pragma unroll 2
-                for (int i = 0; i < ZREC; ++i) {
-                    const uint32_t e = lane + 64 * i, rec = e / ZREC;
-                    const uint32_t jr = wb + 64 * it + rec;
-                    if (jr >= cb && jr < ce) dst[e] = zs[ZS_STRIDE * rec + e % ZREC];
+                constexpr int CH = CREC * (int)sizeof(T) / 16;  // 16-byte chunks per record
+                const uint4* src = reinterpret_cast<const uint4*>(cs);
+                uint4* dst = reinterpret_cast<uint4*>(crec + (size_t)CREC * (wb + 64 * it));
+#pragma unroll
+                for (int i = 0; i < CH; ++i) {
+                    const uint32_t e = lane + 64 * i, jr = wb + 64 * it + e / CH;
+                    if (jr >= cb && jr < ce) dst[e] = src[e];
                 }
                 wave_lds_sync();
             }
@@ -2243,83 +2234,151 @@ struct PairMfma<float> {
 };
 }  // namespace
 
-// Off-diagonal blocks, one wavefront per camera pair (c, j): S_cj = -sum_t Z_o(t)^T Z_o'(t), the
-// sum over the 3 rows of every term's records of their outer products, four rows per 16x16x4
-// matrix-core step (lane l supplies value l%16 of row l/16 of the step's rows of Z_o and Z_o';
-// columns 9..15 and rows past the last term are zero; packing four rows per step instead of one
-// term's three takes a quarter of the steps and loads away).  Terms go in batches of SP_DEPTH = 4
-// (12 rows, three steps) whose loads are issued one batch ahead.  Workgroup slot b holds up to
-// SP_GROUP pairs of one camera row (P.group): rows sweep in ascending order over the whole chip,
-// so at any time every XCD gathers from the same band of cameras' records, which the shared
-// last-level cache holds (dealing each XCD a contiguous pair range measured 1.5x slower), while
-// each row stays on one XCD, whose L2 then serves the row's own records to all of its pairs.
-// The block goes to its slot in row c.
+// Off-diagonal blocks, one wavefront per camera pair (c, j):
+//   S_cj = -sum_t J~_c,o^T K_t J~_c,o',   K_t = J~_X,o V^-1 J~_X,o'^T (2 x 2),
+// i.e. -sum_t W_o V^-1 W_o'^T, over the pair's terms t (the observations o by c and o' by j of
+// each common free point, in point order).  The terms go in rounds of 64: lane l rebuilds term
+// l's Jacobians from the two observations' compact records (64 bytes each in FP64), the point's
+// V^-1 and the two cameras, exactly as the camera-side assembly does, forms T = K J~_c,o' and
+// stages the rows of J~_c,o and T in LDS; then the rank-2 updates run two terms per 16x16x4
+// matrix-core step (lane l supplies value l%16 of row l/16 of the step's four rows; values 9..15
+// are zero), accumulating the block in the wavefront's tile.  A term reads 2 x 64 + 48 bytes
+// (instead of two 216-byte precomputed 3 x 9 records), and a camera row's own records and points
+// are reread from the L2 of its XCD.  Workgroup slot b holds up to SP_GROUP pairs of one camera
+// row (P.group): rows sweep in ascending order over the whole chip, so at any time every XCD
+// gathers from the same band of cameras' records, which the shared last-level cache holds, while
+// each row stays on one XCD.  The block goes to its slot in row c.
 constexpr int SP_WAVES = (int)SP_GROUP;
-constexpr int SP_DEPTH = 4;  // terms per batch: 12 record rows, three whole steps
+// The first `n` values of a compact record (n a multiple of 4), as doubles.
+template <class T, int N>
+__device__ __forceinline__ void load_crec(const T* __restrict__ cr, uint32_t pos, double v[N]) {
+    if constexpr (sizeof(T) == 8) {
+        const double2* q = reinterpret_cast<const double2*>(cr + (size_t)CREC * pos);
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) {
+            const double2 x = q[i];
+            v[2 * i] = x.x;
+            v[2 * i + 1] = x.y;
+        }
+    } else {
+        const float4* q = reinterpret_cast<const float4*>(cr + (size_t)CREC * pos);
+#pragma unroll
+        for (int i = 0; i < N / 4; ++i) {
+            const float4 x = q[i];
+            v[4 * i] = x.x;
+            v[4 * i + 1] = x.y;
+            v[4 * i + 2] = x.z;
+            v[4 * i + 3] = x.w;
+        }
+    }
+}
 template <class T>
-__global__ void __launch_bounds__(64 * SP_WAVES) k_schur_pairs(SchurPairs P, const T* __restrict__ z, SchurRows<T> S) {
+__global__ void __launch_bounds__(64 * SP_WAVES) k_schur_pairs(SchurPairs P, const T* __restrict__ cr,
+                                                             const double* __restrict__ Rg,
+                                                             const double* __restrict__ gp,
+                                                             const uint32_t* __restrict__ cam_group,
+                                                             const uint32_t* __restrict__ cam_dim,
+                                                             const GaugeDev* __restrict__ gauge, SchurRows<T> S) {
     typedef typename PairMfma<T>::v4 v4;
+    __shared__ T sa[SP_WAVES][32][2][9];  // rows of J~_c,o per staged term
+    __shared__ T sb[SP_WAVES][32][2][9];  // rows of T = K J~_c,o'
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar loads below
-    if (w >= P.group_n[blockIdx.x]) return;  // whole wavefronts leave; no barrier follows
+    if (w >= P.group_n[blockIdx.x]) return;  // whole wavefronts leave; no workgroup barrier follows
     const uint32_t k = P.group[blockIdx.x] + w;
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-    // A batch of SP_DEPTH = 4 terms is 12 rows (term u, record row r % 3 for row r = 3u + r % 3),
-    // four per matrix-core step: step s takes rows 4s + lk, i.e. term (4s + lk) / 3, which is
-    // one of the two terms (4s) / 3 and (4s) / 3 + 1.
-    uint32_t off[3];
-    bool hi[3];
+    const uint32_t c = P.cam[2 * k], cj = P.cam[2 * k + 1];
+    const GaugeDev& gg = *gauge;
+    Cam camc, camj;
 #pragma unroll
-    for (int s = 0; s < 3; ++s) {
-        const int r = 4 * s + lk;
-        off[s] = li < 9 ? (uint32_t)((r % 3) * 9 + li) : 0u;  // every lane reads inside the record
-        hi[s] = r / 3 != (4 * s) / 3;
+    for (int i = 0; i < 9; ++i) {
+        camc.R[i] = Rg[9 * (size_t)c + i];
+        camj.R[i] = Rg[9 * (size_t)cj + i];
     }
+    {
+        const double* g3 = gp + 3 * (size_t)cam_group[c];
+        camc.f = g3[0], camc.k1 = g3[1], camc.k2 = g3[2];
+        const double* h3 = gp + 3 * (size_t)cam_group[cj];
+        camj.f = h3[0], camj.k1 = h3[1], camj.k2 = h3[2];
+    }
+    const uint32_t pdc = cam_dim[c], pdj = cam_dim[cj];
+    const bool basec = (int)c == gg.camera, basej = (int)cj == gg.camera;
+    v4 acc = v4{T(0), T(0), T(0), T(0)}, acc2 = acc;
     const uint32_t t0 = P.term_off[k], t1 = P.term_off[k + 1];
-    auto load = [&](uint32_t t, T* x, T* y) {
-        uint32_t tc[SP_DEPTH], tj[SP_DEPTH];
+    const int r = lk & 1, lc = li < 9 ? li : 0;
+    const T keep = li < 9 ? T(1) : T(0);
+    for (uint32_t base = t0; base < t1; base += 64) {
+        double jo[2][9], tm[2][9];
+        const uint32_t t = base + (uint32_t)lane;
+        if (t < t1) {
+            const uint32_t pc = P.term[2 * t], pj = P.term[2 * t + 1];
+            double ro[CREC], rj[CREC_HALF];
+            load_crec<T, CREC>(cr, pc, ro);
+            load_crec<T, CREC_HALF>(cr, pj, rj);
+            const Obs oo = make_obs(ro[0], ro[1], ro[2], ro[3], camc);
+            const Obs oj = make_obs(rj[0], rj[1], rj[2], rj[3], camj);
+            const double RXo[3] = {ro[4], ro[5], ro[6]}, RXj[3] = {rj[4], rj[5], rj[6]};
+            const double* vi = ro + 8;  // V^-1 of the term's point
+            double xo[6], jcj[2][9], xj[6];
+            rebuild_jacobians(oo, camc, RXo, pdc, basec, gg, jo, xo);
+            rebuild_jacobians(oj, camj, RXj, pdj, basej, gg, jcj, xj);
+            double u0[3], u1[3];  // V^-1 J~_X,o'^T
+            sym3_mul(vi, xj, u0);
+            sym3_mul(vi, xj + 3, u1);
+            const double k00 = xo[0] * u0[0] + xo[1] * u0[1] + xo[2] * u0[2];
+            const double k01 = xo[0] * u1[0] + xo[1] * u1[1] + xo[2] * u1[2];
+            const double k10 = xo[3] * u0[0] + xo[4] * u0[1] + xo[5] * u0[2];
+            const double k11 = xo[3] * u1[0] + xo[4] * u1[1] + xo[5] * u1[2];
 #pragma unroll
-        for (int u = 0; u < SP_DEPTH; ++u) {
-            const uint32_t tt = t + u < t1 ? t + u : t1 - 1;
-            tc[u] = P.term[2 * tt];
-            tj[u] = P.term[2 * tt + 1];
+            for (int b = 0; b < 9; ++b) {
+                tm[0][b] = k00 * jcj[0][b] + k01 * jcj[1][b];
+                tm[1][b] = k10 * jcj[0][b] + k11 * jcj[1][b];
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < 9; ++b) jo[0][b] = jo[1][b] = tm[0][b] = tm[1][b] = 0.0;
         }
+        // the round's 64 terms are staged 32 at a time (half the LDS of staging all of them)
+        const uint32_t n = t1 - base < 64u ? t1 - base : 64u;
+        for (uint32_t half = 0; half < 2 && 32 * half < n; ++half) {
+            wave_lds_sync();  // the previous steps have read the staged rows
+            if ((uint32_t)lane >= 32 * half && (uint32_t)lane < 32 * half + 32) {
+                const uint32_t sl = (uint32_t)lane - 32 * half;
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            const int u0 = (4 * s) / 3;
-            const uint32_t pc = hi[s] ? tc[u0 + 1] : tc[u0];
-            const uint32_t pj = hi[s] ? tj[u0 + 1] : tj[u0];
-            x[s] = z[(size_t)ZREC * pc + off[s]];
-            y[s] = z[(size_t)ZREC * pj + off[s]];
-        }
+                for (int rr = 0; rr < 2; ++rr)
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            const uint32_t u = (uint32_t)((4 * s + lk) / 3);
-            if (!(li < 9 && t + u < t1)) x[s] = y[s] = T(0);
-        }
-    };
-    v4 acc = v4{T(0), T(0), T(0), T(0)};
-    if (t0 < t1) {
-        T xa[3], ya[3];
-        load(t0, xa, ya);
-        for (uint32_t t = t0; t < t1; t += SP_DEPTH) {
-            T xn[3], yn[3];
-            if (t + SP_DEPTH < t1) load(t + SP_DEPTH, xn, yn);
+                    for (int b = 0; b < 9; ++b) {
+                        sa[w][sl][rr][b] = (T)jo[rr][b];
+                        sb[w][sl][rr][b] = (T)tm[rr][b];
+                    }
+            }
+            wave_lds_sync();
+            // steps in groups of four (eight terms; rows past the last term are the zeros staged
+            // by the lanes without a term): every LDS read of a group is issued before its
+            // matrix-core steps, which alternate between two accumulators
+            const uint32_t m = n - 32 * half < 32u ? n - 32 * half : 32u;
+            const uint32_t groups = (m + 7) / 8;
+            for (uint32_t gi = 0; gi < groups; ++gi) {
+                T a[4], b[4];
 #pragma unroll
-            for (int s = 0; s < 3; ++s) acc = PairMfma<T>::mac(xa[s], ya[s], acc);
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t tl = 8 * gi + 2 * u + (uint32_t)(lk >> 1);
+                    a[u] = sa[w][tl][r][lc];
+                    b[u] = sb[w][tl][r][lc];
+                }
 #pragma unroll
-            for (int s = 0; s < 3; ++s) {
-                xa[s] = xn[s];
-                ya[s] = yn[s];
+                for (int u = 0; u < 4; u += 2) {
+                    acc = PairMfma<T>::mac(a[u] * keep, b[u], acc);
+                    acc2 = PairMfma<T>::mac(a[u + 1] * keep, b[u + 1], acc2);
+                }
             }
         }
     }
-    const uint32_t c = P.cam[2 * k];
     const uint32_t nnc = S.nbr_off[c + 1] - S.nbr_off[c];
     T* rc = S.vals + S.row_base[c] + 9 * (size_t)(1 + k - S.pair_off[c]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int a = PairMfma<T>::row(lk, v), b = li;
-        if (a < 9 && b < 9) rc[(size_t)a * schur_row_len(nnc) + b] = -acc[v];
+        if (a < 9 && b < 9) rc[(size_t)a * schur_row_len(nnc) + b] = -(acc[v] + acc2[v]);
     }
 }
 
@@ -2619,9 +2678,11 @@ void launch_pcg_step_explicit(hipStream_t s, const Topo& tp, const PcgVecs& V) {
     hipLaunchKernelGGL(k_pcg_dir, dim3(g), dim3(NTS), 0, s, tp, V);
 }
 template <class T>
-void launch_schur_pairs(hipStream_t s, const SchurPairs& P, const T* z, SchurRows<T> S) {
+void launch_schur_pairs(hipStream_t s, const SchurPairs& P, const T* crec, const double* R, const double* gp,
+                        const uint32_t* cam_group, const uint32_t* cam_dim, const GaugeDev* gauge, SchurRows<T> S) {
     if (P.npairs && P.ngroups)
-        hipLaunchKernelGGL(k_schur_pairs<T>, dim3(P.ngroups), dim3(64 * SP_WAVES), 0, s, P, z, S);
+        hipLaunchKernelGGL(k_schur_pairs<T>, dim3(P.ngroups), dim3(64 * SP_WAVES), 0, s, P, crec, R, gp, cam_group,
+                           cam_dim, gauge, S);
 }
 template <class T>
 void launch_pcg_cameras(hipStream_t s, const Topo& tp, const Lin<T> lin, const PcgVecs& V) {
@@ -2655,7 +2716,8 @@ template void launch_gauge<double>(hipStream_t, const double*, int, double, Gaug
                                             const uint32_t*, const uint32_t*, double*, const int32_t*,             \
                                             const uint64_t*, double*, double*, double*, double*, double*, T*,      \
                                             SchurRows<T>);                                                         \
-    template void launch_schur_pairs<T>(hipStream_t, const SchurPairs&, const T*, SchurRows<T>);                    \
+    template void launch_schur_pairs<T>(hipStream_t, const SchurPairs&, const T*, const double*, const double*,      \
+                                        const uint32_t*, const uint32_t*, const GaugeDev*, SchurRows<T>);           \
     template void launch_pcg_product_upper<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&);                  \
     template void launch_pcg_product_lower<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&);                  \
     template void launch_pcg_step_explicit<T>(hipStream_t, const Topo&, const PcgVecs&);                            \

@@ -169,12 +169,18 @@ struct SchurPairs {
     const uint32_t* group;     // per workgroup: first pair (SP_GROUP pairs of one camera row at most)
     const uint8_t* group_n;    // per workgroup: its pair count (0: an empty slot)
 };
-// Per camera-major observation: Z = M^T J~_X^T J~_c (3 x 9, row-major, padded to ZREC), where
-// M M^T = V^-1 of the observation's point; Z_o^T Z_o' = W_o V^-1 W_o'^T.
-constexpr int ZREC = 28;
-// Off-diagonal blocks S_cj = -sum Z_o^T Z_o' over each pair's terms, stored in row c.
+// Per camera-major observation, the compact record that the explicit system's pair assembly
+// rebuilds the observation's Jacobians from: n = (x/z, y/z), sqrt(w)/z, sqrt(w), the rotated point
+// R X, a zero, then the packed V^-1 of its point and two zeros (values 0..7 are the first half
+// line, which is all a partner's record needs).
+constexpr int CREC = 16;
+constexpr int CREC_HALF = 8;
+// Off-diagonal blocks S_cj = -sum_t J~_c,o^T (J~_X,o V^-1 J~_X,o'^T) J~_c,o' (= -W_o V^-1 W_o'^T)
+// over each pair's terms, the Jacobians rebuilt from the records, the cameras' R, f, k1, k2 and
+// the gauge; stored in row c.
 template <class T>
-void launch_schur_pairs(hipStream_t s, const SchurPairs& P, const T* z, SchurRows<T> S);
+void launch_schur_pairs(hipStream_t s, const SchurPairs& P, const T* crec, const double* R, const double* gp,
+                        const uint32_t* cam_group, const uint32_t* cam_dim, const GaugeDev* gauge, SchurRows<T> S);
 // Pairs per workgroup of the pair assembly.
 constexpr uint32_t SP_GROUP = 4;
 // Workgroup slots of the pair assembly: camera row c's pairs go, in groups of SP_GROUP, to slots
@@ -197,7 +203,7 @@ void launch_camera_assembly(hipStream_t s, const Topo& tp, const Lin<T> lin, con
                             const double* gp, const GaugeDev* gauge, uint32_t nseg, const uint32_t* seg_cam,
                             const uint32_t* cseg_off, double* part, const int32_t* diag_pair,
                             const uint64_t* value_off, double* grad, double* full_grad, double* diag, double* u_vals,
-                            double* s_vals, T* z = nullptr, SchurRows<T> S = SchurRows<T>{});
+                            double* s_vals, T* crec = nullptr, SchurRows<T> S = SchurRows<T>{});
 // Per block: full gradient J~^T r~, reduced gradient (full minus W V^-1 g), unclamped diagonal.
 template <class T>
 void launch_block_gradient(hipStream_t s, const Topo& tp, const Lin<T> lin, const PointElim<T> pe,
