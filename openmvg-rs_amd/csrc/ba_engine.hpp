@@ -173,7 +173,9 @@ class Engine {
             ~Tally() { g_device_tally = nullptr; }
         } tally(&device_bytes_);
         B_ = build_block_lists(L_);
-        PL_ = build_pair_lists(L_, B_, !iterative_);
+        fast_cam_ = iterative_ && simple_topology();
+        // the camera-side assembly reads only the pairs' value offsets, not their terms
+        PL_ = build_pair_lists(L_, B_, !iterative_, !fast_cam_);
         const size_t nc = L_.n_cameras(), ng = L_.n_groups(), np = L_.n_points(), no = L_.n_obs(), D = L_.D;
         npad_ = soa_pad(no);
         obs_cam_.upload_padded(L_.obs_cam, npad_, s_);
@@ -210,7 +212,6 @@ class Engine {
         HIP_OK(hipMemcpyAsync(t_[0].p, L_.t.data(), 3 * nc * 8, hipMemcpyHostToDevice, s_));
         HIP_OK(hipMemcpyAsync(gp_[0].p, L_.gp.data(), 3 * ng * 8, hipMemcpyHostToDevice, s_));
         HIP_OK(hipMemcpyAsync(X_[0].p, L_.X.data(), 3 * np * 8, hipMemcpyHostToDevice, s_));
-        fast_cam_ = iterative_ && simple_topology();
         pose_rows_ = keep_pose_rows_ || !fast_cam_;  // the block-gradient pass reads them
         if (fast_cam_) prec_.alloc(PE_REC * np);          // read by the camera assembly
         srec_.alloc(PCG_CAM_REC * std::max<size_t>(1, nc));
@@ -361,7 +362,7 @@ class Engine {
         srev_pair_.upload(SL.rev_pair, s_);
         spair_cam_.upload(SL.pair_cam, s_);
         sterm_off_.upload(SL.term_off, s_);
-        sterm_.upload(SL.term, s_);
+        sterm_.upload(SL.term.data(), SL.term.size(), s_);
         npairs_ = (uint32_t)(SL.pair_cam.size() / 2);
         {
             std::vector<uint32_t> grp;

@@ -70,6 +70,25 @@ inline void mul_atv(const double* a, const double* v, double* o) {
     for (int i = 0; i < 3; ++i) o[i] = (a[i] * v[0] + a[3 + i] * v[1]) + a[6 + i] * v[2];
 }
 
+// Threads for n items of at least `grain` each, at most 16.
+unsigned range_count(size_t n, size_t grain) {
+    const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    return (unsigned)std::max<size_t>(1, std::min(hw, n / std::max<size_t>(grain, 1)));
+}
+// f(t, begin, end) over T contiguous ranges of [0, n), range t on thread t.  Callers combine
+// per-range results in range order, so what they build does not depend on T.
+template <class F>
+void for_ranges(size_t n, unsigned T, F&& f) {
+    if (T <= 1) {
+        f(0u, size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T);
+    for (unsigned t = 0; t < T; ++t) th.emplace_back([&f, t, n, T] { f(t, n * t / T, n * (t + 1) / T); });
+    for (auto& x : th) x.join();
+}
+
 // TrackGeometry of kernel/point_geometry.rs:29-101, the input-quarantine subset.
 struct Geometry {
     double near2, far2;
@@ -166,23 +185,43 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern) {
     // ---- input quarantine (quarantine.rs:33-96) ----
     L.n_input_points = N;
     L.quarantined.assign(N, 0);
+    // observations grouped by input point, in input order inside each point (a counting sort)
+    std::vector<uint32_t> by_off(N + 1, 0), by_pt(O);
+    for (size_t o = 0; o < O; ++o) by_off[p->obs_point[o] + 1]++;
+    for (size_t i = 0; i < N; ++i) by_off[i + 1] += by_off[i];
     {
-        std::vector<uint32_t> count(N, 0);
-        for (size_t o = 0; o < O; ++o) count[p->obs_point[o]] += 1;
-        std::vector<double> off(3 * O), dist(O);
-        for (size_t o = 0; o < O; ++o) {
+        std::vector<uint32_t> cur(by_off.begin(), by_off.end() - 1);
+        for (size_t o = 0; o < O; ++o) by_pt[cur[p->obs_point[o]]++] = (uint32_t)o;
+    }
+    {
+        auto offset = [&](size_t o, double v[3]) {
             const double* X = p->point_xyz + 3 * p->obs_point[o];
             const double* C = p->pose_center + 3 * p->obs_pose[o];
-            for (int k = 0; k < 3; ++k) off[3 * o + k] = X[k] - C[k];
-            dist[o] = std::sqrt(off[3 * o] * off[3 * o] + off[3 * o + 1] * off[3 * o + 1] + off[3 * o + 2] * off[3 * o + 2]);
-        }
+            for (int k = 0; k < 3; ++k) v[k] = X[k] - C[k];
+        };
+        std::vector<double> dist(O);
+        for_ranges(O, range_count(O, 1 << 16), [&](unsigned, size_t b, size_t e) {
+            for (size_t o = b; o < e; ++o) {
+                double v[3];
+                offset(o, v);
+                dist[o] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            }
+        });
         size_t mid = O / 2;
         std::nth_element(dist.begin(), dist.begin() + mid, dist.end());
-        double scene = dist[mid];
-        std::vector<Geometry> geo(N, Geometry(scene));
-        for (size_t o = 0; o < O; ++o) geo[p->obs_point[o]].push(&off[3 * o]);
-        for (size_t i = 0; i < N; ++i)
-            L.quarantined[i] = (count[i] < 2 || geo[i].degenerate()) && !is_fixed_point(p, i);
+        const double scene = dist[mid];
+        // each point's rays pushed in input order, as a pass over the observations would
+        for_ranges(N, range_count(N, 1 << 14), [&](unsigned, size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) {
+                Geometry geo(scene);
+                for (uint32_t k = by_off[i]; k < by_off[i + 1]; ++k) {
+                    double v[3];
+                    offset(by_pt[k], v);
+                    geo.push(v);
+                }
+                L.quarantined[i] = (by_off[i + 1] - by_off[i] < 2 || geo.degenerate()) && !is_fixed_point(p, i);
+            }
+        });
     }
     L.n_quarantined = 0;
     for (auto q : L.quarantined) L.n_quarantined += q;
@@ -378,23 +417,27 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern) {
     }
     L.gauge_camera = gauge_pose < 0 ? -1 : (int)dest[gauge_pose];
     // ---- observations sorted by (point, input index), principal point subtracted ----
-    std::vector<uint32_t> order = kept_obs;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        return new_index[p->obs_point[a]] < new_index[p->obs_point[b]];
-    });
+    // kept observations point by point (new point indices ascend with the input's), input order
+    // inside each point: the stable order by point
+    std::vector<uint32_t> order;
+    order.reserve(kept_obs.size());
+    for (size_t i = 0; i < N; ++i)
+        if (!L.quarantined[i]) order.insert(order.end(), by_pt.begin() + by_off[i], by_pt.begin() + by_off[i + 1]);
     const size_t NO = order.size();
     L.obs_cam.resize(NO);
     L.obs_point.resize(NO);
-    L.obs_source = order;
     L.uv.resize(2 * NO);
-    for (size_t k = 0; k < NO; ++k) {
-        uint32_t s = order[k];
-        const ba_intrinsic& in = p->intrinsics[p->obs_intrinsic[s]];
-        L.obs_cam[k] = dest[p->obs_pose[s]];
-        L.obs_point[k] = new_index[p->obs_point[s]];
-        L.uv[2 * k] = p->obs_pixel[2 * s] - in.cx;
-        L.uv[2 * k + 1] = p->obs_pixel[2 * s + 1] - in.cy;
-    }
+    for_ranges(NO, range_count(NO, 1 << 16), [&](unsigned, size_t b, size_t e) {
+        for (size_t k = b; k < e; ++k) {
+            uint32_t s = order[k];
+            const ba_intrinsic& in = p->intrinsics[p->obs_intrinsic[s]];
+            L.obs_cam[k] = dest[p->obs_pose[s]];
+            L.obs_point[k] = new_index[p->obs_point[s]];
+            L.uv[2 * k] = p->obs_pixel[2 * s] - in.cx;
+            L.uv[2 * k + 1] = p->obs_pixel[2 * s + 1] - in.cy;
+        }
+    });
+    L.obs_source = std::move(order);
     L.point_off.assign(NP + 1, 0);
     for (size_t k = 0; k < NO; ++k) L.point_off[L.obs_point[k] + 1] += 1;
     for (size_t q = 0; q < NP; ++q) L.point_off[q + 1] += L.point_off[q];
@@ -430,6 +473,30 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern) {
             s += L.X[3 * q] * L.X[3 * q] + L.X[3 * q + 1] * L.X[3 * q + 1] + L.X[3 * q + 2] * L.X[3 * q + 2];
         L.scene_scale = std::fmax(std::sqrt(s / (double)NP), DBL_MIN);
     }
+    // ---- number of point parameters: each free point's distinct active camera parameters ----
+    {
+        const size_t NB = P + G;
+        const unsigned T = range_count(NP, 1 << 14);
+        std::vector<uint64_t> part(T, 0);
+        for_ranges(NP, T, [&](unsigned t, size_t b, size_t e) {
+            std::vector<uint32_t> seen(NB, UINT32_MAX);  // last point that counted a block
+            uint64_t n = 0;
+            for (size_t q = b; q < e; ++q) {
+                if (L.point_fixed[q]) continue;
+                for (uint32_t o = L.point_off[q]; o < L.point_off[q + 1]; ++o) {
+                    const uint32_t c = L.obs_cam[o], g = L.cam_group[c];
+                    if (seen[c] != (uint32_t)q) n += L.cam_dim[c];
+                    seen[c] = (uint32_t)q;
+                    if (seen[P + g] != (uint32_t)q) n += L.grp_dim[g];
+                    seen[P + g] = (uint32_t)q;
+                }
+            }
+            part[t] = n;
+        });
+        L.n_cross = 0;
+        for (uint64_t n : part) L.n_cross += n;
+    }
+    if (!build_pattern) return Status{};
     // ---- per-point active parameters, first-encounter order (pattern.rs:306-379) ----
     L.cross_off.assign(NP + 1, 0);
     L.params.clear();
@@ -456,7 +523,6 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern) {
             L.cross_off[q + 1] = (uint32_t)start;
         }
     }
-    if (!build_pattern) return Status{};
     // ---- reduced block pattern (pattern.rs:381-468) ----
     const size_t NB = P + G;
     auto bdim = [&](size_t b) { return b < P ? L.cam_dim[b] : L.grp_dim[b - P]; };
@@ -533,17 +599,13 @@ BlockLists build_block_lists(const Lowered& L) {
     return B;
 }
 
-PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all) {
-    PairLists out;
-    const size_t P = L.n_cameras(), NB = B.n_blocks;
-    // Enumerate terms point by point; key each by its block pair, then bucket by key with a
-    // stable counting sort so every pair's terms stay in (point, i, j) order.
-    struct Term {
-        uint64_t key;
-        uint32_t i, j;
-    };
-    std::vector<Term> terms;
-    const size_t NP = L.n_points();
+namespace {
+
+// The terms of point q in (i, j, x, y) order: emit(A, B, i, j | flags) for block A of observation
+// i (x-th of its blocks) and block B of observation j (y-th), A >= B.
+template <class Emit>
+void point_terms(const Lowered& L, size_t q, bool all, Emit&& emit) {
+    const size_t P = L.n_cameras();
     auto blocks_of = [&](uint32_t o, uint32_t out_b[2]) {
         uint32_t c = L.obs_cam[o], g = L.cam_group[c];
         int n = 0;
@@ -551,36 +613,49 @@ PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all) {
         if (L.grp_dim[g]) out_b[n++] = (uint32_t)(P + g);
         return n;
     };
-    for (size_t q = 0; q < NP; ++q) {
-        uint32_t b0 = L.point_off[q], b1 = L.point_off[q + 1];
-        bool fixed = L.point_fixed[q];
-        for (uint32_t i = b0; i < b1; ++i) {
-            uint32_t bi[2];
-            int ni = blocks_of(i, bi);
-            for (uint32_t j = b0; j < b1; ++j) {
-                if (fixed && i != j) continue;
-                uint32_t bj[2];
-                int nj = blocks_of(j, bj);
-                for (int x = 0; x < ni; ++x)
-                    for (int y = 0; y < nj; ++y) {
-                        uint32_t A = bi[x], Bb = bj[y];
-                        if (A < Bb) continue;
-                        if (!all) {
-                            bool diag = A == Bb;
-                            bool edge = i == j && A >= P && Bb < P && L.cam_group[Bb] == A - P;
-                            if (!diag && !edge) continue;
-                            if (edge && !diag) {  // edges carry only the U Gram term
-                                terms.push_back(Term{(uint64_t)A * NB + Bb, i, j | TERM_U});
-                                continue;
-                            }
+    const uint32_t b0 = L.point_off[q], b1 = L.point_off[q + 1];
+    const bool fixed = L.point_fixed[q];
+    for (uint32_t i = b0; i < b1; ++i) {
+        uint32_t bi[2];
+        const int ni = blocks_of(i, bi);
+        for (uint32_t j = b0; j < b1; ++j) {
+            if (fixed && i != j) continue;
+            uint32_t bj[2];
+            const int nj = blocks_of(j, bj);
+            for (int x = 0; x < ni; ++x)
+                for (int y = 0; y < nj; ++y) {
+                    const uint32_t A = bi[x], Bb = bj[y];
+                    if (A < Bb) continue;
+                    if (!all) {
+                        const bool diag = A == Bb;
+                        const bool edge = i == j && A >= P && Bb < P && L.cam_group[Bb] == A - P;
+                        if (!diag && !edge) continue;
+                        if (edge && !diag) {  // edges carry only the U Gram term
+                            emit(A, Bb, i, j | TERM_U);
+                            continue;
                         }
-                        uint32_t flags = (fixed ? 0u : TERM_SCHUR) | (i == j ? TERM_U : 0u);
-                        if (flags) terms.push_back(Term{(uint64_t)A * NB + Bb, i, j | flags});
                     }
-            }
+                    const uint32_t flags = (fixed ? 0u : TERM_SCHUR) | (i == j ? TERM_U : 0u);
+                    if (flags) emit(A, Bb, i, j | flags);
+                }
         }
     }
-    // distinct keys in ascending order
+}
+
+// Every block pair of the pattern: terms keyed by A * NB + B, bucketed by a stable counting
+// sort over the distinct keys ascending.
+PairLists all_pairs(const Lowered& L, const BlockLists& B) {
+    PairLists out;
+    const size_t NB = B.n_blocks, NP = L.n_points();
+    struct Term {
+        uint64_t key;
+        uint32_t i, j;
+    };
+    std::vector<Term> terms;
+    for (size_t q = 0; q < NP; ++q)
+        point_terms(L, q, true, [&](uint32_t A, uint32_t Bb, uint32_t i, uint32_t j) {
+            terms.push_back(Term{(uint64_t)A * NB + Bb, i, j});
+        });
     std::vector<uint64_t> keys;
     keys.reserve(terms.size());
     for (auto& t : terms) keys.push_back(t.key);
@@ -612,6 +687,79 @@ PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all) {
         out.term_j[at] = terms[k].j;
     }
     return out;
+}
+
+// The diagonal blocks and pose-intrinsic edges only: one slot per diagonal block A (slot A) and
+// per camera c's edge (P + group(c), c) (slot NB + c), so no sort is needed.  Terms are counted
+// per point range on each thread, and each range's terms start after those of the ranges before
+// it, which keeps every pair's terms in point order.
+PairLists local_pairs(const Lowered& L, const BlockLists& B, bool with_terms) {
+    PairLists out;
+    const size_t P = L.n_cameras(), G = L.n_groups(), NB = B.n_blocks, NS = NB + P, NP = L.n_points();
+    auto slot = [NB](uint32_t A, uint32_t Bb) { return A == Bb ? (size_t)A : NB + Bb; };
+    const unsigned T = range_count(NP, 1 << 14);
+    std::vector<uint32_t> cnt((size_t)T * NS, 0);
+    for_ranges(NP, T, [&](unsigned t, size_t b, size_t e) {
+        uint32_t* c = cnt.data() + (size_t)t * NS;
+        for (size_t q = b; q < e; ++q)
+            point_terms(L, q, false, [&](uint32_t A, uint32_t Bb, uint32_t, uint32_t) { c[slot(A, Bb)]++; });
+    });
+    std::vector<uint64_t> total(NS, 0);
+    for (unsigned t = 0; t < T; ++t)
+        for (size_t k = 0; k < NS; ++k) total[k] += cnt[(size_t)t * NS + k];
+    // cameras of each group, ascending
+    std::vector<uint32_t> goff(G + 1, 0), gcam(P);
+    for (size_t c = 0; c < P; ++c) goff[L.cam_group[c] + 1]++;
+    for (size_t g = 0; g < G; ++g) goff[g + 1] += goff[g];
+    {
+        std::vector<uint32_t> cur(goff.begin(), goff.end() - 1);
+        for (size_t c = 0; c < P; ++c) gcam[cur[L.cam_group[c]]++] = (uint32_t)c;
+    }
+    // pairs ascending in (A, B): block A's edges (B = its group's cameras, all < P <= A), then (A, A)
+    std::vector<uint32_t> pair_of(NS, UINT32_MAX);
+    out.term_off.assign(1, 0);
+    out.value_off.assign(1, 0);
+    auto add = [&](uint32_t A, uint32_t Bb) {
+        const size_t s = slot(A, Bb);
+        if (!total[s]) return;
+        pair_of[s] = (uint32_t)out.block_a.size();
+        out.block_a.push_back(A);
+        out.block_b.push_back(Bb);
+        out.term_off.push_back(out.term_off.back() + (uint32_t)total[s]);
+        out.value_off.push_back(out.value_off.back() + (uint64_t)B.block_dim[A] * B.block_dim[Bb]);
+    };
+    for (size_t A = 0; A < NB; ++A) {
+        if (A >= P)
+            for (uint32_t k = goff[A - P]; k < goff[A - P + 1]; ++k) add((uint32_t)A, gcam[k]);
+        add((uint32_t)A, (uint32_t)A);
+    }
+    if (!with_terms) return out;
+    for (size_t k = 0; k < NS; ++k) {  // per range: where its terms of slot k start
+        uint32_t at = pair_of[k] == UINT32_MAX ? 0u : out.term_off[pair_of[k]];
+        for (unsigned t = 0; t < T; ++t) {
+            const uint32_t n = cnt[(size_t)t * NS + k];
+            cnt[(size_t)t * NS + k] = at;
+            at += n;
+        }
+    }
+    out.term_i.resize(out.term_off.back());
+    out.term_j.resize(out.term_off.back());
+    for_ranges(NP, T, [&](unsigned t, size_t b, size_t e) {
+        uint32_t* cur = cnt.data() + (size_t)t * NS;
+        for (size_t q = b; q < e; ++q)
+            point_terms(L, q, false, [&](uint32_t A, uint32_t Bb, uint32_t i, uint32_t j) {
+                const uint32_t at = cur[slot(A, Bb)]++;
+                out.term_i[at] = i;
+                out.term_j[at] = j;
+            });
+    });
+    return out;
+}
+
+}  // namespace
+
+PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all, bool with_terms) {
+    return all ? all_pairs(L, B) : local_pairs(L, B, with_terms);
 }
 
 void build_pair_groups(const std::vector<uint32_t>& pair_cam, uint32_t nc, std::vector<uint32_t>& group,
@@ -653,72 +801,68 @@ uint64_t count_schur_terms(const Lowered& L) {
 SchurLists build_schur_lists(const Lowered& L, const std::vector<uint32_t>& pos) {
     const size_t nc = L.n_cameras(), np = L.n_points();
     SchurLists S;
-    // 1. observation pairs bucketed by their lower camera, in point order
+    // 1. observation pairs bucketed by their lower camera, in point order: counted per point
+    //    range, each range's entries placed after those of the ranges before it
+    const unsigned T = range_count(np, 1 << 13);
+    std::vector<uint64_t> cnt((size_t)T * nc, 0);
+    auto each_pair = [&](size_t b, size_t e, auto&& f) {
+        for (size_t p = b; p < e; ++p) {
+            if (L.point_fixed[p]) continue;
+            for (uint32_t a = L.point_off[p]; a < L.point_off[p + 1]; ++a)
+                for (uint32_t c = a + 1; c < L.point_off[p + 1]; ++c) f(a, c);
+        }
+    };
+    for_ranges(np, T, [&](unsigned t, size_t b, size_t e) {
+        uint64_t* n = cnt.data() + (size_t)t * nc;
+        each_pair(b, e, [&](uint32_t a, uint32_t c) { n[std::min(L.obs_cam[a], L.obs_cam[c])]++; });
+    });
     std::vector<uint64_t> boff(nc + 1, 0);
-    for (size_t p = 0; p < np; ++p) {
-        if (L.point_fixed[p]) continue;
-        for (uint32_t a = L.point_off[p]; a < L.point_off[p + 1]; ++a)
-            for (uint32_t b = a + 1; b < L.point_off[p + 1]; ++b)
-                boff[std::min(L.obs_cam[a], L.obs_cam[b]) + 1]++;
+    for (size_t c = 0; c < nc; ++c) {
+        uint64_t at = boff[c];
+        for (unsigned t = 0; t < T; ++t) {
+            const uint64_t n = cnt[(size_t)t * nc + c];
+            cnt[(size_t)t * nc + c] = at;
+            at += n;
+        }
+        boff[c + 1] = at;
     }
-    for (size_t c = 0; c < nc; ++c) boff[c + 1] += boff[c];
     struct Entry {
         uint32_t j, pc, pj;
     };
-    std::vector<Entry> ent(boff[nc]);
-    {
-        std::vector<uint64_t> cur(boff.begin(), boff.end() - 1);
-        for (size_t p = 0; p < np; ++p) {
-            if (L.point_fixed[p]) continue;
-            for (uint32_t a = L.point_off[p]; a < L.point_off[p + 1]; ++a)
-                for (uint32_t b = a + 1; b < L.point_off[p + 1]; ++b) {
-                    const uint32_t ca = L.obs_cam[a], cb = L.obs_cam[b];
-                    if (ca < cb) ent[cur[ca]++] = Entry{cb, pos[a], pos[b]};
-                    else ent[cur[cb]++] = Entry{ca, pos[b], pos[a]};
-                }
-        }
-    }
-    // 2. per camera: its partner cameras ascending, the entries grouped by partner (stable)
+    std::vector<Entry, NoInitAlloc<Entry>> ent(boff[nc]);
+    for_ranges(np, T, [&](unsigned t, size_t b, size_t e) {
+        uint64_t* cur = cnt.data() + (size_t)t * nc;
+        each_pair(b, e, [&](uint32_t a, uint32_t c) {
+            const uint32_t ca = L.obs_cam[a], cb = L.obs_cam[c];
+            if (ca < cb) ent[cur[ca]++] = Entry{cb, pos[a], pos[c]};
+            else ent[cur[cb]++] = Entry{ca, pos[c], pos[a]};
+        });
+    });
+    // 2. per camera: its partner cameras ascending (all above it), the entries grouped by
+    //    partner in their order
     std::vector<std::vector<uint32_t>> js(nc), jn(nc);
-    auto group = [&](size_t c0, size_t c1) {
+    for_ranges(nc, range_count(nc, 16), [&](unsigned, size_t c0, size_t c1) {
+        std::vector<uint64_t> count(nc, 0);
         std::vector<Entry> tmp;
-        std::vector<uint32_t> all;
         for (size_t c = c0; c < c1; ++c) {
             const uint64_t b = boff[c], e = boff[c + 1];
             if (b == e) continue;
-            all.clear();
-            for (uint64_t i = b; i < e; ++i) all.push_back(ent[i].j);
-            std::sort(all.begin(), all.end());
-            all.erase(std::unique(all.begin(), all.end()), all.end());
-            std::vector<uint64_t> at(all.size() + 1, 0);
-            for (uint64_t i = b; i < e; ++i) {
-                const size_t k = std::lower_bound(all.begin(), all.end(), ent[i].j) - all.begin();
-                at[k + 1]++;
-            }
-            jn[c].resize(all.size());
-            for (size_t k = 0; k < all.size(); ++k) {
-                jn[c][k] = (uint32_t)at[k + 1];
-                at[k + 1] += at[k];
+            for (uint64_t i = b; i < e; ++i) count[ent[i].j]++;
+            uint64_t at = 0;
+            for (size_t j = c + 1; j < nc; ++j) {
+                if (!count[j]) continue;
+                js[c].push_back((uint32_t)j);
+                jn[c].push_back((uint32_t)count[j]);
+                const uint64_t n = count[j];
+                count[j] = at;  // now the partner's first slot
+                at += n;
             }
             tmp.resize(e - b);
-            for (uint64_t i = b; i < e; ++i) {
-                const size_t k = std::lower_bound(all.begin(), all.end(), ent[i].j) - all.begin();
-                tmp[at[k]++] = ent[i];
-            }
+            for (uint64_t i = b; i < e; ++i) tmp[count[ent[i].j]++] = ent[i];
             std::copy(tmp.begin(), tmp.end(), ent.begin() + b);
-            js[c] = all;
+            for (uint32_t j : js[c]) count[j] = 0;
         }
-    };
-    {
-        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        std::vector<std::thread> th;
-        const size_t per = (nc + hw - 1) / hw;
-        for (unsigned t = 0; t < hw; ++t) {
-            const size_t c0 = std::min(nc, t * per), c1 = std::min(nc, c0 + per);
-            if (c0 < c1) th.emplace_back(group, c0, c1);
-        }
-        for (auto& x : th) x.join();
-    }
+    });
     // 3. upper rows: the camera, then its partners j > c; pairs numbered in (c, j) order
     S.nbr_off.assign(nc + 1, 0);
     S.pair_off.assign(nc + 1, 0);
@@ -738,7 +882,8 @@ SchurLists build_schur_lists(const Lowered& L, const std::vector<uint32_t>& pos)
         *out++ = (uint32_t)c;
         for (uint32_t j : js[c]) *out++ = j;
     }
-    // 4. pairs and their terms; per camera j the pairs (c < j, j) in ascending c
+    // 4. pairs and their terms (the entries in their grouped order); per camera j the pairs
+    //    (c < j, j) in ascending c
     const uint32_t npairs = S.pair_off[nc];
     S.rev_off.assign(nc + 1, 0);
     for (size_t c = 0; c < nc; ++c)
@@ -746,23 +891,24 @@ SchurLists build_schur_lists(const Lowered& L, const std::vector<uint32_t>& pos)
     for (size_t c = 0; c < nc; ++c) S.rev_off[c + 1] += S.rev_off[c];
     S.rev_pair.resize(npairs);
     std::vector<uint32_t> rcur(S.rev_off.begin(), S.rev_off.end() - 1);
-    S.term_off.push_back(0);
-    S.term.reserve(2 * ent.size());
+    S.term_off.assign(1, 0);
+    S.term_off.reserve((size_t)npairs + 1);
     S.pair_cam.reserve(2 * (size_t)npairs);
     uint32_t k = 0;
-    for (size_t c = 0; c < nc; ++c) {
-        uint64_t i = boff[c];
+    for (size_t c = 0; c < nc; ++c)
         for (size_t m = 0; m < js[c].size(); ++m, ++k) {
             S.pair_cam.push_back((uint32_t)c);
             S.pair_cam.push_back(js[c][m]);
             S.rev_pair[rcur[js[c][m]]++] = k;
-            for (uint32_t t = 0; t < jn[c][m]; ++t, ++i) {
-                S.term.push_back(ent[i].pc);
-                S.term.push_back(ent[i].pj);
-            }
-            S.term_off.push_back((uint32_t)(S.term.size() / 2));
+            S.term_off.push_back(S.term_off.back() + jn[c][m]);
         }
-    }
+    S.term.resize(2 * ent.size());
+    for_ranges(ent.size(), range_count(ent.size(), 1 << 16), [&](unsigned, size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) {
+            S.term[2 * i] = ent[i].pc;
+            S.term[2 * i + 1] = ent[i].pj;
+        }
+    });
     return S;
 }
 
@@ -875,7 +1021,7 @@ void plan_execution(const Lowered& L, const ba_solve_config* c, ba_execution_pla
         return;
     }
     const uint64_t P = L.n_cameras(), G = L.n_groups(), NP = L.n_points(), D = L.D;
-    const uint64_t cross = L.params.size();
+    const uint64_t cross = L.n_cross;
     const uint64_t limit = c->workspace_budget_bytes ? c->workspace_budget_bytes : AUTO_BUDGET_BYTES;
     out.budget_bytes = limit;
     // estimated_point_cache_bytes (execution_policy.rs:68-96): usize offsets, no retained point

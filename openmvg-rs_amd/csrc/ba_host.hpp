@@ -5,7 +5,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/ba_hip.h"
@@ -43,7 +45,8 @@ struct Lowered {
     // layout
     size_t D = 0;
     std::vector<uint32_t> cam_off, cam_dim, grp_off, grp_dim;
-    // reference index maps
+    uint64_t n_cross = 0;  // point parameters (the size of params), always counted
+    // reference index maps (built only for build_pattern)
     std::vector<uint32_t> cross_off, params, col_ptr, row_idx;
     size_t n_cameras() const { return cam_group.size(); }
     size_t n_groups() const { return gp.size() / 3; }
@@ -81,8 +84,28 @@ struct PairLists {
     std::vector<uint64_t> value_off;            // n_pairs+1: offset of the pair's dA*dB values
 };
 // all = every block pair of the pattern (direct solve); otherwise the diagonal blocks and the
-// pose-intrinsic edges of each camera (the retained operator and its preconditioner).
-PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all);
+// pose-intrinsic edges of each camera (the retained operator and its preconditioner).  Without
+// with_terms (local pairs only) term_i and term_j stay empty; term_off still counts the terms.
+PairLists build_pair_lists(const Lowered& L, const BlockLists& B, bool all, bool with_terms = true);
+
+// An allocator that leaves value-initialised elements uninitialised: a large host array is then
+// first written, page by page, by the threads that fill it rather than zeroed by one thread.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    template <class U>
+    void construct(U*) noexcept {}
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new ((void*)p) U(std::forward<A>(a)...);
+    }
+};
 
 // Lists of the explicit reduced camera system, for topologies with one camera per intrinsic
 // group and at most one observation per point and camera.  S is symmetric and only its diagonal
@@ -96,7 +119,8 @@ struct SchurLists {
     std::vector<uint64_t> row_base;            // nc: first value of row c
     std::vector<uint32_t> pair_off, pair_cam;  // nc + 1 (pairs of row c), 2 per pair: (c, j)
     std::vector<uint32_t> rev_off, rev_pair;   // nc + 1, pair numbers
-    std::vector<uint32_t> term_off, term;      // n_pairs + 1, 2 per term
+    std::vector<uint32_t> term_off;            // n_pairs + 1
+    std::vector<uint32_t, NoInitAlloc<uint32_t>> term;  // 2 per term
     uint64_t n_values = 0;
 };
 // Number of observation pairs (sum of k(k-1)/2 over free points with k observations).

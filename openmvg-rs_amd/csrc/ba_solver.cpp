@@ -425,6 +425,8 @@ int ba_hip_prepare(const ba_problem* p, ba_native_sizes* sizes, ba_native_proble
     Lowered L;
     st = lower_problem(p, L, true);
     if (!st.ok()) return finish(st, msg, len);
+    if (L.n_cross != L.params.size())  // the count the planner reads, against the map itself
+        return finish(error(BA_ERR_INTERNAL, "point parameter count differs from the point parameter map"), msg, len);
     if (sizes) {
         sizes->n_cameras = L.n_cameras();
         sizes->n_groups = L.n_groups();

@@ -108,6 +108,18 @@ def _tiny_problem(obs, n_poses=3, n_points=2, fixed_pose=None):
     return BundleProblem(R, C_, intr, X, pose, pose, point, pix, pose_var_rotation=vr, pose_var_translation=vr.copy())
 
 
+def test_quarantine_and_point_major_order(lib, oracle):
+    """Point 1 is seen once and point 2 twice by one pose along one ray (no parallax): both are
+    held out.  The kept observations, given in a shuffled order, come out point by point with the
+    input order kept inside each point."""
+    obs = [(2, 3), (0, 0), (2, 2), (1, 0), (1, 1), (0, 3), (2, 0), (2, 2)]
+    nat = _same_lowering(lib, oracle, _tiny_problem(obs, n_points=4))
+    assert nat.sizes["n_quarantined"] == 2
+    assert list(nat.arrays["point_source"]) == [0, 3]
+    assert list(nat.arrays["obs_source"]) == [1, 3, 6, 0, 5]
+    assert list(nat.arrays["point_offsets"]) == [0, 3, 5]
+
+
 def test_repeated_track_rows_keep_first_encounter_order(lib, oracle):
     """The reference's incidence example (workspace/topology.rs:611-629) with a second view added to
     point 1 so that it is not quarantined: point 0 seen by poses [2, 0, 2, 1], point 1 by [1, 1, 0];
