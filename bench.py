@@ -219,7 +219,8 @@ def cpu_baseline(prob, steps: int, converge_iterations: int) -> dict:
     oracle = load_oracle_lib()
     cpu = host_cpu()
     threads = omp_threads(cpu["nproc"]) if "OMP_NUM_THREADS" not in os.environ else omp_threads()
-    cfg = SolveConfig(max_iterations=converge_iterations, linear_solver=A.SOLVER_ITERATIVE)
+    cfg = SolveConfig(max_iterations=converge_iterations, linear_solver=A.SOLVER_ITERATIVE,
+                      certification=A.CERTIFY_LOOP_ONLY)
     t0 = time.perf_counter()
     sol = solve_with(oracle, "bao_solve", prob, cfg)
     wall = time.perf_counter() - t0
@@ -228,7 +229,8 @@ def cpu_baseline(prob, steps: int, converge_iterations: int) -> dict:
     c_star = float(sol.report.final_cost)
     ttc, ttc_it = first_hit(tr, c_star * (1.0 + 1e-6))
     omp_threads(1)
-    one = solve_with(oracle, "bao_solve", prob, SolveConfig(max_iterations=1, linear_solver=A.SOLVER_ITERATIVE))
+    one = solve_with(oracle, "bao_solve", prob,
+                     SolveConfig(max_iterations=1, linear_solver=A.SOLVER_ITERATIVE, certification=A.CERTIFY_LOOP_ONLY))
     omp_threads(threads)
     one_s = float(one.trace["seconds"][0])
     return {
@@ -315,7 +317,7 @@ def bench_medium(ba, cpu: bool = False) -> dict:
 
     scene = synthetic.medium_scene()
     prob = scene.problem
-    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT)
+    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT, certification=A.CERTIFY_LOOP_ONLY)
     out = {"workload": f"{prob.n_poses} cameras, {prob.n_points} points, {prob.n_observations} observations, "
                        "free f/k1/k2, squared loss, direct Schur + dense Cholesky, seed 0xBA0002"}
     ba.solve(prob, cfg.with_(max_iterations=1))
@@ -348,7 +350,7 @@ def bench_batched(ba, sample_windows: int = 64, cpu: bool = False) -> dict:
     from openmvg_ba import SolveConfig, _abi as A, synthetic
 
     probs = [w.problem for w in synthetic.window_scenes()]
-    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT)
+    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT, certification=A.CERTIFY_LOOP_ONLY)
     out = {"workload": f"{len(probs)} windows, {sum(p.n_observations for p in probs)} observations, "
                        "10 LM iterations each"}
     for name, prec in (("fp64", A.PRECISION_FP64), ("mixed", A.PRECISION_MIXED)):
@@ -486,7 +488,8 @@ def main() -> int:
     if args.only_form:
         fn = {"medium_direct": lambda: bench_medium(adjuster(), cpu=not args.no_cpu),
               "mixed_large": lambda: bench_mixed(adjuster(), synthetic.large_scene(n_points=args.points).problem,
-                                                 SolveConfig(linear_solver=A.SOLVER_ITERATIVE), args.steps, None),
+                                                 SolveConfig(linear_solver=A.SOLVER_ITERATIVE,
+                                                             certification=A.CERTIFY_LOOP_ONLY), args.steps, None),
               "batched": lambda: bench_batched(adjuster(), cpu=not args.no_cpu),
               "differentiation": lambda: bench_differentiation(20, cpu=not args.no_cpu)}[args.only_form]
         print(json.dumps({args.only_form: fn()}))
@@ -508,7 +511,7 @@ def main() -> int:
     log(f"scene: {prob.n_poses} cameras, {prob.n_points} points, {prob.n_observations} observations "
         f"({time.perf_counter() - t0:.1f} s)")
     ba = adjuster()
-    base = SolveConfig(linear_solver=A.SOLVER_ITERATIVE)
+    base = SolveConfig(linear_solver=A.SOLVER_ITERATIVE, certification=A.CERTIFY_LOOP_ONLY)
 
     # warmup: a separate short solve (module load, first-touch allocations)
     if args.warmup > 0:

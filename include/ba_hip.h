@@ -122,7 +122,32 @@ typedef struct ba_solve_config {
     uint32_t pcg_max_iterations;          /* 0 = the native cap min(600, D) */
     double pcg_relative_tolerance;        /* 0 = the native forcing tolerance 1e-1 */
     int32_t profile_kernel;               /* enum ba_profile_kernel: time this kernel's launches */
+    int32_t certification;                /* enum ba_certification; 0 (the reference's) by default */
 } ba_solve_config;
+
+/* What follows the LM loop.  FULL is the reference's native solve: the final rank certificate
+ * (sparse/self_calibration/mod.rs:498-560, direct.rs:288-334), the intrinsic marginal covariances
+ * with the re-solve that holds unconstrained groups at their input values
+ * (native/self_calibration.rs:223-303), the weak-point re-solves (native/mod.rs:260-350) and the
+ * round-off canonicalisation (solver.rs:1030-1101).  LOOP_ONLY returns the LM loop's result with
+ * the certificate not assessed (no counterpart in the reference). */
+enum ba_certification { BA_CERTIFY_FULL = 0, BA_CERTIFY_LOOP_ONLY = 1 };
+
+/* Observability (solver.rs:187-208) of the final undamped reduced camera system. */
+enum ba_observability {
+    BA_OBSERVABILITY_FULL_RANK = 0,
+    BA_OBSERVABILITY_ONLY_EXPECTED_GAUGE = 1,
+    BA_OBSERVABILITY_RANK_DEFICIENT = 2,
+    BA_OBSERVABILITY_NOT_ASSESSED = 3
+};
+
+/* Limits of the certificate this library forms (a dense factor of the reduced system on the
+ * device, assembled from the block pairs of every point's observations): at most this reduced
+ * dimension and this many block-pair terms (the sum over free points of (2 x observations)^2).
+ * Beyond them the solve reports BA_OBSERVABILITY_NOT_ASSESSED, forms no marginals and holds no
+ * group, where the reference would factor its sparse system. */
+#define BA_CERTIFY_MAX_DIMENSION 8192
+#define BA_CERTIFY_MAX_TERMS 67108864
 
 /* Kernels that ba_solve_config.profile_kernel can time with HIP events on the solve's stream. */
 enum ba_profile_kernel {
@@ -145,7 +170,8 @@ enum ba_solve_status {
     BA_STATUS_CONVERGED_COST = 2,
     BA_STATUS_MAX_ITERATIONS = 3,
     BA_STATUS_STALLED = 4,
-    BA_STATUS_DEGENERATE = 5
+    BA_STATUS_DEGENERATE = 5,
+    BA_STATUS_SKIPPED = 6             /* the input state returned (round-off non-descent) */
 };
 
 /* SolveReport (solver.rs:461-473) and the kernel report fields it is built from
@@ -190,6 +216,12 @@ typedef struct ba_solve_report {
      * norms within the robust-loss scale (every retained observation under squared loss) */
     uint64_t inlier_count;
     double final_inlier_rmse;             /* NaN when no observation is within the scale */
+    /* SolveDiagnostics (solver.rs:297-308, native/reporting.rs:24-100) */
+    int32_t observability;                /* enum ba_observability */
+    uint32_t pose_rank, pose_dimension;   /* ObservabilityDiagnostics; 0, 0 when not assessed */
+    uint32_t unconstrained_intrinsics;    /* groups held at their input values (SolveReport) */
+    uint32_t weak_point_resolves;         /* weak-point re-solves run (at most 4) */
+    uint32_t _pad_certify;
 } ba_solve_report;
 
 /* SparseExecutionReason (kernel/multi_view/sparse/api.rs) of the camera-step choice. */
@@ -250,6 +282,11 @@ typedef struct ba_solution {
     double* point_xyz;                    /* n_points*3 */
     uint8_t* point_quarantined;           /* n_points, or NULL */
     uint8_t* intrinsic_unconstrained;     /* n_intrinsics, or NULL */
+    /* n_intrinsics*9, or NULL: each free group's marginal covariance in its local coordinates
+     * (log f, k1, k2; row-major 3x3, zero beyond its free dimension), weight-normalised, at the
+     * first solve's final state (sparse/self_calibration/mod.rs:110-209); NaN where none was
+     * formed.  No counterpart in the reference's Solution, which keeps only the verdict. */
+    double* intrinsic_covariance;
 } ba_solution;
 
 typedef struct ba_hip_handle ba_hip_handle;

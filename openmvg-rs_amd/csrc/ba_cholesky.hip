@@ -50,7 +50,8 @@ struct Mfma<float> {
 };
 
 template <class T>
-__global__ void __launch_bounds__(1024) k_diag_tol(const T* A, uint32_t n, uint32_t ld, double* work, int* fail) {
+__global__ void __launch_bounds__(1024) k_diag_tol(const T* A, uint32_t n, uint32_t ld, double rel, double* work,
+                                                   int* fail) {
     __shared__ double sh[1024];
     double m = 0.0;
     for (uint32_t i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs((double)A[(size_t)i * ld + i]));
@@ -61,7 +62,7 @@ __global__ void __launch_bounds__(1024) k_diag_tol(const T* A, uint32_t n, uint3
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        work[0] = 64.0 * Mfma<T>::eps * fmax(sh[0], DBL_MIN);
+        work[0] = rel * fmax(sh[0], DBL_MIN);
         *fail = 0;
     }
 }
@@ -456,9 +457,11 @@ inline double* work_w(double* work, uint32_t n) { return work + 16 + 5 * (size_t
 
 // Factor A (lower triangle, in place) tile step by tile step; with `rhs` the forward solve
 // L y = rhs advances with it (rhs is consumed).
+// Pivots must exceed rel * max|diag(A)| (and be finite).
 template <class T>
-void factor(hipStream_t s, T* A, uint32_t n, uint32_t ld, double* work, int* fail, double* rhs, double* y) {
-    hipLaunchKernelGGL(k_diag_tol<T>, dim3(1), dim3(1024), 0, s, A, n, ld, work, fail);
+void factor(hipStream_t s, T* A, uint32_t n, uint32_t ld, double* work, int* fail, double* rhs, double* y,
+            double rel = 64.0 * Mfma<T>::eps) {
+    hipLaunchKernelGGL(k_diag_tol<T>, dim3(1), dim3(1024), 0, s, A, n, ld, rel, work, fail);
     T* Linv = reinterpret_cast<T*>(work + linv_offset(n));
     const uint32_t ntile = (n + TB - 1) / TB;
     for (uint32_t kt = 0; kt < ntile; ++kt) {
@@ -494,6 +497,116 @@ void apply(hipStream_t s, const T* L, uint32_t n, uint32_t ld, const double* b, 
     for (uint32_t kt = 0; kt < ntile; ++kt)
         hipLaunchKernelGGL(k_fwd_step<T>, dim3(ntile - kt), dim3(256), 0, s, L, Linv, n, ld, kt, w, y, skip);
     backward<T>(s, L, n, ld, y, x, work, skip);
+}
+
+// Inverse columns of the trailing factor, for the marginals: R starts as the unit columns e_j,
+// j0 <= j < n, restricted to the rows [k0, n) (k0 = the tile row of j0; row-major, m = n - j0
+// columns, leading dimension ldr).  Step kt (k0 / 64 <= kt < ntile): every workgroup forms
+// Y_k = X_k R_k (X_k = L_kk^-1) for its column block; workgroup 0 stores it in Y, workgroup b > 0
+// updates the rows of tile kt + b by R_i -= L_ik Y_k (R_k itself is only read in this step).
+// After the last step Y = L^-1 E on rows [k0, n) (the rows above are zero).  Grid (1 + the tiles
+// below kt, column blocks of 32); every sum runs in a fixed order.
+constexpr int INV_COLS = 32;
+__global__ void __launch_bounds__(256) k_inverse_step(const double* L, const double* Linv, uint32_t n, uint32_t ld,
+                                                      uint32_t kt, uint32_t k0, double* R, double* Y, uint32_t m,
+                                                      uint32_t ldr, const int* fail) {
+    __shared__ double xs[TB][TB + 1];
+    __shared__ double yb[TB][INV_COLS + 1];
+    if (*fail) return;
+    const uint32_t t0 = kt * TB;                     // first row of tile kt
+    const uint32_t nb = n - t0 < TB ? n - t0 : TB;
+    const uint32_t c0 = blockIdx.y * INV_COLS;       // first column of this block
+    const int tid = threadIdx.x;
+    load_tile<double>(xs, Linv + (size_t)kt * TT, TB, TB, TB);
+    for (int e = tid; e < TB * INV_COLS; e += 256) {
+        const int r = e / INV_COLS, c = e % INV_COLS;
+        yb[r][c] = ((uint32_t)r < nb && c0 + c < m) ? R[(size_t)(t0 - k0 + r) * ldr + c0 + c] : 0.0;
+    }
+    __syncthreads();
+    // Y_k = X_k R_k: thread (row, 8 columns), sums over q <= row ascending
+    const int row = tid / 4, cb = (tid % 4) * 8;
+    double acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = 0.0;
+    for (int q = 0; q <= row; ++q) {
+        const double x = xs[row][q];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] += x * yb[q][cb + c];
+    }
+    __syncthreads();  // every thread has read R_k
+#pragma unroll
+    for (int c = 0; c < 8; ++c) yb[row][cb + c] = acc[c];
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        if ((uint32_t)row < nb)
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (c0 + cb + c < m) Y[(size_t)(t0 - k0 + row) * ldr + c0 + cb + c] = yb[row][cb + c];
+        return;
+    }
+    const uint32_t r0 = t0 + TB * blockIdx.x;         // first row of tile kt + b
+    load_tile<double>(xs, L + (size_t)r0 * ld + t0, ld, n - r0, nb);
+    __syncthreads();
+    if (r0 + row >= n) return;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = 0.0;
+    for (int q = 0; q < TB; ++q) {
+        const double l = xs[row][q];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] += l * yb[q][cb + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        if (c0 + cb + c < m) R[(size_t)(r0 - k0 + row) * ldr + c0 + cb + c] -= acc[c];
+}
+
+// R = the unit columns e_{j0 + c} on the rows [k0, n).
+__global__ void __launch_bounds__(256) k_unit_columns(double* R, uint32_t rows, uint32_t m, uint32_t ldr,
+                                                      uint32_t diag0) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)rows * ldr) return;
+    const uint32_t r = (uint32_t)(i / ldr), c = (uint32_t)(i % ldr);
+    R[i] = (c < m && r == diag0 + c) ? 1.0 : 0.0;
+}
+
+// Per group g (one workgroup each): cov_g[a][b] = w * sum_r Y[r][col(g, a)] Y[r][col(g, b)]
+// (= w (S^-1)_gg, since S^-1 = L^-T L^-1 and the group's columns are among the trailing ones; R
+// here is the Y of k_inverse_step),
+// the rows summed in a fixed partition and order.  col(g, a) = off[g] - j0 + a.
+__global__ void __launch_bounds__(256) k_group_marginals(const double* R, uint32_t rows, uint32_t ldr,
+                                                         const uint32_t* goff, const uint32_t* gdim, uint32_t j0,
+                                                         double w, double* cov) {
+    __shared__ double part[256][6];
+    const uint32_t g = blockIdx.x, d = gdim[g];
+    const int tid = threadIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (d) {
+        const uint32_t c = goff[g] - j0;
+        for (uint32_t r = tid; r < rows; r += 256) {
+            double y[3];
+            for (uint32_t a = 0; a < 3; ++a) y[a] = a < d ? R[(size_t)r * ldr + c + a] : 0.0;
+            acc[0] += y[0] * y[0];
+            acc[1] += y[1] * y[0];
+            acc[2] += y[1] * y[1];
+            acc[3] += y[2] * y[0];
+            acc[4] += y[2] * y[1];
+            acc[5] += y[2] * y[2];
+        }
+    }
+    for (int k = 0; k < 6; ++k) part[tid][k] = acc[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s)
+            for (int k = 0; k < 6; ++k) part[tid][k] += part[tid + s][k];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double v[6] = {part[0][0], part[0][1], part[0][2], part[0][3], part[0][4], part[0][5]};
+        const int idx[3][3] = {{0, 1, 3}, {1, 2, 4}, {3, 4, 5}};
+        for (uint32_t a = 0; a < 3; ++a)
+            for (uint32_t b = 0; b < 3; ++b)
+                cov[9 * (size_t)g + 3 * a + b] = (a < d && b < d) ? w * v[idx[a][b]] : 0.0;
+    }
 }
 
 }  // namespace
@@ -543,6 +656,34 @@ void launch_cholesky_solve_mixed(hipStream_t s, const double* S, float* F, uint3
                            (const int*)done);
         hipLaunchKernelGGL(k_refine_axpy, dim3(nb), dim3(256), 0, s, n, (const double*)dx, x, (const int*)done);
     }
+}
+
+void launch_cholesky_factor(hipStream_t s, double* A, uint32_t n, uint32_t ld, double* work, int* fail,
+                            double rel_floor) {
+    if (n == 0) return;
+    factor<double>(s, A, n, ld, work, fail, nullptr, nullptr, rel_floor);
+}
+
+size_t trailing_inverse_doubles(uint32_t n, uint32_t j0) {
+    const uint32_t k0 = (j0 / TB) * TB, m = n - j0;
+    return 2 * (size_t)(n - k0) * (m ? m : 1);
+}
+
+void launch_group_marginals(hipStream_t s, const double* L, uint32_t n, uint32_t ld, const double* work,
+                            const int* fail, uint32_t j0, uint32_t ngroups, const uint32_t* goff,
+                            const uint32_t* gdim, double w, double* R, double* cov) {
+    if (j0 >= n || ngroups == 0) return;
+    const uint32_t k0 = (j0 / TB) * TB, m = n - j0, rows = n - k0, ldr = m;
+    const double* Linv = work + linv_offset(n);
+    double* Y = R + (size_t)rows * ldr;
+    hipLaunchKernelGGL(k_unit_columns, dim3((unsigned)(((size_t)rows * ldr + 255) / 256)), dim3(256), 0, s, R, rows, m,
+                       ldr, j0 - k0);
+    const uint32_t ntile = (n + TB - 1) / TB;
+    for (uint32_t kt = k0 / TB; kt < ntile; ++kt)
+        hipLaunchKernelGGL(k_inverse_step, dim3(ntile - kt, (m + INV_COLS - 1) / INV_COLS), dim3(256), 0, s, L, Linv,
+                           n, ld, kt, k0, R, Y, m, ldr, fail);
+    hipLaunchKernelGGL(k_group_marginals, dim3(ngroups), dim3(256), 0, s, (const double*)Y, rows, ldr, goff, gdim, j0,
+                       w, cov);
 }
 
 }  // namespace ba

@@ -774,6 +774,7 @@ class Engine {
     const double* obs_cost() const { return ocost_.p; }
     const uint8_t* valid() const { return valid_.p; }
     const double* S() const { return S_.p; }
+    double* S_mut() { return S_.p; }
     const double* grad() const { return grad_.p; }
     const double* full_grad() const { return fgrad_.p; }
     const double* diag() const { return diag_.p; }

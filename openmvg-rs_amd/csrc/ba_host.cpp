@@ -123,6 +123,35 @@ struct Geometry {
 
 }  // namespace
 
+std::vector<uint8_t> degenerate_points(const ba_problem* p, const double* centers, const double* xyz) {
+    const size_t N = p->n_points, O = p->n_observations;
+    std::vector<double> dist(O);
+    auto offset = [&](size_t o, double v[3]) {
+        const double* X = xyz + 3 * p->obs_point[o];
+        const double* C = centers + 3 * p->obs_pose[o];
+        for (int k = 0; k < 3; ++k) v[k] = X[k] - C[k];
+    };
+    for (size_t o = 0; o < O; ++o) {
+        double v[3];
+        offset(o, v);
+        dist[o] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    double scene = 0.0;
+    if (O) {
+        std::nth_element(dist.begin(), dist.begin() + O / 2, dist.end());
+        scene = dist[O / 2];
+    }
+    std::vector<Geometry> geo(N, Geometry(scene));
+    for (size_t o = 0; o < O; ++o) {
+        double v[3];
+        offset(o, v);
+        geo[p->obs_point[o]].push(v);
+    }
+    std::vector<uint8_t> out(N, 0);
+    for (size_t i = 0; i < N; ++i) out[i] = geo[i].degenerate();
+    return out;
+}
+
 Status validate_config(const ba_solve_config* c) {
     if (c->max_iterations == 0) return error(BA_ERR_INVALID_CONFIG, "max_iterations must be positive");
     if (!std::isfinite(c->function_tolerance) || c->function_tolerance <= 0.0)
@@ -180,7 +209,60 @@ Status validate_problem(const ba_problem* p) {
     return Status{};
 }
 
-Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern) {
+// Camera and intrinsic blocks of the reduced system (topology.rs:351-397) and the number of
+// point parameters (each free point's distinct active camera parameters), from the cameras'
+// and groups' fixed flags.
+void build_layout(Lowered& L) {
+    const size_t P = L.n_cameras(), G = L.n_groups(), NP = L.n_points();
+    L.cam_off.assign(P, 0);
+    L.cam_dim.assign(P, 0);
+    size_t D = 0;
+    for (size_t c = 0; c < P; ++c) {
+        uint32_t local = (c == 0 || L.cam_fixed[c]) ? 0u : (L.gauge_camera == (int)c ? 5u : 6u);
+        L.cam_off[c] = (uint32_t)D;
+        L.cam_dim[c] = local;
+        D += local;
+    }
+    L.grp_off.assign(G, 0);
+    L.grp_dim.assign(G, 0);
+    for (size_t g = 0; g < G; ++g) {
+        uint32_t local = L.grp_fixed[g] ? 0u : L.grp_free[g];
+        L.grp_off[g] = (uint32_t)D;
+        L.grp_dim[g] = local;
+        D += local;
+    }
+    L.D = D;
+    {
+        const size_t NB = P + G;
+        const unsigned T = range_count(NP, 1 << 14);
+        std::vector<uint64_t> part(T, 0);
+        for_ranges(NP, T, [&](unsigned t, size_t b, size_t e) {
+            std::vector<uint32_t> seen(NB, UINT32_MAX);  // last point that counted a block
+            uint64_t n = 0;
+            for (size_t q = b; q < e; ++q) {
+                if (L.point_fixed[q]) continue;
+                for (uint32_t o = L.point_off[q]; o < L.point_off[q + 1]; ++o) {
+                    const uint32_t c = L.obs_cam[o], g = L.cam_group[c];
+                    if (seen[c] != (uint32_t)q) n += L.cam_dim[c];
+                    seen[c] = (uint32_t)q;
+                    if (seen[P + g] != (uint32_t)q) n += L.grp_dim[g];
+                    seen[P + g] = (uint32_t)q;
+                }
+            }
+            part[t] = n;
+        });
+        L.n_cross = 0;
+        for (uint64_t n : part) L.n_cross += n;
+    }
+}
+
+void hold_groups(Lowered& L, const std::vector<uint32_t>& groups) {
+    for (uint32_t g : groups)
+        if (g < L.n_groups()) L.grp_fixed[g] = 1;
+    build_layout(L);
+}
+
+Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern, bool quarantine) {
     const size_t P = p->n_poses, N = p->n_points, O = p->n_observations, G = p->n_intrinsics;
     // ---- input quarantine (quarantine.rs:33-96) ----
     L.n_input_points = N;
@@ -193,7 +275,7 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern) {
         std::vector<uint32_t> cur(by_off.begin(), by_off.end() - 1);
         for (size_t o = 0; o < O; ++o) by_pt[cur[p->obs_point[o]]++] = (uint32_t)o;
     }
-    {
+    if (quarantine) {
         auto offset = [&](size_t o, double v[3]) {
             const double* X = p->point_xyz + 3 * p->obs_point[o];
             const double* C = p->pose_center + 3 * p->obs_pose[o];
@@ -444,25 +526,6 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern) {
     // ---- loss (problem.rs:372-382) ----
     L.loss = p->robust_loss;
     L.loss_scale = p->robust_loss == BA_LOSS_CAUCHY ? p->loss_scale / std::sqrt(2.0) : p->loss_scale;
-    // ---- layout (topology.rs:351-397) ----
-    L.cam_off.assign(P, 0);
-    L.cam_dim.assign(P, 0);
-    size_t D = 0;
-    for (size_t c = 0; c < P; ++c) {
-        uint32_t local = (c == 0 || L.cam_fixed[c]) ? 0u : (L.gauge_camera == (int)c ? 5u : 6u);
-        L.cam_off[c] = (uint32_t)D;
-        L.cam_dim[c] = local;
-        D += local;
-    }
-    L.grp_off.assign(G, 0);
-    L.grp_dim.assign(G, 0);
-    for (size_t g = 0; g < G; ++g) {
-        uint32_t local = L.grp_fixed[g] ? 0u : L.grp_free[g];
-        L.grp_off[g] = (uint32_t)D;
-        L.grp_dim[g] = local;
-        D += local;
-    }
-    L.D = D;
     // ---- scene scale at the initial state (optimizer.rs:620-627) ----
     if (L.gauge_camera >= 0) {
         const double* tg = &L.t[3 * L.gauge_camera];
@@ -473,35 +536,13 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern) {
             s += L.X[3 * q] * L.X[3 * q] + L.X[3 * q + 1] * L.X[3 * q + 1] + L.X[3 * q + 2] * L.X[3 * q + 2];
         L.scene_scale = std::fmax(std::sqrt(s / (double)NP), DBL_MIN);
     }
-    // ---- number of point parameters: each free point's distinct active camera parameters ----
-    {
-        const size_t NB = P + G;
-        const unsigned T = range_count(NP, 1 << 14);
-        std::vector<uint64_t> part(T, 0);
-        for_ranges(NP, T, [&](unsigned t, size_t b, size_t e) {
-            std::vector<uint32_t> seen(NB, UINT32_MAX);  // last point that counted a block
-            uint64_t n = 0;
-            for (size_t q = b; q < e; ++q) {
-                if (L.point_fixed[q]) continue;
-                for (uint32_t o = L.point_off[q]; o < L.point_off[q + 1]; ++o) {
-                    const uint32_t c = L.obs_cam[o], g = L.cam_group[c];
-                    if (seen[c] != (uint32_t)q) n += L.cam_dim[c];
-                    seen[c] = (uint32_t)q;
-                    if (seen[P + g] != (uint32_t)q) n += L.grp_dim[g];
-                    seen[P + g] = (uint32_t)q;
-                }
-            }
-            part[t] = n;
-        });
-        L.n_cross = 0;
-        for (uint64_t n : part) L.n_cross += n;
-    }
+    build_layout(L);
     if (!build_pattern) return Status{};
     // ---- per-point active parameters, first-encounter order (pattern.rs:306-379) ----
     L.cross_off.assign(NP + 1, 0);
     L.params.clear();
     {
-        std::vector<uint8_t> mark(D, 0);
+        std::vector<uint8_t> mark(L.D, 0);
         size_t start = 0;
         for (size_t q = 0; q < NP; ++q) {
             if (!L.point_fixed[q])
@@ -1011,6 +1052,14 @@ uint64_t reduced_nonzeros(const Lowered& L) {
     return count;
 }
 }  // namespace
+
+bool rank_certificate_fits(const Lowered& L, const ba_execution_plan& plan) {
+    const uint64_t nnz = plan.reduced_nonzeros ? plan.reduced_nonzeros : reduced_nonzeros(L);
+    const uint64_t cert = nnz * 8 * 4;  // estimated_direct_bytes(nnz, D, 0, 0, false)
+    const uint64_t total = plan.estimated_retained_bytes > UINT64_MAX - cert ? UINT64_MAX
+                                                                              : plan.estimated_retained_bytes + cert;
+    return direct_fits(total, plan.budget_bytes);
+}
 
 void plan_execution(const Lowered& L, const ba_solve_config* c, ba_execution_plan& out) {
     std::memset(&out, 0, sizeof(out));

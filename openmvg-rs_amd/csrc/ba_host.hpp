@@ -55,11 +55,24 @@ struct Lowered {
 };
 
 Status validate_problem(const ba_problem* p);
+// TrackGeometry verdict of every point (quarantine.rs:57-96) at the given camera centres
+// (3 per pose) and points (3 per point): rays without parallax, or a camera on the point.
+std::vector<uint8_t> degenerate_points(const ba_problem* p, const double* centers, const double* xyz);
 Status validate_config(const ba_solve_config* c);
-Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern);
+// quarantine = false keeps every point (the problems of the re-solves, whose input-degenerate
+// points were held out before the first solve).
+Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern, bool quarantine = true);
+// Blocks and point-parameter count from the fixed flags (topology.rs:351-397).
+void build_layout(Lowered& L);
+// Hold the given intrinsic groups at their current values: fixed blocks, layout rebuilt
+// (native/self_calibration.rs:240-247).
+void hold_groups(Lowered& L, const std::vector<uint32_t>& groups);
 
 // LinearSolver::Auto's camera-step choice for a lowered problem (execution_policy.rs:190-227).
 void plan_execution(const Lowered& L, const ba_solve_config* c, ba_execution_plan& out);
+// ExecutionPlan::rank_certificate_fits (execution_policy.rs:48-66) of an Auto plan that chose the
+// retained PCG step: its estimate plus the certificate's reduced values within 4/5 of the budget.
+bool rank_certificate_fits(const Lowered& L, const ba_execution_plan& plan);
 
 // Block b < n_cameras is the pose block of camera b; block n_cameras + g the intrinsic block
 // of group g.  Rows of a per-observation 9-row camera Jacobian: pose rows 0..5, intrinsic 6..8.

@@ -445,4 +445,17 @@ size_t cholesky_work_doubles(uint32_t n);
 void launch_cholesky_solve_mixed(hipStream_t s, const double* S, float* F, uint32_t n, uint32_t ld, const double* b,
                                  double* x, double* work, int* fail, int* done, int max_steps);
 void launch_negate(hipStream_t s, uint32_t n, const double* in, double* out);
+// Factor only (no right-hand side), in place, with the stored L_kk^-1 tiles in `work`; *fail != 0
+// after a pivot that is not finite and above rel_floor * max|diag(A)| (rel_floor = 0: every pivot
+// positive, the condition of a sparse LLT without regularisation).
+void launch_cholesky_factor(hipStream_t s, double* A, uint32_t n, uint32_t ld, double* work, int* fail,
+                            double rel_floor);
+// Marginal covariances of the trailing coordinates [j0, n) of a factored S (launch_cholesky_factor):
+// for each group g with offset goff[g] >= j0 and dimension gdim[g] <= 3, cov[9g..] = w (S^-1)_gg
+// (row-major 3x3, zero beyond gdim[g]); nothing when *fail.  R: trailing_inverse_doubles(n, j0)
+// doubles of scratch.
+size_t trailing_inverse_doubles(uint32_t n, uint32_t j0);
+void launch_group_marginals(hipStream_t s, const double* L, uint32_t n, uint32_t ld, const double* work,
+                            const int* fail, uint32_t j0, uint32_t ngroups, const uint32_t* goff,
+                            const uint32_t* gdim, double w, double* R, double* cov);
 }  // namespace ba

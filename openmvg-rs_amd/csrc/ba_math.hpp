@@ -28,7 +28,7 @@ struct Eps<float> {
 
 template <class T>
 BA_HD bool finite(T x) {
-    return isfinite(x);
+    return __builtin_isfinite(x);
 }
 
 enum : int { LOSS_L2 = 0, LOSS_HUBER = 1, LOSS_CAUCHY = 2 };

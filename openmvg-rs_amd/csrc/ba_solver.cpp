@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -19,9 +20,11 @@
 #include <vector>
 
 #include "../../include/ba_hip.h"
+#include "ba_certify.hpp"
 #include "ba_handle.hpp"
 #include "ba_host.hpp"
 #include "ba_kernels.hpp"
+#include "ba_math.hpp"
 
 #include "ba_engine.hpp"
 
@@ -124,9 +127,11 @@ struct StageDef {
 // The LM loop (sparse/self_calibration/mod.rs:212-639) on the GPU engine.
 // `loop_seconds` receives the host time from the first iteration's start to the end of the last one;
 // every iteration ends in a synchronous scalar readback, so this brackets completed device work.
+// `inlier_limit` and `initial_inliers` (count, RMSE): the inlier statistics at the initial state.
 template <class T>
 Status refine(Engine<T>& E, const Lowered& L, const ba_solve_config* cfg, ba_solve_report& rep,
-              ba_iteration_trace* trace_rows, int* final_state, double* loop_seconds) {
+              ba_iteration_trace* trace_rows, int* final_state, double* loop_seconds, double inlier_limit,
+              uint64_t* initial_inlier_count, double* initial_inlier_rmse) {
     const size_t D = L.D;
     const bool iterative = E.iterative();
     const double no = (double)L.n_obs();
@@ -153,6 +158,7 @@ Status refine(Engine<T>& E, const Lowered& L, const ba_solve_config* cfg, ba_sol
     {
         AsmScalars a0 = E.assemble(state, 0.0, L.loss, L.loss_scale, false);
         if (a0.unprojectable > 0.0) return error(BA_ERR_NUMERICAL, "point not projectable at the initial state");
+        E.inliers(inlier_limit, initial_inlier_count, initial_inlier_rmse);
     }
     TraceSink trace{trace_rows, now()};
     const double initial_damping = iterative ? 1e-1 : 1e-4;  // sparse/api.rs:300-305
@@ -332,10 +338,21 @@ int finish(const Status& st, char* msg, size_t len) {
 
 namespace ba {
 namespace {
+
+// The native final state and report of one LM solve.
+struct Refined {
+    ba_solve_report r{};
+    std::vector<double> R, t, gp, X;
+    uint64_t initial_inliers = 0;
+    double initial_inlier_rmse = NAN;
+};
+
+// One LM solve of a lowered problem on the device (the plan's camera step), with the inlier
+// statistics of its first and last linearisations.
 template <class T>
-Status run_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* cfg, const Lowered& L,
-                 const ba_execution_plan& plan, ba_solution* sol, ba_solve_report& r, ba_iteration_trace* trace,
-                 double t0) {
+Status run_refine(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* cfg, const Lowered& L,
+                  const ba_execution_plan& plan, ba_iteration_trace* trace, Refined& out) {
+    const double t0 = now();
     const bool iterative = plan.linear_solver == BA_SOLVER_ITERATIVE;
     Engine<T> E(h->stream, L, iterative, cfg->profile_kernel);
     E.set_schur_product(h->schur_product);
@@ -347,30 +364,356 @@ Status run_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* c
         return error(BA_ERR_RESOURCE_EXHAUSTED,
                      "native self-calibration workspace " + std::to_string(E.device_bytes()) +
                          " bytes exceeds the workspace budget " + std::to_string(cfg->workspace_budget_bytes));
-    double t1 = now();
+    const double t1 = now();
     int state = 0;
     double loop_seconds = 0.0;
-    Status st = refine(E, L, cfg, r, trace, &state, &loop_seconds);
+    const double limit = p->robust_loss == BA_LOSS_SQUARED ? INFINITY : p->loss_scale;
+    ba_solve_report& r = out.r;
+    Status st = refine(E, L, cfg, r, trace, &state, &loop_seconds, limit, &out.initial_inliers,
+                       &out.initial_inlier_rmse);
     if (iterative) h->last_schur_product = E.explicit_product() ? BA_SCHUR_EXPLICIT : BA_SCHUR_IMPLICIT;
     if (!st.ok()) return st;
-    {  // inlier statistics at the final linearisation (refine's zero-damping assembly)
-        const double limit = p->robust_loss == BA_LOSS_SQUARED ? INFINITY : p->loss_scale;
-        E.inliers(limit, &r.inlier_count, &r.final_inlier_rmse);
-    }
+    E.inliers(limit, &r.inlier_count, &r.final_inlier_rmse);  // at refine's final zero-damping assembly
     r.plan_reason = plan.reason;
     r.workspace_bytes = E.device_bytes();
-    std::vector<double> R, t, gp, X;
-    E.download(state, R, t, gp, X);
-    raise_solution(p, L, R.data(), t.data(), gp.data(), X.data(), sol);
-    double t3 = now();
+    E.download(state, out.R, out.t, out.gp, out.X);
+    const double t3 = now();
     r.seconds_setup = t1 - t0;
     r.seconds_iterations = loop_seconds;
     r.seconds_final = (t3 - t1) - loop_seconds;
     E.profile_result(&r.profile_launches, &r.profile_seconds);
     r.precision_fallbacks = E.mixed_fallbacks();
     r.refinement_residual = E.refinement_residual();
+    r.observability = BA_OBSERVABILITY_NOT_ASSESSED;
     return st;
 }
+
+Status refine_lowered(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* cfg, const Lowered& L,
+                      const ba_execution_plan& plan, ba_iteration_trace* trace, Refined& out) {
+    return cfg->precision == BA_PRECISION_MIXED ? run_refine<float>(h, p, cfg, L, plan, trace, out)
+                                                : run_refine<double>(h, p, cfg, L, plan, trace, out);
+}
+
+// The final rank certificate and the intrinsic marginals of a state, on the device in FP64: the
+// undamped reduced system of the direct form (a second engine on that state), factored in place
+// with every pivot required positive (the sparse LLT of direct.rs:259-264 without
+// regularisation), and (S^-1)_gg of every free group from the factor's trailing columns
+// (sparse/self_calibration/mod.rs:110-209, covariance.rs:562-587).
+struct Certificate {
+    bool formed = false;      // within this library's limits (BA_CERTIFY_MAX_*)
+    int32_t observability = BA_OBSERVABILITY_NOT_ASSESSED;
+    bool marginals = false;   // cov holds every free group's marginal
+    std::vector<double> cov;  // 9 per group, NaN where none
+};
+
+// Whether the dense certificate of a lowering stays within BA_CERTIFY_MAX_DIMENSION and
+// BA_CERTIFY_MAX_TERMS.
+bool certificate_in_reach(const Lowered& L) {
+    if (L.D > BA_CERTIFY_MAX_DIMENSION) return false;
+    uint64_t terms = 0;
+    for (size_t q = 0; q < L.n_points(); ++q) {
+        const uint64_t k = 2ull * (L.point_off[q + 1] - L.point_off[q]);
+        terms += L.point_fixed[q] ? k : k * k;
+    }
+    return terms <= BA_CERTIFY_MAX_TERMS;
+}
+
+Certificate certify(hipStream_t s, const Lowered& L, const Refined& f, bool exact_rank, bool want_marginals) {
+    Certificate c;
+    c.cov.assign(9 * L.n_groups(), NAN);
+    const size_t D = L.D;
+    if (!certificate_in_reach(L)) return c;
+    c.formed = true;
+    if (!exact_rank && !want_marginals) return c;
+    if (D == 0) {  // nothing free: full rank, and no group to certify
+        if (exact_rank) c.observability = BA_OBSERVABILITY_FULL_RANK;
+        c.marginals = want_marginals;
+        return c;
+    }
+    Lowered F = L;
+    F.R = f.R;
+    F.t = f.t;
+    F.gp = f.gp;
+    F.X = f.X;
+    Engine<double> E(s, F, false);
+    E.setup();
+    const AsmScalars a = E.assemble(0, 0.0, L.loss, L.loss_scale, true);
+    const bool valid = a.invalid == 0.0 && a.nonfinite == 0.0 && std::isfinite(a.cost);
+    bool factored = false;
+    if (valid) {
+        Buf<double> work, cov, scratch;
+        Buf<int> fail;
+        work.alloc(cholesky_work_doubles((uint32_t)D));
+        fail.alloc(1);
+        launch_cholesky_factor(s, E.S_mut(), (uint32_t)D, (uint32_t)D, work.p, fail.p, 0.0);
+        int failed = 1;
+        HIP_OK(hipMemcpyAsync(&failed, fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        factored = failed == 0;
+        size_t m = 0;
+        for (uint32_t d : L.grp_dim) m += d;
+        if (factored && want_marginals && m > 0) {
+            const uint32_t j0 = (uint32_t)(D - m);  // the free intrinsic coordinates come last
+            const uint32_t G = (uint32_t)L.n_groups();
+            Buf<uint32_t> goff, gdim;
+            goff.upload(L.grp_off, s);
+            gdim.upload(L.grp_dim, s);
+            scratch.alloc(trailing_inverse_doubles((uint32_t)D, j0));
+            cov.alloc(9 * (size_t)G);
+            const double w0 = loss_sqrt_weight(L.loss, L.loss_scale, 0.0);
+            launch_group_marginals(s, E.S(), (uint32_t)D, (uint32_t)D, work.p, fail.p, j0, G, goff.p, gdim.p,
+                                   w0 * w0, scratch.p, cov.p);
+            std::vector<double> host(9 * (size_t)G);
+            HIP_OK(hipMemcpyAsync(host.data(), cov.p, host.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            bool finite = true;
+            for (size_t g = 0; g < G; ++g)
+                for (uint32_t k = 0; k < 9; ++k)
+                    if (L.grp_dim[g]) finite &= std::isfinite(host[9 * g + k]);
+            if (finite) {
+                for (size_t g = 0; g < G; ++g)
+                    if (L.grp_dim[g]) std::copy(host.begin() + 9 * g, host.begin() + 9 * g + 9, c.cov.begin() + 9 * g);
+                c.marginals = true;
+            }
+        } else if (factored && want_marginals) {
+            c.marginals = true;  // no free group
+        }
+    }
+    if (exact_rank) c.observability = factored ? BA_OBSERVABILITY_FULL_RANK : BA_OBSERVABILITY_RANK_DEFICIENT;
+    return c;
+}
+
+// Whether the exact rank certificate runs (workspace/self_calibration.rs:515-531): always, except
+// when Auto chose the retained PCG step and its estimate plus the certificate's reduced values do
+// not fit 4/5 of the budget.  The marginals need the direct storage, which Auto's budget
+// excludes whenever it chose PCG (workspace.rs:251-265).
+void certificate_policy(const Lowered& L, const ba_solve_config* cfg, const ba_execution_plan& plan, bool* exact,
+                        bool* marginals) {
+    const bool auto_pcg = cfg->linear_solver == BA_SOLVER_AUTO && plan.linear_solver == BA_SOLVER_ITERATIVE;
+    *marginals = !auto_pcg;
+    *exact = !auto_pcg || rank_certificate_fits(L, plan);
+}
+
+// The certificate's verdict in the report (native/reporting.rs:24-100, run_state.rs:170-180).
+void apply_certificate(ba_solve_report& r, const Certificate& c, size_t D) {
+    r.observability = c.observability;
+    if (c.observability == BA_OBSERVABILITY_FULL_RANK) {
+        r.pose_rank = r.pose_dimension = (uint32_t)D;
+    } else if (c.observability == BA_OBSERVABILITY_RANK_DEFICIENT) {
+        r.pose_rank = 0;
+        r.pose_dimension = (uint32_t)D;
+    } else {
+        r.pose_rank = r.pose_dimension = 0;
+    }
+    // ObservabilityDiagnostics needs a positive dimension
+    if (r.pose_dimension == 0) r.pose_rank = 0;
+    const bool well_constrained = c.observability != BA_OBSERVABILITY_RANK_DEFICIENT && r.weak_point_count == 0;
+    if (!well_constrained) r.status = BA_STATUS_DEGENERATE;
+}
+
+// Solution::canonicalize_roundoff_non_descent (solver.rs:1030-1101): a converged solve that only
+// raised the objective by round-off returns its input state, with status Skipped.
+void canonicalize_roundoff(const ba_problem* p, HostSolution& s, const Refined& first) {
+    ba_solve_report& r = s.rep;
+    if (p->gauge == BA_GAUGE_FIXED_BASELINE) {
+        const double* a = p->pose_center + 3 * p->gauge_first;
+        const double* b = p->pose_center + 3 * p->gauge_second;
+        const double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+        const double initial = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        if (!std::isfinite(initial) ||
+            std::fabs(initial - p->gauge_distance) > 512.0 * DBL_EPSILON * std::fmax(initial, p->gauge_distance))
+            return;
+    }
+    const double increase = r.final_cost - r.initial_cost;
+    const double tol = 2.0 * std::sqrt(DBL_EPSILON) * std::fmax(std::fabs(r.initial_cost), 1.0);
+    const bool converged = r.status == BA_STATUS_CONVERGED_GRADIENT || r.status == BA_STATUS_CONVERGED_STEP ||
+                           r.status == BA_STATUS_CONVERGED_COST;
+    if (!(increase > 0.0 && increase <= tol && converged && r.observability != BA_OBSERVABILITY_RANK_DEFICIENT)) return;
+    bool same = true;
+    for (size_t k = 0; k < 9 * p->n_poses && same; ++k) same = s.rot[k] == p->pose_rotation[k];
+    for (size_t k = 0; k < 3 * p->n_poses && same; ++k) same = s.cen[k] == p->pose_center[k];
+    for (size_t g = 0; g < p->n_intrinsics && same; ++g) {
+        const ba_intrinsic& in = p->intrinsics[g];
+        const bool brown = in.model != BA_CAMERA_PINHOLE;
+        same = s.focal[g] == in.fx && s.k1[g] == (brown && in.model >= BA_CAMERA_BROWN_K1 ? in.k1 : 0.0) &&
+               s.k2[g] == (brown && in.model >= BA_CAMERA_BROWN_K1K2 ? in.k2 : 0.0);
+    }
+    for (size_t k = 0; k < 3 * p->n_points && same; ++k) same = s.xyz[k] == p->point_xyz[k];
+    if (!same) {
+        std::copy(p->pose_rotation, p->pose_rotation + 9 * p->n_poses, s.rot.begin());
+        std::copy(p->pose_center, p->pose_center + 3 * p->n_poses, s.cen.begin());
+        for (size_t g = 0; g < p->n_intrinsics; ++g) {
+            const ba_intrinsic& in = p->intrinsics[g];
+            const bool brown = in.model != BA_CAMERA_PINHOLE;
+            s.focal[g] = in.fx;
+            s.k1[g] = brown && in.model >= BA_CAMERA_BROWN_K1 ? in.k1 : 0.0;
+            s.k2[g] = brown && in.model >= BA_CAMERA_BROWN_K1K2 ? in.k2 : 0.0;
+        }
+        std::copy(p->point_xyz, p->point_xyz + 3 * p->n_points, s.xyz.begin());
+        // the input's rank and gradient need not match the accepted proposal
+        r.observability = BA_OBSERVABILITY_NOT_ASSESSED;
+        r.pose_rank = r.pose_dimension = 0;
+        r.weak_point_count = 0;
+    }
+    r.final_cost = r.initial_cost;
+    r.final_rmse = r.initial_rmse;
+    r.status = BA_STATUS_SKIPPED;
+    r.inlier_count = first.initial_inliers;
+    r.final_inlier_rmse = first.initial_inlier_rmse;
+}
+
+// The native solve of one problem (solve_once of native/mod.rs:318-335 with
+// solve_self_calibrating, native/self_calibration.rs:223-249): the LM solve; under FULL the
+// certificate, and when the marginals leave groups unconstrained a second solve from the input
+// state with those groups held; then the round-off canonicalisation.  `pre`: the problem's
+// lowering when it is already at hand.
+Status solve_once(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* cfg, ba_iteration_trace* trace,
+                  bool quarantine, const Lowered* pre, HostSolution& out) {
+    const bool full = cfg->certification == BA_CERTIFY_FULL;
+    Lowered own;
+    if (!pre) {
+        Status st = lower_problem(p, own, false, quarantine);
+        if (!st.ok()) return st;
+    }
+    const Lowered& L = pre ? *pre : own;
+    ba_execution_plan plan;
+    plan_execution(L, cfg, plan);
+    const uint32_t trace_start = trace ? trace->count : 0;
+    Refined first;
+    Status st = refine_lowered(h, p, cfg, L, plan, trace, first);
+    if (!st.ok()) return st;
+    out.alloc(p);
+    if (!full) {
+        ba_solution v = out.view();
+        raise_solution(p, L, first.R.data(), first.t.data(), first.gp.data(), first.X.data(), &v);
+        out.rep = first.r;
+        return st;
+    }
+    bool exact, marginals;
+    certificate_policy(L, cfg, plan, &exact, &marginals);
+    const Certificate c = certify(h->stream, L, first, exact, marginals);
+    apply_certificate(first.r, c, L.D);
+    out.cov = c.cov;
+    // beyond this library's limits nothing is certified and nothing is held
+    const std::vector<uint32_t> held =
+        c.formed ? unconstrained_groups(p, L, first.gp.data(), marginals && c.marginals ? c.cov.data() : nullptr)
+                 : std::vector<uint32_t>{};
+    if (held.empty()) {
+        ba_solution v = out.view();
+        raise_solution(p, L, first.R.data(), first.t.data(), first.gp.data(), first.X.data(), &v);
+        out.rep = first.r;
+    } else {
+        // re-solve from the input state with the unconstrained groups at their input values
+        Lowered L2;
+        st = lower_problem(p, L2, false, quarantine);
+        if (!st.ok()) return st;
+        hold_groups(L2, held);
+        ba_execution_plan plan2;
+        plan_execution(L2, cfg, plan2);
+        if (trace) trace->count = trace_start;
+        Refined second;
+        st = refine_lowered(h, p, cfg, L2, plan2, trace, second);
+        if (!st.ok()) return st;
+        certificate_policy(L2, cfg, plan2, &exact, &marginals);
+        apply_certificate(second.r, certify(h->stream, L2, second, exact, false), L2.D);
+        ba_solution v = out.view();
+        raise_solution(p, L2, second.R.data(), second.t.data(), second.gp.data(), second.X.data(), &v);
+        out.rep = second.r;
+        first.initial_inliers = second.initial_inliers;
+        first.initial_inlier_rmse = second.initial_inlier_rmse;
+    }
+    for (uint32_t g : held) out.unconstrained[g] = 1;
+    out.rep.unconstrained_intrinsics = (uint32_t)held.size();
+    canonicalize_roundoff(p, out, first);
+    return Status{};
+}
+
+// solve_with_quarantine (native/mod.rs:252-316): the input-degenerate points held out, the solve,
+// and up to four re-solves without the weak points of the previous solution, each warm-started
+// from it with at most 8 iterations; held-out points keep their input positions.
+Status solve_native(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* cfg, ba_iteration_trace* trace,
+                    HostSolution& result) {
+    Lowered L;
+    Status st = lower_problem(p, L, false, true);  // input quarantine and every lowering check
+    if (!st.ok()) return st;
+    const size_t N = p->n_points;
+    OwnedProblem working_own;
+    const ba_problem* working = p;
+    std::vector<uint32_t> kept(N);
+    for (size_t i = 0; i < N; ++i) kept[i] = (uint32_t)i;
+    size_t quarantined = L.n_quarantined;
+    if (quarantined) {
+        without_points(p, nullptr, L.quarantined, working_own, kept);
+        working = &working_own.p;
+    }
+    const bool changes_scale = p->gauge == BA_GAUGE_FIXED_BASELINE && std::fabs(L.rebase_scale - 1.0) > 1e-9;
+    HostSolution sol;
+    st = solve_once(h, working, cfg, trace, false, quarantined ? nullptr : &L, sol);
+    if (!st.ok()) return st;
+    if (cfg->certification == BA_CERTIFY_FULL) {
+        ba_solve_config retry = *cfg;
+        retry.max_iterations = std::min<uint32_t>(cfg->max_iterations, RESOLVE_ITERATIONS);
+        OwnedProblem next_own[2];
+        const ba_problem* current = working;
+        bool have_origin = false;
+        ba_solve_report origin{};
+        uint32_t rounds = 0;
+        for (uint32_t round = 0; round < MAX_RESOLVES; ++round) {
+            if (!needs_resolve(sol.rep)) break;
+            std::vector<uint8_t> weak = weak_points(current, sol);
+            size_t nweak = 0;
+            for (size_t i = 0; i < weak.size(); ++i) {
+                if (current->point_fixed && current->point_fixed[i]) weak[i] = 0;
+                nweak += weak[i];
+            }
+            if (!nweak) break;
+            OwnedProblem& next = next_own[round % 2];
+            std::vector<uint32_t> next_kept;
+            without_points(current, &sol, weak, next, next_kept);
+            HostSolution resolved;
+            if (!solve_once(h, &next.p, &retry, trace, false, nullptr, resolved).ok()) break;
+            std::vector<uint32_t> composed(next_kept.size());
+            for (size_t i = 0; i < next_kept.size(); ++i) composed[i] = kept[next_kept[i]];
+            kept.swap(composed);
+            quarantined += nweak;
+            if (have_origin) continue_report(sol.rep, origin);
+            origin = sol.rep;
+            have_origin = true;
+            sol = std::move(resolved);
+            current = &next.p;
+            ++rounds;
+        }
+        if (have_origin) continue_report(sol.rep, origin);
+        sol.rep.weak_point_resolves = rounds;
+    }
+    if (changes_scale && quarantined > 0)
+        return error(BA_ERR_UNSUPPORTED, "native BA cannot change its physical baseline while holding out points");
+    if (kept.size() == N) {
+        result = std::move(sol);
+        result.rep.quarantined_points = 0;
+        return Status{};
+    }
+    // Solution::lifted (solver.rs:891-935): back to the caller's points
+    result.alloc(p);
+    result.rot = sol.rot;
+    result.cen = sol.cen;
+    result.focal = sol.focal;
+    result.k1 = sol.k1;
+    result.k2 = sol.k2;
+    result.cov = sol.cov;
+    result.unconstrained = sol.unconstrained;
+    result.rep = sol.rep;
+    std::copy(p->point_xyz, p->point_xyz + 3 * N, result.xyz.begin());
+    std::fill(result.quarantined.begin(), result.quarantined.end(), 1);
+    for (size_t i = 0; i < kept.size(); ++i) {
+        std::copy(sol.xyz.begin() + 3 * i, sol.xyz.begin() + 3 * i + 3, result.xyz.begin() + 3 * (size_t)kept[i]);
+        result.quarantined[kept[i]] = 0;
+    }
+    result.rep.quarantined_points = (uint32_t)quarantined;
+    result.rep.point_count = N;
+    return Status{};
+}
+
 }  // namespace
 }  // namespace ba
 
@@ -492,18 +835,19 @@ int ba_hip_solve(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* c
     if (!st.ok()) return finish(st, msg, len);
     try {
         if (hipSetDevice(h->device) != hipSuccess) return finish(error(BA_ERR_INTERNAL, "hipSetDevice"), msg, len);
-        double t0 = now();
-        Lowered L;
-        st = lower_problem(p, L, false);
-        if (!st.ok()) return finish(st, msg, len);
         if (trace) trace->count = 0;
-        ba_execution_plan plan;
-        plan_execution(L, cfg, plan);
-        ba_solve_report r;
-        st = cfg->precision == BA_PRECISION_MIXED ? run_solve<float>(h, p, cfg, L, plan, sol, r, trace, t0)
-                                                  : run_solve<double>(h, p, cfg, L, plan, sol, r, trace, t0);
+        HostSolution out;
+        if (cfg->certification == BA_CERTIFY_FULL) {
+            st = solve_native(h, p, cfg, trace, out);
+        } else {
+            Lowered L;
+            st = lower_problem(p, L, false, true);
+            if (st.ok()) st = solve_once(h, p, cfg, trace, true, &L, out);
+            if (st.ok()) out.rep.quarantined_points = (uint32_t)L.n_quarantined;
+        }
         if (!st.ok()) return finish(st, msg, len);
-        if (rep) *rep = r;
+        out.copy_to(sol);
+        if (rep) *rep = out.rep;
         return BA_OK;
     } catch (const HipError& e) {
         return finish(error(BA_ERR_INTERNAL, std::string("HIP error: ") + hipGetErrorString(e.code) + " in " + e.what),

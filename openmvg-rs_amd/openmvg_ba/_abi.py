@@ -62,6 +62,13 @@ STATUS_CONVERGED_COST = 2
 STATUS_MAX_ITERATIONS = 3
 STATUS_STALLED = 4
 STATUS_DEGENERATE = 5
+STATUS_SKIPPED = 6
+CERTIFY_FULL = 0
+CERTIFY_LOOP_ONLY = 1
+OBSERVABILITY_FULL_RANK = 0
+OBSERVABILITY_ONLY_EXPECTED_GAUGE = 1
+OBSERVABILITY_RANK_DEFICIENT = 2
+OBSERVABILITY_NOT_ASSESSED = 3
 STATUS_NAMES = {
     0: "converged(gradient)",
     1: "converged(step)",
@@ -69,6 +76,7 @@ STATUS_NAMES = {
     3: "max_iterations",
     4: "stalled",
     5: "degenerate",
+    6: "skipped",
 }
 
 _d = C.POINTER(C.c_double)
@@ -118,6 +126,7 @@ class SolveConfigC(C.Structure):
         ("pcg_max_iterations", C.c_uint32),
         ("pcg_relative_tolerance", C.c_double),
         ("profile_kernel", C.c_int32),
+        ("certification", C.c_int32),
     ]
 
 
@@ -143,6 +152,9 @@ class SolveReportC(C.Structure):
         ("precision_fallbacks", C.c_uint32), ("refinement_residual", C.c_double),
         ("plan_reason", C.c_int32), ("workspace_bytes", C.c_uint64),
         ("inlier_count", C.c_uint64), ("final_inlier_rmse", C.c_double),
+        ("observability", C.c_int32), ("pose_rank", C.c_uint32), ("pose_dimension", C.c_uint32),
+        ("unconstrained_intrinsics", C.c_uint32), ("weak_point_resolves", C.c_uint32),
+        ("_pad_certify", C.c_uint32),
     ]
 
 
@@ -175,6 +187,7 @@ class SolutionC(C.Structure):
         ("pose_rotation", _d), ("pose_center", _d),
         ("intrinsic_focal", _d), ("intrinsic_k1", _d), ("intrinsic_k2", _d),
         ("point_xyz", _d), ("point_quarantined", _u8), ("intrinsic_unconstrained", _u8),
+        ("intrinsic_covariance", _d),
     ]
 
 

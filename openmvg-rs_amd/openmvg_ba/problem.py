@@ -210,6 +210,7 @@ class SolveConfig:
     pcg_max_iterations: int = 0
     pcg_relative_tolerance: float = 0.0
     profile_kernel: int = A.PROFILE_NONE
+    certification: int = A.CERTIFY_FULL
 
     def with_(self, **kw) -> "SolveConfig":
         return dataclasses.replace(self, **kw)
@@ -226,6 +227,7 @@ class SolveConfig:
         c.pcg_max_iterations = self.pcg_max_iterations
         c.pcg_relative_tolerance = self.pcg_relative_tolerance
         c.profile_kernel = self.profile_kernel
+        c.certification = self.certification
         return c
 
 
@@ -252,6 +254,8 @@ class Solution:
     point_quarantined: np.ndarray
     report: SolveReport
     trace: dict = field(default_factory=dict)
+    intrinsic_unconstrained: np.ndarray | None = None
+    intrinsic_covariance: np.ndarray | None = None  # (G, 3, 3), NaN where none was formed
 
 
 def _solution_buffers(problem: BundleProblem, sol: A.SolutionC) -> dict:
@@ -261,13 +265,14 @@ def _solution_buffers(problem: BundleProblem, sol: A.SolutionC) -> dict:
         pose_rotation=np.zeros((P, 3, 3)), pose_center=np.zeros((P, 3)),
         intrinsic_focal=np.zeros(G), intrinsic_k1=np.zeros(G), intrinsic_k2=np.zeros(G),
         point_xyz=np.zeros((N, 3)), point_quarantined=np.zeros(N, dtype=np.uint8),
-        intrinsic_unconstrained=np.zeros(G, dtype=np.uint8),
+        intrinsic_unconstrained=np.zeros(G, dtype=np.uint8), intrinsic_covariance=np.full((G, 3, 3), np.nan),
     )
     for name in ("pose_rotation", "pose_center", "intrinsic_focal", "intrinsic_k1", "intrinsic_k2",
                  "point_xyz"):
         setattr(sol, name, A.ptr(out[name], C.c_double))
     sol.point_quarantined = A.ptr(out["point_quarantined"], C.c_uint8)
     sol.intrinsic_unconstrained = A.ptr(out["intrinsic_unconstrained"], C.c_uint8)
+    sol.intrinsic_covariance = A.ptr(out["intrinsic_covariance"], C.c_double)
     return out
 
 
@@ -306,7 +311,8 @@ def solve_with(lib, function: str, problem: BundleProblem, config: SolveConfig, 
     trace = {k: v[:n].copy() for k, v in tr_arrays.items()}
     return Solution(out["pose_rotation"], out["pose_center"], out["intrinsic_focal"],
                     out["intrinsic_k1"], out["intrinsic_k2"], out["point_xyz"],
-                    out["point_quarantined"], report, trace)
+                    out["point_quarantined"], report, trace, out["intrinsic_unconstrained"],
+                    out["intrinsic_covariance"])
 
 
 def plan_with(lib, function: str, problem: BundleProblem, config: SolveConfig) -> dict:

@@ -334,6 +334,54 @@ def candidate_rule_scene(seed: int = 0xBA0001):
     return prob, q_near, q_far
 
 
+def narrow_field_scene(seed: int = 0xBA0031, n_points: int = 600, spread: float = 0.3) -> Scene:
+    """The ring's 10 cameras with Brown (k1, k2) free and certification required, looking at
+    points in a cube of half-width `spread` at the ring's centre: every point projects within
+    ~0.04 (normalised) of the principal point, where the radial terms move a pixel by ~1e-4 per
+    unit k1, so k1 and k2 stay unconstrained at the image corner and the certificate holds them
+    at their input values (crates/openmvg-ba/src/native/self_calibration.rs:215-303)."""
+    rng = np.random.default_rng(seed)
+    n_cameras = 10
+    th = 2.0 * np.pi * np.arange(n_cameras) / n_cameras
+    C = np.stack([10.0 * np.cos(th), np.zeros_like(th), 10.0 * np.sin(th)], -1)
+    R = np.stack([look_at(c, np.zeros(3)) for c in C])
+    f = np.full(n_cameras, 1000.0)
+    k1 = np.full(n_cameras, -0.05)
+    k2 = np.full(n_cameras, 0.01)
+    X = rng.uniform(-spread, spread, (n_points, 3))
+    cap = rng.integers(2, 11, n_points)
+    op, opt, pix, kept = _observe(R, C, f, k1, k2, X, rng, cap, 0.5)
+
+    def init(R, C, Xk, f, k1, k2):
+        dR = exp_so3(rng.normal(0.0, 0.002, (len(R), 3)))
+        return (dR @ R, C + rng.normal(0.0, 0.01, C.shape), Xk + rng.normal(0.0, 0.005, Xk.shape),
+                f * 1.01, k1 + 0.02, k2.copy())
+
+    return _finish("narrow", R, C, f, k1, k2, X, op, opt, pix, kept, init, A.CAMERA_BROWN_K1K2, True, waived=False)
+
+
+def rank_deficient_scene(seed: int = 0xBA0001) -> BundleProblem:
+    """The ring scene plus an 11th camera (its own intrinsic group, focal free) that sees one
+    point only: two residuals cannot fix its seven parameters, so the undamped reduced camera
+    system is singular and the rank certificate fails (direct.rs:288-334)."""
+    sc = ring_scene(seed)
+    p = sc.problem
+    C11 = np.array([0.0, 10.0, 0.0])
+    R11 = look_at(C11, np.zeros(3), up=(0.0, 0.0, 1.0))
+    q = int(p.obs_point[0])
+    pc = R11 @ (sc.truth_points[q] - C11)
+    uv = np.array([1000.0 * pc[0] / pc[2] + CX, 1000.0 * pc[1] / pc[2] + CY])
+    intr = list(p.intrinsics) + [Intrinsics(focal=1000.0, cx=CX, cy=CY, width=WIDTH, height=HEIGHT,
+                                            model=A.CAMERA_PINHOLE, var_focal=True, certification_waived=True)]
+    return BundleProblem(
+        pose_rotation=np.concatenate([p.pose_rotation, R11[None]]), pose_center=np.concatenate([p.pose_center, C11[None]]),
+        intrinsics=intr, point_xyz=p.point_xyz.copy(),
+        obs_pose=np.concatenate([p.obs_pose, [len(p.pose_rotation)]]).astype(np.uint32),
+        obs_intrinsic=np.concatenate([p.obs_intrinsic, [len(p.pose_rotation)]]).astype(np.uint32),
+        obs_point=np.concatenate([p.obs_point, [q]]).astype(np.uint32),
+        obs_pixel=np.concatenate([p.obs_pixel, uv[None]]))
+
+
 def medium_scene(seed: int = 0xBA0002) -> Scene:
     """200 cameras, 50,000 points, ~300k observations, (f, k1, k2) free, squared loss."""
     return grid_scene(seed, 20, 10, 50_000)

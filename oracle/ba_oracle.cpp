@@ -1539,10 +1539,13 @@ static bool intrinsic_is_fixed(const ba_intrinsic& in) {
     return !in.var_focal && !in.var_principal_point && !in.var_radial && !in.var_tangential;
 }
 
-static Err lower(const ba_problem* p, Lowered& L) {
+// quarantine: the input pass of solve_with_quarantine (off for the problems of the re-solves,
+// whose degenerate inputs were held out before); hold: intrinsic groups held at their values
+// (the unconstrained re-solve of native/self_calibration.rs:240-247), or NULL.
+static Err lower(const ba_problem* p, Lowered& L, bool quarantine = true, const std::vector<uint8_t>* hold = nullptr) {
     // solve_with_quarantine input pass (native/mod.rs:271-290)
     L.n_input_points = p->n_points;
-    L.quarantined = degenerate_input_points(p);
+    L.quarantined = quarantine ? degenerate_input_points(p) : std::vector<uint8_t>(p->n_points, 0);
     size_t nq = 0;
     for (auto q : L.quarantined) nq += q;
     if (nq == p->n_points)
@@ -1744,7 +1747,7 @@ static Err lower(const ba_problem* p, Lowered& L) {
         L.gp[3 * g] = in.fx;
         L.gp[3 * g + 1] = k1;
         L.gp[3 * g + 2] = k2;
-        L.grp_fixed[g] = intrinsic_is_fixed(in);
+        L.grp_fixed[g] = intrinsic_is_fixed(in) || (hold && (*hold)[g]);
         // free_radial_dimension, native/self_calibration.rs:442-451
         uint32_t dim = 1;
         if (in.model != BA_CAMERA_PINHOLE && in.var_radial) {
@@ -2543,6 +2546,643 @@ using namespace bao;
 
 extern "C" {
 
+// ---------------------------------------------------------------------------------------
+// Post-loop certification: the final rank certificate (sparse/self_calibration/mod.rs:498-560,
+// direct.rs:288-334), the intrinsic marginals and the held re-solve (native/self_calibration.rs:
+// 223-303), the weak-point re-solves (native/mod.rs:252-350, solver.rs:1149-1219) and the round-off
+// canonicalisation (solver.rs:1030-1101).
+// ---------------------------------------------------------------------------------------
+struct OwnedProblemO {
+    std::vector<double> rot, cen, xyz, pix;
+    std::vector<uint8_t> vr, vt, fixed;
+    std::vector<ba_intrinsic> intr;
+    std::vector<uint32_t> pose, group, point;
+    ba_problem p{};
+};
+
+struct SolutionO {
+    std::vector<double> rot, cen, f, k1, k2, xyz, cov;
+    std::vector<uint8_t> quarantined, unconstrained;
+    ba_solve_report rep{};
+    void init(const ba_problem* p) {
+        rot.assign(9 * p->n_poses, 0.0);
+        cen.assign(3 * p->n_poses, 0.0);
+        f.assign(p->n_intrinsics, 0.0);
+        k1.assign(p->n_intrinsics, 0.0);
+        k2.assign(p->n_intrinsics, 0.0);
+        xyz.assign(3 * p->n_points, 0.0);
+        cov.assign(9 * p->n_intrinsics, NaN);
+        quarantined.assign(p->n_points, 0);
+        unconstrained.assign(p->n_intrinsics, 0);
+        std::memset(&rep, 0, sizeof(rep));
+    }
+    ba_solution view() {
+        ba_solution s{};
+        s.pose_rotation = rot.data();
+        s.pose_center = cen.data();
+        s.intrinsic_focal = f.data();
+        s.intrinsic_k1 = k1.data();
+        s.intrinsic_k2 = k2.data();
+        s.point_xyz = xyz.data();
+        s.point_quarantined = quarantined.data();
+        s.intrinsic_unconstrained = unconstrained.data();
+        s.intrinsic_covariance = cov.data();
+        return s;
+    }
+};
+
+// BundleProblem::without_points (problem.rs:650-701)
+static void restrict_problem(const ba_problem* p, const SolutionO* warm, const std::vector<uint8_t>& drop,
+                             OwnedProblemO& o, std::vector<size_t>& kept) {
+    o.rot = warm ? warm->rot : std::vector<double>(p->pose_rotation, p->pose_rotation + 9 * p->n_poses);
+    o.cen = warm ? warm->cen : std::vector<double>(p->pose_center, p->pose_center + 3 * p->n_poses);
+    o.vr.assign(p->pose_var_rotation, p->pose_var_rotation + p->n_poses);
+    o.vt.assign(p->pose_var_translation, p->pose_var_translation + p->n_poses);
+    o.intr.assign(p->intrinsics, p->intrinsics + p->n_intrinsics);
+    if (warm) {
+        for (size_t g = 0; g < o.intr.size(); ++g) {
+            ba_intrinsic& in = o.intr[g];
+            in.fx = warm->f[g];
+            in.fy = warm->f[g];
+            if (in.model == BA_CAMERA_BROWN_K1 || in.model == BA_CAMERA_BROWN_K1K2 || in.model == BA_CAMERA_BROWN_K1K2K3)
+                in.k1 = warm->k1[g];
+            if (in.model == BA_CAMERA_BROWN_K1K2 || in.model == BA_CAMERA_BROWN_K1K2K3) in.k2 = warm->k2[g];
+        }
+    }
+    std::vector<size_t> index(p->n_points, SIZE_MAX);
+    kept.clear();
+    o.xyz.clear();
+    o.fixed.clear();
+    for (size_t i = 0; i < p->n_points; ++i) {
+        if (drop[i]) continue;
+        index[i] = kept.size();
+        kept.push_back(i);
+        for (int k = 0; k < 3; ++k) o.xyz.push_back(warm ? warm->xyz[3 * i + k] : p->point_xyz[3 * i + k]);
+        o.fixed.push_back(point_is_fixed(p, i) ? 1 : 0);
+    }
+    o.pose.clear();
+    o.group.clear();
+    o.point.clear();
+    o.pix.clear();
+    for (size_t k = 0; k < p->n_observations; ++k) {
+        const size_t q = index[p->obs_point[k]];
+        if (q == SIZE_MAX) continue;
+        o.pose.push_back(p->obs_pose[k]);
+        o.group.push_back(p->obs_intrinsic[k]);
+        o.point.push_back((uint32_t)q);
+        o.pix.push_back(p->obs_pixel[2 * k]);
+        o.pix.push_back(p->obs_pixel[2 * k + 1]);
+    }
+    o.p = *p;
+    o.p.pose_rotation = o.rot.data();
+    o.p.pose_center = o.cen.data();
+    o.p.pose_var_rotation = o.vr.data();
+    o.p.pose_var_translation = o.vt.data();
+    o.p.intrinsics = o.intr.data();
+    o.p.n_points = kept.size();
+    o.p.point_xyz = o.xyz.data();
+    o.p.point_fixed = o.fixed.data();
+    o.p.n_observations = o.point.size();
+    o.p.obs_pose = o.pose.data();
+    o.p.obs_intrinsic = o.group.data();
+    o.p.obs_point = o.point.data();
+    o.p.obs_pixel = o.pix.data();
+}
+
+// Eigenvalues of a symmetric 3x3 matrix from its characteristic cubic (trigonometric form).
+static void sym3_eigenvalues(const double a[3][3], double ev[3]) {
+    const double p1 = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    const double q = (a[0][0] + a[1][1] + a[2][2]) / 3.0;
+    if (p1 == 0.0) {
+        ev[0] = a[0][0];
+        ev[1] = a[1][1];
+        ev[2] = a[2][2];
+        return;
+    }
+    const double p2 = (a[0][0] - q) * (a[0][0] - q) + (a[1][1] - q) * (a[1][1] - q) + (a[2][2] - q) * (a[2][2] - q) +
+                      2.0 * p1;
+    const double pp = std::sqrt(p2 / 6.0);
+    double b[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) b[i][j] = (a[i][j] - (i == j ? q : 0.0)) / pp;
+    const double det = b[0][0] * (b[1][1] * b[2][2] - b[1][2] * b[2][1]) - b[0][1] * (b[1][0] * b[2][2] - b[1][2] * b[2][0]) +
+                       b[0][2] * (b[1][0] * b[2][1] - b[1][1] * b[2][0]);
+    const double r = std::fmin(std::fmax(det / 2.0, -1.0), 1.0);
+    const double phi = std::acos(r) / 3.0;
+    ev[0] = q + 2.0 * pp * std::cos(phi);
+    ev[2] = q + 2.0 * pp * std::cos(phi + 2.0 * M_PI / 3.0);
+    ev[1] = 3.0 * q - ev[0] - ev[2];
+}
+
+// CameraModel::project_with_jacobian of the pinhole and the radial Brown-Conrady camera
+// (openmvg-core camera/pinhole.rs:301-322, brown_conrady.rs:240-267, 334-360,
+// distortion.rs:530-633) at a camera-frame point: pixel and d pixel / d point.
+static bool camera_project(const ba_intrinsic& in, double f, double k1, double k2, const double pc[3], double pix[2],
+                           double J[2][3]) {
+    if (!(std::isfinite(pc[0]) && std::isfinite(pc[1]) && std::isfinite(pc[2]))) return false;
+    if (pc[2] <= 0.0) return false;
+    const double iz = 1.0 / pc[2];
+    if (in.model == BA_CAMERA_PINHOLE) {
+        pix[0] = std::fma(f, pc[0] * iz, in.cx);
+        pix[1] = std::fma(f, pc[1] * iz, in.cy);
+        J[0][0] = f * iz;
+        J[0][1] = 0.0;
+        J[0][2] = -f * pc[0] * (iz * iz);
+        J[1][0] = 0.0;
+        J[1][1] = f * iz;
+        J[1][2] = -f * pc[1] * (iz * iz);
+    } else {
+        const double x = pc[0] / pc[2], y = pc[1] / pc[2];
+        if (!(std::isfinite(x) && std::isfinite(y))) return false;
+        // distortion_jacobian: (j11, j12, j22) and its determinant
+        auto dj = [&](double out[4]) {
+            const double q = std::fma(x, x, y * y);
+            const double rf = 1.0 + k1 * q + k2 * q * q;
+            const double rd = k1 + 2.0 * (k2 * q);
+            out[0] = rf + 2.0 * x * x * rd;
+            out[1] = 2.0 * x * y * rd;
+            out[2] = rf + 2.0 * y * y * rd;
+            out[3] = std::fma(out[0], out[2], -out[1] * out[1]);
+            return std::isfinite(out[0]) && std::isfinite(out[1]) && std::isfinite(out[2]) && std::isfinite(out[3]);
+        };
+        double d[4];
+        if (k1 != 0.0 || k2 != 0.0) {
+            const double q = std::fma(x, x, y * y);
+            if (!std::isfinite(q)) return false;
+            double turn;
+            if (first_positive_radial_turn(Radial{f, k1, k2}, &turn) && q >= turn) return false;
+            if (!dj(d) || d[3] <= 1e-12) return false;
+        }
+        const double q = x * x + y * y;
+        const double rf = 1.0 + k1 * q + k2 * q * q;
+        pix[0] = std::fma(f, x * rf, in.cx);
+        pix[1] = std::fma(f, y * rf, in.cy);
+        if (!dj(d)) return false;
+        J[0][0] = f * d[0] * iz;
+        J[0][1] = f * d[1] * iz;
+        J[0][2] = -f * (d[0] * x + d[1] * y) * iz;
+        J[1][0] = f * d[1] * iz;
+        J[1][1] = f * d[2] * iz;
+        J[1][2] = -f * (d[1] * x + d[2] * y) * iz;
+    }
+    if (!(std::isfinite(pix[0]) && std::isfinite(pix[1]))) return false;
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 3; ++c)
+            if (!std::isfinite(J[r][c])) return false;
+    return true;
+}
+
+// Solution::weak_points (solver.rs:1149-1219)
+static std::vector<uint8_t> solution_weak_points(const ba_problem* p, const SolutionO& s) {
+    const size_t N = p->n_points;
+    Loss loss{L2, 0.0};
+    if (p->robust_loss == BA_LOSS_HUBER) loss = Loss{HUBER, p->loss_scale};
+    if (p->robust_loss == BA_LOSS_CAUCHY) loss = Loss{CAUCHY, p->loss_scale / std::sqrt(2.0)};
+    std::vector<std::array<double, 9>> info(N);
+    for (auto& b : info) b.fill(0.0);
+    std::vector<uint8_t> missing(N, 0);
+    for (size_t o = 0; o < p->n_observations; ++o) {
+        const size_t q = p->obs_point[o], i = p->obs_pose[o], g = p->obs_intrinsic[o];
+        if (missing[q]) continue;
+        const double* R = &s.rot[9 * i];
+        const double* C = &s.cen[3 * i];
+        double pc[3];
+        for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; ++k) acc += R[3 * r + k] * (s.xyz[3 * q + k] - C[k]);
+            pc[r] = acc;
+        }
+        double pix[2], J[2][3];
+        if (!camera_project(p->intrinsics[g], s.f[g], s.k1[g], s.k2[g], pc, pix, J)) {
+            missing[q] = 1;
+            continue;
+        }
+        const double res = std::hypot(pix[0] - p->obs_pixel[2 * o], pix[1] - p->obs_pixel[2 * o + 1]);
+        if (!std::isfinite(res)) {
+            missing[q] = 1;
+            continue;
+        }
+        double A[2][3];
+        for (int r = 0; r < 2; ++r)
+            for (int c = 0; c < 3; ++c) A[r][c] = J[r][0] * R[c] + J[r][1] * R[3 + c] + J[r][2] * R[6 + c];
+        const double w = loss_sqrt_weight(loss, res);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) info[q][3 * a + b] += w * w * (A[0][a] * A[0][b] + A[1][a] * A[1][b]);
+    }
+    // TrackGeometry at the solution (quarantine.rs:57-96)
+    std::vector<double> offs(3 * p->n_observations), dist(p->n_observations);
+    for (size_t o = 0; o < p->n_observations; ++o) {
+        for (int k = 0; k < 3; ++k)
+            offs[3 * o + k] = s.xyz[3 * p->obs_point[o] + k] - s.cen[3 * p->obs_pose[o] + k];
+        dist[o] = std::sqrt(offs[3 * o] * offs[3 * o] + offs[3 * o + 1] * offs[3 * o + 1] + offs[3 * o + 2] * offs[3 * o + 2]);
+    }
+    double scene = 0.0;
+    if (!dist.empty()) {
+        std::nth_element(dist.begin(), dist.begin() + dist.size() / 2, dist.end());
+        scene = dist[dist.size() / 2];
+    }
+    std::vector<TrackGeometry> geom(N, TrackGeometry(scene));
+    for (size_t o = 0; o < p->n_observations; ++o) geom[p->obs_point[o]].push(&offs[3 * o]);
+    std::vector<uint8_t> weak(N, 0);
+    for (size_t i = 0; i < N; ++i) {
+        if (s.quarantined.size() == N && s.quarantined[i]) {
+            weak[i] = 1;
+            continue;
+        }
+        if (point_is_fixed(p, i)) continue;
+        if (geom[i].degenerate() || missing[i]) {
+            weak[i] = 1;
+            continue;
+        }
+        double a[3][3], ev[3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) a[r][c] = info[i][3 * r + c];
+        sym3_eigenvalues(a, ev);
+        const double hi = fmax_(fmax_(ev[0], ev[1]), ev[2]), lo = fmin_(fmin_(ev[0], ev[1]), ev[2]);
+        weak[i] = !(lo > 1e-5 * hi);
+    }
+    return weak;
+}
+
+// The exact certificate and marginals at the final state: S of the undamped final assembly,
+// factored with every pivot positive (a sparse LLT without regularisation), and block_marginal
+// (covariance.rs:562-587) of every free group by solves with unit right-hand sides.
+static void certify_view(const View& v, const uint32_t* cross_off, bool exact, bool want, int32_t* obs,
+                         bool* marginals, std::vector<double>& cov) {
+    *obs = BA_OBSERVABILITY_NOT_ASSESSED;
+    *marginals = false;
+    cov.assign(9 * v.ng, NaN);
+    const size_t D = v.D;
+    if (!exact && !want) return;
+    if (D == 0) {
+        if (exact) *obs = BA_OBSERVABILITY_FULL_RANK;
+        *marginals = want;
+        return;
+    }
+    Assembly a = assemble(v, 0.0, true, cross_off);
+    bool ok = a.valid && a.H.finite();
+    std::vector<double> F;
+    if (ok) {  // LLT of the lower triangle: every pivot positive
+        F = a.H.full;
+        for (size_t c = 0; c < D && ok; ++c) {
+            double d = F[c * D + c];
+            for (size_t k = 0; k < c; ++k) d -= F[c * D + k] * F[c * D + k];
+            if (!(std::isfinite(d) && d > 0.0)) {
+                ok = false;
+                break;
+            }
+            const double l = std::sqrt(d);
+            F[c * D + c] = l;
+#pragma omp parallel for schedule(static) if (D - c > 256)
+            for (size_t r = c + 1; r < D; ++r) {
+                double x = F[r * D + c];
+                for (size_t k = 0; k < c; ++k) x -= F[r * D + k] * F[c * D + k];
+                F[r * D + c] = x / l;
+            }
+        }
+    }
+    if (exact) *obs = ok ? BA_OBSERVABILITY_FULL_RANK : BA_OBSERVABILITY_RANK_DEFICIENT;
+    if (!ok || !want) return;
+    const double w0 = loss_sqrt_weight(v.loss, 0.0);
+    const double wn = w0 * w0;
+    std::vector<double> x(D);
+    bool finite = true;
+    for (size_t g = 0; g < v.ng; ++g) {
+        const size_t d = v.grp_dim[g], off = v.grp_off[g];
+        if (!d) continue;
+        double m[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (size_t col = 0; col < d; ++col) {
+            std::fill(x.begin(), x.end(), 0.0);
+            x[off + col] = 1.0;
+            for (size_t r = 0; r < D; ++r) {  // L y = e
+                double v2 = x[r];
+                for (size_t k = 0; k < r; ++k) v2 -= F[r * D + k] * x[k];
+                x[r] = v2 / F[r * D + r];
+            }
+            for (size_t r = D; r-- > 0;) {  // L^T x = y
+                double v2 = x[r];
+                for (size_t k = r + 1; k < D; ++k) v2 -= F[k * D + r] * x[k];
+                x[r] = v2 / F[r * D + r];
+            }
+            for (size_t row = 0; row < d; ++row) m[row][col] = x[off + row] * wn;
+        }
+        for (size_t r = 0; r < d; ++r)  // symmetrize
+            for (size_t c = r + 1; c < d; ++c) m[r][c] = m[c][r] = 0.5 * (m[r][c] + m[c][r]);
+        for (size_t r = 0; r < 3; ++r)
+            for (size_t c = 0; c < 3; ++c) {
+                cov[9 * g + 3 * r + c] = m[r][c];
+                finite = finite && std::isfinite(m[r][c]);
+            }
+    }
+    if (!finite) {
+        std::fill(cov.begin(), cov.end(), NaN);
+        return;
+    }
+    *marginals = true;
+}
+
+static void certify_state(const Lowered& L, const State& st, double scene_scale, bool exact, bool want, int32_t* obs,
+                          bool* marginals, std::vector<double>& cov) {
+    View v = make_view(L, st, L.loss, L.gauge_camera, scene_scale);
+    certify_view(v, L.cross_off.data(), exact, want, obs, marginals, cov);
+}
+
+// unconstrained_intrinsics (native/self_calibration.rs:255-303)
+static std::vector<uint8_t> unconstrained(const ba_problem* p, const Lowered& L, const State& st, const double* cov) {
+    std::vector<uint8_t> out(p->n_intrinsics, 0);
+    for (size_t g = 0; g < L.grp_dim.size(); ++g) {
+        const size_t d = L.grp_dim[g];
+        if (!d) continue;
+        if (!cov) {
+            out[g] = 1;
+            continue;
+        }
+        const ba_intrinsic& in = p->intrinsics[g];
+        const double f = st.gp[3 * g];
+        const double ax = fmax_(in.cx, (double)in.width - in.cx) / f, ay = fmax_(in.cy, (double)in.height - in.cy) / f;
+        const double rr = ax * ax + ay * ay;
+        const double scale[3] = {1.0, rr, rr * rr};
+        for (size_t k = 0; k < d; ++k) {
+            const double dev = std::sqrt(cov[9 * g + 3 * k + k]) * scale[k];
+            if (std::isnan(dev) || (!in.certification_waived && dev > 0.01)) out[g] = 1;
+        }
+    }
+    return out;
+}
+
+int bao_weak_points(const ba_problem* p, const ba_solution* s, uint8_t* weak) {
+    SolutionO so;
+    so.init(p);
+    so.rot.assign(s->pose_rotation, s->pose_rotation + 9 * p->n_poses);
+    so.cen.assign(s->pose_center, s->pose_center + 3 * p->n_poses);
+    so.f.assign(s->intrinsic_focal, s->intrinsic_focal + p->n_intrinsics);
+    so.k1.assign(s->intrinsic_k1, s->intrinsic_k1 + p->n_intrinsics);
+    so.k2.assign(s->intrinsic_k2, s->intrinsic_k2 + p->n_intrinsics);
+    so.xyz.assign(s->point_xyz, s->point_xyz + 3 * p->n_points);
+    if (s->point_quarantined) so.quarantined.assign(s->point_quarantined, s->point_quarantined + p->n_points);
+    const std::vector<uint8_t> w = solution_weak_points(p, so);
+    std::memcpy(weak, w.data(), w.size());
+    return BA_OK;
+}
+
+static bool direct_fits_o(uint64_t est, uint64_t limit) { return est <= limit / 5 * 4 + limit % 5 * 4 / 5; }
+
+// The limits of the product's dense certificate (BA_CERTIFY_MAX_DIMENSION, BA_CERTIFY_MAX_TERMS),
+// restated so that the oracle certifies exactly the solves the product does; beyond them nothing
+// is certified and nothing is held.
+static bool within_limits(const Lowered& L) {
+    if (L.D > BA_CERTIFY_MAX_DIMENSION) return false;
+    uint64_t terms = 0;
+    for (size_t q = 0; q + 1 < L.off.size(); ++q) {
+        const uint64_t k = 2ull * (L.off[q + 1] - L.off[q]);
+        terms += L.point_fixed[q] ? k : k * k;
+    }
+    return terms <= BA_CERTIFY_MAX_TERMS;
+}
+
+// solve_once (native/mod.rs:318-335) with solve_self_calibrating (native/self_calibration.rs:223-249)
+// and canonicalize_roundoff_non_descent.
+static Err solve_once_o(const ba_problem* p, const ba_solve_config* cfg, ba_iteration_trace* trace, bool quarantine,
+                        SolutionO& out) {
+    const bool full = cfg->certification == BA_CERTIFY_FULL;
+    Lowered L;
+    Err e = lower(p, L, quarantine);
+    if (e.code) return e;
+    ba_execution_plan plan;
+    plan_solver(L, cfg, plan);
+    ba_solve_config planned = *cfg;
+    planned.linear_solver = plan.linear_solver;
+    const uint32_t trace_start = trace ? trace->count : 0;
+    State st{L.R, L.t, L.gp, L.X};
+    ba_solve_report r;
+    e = refine(L, &planned, st, r, trace);
+    if (e.code) return e;
+    r.plan_reason = plan.reason;
+    r.observability = BA_OBSERVABILITY_NOT_ASSESSED;
+    out.init(p);
+    std::vector<uint8_t> held(p->n_intrinsics, 0);
+    size_t nheld = 0;
+    auto certify = [&](const Lowered& LL, const ba_execution_plan& pl, const State& s, ba_solve_report& rr,
+                       bool want, bool* mok, std::vector<double>& cv) {
+        // exact certificate unless Auto chose PCG and retained + certificate bytes exceed 4/5 of the budget
+        const bool auto_pcg = cfg->linear_solver == BA_SOLVER_AUTO && pl.linear_solver == BA_SOLVER_ITERATIVE;
+        bool exact = true;
+        if (auto_pcg) {
+            const uint64_t nnz = pl.reduced_nonzeros ? pl.reduced_nonzeros : (uint64_t)LL.row_idx.size();
+            exact = direct_fits_o(pl.estimated_retained_bytes + nnz * 32, pl.budget_bytes);
+        }
+        int32_t obs;
+        certify_state(LL, s, rr.normalization_scale, exact, want && !auto_pcg, &obs, mok, cv);
+        rr.observability = obs;
+        if (obs == BA_OBSERVABILITY_FULL_RANK)
+            rr.pose_rank = rr.pose_dimension = (uint32_t)LL.D;
+        else if (obs == BA_OBSERVABILITY_RANK_DEFICIENT) {
+            rr.pose_rank = 0;
+            rr.pose_dimension = (uint32_t)LL.D;
+        }
+        if (rr.pose_dimension == 0) rr.pose_rank = 0;
+        if (obs == BA_OBSERVABILITY_RANK_DEFICIENT || rr.weak_point_count > 0) rr.status = BA_STATUS_DEGENERATE;
+        return auto_pcg;
+    };
+    if (full && within_limits(L)) {
+        bool mok = false;
+        std::vector<double> cv;
+        const bool auto_pcg = certify(L, plan, st, r, true, &mok, cv);
+        out.cov = cv;
+        held = unconstrained(p, L, st, mok && !auto_pcg ? cv.data() : nullptr);
+        for (auto h : held) nheld += h;
+    }
+    if (nheld == 0) {
+        ba_solution v = out.view();
+        raise_solution(p, L, st, &v);
+        inlier_metrics(p, L, &v, &r.inlier_count, &r.final_inlier_rmse);
+        out.rep = r;
+    } else {
+        Lowered L2;
+        e = lower(p, L2, quarantine, &held);
+        if (e.code) return e;
+        ba_execution_plan plan2;
+        plan_solver(L2, cfg, plan2);
+        ba_solve_config planned2 = *cfg;
+        planned2.linear_solver = plan2.linear_solver;
+        if (trace) trace->count = trace_start;
+        State st2{L2.R, L2.t, L2.gp, L2.X};
+        ba_solve_report r2;
+        e = refine(L2, &planned2, st2, r2, trace);
+        if (e.code) return e;
+        r2.plan_reason = plan2.reason;
+        bool mok2;
+        std::vector<double> cv2;
+        if (within_limits(L2)) certify(L2, plan2, st2, r2, false, &mok2, cv2);
+        ba_solution v = out.view();
+        raise_solution(p, L2, st2, &v);
+        inlier_metrics(p, L2, &v, &r2.inlier_count, &r2.final_inlier_rmse);
+        out.rep = r2;
+        L = std::move(L2);
+    }
+    for (size_t g = 0; g < held.size(); ++g) out.unconstrained[g] = held[g];
+    out.rep.unconstrained_intrinsics = (uint32_t)nheld;
+    if (!full) return Err{};
+    // canonicalize_roundoff_non_descent (solver.rs:1030-1101)
+    ba_solve_report& rr = out.rep;
+    if (p->gauge == BA_GAUGE_FIXED_BASELINE) {
+        const Vec3 a = pose_C(p, p->gauge_first), b = pose_C(p, p->gauge_second);
+        const double initial = std::sqrt(dist2(a, b));
+        if (!std::isfinite(initial) ||
+            std::fabs(initial - p->gauge_distance) > 512.0 * EPS * fmax_(initial, p->gauge_distance))
+            return Err{};
+    }
+    const double increase = rr.final_cost - rr.initial_cost;
+    const double tol = 2.0 * std::sqrt(EPS) * fmax_(std::fabs(rr.initial_cost), 1.0);
+    const bool conv = rr.status <= BA_STATUS_CONVERGED_COST;
+    if (!(increase > 0.0 && increase <= tol && conv && rr.observability != BA_OBSERVABILITY_RANK_DEFICIENT))
+        return Err{};
+    SolutionO input;
+    input.init(p);
+    input.rot.assign(p->pose_rotation, p->pose_rotation + 9 * p->n_poses);
+    input.cen.assign(p->pose_center, p->pose_center + 3 * p->n_poses);
+    for (size_t g = 0; g < p->n_intrinsics; ++g) {
+        const ba_intrinsic& in = p->intrinsics[g];
+        input.f[g] = in.fx;
+        input.k1[g] = (in.model == BA_CAMERA_BROWN_K1 || in.model == BA_CAMERA_BROWN_K1K2 || in.model == BA_CAMERA_BROWN_K1K2K3) ? in.k1 : 0.0;
+        input.k2[g] = (in.model == BA_CAMERA_BROWN_K1K2 || in.model == BA_CAMERA_BROWN_K1K2K3) ? in.k2 : 0.0;
+    }
+    input.xyz.assign(p->point_xyz, p->point_xyz + 3 * p->n_points);
+    const bool same = input.rot == out.rot && input.cen == out.cen && input.f == out.f && input.k1 == out.k1 &&
+                      input.k2 == out.k2 && input.xyz == out.xyz;
+    if (!same) {
+        out.rot = input.rot;
+        out.cen = input.cen;
+        out.f = input.f;
+        out.k1 = input.k1;
+        out.k2 = input.k2;
+        out.xyz = input.xyz;
+        rr.observability = BA_OBSERVABILITY_NOT_ASSESSED;
+        rr.pose_rank = rr.pose_dimension = 0;
+        rr.weak_point_count = 0;
+    }
+    rr.final_cost = rr.initial_cost;
+    rr.final_rmse = rr.initial_rmse;
+    rr.status = BA_STATUS_SKIPPED;
+    ba_solution v = input.view();
+    inlier_metrics(p, L, &v, &rr.inlier_count, &rr.final_inlier_rmse);
+    return Err{};
+}
+
+static void continued_from(ba_solve_report& later, const ba_solve_report& first) {  // solver.rs:527-538
+    later.initial_cost = first.initial_cost;
+    later.initial_rmse = first.initial_rmse;
+    later.iterations += first.iterations;
+    later.accepted_steps += first.accepted_steps;
+    later.rejected_steps += first.rejected_steps;
+    later.rejected_linearization += first.rejected_linearization;
+    later.rejected_step_solve += first.rejected_step_solve;
+    later.rejected_candidate += first.rejected_candidate;
+    later.rejected_insufficient_reduction += first.rejected_insufficient_reduction;
+    later.pcg_iterations += first.pcg_iterations;
+    later.weak_point_resolves += first.weak_point_resolves;
+}
+
+// solve_with_quarantine (native/mod.rs:252-316)
+static Err solve_with_quarantine_o(const ba_problem* p, const ba_solve_config* cfg, ba_iteration_trace* trace,
+                                   SolutionO& result) {
+    Lowered L0;
+    Err e = lower(p, L0);  // the input pass and every lowering check
+    if (e.code) return e;
+    const size_t N = p->n_points;
+    size_t quarantined = 0;
+    for (auto q : L0.quarantined) quarantined += q;
+    OwnedProblemO working_own;
+    const ba_problem* working = p;
+    std::vector<size_t> kept(N);
+    for (size_t i = 0; i < N; ++i) kept[i] = i;
+    if (quarantined) {
+        restrict_problem(p, nullptr, L0.quarantined, working_own, kept);
+        working = &working_own.p;
+    }
+    const bool changes_scale = p->gauge == BA_GAUGE_FIXED_BASELINE && std::fabs(L0.rebase.scale - 1.0) > 1e-9;
+    SolutionO sol;
+    e = solve_once_o(working, cfg, trace, false, sol);
+    if (e.code) return e;
+    ba_solve_config retry = *cfg;
+    retry.max_iterations = cfg->max_iterations < 8 ? cfg->max_iterations : 8;  // RESOLVE_ITERATIONS
+    OwnedProblemO owned[2];
+    const ba_problem* current = working;
+    bool have_origin = false;
+    ba_solve_report origin{};
+    uint32_t rounds = 0;
+    for (int round = 0; round < 4; ++round) {  // MAX_RESOLVES
+        const ba_solve_report& rr = sol.rep;
+        const bool needs = rr.weak_point_count > 0 || rr.observability == BA_OBSERVABILITY_RANK_DEFICIENT ||
+                           rr.status == BA_STATUS_DEGENERATE;
+        if (!needs) break;
+        std::vector<uint8_t> weak = solution_weak_points(current, sol);
+        size_t n = 0;
+        for (size_t i = 0; i < weak.size(); ++i) {
+            if (point_is_fixed(current, i)) weak[i] = 0;
+            n += weak[i];
+        }
+        if (!n) break;
+        OwnedProblemO& next = owned[round % 2];
+        std::vector<size_t> next_kept;
+        restrict_problem(current, &sol, weak, next, next_kept);
+        SolutionO resolved;
+        if (solve_once_o(&next.p, &retry, trace, false, resolved).code) break;
+        std::vector<size_t> composed(next_kept.size());
+        for (size_t i = 0; i < next_kept.size(); ++i) composed[i] = kept[next_kept[i]];
+        kept.swap(composed);
+        quarantined += n;
+        if (have_origin) continued_from(sol.rep, origin);
+        origin = sol.rep;
+        have_origin = true;
+        sol = std::move(resolved);
+        current = &next.p;
+        ++rounds;
+    }
+    if (have_origin) continued_from(sol.rep, origin);
+    sol.rep.weak_point_resolves = rounds;
+    if (changes_scale && quarantined > 0)
+        return fail(BA_ERR_UNSUPPORTED, "native BA cannot change its physical baseline while holding out points");
+    if (kept.size() == N) {
+        result = std::move(sol);
+        result.rep.quarantined_points = 0;
+        return Err{};
+    }
+    result.init(p);  // Solution::lifted (solver.rs:891-935)
+    result.rot = sol.rot;
+    result.cen = sol.cen;
+    result.f = sol.f;
+    result.k1 = sol.k1;
+    result.k2 = sol.k2;
+    result.cov = sol.cov;
+    result.unconstrained = sol.unconstrained;
+    result.rep = sol.rep;
+    result.xyz.assign(p->point_xyz, p->point_xyz + 3 * N);
+    std::fill(result.quarantined.begin(), result.quarantined.end(), 1);
+    for (size_t i = 0; i < kept.size(); ++i) {
+        for (int k = 0; k < 3; ++k) result.xyz[3 * kept[i] + k] = sol.xyz[3 * i + k];
+        result.quarantined[kept[i]] = 0;
+    }
+    result.rep.quarantined_points = (uint32_t)quarantined;
+    result.rep.point_count = N;
+    return Err{};
+}
+
+static void copy_solution(const SolutionO& s, ba_solution* out) {
+    auto cp = [](void* dst, const void* src, size_t bytes) {
+        if (dst && bytes) std::memcpy(dst, src, bytes);
+    };
+    cp(out->pose_rotation, s.rot.data(), s.rot.size() * 8);
+    cp(out->pose_center, s.cen.data(), s.cen.size() * 8);
+    cp(out->intrinsic_focal, s.f.data(), s.f.size() * 8);
+    cp(out->intrinsic_k1, s.k1.data(), s.k1.size() * 8);
+    cp(out->intrinsic_k2, s.k2.data(), s.k2.size() * 8);
+    cp(out->point_xyz, s.xyz.data(), s.xyz.size() * 8);
+    cp(out->point_quarantined, s.quarantined.data(), s.quarantined.size());
+    cp(out->intrinsic_unconstrained, s.unconstrained.data(), s.unconstrained.size());
+    cp(out->intrinsic_covariance, s.cov.data(), s.cov.size() * 8);
+}
+
 int bao_prepare(const ba_problem* p, ba_native_sizes* sizes, ba_native_problem* n, char* msg, size_t len) {
     Err e = validate(p);
     if (e.code) {
@@ -2612,31 +3252,24 @@ int bao_solve(const ba_problem* p, const ba_solve_config* cfg, ba_solution* sol,
         copy_msg(e, msg, len);
         return e.code;
     }
-    Lowered L;
-    e = lower(p, L);
-    if (e.code) {
-        copy_msg(e, msg, len);
-        return e.code;
-    }
     if (trace) trace->count = 0;
-    State st{L.R, L.t, L.gp, L.X};
-    ba_execution_plan plan;
-    plan_solver(L, cfg, plan);
-    ba_solve_config planned = *cfg;
-    planned.linear_solver = plan.linear_solver;
-    ba_solve_report r;
-    e = refine(L, &planned, st, r, trace);
+    SolutionO out;
+    if (cfg->certification == BA_CERTIFY_FULL) {
+        e = solve_with_quarantine_o(p, cfg, trace, out);
+    } else {  // the LM loop alone, input quarantine in the lowering
+        e = solve_once_o(p, cfg, trace, true, out);
+        if (!e.code) {
+            size_t nq = 0;
+            for (auto q : out.quarantined) nq += q;
+            out.rep.quarantined_points = (uint32_t)nq;
+        }
+    }
     if (e.code) {
         copy_msg(e, msg, len);
         return e.code;
     }
-    size_t nq = 0;
-    for (auto q : L.quarantined) nq += q;
-    r.quarantined_points = (uint32_t)nq;
-    r.plan_reason = plan.reason;
-    raise_solution(p, L, st, sol);
-    inlier_metrics(p, L, sol, &r.inlier_count, &r.final_inlier_rmse);
-    if (rep) *rep = r;
+    copy_solution(out, sol);
+    if (rep) *rep = out.rep;
     return BA_OK;
 }
 
@@ -2681,35 +3314,50 @@ int bao_linearize(const ba_device_problem* dp, double* residual, double* pj, dou
     return BA_OK;
 }
 
+// Cross offsets of a view in the order of radial_point_parameters (pattern.rs:306-379).
+static std::vector<uint32_t> cross_offsets(const View& v) {
+    std::vector<uint32_t> cross_off(v.np + 1, 0);
+    std::vector<uint8_t> marks(v.D, 0);
+    std::vector<uint32_t> act;
+    for (size_t q = 0; q < v.np; ++q) {
+        act.clear();
+        if (!(v.point_fixed && v.point_fixed[q]))
+            for (uint32_t o = v.off[q]; o < v.off[q + 1]; ++o) {
+                uint32_t c = v.obs_cam[o], g = v.cam_group[c];
+                for (uint32_t k = 0; k < v.cam_dim[c]; ++k)
+                    if (!marks[v.cam_off[c] + k]) {
+                        marks[v.cam_off[c] + k] = 1;
+                        act.push_back(v.cam_off[c] + k);
+                    }
+                for (uint32_t k = 0; k < v.grp_dim[g]; ++k)
+                    if (!marks[v.grp_off[g] + k]) {
+                        marks[v.grp_off[g] + k] = 1;
+                        act.push_back(v.grp_off[g] + k);
+                    }
+            }
+        for (auto pr : act) marks[pr] = 0;
+        cross_off[q + 1] = cross_off[q] + (uint32_t)act.size();
+    }
+    return cross_off;
+}
+
+int bao_certify(const ba_device_problem* dp, int32_t* observability, double* covariance) {
+    View v = view_from(dp);
+    const std::vector<uint32_t> cross_off = cross_offsets(v);
+    int32_t obs;
+    bool ok;
+    std::vector<double> cov;
+    certify_view(v, cross_off.data(), true, true, &obs, &ok, cov);
+    if (observability) *observability = obs;
+    if (covariance) std::memcpy(covariance, cov.data(), cov.size() * sizeof(double));
+    return ok ? BA_OK : BA_ERR_NUMERICAL;
+}
+
 int bao_assemble(const ba_device_problem* dp, double lambda, double* schur, double* gradient, double* full_gradient,
                  double* diagonal, double* point_inverse, double* inverse_gradient, double* inverse_cross,
                  ba_schur_scalars* sc) {
     View v = view_from(dp);
-    // cross offsets in the order of radial_point_parameters
-    std::vector<uint32_t> cross_off(v.np + 1, 0);
-    {
-        std::vector<uint8_t> marks(v.D, 0);
-        std::vector<uint32_t> act;
-        for (size_t q = 0; q < v.np; ++q) {
-            act.clear();
-            if (!(v.point_fixed && v.point_fixed[q]))
-                for (uint32_t o = v.off[q]; o < v.off[q + 1]; ++o) {
-                    uint32_t c = v.obs_cam[o], g = v.cam_group[c];
-                    for (uint32_t k = 0; k < v.cam_dim[c]; ++k)
-                        if (!marks[v.cam_off[c] + k]) {
-                            marks[v.cam_off[c] + k] = 1;
-                            act.push_back(v.cam_off[c] + k);
-                        }
-                    for (uint32_t k = 0; k < v.grp_dim[g]; ++k)
-                        if (!marks[v.grp_off[g] + k]) {
-                            marks[v.grp_off[g] + k] = 1;
-                            act.push_back(v.grp_off[g] + k);
-                        }
-                }
-            for (auto pr : act) marks[pr] = 0;
-            cross_off[q + 1] = cross_off[q] + (uint32_t)act.size();
-        }
-    }
+    const std::vector<uint32_t> cross_off = cross_offsets(v);
     Assembly a = assemble(v, lambda, true, cross_off.data());
     size_t D = v.D;
     for (size_t r = 0; r < D; ++r)

@@ -54,6 +54,16 @@ int bao_solve(const ba_problem* problem, const ba_solve_config* config, ba_solut
 int bao_plan(const ba_problem* problem, const ba_solve_config* config, ba_execution_plan* plan,
              char* message, size_t message_len);
 
+/* The certificate of a native state at zero damping: the rank verdict (enum ba_observability)
+ * of the undamped reduced system and the weight-normalised marginal covariance of every free
+ * intrinsic group (9 per group, NaN for fixed groups); BA_ERR_NUMERICAL when the marginals cannot
+ * be formed (sparse/self_calibration/mod.rs:110-209, direct.rs:288-334). */
+int bao_certify(const ba_device_problem* problem, int32_t* observability, double* covariance);
+
+/* Solution::weak_points (solver.rs:1149-1219) of a solution of `problem`: weak[i] = 1 for each
+ * weak point. */
+int bao_weak_points(const ba_problem* problem, const ba_solution* solution, uint8_t* weak);
+
 /* Lowering and index maps, same contract as ba_hip_prepare. */
 int bao_prepare(const ba_problem* problem, ba_native_sizes* sizes, ba_native_problem* native,
                 char* message, size_t message_len);

@@ -129,7 +129,7 @@ def scale_fixtures() -> None:
     out = {}
     # medium: direct step, 10 iterations
     scene = synthetic.medium_scene()
-    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT)
+    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT, certification=A.CERTIFY_LOOP_ONLY)
     t0 = time.time()
     sol = solve_with(oracle, "bao_solve", scene.problem, cfg)
     obj = dict(scene="synthetic.medium_scene()", scene_sha256=scene_digest(scene.problem),
@@ -140,7 +140,7 @@ def scale_fixtures() -> None:
                intrinsic_focal=floats(sol.intrinsic_focal), intrinsic_k1=floats(sol.intrinsic_k1),
                intrinsic_k2=floats(sol.intrinsic_k2), point_stride=97, point_xyz=floats(sol.point_xyz[::97]))
     t0 = time.time()
-    conv = solve_with(oracle, "bao_solve", scene.problem, SolveConfig(max_iterations=50, linear_solver=A.SOLVER_DIRECT))
+    conv = solve_with(oracle, "bao_solve", scene.problem, SolveConfig(max_iterations=50, linear_solver=A.SOLVER_DIRECT, certification=A.CERTIFY_LOOP_ONLY))
     obj["converged"] = dict(config=dict(max_iterations=50, linear_solver="direct"), wall_seconds=time.time() - t0,
                             **trajectory(conv), pose_center=floats(conv.pose_center),
                             intrinsic_focal=floats(conv.intrinsic_focal), intrinsic_k1=floats(conv.intrinsic_k1),
@@ -154,10 +154,10 @@ def scale_fixtures() -> None:
                 made_by="tests/golden/make_golden.py --scale", source="oracle (C++ FP64 restatement)",
                 omp_threads=threads)
     t0 = time.time()
-    five = solve_with(oracle, "bao_solve", scene.problem, SolveConfig(max_iterations=5, linear_solver=A.SOLVER_ITERATIVE))
+    five = solve_with(oracle, "bao_solve", scene.problem, SolveConfig(max_iterations=5, linear_solver=A.SOLVER_ITERATIVE, certification=A.CERTIFY_LOOP_ONLY))
     t1 = time.time()
     print("large 5", five.report.final_cost, t1 - t0, flush=True)
-    conv = solve_with(oracle, "bao_solve", scene.problem, SolveConfig(max_iterations=60, linear_solver=A.SOLVER_ITERATIVE))
+    conv = solve_with(oracle, "bao_solve", scene.problem, SolveConfig(max_iterations=60, linear_solver=A.SOLVER_ITERATIVE, certification=A.CERTIFY_LOOP_ONLY))
     t2 = time.time()
     print("large conv", conv.report.final_cost, conv.report.iterations, t2 - t1, flush=True)
     out["large_pcg"] = dict(head, first=dict(config=dict(max_iterations=5, linear_solver="iterative"),
@@ -170,7 +170,7 @@ def scale_fixtures() -> None:
     pick = list(np.argsort(-sizes, kind="stable")[:4])
     rng = np.random.default_rng(0xBA0004)
     pick += [int(i) for i in rng.choice(np.setdiff1d(np.arange(len(wins)), pick), 4, replace=False)]
-    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT)
+    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT, certification=A.CERTIFY_LOOP_ONLY)
     wl = []
     for i in pick:
         sol = solve_with(oracle, "bao_solve", wins[i].problem, cfg)

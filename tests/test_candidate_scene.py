@@ -1,7 +1,9 @@
 """The candidate-rule scene (synthetic.candidate_rule_scene) drives the oracle through the held and
 reject-step branches of crates/openmvg-ba/src/kernel/point_geometry.rs:153-168 and
 sparse/self_calibration/candidate.rs:25-233, for the direct and the PCG step, squared and Huber
-loss.  The GPU suite (tests/test_gpu_candidate.py) compares the HIP path on the same scene."""
+loss, in the LM loop alone (the post-loop certification then holds both added points out: see
+tests/test_certify.py).  The GPU suite (tests/test_gpu_candidate.py) compares the HIP path on the
+same scene."""
 
 import numpy as np
 import pytest
@@ -15,7 +17,8 @@ from openmvg_ba.problem import solve_with
 def test_scene_rejects_steps_and_holds_a_point(oracle, solver, loss):
     prob, q_near, _ = synthetic.candidate_rule_scene()
     prob.robust_loss, prob.loss_scale = loss, (1.0 if loss == A.LOSS_HUBER else 0.0)
-    sol = solve_with(oracle, "bao_solve", prob, SolveConfig(max_iterations=30, linear_solver=solver))
+    cfg = SolveConfig(max_iterations=30, linear_solver=solver, certification=A.CERTIFY_LOOP_ONLY)
+    sol = solve_with(oracle, "bao_solve", prob, cfg)
     rep = sol.report
     assert rep.rejected_candidate >= 2  # the candidate rule rejected steps
     assert rep.accepted_steps >= 8

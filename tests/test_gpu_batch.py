@@ -38,7 +38,7 @@ def windows():
     return synthetic.window_scenes(0xBA0004, 24, cameras=(5, 40), points=(200, 1500))
 
 
-CFG = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT)
+CFG = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT, certification=A.CERTIFY_LOOP_ONLY)
 
 
 def _same(a, b, tol_cost=1e-10, tol_param=1e-8):

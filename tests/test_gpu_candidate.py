@@ -37,7 +37,7 @@ def oracle():
 def test_candidate_rule_branches_match_oracle(adjuster, oracle, solver, loss):
     prob, q_near, _ = synthetic.candidate_rule_scene()
     prob.robust_loss, prob.loss_scale = loss, (1.0 if loss == A.LOSS_HUBER else 0.0)
-    cfg = SolveConfig(max_iterations=30, linear_solver=solver)
+    cfg = SolveConfig(max_iterations=30, linear_solver=solver, certification=A.CERTIFY_LOOP_ONLY)
     want = solve_with(oracle, "bao_solve", prob, cfg)
     got = adjuster.solve(prob, cfg)
     assert want.report.rejected_candidate >= 2
@@ -72,4 +72,6 @@ def test_cauchy_stages_match_oracle(adjuster, oracle, solver):
     want = solve_with(oracle, "bao_solve", prob, cfg)
     got = adjuster.solve(prob, cfg)
     assert want.report.iterations > 5  # the Cauchy stage ran
-    _compare_solutions(got, want)
+    # The inexact PCG steps leave the two final states ~1e-9 apart, and the marginals move ~1e3
+    # times as much through the Cauchy weights of the outliers (measured 4e-6 of the largest entry).
+    _compare_solutions(got, want, tol_cov=1e-8 if solver == A.SOLVER_DIRECT else 2e-5)

@@ -58,7 +58,7 @@ def _last(ba):
 def test_mode_is_validated_and_reported(adjusters, ring):
     ex, im = adjusters
     assert ex.handle.lib.ba_hip_set_schur_product(ex.handle.ptr, 3) == A.BA_ERR_INVALID_CONFIG
-    cfg = SolveConfig(max_iterations=2, linear_solver=A.SOLVER_ITERATIVE)
+    cfg = SolveConfig(max_iterations=2, linear_solver=A.SOLVER_ITERATIVE, certification=A.CERTIFY_LOOP_ONLY)
     ex.solve(ring[0].problem, cfg)
     im.solve(ring[0].problem, cfg)
     assert (_last(ex), _last(im)) == (A.SCHUR_EXPLICIT, A.SCHUR_IMPLICIT)
@@ -101,7 +101,7 @@ def test_ring_iterative_both_forms_match_oracle(adjusters, oracle, ring, loss, s
     prob = sc.problem
     prob.robust_loss, prob.loss_scale = loss, scale
     try:
-        cfg = SolveConfig(max_iterations=50, linear_solver=A.SOLVER_ITERATIVE)
+        cfg = SolveConfig(max_iterations=50, linear_solver=A.SOLVER_ITERATIVE, certification=A.CERTIFY_LOOP_ONLY)
         want = solve_with(oracle, "bao_solve", prob, cfg)
         for ba in adjusters:
             _compare_solutions(ba.solve(prob, cfg), want)
@@ -119,7 +119,7 @@ def test_fixed_points_and_poses(adjusters, oracle):
     var = np.ones(len(prob.pose_center), dtype=np.uint8)
     var[3] = 0  # fixed world points need a fixed observed pose (native/self_calibration.rs:47-85)
     prob.pose_var_rotation, prob.pose_var_translation = var, var.copy()
-    cfg = SolveConfig(max_iterations=30, linear_solver=A.SOLVER_ITERATIVE)
+    cfg = SolveConfig(max_iterations=30, linear_solver=A.SOLVER_ITERATIVE, certification=A.CERTIFY_LOOP_ONLY)
     want = solve_with(oracle, "bao_solve", prob, cfg)
     for ba in adjusters:
         _compare_solutions(ba.solve(prob, cfg), want)
@@ -134,7 +134,7 @@ def test_medium_scene_forms_agree(adjusters):
     its two equivalent camera solvers' states at 2e-5 (tests/sparse_self_calibration/
     dense_parity.rs)."""
     sc = synthetic.medium_scene()
-    cfg = SolveConfig(max_iterations=8, linear_solver=A.SOLVER_ITERATIVE)
+    cfg = SolveConfig(max_iterations=8, linear_solver=A.SOLVER_ITERATIVE, certification=A.CERTIFY_LOOP_ONLY)
     ex, im = (ba.solve(sc.problem, cfg) for ba in adjusters)
     _compare_solutions(ex, im, tol_param=1e-7)
     print(f"medium iterative: explicit loop {ex.report.seconds_iterations:.3f} s, "
@@ -157,7 +157,7 @@ def test_dense_covisibility_forms_agree(adjusters):
         cov[np.ix_(cs, cs)] = True
     below = np.triu(cov, 1).sum(axis=0)  # per camera j: cameras c < j sharing a point with it
     assert below.max() > 256, below.max()
-    cfg = SolveConfig(max_iterations=5, linear_solver=A.SOLVER_ITERATIVE)
+    cfg = SolveConfig(max_iterations=5, linear_solver=A.SOLVER_ITERATIVE, certification=A.CERTIFY_LOOP_ONLY)
     ex, im = (ba.solve(p, cfg) for ba in adjusters)
     assert _last(adjusters[0]) == A.SCHUR_EXPLICIT
     _compare_solutions(ex, im, tol_param=1e-7)
@@ -171,7 +171,7 @@ def test_auto_switches_per_iteration(adjusters):
     follows a solve of 10 or more, so six iterations use both forms (more would carry the forms'
     rounding differences through longer inexact solves beyond these bounds)."""
     sc = synthetic.ring_scene(seed=0xBA0011, n_cameras=300, n_points=20_000)
-    cfg = SolveConfig(max_iterations=6, linear_solver=A.SOLVER_ITERATIVE)
+    cfg = SolveConfig(max_iterations=6, linear_solver=A.SOLVER_ITERATIVE, certification=A.CERTIFY_LOOP_ONLY)
     auto = HipBundleAdjuster(0)
     got = auto.solve(sc.problem, cfg)
     pcg = got.trace["pcg_iterations"]

@@ -116,7 +116,7 @@ def test_mixed_medium_direct_vs_fp64(adjuster):
     ring and fixture scenes)."""
     sc = synthetic.medium_scene()
     cfg = SolveConfig(max_iterations=40, function_tolerance=1e-12, parameter_tolerance=1e-12,
-                      gradient_tolerance=1e-14, linear_solver=A.SOLVER_DIRECT)
+                      gradient_tolerance=1e-14, linear_solver=A.SOLVER_DIRECT, certification=A.CERTIFY_LOOP_ONLY)
     want = adjuster.solve(sc.problem, cfg)
     got = adjuster.solve(sc.problem, cfg.with_(precision=A.PRECISION_MIXED))
     it_err, final, par = _errors(got, want)

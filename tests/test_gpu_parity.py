@@ -185,11 +185,12 @@ def test_pcg_step_matches_direct_solve(torch_cuda, adjuster, oracle, ring):
     assert rel_err(step.cpu().numpy(), so) < 1e-9
 
 
-def _compare_solutions(a, b, tol_cost=1e-10, tol_param=1e-8, cost_floor=0.0, tol_final=None, point_norm=False):
+def _compare_solutions(a, b, tol_cost=1e-10, tol_param=1e-8, cost_floor=0.0, tol_final=None, point_norm=False,
+                       tol_cov=1e-8):
     """Same LM trajectory (accept/reject sequence, per-iteration cost within tol_cost relative
     to max(|cost|, cost_floor)), same status, final cost within tol_final (default tol_cost) and
     parameters within tol_param (relative to max(|value|, 1); with point_norm, a point's three
-    coordinates relative to max(|X|, 1))."""
+    coordinates relative to max(|X|, 1)), and the same certificate (marginals within tol_cov)."""
     tol_final = tol_cost if tol_final is None else tol_final
     ta, tb = a.trace, b.trace
     n = min(len(ta["cost"]), len(tb["cost"]))
@@ -207,6 +208,25 @@ def _compare_solutions(a, b, tol_cost=1e-10, tol_param=1e-8, cost_floor=0.0, tol
         if point_norm and name == "point_xyz":
             scale = np.maximum(np.linalg.norm(y, axis=1, keepdims=True), 1.0)
         assert np.all(np.abs(x - y) <= tol_param * scale), name
+    _compare_certificates(a, b, tol_cov)
+
+
+def _compare_certificates(a, b, tol_cov=1e-8):
+    """The post-loop certificate: the same observability verdict and pose rank, the same held
+    groups, quarantined points and re-solve count, and the marginal covariances within tol_cov of
+    each group's largest entry (NaN where neither formed one)."""
+    ra, rb = a.report, b.report
+    for name in ("observability", "pose_rank", "pose_dimension", "unconstrained_intrinsics", "weak_point_resolves",
+                 "quarantined_points"):
+        assert getattr(ra, name) == getattr(rb, name), (name, getattr(ra, name), getattr(rb, name))
+    assert np.array_equal(a.intrinsic_unconstrained, b.intrinsic_unconstrained)
+    assert np.array_equal(a.point_quarantined, b.point_quarantined)
+    ca, cb = a.intrinsic_covariance, b.intrinsic_covariance
+    assert np.array_equal(np.isnan(ca), np.isnan(cb))
+    fin = ~np.isnan(cb).any(axis=(1, 2))
+    if fin.any():
+        scale = np.max(np.abs(cb[fin]), axis=(1, 2), keepdims=True)
+        assert np.all(np.abs(ca[fin] - cb[fin]) <= tol_cov * scale)
 
 
 @pytest.mark.parametrize("solver", [A.SOLVER_DIRECT, A.SOLVER_ITERATIVE])

@@ -69,7 +69,7 @@ def medium():
 
 def test_medium_direct_matches_oracle(adjuster, medium):
     prob, fx = medium
-    got = adjuster.solve(prob, SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT))
+    got = adjuster.solve(prob, SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT, certification=A.CERTIFY_LOOP_ONLY))
     err = check_trajectory(got, fx, 1e-10)
     assert abs(got.report.final_cost - fx["final_cost"]) <= 1e-10 * fx["final_cost"]
     stride = fx["point_stride"]
@@ -88,7 +88,7 @@ def test_medium_mixed_error_against_oracle(adjuster, medium):
     7 iterations), where the states may still differ along the weakly observed camera depths:
     measured 1.4e-8 in the final cost and 4e-4 in the parameters (relative to max(|x|, 1))."""
     prob, fx = medium
-    got = adjuster.solve(prob, SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT,
+    got = adjuster.solve(prob, SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT, certification=A.CERTIFY_LOOP_ONLY,
                                            precision=A.PRECISION_MIXED))
     cerr = abs(got.report.final_cost - fx["final_cost"]) / fx["final_cost"]
     perr = max(param_err(got.pose_center, arr(fx["pose_center"]).reshape(-1, 3)),
@@ -110,7 +110,7 @@ def large():
 def test_large_first_iterations_match_oracle(adjuster, large):
     prob, fx = large
     want = fx["first"]
-    got = adjuster.solve(prob, SolveConfig(max_iterations=5, linear_solver=A.SOLVER_ITERATIVE))
+    got = adjuster.solve(prob, SolveConfig(max_iterations=5, linear_solver=A.SOLVER_ITERATIVE, certification=A.CERTIFY_LOOP_ONLY))
     assert got.trace["pcg_iterations"].tolist() == want["pcg_iterations"]
     err = check_trajectory(got, want, 1e-10)
     ferr = abs(got.report.final_cost - want["final_cost"]) / want["final_cost"]
@@ -124,7 +124,7 @@ def test_large_converged_cost(adjuster, large, precision):
     prob, fx = large
     want = fx["converged"]
     c_star = want["final_cost"]
-    got = adjuster.solve(prob, SolveConfig(max_iterations=60, linear_solver=A.SOLVER_ITERATIVE, precision=precision))
+    got = adjuster.solve(prob, SolveConfig(max_iterations=60, linear_solver=A.SOLVER_ITERATIVE, precision=precision, certification=A.CERTIFY_LOOP_ONLY))
     tr = got.trace
     hit = [i for i in range(len(tr["cost"])) if tr["accepted"][i] and tr["candidate_cost"][i] <= c_star * (1 + 1e-6)]
     err = abs(got.report.final_cost - c_star) / c_star
@@ -139,7 +139,7 @@ def test_512_windows_batched_equal_single_and_oracle(adjuster):
     fx = load("windows")
     wins = synthetic.window_scenes()
     assert len(wins) == 512
-    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT)
+    cfg = SolveConfig(max_iterations=10, linear_solver=A.SOLVER_DIRECT, certification=A.CERTIFY_LOOP_ONLY)
     got, info = adjuster.solve_batched([w.problem for w in wins], cfg)
     worst = 0.0
     for w, scene in enumerate(wins):
