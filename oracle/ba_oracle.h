@@ -35,6 +35,16 @@
  *                     tests/sparse_self_calibration/main.rs:48-135 (final rmse < 1e-6)
  *   Auto policy       bao_plan: the reference's rule and estimate formulas, and the pattern
  *                     count against the product's independent count: test_plan.py
+ *   certificate       bao_certify: FULL_RANK exactly when numpy's smallest eigenvalue of the
+ *                     dense-normal-equation S is positive (and RANK_DEFICIENT for a camera seen
+ *                     through one point); marginals equal the blocks of numpy's inverse of that
+ *                     S to 1e-8: test_certify.py
+ *   weak points       bao_weak_points against a numpy restatement with central-difference
+ *                     projection Jacobians and numpy eigenvalues: test_certify.py
+ *   certified solve   flows of native/mod.rs:252-350 and native/self_calibration.rs:223-303
+ *                     (held groups, re-solves, quarantine) on four scenes: test_certify.py;
+ *                     the round-off canonicalisation (solver.rs:1030-1101) is not triggered by
+ *                     any test scene (parity unpinned)
  */
 #ifndef OPENMVG_BA_ORACLE_H
 #define OPENMVG_BA_ORACLE_H
