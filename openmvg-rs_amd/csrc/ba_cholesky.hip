@@ -95,14 +95,44 @@ __device__ __forceinline__ void load_tile(T (*dst)[TB + 1], const T* src, size_t
     }
 }
 
+// The value of v on lane `lane` (uniform), read across lanes without LDS.
+template <class T>
+__device__ __forceinline__ T lane_value(T v, int lane) {
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, lane);
+        const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), lane);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(unsigned, v), lane));
+    }
+}
+
+// 1 / sqrt(p) from the hardware estimate refined by two Newton steps y (3 - p y^2) / 2: a short
+// dependent chain in place of a correctly rounded square root and two divisions (within an ulp
+// or two of them).
+template <class T>
+__device__ __forceinline__ T inv_sqrt(T p) {
+    T y;
+    if constexpr (sizeof(T) == 8)
+        y = __builtin_amdgcn_rsq(p);
+    else
+        y = __builtin_amdgcn_rsqf(p);
+    y = y * (T(1.5) - T(0.5) * (p * y) * y);
+    y = y * (T(1.5) - T(0.5) * (p * y) * y);
+    return y;
+}
+
 // One wavefront factors the diagonal tile (size nb <= 64; identity beyond nb) held in `t` (lower
 // triangle) and forms X = L_kk^-1 into `xs` (row-major).  Lane i keeps row i in registers.
-// Column j < nb: the pivot a_jj is checked against 64 eps max|diag(A)| as the reference's dense camera
-// solve does (kernel/multi_view/dense/linear_solver.rs:9-61), L_ij = a_ij / sqrt(a_jj), and the
-// rows below update a_ic -= (a_ij / a_jj) a_cj with column j broadcast through LDS (`col`).
-// Entries above the diagonal of a row are never read.  Then L goes to `t` and lane j solves
-// column j of L X = I by forward substitution, x_k = (delta_kj - sum_{m<k} L_km x_m) / L_kk, with
-// row k of L read from `t`.  Returns false, on every lane, when a pivot fails.
+// Column j < nb: the pivot a_jj (read from lane j) is checked against 64 eps max|diag(A)| as the
+// reference's dense camera solve does (kernel/multi_view/dense/linear_solver.rs:9-61);
+// L_ij = a_ij / sqrt(a_jj), and the rows below update a_ic -= (a_ij / a_jj) a_cj with column j
+// broadcast through LDS (`col`, two alternating buffers, so that a column's reads never meet the
+// next column's writes).  Entries above the diagonal of a row are never read.  Then L goes to
+// `t` and lane j forms column j of X by eliminating column by column of L:
+// x_k = x_k / L_kk, then x_m -= L_mk x_k for m > k.  Returns false, on every lane, when a pivot
+// fails.  `col` holds 2 * 64 values.
 template <class T>
 __device__ bool factor_diag_wave(T (*t)[TB + 1], T* col, T (*xs)[TB + 1], uint32_t nb, T tol) {
     const int i = threadIdx.x & 63;
@@ -112,34 +142,41 @@ __device__ bool factor_diag_wave(T (*t)[TB + 1], T* col, T (*xs)[TB + 1], uint32
     bool ok = true;  // (a return inside the unrolled loop would keep r in scratch memory)
 #pragma unroll
     for (int j = 0; j < TB; ++j) {
+        T* cj = col + TB * (j & 1);
+        const T piv = lane_value(r[j], j);
+        cj[i] = r[j];
         wave_sync();
-        col[i] = r[j];
-        wave_sync();
-        const T piv = col[j];
         // rows at and beyond nb are the identity padding of a partial tile: their unit pivots are
         // not the matrix's and are not held to its (possibly much larger) tolerance
         if (j < (int)nb && !(isfinite(piv) && piv > tol)) {
             ok = false;  // uniform: every lane read the same pivot
             break;
         }
-        const T l = sqrt(piv);
-        const T sj = r[j] / piv;
-        r[j] = i > j ? r[j] / l : (i == j ? l : r[j]);
+        const T il = inv_sqrt(piv);
+        const T ip = il * il, l = piv * il;
+        const T sj = r[j] * ip;
+        r[j] = i > j ? r[j] * il : (i == j ? l : r[j]);
 #pragma unroll
-        for (int c = j + 1; c < TB; ++c) r[c] -= sj * col[c];
+        for (int c = j + 1; c < TB; ++c) r[c] -= sj * cj[c];
     }
     if (!ok) return false;
     wave_sync();
+    T d = T(1);
 #pragma unroll
-    for (int c = 0; c < TB; ++c) t[i][c] = c <= i ? r[c] : T(0);
+    for (int c = 0; c < TB; ++c) {
+        t[i][c] = c <= i ? r[c] : T(0);
+        d = c == i ? r[c] : d;  // (a register array indexed by the lane would live in scratch)
+    }
+    col[i] = T(1) / d;  // 1 / L_ii
     wave_sync();
     const int j = i;  // lane j: column j of X, in r
 #pragma unroll
-    for (int k = 0; k < TB; ++k) {
-        T s = k == j ? T(1) : T(0);
+    for (int k = 0; k < TB; ++k) r[k] = k == j ? T(1) : T(0);
 #pragma unroll
-        for (int m = 0; m < k; ++m) s -= t[k][m] * r[m];
-        r[k] = s / t[k][k];
+    for (int k = 0; k < TB; ++k) {
+        r[k] *= col[k];
+#pragma unroll
+        for (int m = k + 1; m < TB; ++m) r[m] -= t[m][k] * r[k];
     }
 #pragma unroll
     for (int k = 0; k < TB; ++k) xs[k][j] = r[k];
@@ -159,7 +196,7 @@ __global__ void __launch_bounds__(256) k_panel(T* A, uint32_t n, uint32_t ld, ui
     typedef typename Mfma<T>::v4 v4;
     __shared__ T t[TB][TB + 1];
     __shared__ T xs[TB][TB + 1];
-    __shared__ T col[TB];
+    __shared__ T col[2 * TB];
     __shared__ double yk[TB];
     __shared__ int ok_flag;
     if (*fail) return;
