@@ -382,6 +382,44 @@ def rank_deficient_scene(seed: int = 0xBA0001) -> BundleProblem:
         obs_pixel=np.concatenate([p.obs_pixel, uv[None]]))
 
 
+def bal_file_text(seed: int = 0xBA0041, n_cameras: int = 12, n_points: int = 800, noise: float = 0.5) -> str:
+    """A BAL file written with BAL's own conventions (not through bal.py): cameras on an arc
+    looking down their -z axis at points 8-14 units away, P = R(w) X + t with P.z < 0, measurement
+    f (1 + k1 |p|^2 + k2 |p|^4) p, p = -P.xy / P.z, plus Gaussian noise; every point seen by 2-6
+    cameras; the state perturbed.  noise = 0 gives exact measurements of the unperturbed state,
+    whose cost is then zero."""
+    rng = np.random.default_rng(seed)
+    th = np.linspace(-0.4, 0.4, n_cameras)
+    # camera i at (6 sin th, 0, -6 cos th) looking at the origin: its -z axis points there
+    C = np.stack([6.0 * np.sin(th), rng.normal(0, 0.2, n_cameras), -6.0 * np.cos(th)], -1)
+    R = np.stack([look_at(c, np.zeros(3)) for c in C])  # +z toward the origin
+    Rb = np.diag([1.0, -1.0, -1.0])[None] @ R           # BAL: -z toward the origin
+    t = -np.einsum("nij,nj->ni", Rb, C)
+    f = rng.uniform(900.0, 1100.0, n_cameras)
+    k1 = rng.uniform(-0.05, 0.05, n_cameras)
+    k2 = rng.uniform(-0.01, 0.01, n_cameras)
+    X = rng.uniform(-1.5, 1.5, (n_points, 3)) + np.array([0.0, 0.0, 4.0])
+    obs = []
+    for j in range(n_points):
+        for c in rng.choice(n_cameras, int(rng.integers(2, min(7, n_cameras + 1))), replace=False):
+            P = Rb[c] @ X[j] + t[c]
+            p = -P[:2] / P[2]
+            r2 = p @ p
+            uv = f[c] * (1.0 + k1[c] * r2 + k2[c] * r2 * r2) * p + (rng.normal(0.0, noise, 2) if noise else 0.0)
+            obs.append((int(c), j, uv[0], uv[1]))
+    w = log_so3(Rb)
+    if noise:
+        w = w + rng.normal(0.0, 0.002, w.shape)
+        t = t + rng.normal(0.0, 0.01, t.shape)
+        X = X + rng.normal(0.0, 0.01, X.shape)
+    lines = [f"{n_cameras} {n_points} {len(obs)}"]
+    lines += [f"{c} {j} {u:.17g} {v:.17g}" for c, j, u, v in obs]
+    for i in range(n_cameras):
+        lines += [f"{v:.17g}" for v in (*w[i], *t[i], f[i], k1[i], k2[i])]
+    lines += [f"{v:.17g}" for v in X.reshape(-1)]
+    return "\n".join(lines) + "\n"
+
+
 def medium_scene(seed: int = 0xBA0002) -> Scene:
     """200 cameras, 50,000 points, ~300k observations, (f, k1, k2) free, squared loss."""
     return grid_scene(seed, 20, 10, 50_000)
