@@ -229,7 +229,9 @@ enum ba_plan_reason {
     BA_PLAN_EXPLICIT_SOLVER = 0,          /* ExplicitBackend: linear_solver DIRECT or ITERATIVE */
     BA_PLAN_DIRECT_FITS_BUDGET = 1,       /* DirectFitsBudget: AUTO picks the direct step */
     BA_PLAN_DIRECT_EXCEEDS_BUDGET = 2,    /* DirectExceedsBudget: AUTO picks PCG, which fits */
-    BA_PLAN_LOWEST_MEMORY = 3             /* LowestMemoryBackend: neither fits; AUTO picks PCG */
+    BA_PLAN_LOWEST_MEMORY = 3,            /* LowestMemoryBackend: neither fits; AUTO picks PCG */
+    BA_PLAN_WORKLOAD_PREFERS_RETAINED = 4 /* WorkloadPrefersCachedRetained (calibrated path only):
+                                             at least 250,000 observations and D >= 768 */
 };
 
 /* The camera-step choice of LinearSolver::Auto (execution_policy.rs:190-227, estimates of

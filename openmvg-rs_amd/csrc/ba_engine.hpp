@@ -239,7 +239,7 @@ class Engine {
         cand_.alloc(1);
         gauge_.alloc(1);
         fail_.alloc(1);
-        dots_.alloc(4);
+        dots_.alloc(5);
         pair_a_.upload(PL_.block_a, s_);
         pair_b_.upload(PL_.block_b, s_);
         term_off_.upload(PL_.term_off, s_);
@@ -715,9 +715,18 @@ class Engine {
         return fin->accepted != 0;
     }
 
+    // The calibrated path's step limit (optimizer.rs:677-702), between the camera solve and the
+    // candidate pass: scales the step, and the PCG product of the step when with_product, so
+    // that the candidate, its dot products and the model reduction all use the limited step.
+    void limit_step(double maximum, bool with_product) {
+        launch_limit_step(s_, tp_, maximum, step_.p, with_product ? pq_.p : nullptr, dots_.p + 4);
+        limited_ = true;
+    }
+
     struct Candidate {
         CandScalars c;
         double dots[4];
+        double alpha;  // the step limit's factor (1 when not limited or no limit ran)
     };
     Candidate candidate(int state, int loss, double loss_scale, bool with_product) {
         const int cs = 1 - state;
@@ -732,11 +741,13 @@ class Engine {
         launch_reduce_candidates(s_, cscal_.p, tp.np, partials_.p, cand_.p);
         launch_step_dots(s_, tp.D, grad_.p, diag_.p, step_.p, with_product ? pq_.p : nullptr, dots_.p);
         HIP_OK(hipMemcpyAsync(h_cand_.p, cand_.p, sizeof(CandScalars), hipMemcpyDeviceToHost, s_));
-        HIP_OK(hipMemcpyAsync(h_dots_, dots_.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s_));
+        HIP_OK(hipMemcpyAsync(h_dots_, dots_.p, (limited_ ? 5 : 4) * sizeof(double), hipMemcpyDeviceToHost, s_));
         HIP_OK(hipStreamSynchronize(s_));
         Candidate out;
         out.c = *h_cand_.p;
         std::memcpy(out.dots, h_dots_, sizeof(out.dots));
+        out.alpha = limited_ ? h_dots_[4] : 1.0;
+        limited_ = false;
         return out;
     }
 
@@ -818,7 +829,8 @@ class Engine {
     Pinned<int> h_int_;
     Pinned<PcgState> h_pcg_[2];
     hipEvent_t pcg_ev_[2] = {nullptr, nullptr};
-    double h_dots_[4];
+    double h_dots_[5];
+    bool limited_ = false;
     bool fixed_gauge_ = false;
     struct {
         bool valid;

@@ -39,9 +39,6 @@ inline bool is_fixed_pose(const ba_problem* p, size_t i) {
     return !p->pose_var_rotation[i] && !p->pose_var_translation[i];
 }
 inline bool is_fixed_point(const ba_problem* p, size_t i) { return p->point_fixed && p->point_fixed[i]; }
-inline bool is_fixed_intrinsic(const ba_intrinsic& in) {
-    return !in.var_focal && !in.var_principal_point && !in.var_radial && !in.var_tangential;
-}
 inline double sq_dist(const double a[3], const double b[3]) {
     double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
     return d0 * d0 + d1 * d1 + d2 * d2;
@@ -218,7 +215,7 @@ void build_layout(Lowered& L) {
     L.cam_dim.assign(P, 0);
     size_t D = 0;
     for (size_t c = 0; c < P; ++c) {
-        uint32_t local = (c == 0 || L.cam_fixed[c]) ? 0u : (L.gauge_camera == (int)c ? 5u : 6u);
+        uint32_t local = ((c == 0 && !L.anchor_free) || L.cam_fixed[c]) ? 0u : (L.gauge_camera == (int)c ? 5u : 6u);
         L.cam_off[c] = (uint32_t)D;
         L.cam_dim[c] = local;
         D += local;
@@ -316,38 +313,90 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern, bool q
     kept_obs.reserve(O);
     for (size_t o = 0; o < O; ++o)
         if (!L.quarantined[p->obs_point[o]]) kept_obs.push_back((uint32_t)o);
-    // ---- native/mod.rs:457-497 ----
+    // ---- native/mod.rs:318-335, 457-497: the self-calibrating or the calibrated path ----
     bool free_intrinsic = false;
     for (uint32_t o : kept_obs) free_intrinsic |= !is_fixed_intrinsic(p->intrinsics[p->obs_intrinsic[o]]);
-    if (!free_intrinsic)
-        return error(BA_ERR_UNSUPPORTED, "every observed intrinsic is fixed: the calibrated path is not this adjuster's");
+    L.calibrated = !free_intrinsic;
+    L.anchor_free = false;
     std::vector<int64_t> pose_intrinsic(P, -1);
     std::vector<uint8_t> observed(P, 0);
     for (uint32_t o : kept_obs) {
         int64_t& slot = pose_intrinsic[p->obs_pose[o]];
         if (slot >= 0 && slot != (int64_t)p->obs_intrinsic[o])
-            return error(BA_ERR_UNSUPPORTED, "native self-calibration requires one intrinsic group per observed pose");
+            return error(BA_ERR_UNSUPPORTED, L.calibrated
+                                                 ? "calibrated pixel BA here requires one camera model per observed pose"
+                                                 : "native self-calibration requires one intrinsic group per observed pose");
         slot = p->obs_intrinsic[o];
         observed[p->obs_pose[o]] = 1;
     }
     for (auto& s : pose_intrinsic)
         if (s < 0) s = 0;
-    // ---- common_pose_variables (gauge.rs:9-40) ----
-    for (size_t i = 0; i < P; ++i) {
-        if (!observed[i] || is_fixed_pose(p, i)) continue;
-        if (!p->pose_var_translation[i])
-            return error(BA_ERR_UNSUPPORTED, "direct native lowering cannot optimize rotation without translation");
-        if (!p->pose_var_rotation[i])
-            return error(BA_ERR_UNSUPPORTED, "self-calibration requires full-pose or fixed camera blocks");
-    }
-    // ---- intrinsic checks (native/self_calibration.rs:63-85) ----
-    for (size_t g = 0; g < G; ++g) {
-        const ba_intrinsic& in = p->intrinsics[g];
-        if (in.var_radial && in.model == BA_CAMERA_BROWN_K1K2K3)
-            return error(BA_ERR_UNSUPPORTED, "native self-calibration refines at most two declared radial coefficients");
-        if (!is_fixed_intrinsic(in) && (!in.var_focal || in.var_principal_point || in.var_tangential))
-            return error(BA_ERR_UNSUPPORTED,
-                         "self-calibration refines the focal length and radial coefficients with a fixed principal point");
+    if (L.calibrated) {
+        // PreparedCalibratedProblem::try_new (native/calibrated.rs:200-226): fixed pinhole and
+        // Brown-Conrady cameras; this path covers the radial k1/k2 models with one focal length
+        for (uint32_t o : kept_obs) {
+            const int m = p->intrinsics[p->obs_intrinsic[o]].model;
+            if (m == BA_CAMERA_FISHEYE || m == BA_CAMERA_SPHERICAL)
+                return error(BA_ERR_UNSUPPORTED,
+                             "pixel BA supports fixed pinhole and Brown-Conrady cameras; use angular residuals for "
+                             "wide-angle cameras");
+        }
+        // can_use_two_view (native/two_view.rs:14-50): the reference's two-view solver is declined
+        if (P == 2) {
+            size_t kept_points = 0;
+            bool any_fixed_point = false;
+            for (size_t i = 0; i < N; ++i)
+                if (!L.quarantined[i]) {
+                    kept_points += 1;
+                    any_fixed_point |= is_fixed_point(p, i);
+                }
+            bool pinholes = true;
+            for (size_t i = 0; i < P; ++i) {
+                const ba_intrinsic& in = p->intrinsics[pose_intrinsic[i]];
+                const bool distortion = in.model != BA_CAMERA_PINHOLE &&
+                                        ((in.model >= BA_CAMERA_BROWN_K1 && in.k1 != 0.0) ||
+                                         (in.model >= BA_CAMERA_BROWN_K1K2 && in.k2 != 0.0) ||
+                                         (in.model >= BA_CAMERA_BROWN_K1K2K3 && in.k3 != 0.0) || in.p1 != 0.0 ||
+                                         in.p2 != 0.0);
+                pinholes &= in.model != BA_CAMERA_FISHEYE && in.model != BA_CAMERA_SPHERICAL && !distortion;
+            }
+            std::vector<std::pair<uint32_t, uint32_t>> pp;
+            pp.reserve(kept_obs.size());
+            for (uint32_t o : kept_obs) pp.emplace_back(p->obs_pose[o], p->obs_point[o]);
+            std::sort(pp.begin(), pp.end());
+            const bool repeated = std::adjacent_find(pp.begin(), pp.end()) != pp.end();
+            if (!any_fixed_point && kept_obs.size() == 2 * kept_points && !repeated && pinholes) {
+                int64_t a = -1;
+                if (p->gauge == BA_GAUGE_FIXED_BASELINE)
+                    a = p->gauge_first;
+                else
+                    for (size_t i = 0; i < 2 && a < 0; ++i)
+                        if (is_fixed_pose(p, i)) a = (int64_t)i;
+                if (a >= 0 && is_fixed_pose(p, (size_t)a) && p->pose_var_rotation[1 - a] && p->pose_var_translation[1 - a])
+                    return error(BA_ERR_UNSUPPORTED, "two-view refinement (native/two_view.rs) is not this adjuster's");
+            }
+        }
+        for (size_t i = 0; i < P; ++i)
+            if (observed[i] && !is_fixed_pose(p, i) && !(p->pose_var_rotation[i] && p->pose_var_translation[i]))
+                return error(BA_ERR_UNSUPPORTED, "calibrated pixel BA here requires full-pose or fixed camera blocks");
+    } else {
+        // ---- common_pose_variables (gauge.rs:9-40) ----
+        for (size_t i = 0; i < P; ++i) {
+            if (!observed[i] || is_fixed_pose(p, i)) continue;
+            if (!p->pose_var_translation[i])
+                return error(BA_ERR_UNSUPPORTED, "direct native lowering cannot optimize rotation without translation");
+            if (!p->pose_var_rotation[i])
+                return error(BA_ERR_UNSUPPORTED, "self-calibration requires full-pose or fixed camera blocks");
+        }
+        // ---- intrinsic checks (native/self_calibration.rs:63-85) ----
+        for (size_t g = 0; g < G; ++g) {
+            const ba_intrinsic& in = p->intrinsics[g];
+            if (in.var_radial && in.model == BA_CAMERA_BROWN_K1K2K3)
+                return error(BA_ERR_UNSUPPORTED, "native self-calibration refines at most two declared radial coefficients");
+            if (!is_fixed_intrinsic(in) && (!in.var_focal || in.var_principal_point || in.var_tangential))
+                return error(BA_ERR_UNSUPPORTED,
+                             "self-calibration refines the focal length and radial coefficients with a fixed principal point");
+        }
     }
     // ---- gauge_plan (gauge.rs:84-107, 231-330) ----
     bool observes_fixed_point = false;
@@ -355,7 +404,48 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern, bool q
     size_t anchor = 0;
     double scale = 1.0;
     int64_t gauge_pose = -1;
-    if (p->gauge == BA_GAUGE_FREE) {
+    if (L.calibrated && p->gauge == BA_GAUGE_FREE) {
+        // calibrated_gauge_plan (gauge.rs:123-229) for full-pose or fixed cameras: its held
+        // centres and known rotations are then exactly the fixed observed cameras
+        int64_t first_observed = -1, fixed_anchor = -1;
+        for (size_t i = 0; i < P; ++i) {
+            if (observed[i] && first_observed < 0) first_observed = (int64_t)i;
+            if (observed[i] && is_fixed_pose(p, i) && fixed_anchor < 0) fixed_anchor = (int64_t)i;
+        }
+        if (first_observed < 0) return error(BA_ERR_INTERNAL, "checked bundle has no observed pose");
+        anchor = (size_t)(fixed_anchor >= 0 ? fixed_anchor : first_observed);
+        double ca[3];
+        center_of(p, anchor, ca);
+        if (observes_fixed_point && fixed_anchor < 0) {
+            L.anchor_free = true;  // survey controls fix the frame; the anchor only sets the native frame
+        } else {
+            bool scale_set = false;
+            for (size_t i = 0; i < P && !scale_set; ++i) {
+                double c[3];
+                center_of(p, i, c);
+                scale_set = observed[i] && is_fixed_pose(p, i) && sq_dist(c, ca) > 0.0;
+            }
+            for (size_t k = 0; k < kept_obs.size() && !scale_set; ++k) {
+                size_t q = p->obs_point[kept_obs[k]];
+                scale_set = is_fixed_point(p, q) && sq_dist(p->point_xyz + 3 * q, ca) > 0.0;
+            }
+            if (!scale_set) {
+                double best = -1.0;
+                for (size_t i = 0; i < P; ++i) {
+                    if (i == anchor || !observed[i] || is_fixed_pose(p, i)) continue;
+                    double c[3];
+                    center_of(p, i, c);
+                    double d = sq_dist(c, ca);
+                    if (std::isfinite(d) && d > best) {
+                        best = d;
+                        gauge_pose = (int64_t)i;
+                    }
+                }
+                if (gauge_pose < 0 || !(best > 0.0))
+                    return error(BA_ERR_UNSUPPORTED, "free-scale BA has no nonzero active baseline");
+            }
+        }
+    } else if (p->gauge == BA_GAUGE_FREE) {
         if (observes_fixed_point) {
             bool fixed_observed = false;
             for (size_t i = 0; i < P; ++i) fixed_observed |= observed[i] && is_fixed_pose(p, i);
@@ -469,8 +559,27 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern, bool q
     L.gp.assign(3 * G, 0.0);
     L.grp_fixed.assign(G, 0);
     L.grp_free.assign(G, 1);
+    std::vector<uint8_t> used(G, 0);
+    for (uint32_t o : kept_obs) used[p->obs_intrinsic[o]] = 1;
     for (size_t g = 0; g < G; ++g) {
         const ba_intrinsic& in = p->intrinsics[g];
+        if (L.calibrated) {
+            // fixed cameras: the radial projection with f = fx and the declared k1, k2; the models
+            // of unobserved intrinsics never enter the objective
+            L.grp_fixed[g] = 1;
+            L.gp[3 * g] = 1.0;
+            if (!used[g]) continue;
+            const bool brown = in.model != BA_CAMERA_PINHOLE;
+            const double k3 = brown && in.model >= BA_CAMERA_BROWN_K1K2K3 ? in.k3 : 0.0;
+            if (in.fx != in.fy || k3 != 0.0 || (brown && (in.p1 != 0.0 || in.p2 != 0.0)))
+                return error(BA_ERR_UNSUPPORTED,
+                             "calibrated pixel BA here supports one focal length and k1+k2 radial distortion without "
+                             "tangential terms");
+            L.gp[3 * g] = in.fx;
+            L.gp[3 * g + 1] = brown && in.model >= BA_CAMERA_BROWN_K1 ? in.k1 : 0.0;
+            L.gp[3 * g + 2] = brown && in.model >= BA_CAMERA_BROWN_K1K2 ? in.k2 : 0.0;
+            continue;
+        }
         if (in.model == BA_CAMERA_FISHEYE || in.model == BA_CAMERA_SPHERICAL)
             return error(BA_ERR_UNSUPPORTED, "self-calibration supports perspective cameras only");
         if (in.model < BA_CAMERA_PINHOLE || in.model > BA_CAMERA_SPHERICAL)
@@ -495,7 +604,7 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern, bool q
     for (size_t d = 0; d < P; ++d) {
         size_t s = L.camera_source[d];
         L.cam_group[d] = (uint32_t)pose_intrinsic[s];
-        L.cam_fixed[d] = s == anchor || is_fixed_pose(p, s);
+        L.cam_fixed[d] = (s == anchor && !L.anchor_free) || is_fixed_pose(p, s);
     }
     L.gauge_camera = gauge_pose < 0 ? -1 : (int)dest[gauge_pose];
     // ---- observations sorted by (point, input index), principal point subtracted ----
@@ -530,6 +639,33 @@ Status lower_problem(const ba_problem* p, Lowered& L, bool build_pattern, bool q
     if (L.gauge_camera >= 0) {
         const double* tg = &L.t[3 * L.gauge_camera];
         L.scene_scale = std::sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
+    } else if (L.calibrated) {
+        // observed_scene_scale (optimizer.rs:629-668): the calibrated workspace carries explicit
+        // pose masks, so the length comes from observed points, by a scaled running sum of squares
+        double sc = 0.0, sqs = 1.0;
+        size_t count = 0;
+        for (size_t q = 0; q < NP; ++q) {
+            if (L.point_off[q] == L.point_off[q + 1]) continue;
+            count += 1;
+            for (int k = 0; k < 3; ++k) {
+                const double m = std::fabs(L.X[3 * q + k]);
+                if (m == 0.0) continue;
+                if (sc < m) {
+                    const double ratio = sc / m;
+                    sqs = 1.0 + sqs * ratio * ratio;
+                    sc = m;
+                } else {
+                    const double ratio = m / sc;
+                    sqs += ratio * ratio;
+                }
+            }
+        }
+        if (sc == 0.0 || count == 0) {
+            L.scene_scale = DBL_MIN;
+        } else {
+            const double rms = sc * std::sqrt(sqs / (double)count);
+            L.scene_scale = std::isfinite(rms) ? std::fmax(rms, DBL_MIN) : sc;
+        }
     } else {
         double s = 0.0;
         for (size_t q = 0; q < NP; ++q)
@@ -1073,6 +1209,42 @@ void plan_execution(const Lowered& L, const ba_solve_config* c, ba_execution_pla
     const uint64_t cross = L.n_cross;
     const uint64_t limit = c->workspace_budget_bytes ? c->workspace_budget_bytes : AUTO_BUDGET_BYTES;
     out.budget_bytes = limit;
+    if (L.calibrated) {
+        // select_calibrated (execution_policy.rs:123-183) over the serial estimates of
+        // workspace/calibrated.rs:480-530: the cached point cache keeps point steps and f64 cross
+        // rows (execution_policy.rs:68-96), the direct estimate adds the dense rank certificate up
+        // to dimension 2048 (workspace.rs:126-132), the retained estimate is
+        // estimated_calibrated_retained_bytes (retained/mod.rs:156-191) without worker shards
+        const uint64_t cached_points = (NP + 1) * 8 + NP * (1 + 2 * 24) + cross * 24;
+        const uint64_t dense_rank = D <= 2048 ? (D + 2) * D * 8 : 0;
+        out.estimated_retained_bytes = P * (2 * 168 + 216) + NP * 48 + D * 48 + cached_points;
+        const bool prefers_cached = L.n_obs() >= 250000 && D >= 768;  // execution_policy.rs:14-22
+        // the exact pattern count only where the bounds (diagonal blocks, full lower triangle) do
+        // not decide
+        uint64_t diagonal_nnz = 0;
+        for (size_t k = 0; k < P; ++k) diagonal_nnz += (uint64_t)L.cam_dim[k] * (L.cam_dim[k] + 1) / 2;
+        const uint64_t lower = diagonal_nnz * 32 + dense_rank + cached_points;
+        const uint64_t upper = D * (D + 1) / 2 * 32 + dense_rank + cached_points;
+        if ((prefers_cached && out.estimated_retained_bytes <= limit) || !direct_fits(lower, limit)) {
+            out.estimated_direct_bytes = lower;
+        } else if (direct_fits(upper, limit)) {
+            out.estimated_direct_bytes = upper;
+        } else {
+            out.reduced_nonzeros = reduced_nonzeros(L);
+            out.estimated_direct_bytes = out.reduced_nonzeros * 32 + dense_rank + cached_points;
+        }
+        if (prefers_cached && out.estimated_retained_bytes <= limit) {
+            out.linear_solver = BA_SOLVER_ITERATIVE;
+            out.reason = BA_PLAN_WORKLOAD_PREFERS_RETAINED;
+        } else if (direct_fits(out.estimated_direct_bytes, limit)) {
+            out.linear_solver = BA_SOLVER_DIRECT;
+            out.reason = BA_PLAN_DIRECT_FITS_BUDGET;
+        } else {
+            out.linear_solver = BA_SOLVER_ITERATIVE;
+            out.reason = out.estimated_retained_bytes <= limit ? BA_PLAN_DIRECT_EXCEEDS_BUDGET : BA_PLAN_LOWEST_MEMORY;
+        }
+        return;
+    }
     // estimated_point_cache_bytes (execution_policy.rs:68-96): usize offsets, no retained point
     // steps (serial direct assembly), one [f64; 3] row per point parameter
     const uint64_t direct_points = (NP + 1) * 8 + cross * 24;

@@ -21,6 +21,11 @@ struct Status {
 };
 inline Status error(int code, std::string message) { return Status{code, std::move(message)}; }
 
+// IntrinsicVariables::is_fixed (problem.rs:136-185): no calibration coordinate is variable.
+inline bool is_fixed_intrinsic(const ba_intrinsic& in) {
+    return !in.var_focal && !in.var_principal_point && !in.var_radial && !in.var_tangential;
+}
+
 // The lowered problem: PreparedSelfCalibratingProblem (native/self_calibration.rs:30-44) plus the
 // workspace layout and index maps (workspace/topology.rs:351-397, workspace/pattern.rs:306-468).
 struct Lowered {
@@ -39,6 +44,12 @@ struct Lowered {
     std::vector<uint8_t> cam_fixed, point_fixed, grp_fixed;
     std::vector<uint32_t> grp_free;
     int gauge_camera = -1;
+    // the calibrated (fixed-intrinsic) path of native/calibrated.rs: every intrinsic group fixed,
+    // the camera step limited (optimizer.rs:677-702) and the direct step's model reduction from
+    // the damped quadratic form (calibrated/camera_step.rs:40-62); its anchor stays free when
+    // survey controls fix the frame without a fixed observed pose (native/gauge.rs:153-162)
+    bool calibrated = false;
+    bool anchor_free = false;
     int loss = 0;
     double loss_scale = 0.0;               // kernel scale (Cauchy: scale / sqrt 2)
     double scene_scale = 1.0;

@@ -1565,6 +1565,37 @@ __global__ void __launch_bounds__(1024) k_step_dots(uint32_t D, const double* g,
     }
 }
 
+// limit_camera_step (kernel/multi_view/optimizer.rs:677-702) of the calibrated path: the largest
+// norm of a camera's rotation block (its first min(dim, 3) values) or translation block (the rest)
+// is held to `maximum` by scaling the whole step, and the PCG product S x with it.  One workgroup:
+// each thread takes cameras t, t + 1024, ...; each norm is the reference's sequential sum of
+// squares; the maximum is exact in any order.  The scale factor (1 when not limited) goes to
+// *alpha.
+__global__ void __launch_bounds__(1024) k_limit_step(Topo tp, double maximum, double* step, double* product,
+                                                     double* alpha) {
+    __shared__ double sh[1024];
+    double largest = 0.0;
+    for (uint32_t c = threadIdx.x; c < tp.nc; c += 1024) {
+        const uint32_t dim = tp.cam_dim[c], off = tp.cam_off[c];
+        if (dim == 0) continue;
+        const uint32_t first = dim < 3 ? dim : 3;
+        double s1 = 0.0, s2 = 0.0;
+        for (uint32_t k = 0; k < first; ++k) s1 += step[off + k] * step[off + k];
+        for (uint32_t k = first; k < dim; ++k) s2 += step[off + k] * step[off + k];
+        largest = fmax(largest, sqrt(s1));
+        largest = fmax(largest, sqrt(s2));
+    }
+    largest = tree_max<1024>(largest, sh);
+    const bool limit = largest > maximum;
+    const double scale = limit ? maximum / largest : 1.0;
+    if (threadIdx.x == 0) *alpha = scale;
+    if (!limit) return;
+    for (uint32_t i = threadIdx.x; i < tp.D; i += 1024) {
+        step[i] *= scale;
+        if (product) product[i] *= scale;
+    }
+}
+
 // ----------------------------------------------------------------------------------------------
 // (c) PCG.  Preconditioner blocks: Jacobi-scaled Cholesky of U_AA(damped) - Schur_AA
 // (retained/block.rs:171-250); stored per block as 45 packed lower values + 9 scales.
@@ -2628,6 +2659,9 @@ void launch_reduce_candidates(hipStream_t s, const double* cscal, uint32_t np, d
 void launch_step_dots(hipStream_t s, uint32_t D, const double* g, const double* diag, const double* x,
                       const double* product, double* dots) {
     hipLaunchKernelGGL(k_step_dots, dim3(1), dim3(1024), 0, s, D, g, diag, x, product, dots);
+}
+void launch_limit_step(hipStream_t s, const Topo& tp, double maximum, double* step, double* product, double* alpha) {
+    hipLaunchKernelGGL(k_limit_step, dim3(1), dim3(1024), 0, s, tp, maximum, step, product, alpha);
 }
 void launch_precond_factor(hipStream_t s, const Topo& tp, const int32_t* diag_pair, const uint64_t* value_off,
                            const double* u_vals, const double* s_vals, double* factors, int* fail) {

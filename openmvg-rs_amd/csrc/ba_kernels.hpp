@@ -239,6 +239,10 @@ void launch_reduce_candidates(hipStream_t s, const double* cscal, uint32_t np, d
 // dots[0] = g.x, dots[1] = diag.x^2, dots[2] = x.x, dots[3] = x.product
 void launch_step_dots(hipStream_t s, uint32_t D, const double* g, const double* diag, const double* x,
                       const double* product, double* dots);
+// Calibrated step limit (optimizer.rs:677-702): when the largest camera rotation or translation
+// block norm of `step` exceeds `maximum`, scale step (and product, when not null) by
+// maximum / largest; *alpha receives the factor (1 when not limited).
+void launch_limit_step(hipStream_t s, const Topo& tp, double maximum, double* step, double* product, double* alpha);
 
 // ---- implicit-Schur PCG (kernel (c)) ----
 // The camera pass reads the structure-of-arrays rows in 16-byte groups, pcg_vec<T>() consecutive

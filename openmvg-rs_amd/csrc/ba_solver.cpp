@@ -237,6 +237,7 @@ Status refine(Engine<T>& E, const Lowered& L, const ba_solve_config* cfg, ba_sol
                 trace_push(trace, current, lambda, NAN, NAN, NAN, false, terms);
                 continue;
             }
+            if (L.calibrated) E.limit_step(0.5, iterative);  // calibrated/candidate.rs:36-44
             auto c = E.candidate(state, sched[si].loss, sched[si].scale, iterative);
             if (c.c.rejected != 0.0) {
                 rej[2]++;
@@ -247,7 +248,12 @@ Status refine(Engine<T>& E, const Lowered& L, const ba_solve_config* cfg, ba_sol
             const double point_rms = std::sqrt(c.c.step2 / npts);
             const double gdot = c.dots[0], cdn = c.dots[1], xx = c.dots[2], xq = c.dots[3];
             double predicted;
-            if (!iterative) {  // exact_model_reduction (self_calibration/optimizer.rs:30-52)
+            if (!iterative && L.calibrated) {
+                // calibrated_model_reduction (calibrated/camera_step.rs:40-62): pmr - g.x - x.S x / 2
+                // with the damped S; the limited step x = alpha x0 of the direct solve S x0 = -g
+                // gives x.S x = -alpha g.x
+                predicted = (a.pmr - gdot + 0.5 * (c.alpha * gdot)) + 0.5 * lambda * (cdn + c.c.damping_norm);
+            } else if (!iterative) {  // exact_model_reduction (self_calibration/optimizer.rs:30-52)
                 predicted = (a.pmr - 0.5 * gdot) + 0.5 * lambda * (cdn + c.c.damping_norm);
             } else {  // predicted_reduction (self_calibration_retained.rs:585-598) + undamped
                 double pr = a.pmr + (-gdot - 0.5 * xq);
@@ -583,10 +589,21 @@ Status solve_once(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* 
     Status st = refine_lowered(h, p, cfg, L, plan, trace, first);
     if (!st.ok()) return st;
     out.alloc(p);
+    // the calibrated path reports the unobserved free intrinsics as unconstrained
+    // (native/calibrated.rs:485-487, native/mod.rs:465-477) and re-solves nothing
+    std::vector<uint32_t> unobserved_free;
+    if (L.calibrated) {
+        std::vector<uint8_t> seen(p->n_intrinsics, 0);
+        for (size_t o = 0; o < p->n_observations; ++o) seen[p->obs_intrinsic[o]] = 1;
+        for (size_t g = 0; g < p->n_intrinsics; ++g)
+            if (!seen[g] && !is_fixed_intrinsic(p->intrinsics[g])) unobserved_free.push_back((uint32_t)g);
+        for (uint32_t g : unobserved_free) out.unconstrained[g] = 1;
+    }
     if (!full) {
         ba_solution v = out.view();
         raise_solution(p, L, first.R.data(), first.t.data(), first.gp.data(), first.X.data(), &v);
         out.rep = first.r;
+        out.rep.unconstrained_intrinsics = (uint32_t)unobserved_free.size();
         return st;
     }
     bool exact, marginals;
@@ -623,7 +640,7 @@ Status solve_once(ba_hip_handle* h, const ba_problem* p, const ba_solve_config* 
         first.initial_inlier_rmse = second.initial_inlier_rmse;
     }
     for (uint32_t g : held) out.unconstrained[g] = 1;
-    out.rep.unconstrained_intrinsics = (uint32_t)held.size();
+    out.rep.unconstrained_intrinsics = (uint32_t)(held.size() + unobserved_free.size());
     canonicalize_roundoff(p, out, first);
     return Status{};
 }

@@ -163,6 +163,7 @@ PLAN_EXPLICIT_SOLVER = 0
 PLAN_DIRECT_FITS_BUDGET = 1
 PLAN_DIRECT_EXCEEDS_BUDGET = 2
 PLAN_LOWEST_MEMORY = 3
+PLAN_WORKLOAD_PREFERS_RETAINED = 4
 
 
 class ExecutionPlanC(C.Structure):

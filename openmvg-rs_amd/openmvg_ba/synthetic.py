@@ -21,7 +21,7 @@ crates/openmvg-ba/src/kernel/multi_view/self_calibration/model.rs:26-75.
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, replace
 
 import numpy as np
 
@@ -332,6 +332,55 @@ def candidate_rule_scene(seed: int = 0xBA0001):
                          obs_pose=np.array(op), obs_intrinsic=np.array(op), obs_point=np.array(opt),
                          obs_pixel=np.array(pix), pose_var_rotation=var, pose_var_translation=var.copy())
     return prob, q_near, q_far
+
+
+def fixed_intrinsics(scene: Scene) -> Scene:
+    """The same scene with every intrinsic fixed at its initial value: the calibrated pixel path
+    (crates/openmvg-ba/src/native/calibrated.rs) instead of self-calibration."""
+    intr = [replace(i, var_focal=False, var_radial=False, var_principal_point=False, var_tangential=False)
+            for i in scene.problem.intrinsics]
+    return replace(scene, problem=replace(scene.problem, intrinsics=intr), name=scene.name + "-calibrated")
+
+
+def calibrated_ring_scene(seed: int = 0xBA0001, noise: float = 0.5, loss: int = A.LOSS_SQUARED,
+                          loss_scale: float = 0.0) -> Scene:
+    """The ring scene with its pinhole cameras (f = 1000) fixed: the calibrated pixel path with
+    6-parameter pose blocks (the gauge camera 5)."""
+    sc = fixed_intrinsics(ring_scene(seed, noise=noise))
+    sc.problem.robust_loss, sc.problem.loss_scale = loss, loss_scale
+    return sc
+
+
+def calibrated_brown_scene(seed: int = 0xBA0006, nx: int = 6, ny: int = 5, n_points: int = 3000) -> Scene:
+    """A small grid scene whose Brown-Conrady (k1, k2) cameras are fixed at their true values
+    (k1, k2 ~ U[-0.1, 0.1]), so the calibrated residuals carry the radial distortion."""
+    sc = grid_scene(seed, nx, ny, n_points, perturb=3.0)
+    intr = [replace(i, focal=float(f), k1=float(a), k2=float(b)) for i, f, a, b in
+            zip(sc.problem.intrinsics, sc.truth_focal, sc.truth_k1, sc.truth_k2)]
+    sc = replace(sc, problem=replace(sc.problem, intrinsics=intr))
+    return fixed_intrinsics(sc)
+
+
+def step_limit_scene(seed: int = 0xBA0001, roll: float = 0.6, camera: int = 3) -> Scene:
+    """The calibrated ring with one camera rolled about its optical axis by `roll` radians: the
+    Gauss-Newton correction of that rotation exceeds the calibrated step limit of 0.5
+    (crates/openmvg-ba/src/kernel/multi_view/optimizer.rs:677-702), so the first steps are scaled."""
+    sc = calibrated_ring_scene(seed)
+    rot = sc.problem.pose_rotation.copy()
+    rot[camera] = exp_so3(np.array([0.0, 0.0, roll])) @ rot[camera]
+    return replace(sc, problem=replace(sc.problem, pose_rotation=rot))
+
+
+def controls_scene(seed: int = 0xBA0001, n_controls: int = 40) -> Scene:
+    """The calibrated ring with `n_controls` points fixed at their true positions (survey
+    controls), no fixed pose and a free gauge: the controls fix the frame, so the calibrated
+    gauge plan leaves every camera free, the anchor included (native/gauge.rs:153-162)."""
+    sc = calibrated_ring_scene(seed)
+    pts = sc.problem.point_xyz.copy()
+    fixed = np.zeros(len(pts), dtype=np.uint8)
+    fixed[:n_controls] = 1
+    pts[:n_controls] = sc.truth_points[:n_controls]
+    return replace(sc, problem=replace(sc.problem, point_xyz=pts, point_fixed=fixed))
 
 
 def narrow_field_scene(seed: int = 0xBA0031, n_points: int = 600, spread: float = 0.3) -> Scene:

@@ -1420,6 +1420,10 @@ struct Lowered {
     std::vector<uint8_t> cam_fixed, point_fixed, grp_fixed;
     std::vector<uint32_t> grp_free;
     int gauge_camera = -1;
+    // the calibrated (fixed-intrinsic) path of native/calibrated.rs; its anchor stays free when
+    // survey controls fix the frame without a fixed observed pose (native/gauge.rs:153-162)
+    bool calibrated = false;
+    bool anchor_free = false;
     // layout
     size_t D = 0;
     std::vector<uint32_t> cam_off, cam_dim, grp_off, grp_dim, cross_off, params, col_ptr, row_idx;
@@ -1561,44 +1565,137 @@ static Err lower(const ba_problem* p, Lowered& L, bool quarantine = true, const 
     bool free_intr = false;  // has_free_intrinsics, native/mod.rs:457-463
     for (uint32_t o : kobs)
         if (!intrinsic_is_fixed(p->intrinsics[p->obs_intrinsic[o]])) free_intr = true;
-    if (!free_intr) return fail(BA_ERR_UNSUPPORTED, "fixed-intrinsic (calibrated) path is not this adjuster's");
+    L.calibrated = !free_intr;
     // single_pose_intrinsics, native/mod.rs:482-497
     std::vector<size_t> pose_intr(p->n_poses, SIZE_MAX);
     for (uint32_t o : kobs) {
         size_t& s = pose_intr[p->obs_pose[o]];
         if (s != SIZE_MAX && s != p->obs_intrinsic[o])
-            return fail(BA_ERR_UNSUPPORTED, "native self-calibration requires one intrinsic group per observed pose");
+            return fail(BA_ERR_UNSUPPORTED, L.calibrated
+                                                ? "calibrated pixel BA here requires one camera model per observed pose"
+                                                : "native self-calibration requires one intrinsic group per observed pose");
         s = p->obs_intrinsic[o];
     }
     for (auto& s : pose_intr)
         if (s == SIZE_MAX) s = 0;
     std::vector<uint8_t> observed(p->n_poses, 0);
     for (uint32_t o : kobs) observed[p->obs_pose[o]] = 1;
-    // common_pose_variables, native/gauge.rs:9-40
-    for (size_t i = 0; i < p->n_poses; ++i) {
-        if (!observed[i] || pose_fixed(p, i)) continue;
-        if (!p->pose_var_translation[i])
-            return fail(BA_ERR_UNSUPPORTED, "direct native lowering cannot optimize rotation without translation");
-        if (!p->pose_var_rotation[i])
-            return fail(BA_ERR_UNSUPPORTED, "self-calibration requires full-pose or fixed camera blocks");
+    if (L.calibrated) {
+        // PreparedCalibratedProblem::try_new (native/calibrated.rs:200-226): fixed pinhole and
+        // Brown-Conrady cameras only; the lowering restated here covers the radial k1/k2 models
+        // with one focal length (the radial projection of the self-calibrating path)
+        for (uint32_t o : kobs) {
+            const ba_intrinsic& in = p->intrinsics[p->obs_intrinsic[o]];
+            if (in.model == BA_CAMERA_FISHEYE || in.model == BA_CAMERA_SPHERICAL)
+                return fail(BA_ERR_UNSUPPORTED,
+                            "pixel BA supports fixed pinhole and Brown-Conrady cameras; use angular residuals for "
+                            "wide-angle cameras");
+        }
+        // can_use_two_view (native/two_view.rs:14-50): the reference's two-view solver
+        size_t kept_points = 0;
+        for (size_t i = 0; i < p->n_points; ++i) kept_points += !L.quarantined[i];
+        bool any_fixed_point = false, pinholes = true, repeated = false;
+        for (size_t i = 0; i < p->n_points; ++i) any_fixed_point |= !L.quarantined[i] && point_is_fixed(p, i);
+        {
+            std::vector<std::pair<uint32_t, uint32_t>> pp;
+            for (uint32_t o : kobs) pp.emplace_back(p->obs_pose[o], p->obs_point[o]);
+            std::sort(pp.begin(), pp.end());
+            repeated = std::adjacent_find(pp.begin(), pp.end()) != pp.end();
+        }
+        for (size_t i = 0; i < p->n_poses; ++i) {
+            const ba_intrinsic& in = p->intrinsics[pose_intr[i]];
+            const bool distortion = in.model != BA_CAMERA_PINHOLE &&
+                                    ((in.model >= BA_CAMERA_BROWN_K1 && in.k1 != 0.0) ||
+                                     (in.model >= BA_CAMERA_BROWN_K1K2 && in.k2 != 0.0) ||
+                                     (in.model >= BA_CAMERA_BROWN_K1K2K3 && in.k3 != 0.0) || in.p1 != 0.0 ||
+                                     in.p2 != 0.0);
+            pinholes &= in.model != BA_CAMERA_FISHEYE && in.model != BA_CAMERA_SPHERICAL && !distortion;
+        }
+        if (p->n_poses == 2 && !any_fixed_point && kobs.size() == 2 * kept_points && !repeated && pinholes) {
+            long a = -1;
+            if (p->gauge == BA_GAUGE_FIXED_BASELINE)
+                a = (long)p->gauge_first;
+            else
+                for (size_t i = 0; i < 2 && a < 0; ++i)
+                    if (pose_fixed(p, i)) a = (long)i;
+            if (a >= 0 && pose_fixed(p, (size_t)a) && p->pose_var_rotation[1 - a] && p->pose_var_translation[1 - a])
+                return fail(BA_ERR_UNSUPPORTED, "two-view refinement (native/two_view.rs) is not this adjuster's");
+        }
+        // pose masks: every observed camera full-pose or fixed (the 6-dimensional blocks)
+        for (size_t i = 0; i < p->n_poses; ++i)
+            if (observed[i] && !pose_fixed(p, i) && !(p->pose_var_rotation[i] && p->pose_var_translation[i]))
+                return fail(BA_ERR_UNSUPPORTED, "calibrated pixel BA here requires full-pose or fixed camera blocks");
+    } else {
+        // common_pose_variables, native/gauge.rs:9-40
+        for (size_t i = 0; i < p->n_poses; ++i) {
+            if (!observed[i] || pose_fixed(p, i)) continue;
+            if (!p->pose_var_translation[i])
+                return fail(BA_ERR_UNSUPPORTED, "direct native lowering cannot optimize rotation without translation");
+            if (!p->pose_var_rotation[i])
+                return fail(BA_ERR_UNSUPPORTED, "self-calibration requires full-pose or fixed camera blocks");
+        }
+        // intrinsic model checks, native/self_calibration.rs:63-85, 483-512
+        for (size_t g = 0; g < p->n_intrinsics; ++g) {
+            const ba_intrinsic& in = p->intrinsics[g];
+            if (in.var_radial && in.model == BA_CAMERA_BROWN_K1K2K3)
+                return fail(BA_ERR_UNSUPPORTED, "native self-calibration refines at most two declared radial coefficients");
+            if (!intrinsic_is_fixed(in) && (!in.var_focal || in.var_principal_point || in.var_tangential))
+                return fail(BA_ERR_UNSUPPORTED,
+                            "self-calibration refines the focal length and radial coefficients with a fixed principal point");
+        }
     }
-    // intrinsic model checks, native/self_calibration.rs:63-85, 483-512
-    for (size_t g = 0; g < p->n_intrinsics; ++g) {
-        const ba_intrinsic& in = p->intrinsics[g];
-        if (in.var_radial && in.model == BA_CAMERA_BROWN_K1K2K3)
-            return fail(BA_ERR_UNSUPPORTED, "native self-calibration refines at most two declared radial coefficients");
-        if (!intrinsic_is_fixed(in) && (!in.var_focal || in.var_principal_point || in.var_tangential))
-            return fail(BA_ERR_UNSUPPORTED,
-                        "self-calibration refines the focal length and radial coefficients with a fixed principal point");
-    }
-    // gauge_plan, native/gauge.rs:231-330 (validate_anchor_policy first)
     bool fixed_point_observed = false;
     for (uint32_t o : kobs)
         if (point_is_fixed(p, p->obs_point[o])) fixed_point_observed = true;
     size_t anchor = 0;
     double scale = 1.0;
     int native_gauge = -1;  // input pose index of the FixedBaseline camera
-    if (p->gauge == BA_GAUGE_FREE) {
+    if (L.calibrated && p->gauge == BA_GAUGE_FREE) {
+        // calibrated_gauge_plan (native/gauge.rs:123-229) for full-pose or fixed cameras: held
+        // centres and known rotations are then exactly the fixed observed cameras
+        size_t first_observed = SIZE_MAX;
+        long fixed_anchor = -1;
+        for (size_t i = 0; i < p->n_poses; ++i) {
+            if (observed[i] && first_observed == SIZE_MAX) first_observed = i;
+            if (observed[i] && pose_fixed(p, i) && fixed_anchor < 0) fixed_anchor = (long)i;
+        }
+        if (first_observed == SIZE_MAX) return fail(BA_ERR_INTERNAL, "checked bundle has no observed pose");
+        anchor = fixed_anchor >= 0 ? (size_t)fixed_anchor : first_observed;
+        const Vec3 Ca = pose_C(p, anchor);
+        if (fixed_point_observed && fixed_anchor < 0) {
+            // survey controls fix the frame: the anchor only sets the native frame
+            L.anchor_free = true;
+        } else {
+            bool distinct_held = false;
+            for (size_t i = 0; i < p->n_poses; ++i)
+                if (observed[i] && pose_fixed(p, i) && dist2(pose_C(p, i), Ca) > 0.0) distinct_held = true;
+            bool control_sets_scale = false;
+            for (uint32_t o : kobs) {
+                size_t q = p->obs_point[o];
+                if (point_is_fixed(p, q)) {
+                    Vec3 X{p->point_xyz[3 * q], p->point_xyz[3 * q + 1], p->point_xyz[3 * q + 2]};
+                    if (dist2(X, Ca) > 0.0) control_sets_scale = true;
+                }
+            }
+            if (!(distinct_held || control_sets_scale)) {
+                long best = -1;
+                double best_d = 0.0;
+                for (size_t i = 0; i < p->n_poses; ++i) {
+                    if (i == anchor || !observed[i] || pose_fixed(p, i)) continue;
+                    double d = dist2(pose_C(p, i), Ca);
+                    if (!std::isfinite(d)) continue;
+                    if (best < 0 || d > best_d) {
+                        best = (long)i;
+                        best_d = d;
+                    }
+                }
+                if (best < 0 || !(best_d > 0.0))
+                    return fail(BA_ERR_UNSUPPORTED, "free-scale BA has no nonzero active baseline");
+                native_gauge = (int)best;
+            }
+        }
+    } else if (p->gauge == BA_GAUGE_FREE) {
+        // gauge_plan, native/gauge.rs:231-330 (validate_anchor_policy first)
         if (fixed_point_observed) {
             bool any = false;
             for (size_t i = 0; i < p->n_poses; ++i)
@@ -1729,8 +1826,28 @@ static Err lower(const ba_problem* p, Lowered& L, bool quarantine = true, const 
     L.gp.assign(3 * ng, 0.0);
     L.grp_fixed.assign(ng, 0);
     L.grp_free.assign(ng, 0);
+    std::vector<uint8_t> used(ng, 0);
+    for (uint32_t o : kobs) used[p->obs_intrinsic[o]] = 1;
     for (size_t g = 0; g < ng; ++g) {
         const ba_intrinsic& in = p->intrinsics[g];
+        if (L.calibrated) {
+            // fixed cameras: the radial projection with f = fx and the declared k1, k2; the
+            // models of unobserved intrinsics never enter the objective
+            L.grp_fixed[g] = 1;
+            L.grp_free[g] = 0;
+            L.gp[3 * g] = 1.0;
+            if (!used[g]) continue;
+            const bool brown = in.model != BA_CAMERA_PINHOLE;
+            const double k3 = brown && in.model >= BA_CAMERA_BROWN_K1K2K3 ? in.k3 : 0.0;
+            if (in.fx != in.fy || k3 != 0.0 || (brown && (in.p1 != 0.0 || in.p2 != 0.0)))
+                return fail(BA_ERR_UNSUPPORTED,
+                            "calibrated pixel BA here supports one focal length and k1+k2 radial distortion without "
+                            "tangential terms");
+            L.gp[3 * g] = in.fx;
+            L.gp[3 * g + 1] = brown && in.model >= BA_CAMERA_BROWN_K1 ? in.k1 : 0.0;
+            L.gp[3 * g + 2] = brown && in.model >= BA_CAMERA_BROWN_K1K2 ? in.k2 : 0.0;
+            continue;
+        }
         if (in.model == BA_CAMERA_FISHEYE || in.model == BA_CAMERA_SPHERICAL)
             return fail(BA_ERR_UNSUPPORTED, "self-calibration supports perspective cameras only");
         double k1 = 0, k2 = 0, k3 = 0, p1 = 0, p2 = 0;
@@ -1762,7 +1879,7 @@ static Err lower(const ba_problem* p, Lowered& L, bool quarantine = true, const 
     for (size_t d = 0; d < nc; ++d) {
         size_t s = L.cam_source[d];
         L.cam_group[d] = (uint32_t)pose_intr[s];
-        L.cam_fixed[d] = (s == anchor) || pose_fixed(p, s);
+        L.cam_fixed[d] = (s == anchor && !L.anchor_free) || pose_fixed(p, s);
     }
     L.gauge_camera = native_gauge < 0 ? -1 : (int)dest[native_gauge];
     // observations sorted by (point, source) over the kept points (:119-137)
@@ -1792,7 +1909,7 @@ static Err lower(const ba_problem* p, Lowered& L, bool quarantine = true, const 
     L.cam_dim.assign(nc, 0);
     size_t D = 0;
     for (size_t c = 0; c < nc; ++c) {
-        uint32_t local = (c == 0 || L.cam_fixed[c]) ? 0 : (L.gauge_camera == (int)c ? 5 : 6);
+        uint32_t local = ((c == 0 && !L.anchor_free) || L.cam_fixed[c]) ? 0 : (L.gauge_camera == (int)c ? 5 : 6);
         L.cam_off[c] = (uint32_t)D;
         L.cam_dim[c] = local;
         D += local;
@@ -2050,6 +2167,40 @@ static void trace_push(ba_iteration_trace* tr, double cost, double damping, doub
     if (tr->pcg_iterations) tr->pcg_iterations[i] = pcg;
 }
 
+// scene_scale (optimizer.rs:620-627), or observed_scene_scale (optimizer.rs:629-668) for the
+// calibrated workspace, whose explicit pose masks make the length come from observed points only
+// (a scaled running sum of squares)
+static double scene_scale_of(const Lowered& L, const double* X, const double* t) {
+    const size_t np = L.kept_points.size();
+    if (L.gauge_camera >= 0) return norm3(&t[3 * L.gauge_camera]);
+    if (L.calibrated) {
+        double sc = 0.0, sqs = 1.0;
+        size_t count = 0;
+        for (size_t q = 0; q < np; ++q) {
+            if (L.off[q] == L.off[q + 1]) continue;
+            count += 1;
+            for (int k = 0; k < 3; ++k) {
+                const double m = std::fabs(X[3 * q + k]);
+                if (m == 0.0) continue;
+                if (sc < m) {
+                    const double ratio = sc / m;
+                    sqs = 1.0 + sqs * ratio * ratio;
+                    sc = m;
+                } else {
+                    const double ratio = m / sc;
+                    sqs += ratio * ratio;
+                }
+            }
+        }
+        if (sc == 0.0 || count == 0) return MINPOS;
+        const double rms = sc * std::sqrt(sqs / (double)count);
+        return std::isfinite(rms) ? fmax_(rms, MINPOS) : sc;
+    }
+    double s = 0.0;
+    for (size_t q = 0; q < np; ++q) s += X[3 * q] * X[3 * q] + X[3 * q + 1] * X[3 * q + 1] + X[3 * q + 2] * X[3 * q + 2];
+    return fmax_(std::sqrt(s / (double)np), MINPOS);
+}
+
 static Err refine(const Lowered& L, const ba_solve_config* cfg, State& st, ba_solve_report& rep,
                   ba_iteration_trace* trace) {
     const size_t D = L.D;
@@ -2069,16 +2220,7 @@ static Err refine(const Lowered& L, const ba_solve_config* cfg, State& st, ba_so
                     return fail(BA_ERR_NUMERICAL, "point not projectable at the initial state");
             }
     }
-    // scene_scale, optimizer.rs:620-627
-    double scene_scale;
-    if (L.gauge_camera >= 0) {
-        scene_scale = norm3(&st.t[3 * L.gauge_camera]);
-    } else {
-        double s = 0.0;
-        for (size_t q = 0; q < np; ++q)
-            s += st.X[3 * q] * st.X[3 * q] + st.X[3 * q + 1] * st.X[3 * q + 1] + st.X[3 * q + 2] * st.X[3 * q + 2];
-        scene_scale = fmax_(std::sqrt(s / (double)np), MINPOS);
-    }
+    const double scene_scale = scene_scale_of(L, st.X.data(), st.t.data());
     Stage sched[3];
     stages(L.loss, cfg->max_iterations, sched);
     int target = 0;
@@ -2161,6 +2303,8 @@ static Err refine(const Lowered& L, const ba_solve_config* cfg, State& st, ba_so
             uint32_t pcg_terms = 0;
             double pcg_rel = NaN;
             std::vector<double> product;
+            Retained ret;
+            std::vector<BlockFactor> factors;
             if (!iterative) {
                 std::vector<double> F = a.H.full;
                 std::vector<double> rhs(D);
@@ -2170,8 +2314,6 @@ static Err refine(const Lowered& L, const ba_solve_config* cfg, State& st, ba_so
                 double base_tol = cfg->pcg_relative_tolerance > 0.0 ? cfg->pcg_relative_tolerance : 1e-1;
                 double tol = confirm ? fmin_(base_tol, iterative_conv_tol) : base_tol;
                 size_t limit = confirm ? pcg_cap : std::min(pcg_cap, D);
-                Retained ret;
-                std::vector<BlockFactor> factors;
                 if (!build_retained(v, a, L.cross_off.data(), L.params.data(), ret, factors)) {
                     solved = false;
                 } else {
@@ -2194,6 +2336,29 @@ static Err refine(const Lowered& L, const ba_solve_config* cfg, State& st, ba_so
                 damping.reject();
                 trace_push(trace, current, lambda, NaN, NaN, NaN, false, pcg_terms);
                 continue;
+            }
+            if (L.calibrated) {
+                // limit_camera_step (optimizer.rs:677-702) at the start of the calibrated candidate
+                // evaluation (calibrated/candidate.rs:36-44): the largest rotation or translation
+                // block norm of any camera is held to 0.5 by scaling the whole step
+                double largest = 0.0;
+                for (size_t c = 0; c < L.cam_dim.size(); ++c) {
+                    const size_t off = L.cam_off[c], dim = L.cam_dim[c];
+                    if (dim == 0) continue;
+                    const size_t first = std::min<size_t>(dim, 3);
+                    double s1 = 0.0, s2 = 0.0;
+                    for (size_t k = 0; k < first; ++k) s1 += step[off + k] * step[off + k];
+                    for (size_t k = first; k < dim; ++k) s2 += step[off + k] * step[off + k];
+                    largest = fmax_(largest, std::sqrt(s1));
+                    largest = fmax_(largest, std::sqrt(s2));
+                }
+                if (largest > 0.5) {
+                    const double sc = 0.5 / largest;
+                    for (size_t i = 0; i < D; ++i) step[i] *= sc;
+                }
+                // the retained reduction applies the operator to the step it predicts for
+                // (retained/mod.rs:781-800)
+                if (iterative) ret.apply(step.data(), product.data());
             }
             // retract_poses (optimizer.rs:386-437) and retract_cameras (self_calibration/optimizer.rs:565-586)
             const GaugeState& gs = v.gauge;
@@ -2317,7 +2482,23 @@ static Err refine(const Lowered& L, const ba_solve_config* cfg, State& st, ba_so
             double predicted;
             double cdn = 0.0;
             for (size_t i = 0; i < D; ++i) cdn += a.diag[i] * step[i] * step[i];
-            if (!iterative) {
+            if (!iterative && L.calibrated) {
+                // calibrated_model_reduction (calibrated/camera_step.rs:40-62): the quadratic form
+                // of the damped reduced matrix over its lower pattern (direct.rs:235-257), then
+                // reduced_model_reduction and undamped_model_reduction (optimizer.rs:590-618)
+                double quad = 0.0;
+                for (size_t col = 0; col < D; ++col)
+                    for (size_t row = col; row < D; ++row) {
+                        const double hv = a.H.full[row * D + col];
+                        if (hv == 0.0) continue;
+                        const double contribution = hv * step[row] * step[col];
+                        quad += row == col ? contribution : 2.0 * contribution;
+                    }
+                double gs_dot = 0.0;
+                for (size_t i = 0; i < D; ++i) gs_dot += a.g[i] * step[i];
+                const double damped = a.pmr - gs_dot - 0.5 * quad;
+                predicted = damped + 0.5 * lambda * (cdn + pdn);
+            } else if (!iterative) {
                 double gs_dot = 0.0;
                 for (size_t i = 0; i < D; ++i) gs_dot += a.g[i] * step[i];
                 double damped = a.pmr - 0.5 * gs_dot;
@@ -2479,6 +2660,33 @@ static void plan_solver(const Lowered& L, const ba_solve_config* c, ba_execution
     const uint64_t limit = c->workspace_budget_bytes ? c->workspace_budget_bytes : (uint64_t)1 << 30;
     const uint64_t np = L.kept_points.size(), cross = L.params.size(), D = L.D;
     const uint64_t nnz = L.row_idx.size();
+    if (L.calibrated) {
+        // select_calibrated (execution_policy.rs:123-183) over the serial estimates of
+        // workspace/calibrated.rs:480-530: the cached point cache keeps point steps and f64 cross
+        // rows (execution_policy.rs:68-96); the direct estimate adds the dense rank certificate up
+        // to dimension 2048 (workspace.rs:126-132); the retained estimate is
+        // estimated_calibrated_retained_bytes (retained/mod.rs:156-191) without worker shards
+        const uint64_t nc = L.cam_dim.size(), no = L.obs_cam.size();
+        const uint64_t cached_points = (np + 1) * 8 + np * (1 + 2 * 24) + cross * 24;
+        const uint64_t dense_rank = D <= 2048 ? (D + 2) * D * 8 : 0;
+        out.budget_bytes = limit;
+        out.reduced_nonzeros = nnz;
+        out.estimated_direct_bytes = nnz * 8 * 4 + dense_rank + cached_points;
+        out.estimated_retained_bytes = nc * (2 * 168 + 216) + np * 48 + D * 48 + cached_points;
+        const bool prefers_cached = no >= 250000 && D >= 768;  // execution_policy.rs:14-22
+        const uint64_t guarded = limit / 5 * 4 + limit % 5 * 4 / 5;
+        if (prefers_cached && out.estimated_retained_bytes <= limit) {
+            out.linear_solver = BA_SOLVER_ITERATIVE;
+            out.reason = BA_PLAN_WORKLOAD_PREFERS_RETAINED;
+        } else if (out.estimated_direct_bytes <= guarded) {
+            out.linear_solver = BA_SOLVER_DIRECT;
+            out.reason = BA_PLAN_DIRECT_FITS_BUDGET;
+        } else {
+            out.linear_solver = BA_SOLVER_ITERATIVE;
+            out.reason = out.estimated_retained_bytes <= limit ? BA_PLAN_DIRECT_EXCEEDS_BUDGET : BA_PLAN_LOWEST_MEMORY;
+        }
+        return;
+    }
     const uint64_t point_cache = (np + 1) * sizeof(uint64_t) + cross * 3 * sizeof(double);
     out.budget_bytes = limit;
     out.reduced_nonzeros = nnz;
@@ -3020,6 +3228,17 @@ static Err solve_once_o(const ba_problem* p, const ba_solve_config* cfg, ba_iter
         out.rep = r2;
         L = std::move(L2);
     }
+    if (L.calibrated) {
+        // solve_calibrated reports the unobserved free intrinsics as unconstrained
+        // (native/calibrated.rs:485-487, native/mod.rs:465-477); nothing is re-solved
+        std::vector<uint8_t> seen(p->n_intrinsics, 0);
+        for (size_t o = 0; o < p->n_observations; ++o) seen[p->obs_intrinsic[o]] = 1;
+        for (size_t g = 0; g < p->n_intrinsics; ++g)
+            if (!seen[g] && !intrinsic_is_fixed(p->intrinsics[g])) {
+                held[g] = 1;
+                nheld += 1;
+            }
+    }
     for (size_t g = 0; g < held.size(); ++g) out.unconstrained[g] = held[g];
     out.rep.unconstrained_intrinsics = (uint32_t)nheld;
     if (!full) return Err{};
@@ -3231,16 +3450,7 @@ int bao_prepare(const ba_problem* p, ba_native_sizes* sizes, ba_native_problem* 
     std::memcpy(n->pattern_col_ptr, L.col_ptr.data(), L.col_ptr.size() * 4);
     std::memcpy(n->pattern_row_idx, L.row_idx.data(), L.row_idx.size() * 4);
     n->gauge_camera = L.gauge_camera;
-    double scene_scale;
-    if (L.gauge_camera >= 0) {
-        scene_scale = norm3(&L.t[3 * L.gauge_camera]);
-    } else {
-        double s = 0.0;
-        for (size_t q = 0; q < np; ++q)
-            s += L.X[3 * q] * L.X[3 * q] + L.X[3 * q + 1] * L.X[3 * q + 1] + L.X[3 * q + 2] * L.X[3 * q + 2];
-        scene_scale = fmax_(std::sqrt(s / (double)np), MINPOS);
-    }
-    n->scene_scale = scene_scale;
+    n->scene_scale = scene_scale_of(L, L.X.data(), L.t.data());
     return BA_OK;
 }
 
