@@ -343,6 +343,41 @@ def bench_medium(ba, cpu: bool = False) -> dict:
     return out
 
 
+def bench_calibrated(ba, scene, steps: int, cpu: bool = False) -> dict:
+    """The calibrated pixel path on the large scene with every intrinsic fixed (native/calibrated.rs):
+    Auto takes the retained PCG step by the workload rule (at least 250,000 observations and
+    D >= 768); LM iterations/s over `steps` iterations in FP64, and with `cpu` the oracle's same
+    solve on all host cores and the largest per-iteration cost difference between the two."""
+    from openmvg_ba import SolveConfig, _abi as A, synthetic
+
+    prob = synthetic.fixed_intrinsics(scene).problem
+    cfg = SolveConfig(max_iterations=steps, certification=A.CERTIFY_LOOP_ONLY)
+    ba.solve(prob, cfg.with_(max_iterations=2))
+    sol = ba.solve(prob, cfg)
+    rep = sol.report
+    out = {"workload": f"{prob.n_poses} cameras (intrinsics fixed: calibrated path), {prob.n_points} points, "
+                       f"{prob.n_observations} observations, Huber 1px, Auto camera step, seed 0xBA0003",
+           "reduced_dimension": int(rep.reduced_dimension),
+           "linear_solver": "pcg" if rep.linear_solver_used == A.SOLVER_ITERATIVE else "direct",
+           "plan_reason": int(rep.plan_reason),
+           "fp64": {"lm_iter_per_s": rep.iterations / rep.seconds_iterations,
+                    "ms_per_step": 1e3 * rep.seconds_iterations / max(rep.iterations, 1),
+                    "pcg_iterations": int(rep.pcg_iterations), "final_cost": float(rep.final_cost)}}
+    if cpu:
+        from openmvg_ba.problem import solve_with
+
+        want = solve_with(load_oracle_lib(), "bao_solve", prob, cfg)
+        loop = float(want.trace["seconds"][-1])
+        n = min(len(want.trace["cost"]), len(sol.trace["cost"]))
+        c0, c1 = np.asarray(sol.trace["cost"][:n]), np.asarray(want.trace["cost"][:n])
+        out["cpu"] = {"lm_iter_per_s": want.report.iterations / loop, "threads": omp_threads(), "kind": "port",
+                      "final_cost": float(want.report.final_cost),
+                      "sample": f"oracle, the same {steps}-iteration solve, all host cores"}
+        out["max_iteration_cost_rel_diff_vs_oracle"] = float(np.max(np.abs(c0 - c1) / np.abs(c1)))
+        out["same_accept_sequence"] = bool(np.array_equal(sol.trace["accepted"][:n], want.trace["accepted"][:n]))
+    return out
+
+
 def bench_batched(ba, sample_windows: int = 64, cpu: bool = False) -> dict:
     """512 local windows (seed 0xBA0004: U{5..40} cameras, U{200..5000} points, 2-8 views), ten LM
     iterations each with the direct step: one batched solve (graph-replayed iterations) in FP64 and
@@ -461,7 +496,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES between two device buffers three times (a known-size kernel that "
                          "scales HBM counters in a profiled run)")
-    ap.add_argument("--only-form", choices=["medium_direct", "batched", "differentiation", "mixed_large"],
+    ap.add_argument("--only-form", choices=["medium_direct", "batched", "differentiation", "mixed_large",
+                                            "calibrated_large"],
                     help="run only this secondary measurement and print its JSON (no headline line)")
     args = ap.parse_args()
 
@@ -490,6 +526,8 @@ def main() -> int:
               "mixed_large": lambda: bench_mixed(adjuster(), synthetic.large_scene(n_points=args.points).problem,
                                                  SolveConfig(linear_solver=A.SOLVER_ITERATIVE,
                                                              certification=A.CERTIFY_LOOP_ONLY), args.steps, None),
+              "calibrated_large": lambda: bench_calibrated(adjuster(), synthetic.large_scene(n_points=args.points),
+                                                           args.steps, cpu=not args.no_cpu),
               "batched": lambda: bench_batched(adjuster(), cpu=not args.no_cpu),
               "differentiation": lambda: bench_differentiation(20, cpu=not args.no_cpu)}[args.only_form]
         print(json.dumps({args.only_form: fn()}))
@@ -584,6 +622,8 @@ def main() -> int:
         log(f"mixed: {forms['mixed']}")
         if ttc is not None:
             ttc["mixed_seconds"] = forms["mixed"].get("time_to_cost_seconds")
+        forms["calibrated"] = bench_calibrated(ba, scene, args.steps, cpu=not args.no_cpu)
+        log(f"calibrated: {forms['calibrated']}")
         forms["medium_direct"] = bench_medium(ba, cpu=not args.no_cpu)
         log(f"medium direct: {forms['medium_direct']}")
         forms["batched"] = bench_batched(ba, cpu=not args.no_cpu)

@@ -8,10 +8,13 @@
  *         fn solve(&mut self, &BundleProblem, SolveConfig) -> Result<Solution, SolveError>;
  *     }                                   (crates/openmvg-ba/src/solver.rs:1562-1573)
  *
- * restricted to the path the native adjuster takes for problems with free intrinsics
+ * restricted to two of the paths the native adjuster takes: problems with free intrinsics
  * (crates/openmvg-ba/src/native/mod.rs:352-367 -> native/self_calibration.rs:223-249):
  * pixel residuals, perspective cameras with one focal length, free (f[, k1[, k2]]),
- * a fixed principal point, full-pose or fixed poses.  Anything else returns
+ * a fixed principal point, full-pose or fixed poses; and problems whose observed intrinsics
+ * are all fixed (native/mod.rs:336-455 -> native/calibrated.rs): pixel residuals, pinhole or
+ * Brown-Conrady cameras with one focal length and radial k1/k2, full-pose or fixed poses,
+ * not a two-view problem (native/two_view.rs:14-50).  Anything else returns
  * BA_ERR_UNSUPPORTED, which callers treat like SolveError::Unsupported
  * (crates/openmvg-sfm/src/ba/native.rs:79-85: the caller falls back to another adjuster).
  *
@@ -317,8 +320,8 @@ int ba_hip_set_schur_product(ba_hip_handle* handle, int32_t mode);
  * explicit system, else IMPLICIT (AUTO before any). */
 int32_t ba_hip_last_schur_product(const ba_hip_handle* handle);
 
-/* BundleAdjuster::solve (solver.rs:1572) for the self-calibrating pixel path
- * (native/mod.rs:260-367, native/self_calibration.rs:223-249).  Host buffers.
+/* BundleAdjuster::solve (solver.rs:1572) for the self-calibrating and the calibrated pixel paths
+ * (native/mod.rs:260-455, native/self_calibration.rs:223-249, native/calibrated.rs).  Host buffers.
  * `trace` and `message` may be NULL.  On error, `solution` is left unspecified and the
  * problem is untouched (the reference's atomic-error contract, sparse/self_calibration/mod.rs:44-48). */
 int ba_hip_solve(ba_hip_handle* handle, const ba_problem* problem, const ba_solve_config* config,
