@@ -2,9 +2,10 @@
 //
 // Right-looking blocked LL^T on a row-major lower triangle with 64x64 tiles, two launches per
 // diagonal tile k:
-//   k_panel        every workgroup's first wave factors the diagonal tile in registers, checking
-//                  every pivot against 64 eps max|diag(A)| as the reference's dense camera solve
-//                  does (kernel/multi_view/dense/linear_solver.rs:9-61), and forms L_kk^-1;
+//   k_panel        every workgroup factors the diagonal tile (blocked by 16: register-resident
+//                  16x16 diagonal blocks, matrix-core panel and trailing products), checking every
+//                  pivot against 64 eps max|diag(A)| as the reference's dense camera solve does
+//                  (kernel/multi_view/dense/linear_solver.rs:9-61), and forms L_kk^-1;
 //                  workgroup b > 0 then forms its panel tile L_ik = A_ik L_kk^-T on the matrix
 //                  cores, and advances the forward solve L y = b by one tile;
 //   k_syrk_update  the trailing update A_ij -= L_ik L_jk^T on the matrix cores
@@ -75,18 +76,19 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Copies the 64x64 tile at src (row-major, leading dimension ld) into LDS; entries in rows >= nr
+// Copies the 64x64 tile at src (row-major, leading dimension ld) into LDS, converted to the
+// destination's type; entries in rows >= nr
 // or columns >= nc (or above the diagonal, with `lower`) read as zero.  Every thread issues all
 // of its loads before the first LDS store, so the tile arrives in one memory round trip.
-template <class T>
-__device__ __forceinline__ void load_tile(T (*dst)[TB + 1], const T* src, size_t ld, uint32_t nr, uint32_t nc,
+template <class T, class S>
+__device__ __forceinline__ void load_tile(T (*dst)[TB + 1], const S* src, size_t ld, uint32_t nr, uint32_t nc,
                                           bool lower = false) {
     constexpr int PER = TT / 256;
     T v[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = (int)threadIdx.x + 256 * q, r = e / TB, c = e % TB;
-        v[q] = ((uint32_t)r < nr && (uint32_t)c < nc && (!lower || c <= r)) ? src[(size_t)r * ld + c] : T(0);
+        v[q] = ((uint32_t)r < nr && (uint32_t)c < nc && (!lower || c <= r)) ? (T)src[(size_t)r * ld + c] : T(0);
     }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -123,68 +125,164 @@ __device__ __forceinline__ T inv_sqrt(T p) {
     return y;
 }
 
-// One wavefront factors the diagonal tile (size nb <= 64; identity beyond nb) held in `t` (lower
-// triangle) and forms X = L_kk^-1 into `xs` (row-major).  Lane i keeps row i in registers.
-// Column j < nb: the pivot a_jj (read from lane j) is checked against 64 eps max|diag(A)| as the
-// reference's dense camera solve does (kernel/multi_view/dense/linear_solver.rs:9-61);
-// L_ij = a_ij / sqrt(a_jj), and the rows below update a_ic -= (a_ij / a_jj) a_cj with column j
-// broadcast through LDS (`col`, two alternating buffers, so that a column's reads never meet the
-// next column's writes).  Entries above the diagonal of a row are never read.  Then L goes to
-// `t` and lane j forms column j of X by eliminating column by column of L:
-// x_k = x_k / L_kk, then x_m -= L_mk x_k for m > k.  Returns false, on every lane, when a pivot
-// fails.  `col` holds 2 * 64 values.
+// Matrix-core product of 16x16 blocks by one wavefront: acc += A op(B), A at `a` (row stride
+// lda), B at `b` (row stride ldb), op(B) = B or, with `bt`, B^T.  Lane l supplies A[l%16][k + l/16]
+// and op(B)[k + l/16][l%16] for k = 0, 4, 8, 12; element v of acc is C[row(l/16, v)][l%16].
 template <class T>
-__device__ bool factor_diag_wave(T (*t)[TB + 1], T* col, T (*xs)[TB + 1], uint32_t nb, T tol) {
-    const int i = threadIdx.x & 63;
-    T r[TB];
+__device__ __forceinline__ typename Mfma<T>::v4 mm16(const T* a, int lda, const T* b, int ldb, bool bt,
+                                                     typename Mfma<T>::v4 acc) {
+    const int lane = threadIdx.x & 63, li = lane % 16, lk = lane / 16;
 #pragma unroll
-    for (int c = 0; c < TB; ++c) r[c] = (i < (int)nb) ? (c <= i ? t[i][c] : T(0)) : (i == c ? T(1) : T(0));
-    bool ok = true;  // (a return inside the unrolled loop would keep r in scratch memory)
+    for (int k = 0; k < 16; k += 4) {
+        const T av = a[li * lda + k + lk];
+        const T bv = bt ? b[li * ldb + k + lk] : b[(k + lk) * ldb + li];
+        acc = Mfma<T>::mac(av, bv, acc);
+    }
+    return acc;
+}
+// Stores sign * acc (a 16x16 block in the matrix-core layout) at `d` (row stride ld).
+template <class T>
+__device__ __forceinline__ void store16(T* d, int ld, typename Mfma<T>::v4 acc, T sign) {
+    const int lane = threadIdx.x & 63, li = lane % 16, lk = lane / 16;
 #pragma unroll
-    for (int j = 0; j < TB; ++j) {
-        T* cj = col + TB * (j & 1);
-        const T piv = lane_value(r[j], j);
-        cj[i] = r[j];
+    for (int v = 0; v < 4; ++v) d[Mfma<T>::row(lk, v) * ld + li] = sign * acc[v];
+}
+
+// Factor and inverse of the 16x16 diagonal block at (p0, p0) of t by one wavefront (see
+// factor_diag_blocked); X_pp goes to xs.  Clears *ok_flag when a pivot fails.
+template <class T>
+__device__ __forceinline__ void factor_block16(T (*t)[TB + 1], T (*xs)[TB + 1], T (*sc)[16][17], T* colb,
+                                                         int p0, uint32_t nb, T tol, const double* tol_row,
+                                                         int* ok_flag) {
+    // lane i < 16 holds row p0 + i of the block (lanes 16..63 repeat rows 0..15; entries above the
+    // diagonal are carried along and never read)
+    const int i = threadIdx.x % 64, ri = i & 15;
+    T r[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) r[c] = t[p0 + ri][p0 + c];
+    // a failed pivot is recorded and the (then meaningless) elimination runs on, so that the loops
+    // stay fully unrolled with r in registers
+    bool ok = true;
+    T ilv[16];  // 1 / L_jj, the same on every lane
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        // column j through LDS (two alternating buffers, so that a column's reads never meet the
+        // next column's writes), its values read together after one wait
+        T* cb = colb + 16 * (j & 1);
+        const T aj = r[j];
+        if (i < 16) cb[i] = aj;
         wave_sync();
-        // rows at and beyond nb are the identity padding of a partial tile: their unit pivots are
-        // not the matrix's and are not held to its (possibly much larger) tolerance
-        if (j < (int)nb && !(isfinite(piv) && piv > tol)) {
-            ok = false;  // uniform: every lane read the same pivot
-            break;
-        }
+        T cv[16];
+#pragma unroll
+        for (int c = j; c < 16; ++c) cv[c] = cb[c];
+        const T piv = cv[j];
+        // the identity padding of a partial tile is not held to the matrix's tolerance
+        const T tj = tol_row && p0 + j < (int)nb ? (T)tol_row[p0 + j] : tol;
+        if (p0 + j < (int)nb && !(isfinite(piv) && piv > tj)) ok = false;  // uniform pivot
         const T il = inv_sqrt(piv);
-        const T ip = il * il, l = piv * il;
-        const T sj = r[j] * ip;
-        r[j] = i > j ? r[j] * il : (i == j ? l : r[j]);
+        ilv[j] = il;
+        const T sj = aj * (il * il);
+        r[j] = aj * il;  // L_ij below the diagonal, sqrt(a_jj) on lane j
 #pragma unroll
-        for (int c = j + 1; c < TB; ++c) r[c] -= sj * cj[c];
+        for (int c = j + 1; c < 16; ++c) r[c] -= sj * cv[c];
     }
-    if (!ok) return false;
-    wave_sync();
-    T d = T(1);
-#pragma unroll
-    for (int c = 0; c < TB; ++c) {
-        t[i][c] = c <= i ? r[c] : T(0);
-        d = c == i ? r[c] : d;  // (a register array indexed by the lane would live in scratch)
+    if (!ok) {
+        if (i == 0) *ok_flag = 0;
+        return;
     }
-    col[i] = T(1) / d;  // 1 / L_ii
-    wave_sync();
-    const int j = i;  // lane j: column j of X, in r
+    T x[16];  // lane i: column i of X_pp = L_pp^-1, from the unit vector e_i
 #pragma unroll
-    for (int k = 0; k < TB; ++k) r[k] = k == j ? T(1) : T(0);
+    for (int k = 0; k < 16; ++k) x[k] = sc[3][ri][k];
 #pragma unroll
-    for (int k = 0; k < TB; ++k) {
-        r[k] *= col[k];
+    for (int k = 0; k < 16; ++k) {
+        T* cb = colb + 16 * (k & 1);  // column k of L_pp
+        if (i < 16) cb[i] = r[k];
+        wave_sync();
+        T lv[16];
 #pragma unroll
-        for (int m = k + 1; m < TB; ++m) r[m] -= t[m][k] * r[k];
+        for (int m = k + 1; m < 16; ++m) lv[m] = cb[m];
+        x[k] *= ilv[k];
+#pragma unroll
+        for (int m = k + 1; m < 16; ++m) x[m] -= lv[m] * x[k];
     }
+    if (i < 16) {
 #pragma unroll
-    for (int k = 0; k < TB; ++k) xs[k][j] = r[k];
+        for (int c = 0; c < 16; ++c) xs[p0 + c][p0 + i] = x[c];
+    }
+}
+
+// Factor and inverse of the diagonal tile (size nb <= 64; identity beyond nb) by the four
+// wavefronts of the workgroup, blocked by 16:  for each 16-column block p, wave 0 factors the
+// 16x16 diagonal block with lanes 0..15 holding its rows in registers (column j broadcast through
+// LDS; every pivot checked against 64 eps max|diag(A)| as the reference's dense
+// camera solve does, kernel/multi_view/dense/linear_solver.rs:9-61) and inverts it (lane j: column
+// j of X_pp = L_pp^-1 by elimination), then the rows below form L_qp = A_qp X_pp^T and the
+// trailing blocks A_qs -= L_qp L_sp^T on the matrix cores, one 16x16 block per wavefront.  The
+// off-diagonal blocks of X = L^-1 follow by distance from the diagonal:
+// X_qp = -X_qq sum_{m=p}^{q-1} L_qm X_mp.  On return `xs` holds X (row-major, zero above the
+// diagonal) and `t` the factor; returns false, in every thread, when a pivot fails.  Every
+// product sums in a fixed order, so the result does not depend on scheduling.
+template <class T>
+__device__ __forceinline__ bool factor_diag_blocked(T (*t)[TB + 1], T (*xs)[TB + 1], T (*sc)[16][17], T* colb, uint32_t nb, T tol,
+                                    const double* tol_row, int* ok_flag) {
+    typedef typename Mfma<T>::v4 v4;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    for (int e = tid; e < TT; e += 256) xs[e / TB][e % TB] = T(0);
+    if (tid >= (int)nb && tid < TB) t[tid][tid] = T(1);  // identity padding of a partial tile
+    sc[3][tid / 16][tid % 16] = tid / 16 == tid % 16 ? T(1) : T(0);  // unit vectors (sc[3] is free until the end)
+    if (tid == 0) *ok_flag = 1;
+    __syncthreads();
+#pragma unroll 1
+    for (int p = 0; p < 4; ++p) {
+        const int p0 = 16 * p;
+        if (wave == 0) factor_block16<T>(t, xs, sc, colb, p0, nb, tol, tol_row, ok_flag);
+        __syncthreads();
+        if (!*ok_flag) return false;
+        // rows below the block: L_qp = A_qp X_pp^T, one block per wavefront
+        const int below = 3 - p;
+        if (wave < below) {
+            const int q0 = p0 + 16 * (wave + 1);
+            v4 acc = v4{T(0), T(0), T(0), T(0)};
+            acc = mm16<T>(&t[q0][p0], TB + 1, &xs[p0][p0], TB + 1, true, acc);
+            wave_sync();  // every lane has read the block before any lane overwrites it
+            store16<T>(&t[q0][p0], TB + 1, acc, T(1));
+        }
+        __syncthreads();
+        // trailing blocks A_qs -= L_qp L_sp^T (p < s <= q), dealt to the wavefronts in turn
+        for (int b = wave; b < below * (below + 1) / 2; b += 4) {
+            int q = 0;
+            while ((q + 1) * (q + 2) / 2 <= b) ++q;
+            const int sblk = b - q * (q + 1) / 2;
+            const int q0 = p0 + 16 * (q + 1), s0 = p0 + 16 * (sblk + 1);
+            v4 acc = v4{T(0), T(0), T(0), T(0)};
+            acc = mm16<T>(&t[q0][p0], TB + 1, &t[s0][p0], TB + 1, true, acc);
+            const int li = lane % 16, lk = lane / 16;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) t[q0 + Mfma<T>::row(lk, v)][s0 + li] -= acc[v];
+        }
+        __syncthreads();
+    }
+    // off-diagonal blocks of X by distance from the diagonal
+#pragma unroll 1
+    for (int dist = 1; dist < 4; ++dist) {
+        const int p = wave;  // block (p + dist, p)
+        if (p + dist < 4) {
+            const int q = p + dist, p0 = 16 * p, q0 = 16 * q;
+            v4 acc = v4{T(0), T(0), T(0), T(0)};
+            for (int m = p; m < q; ++m) acc = mm16<T>(&t[q0][16 * m], TB + 1, &xs[16 * m][p0], TB + 1, false, acc);
+            store16<T>(&sc[wave][0][0], 17, acc, T(1));
+            wave_sync();
+            v4 out = v4{T(0), T(0), T(0), T(0)};
+            out = mm16<T>(&xs[q0][q0], TB + 1, &sc[wave][0][0], 17, false, out);
+            store16<T>(&xs[q0][p0], TB + 1, out, T(-1));
+        }
+        __syncthreads();
+    }
     return true;
 }
 
 // Tile step kt of the factor, one launch (grid 1 + the row tiles below kt, 256 threads):
-// every workgroup's first wave factors the diagonal tile A_kk (read-only here) and forms
+// every workgroup factors the diagonal tile A_kk (read-only here) and forms
 // X = L_kk^-1 (the same arithmetic in every workgroup); workgroup 0 stores X, workgroup b > 0 the
 // panel tile L_ik = A_ik X^T (i = kt + b) in place, as a 64x64x64 matrix-core product.  With `rhs`
 // set the forward solve L y = rhs advances with the factor: every workgroup forms
@@ -192,27 +290,40 @@ __device__ bool factor_diag_wave(T (*t)[TB + 1], T* col, T (*xs)[TB + 1], uint32
 // sets *fail, and later steps return at once.
 template <class T>
 __global__ void __launch_bounds__(256) k_panel(T* A, uint32_t n, uint32_t ld, uint32_t kt, const double* work,
-                                               int* fail, T* Linv, double* rhs, double* y) {
+                                               int* fail, T* Linv, double* rhs, double* y, const double* tol_row) {
     typedef typename Mfma<T>::v4 v4;
-    __shared__ T t[TB][TB + 1];
-    __shared__ T xs[TB][TB + 1];
-    __shared__ T col[2 * TB];
+    // the diagonal tile is factored and inverted in FP64 for either storage type, so that the FP32
+    // factor of the mixed form keeps only its panel and trailing products in FP32
+    __shared__ double td[TB][TB + 1];
+    __shared__ double xd[TB][TB + 1];
+    __shared__ double sc[4][16][17];
+    __shared__ double colb[32];
     __shared__ double yk[TB];
     __shared__ int ok_flag;
+    T (*t)[TB + 1];
+    T (*xs)[TB + 1];
+    if constexpr (sizeof(T) == sizeof(double)) {
+        t = reinterpret_cast<T (*)[TB + 1]>(td);
+        xs = reinterpret_cast<T (*)[TB + 1]>(xd);
+    } else {
+        __shared__ T tf[TB][TB + 1];
+        __shared__ T xf[TB][TB + 1];
+        t = tf;
+        xs = xf;
+    }
     if (*fail) return;
     const uint32_t k0 = kt * TB;
     const uint32_t nb = n - k0 < TB ? n - k0 : TB;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    load_tile<T>(t, A + (size_t)k0 * ld + k0, ld, nb, nb, true);
+    load_tile<double>(td, A + (size_t)k0 * ld + k0, ld, nb, nb, true);
     __syncthreads();
-    if (wave == 0) {
-        const bool ok = factor_diag_wave<T>(t, col, xs, nb, (T)work[0]);
-        if (lane == 0) ok_flag = ok ? 1 : 0;
-    }
-    __syncthreads();
-    if (!ok_flag) {
+    if (!factor_diag_blocked<double>(td, xd, sc, colb, nb, work[0], tol_row ? tol_row + k0 : nullptr, &ok_flag)) {
         if (tid == 0) *fail = 1;
         return;
+    }
+    if constexpr (sizeof(T) != sizeof(double)) {
+        for (int e = tid; e < TT; e += 256) xs[e / TB][e % TB] = (T)xd[e / TB][e % TB];
+        __syncthreads();
     }
     if (rhs && wave == 0) {  // y_k = X rhs_k, rows in a fixed order
         yk[lane] = lane < (int)nb ? rhs[k0 + lane] : 0.0;
@@ -402,6 +513,63 @@ __global__ void __launch_bounds__(256) k_bwd_step(const T* L, const T* Linv, uin
     if (tid < TB) r[j0 + tid] -= ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]));
 }
 
+// Per-pivot tolerances of the rank certificate.  The reference certifies a reduced system of
+// dimension <= 2048 by its spectral rank after diagonal equilibration (direct.rs:342-457): with
+// s_i = 1 / sqrt(max(A_ii, 64 eps max_j A_jj)), A' = S A S is rank deficient when an eigenvalue
+// is at most 64 eps n lambda_max(A').  The factor here applies the same scale-aware test to the
+// pivots: the k-th pivot of A' is d_k s_k^2 (d_k the pivot of A), and A' is taken as deficient when
+// one is at most tau = 64 eps n g, with g = max_i sum_j |A'_ij| (the Gershgorin bound of lambda_max),
+// i.e. when d_k <= tau max(A_kk, floor).  A pivot above it bounds lambda_min(A') from below only
+// loosely, so the two rules agree except for eigenvalues within a factor g / lambda_max of the
+// tolerance; an exactly singular system fails both.  scratch: s (n), row sums (n), then the
+// tolerances (n).  A is the lower triangle (row-major, leading dimension ld).
+__global__ void __launch_bounds__(1024) k_rank_scale(const double* A, uint32_t n, uint32_t ld, double* scratch) {
+    __shared__ double sh[1024];
+    double m = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(A[(size_t)i * ld + i]));
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const double floor = 64.0 * DBL_EPSILON * sh[0];
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) scratch[i] = 1.0 / sqrt(fmax(A[(size_t)i * ld + i], floor));
+}
+__global__ void __launch_bounds__(256) k_rank_rows(const double* A, uint32_t n, uint32_t ld, double* scratch) {
+    __shared__ double sh[256];
+    const uint32_t i = blockIdx.x;
+    const double* sc = scratch;
+    double v = 0.0;
+    for (uint32_t j = threadIdx.x; j < n; j += 256) {
+        const double a = j <= i ? A[(size_t)i * ld + j] : A[(size_t)j * ld + i];
+        v += fabs(a) * sc[j];
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) scratch[n + i] = sh[0] * sc[i];
+}
+__global__ void __launch_bounds__(1024) k_rank_tol(const double* A, uint32_t n, uint32_t ld, double* scratch) {
+    __shared__ double sh[1024];
+    double g = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) g = fmax(g, scratch[n + i]);
+    sh[threadIdx.x] = g;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const double tau = 64.0 * DBL_EPSILON * (double)n * sh[0];
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) {
+        const double si = scratch[i];
+        scratch[2 * (size_t)n + i] = tau / (si * si);  // tau max(A_ii, floor)
+    }
+}
+
 __global__ void __launch_bounds__(256) k_copy_vec(uint32_t n, const double* a, double* b, const int* skip) {
     if (skip && *skip) return;
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -497,7 +665,7 @@ inline double* work_w(double* work, uint32_t n) { return work + 16 + 5 * (size_t
 // Pivots must exceed rel * max|diag(A)| (and be finite).
 template <class T>
 void factor(hipStream_t s, T* A, uint32_t n, uint32_t ld, double* work, int* fail, double* rhs, double* y,
-            double rel = 64.0 * Mfma<T>::eps) {
+            double rel = 64.0 * Mfma<T>::eps, const double* tol_row = nullptr) {
     hipLaunchKernelGGL(k_diag_tol<T>, dim3(1), dim3(1024), 0, s, A, n, ld, rel, work, fail);
     T* Linv = reinterpret_cast<T*>(work + linv_offset(n));
     const uint32_t ntile = (n + TB - 1) / TB;
@@ -506,7 +674,7 @@ void factor(hipStream_t s, T* A, uint32_t n, uint32_t ld, double* work, int* fai
         const uint32_t nb = n - k0 < TB ? n - k0 : TB;
         const uint32_t rest = ntile - kt - 1;
         hipLaunchKernelGGL(k_panel<T>, dim3(1 + rest), dim3(256), 0, s, A, n, ld, kt, (const double*)work, fail, Linv,
-                           rhs, y);
+                           rhs, y, tol_row);
         if (rest == 0) break;
         const uint32_t pairs = rest * (rest + 1) / 2;
         hipLaunchKernelGGL(k_syrk_update<T>, dim3(pairs), dim3(256), 0, s, A, n, ld, k0, nb, kt + 1, (const int*)fail);
@@ -696,9 +864,16 @@ void launch_cholesky_solve_mixed(hipStream_t s, const double* S, float* F, uint3
 }
 
 void launch_cholesky_factor(hipStream_t s, double* A, uint32_t n, uint32_t ld, double* work, int* fail,
-                            double rel_floor) {
+                            double rel_floor, const double* tol_row) {
     if (n == 0) return;
-    factor<double>(s, A, n, ld, work, fail, nullptr, nullptr, rel_floor);
+    factor<double>(s, A, n, ld, work, fail, nullptr, nullptr, rel_floor, tol_row);
+}
+
+void launch_rank_tolerances(hipStream_t s, const double* A, uint32_t n, uint32_t ld, double* scratch) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_rank_scale, dim3(1), dim3(1024), 0, s, A, n, ld, scratch);
+    hipLaunchKernelGGL(k_rank_rows, dim3(n), dim3(256), 0, s, A, n, ld, scratch);
+    hipLaunchKernelGGL(k_rank_tol, dim3(1), dim3(1024), 0, s, A, n, ld, scratch);
 }
 
 size_t trailing_inverse_doubles(uint32_t n, uint32_t j0) {

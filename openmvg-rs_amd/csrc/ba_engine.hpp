@@ -618,7 +618,7 @@ class Engine {
         if (std::is_same<T, float>::value) {
             timed(BA_PROFILE_CHOLESKY, [&] {
                 launch_cholesky_solve_mixed(s_, S_.p, S32_.p, tp_.D, tp_.D, rhs_.p, step_.p, work_.p, fail_.p, done_.p,
-                                            5);
+                                            8);
             });
             double st[3];
             HIP_OK(hipMemcpyAsync(h_int_.p, fail_.p, sizeof(int), hipMemcpyDeviceToHost, s_));

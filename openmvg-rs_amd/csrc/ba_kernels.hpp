@@ -450,10 +450,16 @@ void launch_cholesky_solve_mixed(hipStream_t s, const double* S, float* F, uint3
                                  double* x, double* work, int* fail, int* done, int max_steps);
 void launch_negate(hipStream_t s, uint32_t n, const double* in, double* out);
 // Factor only (no right-hand side), in place, with the stored L_kk^-1 tiles in `work`; *fail != 0
-// after a pivot that is not finite and above rel_floor * max|diag(A)| (rel_floor = 0: every pivot
-// positive, the condition of a sparse LLT without regularisation).
+// after a pivot that is not finite and above rel_floor * max|diag(A)|, or, with tol_row, above
+// tol_row[k] for pivot k (the rank certificate's tolerances of launch_rank_tolerances).
 void launch_cholesky_factor(hipStream_t s, double* A, uint32_t n, uint32_t ld, double* work, int* fail,
-                            double rel_floor);
+                            double rel_floor, const double* tol_row = nullptr);
+// The rank certificate's per-pivot tolerances of the lower triangle A (n x n, leading dimension
+// ld): scratch holds 3 n doubles, the tolerances at scratch + 2 n.  Pivot k of A fails when it is
+// at most 64 eps n g max(A_kk, 64 eps max_j A_jj), g the Gershgorin bound of the diagonally
+// equilibrated matrix: the scale-aware test of the reference's spectral rank certificate
+// (kernel/multi_view/sparse/direct.rs:342-457) applied to the pivots.
+void launch_rank_tolerances(hipStream_t s, const double* A, uint32_t n, uint32_t ld, double* scratch);
 // Marginal covariances of the trailing coordinates [j0, n) of a factored S (launch_cholesky_factor):
 // for each group g with offset goff[g] >= j0 and dimension gdim[g] <= 3, cov[9g..] = w (S^-1)_gg
 // (row-major 3x3, zero beyond gdim[g]); nothing when *fail.  R: trailing_inverse_doubles(n, j0)

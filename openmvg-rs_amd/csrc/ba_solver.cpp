@@ -451,7 +451,10 @@ Certificate certify(hipStream_t s, const Lowered& L, const Refined& f, bool exac
         Buf<int> fail;
         work.alloc(cholesky_work_doubles((uint32_t)D));
         fail.alloc(1);
-        launch_cholesky_factor(s, E.S_mut(), (uint32_t)D, (uint32_t)D, work.p, fail.p, 0.0);
+        Buf<double> tols;
+        tols.alloc(3 * D);
+        launch_rank_tolerances(s, E.S(), (uint32_t)D, (uint32_t)D, tols.p);
+        launch_cholesky_factor(s, E.S_mut(), (uint32_t)D, (uint32_t)D, work.p, fail.p, 0.0, tols.p + 2 * D);
         int failed = 1;
         HIP_OK(hipMemcpyAsync(&failed, fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));

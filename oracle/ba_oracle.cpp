@@ -3013,7 +3013,7 @@ static std::vector<uint8_t> solution_weak_points(const ba_problem* p, const Solu
 }
 
 // The exact certificate and marginals at the final state: S of the undamped final assembly,
-// factored with every pivot positive (a sparse LLT without regularisation), and block_marginal
+// factored with every pivot above the rank rule's scale-aware tolerance, and block_marginal
 // (covariance.rs:562-587) of every free group by solves with unit right-hand sides.
 static void certify_view(const View& v, const uint32_t* cross_off, bool exact, bool want, int32_t* obs,
                          bool* marginals, std::vector<double>& cov) {
@@ -3030,12 +3030,35 @@ static void certify_view(const View& v, const uint32_t* cross_off, bool exact, b
     Assembly a = assemble(v, 0.0, true, cross_off);
     bool ok = a.valid && a.H.finite();
     std::vector<double> F;
-    if (ok) {  // LLT of the lower triangle: every pivot positive
+    std::vector<double> tol_row(D, 0.0);
+    if (ok) {
+        // the rank rule: the reference certifies a reduced system by its spectral rank after
+        // diagonal equilibration (direct.rs:342-457: A' = S A S with s_i = 1 / sqrt(max(A_ii,
+        // 64 eps max_j A_jj)); deficient when an eigenvalue is at most 64 eps D lambda_max(A')).
+        // Restated on the pivots of the LLT: pivot k of A' is d_k s_k^2, and the system is taken
+        // as deficient when one is at most tau = 64 eps D g, g = max_i sum_j |A'_ij| (the
+        // Gershgorin bound of lambda_max), i.e. when d_k <= tau max(A_kk, floor)
+        const std::vector<double>& H = a.H.full;
+        double maxd = 0.0;
+        for (size_t i = 0; i < D; ++i) maxd = fmax_(maxd, std::fabs(H[i * D + i]));
+        const double floor = 64.0 * EPS * maxd;
+        std::vector<double> sc(D);
+        for (size_t i = 0; i < D; ++i) sc[i] = 1.0 / std::sqrt(fmax_(H[i * D + i], floor));
+        double g = 0.0;
+        for (size_t i = 0; i < D; ++i) {
+            double row = 0.0;
+            for (size_t j = 0; j < D; ++j) row += std::fabs(j <= i ? H[i * D + j] : H[j * D + i]) * sc[j];
+            g = fmax_(g, row * sc[i]);
+        }
+        const double tau = 64.0 * EPS * (double)D * g;
+        for (size_t i = 0; i < D; ++i) tol_row[i] = tau / (sc[i] * sc[i]);
+    }
+    if (ok) {  // LLT of the lower triangle, every pivot above its tolerance
         F = a.H.full;
         for (size_t c = 0; c < D && ok; ++c) {
             double d = F[c * D + c];
             for (size_t k = 0; k < c; ++k) d -= F[c * D + k] * F[c * D + k];
-            if (!(std::isfinite(d) && d > 0.0)) {
+            if (!(std::isfinite(d) && d > tol_row[c])) {
                 ok = false;
                 break;
             }

@@ -4,7 +4,8 @@ Pins:
 * rank and marginals (bao_certify): from the oracle's per-observation Jacobians (themselves
   pinned by central differences in tests/test_oracle_assembly.py) the test forms the undamped
   reduced system S in numpy by dense normal equations and point elimination; the oracle's verdict
-  is FULL_RANK exactly when numpy's smallest eigenvalue of S is positive (clearly so here), and its
+  is the reference's spectral rank rule (direct.rs:342-457) computed in numpy: after diagonal
+  equilibration, FULL_RANK when every eigenvalue exceeds 64 eps D lambda_max (clearly so here), and its
   marginal covariance of each free intrinsic group equals the group's diagonal block of numpy's
   inverse of S (crates/openmvg-ba/src/kernel/multi_view/sparse/covariance.rs:562-587), to 1e-8;
   a camera seen through one point makes S singular (numpy: a null eigenvalue) and the oracle
@@ -57,6 +58,17 @@ def undamped_schur(nat, arrays, J, r):
     return S
 
 
+def spectral_rank(S):
+    """The reference's rank of a reduced system (direct.rs:402-457): diagonal equilibration with
+    the floor 64 eps max diag, then the eigenvalues above 64 eps D lambda_max."""
+    d = np.diag(S)
+    floor = 64.0 * np.finfo(float).eps * np.abs(d).max()
+    s = 1.0 / np.sqrt(np.maximum(d, floor))
+    ev = np.linalg.eigvalsh(S * s[:, None] * s[None, :])
+    tol = 64.0 * np.finfo(float).eps * len(S) * np.abs(ev).max()
+    return int((ev > tol).sum())
+
+
 def certify(oracle, nat, arrays, loss=A.LOSS_SQUARED, scale=0.0):
     G = nat.sizes["n_groups"]
     obs = C.c_int32()
@@ -78,6 +90,7 @@ def test_marginals_are_blocks_of_the_inverse_reduced_system(cert, scene, loss):
     S = undamped_schur(nat, arrays, dense_jacobian(nat, arrays, jx, jc), r)
     ev = np.linalg.eigvalsh(S)
     assert ev[0] > 1e-9 * ev[-1]  # clearly positive definite
+    assert spectral_rank(S) == len(S)
     code, obs, cov = certify(oracle, nat, arrays, loss, scale)
     assert code == A.BA_OK and obs == A.OBSERVABILITY_FULL_RANK
     inv = np.linalg.inv(S)
@@ -101,6 +114,7 @@ def test_singular_reduced_system_is_rank_deficient(cert):
     S = undamped_schur(nat, arrays, dense_jacobian(nat, arrays, jx, jc), r)
     ev = np.linalg.eigvalsh(S)
     assert ev[0] <= 1e-9 * ev[-1]  # a null direction (the lone camera's unobserved motion)
+    assert spectral_rank(S) < len(S)
     code, obs, cov = certify(oracle, nat, arrays)
     assert obs == A.OBSERVABILITY_RANK_DEFICIENT and code == A.BA_ERR_NUMERICAL
     assert np.isnan(cov).all()
