@@ -163,7 +163,11 @@ __device__ __forceinline__ void factor_block16(T (*t)[TB + 1], T (*xs)[TB + 1], 
     // a failed pivot is recorded and the (then meaningless) elimination runs on, so that the loops
     // stay fully unrolled with r in registers
     bool ok = true;
-    T ilv[16];  // 1 / L_jj, the same on every lane
+    // lane i: column i of X_pp = L_pp^-1 by column substitution from the unit vector e_i, advanced
+    // with the factor: column j of L is the broadcast column times 1 / L_jj
+    T x[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = sc[3][ri][k];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         // column j through LDS (two alternating buffers, so that a column's reads never meet the
@@ -180,30 +184,18 @@ __device__ __forceinline__ void factor_block16(T (*t)[TB + 1], T (*xs)[TB + 1], 
         const T tj = tol_row && p0 + j < (int)nb ? (T)tol_row[p0 + j] : tol;
         if (p0 + j < (int)nb && !(isfinite(piv) && piv > tj)) ok = false;  // uniform pivot
         const T il = inv_sqrt(piv);
-        ilv[j] = il;
         const T sj = aj * (il * il);
         r[j] = aj * il;  // L_ij below the diagonal, sqrt(a_jj) on lane j
 #pragma unroll
         for (int c = j + 1; c < 16; ++c) r[c] -= sj * cv[c];
+        x[j] *= il;
+        const T xj = x[j] * il;
+#pragma unroll
+        for (int m = j + 1; m < 16; ++m) x[m] -= cv[m] * xj;  // L_mj x_j = (cv[m] / L_jj) x_j
     }
     if (!ok) {
         if (i == 0) *ok_flag = 0;
         return;
-    }
-    T x[16];  // lane i: column i of X_pp = L_pp^-1, from the unit vector e_i
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = sc[3][ri][k];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        T* cb = colb + 16 * (k & 1);  // column k of L_pp
-        if (i < 16) cb[i] = r[k];
-        wave_sync();
-        T lv[16];
-#pragma unroll
-        for (int m = k + 1; m < 16; ++m) lv[m] = cb[m];
-        x[k] *= ilv[k];
-#pragma unroll
-        for (int m = k + 1; m < 16; ++m) x[m] -= lv[m] * x[k];
     }
     if (i < 16) {
 #pragma unroll
