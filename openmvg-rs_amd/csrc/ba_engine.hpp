@@ -173,9 +173,14 @@ class Engine {
             ~Tally() { g_device_tally = nullptr; }
         } tally(&device_bytes_);
         B_ = build_block_lists(L_);
-        fast_cam_ = iterative_ && simple_topology();
+        // With one camera per intrinsic group and no camera observing a point twice, the FP64
+        // direct step assembles its dense S from the explicit system's camera rows (camera-side
+        // assembly, camera-pair matrix-core pass, scatter) instead of the block pairs' terms.
+        const bool simple = (iterative_ || (!batched_ && std::is_same<T, double>::value)) && simple_topology();
+        dense_rows_ = !iterative_ && !pair_terms_ && simple && count_schur_terms(L_) < (uint64_t)UINT32_MAX;
+        fast_cam_ = simple && (iterative_ || dense_rows_);
         // the camera-side assembly reads only the pairs' value offsets, not their terms
-        PL_ = build_pair_lists(L_, B_, !iterative_, !fast_cam_);
+        PL_ = build_pair_lists(L_, B_, !iterative_ && !dense_rows_, !fast_cam_);
         const size_t nc = L_.n_cameras(), ng = L_.n_groups(), np = L_.n_points(), no = L_.n_obs(), D = L_.D;
         npad_ = soa_pad(no);
         obs_cam_.upload_padded(L_.obs_cam, npad_, s_);
@@ -215,7 +220,7 @@ class Engine {
         pose_rows_ = keep_pose_rows_ || !fast_cam_;  // the block-gradient pass reads them
         if (fast_cam_) prec_.alloc(PE_REC * np);          // read by the camera assembly
         srec_.alloc(PCG_CAM_REC * std::max<size_t>(1, nc));
-        build_matvec_order(iterative_);
+        build_matvec_order(iterative_ || dense_rows_);
         jpm_.alloc(SOA_ROWS * npad_);
         HIP_OK(hipMemsetAsync(jpm_.p, 0, jpm_.n * sizeof(T), s_));  // finite padding
         if (!fast_cam_) jc_.alloc(18 * no);  // block-pair path
@@ -256,15 +261,15 @@ class Engine {
                 S32_.alloc(D * D);
                 done_.alloc(1);
             }
-        } else {
-            std::vector<int32_t> dp(B_.n_blocks, -1), ep(nc, -1);
-            for (size_t k = 0; k < PL_.block_a.size(); ++k) {
-                uint32_t a = PL_.block_a[k], b = PL_.block_b[k];
-                if (a == b) dp[a] = (int32_t)k;
-                else if (a >= nc && b < nc) ep[b] = (int32_t)k;
+            if (dense_rows_) {
+                upload_local_pairs();
+                jcm_.alloc(SOA_ROWS * npad_);
+                HIP_OK(hipMemsetAsync(jcm_.p, 0, jcm_.n * sizeof(T), s_));  // finite padding
+                casm_.alloc((size_t)CAM_ASM_VALUES * std::max<size_t>(1, nseg_));
+                build_explicit();
             }
-            diag_pair_.upload(dp, s_);
-            edge_pair_.upload(ep, s_);
+        } else {
+            upload_local_pairs();
             factors_.alloc(54 * B_.n_blocks);
             jcm_.alloc(SOA_ROWS * npad_);
             HIP_OK(hipMemsetAsync(jcm_.p, 0, jcm_.n * sizeof(T), s_));  // finite padding
@@ -349,6 +354,23 @@ class Engine {
         // the rest of the workspace is allocated before this point
         if (budget_ && (double)device_bytes_ + bytes > (double)budget_) return;
         if (terms >= (uint64_t)UINT32_MAX) return;
+        build_explicit();
+    }
+    // The diagonal pair of each block and the (group, camera) pair of each camera.
+    void upload_local_pairs() {
+        const size_t nc = L_.n_cameras();
+        std::vector<int32_t> dp(B_.n_blocks, -1), ep(nc, -1);
+        for (size_t k = 0; k < PL_.block_a.size(); ++k) {
+            uint32_t a = PL_.block_a[k], b = PL_.block_b[k];
+            if (a == b) dp[a] = (int32_t)k;
+            else if (a >= nc && b < nc) ep[b] = (int32_t)k;
+        }
+        diag_pair_.upload(dp, s_);
+        edge_pair_.upload(ep, s_);
+        HIP_OK(hipStreamSynchronize(s_));  // the host vectors end with this scope
+    }
+    void build_explicit() {
+        const size_t nc = L_.n_cameras(), no = L_.n_obs();
         std::vector<uint32_t> pos(no), cur(nc + 1, 0);
         for (size_t o = 0; o < no; ++o) cur[L_.obs_cam[o] + 1]++;
         for (size_t c = 0; c < nc; ++c) cur[c + 1] += cur[c];
@@ -531,6 +553,9 @@ class Engine {
     bool explicit_product() const { return used_explicit_; }
     // The product of the next LM iteration (before its assembly): use_explicit_product.
     void plan_product(uint32_t expected_pcg) { use_explicit_ = use_explicit_product(explicit_, schur_mode_, expected_pcg); }
+    // Assemble a direct step's S from the block pairs' terms, U - sum W V^-1 W^T in the oracle's
+    // order of operations, also for a simple topology (before setup()).
+    void keep_pair_terms() { pair_terms_ = true; }
     // Keep the pose columns of J~_c in the point-major copy (for the per-observation hook).
     void keep_pose_rows() { keep_pose_rows_ = true; }
     PointElim<T> pe() { return PointElim<T>{vinv_.p, vdamp_.p, y_.p, ddiag_.p, rrows_.p, prec_.p, pscal_.p}; }
@@ -552,7 +577,7 @@ class Engine {
         tp.loss = loss;
         tp.loss_scale = loss_scale;
         lin_state_ = state;
-        const bool cm = iterative_ && (reduced_system || fast_cam_);
+        const bool cm = (iterative_ && reduced_system) || fast_cam_;
         const bool keep = same_state && lin_kept_.valid && lin_kept_.state == state && lin_kept_.loss == loss &&
                           lin_kept_.scale == loss_scale && (lin_kept_.cm || !cm);
         if (!keep) {
@@ -593,8 +618,11 @@ class Engine {
         }
         if (reduced_system && !iterative_) {  // dense S with lambda diag on its diagonal
             HIP_OK(hipMemsetAsync(S_.p, 0, S_.n * sizeof(double), s_));
-            launch_scatter_dense(s_, tp, (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p, value_off_.p, u_vals_.p,
-                                 s_vals_.p, S_.p, tp.D);
+            if (dense_rows_)
+                launch_scatter_rows<T>(s_, tp, schur_rows(), S_.p, tp.D);
+            else
+                launch_scatter_dense(s_, tp, (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p, value_off_.p,
+                                     u_vals_.p, s_vals_.p, S_.p, tp.D);
             launch_finalize(s_, tp, lambda, diag_.p, fgrad_.p, grad_.p, S_.p, tp.D, nullptr, value_off_.p, u_vals_.p,
                             partials_.p, asm_.p);
         } else if (reduced_system) {  // lambda diag on the diagonal blocks of the preconditioner
@@ -843,6 +871,8 @@ class Engine {
     double refine_rel_ = NAN;
     uint32_t mixed_fallbacks_ = 0;
     bool fast_cam_ = false;
+    bool dense_rows_ = false;  // the direct step's dense S from the explicit system's rows
+    bool pair_terms_ = false;  // ... never (keep_pair_terms)
     int schur_mode_ = BA_SCHUR_AUTO;
     bool explicit_ = false;      // the explicit system is set up
     bool use_explicit_ = false;  // ... and the current LM iteration's PCG uses it

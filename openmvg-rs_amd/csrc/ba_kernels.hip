@@ -1159,6 +1159,33 @@ __global__ void __launch_bounds__(NT) k_scatter_dense(Topo tp, uint32_t npairs, 
     S[col * ld + row] = v;
 }
 
+// The explicit system's camera rows into the dense reduced matrix (the direct step of a simple
+// topology): workgroup per camera row c, lane per value of the row's 9 x 9 blocks (slot 0 the
+// camera's diagonal block, then the pairs (c, j > c)).  Block coordinates 0-5 are the pose (its
+// first cam_dim free), 6-8 the camera's intrinsic group (its first grp_dim free).  A pair value
+// goes to (row, col) and (col, row); the diagonal block, stored whole and symmetric, once.
+__device__ __forceinline__ int block_coordinate(const Topo& tp, uint32_t cam, uint32_t a) {
+    if (a < 6) return a < tp.cam_dim[cam] ? (int)(tp.cam_off[cam] + a) : -1;
+    const uint32_t g = tp.cam_group[cam];
+    return a - 6 < tp.grp_dim[g] ? (int)(tp.grp_off[g] + a - 6) : -1;
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_scatter_rows(Topo tp, SchurRows<T> S, double* __restrict__ Sd,
+                                                     uint32_t ld) {
+    const uint32_t c = blockIdx.x;
+    const uint32_t n0 = S.nbr_off[c], nn = S.nbr_off[c + 1] - n0, rl = schur_row_len(nn), w = 9 * nn;
+    const T* __restrict__ row = S.vals + S.row_base[c];
+    for (uint32_t e = threadIdx.x; e < 9 * w; e += NT) {
+        const uint32_t a = e / w, rest = e - a * w, slot = rest / 9, b = rest - 9 * slot;
+        const int ia = block_coordinate(tp, c, a);
+        const int ib = block_coordinate(tp, S.nbr[n0 + slot], b);
+        if (ia < 0 || ib < 0) continue;
+        const double v = (double)row[(size_t)a * rl + rest];
+        Sd[(size_t)ia * ld + ib] = v;
+        if (slot > 0) Sd[(size_t)ib * ld + ia] = v;
+    }
+}
+
 // ----------------------------------------------------------------------------------------------
 // k_finalize (single workgroup): damping-diagonal clamp per block (schur/assembly.rs:729-766),
 // lambda * diag on the reduced diagonal, scaled gradient norm (:1098-1112), finiteness of g.
@@ -2609,6 +2636,10 @@ void launch_scatter_dense(hipStream_t s, const Topo& tp, uint32_t npairs, const 
     hipLaunchKernelGGL(k_scatter_dense, dim3(npairs), dim3(64), 0, s, tp, npairs, pair_a, pair_b, value_off, u_vals,
                        s_vals, S, ld);
 }
+template <class T>
+void launch_scatter_rows(hipStream_t s, const Topo& tp, const SchurRows<T> S, double* Sd, uint32_t ld) {
+    if (tp.nc) hipLaunchKernelGGL(k_scatter_rows<T>, dim3(tp.nc), dim3(NT), 0, s, tp, S, Sd, ld);
+}
 void launch_finalize(hipStream_t s, const Topo& tp, double lambda, double* diag, const double* full_grad,
                      const double* grad, double* S, uint32_t ld, const int32_t* diag_pair, const uint64_t* value_off,
                      double* u_vals, double* partials, AsmScalars* sc) {
@@ -2752,6 +2783,7 @@ template void launch_gauge<double>(hipStream_t, const double*, int, double, Gaug
                                             SchurRows<T>);                                                         \
     template void launch_schur_pairs<T>(hipStream_t, const SchurPairs&, const T*, const double*, const double*,      \
                                         const uint32_t*, const uint32_t*, const GaugeDev*, SchurRows<T>);           \
+    template void launch_scatter_rows<T>(hipStream_t, const Topo&, const SchurRows<T>, double*, uint32_t);            \
     template void launch_pcg_product_upper<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&);                  \
     template void launch_pcg_product_lower<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&);                  \
     template void launch_pcg_step_explicit<T>(hipStream_t, const Topo&, const PcgVecs&);                            \

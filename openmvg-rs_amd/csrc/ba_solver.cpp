@@ -442,6 +442,10 @@ Certificate certify(hipStream_t s, const Lowered& L, const Refined& f, bool exac
     F.gp = f.gp;
     F.X = f.X;
     Engine<double> E(s, F, false);
+    // The rank verdict is a threshold test at the rounding level of the undamped S: it is formed
+    // as U - sum W V^-1 W^T, as the reference forms it, rather than from the explicit rows (whose
+    // J^T (I - M) J diagonal rounds differently when a point block is nearly singular).
+    E.keep_pair_terms();
     E.setup();
     const AsmScalars a = E.assemble(0, 0.0, L.loss, L.loss_scale, true);
     const bool valid = a.invalid == 0.0 && a.nonfinite == 0.0 && std::isfinite(a.cost);
