@@ -280,6 +280,49 @@ __device__ __forceinline__ bool factor_diag_blocked(T (*t)[TB + 1], T (*xs)[TB +
 // set the forward solve L y = rhs advances with the factor: every workgroup forms
 // y_k = X rhs_k, workgroup 0 stores it, workgroup b updates rhs_i -= L_ik y_k.  A failing pivot
 // sets *fail, and later steps return at once.
+// Panel tile: L_ik = A_ik X_kk^T from the tile A_ik in t and X_kk = L_kk^-1 in xs (LDS, four
+// waves on the matrix cores), stored in A; with rhs, rhs_i -= L_ik y_k (y_k in LDS).
+template <class T>
+__device__ __forceinline__ void panel_tile(T (*t)[TB + 1], T (*xs)[TB + 1], const double* yk, T* A, uint32_t ld,
+                                           uint32_t n, uint32_t r0, uint32_t k0, uint32_t nb, double* rhs) {
+    typedef typename Mfma<T>::v4 v4;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int qr = (wave / 2) * 32, qc = (wave % 2) * 32;
+    const int li = lane % 16, lk = lane / 16;
+    v4 acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int k = 0; k < TB; k += 4) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const T av = t[qr + 16 * a + li][k + lk];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const T bv = xs[qc + 16 * b + li][k + lk];  // B[k][j] = X[j][k]
+                acc[a][b] = Mfma<T>::mac(av, bv, acc[a][b]);
+            }
+        }
+    }
+    __syncthreads();  // every wave has read t: overwrite it with L_ik
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int v = 0; v < 4; ++v) {
+                const int r = qr + 16 * a + Mfma<T>::row(lk, v), c = qc + 16 * b + li;
+                t[r][c] = acc[a][b][v];
+                if (r0 + r < n && c < (int)nb) A[(size_t)(r0 + r) * ld + k0 + c] = acc[a][b][v];
+            }
+    if (!rhs) return;
+    __syncthreads();
+    // rhs_i -= L_ik y_k: thread (row, quarter) sums 16 columns, quarters combined in lane order
+    const int row = tid / 4, q = tid % 4;
+    double s = 0.0;
+    for (int c = 16 * q; c < 16 * q + 16; ++c) s += (double)t[row][c] * yk[c];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q == 0 && r0 + row < n) rhs[r0 + row] -= s;
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) k_panel(T* A, uint32_t n, uint32_t ld, uint32_t kt, const double* work,
                                                int* fail, T* Linv, double* rhs, double* y, const double* tol_row) {
@@ -334,67 +377,20 @@ __global__ void __launch_bounds__(256) k_panel(T* A, uint32_t n, uint32_t ld, ui
     const uint32_t r0 = k0 + TB * blockIdx.x;
     load_tile<T>(t, A + (size_t)r0 * ld + k0, ld, n - r0, nb);
     __syncthreads();
-    const int qr = (wave / 2) * 32, qc = (wave % 2) * 32;
-    const int li = lane % 16, lk = lane / 16;
-    v4 acc[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) acc[a][b] = v4{T(0), T(0), T(0), T(0)};
-#pragma unroll
-    for (int k = 0; k < TB; k += 4) {
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const T av = t[qr + 16 * a + li][k + lk];
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const T bv = xs[qc + 16 * b + li][k + lk];  // B[k][j] = X[j][k]
-                acc[a][b] = Mfma<T>::mac(av, bv, acc[a][b]);
-            }
-        }
-    }
-    __syncthreads();  // every wave has read t: overwrite it with L_ik
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-            for (int v = 0; v < 4; ++v) {
-                const int r = qr + 16 * a + Mfma<T>::row(lk, v), c = qc + 16 * b + li;
-                t[r][c] = acc[a][b][v];
-                if (r0 + r < n && c < (int)nb) A[(size_t)(r0 + r) * ld + k0 + c] = acc[a][b][v];
-            }
-    if (!rhs) return;
-    __syncthreads();
-    // rhs_i -= L_ik y_k: thread (row, quarter) sums 16 columns, quarters combined in lane order
-    const int row = tid / 4, q = tid % 4;
-    double s = 0.0;
-    for (int c = 16 * q; c < 16 * q + 16; ++c) s += (double)t[row][c] * yk[c];
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
-    if (q == 0 && r0 + row < n) rhs[r0 + row] -= s;
+    panel_tile<T>(t, xs, yk, A, ld, n, r0, k0, nb, rhs);
 }
 
 // Trailing update of tile pair (ti, tj), tj <= ti, both after the panel at k0:
 // A_ij -= L_ik L_jk^T with K = nb, both panel tiles staged whole in LDS.  Four waves, each a
 // 32x32 quadrant of 2x2 16x16x4 matrix-core tiles.
+// Update product of the trailing tile (rows of Li, columns of Lj): acc = Li Lj^T over the panel
+// column on the matrix cores, four waves of 2 x 2 16x16 blocks.
 template <class T>
-__global__ void __launch_bounds__(256) k_syrk_update(T* A, uint32_t n, uint32_t ld, uint32_t k0, uint32_t nb,
-                                                     uint32_t first_tile, const int* fail) {
+__device__ __forceinline__ void update_product(T (*Li)[TB + 1], T (*Lj)[TB + 1], typename Mfma<T>::v4 acc[2][2]) {
     typedef typename Mfma<T>::v4 v4;
-    __shared__ T Li[TB][TB + 1];
-    __shared__ T Lj[TB][TB + 1];
-    if (*fail) return;
-    // blockIdx.x enumerates the lower-triangular tile pairs (ti, tj) of the trailing matrix
-    const uint32_t idx = blockIdx.x;
-    uint32_t ti = (uint32_t)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
-    while ((ti + 1) * (ti + 2) / 2 <= idx) ++ti;
-    while (ti * (ti + 1) / 2 > idx) --ti;
-    const uint32_t tj = idx - ti * (ti + 1) / 2;
-    const uint32_t r0 = (first_tile + ti) * TB, c0 = (first_tile + tj) * TB;
-    load_tile<T>(Li, A + (size_t)r0 * ld + k0, ld, n - r0, nb);
-    load_tile<T>(Lj, A + (size_t)c0 * ld + k0, ld, n - c0, nb);
-    __syncthreads();
-    const int t = threadIdx.x;
-    const int wave = t / 64, lane = t % 64;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int qr = (wave / 2) * 32, qc = (wave % 2) * 32;
     const int li = lane % 16, lk = lane / 16;
-    v4 acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) acc[a][b] = v4{T(0), T(0), T(0), T(0)};
 #pragma unroll
@@ -409,6 +405,15 @@ __global__ void __launch_bounds__(256) k_syrk_update(T* A, uint32_t n, uint32_t 
             }
         }
     }
+}
+// The updated values A_rc - acc of tile (r0, c0) in the lower triangle (zero elsewhere), in the
+// lanes' matrix-core layout.
+template <class T>
+__device__ __forceinline__ void updated_values(const T* A, uint32_t n, uint32_t ld, uint32_t r0, uint32_t c0,
+                                               const typename Mfma<T>::v4 acc[2][2], T nv[2][2][4]) {
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int qr = (wave / 2) * 32, qc = (wave % 2) * 32;
+    const int li = lane % 16, lk = lane / 16;
     T old[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -426,8 +431,119 @@ __global__ void __launch_bounds__(256) k_syrk_update(T* A, uint32_t n, uint32_t 
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const uint32_t r = r0 + qr + 16 * a + Mfma<T>::row(lk, v), c = c0 + qc + 16 * b + li;
-                if (r < n && c <= r) A[(size_t)r * ld + c] = old[a][b][v] - acc[a][b][v];
+                nv[a][b][v] = (r < n && c <= r) ? T(old[a][b][v] - acc[a][b][v]) : T(0);
             }
+}
+
+// Trailing update A_ij -= L_ik L_jk^T (k the panel column k0), tile pair per workgroup, with a
+// look-ahead when Linv is given: the workgroups of the next column (tj = 0: the diagonal tile
+// k + 1 and the tiles below it) each update the diagonal tile k + 1 themselves (the same products
+// in the same order, so bit-identical in every workgroup), factor and invert it in FP64 (as the
+// panel pass does for tile 0), and workgroup 0 stores the inverse and y_{k+1} while the others form
+// their panel tiles L_i,k+1 and the forward-solve update.  The diagonal tile k + 1 of A is only
+// read in this launch (its inverse is what the solves use), so no workgroup waits for another.
+template <class T>
+__global__ void __launch_bounds__(256) k_syrk_update(T* A, uint32_t n, uint32_t ld, uint32_t k0, uint32_t nb,
+                                                     uint32_t first_tile, int* fail, const double* work, T* Linv,
+                                                     double* rhs, double* y, const double* tol_row) {
+    typedef typename Mfma<T>::v4 v4;
+    // the two panel tiles; then this workgroup's updated tile (storage type), the diagonal tile
+    // and its inverse (FP64)
+    __shared__ double raw0[TB][TB + 1];
+    __shared__ double raw1[TB][TB + 1];
+    __shared__ double raw2[TB][TB + 1];
+    __shared__ double sc[4][16][17];
+    __shared__ double colb[32];
+    __shared__ double yk[TB];
+    __shared__ int ok_flag;
+    T (*Li)[TB + 1] = reinterpret_cast<T (*)[TB + 1]>(raw0);
+    T (*Lj)[TB + 1] = reinterpret_cast<T (*)[TB + 1]>(raw1);
+    if (*fail) return;
+    // blockIdx.x enumerates the lower-triangular tile pairs (ti, tj) of the trailing matrix
+    const uint32_t idx = blockIdx.x;
+    uint32_t ti = (uint32_t)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+    while ((ti + 1) * (ti + 2) / 2 <= idx) ++ti;
+    while (ti * (ti + 1) / 2 > idx) --ti;
+    const uint32_t tj = idx - ti * (ti + 1) / 2;
+    const uint32_t r0 = (first_tile + ti) * TB, c0 = (first_tile + tj) * TB;
+    load_tile<T>(Li, A + (size_t)r0 * ld + k0, ld, n - r0, nb);
+    load_tile<T>(Lj, A + (size_t)c0 * ld + k0, ld, n - c0, nb);
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int wave = t / 64, lane = t % 64;
+    const int qr = (wave / 2) * 32, qc = (wave % 2) * 32;
+    const int li = lane % 16, lk = lane / 16;
+    v4 acc[2][2];
+    T nv[2][2][4];
+    update_product<T>(Li, Lj, acc);
+    updated_values<T>(A, n, ld, r0, c0, acc, nv);
+    const bool look = Linv != nullptr && tj == 0;  // uniform in the workgroup
+    if (!look) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const uint32_t r = r0 + qr + 16 * a + Mfma<T>::row(lk, v), c = c0 + qc + 16 * b + li;
+                    if (r < n && c <= r) A[(size_t)r * ld + c] = nv[a][b][v];
+                }
+        return;
+    }
+    // the diagonal tile k + 1 updated (for workgroup 0 it is its own tile)
+    T nd_v[2][2][4];
+    if (ti == 0) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) nd_v[a][b][v] = nv[a][b][v];
+    } else {
+        update_product<T>(Lj, Lj, acc);
+        updated_values<T>(A, n, ld, c0, c0, acc, nd_v);
+    }
+    __syncthreads();  // every wave has read Li and Lj
+    T (*tile)[TB + 1] = reinterpret_cast<T (*)[TB + 1]>(raw0);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int lr = qr + 16 * a + Mfma<T>::row(lk, v), lc = qc + 16 * b + li;
+                raw1[lr][lc] = (double)nd_v[a][b][v];  // lower triangle, as load_tile gives it
+                if (ti > 0) tile[lr][lc] = nv[a][b][v];
+            }
+    __syncthreads();
+    const uint32_t nd = n - c0 < TB ? n - c0 : TB;  // the diagonal tile's width
+    if (!factor_diag_blocked<double>(raw1, raw2, sc, colb, nd, work[0], tol_row ? tol_row + c0 : nullptr,
+                                     &ok_flag)) {
+        if (ti == 0 && t == 0) *fail = 1;
+        return;
+    }
+    // X in the storage type (for FP32, into raw1, free after the factor)
+    T (*xs)[TB + 1] = reinterpret_cast<T (*)[TB + 1]>(raw2);
+    if constexpr (sizeof(T) != sizeof(double)) {
+        xs = reinterpret_cast<T (*)[TB + 1]>(raw1);
+        for (int e = t; e < TT; e += 256) xs[e / TB][e % TB] = (T)raw2[e / TB][e % TB];
+        __syncthreads();
+    }
+    if (rhs && wave == 0) {  // y_{k+1} = X rhs_{k+1}, rows in a fixed order
+        yk[lane] = lane < (int)nd ? rhs[c0 + lane] : 0.0;
+        wave_sync();
+        double v = 0.0;
+        for (int c = 0; c <= lane; ++c) v += (double)xs[lane][c] * yk[c];
+        wave_sync();
+        yk[lane] = v;
+    }
+    __syncthreads();
+    if (ti == 0) {
+        for (int e = t; e < TT; e += 256) Linv[(size_t)first_tile * TT + e] = xs[e / TB][e % TB];
+        if (rhs && t < (int)nd) y[c0 + t] = yk[t];
+        return;
+    }
+    panel_tile<T>(tile, xs, yk, A, ld, n, r0, c0, nd, rhs);
 }
 
 // Forward solve step kt of L y = r (FP64 vectors; grid 1 + the row tiles below kt): every
@@ -661,15 +777,19 @@ void factor(hipStream_t s, T* A, uint32_t n, uint32_t ld, double* work, int* fai
     hipLaunchKernelGGL(k_diag_tol<T>, dim3(1), dim3(1024), 0, s, A, n, ld, rel, work, fail);
     T* Linv = reinterpret_cast<T*>(work + linv_offset(n));
     const uint32_t ntile = (n + TB - 1) / TB;
-    for (uint32_t kt = 0; kt < ntile; ++kt) {
+    // Tile 0 and its panel by k_panel; then each trailing update k also factors tile k + 1 (its
+    // workgroup 0) and forms the panel below it (the workgroups of its column, once workgroup 0 has
+    // published the inverse), while its other workgroups update the rest of the trailing matrix.
+    const uint32_t ntile0 = ntile;
+    hipLaunchKernelGGL(k_panel<T>, dim3(ntile0), dim3(256), 0, s, A, n, ld, 0u, (const double*)work, fail, Linv, rhs,
+                       y, tol_row);
+    for (uint32_t kt = 0; kt + 1 < ntile; ++kt) {
         const uint32_t k0 = kt * TB;
         const uint32_t nb = n - k0 < TB ? n - k0 : TB;
         const uint32_t rest = ntile - kt - 1;
-        hipLaunchKernelGGL(k_panel<T>, dim3(1 + rest), dim3(256), 0, s, A, n, ld, kt, (const double*)work, fail, Linv,
-                           rhs, y, tol_row);
-        if (rest == 0) break;
         const uint32_t pairs = rest * (rest + 1) / 2;
-        hipLaunchKernelGGL(k_syrk_update<T>, dim3(pairs), dim3(256), 0, s, A, n, ld, k0, nb, kt + 1, (const int*)fail);
+        hipLaunchKernelGGL(k_syrk_update<T>, dim3(pairs), dim3(256), 0, s, A, n, ld, k0, nb, kt + 1, fail, work, Linv,
+                           rhs, y, tol_row);
     }
 }
 // x = L^-T y (y is left unchanged).
