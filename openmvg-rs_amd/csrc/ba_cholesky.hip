@@ -1,16 +1,18 @@
 // ba_cholesky.hip — kernel (d): dense blocked Cholesky of the reduced camera system on gfx950.
 //
-// Right-looking blocked LL^T on a row-major lower triangle with 64x64 tiles, two launches per
-// diagonal tile k:
-//   k_panel        every workgroup factors the diagonal tile (blocked by 16: register-resident
-//                  16x16 diagonal blocks, matrix-core panel and trailing products), checking every
-//                  pivot against 64 eps max|diag(A)| as the reference's dense camera solve does
-//                  (kernel/multi_view/dense/linear_solver.rs:9-61), and forms L_kk^-1;
-//                  workgroup b > 0 then forms its panel tile L_ik = A_ik L_kk^-T on the matrix
-//                  cores, and advances the forward solve L y = b by one tile;
-//   k_syrk_update  the trailing update A_ij -= L_ik L_jk^T on the matrix cores
+// Right-looking blocked LL^T on a row-major lower triangle with 64x64 tiles:
+//   k_panel        (tile 0) every workgroup factors the diagonal tile in FP64 (blocked by 16:
+//                  register-resident 16x16 diagonal blocks, matrix-core panel and trailing
+//                  products), checking every pivot against 64 eps max|diag(A)| as the reference's
+//                  dense camera solve does (kernel/multi_view/dense/linear_solver.rs:9-61), and forms
+//                  L_kk^-1; workgroup b > 0 then forms its panel tile L_ik = A_ik L_kk^-T on the
+//                  matrix cores and advances the forward solve L y = b by one tile;
+//   k_syrk_update  (one launch per further tile) the trailing update A_ij -= L_ik L_jk^T on the
+//                  matrix cores, with a look-ahead: the workgroups of the next column update,
+//                  factor and invert the next diagonal tile themselves, as k_panel does, and form
+//                  the panel below it and the next forward-solve step.
 // (v_mfma_f64_16x16x4f64, or v_mfma_f32_16x16x4f32 for the FP32 factor; four waves per 64x64
-// tile).  Only the off-diagonal tiles of L stay in A; the solves use the stored L_kk^-1.  Further
+// tile.)  Only the off-diagonal tiles of L stay in A; the solves use the stored L_kk^-1.  Further
 // solves (iterative refinement) and L^T x = y run one launch per tile step across workgroups.
 // The mixed form factors a Jacobi-scaled FP32 copy and recovers the FP64 solution by iterative
 // refinement against the FP64 matrix (residuals r = b - S x in FP64).
