@@ -386,6 +386,8 @@ class Engine {
         sterm_off_.upload(SL.term_off, s_);
         sterm_.upload(SL.term.data(), SL.term.size(), s_);
         npairs_ = (uint32_t)(SL.pair_cam.size() / 2);
+        max_row_slots_ = 0;
+        for (size_t c = 0; c < nc; ++c) max_row_slots_ = std::max(max_row_slots_, SL.nbr_off[c + 1] - SL.nbr_off[c]);
         {
             std::vector<uint32_t> grp;
             std::vector<uint8_t> grn;
@@ -619,7 +621,7 @@ class Engine {
         if (reduced_system && !iterative_) {  // dense S with lambda diag on its diagonal
             HIP_OK(hipMemsetAsync(S_.p, 0, S_.n * sizeof(double), s_));
             if (dense_rows_)
-                launch_scatter_rows<T>(s_, tp, schur_rows(), S_.p, tp.D);
+                launch_scatter_rows<T>(s_, tp, schur_rows(), max_row_slots_, S_.p, tp.D);
             else
                 launch_scatter_dense(s_, tp, (uint32_t)PL_.block_a.size(), pair_a_.p, pair_b_.p, value_off_.p,
                                      u_vals_.p, s_vals_.p, S_.p, tp.D);
@@ -872,6 +874,7 @@ class Engine {
     uint32_t mixed_fallbacks_ = 0;
     bool fast_cam_ = false;
     bool dense_rows_ = false;  // the direct step's dense S from the explicit system's rows
+    uint32_t max_row_slots_ = 0;  // the longest camera row of the explicit system (blocks)
     bool pair_terms_ = false;  // ... never (keep_pair_terms)
     int schur_mode_ = BA_SCHUR_AUTO;
     bool explicit_ = false;      // the explicit system is set up

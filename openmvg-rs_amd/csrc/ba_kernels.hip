@@ -1160,29 +1160,50 @@ __global__ void __launch_bounds__(NT) k_scatter_dense(Topo tp, uint32_t npairs, 
 }
 
 // The explicit system's camera rows into the dense reduced matrix (the direct step of a simple
-// topology): workgroup per camera row c, lane per value of the row's 9 x 9 blocks (slot 0 the
-// camera's diagonal block, then the pairs (c, j > c)).  Block coordinates 0-5 are the pose (its
-// first cam_dim free), 6-8 the camera's intrinsic group (its first grp_dim free).  A pair value
-// goes to (row, col) and (col, row); the diagonal block, stored whole and symmetric, once.
+// topology): workgroup per camera row c (slot 0 the camera's diagonal 9 x 9 block, then the pairs
+// (c, j > c)).  Block coordinates 0-5 are the pose (its first cam_dim free), 6-8 the camera's
+// intrinsic group (its first grp_dim free).  Workgroup (c, y) takes slots [y SCATTER_SLOTS,
+// (y + 1) SCATTER_SLOTS) of the row, staged in LDS; each value goes to (row, col) with the column
+// index fastest and, for a pair, to (col, row) with the row index fastest, so that both writes run
+// along rows of S.
 __device__ __forceinline__ int block_coordinate(const Topo& tp, uint32_t cam, uint32_t a) {
     if (a < 6) return a < tp.cam_dim[cam] ? (int)(tp.cam_off[cam] + a) : -1;
     const uint32_t g = tp.cam_group[cam];
     return a - 6 < tp.grp_dim[g] ? (int)(tp.grp_off[g] + a - 6) : -1;
 }
+constexpr uint32_t SCATTER_SLOTS = 8;  // slots per workgroup
 template <class T>
 __global__ void __launch_bounds__(NT) k_scatter_rows(Topo tp, SchurRows<T> S, double* __restrict__ Sd,
                                                      uint32_t ld) {
+    __shared__ double blk[SCATTER_SLOTS][81];
+    __shared__ int col[SCATTER_SLOTS][9];
+    __shared__ int own[9];
     const uint32_t c = blockIdx.x;
-    const uint32_t n0 = S.nbr_off[c], nn = S.nbr_off[c + 1] - n0, rl = schur_row_len(nn), w = 9 * nn;
+    const uint32_t n0 = S.nbr_off[c], nn = S.nbr_off[c + 1] - n0, rl = schur_row_len(nn);
     const T* __restrict__ row = S.vals + S.row_base[c];
-    for (uint32_t e = threadIdx.x; e < 9 * w; e += NT) {
-        const uint32_t a = e / w, rest = e - a * w, slot = rest / 9, b = rest - 9 * slot;
-        const int ia = block_coordinate(tp, c, a);
-        const int ib = block_coordinate(tp, S.nbr[n0 + slot], b);
-        if (ia < 0 || ib < 0) continue;
-        const double v = (double)row[(size_t)a * rl + rest];
-        Sd[(size_t)ia * ld + ib] = v;
-        if (slot > 0) Sd[(size_t)ib * ld + ia] = v;
+    const uint32_t s0 = blockIdx.y * SCATTER_SLOTS;
+    if (s0 >= nn) return;
+    if (threadIdx.x < 9) own[threadIdx.x] = block_coordinate(tp, c, threadIdx.x);
+    {
+        const uint32_t ns = nn - s0 < SCATTER_SLOTS ? nn - s0 : SCATTER_SLOTS;
+        for (uint32_t e = threadIdx.x; e < 81 * ns; e += NT) {  // coalesced along the row
+            const uint32_t a = e / (9 * ns), rest = e - a * 9 * ns, sl = rest / 9, b = rest - 9 * sl;
+            blk[sl][9 * a + b] = (double)row[(size_t)a * rl + 9 * s0 + rest];
+        }
+        for (uint32_t e = threadIdx.x; e < 9 * ns; e += NT)
+            col[e / 9][e % 9] = block_coordinate(tp, S.nbr[n0 + s0 + e / 9], e % 9);
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < 81 * ns; e += NT) {  // (row, col): b fastest
+            const uint32_t sl = e / 81, a = (e / 9) % 9, b = e % 9;
+            const int ia = own[a], ib = col[sl][b];
+            if (ia >= 0 && ib >= 0) Sd[(size_t)ia * ld + ib] = blk[sl][9 * a + b];
+        }
+        for (uint32_t e = threadIdx.x; e < 81 * ns; e += NT) {  // (col, row) of a pair: a fastest
+            const uint32_t sl = e / 81, b = (e / 9) % 9, a = e % 9;
+            if (s0 + sl == 0) continue;  // the diagonal block is stored whole
+            const int ia = own[a], ib = col[sl][b];
+            if (ia >= 0 && ib >= 0) Sd[(size_t)ib * ld + ia] = blk[sl][9 * a + b];
+        }
     }
 }
 
@@ -2637,8 +2658,11 @@ void launch_scatter_dense(hipStream_t s, const Topo& tp, uint32_t npairs, const 
                        s_vals, S, ld);
 }
 template <class T>
-void launch_scatter_rows(hipStream_t s, const Topo& tp, const SchurRows<T> S, double* Sd, uint32_t ld) {
-    if (tp.nc) hipLaunchKernelGGL(k_scatter_rows<T>, dim3(tp.nc), dim3(NT), 0, s, tp, S, Sd, ld);
+void launch_scatter_rows(hipStream_t s, const Topo& tp, const SchurRows<T> S, uint32_t max_slots, double* Sd,
+                         uint32_t ld) {
+    if (tp.nc && max_slots)
+        hipLaunchKernelGGL(k_scatter_rows<T>, dim3(tp.nc, (max_slots + SCATTER_SLOTS - 1) / SCATTER_SLOTS), dim3(NT), 0,
+                           s, tp, S, Sd, ld);
 }
 void launch_finalize(hipStream_t s, const Topo& tp, double lambda, double* diag, const double* full_grad,
                      const double* grad, double* S, uint32_t ld, const int32_t* diag_pair, const uint64_t* value_off,
@@ -2783,7 +2807,7 @@ template void launch_gauge<double>(hipStream_t, const double*, int, double, Gaug
                                             SchurRows<T>);                                                         \
     template void launch_schur_pairs<T>(hipStream_t, const SchurPairs&, const T*, const double*, const double*,      \
                                         const uint32_t*, const uint32_t*, const GaugeDev*, SchurRows<T>);           \
-    template void launch_scatter_rows<T>(hipStream_t, const Topo&, const SchurRows<T>, double*, uint32_t);            \
+    template void launch_scatter_rows<T>(hipStream_t, const Topo&, const SchurRows<T>, uint32_t, double*, uint32_t);            \
     template void launch_pcg_product_upper<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&);                  \
     template void launch_pcg_product_lower<T>(hipStream_t, const Topo&, const SchurRows<T>, const PcgVecs&);                  \
     template void launch_pcg_step_explicit<T>(hipStream_t, const Topo&, const PcgVecs&);                            \

@@ -219,9 +219,10 @@ void launch_scatter_dense(hipStream_t s, const Topo& tp, uint32_t npairs, const 
                           const uint32_t* pair_b, const uint64_t* value_off, const double* u_vals,
                           const double* s_vals, double* S, uint32_t ld);
 // Dense S (row-major, both triangles) from the explicit system's camera rows (one camera per
-// intrinsic group, no camera observing a point twice).
+// intrinsic group, no camera observing a point twice); max_slots: the longest row's blocks.
 template <class T>
-void launch_scatter_rows(hipStream_t s, const Topo& tp, const SchurRows<T> S, double* Sd, uint32_t ld);
+void launch_scatter_rows(hipStream_t s, const Topo& tp, const SchurRows<T> S, uint32_t max_slots, double* Sd,
+                         uint32_t ld);
 // Clamp the damping diagonal per block, add lambda * diag to the reduced diagonal (dense S when
 // S != null, else the diagonal pairs' U and Schur-block values), and finish the scaled gradient.
 void launch_finalize(hipStream_t s, const Topo& tp, double lambda, double* diag, const double* full_grad,
