@@ -328,7 +328,6 @@ __device__ __forceinline__ void panel_tile(T (*t)[TB + 1], T (*xs)[TB + 1], cons
 template <class T>
 __global__ void __launch_bounds__(256) k_panel(T* A, uint32_t n, uint32_t ld, uint32_t kt, const double* work,
                                                int* fail, T* Linv, double* rhs, double* y, const double* tol_row) {
-    typedef typename Mfma<T>::v4 v4;
     // the diagonal tile is factored and inverted in FP64 for either storage type, so that the FP32
     // factor of the mixed form keeps only its panel and trailing products in FP32
     __shared__ double td[TB][TB + 1];
